@@ -1,0 +1,488 @@
+/* bw_oracle.c -- CPU oracle for the bwgpu batch path.  TEST INFRASTRUCTURE ONLY.
+ *
+ * This program is the checker.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg run it; no product code links or calls it.
+ *
+ * Modes
+ *   sha FILE                    SHA-256 of a file (FIPS 180-4, bw_sha256.h).
+ *   ref  --fixture F --topology T --seed S --factor-period P --q Q --states K
+ *        [--per-state] [--factor-pct X] --out ANS
+ *       Restatement of the reference oracle (src/oracle.cpp:225-451) over the reference's
+ *       declared workload (src/common.hpp:279-374): iterative DFS over the version tree,
+ *       apply the node's edit on entry / restore it on exit, answer each state's first q ranges
+ *       by a direct scan over all base + inserted rows accumulating amount*factor in __int128,
+ *       checked into int64 (exit 6 on overflow, src/oracle.cpp:203-211).  K is the reference's
+ *       compile-time kStates (src/common.hpp:184-192) given at run time.
+ *   tree --cfg C [--S n --U n --Q n --N n --seed n] [--algo fenwick|brute] --out ANS
+ *       Oracle for the branching multi-update point-update workloads of the benchmark configs
+ *       (bw_twl.h).  A point update (key, d) is the reference's OP_INSERT of a fresh row with
+ *       t=key, factor 1 (src/oracle.cpp:280-330, 403), so answer[v][k] = sum of d over every
+ *       update on the root->v path with lo <= key < hi.  "fenwick" is the rollback-Fenwick design
+ *       of the reference's independent rf arm (src/arms_classic.hpp:326-340): DFS apply / query /
+ *       undo with a Fenwick tree over keys (ranks for cfg 3).  "brute" walks the ancestor chain
+ *       of every state and sums in __int128 (tiny inputs only; it shares no code with "fenwick").
+ *       --workload-sha prints the SHA-256 of the generated arrays instead of solving.
+ *
+ * Output: answers as little-endian int64 in the reference's canonical order (state, query)
+ * and one JSON line {"n_answers":..,"answers_sha256":..,"wall_s":..}.
+ */
+#include <errno.h>
+#include <inttypes.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "bw_sha256.h"
+#include "bw_twl.h"
+
+static double now_s(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+static void die(int code, const char* msg) {
+  fprintf(stderr, "bw_oracle: %s\n", msg);
+  exit(code);
+}
+
+static int64_t checked_i64(__int128 x) {
+  if (x < (__int128)INT64_MIN || x > (__int128)INT64_MAX) die(6, "query answer overflowed int64");
+  return (int64_t)x;
+}
+
+static void write_answers(const char* path, const int64_t* ans, size_t n, double t0) {
+  if (path) {
+    FILE* fp = fopen(path, "wb");
+    if (!fp) die(4, "cannot write answers");
+    if (n && fwrite(ans, sizeof(int64_t), n, fp) != n) die(4, "short write");
+    if (fclose(fp) != 0) die(4, "close failed");
+  }
+  char hex[65];
+  bw_sha256_hex(ans, n * sizeof(int64_t), hex);
+  printf("{\"n_answers\":%zu,\"answers_sha256\":\"%s\",\"wall_s\":%.6f}\n", n, hex, now_s() - t0);
+}
+
+/* ------------------------------------------------------------------ reference semantics */
+/* splitmix64 stream, src/common.hpp:40-54 */
+typedef struct { uint64_t s; } ref_rng;
+static void rr_init(ref_rng* r, uint64_t seed) { r->s = seed ? seed : 0x9E3779B97F4A7C15ULL; }
+static uint64_t rr_next(ref_rng* r) {
+  r->s += 0x9E3779B97F4A7C15ULL;
+  uint64_t z = r->s;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+static uint32_t rr_bounded(ref_rng* r, uint32_t n) { return n ? (uint32_t)(rr_next(r) % n) : 0; }
+static int64_t rr_range(ref_rng* r, int64_t lo, int64_t hi) {
+  if (hi <= lo) return lo;
+  return lo + (int64_t)(rr_next(r) % (uint64_t)(hi - lo));
+}
+
+typedef struct { int64_t t, amount; int32_t g, pad; } base_rec;
+typedef struct {
+  int64_t n, tmin, tmax;
+  int32_t G;
+  base_rec* rows;
+  int64_t* factor;
+} fixture_t;
+
+static int load_fixture(const char* path, fixture_t* f) {
+  FILE* fp = fopen(path, "rb");
+  if (!fp) return 0;
+  unsigned char h[64];
+  if (fread(h, 1, 64, fp) != 64 || memcmp(h, "BWBASE02", 8) != 0) { fclose(fp); return 0; }
+  memcpy(&f->n, h + 8, 8);
+  memcpy(&f->G, h + 16, 4);
+  memcpy(&f->tmin, h + 40, 8);
+  memcpy(&f->tmax, h + 48, 8);
+  if (f->n <= 0 || f->G <= 0) { fclose(fp); return 0; }
+  f->rows = (base_rec*)malloc(sizeof(base_rec) * (size_t)f->n);
+  f->factor = (int64_t*)malloc(sizeof(int64_t) * (size_t)f->G);
+  if (fread(f->rows, sizeof(base_rec), (size_t)f->n, fp) != (size_t)f->n) { fclose(fp); return 0; }
+  if (fread(f->factor, sizeof(int64_t), (size_t)f->G, fp) != (size_t)f->G) { fclose(fp); return 0; }
+  fclose(fp);
+  for (int32_t g = 0; g < f->G; ++g)
+    if (f->factor[g] <= 0) return 0;
+  return 1;
+}
+
+enum { OP_REPLACE = 0, OP_INSERT = 1, OP_DELETE = 2, OP_MOVE = 3, OP_FACTOR = 4 };
+typedef struct { uint8_t op; int32_t g; int64_t id, val, t; } edit_t;
+
+/* nested ranges of one state, src/common.hpp:229-246 */
+static void gen_nested(ref_rng* r, int64_t tmin, int64_t tmax, int nq, int64_t* lo_out, int64_t* hi_out) {
+  int64_t lo = tmin, hi = tmax + 1;
+  const int64_t span0 = hi - lo;
+  for (int k = 0; k < nq; ++k) {
+    int64_t span = hi - lo;
+    if (span <= 4096) {
+      lo = tmin + span0 / 2;
+      hi = lo + 2048;
+      span = hi - lo;
+    }
+    const int64_t a = span * (int64_t)(3 + rr_bounded(r, 16)) / 100;
+    const int64_t b = span * (int64_t)(3 + rr_bounded(r, 16)) / 100;
+    lo += a;
+    hi -= b;
+    if (hi - lo < 1024) hi = lo + 1024;
+    lo_out[k] = lo;
+    hi_out[k] = hi;
+  }
+}
+
+static int ref_parent(int topology, int32_t v) {
+  if (v <= 0) return -1;
+  if (topology == 0) return v - 1;
+  if (topology == 1) return (v - 1) / 2;
+  return 0;
+}
+
+static int mode_ref(int argc, char** argv) {
+  const char* fx = NULL; const char* out = NULL;
+  int64_t topo = -1, seed = 0, fp = -1, q = -1, K = 2049, fpct = -1;
+  int per_state = 0;
+  for (int i = 2; i < argc; ++i) {
+    if (!strcmp(argv[i], "--per-state")) { per_state = 1; continue; }
+    if (i + 1 >= argc) die(2, "missing value");
+    const char* k = argv[i]; const char* v = argv[++i];
+    if (!strcmp(k, "--fixture")) fx = v;
+    else if (!strcmp(k, "--out")) out = v;
+    else if (!strcmp(k, "--topology")) topo = atoll(v);
+    else if (!strcmp(k, "--seed")) seed = atoll(v);
+    else if (!strcmp(k, "--factor-period")) fp = atoll(v);
+    else if (!strcmp(k, "--q")) q = atoll(v);
+    else if (!strcmp(k, "--states")) K = atoll(v);
+    else if (!strcmp(k, "--factor-pct")) fpct = atoll(v);
+    else die(2, "unknown argument");
+  }
+  if (!fx || topo < 0 || topo > 2 || fp < 0 || q < 1 || K < 2) die(2, "usage");
+  const double t0 = now_s();
+  fixture_t f;
+  if (!load_fixture(fx, &f)) die(1, "cannot load fixture");
+  const int64_t N = f.n;
+  const int32_t G = f.G;
+  const uint64_t useed = (uint64_t)seed, utopo = (uint64_t)topo, ufp = (uint64_t)fp;
+  /* declared queries, src/common.hpp:292-303 */
+  const size_t nqs = per_state ? (size_t)K : 1;
+  int64_t* qlo = (int64_t*)calloc(nqs * (size_t)q, sizeof(int64_t));
+  int64_t* qhi = (int64_t*)calloc(nqs * (size_t)q, sizeof(int64_t));
+  if (per_state) {
+    for (int64_t v = 1; v < K; ++v) {
+      ref_rng qr;
+      rr_init(&qr, useed * 1000000007ULL + utopo * 65537ULL + ufp * 131ULL + (uint64_t)v * 2654435761ULL + 17ULL);
+      gen_nested(&qr, f.tmin, f.tmax, (int)q, &qlo[(size_t)v * q], &qhi[(size_t)v * q]);
+    }
+  } else {
+    ref_rng qr;
+    rr_init(&qr, useed * 1000000007ULL + utopo * 65537ULL + ufp * 131ULL + 17ULL);
+    gen_nested(&qr, f.tmin, f.tmax, (int)q, qlo, qhi);
+  }
+  /* edits, src/common.hpp:304-355 */
+  edit_t* ed = (edit_t*)calloc((size_t)K, sizeof(edit_t));
+  int64_t* gfac = (int64_t*)malloc(sizeof(int64_t) * (size_t)G);
+  for (int32_t g = 0; g < G; ++g) gfac[g] = 1000;
+  ref_rng rng;
+  rr_init(&rng, useed * 1000003ULL + utopo * 7919ULL + ufp * 104729ULL + 20261025ULL);
+  int32_t next_group = 0;
+  int64_t next_ins = 0;
+  int64_t* ins_t = (int64_t*)malloc(sizeof(int64_t) * (size_t)K);
+  int32_t* ins_g = (int32_t*)malloc(sizeof(int32_t) * (size_t)K);
+  for (int64_t v = 1; v < K; ++v) {
+    edit_t e;
+    memset(&e, 0, sizeof(e));
+    int factor_step;
+    if (fpct >= 0) factor_step = rr_bounded(&rng, 100) < (uint32_t)fpct;
+    else factor_step = (fp > 0) && (v % fp == 0);
+    if (factor_step) {
+      const int32_t g = next_group++ % G;
+      const int64_t cur = gfac[g];
+      const int64_t nf = (cur == 950) ? 1000 : (cur == 1000 ? 1050 : 950);
+      gfac[g] = nf;
+      e.op = OP_FACTOR; e.g = g; e.val = nf;
+    } else {
+      const uint32_t roll = rr_bounded(&rng, 100);
+      if (roll < 55) {
+        e.op = OP_REPLACE;
+        e.id = rr_range(&rng, 0, N);
+        e.val = f.rows[e.id].amount + rr_range(&rng, -100000, 100000);
+      } else if (roll < 75) {
+        e.op = OP_INSERT;
+        e.id = N + next_ins;
+        e.t = rr_range(&rng, f.tmin, f.tmax + 1);
+        e.g = (int32_t)rr_bounded(&rng, (uint32_t)G);
+        const int64_t base_amt = f.rows[rr_bounded(&rng, (uint32_t)N)].amount;
+        e.val = base_amt + rr_range(&rng, -100000, 100000);
+        ins_t[next_ins] = e.t;
+        ins_g[next_ins] = e.g;
+        ++next_ins;
+      } else if (roll < 90) {
+        e.op = OP_DELETE;
+        e.id = rr_range(&rng, 0, N);
+      } else {
+        e.op = OP_MOVE;
+        e.id = rr_range(&rng, 0, N);
+        e.t = rr_range(&rng, f.tmin, f.tmax + 1);
+        e.g = (int32_t)rr_bounded(&rng, (uint32_t)G);
+        e.val = f.rows[e.id].amount + rr_range(&rng, -100000, 100000);
+      }
+    }
+    ed[v] = e;
+  }
+  /* mutable row state (src/oracle.cpp:240-278) */
+  const int64_t M = N + next_ins;
+  int64_t* t = (int64_t*)malloc(sizeof(int64_t) * (size_t)M);
+  int64_t* amt = (int64_t*)malloc(sizeof(int64_t) * (size_t)M);
+  int32_t* g = (int32_t*)malloc(sizeof(int32_t) * (size_t)M);
+  uint8_t* ex = (uint8_t*)malloc((size_t)M);
+  for (int64_t i = 0; i < N; ++i) { t[i] = f.rows[i].t; amt[i] = f.rows[i].amount; g[i] = f.rows[i].g; ex[i] = 1; }
+  for (int64_t k = 0; k < next_ins; ++k) { t[N + k] = ins_t[k]; g[N + k] = ins_g[k]; amt[N + k] = 0; ex[N + k] = 0; }
+  int64_t* fac = (int64_t*)malloc(sizeof(int64_t) * (size_t)G);
+  for (int32_t gg = 0; gg < G; ++gg) fac[gg] = f.factor[gg];
+  /* children in ascending order (CSR) */
+  int64_t* koff = (int64_t*)calloc((size_t)K + 1, sizeof(int64_t));
+  int32_t* kid = (int32_t*)malloc(sizeof(int32_t) * (size_t)K);
+  for (int64_t v = 1; v < K; ++v) koff[ref_parent((int)topo, (int32_t)v) + 1]++;
+  for (int64_t v = 0; v < K; ++v) koff[v + 1] += koff[v];
+  {
+    int64_t* cur = (int64_t*)malloc(sizeof(int64_t) * (size_t)K);
+    memcpy(cur, koff, sizeof(int64_t) * (size_t)K);
+    for (int64_t v = 1; v < K; ++v) kid[cur[ref_parent((int)topo, (int32_t)v)]++] = (int32_t)v;
+    free(cur);
+  }
+  /* undo records */
+  typedef struct { int64_t amt, t, fac; int32_t g; uint8_t ex; } undo_t;
+  undo_t* un = (undo_t*)calloc((size_t)K, sizeof(undo_t));
+  const size_t nans = (size_t)(K - 1) * (size_t)q;
+  int64_t* ans = (int64_t*)calloc(nans ? nans : 1, sizeof(int64_t));
+  int64_t* stk_v = (int64_t*)malloc(sizeof(int64_t) * (size_t)K);
+  int64_t* stk_c = (int64_t*)malloc(sizeof(int64_t) * (size_t)K);
+  __int128* acc = (__int128*)malloc(sizeof(__int128) * (size_t)q);
+  int64_t sp = 0;
+  stk_v[0] = 0; stk_c[0] = koff[0]; sp = 1;
+  while (sp > 0) {
+    const int64_t v = stk_v[sp - 1];
+    if (stk_c[sp - 1] < koff[v + 1]) {
+      const int32_t c = kid[stk_c[sp - 1]++];
+      /* apply (src/oracle.cpp:280-330) */
+      const edit_t* e = &ed[c];
+      undo_t* u = &un[c];
+      switch (e->op) {
+        case OP_REPLACE: u->amt = amt[e->id]; u->ex = ex[e->id]; amt[e->id] = e->val; ex[e->id] = 1; break;
+        case OP_INSERT:
+        case OP_MOVE:
+          u->amt = amt[e->id]; u->t = t[e->id]; u->g = g[e->id]; u->ex = ex[e->id];
+          t[e->id] = e->t; g[e->id] = e->g; amt[e->id] = e->val; ex[e->id] = 1; break;
+        case OP_DELETE: u->ex = ex[e->id]; ex[e->id] = 0; break;
+        case OP_FACTOR: u->fac = fac[e->g]; fac[e->g] = e->val; break;
+        default: die(7, "unknown op");
+      }
+      /* answer: direct scan of every row (src/oracle.cpp:359-422) */
+      const int64_t* lo = per_state ? &qlo[(size_t)c * q] : qlo;
+      const int64_t* hi = per_state ? &qhi[(size_t)c * q] : qhi;
+      for (int64_t k = 0; k < q; ++k) acc[k] = 0;
+      for (int64_t i = 0; i < M; ++i) {
+        if (!ex[i]) continue;
+        const int64_t ti = t[i];
+        const __int128 w = (__int128)amt[i] * (__int128)fac[g[i]];
+        for (int64_t k = 0; k < q; ++k)
+          if (ti >= lo[k] && ti < hi[k]) acc[k] += w;
+      }
+      for (int64_t k = 0; k < q; ++k) ans[(size_t)(c - 1) * q + k] = checked_i64(acc[k]);
+      stk_v[sp] = c; stk_c[sp] = koff[c]; ++sp;
+    } else {
+      if (v > 0) {
+        /* revert (src/oracle.cpp:332-355) */
+        const edit_t* e = &ed[v];
+        const undo_t* u = &un[v];
+        switch (e->op) {
+          case OP_REPLACE: amt[e->id] = u->amt; ex[e->id] = u->ex; break;
+          case OP_INSERT:
+          case OP_MOVE: t[e->id] = u->t; g[e->id] = u->g; amt[e->id] = u->amt; ex[e->id] = u->ex; break;
+          case OP_DELETE: ex[e->id] = u->ex; break;
+          case OP_FACTOR: fac[e->g] = u->fac; break;
+          default: break;
+        }
+      }
+      --sp;
+    }
+  }
+  write_answers(out, ans, nans, t0);
+  return 0;
+}
+
+/* ------------------------------------------------------------------ tree workloads */
+static uint64_t lower_bound_u32(const uint32_t* a, uint64_t n, uint64_t x) {
+  uint64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if ((uint64_t)a[mid] < x) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+static int mode_tree(int argc, char** argv) {
+  int cfg = 2;
+  for (int i = 2; i + 1 < argc; ++i)
+    if (!strcmp(argv[i], "--cfg")) cfg = atoi(argv[i + 1]);
+  twl_t w;
+  twl_defaults(cfg, &w);
+  const char* out = NULL;
+  const char* algo = "fenwick";
+  const char* dump = NULL;
+  int wsha = 0;
+  for (int i = 2; i < argc; ++i) {
+    if (!strcmp(argv[i], "--workload-sha")) { wsha = 1; continue; }
+    if (i + 1 >= argc) die(2, "missing value");
+    const char* k = argv[i]; const char* v = argv[++i];
+    if (!strcmp(k, "--cfg")) continue;
+    else if (!strcmp(k, "--S")) w.S = atoll(v);
+    else if (!strcmp(k, "--U")) w.U = atoi(v);
+    else if (!strcmp(k, "--Q")) w.Q = atoi(v);
+    else if (!strcmp(k, "--N")) w.N = strtoull(v, NULL, 10);
+    else if (!strcmp(k, "--seed")) w.seed = strtoull(v, NULL, 10);
+    else if (!strcmp(k, "--algo")) algo = v;
+    else if (!strcmp(k, "--out")) out = v;
+    else if (!strcmp(k, "--dump")) dump = v;
+    else die(2, "unknown argument");
+  }
+  if (w.S < 1 || w.U < 0 || w.Q < 1 || w.N < 1 || w.N > 0xFFFFFFFFull) die(2, "bad sizes");
+  if (cfg < 1 || cfg > 5) die(2, "bad cfg");
+  const double t0 = now_s();
+  if (twl_generate(&w) != 0) die(3, "out of memory");
+  const int64_t S = w.S;
+  const int U = w.U, Q = w.Q;
+  const size_t nu = (size_t)S * U, nq = (size_t)S * Q;
+  if (dump) {
+    /* plain arrays for oracle/ref_tree_harness.cpp: S,U,Q (i64) then the generated arrays */
+    FILE* fp = fopen(dump, "wb");
+    if (!fp) die(4, "cannot write dump");
+    const int64_t hdr[3] = {S, U, Q};
+    int ok = fwrite(hdr, sizeof(hdr), 1, fp) == 1;
+    ok &= fwrite(w.parent + 1, sizeof(int32_t), (size_t)S, fp) == (size_t)S;
+    ok &= fwrite(w.key, sizeof(uint32_t), nu, fp) == nu;
+    ok &= fwrite(w.delta, sizeof(int64_t), nu, fp) == nu;
+    ok &= fwrite(w.qlo, sizeof(uint32_t), nq, fp) == nq;
+    ok &= fwrite(w.qhi, sizeof(uint32_t), nq, fp) == nq;
+    if (fclose(fp) != 0 || !ok) die(4, "short dump write");
+  }
+  if (wsha) {
+    bw_sha256_ctx c;
+    bw_sha256_init(&c);
+    bw_sha256_update(&c, w.parent + 1, sizeof(int32_t) * (size_t)S);
+    bw_sha256_update(&c, w.key, sizeof(uint32_t) * nu);
+    bw_sha256_update(&c, w.delta, sizeof(int64_t) * nu);
+    bw_sha256_update(&c, w.qlo, sizeof(uint32_t) * nq);
+    bw_sha256_update(&c, w.qhi, sizeof(uint32_t) * nq);
+    if (w.dkey) bw_sha256_update(&c, w.dkey, sizeof(uint32_t) * w.D);
+    char hex[65];
+    bw_sha256_final_hex(&c, hex);
+    printf("{\"workload_sha256\":\"%s\"}\n", hex);
+    twl_free(&w);
+    return 0;
+  }
+  /* key -> Fenwick coordinate: the key itself, or its rank among the distinct keys (cfg 3) */
+  const uint64_t NK = (cfg == 3) ? w.D : w.N;
+  int64_t* ans = (int64_t*)calloc(nq, sizeof(int64_t));
+  if (!strcmp(algo, "brute")) {
+    /* independent: walk the ancestor chain, compare raw keys against raw endpoints */
+    __int128* acc = (__int128*)malloc(sizeof(__int128) * (size_t)Q);
+    for (int64_t v = 1; v <= S; ++v) {
+      for (int k = 0; k < Q; ++k) acc[k] = 0;
+      for (int64_t a = v; a > 0; a = w.parent[a]) {
+        const size_t base = (size_t)(a - 1) * U;
+        for (int i = 0; i < U; ++i) {
+          const uint64_t key = w.key[base + i];
+          for (int k = 0; k < Q; ++k) {
+            const size_t qi = (size_t)(v - 1) * Q + k;
+            if (key >= w.qlo[qi] && key < w.qhi[qi]) acc[k] += w.delta[base + i];
+          }
+        }
+      }
+      for (int k = 0; k < Q; ++k) ans[(size_t)(v - 1) * Q + k] = checked_i64(acc[k]);
+    }
+    free(acc);
+  } else {
+    uint32_t* krank = (uint32_t*)malloc(sizeof(uint32_t) * (nu ? nu : 1));
+    uint32_t* lrank = (uint32_t*)malloc(sizeof(uint32_t) * nq);
+    uint32_t* hrank = (uint32_t*)malloc(sizeof(uint32_t) * nq);
+    for (size_t i = 0; i < nu; ++i)
+      krank[i] = (cfg == 3) ? (uint32_t)lower_bound_u32(w.dkey, w.D, w.key[i]) : w.key[i];
+    for (size_t i = 0; i < nq; ++i) {
+      lrank[i] = (cfg == 3) ? (uint32_t)lower_bound_u32(w.dkey, w.D, w.qlo[i]) : w.qlo[i];
+      hrank[i] = (cfg == 3) ? (uint32_t)lower_bound_u32(w.dkey, w.D, w.qhi[i]) : w.qhi[i];
+    }
+    int64_t* fen = (int64_t*)calloc((size_t)NK + 1, sizeof(int64_t));
+    int64_t* koff = (int64_t*)calloc((size_t)S + 2, sizeof(int64_t));
+    int32_t* kid = (int32_t*)malloc(sizeof(int32_t) * (size_t)S);
+    for (int64_t v = 1; v <= S; ++v) koff[w.parent[v] + 1]++;
+    for (int64_t v = 0; v <= S; ++v) koff[v + 1] += koff[v];
+    {
+      int64_t* cur = (int64_t*)malloc(sizeof(int64_t) * (size_t)(S + 1));
+      memcpy(cur, koff, sizeof(int64_t) * (size_t)(S + 1));
+      for (int64_t v = 1; v <= S; ++v) kid[cur[w.parent[v]]++] = (int32_t)v;
+      free(cur);
+    }
+    int64_t* stk_v = (int64_t*)malloc(sizeof(int64_t) * (size_t)(S + 1));
+    int64_t* stk_c = (int64_t*)malloc(sizeof(int64_t) * (size_t)(S + 1));
+    int64_t sp = 1;
+    stk_v[0] = 0; stk_c[0] = koff[0];
+    while (sp > 0) {
+      const int64_t v = stk_v[sp - 1];
+      if (stk_c[sp - 1] < koff[v + 1]) {
+        const int64_t c = kid[stk_c[sp - 1]++];
+        const size_t base = (size_t)(c - 1) * U;
+        for (int i = 0; i < U; ++i)
+          for (uint64_t j = (uint64_t)krank[base + i] + 1; j <= NK; j += j & (~j + 1)) fen[j] += w.delta[base + i];
+        for (int k = 0; k < Q; ++k) {
+          const size_t qi = (size_t)(c - 1) * Q + k;
+          int64_t s = 0;
+          for (uint64_t j = hrank[qi]; j > 0; j -= j & (~j + 1)) s += fen[j];
+          for (uint64_t j = lrank[qi]; j > 0; j -= j & (~j + 1)) s -= fen[j];
+          ans[qi] = s;
+        }
+        stk_v[sp] = c; stk_c[sp] = koff[c]; ++sp;
+      } else {
+        if (v > 0) {
+          const size_t base = (size_t)(v - 1) * U;
+          for (int i = 0; i < U; ++i)
+            for (uint64_t j = (uint64_t)krank[base + i] + 1; j <= NK; j += j & (~j + 1)) fen[j] -= w.delta[base + i];
+        }
+        --sp;
+      }
+    }
+    free(fen); free(koff); free(kid); free(stk_v); free(stk_c); free(krank); free(lrank); free(hrank);
+  }
+  write_answers(out, ans, nq, t0);
+  free(ans);
+  twl_free(&w);
+  return 0;
+}
+
+static int mode_sha(int argc, char** argv) {
+  if (argc < 3) die(2, "usage: sha FILE");
+  FILE* fp = fopen(argv[2], "rb");
+  if (!fp) die(1, "cannot open");
+  bw_sha256_ctx c;
+  bw_sha256_init(&c);
+  unsigned char buf[1 << 16];
+  size_t n;
+  while ((n = fread(buf, 1, sizeof(buf), fp)) > 0) bw_sha256_update(&c, buf, n);
+  fclose(fp);
+  char hex[65];
+  bw_sha256_final_hex(&c, hex);
+  printf("%s\n", hex);
+  return 0;
+}
+
+int main(int argc, char** argv) {
+  if (argc < 2) die(2, "usage: bw_oracle sha|ref|tree ...");
+  if (!strcmp(argv[1], "sha")) return mode_sha(argc, argv);
+  if (!strcmp(argv[1], "ref")) return mode_ref(argc, argv);
+  if (!strcmp(argv[1], "tree")) return mode_tree(argc, argv);
+  die(2, "unknown mode");
+  return 2;
+}
