@@ -1,0 +1,197 @@
+// tree_workload.cpp -- generator of the benchmark tree workloads (see tree_workload.hpp).
+#include "tree_workload.hpp"
+
+#include <openssl/sha.h>
+#include <omp.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+namespace bwg {
+namespace {
+
+inline uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+// Counter-based draw: element `idx` of the splitmix64 sequence seeded by (seed, stream).
+struct Stream {
+  uint64_t base;
+  Stream(uint64_t seed, uint64_t stream) : base(mix64(seed ^ (stream * 0xD1B54A32D192ED03ULL))) {}
+  uint64_t at(uint64_t idx) const { return mix64(base + (idx + 1) * 0x9E3779B97F4A7C15ULL); }
+};
+
+inline double unit(uint64_t x) { return double(x >> 11) * (1.0 / 9007199254740992.0); }
+
+// Zipf(s) over M ranks by inverse CDF; the cumulative table is summed in rank order.
+struct Zipf {
+  std::vector<double> cum;
+  Zipf(uint64_t m, double s) : cum(m) {
+    double acc = 0.0;
+    for (uint64_t r = 0; r < m; ++r) {
+      acc += std::pow(double(r + 1), -s);
+      cum[r] = acc;
+    }
+  }
+  uint64_t pick(double u) const {
+    const double t = u * cum.back();
+    const size_t i = size_t(std::upper_bound(cum.begin(), cum.end(), t) - cum.begin());
+    return i < cum.size() ? i : cum.size() - 1;
+  }
+};
+
+void log_range(uint64_t dlen, uint64_t dpos, uint64_t n, uint32_t& lo, uint32_t& hi) {
+  const double l = std::floor(std::exp(unit(dlen) * std::log(double(n))));
+  uint64_t len = l < 1.0 ? 1 : uint64_t(l);
+  if (len > n) len = n;
+  const uint64_t a = dpos % (n - len + 1);
+  lo = uint32_t(a);
+  hi = uint32_t(a + len);
+}
+
+}  // namespace
+
+TreeSpec tree_defaults(int cfg) {
+  TreeSpec s;
+  s.cfg = cfg;
+  s.seed = 20261203ULL + uint64_t(cfg);
+  switch (cfg) {
+    case 1: s.S = 1000; s.U = 64; s.Q = 16; s.N = 65536; break;
+    case 3: s.S = 300000; s.U = 256; s.Q = 32; s.N = 4194304; break;
+    default: s.S = 300000; s.U = 128; s.Q = 32; s.N = 1u << 20; break;
+  }
+  return s;
+}
+
+void tree_generate(const TreeSpec& spec, TreeWorkload& w) {
+  w.spec = spec;
+  const int64_t S = spec.S;
+  const int U = spec.U, Q = spec.Q;
+  const uint64_t seed = spec.seed;
+  w.parent.assign(size_t(S) + 1, -1);
+  // parents: sequential for the run-length model, otherwise per state
+  const Stream p1(seed, 1), p2(seed, 2);
+  int64_t run = 1;
+  for (int64_t v = 1; v <= S; ++v) {
+    int64_t p;
+    if (spec.cfg == 1) {
+      p = v - 1 - int64_t(p1.at(uint64_t(v)) % uint64_t(std::min<int64_t>(v, 32)));
+    } else if (spec.cfg == 4) {
+      if (run < 4096 && unit(p1.at(uint64_t(v))) < 63.0 / 64.0) {
+        p = v - 1;
+        ++run;
+      } else {
+        p = v - 1 - int64_t(p2.at(uint64_t(v)) % uint64_t(std::min<int64_t>(v, 1024)));
+        run = 1;
+      }
+    } else {
+      p = unit(p1.at(uint64_t(v))) < 0.9 ? v - 1
+                                         : v - 1 - int64_t(p2.at(uint64_t(v)) % uint64_t(std::min<int64_t>(v, 1024)));
+    }
+    w.parent[size_t(v)] = int32_t(p);
+  }
+  const size_t nu = size_t(S) * size_t(U), nq = size_t(S) * size_t(Q);
+  w.key.resize(nu);
+  w.delta.resize(nu);
+  w.qlo.resize(nq);
+  w.qhi.resize(nq);
+  w.dkey.clear();
+  const Stream sk(seed, 3), sd(seed, 4), sq5(seed, 5), sq6(seed, 6);
+  if (spec.cfg == 1) {
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)nu; ++i) w.key[size_t(i)] = uint32_t(sk.at(uint64_t(i)) % spec.N);
+  } else if (spec.cfg == 3) {
+    // first D distinct values of stream 7 (open addressing over key+1), then sorted
+    const uint64_t D = spec.N;
+    uint64_t cap = 1;
+    while (cap < 2 * D) cap <<= 1;
+    std::vector<uint32_t> tab(cap, 0);
+    bool top_seen = false;
+    const Stream s7(seed, 7);
+    w.dkey.reserve(D);
+    for (uint64_t j = 0; w.dkey.size() < D; ++j) {
+      const uint32_t c = uint32_t(s7.at(j) >> 32);
+      if (c == 0xFFFFFFFFu) {
+        if (!top_seen) { top_seen = true; w.dkey.push_back(c); }
+        continue;
+      }
+      uint64_t h = (mix64(uint64_t(c)) >> 7) & (cap - 1);
+      bool fresh = true;
+      while (tab[h] != 0) {
+        if (tab[h] == c + 1u) { fresh = false; break; }
+        h = (h + 1) & (cap - 1);
+      }
+      if (fresh) { tab[h] = c + 1u; w.dkey.push_back(c); }
+    }
+    std::sort(w.dkey.begin(), w.dkey.end());
+    const Zipf z(D, 0.9);
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)nu; ++i) w.key[size_t(i)] = w.dkey[z.pick(unit(sk.at(uint64_t(i))))];
+  } else {
+    const Zipf z(spec.N, 1.1);
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)nu; ++i) w.key[size_t(i)] = uint32_t(z.pick(unit(sk.at(uint64_t(i)))));
+  }
+  const int64_t R = spec.cfg == 1 ? 1000 : (spec.cfg == 3 ? 10000 : 1000000);
+#pragma omp parallel for schedule(static)
+  for (long long i = 0; i < (long long)nu; ++i)
+    w.delta[size_t(i)] = int64_t(sd.at(uint64_t(i)) % uint64_t(2 * R + 1)) - R;
+#pragma omp parallel for schedule(static)
+  for (long long i = 0; i < (long long)nq; ++i) {
+    const uint64_t a5 = sq5.at(uint64_t(i)), a6 = sq6.at(uint64_t(i));
+    if (spec.cfg == 1) {
+      const uint32_t a = uint32_t(a5 % spec.N), b = uint32_t(a6 % spec.N);
+      w.qlo[size_t(i)] = std::min(a, b);
+      w.qhi[size_t(i)] = std::max(a, b) + 1u;
+    } else {
+      log_range(a5, a6, spec.cfg == 3 ? 0xFFFFFFFFull : spec.N, w.qlo[size_t(i)], w.qhi[size_t(i)]);
+    }
+  }
+}
+
+void tree_workload_sha(const TreeWorkload& w, char out_hex[65]) {
+  SHA256_CTX c;
+  SHA256_Init(&c);
+  SHA256_Update(&c, w.parent.data() + 1, sizeof(int32_t) * size_t(w.spec.S));
+  SHA256_Update(&c, w.key.data(), sizeof(uint32_t) * w.key.size());
+  SHA256_Update(&c, w.delta.data(), sizeof(int64_t) * w.delta.size());
+  SHA256_Update(&c, w.qlo.data(), sizeof(uint32_t) * w.qlo.size());
+  SHA256_Update(&c, w.qhi.data(), sizeof(uint32_t) * w.qhi.size());
+  if (!w.dkey.empty()) SHA256_Update(&c, w.dkey.data(), sizeof(uint32_t) * w.dkey.size());
+  unsigned char d[32];
+  SHA256_Final(d, &c);
+  static const char* hx = "0123456789abcdef";
+  for (int i = 0; i < 32; ++i) {
+    out_hex[2 * i] = hx[d[i] >> 4];
+    out_hex[2 * i + 1] = hx[d[i] & 15];
+  }
+  out_hex[64] = 0;
+}
+
+RankLut build_rank_lut(const std::vector<uint32_t>& keys) {
+  RankLut l;
+  l.row_ptr.assign(65537, 0);
+  l.low16.resize(keys.size());
+  for (size_t i = 0; i < keys.size(); ++i) {
+    l.row_ptr[(keys[i] >> 16) + 1]++;
+    l.low16[i] = uint16_t(keys[i] & 0xFFFFu);
+  }
+  for (size_t h = 0; h < 65536; ++h) l.row_ptr[h + 1] += l.row_ptr[h];
+  return l;
+}
+
+void host_ranks(const RankLut& l, const uint32_t* in, uint32_t* out, size_t n) {
+#pragma omp parallel for schedule(static)
+  for (long long i = 0; i < (long long)n; ++i) {
+    const uint32_t x = in[i];
+    const uint16_t* b = l.low16.data() + l.row_ptr[x >> 16];
+    const uint16_t* e = l.low16.data() + l.row_ptr[(x >> 16) + 1];
+    out[i] = uint32_t(std::lower_bound(b, e, uint16_t(x & 0xFFFFu)) - l.low16.data());
+  }
+}
+
+}  // namespace bwg

@@ -1,0 +1,58 @@
+// tree_workload.hpp -- branching multi-update point-update workloads of the benchmark configs.
+//
+// The reference's declared workload has one edit per state on chain/bushy/star trees
+// (src/common.hpp:194-374).  These workloads keep its contract (states answer their own range
+// queries after their own updates, answers in (state, query) order, src/oracle.cpp:10-18) and
+// its counter-based splitmix64 generation style (src/batch_engine_v7c.hpp:240-249), with the
+// shapes of the five benchmark configs:
+//   cfg 1  N=65,536, 1,000 states, parent uniform over the 32 most recent, 64 updates
+//          (uniform keys, deltas in [-1000,1000]), 16 uniform [lo,hi) queries per state
+//   cfg 2  N=2^20, 300,000 states, parent = previous w.p. 0.9 else uniform over the prior
+//          1,024, 128 Zipf(1.1) updates (deltas in [-10^6,10^6]), 32 log-uniform ranges
+//   cfg 3  4,194,304 distinct 32-bit keys, 300,000 states, 256 Zipf(0.9) updates over the
+//          distinct set (deltas in [-10^4,10^4]), 32 queries with raw 32-bit endpoints
+//   cfg 4  cfg 2 with geometric runs (mean 64, max 4,096)
+//   cfg 5  cfg 2 (served)
+// The exact draw definitions are restated in the header comment of the test oracle's
+// generator; the two implementations are checked byte for byte by the test suite.
+#pragma once
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+namespace bwg {
+
+struct TreeSpec {
+  int cfg = 2;
+  int64_t S = 300000;  // edited states (1..S); state 0 is the empty root
+  int U = 128, Q = 32;
+  uint64_t N = 1u << 20;  // key domain, or distinct-key count for cfg 3
+  uint64_t seed = 0;
+};
+
+struct TreeWorkload {
+  TreeSpec spec;
+  std::vector<int32_t> parent;  // S+1, parent[0] = -1
+  std::vector<uint32_t> key;    // S*U raw keys
+  std::vector<int64_t> delta;   // S*U
+  std::vector<uint32_t> qlo, qhi;  // S*Q raw endpoints
+  std::vector<uint32_t> dkey;   // cfg 3: sorted distinct keys
+  uint64_t coord_n() const { return spec.cfg == 3 ? dkey.size() : spec.N; }
+};
+
+TreeSpec tree_defaults(int cfg);
+void tree_generate(const TreeSpec& spec, TreeWorkload& w);
+// SHA-256 (hex) over parent[1..S], key, delta, qlo, qhi (and dkey for cfg 3), little-endian.
+// Matches the oracle's --workload-sha.
+void tree_workload_sha(const TreeWorkload& w, char out_hex[65]);
+
+// cfg 3: CSR rank-LUT over the distinct keys (row = high 16 bits): 65,537 row pointers and the
+// low 16 bits of every key, plus host ranks of raw values (#keys < x).
+struct RankLut {
+  std::vector<uint32_t> row_ptr;
+  std::vector<uint16_t> low16;
+};
+RankLut build_rank_lut(const std::vector<uint32_t>& sorted_keys);
+void host_ranks(const RankLut& lut, const uint32_t* in, uint32_t* out, size_t n);
+
+}  // namespace bwg

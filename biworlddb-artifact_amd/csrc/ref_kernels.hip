@@ -1,0 +1,565 @@
+// ref_kernels.hip -- CDNA4 (gfx950) implementation of the reference drop-in ABI
+// (gpu_v7.h, gpu_v7b.h, gpu_session.h) and the device carry (bwgpu_carry.h).
+//
+// Semantics follow the reference kernels (src/gpu_v7.cu:10-96, src/gpu_v7b.cu:13-112,
+// src/gpu_session.cu:11-119) exactly -- same lvals, dV, devbuf updates and answers -- but the
+// execution is redesigned for 64-wide wavefronts:
+//  * Row edits are not applied one state at a time by thread 0 behind two barriers per state
+//    (src/gpu_v7.cu:24-40,79).  Consecutive row-edit states form a window (up to 64 states,
+//    ending before the next factor edit): one lane per state computes its two weighted terms,
+//    every query of the window is the current prefix plus the window terms of states up to its
+//    own, then the terms are added and the prefix is rebuilt -- four barriers per window.
+//  * The LDS variant keeps a dense inclusive prefix (pref[i] = sum_{j<=i} leaf[j], exactly the
+//    reference Fenwick prefix) rebuilt by a workgroup scan built on DPP wave scans, instead of a
+//    Fenwick tree rebuilt level by level (src/gpu_v7.cu:55-65).
+//  * Window boundaries come from a wave ballot over the next 64 node kinds, computed redundantly
+//    by every wave (no LDS round trip).
+//  * The session variant keeps a Fenwick tree in global memory (no class limit) with the same
+//    windows, and -- unlike src/gpu_session.cu -- writes dV, so multi-block runs are exact.
+//  * Snapshot terms with a zero amount are skipped (they add 0; the reference would still touch
+//    devbuf at an unset ti, src/gpu_v7b.cu:23-24).
+#include <hip/hip_runtime.h>
+#include <mutex>
+#include <stdint.h>
+#include <stdio.h>
+
+#include "bwgpu_carry.h"
+#include "gpu_session.h"
+#include "gpu_v7.h"
+#include "gpu_v7b.h"
+#include "wave64.hpp"
+
+using namespace bw;
+
+namespace {
+
+constexpr int kMaxWin = 64;
+
+// Common view of the three block-replay argument structs.
+struct Replay {
+  int B, G, Cpfx, ncls, q, win;
+  const void* nodes;
+  const int32_t *bv0, *bv1;
+  const int64_t* fac_start;
+  const int32_t* g2row;
+  const int32_t* ms_off;
+  const int64_t *ms_cls, *ms_base;
+  const int32_t* ms_dev;
+  int64_t* devbuf;
+  const int64_t* need_off;
+  const int32_t *qlb, *qhb;
+  int64_t *lvals, *dV;
+  int64_t *gfen, *gleaf;  // session (global) variant
+};
+
+struct Node {
+  int kind;  // 1 row, 2 factor
+  bool ex_old, ex_new;
+  int g_old, g_new, c_old, c_new, ti_old, ti_new;
+  int64_t a, b;  // row: amt_old, amt_new; factor: fac_val, fac_prev
+};
+
+template <int FMT>
+__device__ __forceinline__ Node load_node(const void* p, int v);
+
+template <>
+__device__ __forceinline__ Node load_node<0>(const void* p, int v) {  // GpuV7Node, 72 bytes
+  const GpuV7Node& s = static_cast<const GpuV7Node*>(p)[v];
+  Node n;
+  n.kind = s.kind;
+  n.ex_old = s.ex_old != 0; n.ex_new = s.ex_new != 0;
+  n.g_old = s.g_old; n.g_new = s.g_new; n.c_old = s.c_old; n.c_new = s.c_new;
+  n.ti_old = s.ti_old; n.ti_new = s.ti_new;
+  if (s.kind == 2) { n.a = s.fac_val; n.b = s.fac_prev; } else { n.a = s.amt_old; n.b = s.amt_new; }
+  return n;
+}
+
+template <>
+__device__ __forceinline__ Node load_node<1>(const void* p, int v) {  // GpuV7bNode / GpuSessNode, 40 bytes
+  const GpuV7bNode& s = static_cast<const GpuV7bNode*>(p)[v];
+  Node n;
+  n.kind = s.kind & 3;
+  n.ex_old = (s.kind & 4) != 0; n.ex_new = (s.kind & 8) != 0;
+  n.g_old = s.g_old; n.g_new = s.g_new; n.c_old = s.c_old; n.c_new = s.c_new;
+  n.ti_old = s.ti_old; n.ti_new = s.ti_new;
+  n.a = s.a; n.b = s.b;
+  return n;
+}
+
+template <int FMT>
+__device__ __forceinline__ int node_kind(const void* p, int v) {
+  if (FMT == 0) return static_cast<const GpuV7Node*>(p)[v].kind;
+  return static_cast<const GpuV7bNode*>(p)[v].kind & 3;
+}
+
+__device__ __forceinline__ int64_t ld_agent(const int64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void add_agent(int64_t* p, int64_t v) {
+  atomicAdd(reinterpret_cast<unsigned long long*>(p), (unsigned long long)v);
+}
+
+// End (exclusive) of the row-edit window starting at v: the first factor edit in
+// [v, min(v+win, v1)), else that bound.  Every wave computes the same value.
+template <int FMT>
+__device__ __forceinline__ int window_end(const Replay& a, int v, int v1) {
+  const int lane = threadIdx.x & 63;
+  const int lim = min(v1, v + a.win);
+  const bool fac = (v + lane < lim) && node_kind<FMT>(a.nodes, v + lane) == 2;
+  const uint64_t m = __ballot(fac);
+  return m ? v + __builtin_ctzll(m) : lim;
+}
+
+// Window terms: lane s of the window computes state v+s's two weighted terms (position c+1 or
+// 0 for none) and applies its devbuf deviations (src/gpu_v7.cu:26-38 per state).
+template <int FMT>
+__device__ __forceinline__ void window_terms(const Replay& a, int b, int v, int n, const int64_t* fac,
+                                             int32_t* wc, int64_t* ww) {
+  for (int s = threadIdx.x; s < n; s += blockDim.x) {
+    const Node p = load_node<FMT>(a.nodes, v + s);
+    int32_t c0 = 0, c1 = 0;
+    int64_t w0 = 0, w1 = 0;
+    if (p.ex_old && p.a != 0) {
+      c0 = p.c_old + 1;
+      w0 = -(fac[p.g_old] * p.a);
+      const int64_t off = a.need_off[(size_t)b * a.G + p.g_old];
+      if (off >= 0) add_agent(&a.devbuf[off + p.ti_old], -p.a);
+    }
+    if (p.ex_new && p.b != 0) {
+      c1 = p.c_new + 1;
+      w1 = fac[p.g_new] * p.b;
+      const int64_t off = a.need_off[(size_t)b * a.G + p.g_new];
+      if (off >= 0) add_agent(&a.devbuf[off + p.ti_new], p.b);
+    }
+    wc[2 * s] = c0; ww[2 * s] = w0;
+    wc[2 * s + 1] = c1; ww[2 * s + 1] = w1;
+  }
+}
+
+// Sum of the window terms of states v..v+s (inclusive) whose position is <= idx.
+__device__ __forceinline__ int64_t window_correction(const int32_t* wc, const int64_t* ww, int s, int idx) {
+  int64_t r = 0;
+  for (int j = 0; j < 2 * (s + 1); ++j) {
+    const int32_t c = wc[j];
+    if (c != 0 && c <= idx) r += ww[j];
+  }
+  return r;
+}
+
+// Inclusive workgroup scan pref[i] = sum_{j<=i} leaf[j] over n LDS values.
+__device__ inline void block_incl_scan(const int64_t* leaf, int64_t* pref, int n, int64_t* tmp) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int lane = tid & 63, wv = tid >> 6, nw = nt >> 6;
+  const int per = (n + nt - 1) / nt;
+  const int b0 = min(n, tid * per), b1 = min(n, b0 + per);
+  int64_t s = 0;
+  for (int i = b0; i < b1; ++i) s += leaf[i];
+  const int64_t inc = wave_incl_scan(s);
+  if (lane == 63) tmp[wv] = inc;
+  __syncthreads();
+  if (tid < 64) {
+    const int64_t t = (tid < nw) ? tmp[tid] : 0;
+    const int64_t ti = wave_incl_scan(t);
+    if (tid < nw) tmp[tid] = ti - t;
+  }
+  __syncthreads();
+  int64_t run = tmp[wv] + inc - s;
+  for (int i = b0; i < b1; ++i) { run += leaf[i]; pref[i] = run; }
+  __syncthreads();
+}
+
+// ------------------------------------------------------------- LDS block replay (v7, v7b, session)
+template <int FMT>
+__global__ void __launch_bounds__(1024) ref_block_lds_kernel(const Replay a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int Cpfx = a.Cpfx, G = a.G, q = a.q;
+  int64_t* leaf = reinterpret_cast<int64_t*>(smem_raw);
+  int64_t* pref = leaf + Cpfx;
+  int64_t* fac = pref + Cpfx;
+  int64_t* ww = fac + G;
+  int64_t* tmp = ww + 2 * kMaxWin;
+  int32_t* wc = reinterpret_cast<int32_t*>(tmp + 17);
+  const int b = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+  for (int i = tid; i < Cpfx; i += nt) { leaf[i] = 0; pref[i] = 0; }
+  for (int i = tid; i < G; i += nt) fac[i] = a.fac_start[(size_t)b * G + i];
+  __syncthreads();
+  const int v1 = a.bv1[b];
+  int v = a.bv0[b];
+  while (v < v1) {
+    if (node_kind<FMT>(a.nodes, v) == 2) {
+      const Node p = load_node<FMT>(a.nodes, v);
+      const int g = p.g_old;
+      const int row = a.g2row[g];
+      const int64_t off = a.need_off[(size_t)b * G + g];
+      if (row >= 0 && off >= 0) {
+        const int64_t ds = p.a - p.b;
+        const int kb = a.ms_off[row], ke = a.ms_off[row + 1];
+        for (int k = kb + tid; k < ke; k += nt) {
+          const int di = a.ms_dev[k];
+          const int64_t dev = (di >= 0) ? ld_agent(&a.devbuf[off + di]) : 0;
+          const int64_t d = ds * (a.ms_base[k] + dev);
+          if (d) leaf[(int)a.ms_cls[k] + 1] += d;  // support classes are distinct
+        }
+        __syncthreads();
+        block_incl_scan(leaf, pref, Cpfx, tmp);
+      }
+      if (tid == 0) fac[g] = p.a;
+      for (int e = tid; e < 2 * q; e += nt) {
+        const int idx = (e & 1) ? a.qhb[v * q + (e >> 1)] : a.qlb[v * q + (e >> 1)];
+        a.lvals[(size_t)(v - 1) * (2 * q) + e] = pref[idx];
+      }
+      __syncthreads();
+      ++v;
+      continue;
+    }
+    const int ve = window_end<FMT>(a, v, v1);
+    const int n = ve - v;
+    window_terms<FMT>(a, b, v, n, fac, wc, ww);
+    __syncthreads();
+    for (int pidx = tid; pidx < n * 2 * q; pidx += nt) {
+      const int s = pidx / (2 * q), e = pidx - s * 2 * q;
+      const int vs = v + s;
+      const int idx = (e & 1) ? a.qhb[vs * q + (e >> 1)] : a.qlb[vs * q + (e >> 1)];
+      a.lvals[(size_t)(vs - 1) * (2 * q) + e] = pref[idx] + window_correction(wc, ww, s, idx);
+    }
+    for (int j = tid; j < 2 * n; j += nt)
+      if (wc[j] != 0) lds_add64(&leaf[wc[j]], ww[j]);
+    __syncthreads();
+    block_incl_scan(leaf, pref, Cpfx, tmp);
+    v = ve;
+  }
+  __syncthreads();
+  for (int i = tid; i < Cpfx; i += nt) a.dV[(size_t)b * Cpfx + i] = leaf[i];
+}
+
+// ------------------------------------------------------------- global-memory block replay (session)
+__device__ __forceinline__ int64_t gfen_prefix(const int64_t* fen, int idx) {
+  int64_t r = 0;
+  for (int i = idx; i > 0; i -= i & -i) r += ld_agent(&fen[i]);
+  return r;
+}
+
+__global__ void __launch_bounds__(1024) ref_block_global_kernel(const Replay a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int Cpfx = a.Cpfx, G = a.G, q = a.q, ncls = a.ncls;
+  int64_t* fac = reinterpret_cast<int64_t*>(smem_raw);
+  int64_t* ww = fac + G;
+  int32_t* wc = reinterpret_cast<int32_t*>(ww + 2 * kMaxWin);
+  const int b = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+  int64_t* fen = a.gfen + (size_t)b * Cpfx;
+  int64_t* leaf = a.gleaf + (size_t)b * Cpfx;
+  for (int i = tid; i < Cpfx; i += nt) {
+    __hip_atomic_store(&fen[i], (int64_t)0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&leaf[i], (int64_t)0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  for (int i = tid; i < G; i += nt) fac[i] = a.fac_start[(size_t)b * G + i];
+  __syncthreads();
+  const int v1 = a.bv1[b];
+  int v = a.bv0[b];
+  while (v < v1) {
+    if (node_kind<1>(a.nodes, v) == 2) {
+      const Node p = load_node<1>(a.nodes, v);
+      const int g = p.g_old;
+      const int row = a.g2row[g];
+      const int64_t off = a.need_off[(size_t)b * G + g];
+      if (row >= 0 && off >= 0) {
+        const int64_t ds = p.a - p.b;
+        const int kb = a.ms_off[row], ke = a.ms_off[row + 1];
+        for (int k = kb + tid; k < ke; k += nt) {
+          const int di = a.ms_dev[k];
+          const int64_t dev = (di >= 0) ? ld_agent(&a.devbuf[off + di]) : 0;
+          const int64_t d = ds * (a.ms_base[k] + dev);
+          if (d) add_agent(&leaf[(int)a.ms_cls[k] + 1], d);
+        }
+        __syncthreads();
+        // rebuild the Fenwick tree from the leaf: copy, then one level per lowest-set-bit s
+        for (int i = tid; i < Cpfx; i += nt)
+          __hip_atomic_store(&fen[i], ld_agent(&leaf[i]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        for (int s = 1; s < Cpfx; s <<= 1) {
+          for (int64_t i = (int64_t)s * (2 * tid + 1); i + s < Cpfx; i += 2 * (int64_t)s * nt)
+            add_agent(&fen[i + s], ld_agent(&fen[i]));
+          __syncthreads();
+        }
+      }
+      if (tid == 0) fac[g] = p.a;
+      for (int e = tid; e < 2 * q; e += nt) {
+        const int idx = (e & 1) ? a.qhb[v * q + (e >> 1)] : a.qlb[v * q + (e >> 1)];
+        a.lvals[(size_t)(v - 1) * (2 * q) + e] = gfen_prefix(fen, idx);
+      }
+      __syncthreads();
+      ++v;
+      continue;
+    }
+    const int ve = window_end<1>(a, v, v1);
+    const int n = ve - v;
+    window_terms<1>(a, b, v, n, fac, wc, ww);
+    __syncthreads();
+    for (int pidx = tid; pidx < n * 2 * q; pidx += nt) {
+      const int s = pidx / (2 * q), e = pidx - s * 2 * q;
+      const int vs = v + s;
+      const int idx = (e & 1) ? a.qhb[vs * q + (e >> 1)] : a.qlb[vs * q + (e >> 1)];
+      a.lvals[(size_t)(vs - 1) * (2 * q) + e] = gfen_prefix(fen, idx) + window_correction(wc, ww, s, idx);
+    }
+    __syncthreads();
+    for (int j = tid; j < 2 * n; j += nt) {
+      const int c = wc[j];
+      if (c == 0) continue;
+      add_agent(&leaf[c], ww[j]);
+      for (int i = c; i <= ncls; i += i & -i) add_agent(&fen[i], ww[j]);
+    }
+    __syncthreads();
+    v = ve;
+  }
+  __syncthreads();
+  for (int i = tid; i < Cpfx; i += nt) a.dV[(size_t)b * Cpfx + i] = ld_agent(&leaf[i]);
+}
+
+// ------------------------------------------------------------- snapshots
+template <int FMT>
+__global__ void __launch_bounds__(256) ref_snap_kernel(int nstates, const void* nodes, const int32_t* bidx,
+                                                      const int32_t* gb_off, const int32_t* gb_block,
+                                                      const int64_t* gb_reg, int64_t* devbuf) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  if (v >= nstates) return;
+  const Node p = load_node<FMT>(nodes, v);
+  if (p.kind != 1) return;
+  const int b0 = bidx[v];
+  for (int t = 0; t < 2; ++t) {
+    const bool on = t == 0 ? (p.ex_old && p.a != 0) : (p.ex_new && p.b != 0);
+    if (!on) continue;
+    const int g = t == 0 ? p.g_old : p.g_new;
+    const int ti = t == 0 ? p.ti_old : p.ti_new;
+    const int64_t d = t == 0 ? -p.a : p.b;
+    for (int at = gb_off[g]; at < gb_off[g + 1]; ++at)
+      if (gb_block[at] > b0) add_agent(&devbuf[gb_reg[at] + ti], d);
+  }
+}
+
+// ------------------------------------------------------------- reset, combine, carry
+__global__ void __launch_bounds__(256) ref_reset_kernel(int64_t n, int64_t* fen, int64_t* leaf) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    fen[i] = 0;
+    leaf[i] = 0;
+  }
+}
+
+__global__ void __launch_bounds__(256) ref_combine_kernel(int nstates, int q, int Cpfx, const int32_t* qlb,
+                                                         const int32_t* qhb, const int64_t* carrypref,
+                                                         const int32_t* bidx, const int64_t* lvals, int64_t* answers) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)nstates * q) return;
+  const int v = (int)(i / q) + 1, k = (int)(i % q);
+  const int64_t* cp = carrypref + (size_t)bidx[v] * Cpfx;
+  answers[i] = (cp[qhb[v * q + k]] - cp[qlb[v * q + k]]) + (lvals[2 * i + 1] - lvals[2 * i]);
+}
+
+// carrypref[b][k] (k>=1) <- vbase[k] + sum_{b'<b} dV[b'][k]; then an inclusive scan per row.
+__global__ void __launch_bounds__(256) carry_columns_kernel(BwCarryArgs a) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= a.Cpfx) return;
+  int64_t run = (k == 0) ? 0 : a.vbase[k];
+  for (int b = 0; b < a.B; ++b) {
+    const int64_t t = (k == 0) ? 0 : a.dV[(size_t)b * a.Cpfx + k];
+    a.carrypref[(size_t)b * a.Cpfx + k] = run;
+    run += t;
+  }
+}
+
+__global__ void __launch_bounds__(256) carry_rows_kernel(BwCarryArgs a) {
+  __shared__ int64_t tmp[8];
+  int64_t* row = a.carrypref + (size_t)blockIdx.x * a.Cpfx;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  int64_t carry = 0;
+  for (int base = 0; base < a.Cpfx; base += 256) {
+    const int k = base + tid;
+    const int64_t x = (k < a.Cpfx && k > 0) ? row[k] : 0;
+    const int64_t inc = wave_incl_scan(x);
+    if (lane == 63) tmp[wv] = inc;
+    __syncthreads();
+    int64_t before = carry;
+    for (int w = 0; w < wv; ++w) before += tmp[w];
+    const int64_t tile = tmp[0] + tmp[1] + tmp[2] + tmp[3];
+    if (k < a.Cpfx) row[k] = before + inc;
+    __syncthreads();
+    carry += tile;
+  }
+}
+
+// ------------------------------------------------------------- launch helpers
+constexpr size_t kLdsLimit = 160 * 1024;
+
+int clamp_threads(int t) {
+  if (t < 64) t = 64;
+  if (t > 1024) t = 1024;
+  return (t + 63) & ~63;
+}
+
+int window_for(int q, int threads) {
+  int w = threads / (q > 0 ? q : 1);
+  if (w > kMaxWin) w = kMaxWin;
+  if (w < 1) w = 1;
+  return w;
+}
+
+size_t lds_dense_bytes(int Cpfx, int G) {
+  return (2 * (size_t)Cpfx + (size_t)G + 2 * kMaxWin + 17) * sizeof(int64_t) + 2 * kMaxWin * sizeof(int32_t);
+}
+size_t lds_global_bytes(int G) { return ((size_t)G + 2 * kMaxWin) * sizeof(int64_t) + 2 * kMaxWin * sizeof(int32_t); }
+
+int report(const char* what, hipError_t e) {
+  if (e != hipSuccess) fprintf(stderr, "bwgpu: %s: %s\n", what, hipGetErrorString(e));
+  return (int)e;
+}
+
+int set_limits() {
+  static std::once_flag once;
+  static hipError_t err = hipSuccess;
+  std::call_once(once, [] {
+    const void* ks[] = {reinterpret_cast<const void*>(ref_block_lds_kernel<0>),
+                        reinterpret_cast<const void*>(ref_block_lds_kernel<1>),
+                        reinterpret_cast<const void*>(ref_block_global_kernel)};
+    for (const void* k : ks) {
+      hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsLimit);
+      if (e != hipSuccess && err == hipSuccess) err = e;
+    }
+  });
+  return (int)err;
+}
+
+template <typename A>
+Replay make_replay(const A* a, const void* nodes, const int32_t* bv0, const int32_t* bv1, int threads) {
+  Replay r{};
+  r.B = a->B; r.G = a->G; r.Cpfx = a->Cpfx; r.ncls = a->ncls; r.q = a->q;
+  r.win = window_for(a->q, threads);
+  r.nodes = nodes; r.bv0 = bv0; r.bv1 = bv1;
+  r.fac_start = a->fac_start; r.g2row = a->g2row; r.ms_off = a->ms_off; r.ms_cls = a->ms_cls;
+  r.ms_base = a->ms_base; r.ms_dev = a->ms_dev; r.devbuf = a->devbuf; r.need_off = a->need_off;
+  r.qlb = a->qlb; r.qhb = a->qhb; r.lvals = a->lvals; r.dV = a->dV;
+  return r;
+}
+
+bool bad_replay(const Replay& r) {
+  // B = 0 is what the reference drivers pass for a non-chain tree (no v7 blocks); refuse it
+  return r.B < 1 || r.G < 1 || r.Cpfx < 2 || r.ncls != r.Cpfx - 1 || r.q < 1;
+}
+
+template <int FMT>
+int launch_lds(const Replay& r, int threads, const char* what) {
+  if (bad_replay(r)) return (int)hipErrorInvalidValue;
+  const size_t lds = lds_dense_bytes(r.Cpfx, r.G);
+  if (lds > kLdsLimit) {
+    fprintf(stderr, "bwgpu: %s: Cpfx=%d G=%d needs %zu B of LDS (> %zu); use the session path\n", what, r.Cpfx, r.G,
+            lds, kLdsLimit);
+    return (int)hipErrorInvalidValue;
+  }
+  int rc = set_limits();
+  if (rc) return rc;
+  hipLaunchKernelGGL(ref_block_lds_kernel<FMT>, dim3(r.B), dim3(threads), lds, (hipStream_t)0, r);
+  return report(what, hipGetLastError());
+}
+
+}  // namespace
+
+// ------------------------------------------------------------- exported C-ABI
+extern "C" int gpu_v7_launch(const GpuV7Args* a) {
+  if (!a) return (int)hipErrorInvalidValue;
+  const int t = clamp_threads(a->threads);
+  return launch_lds<0>(make_replay(a, a->nodes, a->block_v0, a->block_v1, t), t, "gpu_v7_launch");
+}
+
+extern "C" int gpu_combine_launch(const GpuV7CombineArgs* a) {
+  if (!a || a->nstates < 0 || a->q < 1) return (int)hipErrorInvalidValue;
+  const int64_t n = (int64_t)a->nstates * a->q;
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(ref_combine_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)0,
+                     a->nstates, a->q, a->Cpfx, a->qlb, a->qhb, a->carrypref, a->bidx, a->lvals, a->answers);
+  return report("gpu_combine_launch", hipGetLastError());
+}
+
+extern "C" int gpu_v7b_snap_launch(const GpuV7bSnapArgs* a) {
+  if (!a || a->nstates < 1) return (int)hipErrorInvalidValue;
+  const int total = a->nstates - 1;
+  if (total == 0) return 0;
+  hipLaunchKernelGGL(ref_snap_kernel<1>, dim3((total + 255) / 256), dim3(256), 0, (hipStream_t)0, a->nstates,
+                     (const void*)a->nodes, a->bidx, a->gb_off, a->gb_block, a->gb_reg, a->devbuf);
+  return report("gpu_v7b_snap_launch", hipGetLastError());
+}
+
+extern "C" int gpu_v7b_launch(const GpuV7bArgs* a) {
+  if (!a) return (int)hipErrorInvalidValue;
+  const int t = clamp_threads(a->threads);
+  return launch_lds<1>(make_replay(a, a->nodes, a->bv0, a->bv1, t), t, "gpu_v7b_launch");
+}
+
+extern "C" int gpu_v7b_combine_launch(const GpuV7bCombineArgs* a) {
+  if (!a) return (int)hipErrorInvalidValue;
+  const GpuV7CombineArgs c{a->nstates, a->q, a->Cpfx, a->qlb, a->qhb, a->carrypref, a->bidx, a->lvals, a->answers};
+  return gpu_combine_launch(&c);
+}
+
+extern "C" int gpu_sess_reset_launch(const GpuSessResetArgs* a) {
+  if (!a || a->B < 0 || a->Cpfx < 0) return (int)hipErrorInvalidValue;
+  const int64_t n = (int64_t)a->B * a->Cpfx;
+  if (n == 0) return 0;
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 65535) blocks = 65535;
+  hipLaunchKernelGGL(ref_reset_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)0, n, a->fen, a->leaf);
+  return report("gpu_sess_reset_launch", hipGetLastError());
+}
+
+extern "C" int gpu_sess_snap_launch(const GpuSessSnapArgs* a) {
+  if (!a || a->nstates < 1) return (int)hipErrorInvalidValue;
+  const int total = a->nstates - 1;
+  if (total == 0) return 0;
+  hipLaunchKernelGGL(ref_snap_kernel<1>, dim3((total + 255) / 256), dim3(256), 0, (hipStream_t)0, a->nstates,
+                     (const void*)a->nodes, a->bidx, a->gb_off, a->gb_block, a->gb_reg, a->devbuf);
+  return report("gpu_sess_snap_launch", hipGetLastError());
+}
+
+extern "C" int gpu_sess_run_launch(const GpuSessRunArgs* a) {
+  if (!a) return (int)hipErrorInvalidValue;
+  const int t = clamp_threads(a->threads);
+  Replay r = make_replay(a, a->nodes, a->bv0, a->bv1, t);
+  r.gfen = a->fen;
+  r.gleaf = a->leaf;
+  if (bad_replay(r)) return (int)hipErrorInvalidValue;
+  if (lds_dense_bytes(r.Cpfx, r.G) <= kLdsLimit) return launch_lds<1>(r, t, "gpu_sess_run_launch");
+  const size_t lds = lds_global_bytes(r.G);
+  if (lds > kLdsLimit || !r.gfen || !r.gleaf) return (int)hipErrorInvalidValue;
+  int rc = set_limits();
+  if (rc) return rc;
+  hipLaunchKernelGGL(ref_block_global_kernel, dim3(r.B), dim3(t), lds, (hipStream_t)0, r);
+  return report("gpu_sess_run_launch", hipGetLastError());
+}
+
+extern "C" int gpu_sess_combine_launch(const GpuSessCombineArgs* a) {
+  if (!a) return (int)hipErrorInvalidValue;
+  const GpuV7CombineArgs c{a->nstates, a->q, a->Cpfx, a->qlb, a->qhb, a->carrypref, a->bidx, a->lvals, a->answers};
+  return gpu_combine_launch(&c);
+}
+
+extern "C" int bw_carry_launch(const BwCarryArgs* a, void* stream) {
+  if (!a || a->B < 0 || a->Cpfx < 1) return (int)hipErrorInvalidValue;
+  if (a->B == 0) return 0;
+  hipLaunchKernelGGL(carry_columns_kernel, dim3((a->Cpfx + 255) / 256), dim3(256), 0, (hipStream_t)stream, *a);
+  int rc = report("bw_carry_launch (columns)", hipGetLastError());
+  if (rc) return rc;
+  hipLaunchKernelGGL(carry_rows_kernel, dim3(a->B), dim3(256), 0, (hipStream_t)stream, *a);
+  return report("bw_carry_launch (rows)", hipGetLastError());
+}
+
+// Test hook: the forced global-memory (no class limit) session kernel, used by the parity tests to
+// exercise it on inputs that would otherwise take the LDS path.
+extern "C" int bw_sess_run_global_launch(const GpuSessRunArgs* a) {
+  if (!a) return (int)hipErrorInvalidValue;
+  const int t = clamp_threads(a->threads);
+  Replay r = make_replay(a, a->nodes, a->bv0, a->bv1, t);
+  r.gfen = a->fen;
+  r.gleaf = a->leaf;
+  if (bad_replay(r) || !r.gfen || !r.gleaf) return (int)hipErrorInvalidValue;
+  int rc = set_limits();
+  if (rc) return rc;
+  hipLaunchKernelGGL(ref_block_global_kernel, dim3(r.B), dim3(t), lds_global_bytes(r.G), (hipStream_t)0, r);
+  return report("bw_sess_run_global_launch", hipGetLastError());
+}

@@ -53,6 +53,7 @@ static int64_t checked_i64(__int128 x) {
   return (int64_t)x;
 }
 
+static double g_solve_s = -1.0;
 static void write_answers(const char* path, const int64_t* ans, size_t n, double t0) {
   if (path) {
     FILE* fp = fopen(path, "wb");
@@ -62,7 +63,8 @@ static void write_answers(const char* path, const int64_t* ans, size_t n, double
   }
   char hex[65];
   bw_sha256_hex(ans, n * sizeof(int64_t), hex);
-  printf("{\"n_answers\":%zu,\"answers_sha256\":\"%s\",\"wall_s\":%.6f}\n", n, hex, now_s() - t0);
+  printf("{\"n_answers\":%zu,\"answers_sha256\":\"%s\",\"wall_s\":%.6f,\"solve_s\":%.6f}\n", n, hex, now_s() - t0,
+         g_solve_s);
 }
 
 /* ------------------------------------------------------------------ reference semantics */
@@ -387,6 +389,7 @@ static int mode_tree(int argc, char** argv) {
   /* key -> Fenwick coordinate: the key itself, or its rank among the distinct keys (cfg 3) */
   const uint64_t NK = (cfg == 3) ? w.D : w.N;
   int64_t* ans = (int64_t*)calloc(nq, sizeof(int64_t));
+  const double ts0 = now_s();
   if (!strcmp(algo, "brute")) {
     /* independent: walk the ancestor chain, compare raw keys against raw endpoints */
     __int128* acc = (__int128*)malloc(sizeof(__int128) * (size_t)Q);
@@ -456,6 +459,7 @@ static int mode_tree(int argc, char** argv) {
     }
     free(fen); free(koff); free(kid); free(stk_v); free(stk_c); free(krank); free(lrank); free(hrank);
   }
+  g_solve_s = now_s() - ts0;
   write_answers(out, ans, nq, t0);
   free(ans);
   twl_free(&w);

@@ -1,0 +1,48 @@
+"""GPU parity of the generalised tree path: bw_tree (HIP kernels behind bwt_* C-ABI) against the
+oracle's golden hashes, which independent solvers (brute ancestor walk, the reference's own
+Solver) pinned.  Integer work: the bar is bit-exact (SHA-256 equality of the answer vector)."""
+import pytest
+
+from conftest import golden, run_json
+
+TREE = golden("tree_cases.json")
+
+# (case, knobs): small blocks / chunks / segments stress every block, chunk and segment boundary
+CASES = [
+    ("t1_cfg1_small", []),
+    ("t1_cfg1_small", ["--rmax", "64", "--tile-shift", "6", "--chunk", "256", "--seg", "2", "--threads", "64",
+                       "--waves", "1"]),
+    ("t2_cfg2_small", []),
+    ("t2_cfg2_small", ["--rmax", "32", "--chunk", "128", "--seg", "3", "--threads", "128", "--waves", "2"]),
+    ("t3_cfg3_small", []),
+    ("t3_cfg3_small", ["--rmax", "48", "--tile-shift", "5", "--chunk", "200", "--seg", "1", "--waves", "3"]),
+    ("t4_cfg4_small", ["--rmax", "16", "--tile-shift", "9", "--chunk", "64"]),
+    ("t5_cfg2_dense", []),
+    ("t5_cfg2_dense", ["--rmax", "17", "--tile-shift", "3", "--chunk", "100", "--seg", "4", "--threads", "1024"]),
+    ("t6_cfg1_single", []),
+    ("t7_cfg2_deep", ["--rmax", "512", "--chunk", "4096"]),
+    ("c1_cfg1", []),
+]
+
+
+def _cmd(pkg, g, knobs):
+    return [pkg.BW_TREE, "--cfg", str(g["cfg"]), "--S", str(g["S"]), "--U", str(g["U"]), "--Q", str(g["Q"]),
+            "--N", str(g["N"]), "--seed", str(g["seed"]), "--warmup", "0", "--repeat", "2"] + knobs
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case,knobs", CASES)
+def test_tree_gpu_matches_golden(pkg, case, knobs):
+    g = TREE[case]
+    r = run_json(_cmd(pkg, g, knobs), timeout=300)
+    assert r["answers_sha256"] == g["oracle"], (case, knobs)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["c2_cfg2", "c4_cfg4", "c3_cfg3"])
+def test_tree_gpu_full_size(pkg, case):
+    """The benchmark configs at full size (300,000 states): same complete-answer SHA as the oracle."""
+    g = TREE[case]
+    r = run_json(_cmd(pkg, g, []), timeout=600)
+    assert r["answers_sha256"] == g["oracle"], case
+    assert r["n_answers"] == g["S"] * g["Q"]
