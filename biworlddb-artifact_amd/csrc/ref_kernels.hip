@@ -265,21 +265,31 @@ __global__ void __launch_bounds__(1024) ref_block_global_kernel(const Replay a) 
       if (row >= 0 && off >= 0) {
         const int64_t ds = p.a - p.b;
         const int kb = a.ms_off[row], ke = a.ms_off[row + 1];
+        // a support much smaller than the class domain goes in as Fenwick point updates; a large
+        // one is swept into the leaf and the tree is rebuilt (the reference always rebuilds)
+        const int lg = 32 - __builtin_clz((unsigned)Cpfx);
+        const bool point = (int64_t)(ke - kb) * lg * 2 < (int64_t)Cpfx;
         for (int k = kb + tid; k < ke; k += nt) {
           const int di = a.ms_dev[k];
           const int64_t dev = (di >= 0) ? ld_agent(&a.devbuf[off + di]) : 0;
           const int64_t d = ds * (a.ms_base[k] + dev);
-          if (d) add_agent(&leaf[(int)a.ms_cls[k] + 1], d);
+          if (!d) continue;
+          const int c1 = (int)a.ms_cls[k] + 1;
+          add_agent(&leaf[c1], d);
+          if (point)
+            for (int i = c1; i <= ncls; i += i & -i) add_agent(&fen[i], d);
         }
         __syncthreads();
-        // rebuild the Fenwick tree from the leaf: copy, then one level per lowest-set-bit s
-        for (int i = tid; i < Cpfx; i += nt)
-          __hip_atomic_store(&fen[i], ld_agent(&leaf[i]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __syncthreads();
-        for (int s = 1; s < Cpfx; s <<= 1) {
-          for (int64_t i = (int64_t)s * (2 * tid + 1); i + s < Cpfx; i += 2 * (int64_t)s * nt)
-            add_agent(&fen[i + s], ld_agent(&fen[i]));
+        if (!point) {
+          // rebuild the Fenwick tree from the leaf: copy, then one level per lowest-set-bit s
+          for (int i = tid; i < Cpfx; i += nt)
+            __hip_atomic_store(&fen[i], ld_agent(&leaf[i]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           __syncthreads();
+          for (int s = 1; s < Cpfx; s <<= 1) {
+            for (int64_t i = (int64_t)s * (2 * tid + 1); i + s < Cpfx; i += 2 * (int64_t)s * nt)
+              add_agent(&fen[i + s], ld_agent(&fen[i]));
+            __syncthreads();
+          }
         }
       }
       if (tid == 0) fac[g] = p.a;
