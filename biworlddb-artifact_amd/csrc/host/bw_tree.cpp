@@ -1,11 +1,14 @@
 // bw_tree.cpp -- timed worker for the generalised GPU batch path (one process per run), with the
 // phase/JSON contract of the reference GPU workers (src/panel_batch_gpu.cpp:41-67,258-293):
-// load/generate, setup (host plan), upload, execute (device passes), extract (D2H + SHA-256).
+// generate (load), setup (host plans), upload, execute (device passes), extract (D2H + SHA-256).
 //
-//   bw_tree --cfg 2 [--S n --U n --Q n --N n --seed n] [--rmax 2048 --tile-shift 11
-//           --chunk 16384 --seg 32 --threads 256 --waves 4] [--warmup 1 --repeat 5]
-//           [--answers FILE] [--json FILE]
-//
+//   bw_tree --cfg 2 [--S n --U n --Q n --N n --seed n] [--mode pass|query]
+//           [--rmax 2048 --tile-shift 11 --chunk 16384 --seg 32 --threads 256 --waves 4]
+//           [--warmup 1 --repeat 5] [--answers FILE] [--json FILE]
+// --mode pass  : the plain form -- every timed pass rebuilds the index (block replay with weights,
+//                chunk histograms, carry scan) and answers the declared queries.
+// --mode query : the index is built once (timed separately), every timed pass answers the
+//                declared queries over the resident index.
 // Exit codes follow the reference: 2 usage, 3 unsupported input, 4 I/O, 5 GPU error.
 #include <hip/hip_runtime_api.h>
 #include <openssl/sha.h>
@@ -26,33 +29,42 @@
 using namespace bwg;
 using clk = std::chrono::steady_clock;
 
-#define HIPX(x)                                                                                 \
-  do {                                                                                          \
-    hipError_t e_ = (x);                                                                        \
-    if (e_ != hipSuccess) {                                                                     \
+#define HIPX(x)                                                                                   \
+  do {                                                                                            \
+    hipError_t e_ = (x);                                                                          \
+    if (e_ != hipSuccess) {                                                                       \
       std::fprintf(stderr, "HIP error %s at %s:%d\n", hipGetErrorString(e_), __FILE__, __LINE__); \
-      std::exit(5);                                                                             \
-    }                                                                                           \
+      std::exit(5);                                                                               \
+    }                                                                                             \
+  } while (0)
+#define RUN(x)                                                                       \
+  do {                                                                               \
+    const int rc_ = (x);                                                             \
+    if (rc_ != 0) {                                                                  \
+      std::fprintf(stderr, "launch failed (%d) at %s:%d\n", rc_, __FILE__, __LINE__); \
+      std::exit(5);                                                                  \
+    }                                                                                \
   } while (0)
 
 static double ms_between(clk::time_point a, clk::time_point b) {
   return std::chrono::duration<double, std::milli>(b - a).count();
 }
 
+static size_t g_dev_bytes = 0;
 template <typename T>
-static T* upload(const std::vector<T>& v, size_t* total) {
+static T* upload(const std::vector<T>& v) {
   T* d = nullptr;
   const size_t n = std::max<size_t>(v.size(), 1) * sizeof(T);
   HIPX(hipMalloc(&d, n));
   if (!v.empty()) HIPX(hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
-  *total += n;
+  g_dev_bytes += n;
   return d;
 }
 template <typename T>
-static T* scratch(size_t n, size_t* total) {
+static T* scratch(size_t n) {
   T* d = nullptr;
   HIPX(hipMalloc(&d, std::max<size_t>(n, 1) * sizeof(T)));
-  *total += std::max<size_t>(n, 1) * sizeof(T);
+  g_dev_bytes += std::max<size_t>(n, 1) * sizeof(T);
   return d;
 }
 
@@ -79,10 +91,12 @@ int main(int argc, char** argv) {
   TreeSpec spec = tree_defaults(cfg);
   PlanKnobs kn;
   int threads = 256, waves = 4, warmup = 1, repeat = 5;
-  std::string answers_path, json_path;
-  bool wsha = false;
+  std::string answers_path, json_path, mode = "pass";
+  bool wsha = false, verify_remap = false;
+  int64_t copy_probe = 0;  // bytes of a device-to-device copy run first (HBM counter scale)
   for (int i = 1; i < argc; ++i) {
     if (!std::strcmp(argv[i], "--workload-sha")) { wsha = true; continue; }
+    if (!std::strcmp(argv[i], "--verify-remap")) { verify_remap = true; continue; }
     if (i + 1 >= argc) { std::fprintf(stderr, "bw_tree: missing value for %s\n", argv[i]); return 2; }
     const char* k = argv[i];
     const char* v = argv[++i];
@@ -92,9 +106,12 @@ int main(int argc, char** argv) {
     else if (!std::strcmp(k, "--Q")) spec.Q = std::atoi(v);
     else if (!std::strcmp(k, "--N")) spec.N = std::strtoull(v, nullptr, 10);
     else if (!std::strcmp(k, "--seed")) spec.seed = std::strtoull(v, nullptr, 10);
+    else if (!std::strcmp(k, "--mode")) mode = v;
     else if (!std::strcmp(k, "--rmax")) kn.rmax = std::atoi(v);
     else if (!std::strcmp(k, "--tile-shift")) kn.tile_shift = std::atoi(v);
     else if (!std::strcmp(k, "--chunk")) kn.chunk_items = std::atoll(v);
+    else if (!std::strcmp(k, "--chunk-c")) kn.chunk_items_c = std::atoll(v);
+    else if (!std::strcmp(k, "--ep-per-work")) kn.ep_per_work = std::atoll(v);
     else if (!std::strcmp(k, "--seg")) kn.seg_chunks = std::atoi(v);
     else if (!std::strcmp(k, "--max-block-steps")) kn.max_block_steps = std::atoi(v);
     else if (!std::strcmp(k, "--threads")) threads = std::atoi(v);
@@ -103,10 +120,12 @@ int main(int argc, char** argv) {
     else if (!std::strcmp(k, "--repeat")) repeat = std::atoi(v);
     else if (!std::strcmp(k, "--answers")) answers_path = v;
     else if (!std::strcmp(k, "--json")) json_path = v;
+    else if (!std::strcmp(k, "--copy-probe")) copy_probe = std::atoll(v);
     else { std::fprintf(stderr, "bw_tree: unknown argument %s\n", k); return 2; }
   }
-  if (spec.S < 1 || spec.U < 1 || spec.Q < 1 || spec.N < 1 || repeat < 1 || warmup < 0) {
-    std::fprintf(stderr, "bw_tree: bad sizes\n");
+  if (spec.S < 1 || spec.U < 1 || spec.Q < 1 || spec.N < 1 || repeat < 1 || warmup < 0 ||
+      (mode != "pass" && mode != "query")) {
+    std::fprintf(stderr, "bw_tree: bad sizes or mode\n");
     return 2;
   }
 
@@ -122,8 +141,9 @@ int main(int argc, char** argv) {
   }
   const auto t1 = clk::now();
 
-  // ---- setup: coordinates (ranks for the sparse key space) and the host plan
-  const size_t nu = size_t(spec.S) * size_t(spec.U), nq = size_t(spec.S) * size_t(spec.Q);
+  // ---- setup: coordinates (ranks for the sparse key space), index plan, declared batch plan
+  const int64_t S = spec.S;
+  const size_t nu = size_t(S) * size_t(spec.U), nq = size_t(S) * size_t(spec.Q);
   RankLut lut;
   std::vector<uint32_t> kc, lc, hc;
   const uint32_t *kcoord = w.key.data(), *lcoord = w.qlo.data(), *hcoord = w.qhi.data();
@@ -135,114 +155,163 @@ int main(int argc, char** argv) {
     host_ranks(lut, w.qhi.data(), hc.data(), nq);
     kcoord = kc.data(); lcoord = lc.data(); hcoord = hc.data();
   }
-  TreePlan P;
+  TreeIndexPlan P;
   char err[256];
-  if (!build_tree_plan(spec.S, spec.U, spec.Q, w.coord_n(), w.parent.data(), kcoord, lcoord, hcoord, kn, P, err,
-                       sizeof(err))) {
-    std::fprintf(stderr, "bw_tree: plan: %s\n", err);
+  if (!build_tree_index(S, spec.U, w.coord_n(), w.parent.data(), kcoord, kn, P, err, sizeof(err))) {
+    std::fprintf(stderr, "bw_tree: index plan: %s\n", err);
+    return 3;
+  }
+  std::vector<int32_t> qstate(nq);
+  for (size_t q = 0; q < nq; ++q) qstate[q] = int32_t(q / size_t(spec.Q) + 1);
+  BatchPlan BP;
+  if (!build_batch_plan(P, int64_t(nq), qstate.data(), lcoord, hcoord, mode == "pass", BP, err, sizeof(err))) {
+    std::fprintf(stderr, "bw_tree: batch plan: %s\n", err);
     return 3;
   }
   const auto t2 = clk::now();
 
-  // ---- upload (inputs + plan), allocate scratch
+  // ---- upload (inputs + plans), allocate scratch
   HIPX(hipSetDevice(0));
   HIPX(hipFree(nullptr));
   const auto tu0 = clk::now();
-  size_t dev_bytes = 0;
-  BwtPassArgs a{};
-  a.S = spec.S; a.U = spec.U; a.Q = spec.Q;
-  a.nblk = P.nblk();
-  a.block_threads = threads;
-  a.rmax = kn.rmax;
-  a.max_cls = P.max_cls;
-  a.nchunk = P.nchunk();
-  a.nchain = P.nchain();
-  a.chunk_waves = waves;
-  uint32_t *d_key, *d_qlo, *d_qhi;
-  uint32_t *d_key_raw = nullptr, *d_qlo_raw = nullptr, *d_qhi_raw = nullptr, *d_row = nullptr;
+  BwtIndex X{};
+  X.S = S; X.U = spec.U; X.nblk = P.nblk(); X.rmax = kn.rmax; X.max_cls = P.max_cls; X.nchain = P.nchain();
+  X.nchunk = P.nchunk(); X.nseg = P.nseg();
+  X.ncls_c = P.chain_ncls[0];
+  X.ncls_b = P.chain_ncls.size() > 1 ? P.chain_ncls[1] : 1;
+  X.nchunk_c = P.chain_chunk0[1];
+  X.delta = upload(w.delta);
+  X.steps = upload(P.steps);
+  X.blk_step0 = upload(P.blk_step0);
+  X.evrank = upload(P.evrank);
+  X.blk_key0 = upload(P.blk_key0);
+  X.bkey = upload(P.bkey);
+  X.blk_tile0 = upload(P.blk_tile0);
+  X.tile_rank0 = upload(P.tile_rank0);
+  X.kitems = upload(P.kitems);
+  X.chunk_key0 = upload(P.chunk_key0);
+  X.chunk_chain = upload(P.chunk_chain);
+  X.chunk_cp0 = upload(P.chunk_cp0);
+  X.chain_ncls = upload(P.chain_ncls);
+  X.chain_src = upload(P.chain_src);
+  X.chain_out = upload(P.chain_out);
+  X.chain_chunk0 = upload(P.chain_chunk0);
+  X.seg_chunk0 = upload(P.seg_chunk0);
+  X.seg_cp0 = upload(P.seg_cp0);
+  X.chain_seg0 = upload(P.chain_seg0);
+  X.agg = scratch<int64_t>(P.bkey.size());
+  X.tagg = scratch<int64_t>(P.tile_rank0.size());
+  X.cp = scratch<int64_t>(size_t(P.n_cp));
+  X.segsum = scratch<int64_t>(size_t(P.n_segsum));
+  BwtBatch Bt{};
+  Bt.M = int64_t(nq);
+  uint32_t *d_qlo_raw = nullptr, *d_qhi_raw = nullptr, *d_row = nullptr;
   uint16_t* d_low = nullptr;
-  if (cfg == 3) {
-    d_key_raw = upload(w.key, &dev_bytes);
-    d_qlo_raw = upload(w.qlo, &dev_bytes);
-    d_qhi_raw = upload(w.qhi, &dev_bytes);
-    d_row = upload(lut.row_ptr, &dev_bytes);
-    d_low = upload(lut.low16, &dev_bytes);
-    d_key = scratch<uint32_t>(nu, &dev_bytes);
-    d_qlo = scratch<uint32_t>(nq, &dev_bytes);
-    d_qhi = scratch<uint32_t>(nq, &dev_bytes);
+  uint32_t* d_qlo;
+  uint32_t* d_qhi;
+  if (cfg == 3) {  // raw 32-bit endpoints are resolved by rank on the device, inside the pass
+    d_qlo_raw = upload(w.qlo);
+    d_qhi_raw = upload(w.qhi);
+    d_row = upload(lut.row_ptr);
+    d_low = upload(lut.low16);
+    d_qlo = scratch<uint32_t>(nq);
+    d_qhi = scratch<uint32_t>(nq);
   } else {
-    d_key = upload(w.key, &dev_bytes);
-    d_qlo = upload(w.qlo, &dev_bytes);
-    d_qhi = upload(w.qhi, &dev_bytes);
+    d_qlo = upload(w.qlo);
+    d_qhi = upload(w.qhi);
   }
-  a.key = d_key; a.qlo = d_qlo; a.qhi = d_qhi;
-  a.delta = upload(w.delta, &dev_bytes);
-  a.steps = upload(P.steps, &dev_bytes);
-  a.blk_step0 = upload(P.blk_step0, &dev_bytes);
-  a.blk_key0 = upload(P.blk_key0, &dev_bytes);
-  a.bkey = upload(P.bkey, &dev_bytes);
-  a.blk_tile0 = upload(P.blk_tile0, &dev_bytes);
-  a.tile_rank0 = upload(P.tile_rank0, &dev_bytes);
-  a.items = upload(P.items, &dev_bytes);
-  a.chunk_item0 = upload(P.chunk_item0, &dev_bytes);
-  a.chunk_chain = upload(P.chunk_chain, &dev_bytes);
-  a.chunk_cp0 = upload(P.chunk_cp0, &dev_bytes);
-  a.chain_ncls = upload(P.chain_ncls, &dev_bytes);
-  a.chain_src = upload(P.chain_src, &dev_bytes);
-  a.chain_out = upload(P.chain_out, &dev_bytes);
-  a.chain_chunk0 = upload(P.chain_chunk0, &dev_bytes);
-  a.nseg = P.nseg();
-  a.seg_chunk0 = upload(P.seg_chunk0, &dev_bytes);
-  a.seg_cp0 = upload(P.seg_cp0, &dev_bytes);
-  a.chain_seg0 = upload(P.chain_seg0, &dev_bytes);
-  a.segsum = scratch<int64_t>(size_t(P.n_segsum), &dev_bytes);
-  a.agg = scratch<int64_t>(P.bkey.size(), &dev_bytes);
-  a.tagg = scratch<int64_t>(P.tile_rank0.size(), &dev_bytes);
-  a.cp = scratch<int64_t>(size_t(P.n_cp), &dev_bytes);
-  a.g = scratch<int64_t>(4 * nq, &dev_bytes);
-  a.ansr = scratch<int64_t>(nq, &dev_bytes);
-  a.answers = scratch<int64_t>(nq, &dev_bytes);
+  Bt.qlo = d_qlo; Bt.qhi = d_qhi;
+  Bt.nqblk = int32_t(BP.qblk.size());
+  Bt.qblk = upload(BP.qblk);
+  Bt.qblk_q0 = upload(BP.qblk_q0);
+  Bt.qorder = upload(BP.qorder);
+  Bt.qstep = upload(BP.qstep);
+  Bt.qrank = upload(BP.qrank);
+  Bt.nwork = int64_t(BP.wk_chunk.size());
+  Bt.nwork_c = int64_t(std::lower_bound(BP.wk_chunk.begin(), BP.wk_chunk.end(), P.chain_chunk0[1]) - BP.wk_chunk.begin());
+  Bt.wk_chunk = upload(BP.wk_chunk);
+  Bt.wk_e0 = upload(BP.wk_e0);
+  Bt.eitems = upload(BP.eitems);
+  Bt.ansr = scratch<int64_t>(nq);
+  Bt.g = scratch<int64_t>(4 * nq);
+  Bt.answers = scratch<int64_t>(nq);
   HIPX(hipDeviceSynchronize());
   const auto tu1 = clk::now();
 
   hipStream_t stream;
   HIPX(hipStreamCreate(&stream));
-  BwtRankArgs rk[3];
-  if (cfg == 3) {
-    rk[0] = BwtRankArgs{int64_t(nu), d_key_raw, d_key, d_row, d_low};
-    rk[1] = BwtRankArgs{int64_t(nq), d_qlo_raw, d_qlo, d_row, d_low};
-    rk[2] = BwtRankArgs{int64_t(nq), d_qhi_raw, d_qhi, d_row, d_low};
+  if (copy_probe > 0) {
+    void *ca = nullptr, *cb = nullptr;
+    HIPX(hipMalloc(&ca, size_t(copy_probe)));
+    HIPX(hipMalloc(&cb, size_t(copy_probe)));
+    HIPX(hipMemsetAsync(ca, 1, size_t(copy_probe), stream));
+    HIPX(hipMemcpyAsync(cb, ca, size_t(copy_probe), hipMemcpyDeviceToDevice, stream));
+    HIPX(hipStreamSynchronize(stream));
+    HIPX(hipFree(ca));
+    HIPX(hipFree(cb));
   }
+  BwtLaunch L{threads, waves, 0, 0};
+  BwtRankArgs rk[2];
+  if (cfg == 3) {
+    rk[0] = BwtRankArgs{int64_t(nq), d_qlo_raw, d_qlo, d_row, d_low};
+    rk[1] = BwtRankArgs{int64_t(nq), d_qhi_raw, d_qhi, d_row, d_low};
+  }
+  // index build (query mode): block replay with weights over every block, no queries
+  std::vector<int32_t> allb(static_cast<size_t>(P.nblk()));
+  for (int32_t b = 0; b < P.nblk(); ++b) allb[size_t(b)] = b;
+  std::vector<int64_t> zq0(size_t(P.nblk()) + 1, 0);
+  BwtBatch Bidx = Bt;
+  Bidx.nqblk = P.nblk();
+  Bidx.qblk = upload(allb);
+  Bidx.qblk_q0 = upload(zq0);
+  Bidx.nwork = 0;
+  Bidx.nwork_c = 0;
+  double index_ms = 0;
   // stage events: remap | block | hist | scan | replay | combine
   const int NST = 6;
   hipEvent_t ev[NST + 1];
   for (auto& e : ev) HIPX(hipEventCreate(&e));
+  auto build_index = [&]() {
+    BwtLaunch wl = L;
+    wl.write_index = 1;
+    RUN(bwt_block_replay_launch(&X, &Bidx, &wl, stream));
+    RUN(bwt_chunk_hist_launch(&X, &wl, stream));
+    RUN(bwt_carry_scan_launch(&X, &wl, stream));
+  };
+  if (mode == "query") {
+    build_index();  // warm
+    HIPX(hipEventRecord(ev[0], stream));
+    build_index();
+    HIPX(hipEventRecord(ev[1], stream));
+    HIPX(hipEventSynchronize(ev[1]));
+    float t;
+    HIPX(hipEventElapsedTime(&t, ev[0], ev[1]));
+    index_ms = t;
+  }
   auto run_pass = [&](bool timed) {
-    int rc = 0;
+    BwtLaunch wl = L;
+    wl.write_index = mode == "pass";
     if (timed) HIPX(hipEventRecord(ev[0], stream));
     if (cfg == 3)
-      for (auto& r : rk)
-        if ((rc = bwt_rank_remap_launch(&r, stream))) return rc;
+      for (auto& r : rk) RUN(bwt_rank_remap_launch(&r, stream));
     if (timed) HIPX(hipEventRecord(ev[1], stream));
-    if ((rc = bwt_block_replay_launch(&a, stream))) return rc;
+    RUN(bwt_block_replay_launch(&X, &Bt, &wl, stream));
     if (timed) HIPX(hipEventRecord(ev[2], stream));
-    if ((rc = bwt_chunk_hist_launch(&a, stream))) return rc;
+    if (mode == "pass") RUN(bwt_chunk_hist_launch(&X, &wl, stream));
     if (timed) HIPX(hipEventRecord(ev[3], stream));
-    if ((rc = bwt_carry_scan_launch(&a, stream))) return rc;
+    if (mode == "pass") RUN(bwt_carry_scan_launch(&X, &wl, stream));
     if (timed) HIPX(hipEventRecord(ev[4], stream));
-    if ((rc = bwt_chunk_replay_launch(&a, stream))) return rc;
+    RUN(bwt_chunk_replay_launch(&X, &Bt, &wl, stream));
     if (timed) HIPX(hipEventRecord(ev[5], stream));
-    if ((rc = bwt_combine_launch(&a, stream))) return rc;
+    RUN(bwt_combine_launch(&X, &Bt, &wl, stream));
     if (timed) HIPX(hipEventRecord(ev[6], stream));
-    return 0;
   };
-  for (int i = 0; i < warmup; ++i)
-    if (run_pass(false)) return 5;
+  for (int i = 0; i < warmup; ++i) run_pass(false);
   HIPX(hipStreamSynchronize(stream));
   std::vector<double> pass_ms;
   double stage_ms[NST] = {0, 0, 0, 0, 0, 0};
   for (int i = 0; i < repeat; ++i) {
-    if (run_pass(true)) return 5;
+    run_pass(true);
     HIPX(hipEventSynchronize(ev[NST]));
     float t = 0;
     HIPX(hipEventElapsedTime(&t, ev[0], ev[NST]));
@@ -252,9 +321,17 @@ int main(int argc, char** argv) {
       stage_ms[s] += t / repeat;
     }
   }
+  int64_t remap_mismatch = -1;
+  if (cfg == 3 && verify_remap) {  // device CSR rank-LUT output against the host ranks
+    std::vector<uint32_t> dl(nq), dh(nq);
+    HIPX(hipMemcpy(dl.data(), d_qlo, nq * 4, hipMemcpyDeviceToHost));
+    HIPX(hipMemcpy(dh.data(), d_qhi, nq * 4, hipMemcpyDeviceToHost));
+    remap_mismatch = 0;
+    for (size_t i = 0; i < nq; ++i) remap_mismatch += (dl[i] != lc[i]) + (dh[i] != hc[i]);
+  }
   const auto tx0 = clk::now();
   std::vector<int64_t> ans(nq);
-  HIPX(hipMemcpy(ans.data(), a.answers, nq * sizeof(int64_t), hipMemcpyDeviceToHost));
+  HIPX(hipMemcpy(ans.data(), Bt.answers, nq * sizeof(int64_t), hipMemcpyDeviceToHost));
   const auto tx1 = clk::now();
   if (!answers_path.empty()) {
     FILE* f = std::fopen(answers_path.c_str(), "wb");
@@ -272,44 +349,60 @@ int main(int argc, char** argv) {
   avg /= double(pass_ms.size());
   hipDeviceProp_t prop{};
   HIPX(hipGetDeviceProperties(&prop, 0));
+  const char* dev = prop.name[0] ? prop.name : prop.gcnArchName;
   // algorithmic bytes of one pass: every update (u32 key + i64 delta) and every query
   // (2 x u32 endpoint + i64 answer) once
   const double bytes_alg = double(nu) * 12.0 + double(nq) * 16.0;
-  // algorithmic bytes of each stage: what the stage must read and write once
-  int64_t n_enter = 0, n_keyitems = 0;
-  for (int32_t st : P.steps) n_enter += st > 0;
-  for (uint64_t x : P.items) n_keyitems += !(x & BWT_ITEM_EP);
-  const double nsteps_d = double(P.steps.size()), nkeys_d = double(P.bkey.size()), ntile_d = double(P.tile_rank0.size());
-  const double nitems_d = double(P.items.size()), nep_d = double(2 * nq);
-  const double sb_remap = cfg == 3 ? double(nu + 2 * nq) * 8.0 : 0.0;
-  const double sb_block = nsteps_d * spec.U * 12.0 + nsteps_d * 4.0 + double(n_enter) * spec.Q * 16.0 +
-                          nkeys_d * 12.0 + ntile_d * 10.0;
-  const double sb_hist = nitems_d * 8.0 + double(n_keyitems) * 8.0 + double(P.n_cp) * 8.0;
-  const double sb_scan = double(P.n_cp) * 16.0;
-  const double sb_replay = nitems_d * 8.0 + double(n_keyitems) * 8.0 + nep_d * 2.0 * 8.0;
+  // algorithmic bytes of each stage (what the stage must read and write once):
+  //  block replay: per walked event u16 rank + i64 delta; per step its id; per query 2 u32
+  //    endpoints + i64 result + u32 order + i32 step; with the index write the block's keys
+  //    (u32 in, i64 weight out) and tiles (u16 in, i64 out)
+  //  chunk histogram: every key item (u64) + its i64 weight, the dense i64 rows out
+  //  carry scan: the rows in and out; chunk replay: walked key items + weights, every endpoint
+  //    item (u64) + its carry (i64) + the result (i64); combine: 5 x i64 in, 1 out per query
+  int64_t steps_run = 0, kit_run = 0;
+  if (mode == "pass") steps_run = int64_t(P.steps.size());
+  else
+    for (size_t i = 0; i < BP.qblk.size(); ++i)
+      if (BP.qblk_q0[i + 1] > BP.qblk_q0[i])
+        steps_run += BP.qstep[size_t(BP.qblk_q0[i + 1] - 1)] + 1 - P.blk_step0[size_t(BP.qblk[i])];
+  for (size_t i = 0; i < BP.wk_chunk.size(); ++i) {
+    const int64_t c = BP.wk_chunk[i];
+    kit_run += std::min<int64_t>(P.chunk_key0[size_t(c) + 1] - P.chunk_key0[size_t(c)],
+                                 int64_t(BP.eitems[size_t(BP.wk_e0[i + 1] - 1)] >> BWT_EP_POS_SHIFT));
+  }
+  const double nkeys_d = double(P.bkey.size()), ntile_d = double(P.tile_rank0.size());
+  const double sb_remap = cfg == 3 ? double(nq) * 16.0 : 0.0;
+  const double sb_block = double(steps_run) * spec.U * 10.0 + double(steps_run) * 4.0 + double(nq) * 20.0 +
+                          (mode == "pass" ? nkeys_d * 12.0 + ntile_d * 10.0 : 0.0);
+  const double sb_hist = mode == "pass" ? double(P.kitems.size()) * 16.0 + double(P.n_cp) * 8.0 : 0.0;
+  const double sb_scan = mode == "pass" ? double(P.n_cp) * 16.0 : 0.0;
+  const double sb_replay = double(kit_run) * 16.0 + double(BP.eitems.size()) * 24.0;
   const double sb_combine = double(nq) * 48.0;
   char buf[8192];
   int n = std::snprintf(
       buf, sizeof(buf),
-      "{\"arm\":\"bwgpu_tree\",\"cfg\":%d,\"S\":%" PRId64 ",\"U\":%d,\"Q\":%d,\"N\":%" PRIu64 ",\"NK\":%" PRIu64
-      ",\"seed\":%" PRIu64 ",\"n_answers\":%zu,\"answers_sha256\":\"%s\",\"answers_fn\":\"%016" PRIx64 "\","
-      "\"plan\":{\"nsteps\":%zu,\"nblk\":%d,\"nkeys\":%zu,\"ntiles\":%zu,\"nitems\":%zu,\"nchunk\":%" PRId64
-      ",\"nchain\":%d,\"nseg\":%" PRId64 ",\"max_cls\":%d,\"rmax\":%d,\"tile_shift\":%d,\"chunk_items\":%" PRId64
-      ",\"seg_chunks\":%d,\"threads\":%d,\"waves\":%d},"
-      "\"phases_ms\":{\"generate\":%.3f,\"setup\":%.3f,\"upload\":%.3f,\"download\":%.3f,\"extract\":%.3f},"
+      "{\"arm\":\"bwgpu_tree\",\"mode\":\"%s\",\"cfg\":%d,\"S\":%" PRId64 ",\"U\":%d,\"Q\":%d,\"N\":%" PRIu64
+      ",\"NK\":%" PRIu64 ",\"seed\":%" PRIu64 ",\"n_answers\":%zu,\"answers_sha256\":\"%s\",\"answers_fn\":\"%016" PRIx64
+      "\",\"plan\":{\"nsteps\":%zu,\"nblk\":%d,\"nkeys\":%zu,\"ntiles\":%zu,\"nkitems\":%zu,\"neitems\":%zu,"
+      "\"nchunk\":%" PRId64 ",\"nwork\":%zu,\"nchain\":%d,\"nseg\":%" PRId64 ",\"max_cls\":%d,\"rmax\":%d,"
+      "\"tile_shift\":%d,\"chunk_items\":%" PRId64 ",\"chunk_items_c\":%" PRId64 ",\"ep_per_work\":%" PRId64 ",\"seg_chunks\":%d,\"threads\":%d,\"waves\":%d},"
+      "\"phases_ms\":{\"generate\":%.3f,\"setup\":%.3f,\"upload\":%.3f,\"download\":%.3f,\"extract\":%.3f,"
+      "\"index_build\":%.4f},"
       "\"pass_ms\":{\"avg\":%.4f,\"min\":%.4f,\"max\":%.4f,\"repeat\":%d,\"warmup\":%d},"
       "\"stage_ms\":{\"rank_remap\":%.4f,\"block_replay\":%.4f,\"chunk_hist\":%.4f,\"carry_scan\":%.4f,"
       "\"chunk_replay\":%.4f,\"combine\":%.4f},"
       "\"stage_bytes\":{\"rank_remap\":%.0f,\"block_replay\":%.0f,\"chunk_hist\":%.0f,\"carry_scan\":%.0f,"
       "\"chunk_replay\":%.0f,\"combine\":%.0f},"
-      "\"queries_per_s\":%.1f,\"bytes_alg\":%.0f,\"device_bytes\":%zu,\"device\":\"%s\",\"cus\":%d}\n",
-      cfg, spec.S, spec.U, spec.Q, spec.N, w.coord_n(), spec.seed, nq, sha.c_str(), fn, P.steps.size(), P.nblk(),
-      P.bkey.size(), P.tile_rank0.size(), P.items.size(), P.nchunk(), P.nchain(), P.nseg(), P.max_cls, kn.rmax,
-      kn.tile_shift, kn.chunk_items, kn.seg_chunks, threads, waves, ms_between(t0, t1), ms_between(t1, t2),
-      ms_between(tu0, tu1), ms_between(tx0, tx1), ms_between(tx1, tx2), avg, pass_ms.front(), pass_ms.back(), repeat,
-      warmup, stage_ms[0], stage_ms[1], stage_ms[2], stage_ms[3], stage_ms[4], stage_ms[5], sb_remap, sb_block,
-      sb_hist, sb_scan, sb_replay, sb_combine, double(nq) / (avg * 1e-3),
-      bytes_alg, dev_bytes, prop.name, prop.multiProcessorCount);
+      "\"remap_mismatch\":%" PRId64 ",\"queries_per_s\":%.1f,\"bytes_alg\":%.0f,\"device_bytes\":%zu,\"device\":\"%s\",\"cus\":%d}\n",
+      mode.c_str(), cfg, S, spec.U, spec.Q, spec.N, w.coord_n(), spec.seed, nq, sha.c_str(), fn, P.steps.size(),
+      P.nblk(), P.bkey.size(), P.tile_rank0.size(), P.kitems.size(), BP.eitems.size(), P.nchunk(), BP.wk_chunk.size(),
+      P.nchain(), P.nseg(), P.max_cls, kn.rmax, P.knobs.tile_shift, P.knobs.chunk_items, P.knobs.chunk_items_c, kn.ep_per_work,
+      kn.seg_chunks, threads, waves,
+      ms_between(t0, t1), ms_between(t1, t2), ms_between(tu0, tu1), ms_between(tx0, tx1), ms_between(tx1, tx2),
+      index_ms, avg, pass_ms.front(), pass_ms.back(), repeat, warmup, stage_ms[0], stage_ms[1], stage_ms[2],
+      stage_ms[3], stage_ms[4], stage_ms[5], sb_remap, sb_block, sb_hist, sb_scan, sb_replay, sb_combine,
+      remap_mismatch, double(nq) / (avg * 1e-3), bytes_alg, g_dev_bytes, dev, prop.multiProcessorCount);
   std::fwrite(buf, 1, size_t(n), stdout);
   if (!json_path.empty()) {
     FILE* f = std::fopen((json_path + ".tmp").c_str(), "w");

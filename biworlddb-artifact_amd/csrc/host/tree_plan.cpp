@@ -1,4 +1,4 @@
-// tree_plan.cpp -- host plan of the generalised batch path (see tree_plan.hpp).
+// tree_plan.cpp -- host plans of the generalised batch path (see tree_plan.hpp).
 #include "tree_plan.hpp"
 
 #include <omp.h>
@@ -11,43 +11,44 @@
 
 namespace bwg {
 
-size_t TreePlan::host_bytes() const {
-  return steps.size() * 4 + blk_step0.size() * 4 + blk_key0.size() * 8 + blk_tile0.size() * 8 + bkey.size() * 4 +
-         tile_rank0.size() * 2 + items.size() * 8 + chunk_item0.size() * 8 + chunk_cp0.size() * 8 +
-         chunk_chain.size() * 4 + seg_chunk0.size() * 8 + seg_cp0.size() * 8;
-}
-
 namespace {
-inline uint64_t make_item(bool ep, uint32_t cls, uint64_t ref) {
-  return (ep ? BWT_ITEM_EP : 0ull) | (uint64_t(cls) << BWT_CLASS_SHIFT) | (ref & BWT_REF_MASK);
+inline uint64_t key_item(uint32_t cls, uint64_t ref) { return (uint64_t(cls) << BWT_CLASS_SHIFT) | (ref & BWT_REF_MASK); }
+inline uint64_t ep_item(uint64_t pos, uint32_t cls, uint64_t ep) {
+  return (pos << BWT_EP_POS_SHIFT) | (uint64_t(cls) << BWT_EP_CLASS_SHIFT) | (ep & 0xFFFFFFFFull);
 }
 }  // namespace
 
-bool build_tree_plan(int64_t S, int U, int Q, uint64_t NK, const int32_t* parent, const uint32_t* key,
-                     const uint32_t* qlo, const uint32_t* qhi, const PlanKnobs& kn, TreePlan& P, char* err,
-                     size_t errlen) {
+bool build_tree_index(int64_t S, int U, uint64_t NK, const int32_t* parent, const uint32_t* key,
+                      const PlanKnobs& kn, TreeIndexPlan& P, char* err, size_t errlen) {
   auto fail = [&](const char* m) {
     if (err && errlen) std::snprintf(err, errlen, "%s", m);
     return false;
   };
-  const int ts = kn.tile_shift;
-  if (S < 1 || U < 1 || Q < 1 || NK < 1 || NK > 0xFFFFFFFFull) return fail("bad sizes");
+  if (S < 1 || U < 1 || NK < 1 || NK > 0xFFFFFFFFull) return fail("bad sizes");
+  int ts = kn.tile_shift;
+  if (ts <= 0) {  // balanced split: the smallest max(level-C classes, level-B classes)
+    uint64_t best = ~0ull;
+    for (int t = 1; t <= 12; ++t) {
+      const uint64_t m = std::max<uint64_t>((NK >> t) + 1, uint64_t(1) << t);
+      if (m < best) { best = m; ts = t; }
+    }
+  }
   if (ts < 1 || ts > 12) return fail("tile_shift must be in [1,12]");
   const uint64_t KT = (NK >> ts) + 1;
-  if (KT > 4096) return fail("key space too large for the tile classes (NK >> tile_shift must be < 4096)");
+  if (KT > 4095) return fail("key space too large for the tile classes (NK >> tile_shift must be < 4095)");
   if (kn.rmax < U || kn.rmax > 8192) return fail("rmax must be in [U, 8192]");
-  if (kn.chunk_items < 64 || kn.seg_chunks < 1) return fail("bad chunking");
-  if (uint64_t(2 * S * Q) > BWT_REF_MASK) return fail("too many endpoints");
+  if ((kn.chunk_items != 0 && (kn.chunk_items < 64 || kn.chunk_items >= (int64_t(1) << 20))) ||
+      (kn.chunk_items_c != 0 && (kn.chunk_items_c < 64 || kn.chunk_items_c >= (int64_t(1) << 20))) ||
+      kn.seg_chunks < 1 || kn.ep_per_work < 64)
+    return fail("bad chunking");
   for (int64_t v = 1; v <= S; ++v)
     if (parent[v] < 0 || parent[v] >= v) return fail("parent[v] must be in [0, v)");
-  const size_t nu = size_t(S) * size_t(U), nq = size_t(S) * size_t(Q);
+  const size_t nu = size_t(S) * size_t(U);
   for (size_t i = 0; i < nu; ++i)
     if (key[i] >= NK) return fail("key coordinate out of range");
-  for (size_t i = 0; i < nq; ++i)
-    if (qlo[i] > NK || qhi[i] > NK) return fail("query coordinate out of range");
 
-  P = TreePlan();
-  P.S = S; P.U = U; P.Q = Q; P.NK = NK; P.knobs = kn;
+  P = TreeIndexPlan();
+  P.S = S; P.U = U; P.NK = NK; P.knobs = kn; P.knobs.tile_shift = ts; P.KT = int(KT);
   const uint32_t W = 1u << ts, mask = W - 1;
 
   // ---- 1. children (ascending) and the Euler walk: +v on entry, -v after the subtree
@@ -75,9 +76,10 @@ bool build_tree_plan(int64_t S, int U, int Q, uint64_t NK, const int32_t* parent
         st.pop_back();
       }
     }
-    while (!P.steps.empty() && P.steps.back() < 0) P.steps.pop_back();  // undo after the last query
+    while (!P.steps.empty() && P.steps.back() < 0) P.steps.pop_back();  // undo after the last state
   }
   const int64_t nsteps = int64_t(P.steps.size());
+  if (nsteps >= (int64_t(1) << 31) / U) return fail("too many steps");
 
   // ---- 2. blocks: greedy, at most rmax distinct keys per block
   {
@@ -112,23 +114,31 @@ bool build_tree_plan(int64_t S, int U, int Q, uint64_t NK, const int32_t* parent
   }
   const int32_t nblk = P.nblk();
 
-  // ---- 3. per-block sorted distinct keys, key tiles, enter-step counts
+  // ---- 3. per block: sorted distinct keys, key tiles, in-block ranks of every event
   std::vector<std::vector<uint32_t>> bk(static_cast<size_t>(nblk));
   std::vector<std::vector<uint16_t>> bt(static_cast<size_t>(nblk));
-  std::vector<int64_t> nenter(static_cast<size_t>(nblk), 0);
   P.state_block.assign(size_t(S) + 1, -1);
+  P.state_step.assign(size_t(S) + 1, -1);
+  P.evrank.resize(size_t(nsteps) * size_t(U));
 #pragma omp parallel for schedule(dynamic, 16)
   for (int32_t b = 0; b < nblk; ++b) {
     auto& ks = bk[size_t(b)];
     for (int32_t s = P.blk_step0[size_t(b)]; s < P.blk_step0[size_t(b) + 1]; ++s) {
       const int32_t st = P.steps[size_t(s)];
       const int64_t v = st > 0 ? st : -st;
-      if (st > 0) { ++nenter[size_t(b)]; P.state_block[size_t(v)] = b; }
+      if (st > 0) { P.state_block[size_t(v)] = b; P.state_step[size_t(v)] = s; }
       const uint32_t* kk = key + size_t(v - 1) * size_t(U);
       ks.insert(ks.end(), kk, kk + U);
     }
     std::sort(ks.begin(), ks.end());
     ks.erase(std::unique(ks.begin(), ks.end()), ks.end());
+    for (int32_t s = P.blk_step0[size_t(b)]; s < P.blk_step0[size_t(b) + 1]; ++s) {
+      const int32_t st = P.steps[size_t(s)];
+      const int64_t v = st > 0 ? st : -st;
+      const uint32_t* kk = key + size_t(v - 1) * size_t(U);
+      uint16_t* er = &P.evrank[size_t(s) * size_t(U)];
+      for (int i = 0; i < U; ++i) er[i] = uint16_t(std::lower_bound(ks.begin(), ks.end(), kk[i]) - ks.begin());
+    }
     auto& tr = bt[size_t(b)];
     for (size_t r = 0; r < ks.size(); ++r)
       if (r == 0 || (ks[r] >> ts) != (ks[r - 1] >> ts)) tr.push_back(uint16_t(r));
@@ -147,30 +157,14 @@ bool build_tree_plan(int64_t S, int U, int Q, uint64_t NK, const int32_t* parent
     std::copy(bk[size_t(b)].begin(), bk[size_t(b)].end(), P.bkey.begin() + P.blk_key0[size_t(b)]);
     std::copy(bt[size_t(b)].begin(), bt[size_t(b)].end(), P.tile_rank0.begin() + P.blk_tile0[size_t(b)]);
   }
+  bk.clear();
+  bk.shrink_to_fit();
   bt.clear();
   bt.shrink_to_fit();
 
-  // per block: its endpoints in step order; endpoint id 2*((v-1)*Q+k)+side
-  auto for_block_eps = [&](int32_t b, auto&& fn) {
-    for (int32_t s = P.blk_step0[size_t(b)]; s < P.blk_step0[size_t(b) + 1]; ++s) {
-      const int32_t st = P.steps[size_t(s)];
-      if (st <= 0) continue;
-      const size_t qb = size_t(st - 1) * size_t(Q);
-      for (int k = 0; k < Q; ++k) {
-        fn(uint64_t(2 * (qb + size_t(k))), qlo[qb + size_t(k)]);
-        fn(uint64_t(2 * (qb + size_t(k)) + 1), qhi[qb + size_t(k)]);
-      }
-    }
-  };
-
-  // ---- 4. level C: one chain over all blocks, class = key tile
-  std::vector<int64_t> cstart(size_t(nblk) + 1, 0);
-  for (int32_t b = 0; b < nblk; ++b)
-    cstart[size_t(b) + 1] = cstart[size_t(b)] + 2 * int64_t(Q) * nenter[size_t(b)] +
-                            (P.blk_tile0[size_t(b) + 1] - P.blk_tile0[size_t(b)]);
-  const int64_t nC = cstart[size_t(nblk)];
-  // ---- 5. level B: one chain per tile, stable counting sort by tile of
-  //          [block 0 endpoints, block 0 keys, block 1 endpoints, block 1 keys, ...]
+  // ---- 4. chains.  Level C (chain 0): every block's tile slots in block order, class = tile.
+  //          Level B (chain 1+t): every block's keys of tile t in block order, class = key & mask.
+  const int64_t nC = P.blk_tile0[size_t(nblk)];
   const int nth = std::max(1, omp_get_max_threads());
   std::vector<int64_t> hist(size_t(nth) * KT, 0);
   std::vector<int32_t> tb0(size_t(nth) + 1);
@@ -179,96 +173,246 @@ bool build_tree_plan(int64_t S, int U, int Q, uint64_t NK, const int32_t* parent
   {
     const int t = omp_get_thread_num();
     int64_t* h = &hist[size_t(t) * KT];
-    for (int32_t b = tb0[size_t(t)]; b < tb0[size_t(t) + 1]; ++b) {
-      for_block_eps(b, [&](uint64_t, uint32_t x) { h[x >> ts]++; });
+    for (int32_t b = tb0[size_t(t)]; b < tb0[size_t(t) + 1]; ++b)
       for (int64_t r = P.blk_key0[size_t(b)]; r < P.blk_key0[size_t(b) + 1]; ++r) h[P.bkey[size_t(r)] >> ts]++;
-    }
   }
   std::vector<int64_t> tile_tot(KT, 0);
   for (uint64_t tile = 0; tile < KT; ++tile)
     for (int t = 0; t < nth; ++t) tile_tot[tile] += hist[size_t(t) * KT + tile];
-  // chain ids: 0 = level C; then every tile with items
+  P.chain_key0.assign(KT + 2, 0);
+  P.chain_key0[1] = nC;
+  for (uint64_t tile = 0; tile < KT; ++tile) P.chain_key0[tile + 2] = P.chain_key0[tile + 1] + tile_tot[tile];
   P.chain_ncls.push_back(int32_t(KT));
   P.chain_src.push_back(1);
   P.chain_out.push_back(0);
-  std::vector<int64_t> tile_base(KT, -1);
-  int64_t nB = 0;
-  std::vector<std::pair<int64_t, int64_t>> chain_rng;  // item ranges
-  chain_rng.push_back({0, nC});
   for (uint64_t tile = 0; tile < KT; ++tile) {
-    if (tile_tot[tile] == 0) continue;
-    tile_base[tile] = nC + nB;
-    chain_rng.push_back({nC + nB, nC + nB + tile_tot[tile]});
-    nB += tile_tot[tile];
     P.chain_ncls.push_back(int32_t(W));
     P.chain_src.push_back(0);
     P.chain_out.push_back(1);
   }
-  // per (thread, tile) write cursors
   std::vector<int64_t> cur(size_t(nth) * KT, 0);
   for (uint64_t tile = 0; tile < KT; ++tile) {
-    int64_t o = tile_base[tile];
+    int64_t o = P.chain_key0[tile + 1];
     for (int t = 0; t < nth; ++t) {
       cur[size_t(t) * KT + tile] = o;
       o += hist[size_t(t) * KT + tile];
     }
   }
-  P.items.resize(size_t(nC + nB));
-  uint64_t* items = P.items.data();
+  P.kitems.resize(size_t(P.chain_key0[KT + 1]));
+  uint64_t* items = P.kitems.data();
 #pragma omp parallel num_threads(nth)
   {
     const int t = omp_get_thread_num();
     int64_t* c = &cur[size_t(t) * KT];
     for (int32_t b = tb0[size_t(t)]; b < tb0[size_t(t) + 1]; ++b) {
-      int64_t oc = cstart[size_t(b)];
-      for_block_eps(b, [&](uint64_t ep, uint32_t x) {
-        items[oc++] = make_item(true, x >> ts, ep);
-        items[c[x >> ts]++] = make_item(true, x & mask, ep);
-      });
       for (int64_t s = P.blk_tile0[size_t(b)]; s < P.blk_tile0[size_t(b) + 1]; ++s) {
         const uint32_t tile = P.bkey[size_t(P.blk_key0[size_t(b)] + P.tile_rank0[size_t(s)])] >> ts;
-        items[oc++] = make_item(false, tile, uint64_t(s));
+        items[s] = key_item(tile, uint64_t(s));
       }
       for (int64_t r = P.blk_key0[size_t(b)]; r < P.blk_key0[size_t(b) + 1]; ++r) {
         const uint32_t k = P.bkey[size_t(r)];
-        items[c[k >> ts]++] = make_item(false, k & mask, uint64_t(r));
+        items[c[k >> ts]++] = key_item(k & mask, uint64_t(r));
+      }
+    }
+  }
+  // level-B positions: per tile, the blocks holding its keys and where their keys start
+  {
+    std::vector<int64_t> cnt(KT + 1, 0);
+    for (int32_t b = 0; b < nblk; ++b)
+      for (int64_t s = P.blk_tile0[size_t(b)]; s < P.blk_tile0[size_t(b) + 1]; ++s)
+        cnt[(P.bkey[size_t(P.blk_key0[size_t(b)] + P.tile_rank0[size_t(s)])] >> ts) + 1]++;
+    for (uint64_t t = 0; t < KT; ++t) cnt[t + 1] += cnt[t];
+    P.tile_tb0 = cnt;
+    P.tb_block.resize(size_t(cnt[KT]));
+    P.tb_pos.resize(size_t(cnt[KT]));
+    std::vector<int64_t> run(KT, 0), fill(cnt.begin(), cnt.end() - 1);
+    for (int32_t b = 0; b < nblk; ++b) {
+      const int64_t t0 = P.blk_tile0[size_t(b)], t1 = P.blk_tile0[size_t(b) + 1];
+      const int64_t R = P.blk_key0[size_t(b) + 1] - P.blk_key0[size_t(b)];
+      for (int64_t s = t0; s < t1; ++s) {
+        const uint32_t tile = P.bkey[size_t(P.blk_key0[size_t(b)] + P.tile_rank0[size_t(s)])] >> ts;
+        const int64_t n = ((s + 1 < t1) ? P.tile_rank0[size_t(s) + 1] : R) - P.tile_rank0[size_t(s)];
+        P.tb_block[size_t(fill[tile])] = b;
+        P.tb_pos[size_t(fill[tile]++)] = run[tile];
+        run[tile] += n;
       }
     }
   }
 
-  // ---- 6. chunks and scan segments
+  // ---- 5. chunks (at least one per chain) and carry-scan segments.  Auto chunk sizes (knob 0)
+  //          keep each level's dense carry rows near 64 MiB: the smallest power of two >= 1024
+  //          with (key items / chunk) * classes * 8 B <= 64 MiB.
+  auto auto_chunk = [](int64_t items, int64_t ncls) {
+    int64_t c = 1024;
+    while (c < (int64_t(1) << 19) && (items / c) * ncls * 8 > (int64_t(64) << 20)) c <<= 1;
+    return c;
+  };
+  if (P.knobs.chunk_items_c == 0) P.knobs.chunk_items_c = auto_chunk(nC, int64_t(KT));
+  if (P.knobs.chunk_items == 0) P.knobs.chunk_items = auto_chunk(P.chain_key0[KT + 1] - nC, int64_t(W));
   P.chain_chunk0.push_back(0);
   P.chain_seg0.push_back(0);
   P.seg_chunk0.push_back(0);
   int64_t cp = 0, sg = 0;
-  for (int32_t ch = 0; ch < int32_t(chain_rng.size()); ++ch) {
-    const int64_t a = chain_rng[size_t(ch)].first, e = chain_rng[size_t(ch)].second;
+  for (int32_t ch = 0; ch < P.nchain(); ++ch) {
+    const int64_t CH = ch == 0 ? P.knobs.chunk_items_c : P.knobs.chunk_items;
+    const int64_t a = P.chain_key0[size_t(ch)], e = P.chain_key0[size_t(ch) + 1];
     const int ncls = P.chain_ncls[size_t(ch)];
-    int64_t nck = 0;
-    for (int64_t i = a; i < e || (nck == 0 && i == a); i += kn.chunk_items) {
-      P.chunk_item0.push_back(i);
+    const int64_t nck = std::max<int64_t>(1, (e - a + CH - 1) / CH);
+    for (int64_t k = 0; k < nck; ++k) {
+      P.chunk_key0.push_back(a + k * CH);
       P.chunk_chain.push_back(ch);
       P.chunk_cp0.push_back(cp);
       cp += ncls;
-      ++nck;
-      if (nck % kn.seg_chunks == 0) {
+      if ((k + 1) % kn.seg_chunks == 0 || k + 1 == nck) {
         P.seg_chunk0.push_back(int64_t(P.chunk_chain.size()));
         P.seg_cp0.push_back(sg);
         sg += ncls;
       }
     }
-    if (nck % kn.seg_chunks != 0) {
-      P.seg_chunk0.push_back(int64_t(P.chunk_chain.size()));
-      P.seg_cp0.push_back(sg);
-      sg += ncls;
-    }
     P.chain_chunk0.push_back(int64_t(P.chunk_chain.size()));
     P.chain_seg0.push_back(int64_t(P.seg_cp0.size()));
   }
-  P.chunk_item0.push_back(nC + nB);
+  P.chunk_key0.push_back(P.chain_key0.back());
   P.n_cp = cp;
   P.n_segsum = sg;
   P.max_cls = int(std::max<uint64_t>(KT, W));
+  return true;
+}
+
+bool build_batch_plan(const TreeIndexPlan& X, int64_t M, const int32_t* qstate, const uint32_t* qlo,
+                      const uint32_t* qhi, bool all_blocks, BatchPlan& B, char* err, size_t errlen) {
+  auto fail = [&](const char* m) {
+    if (err && errlen) std::snprintf(err, errlen, "%s", m);
+    return false;
+  };
+  if (M < 0 || 2 * M > int64_t(0xFFFFFFFF)) return fail("bad batch size");
+  for (int64_t q = 0; q < M; ++q)
+    if (qstate[q] < 1 || qstate[q] > X.S || qlo[q] > X.NK || qhi[q] > X.NK) return fail("query out of range");
+  B = BatchPlan();
+  B.M = M;
+  const int32_t nblk = X.nblk();
+  const int64_t nsteps = int64_t(X.steps.size());
+  const int ts = X.knobs.tile_shift;
+  const uint32_t mask = (1u << ts) - 1;
+  const int nth = std::max(1, omp_get_max_threads());
+  // ---- queries ordered by enter step (stable), hence grouped by block
+  std::vector<int64_t> scnt(size_t(nsteps) + 1, 0);
+  for (int64_t q = 0; q < M; ++q) scnt[size_t(X.state_step[size_t(qstate[q])]) + 1]++;
+  for (int64_t s = 0; s < nsteps; ++s) scnt[size_t(s) + 1] += scnt[size_t(s)];
+  B.qorder.resize(size_t(M));
+  B.qstep.resize(size_t(M));
+  {
+    std::vector<int64_t> f(scnt.begin(), scnt.end() - 1);
+    for (int64_t q = 0; q < M; ++q) {
+      const int32_t s = X.state_step[size_t(qstate[q])];
+      const int64_t at = f[size_t(s)]++;
+      B.qorder[size_t(at)] = uint32_t(q);
+      B.qstep[size_t(at)] = s;
+    }
+  }
+  // per listed block: [qblk_q0[i], qblk_q0[i+1]) in qorder (listed blocks are ascending, and
+  // the queries of unlisted blocks are none, so consecutive ranges stay contiguous)
+  B.qblk_q0.push_back(0);
+  for (int32_t b = 0; b < nblk; ++b) {
+    const int64_t e = scnt[size_t(X.blk_step0[size_t(b) + 1])];
+    if (all_blocks || e > B.qblk_q0.back()) {
+      B.qblk.push_back(b);
+      B.qblk_q0.push_back(e);
+    }
+  }
+  // ---- endpoint items: (chunk, item) on level C and level B for every endpoint
+  const int64_t nE = 4 * M;
+  std::vector<int64_t> ech(static_cast<size_t>(nE));
+  std::vector<uint64_t> eit(static_cast<size_t>(nE));
+  auto chunk_of = [&](int32_t chain, int64_t pos, int64_t& local) {
+    const int64_t CH = chain == 0 ? X.knobs.chunk_items_c : X.knobs.chunk_items;
+    const int64_t n = X.chain_chunk0[size_t(chain) + 1] - X.chain_chunk0[size_t(chain)];
+    int64_t k = pos / CH;
+    if (k >= n) k = n - 1;
+    local = pos - k * CH;
+    return X.chain_chunk0[size_t(chain)] + k;
+  };
+#pragma omp parallel for schedule(static)
+  for (int64_t q = 0; q < M; ++q) {
+    const int32_t b = X.state_block[size_t(qstate[q])];
+    for (int side = 0; side < 2; ++side) {
+      const uint32_t x = side ? qhi[q] : qlo[q];
+      const uint64_t ep = uint64_t(2 * q + side);
+      int64_t local;
+      // level C: the tile slots of blocks < b precede the endpoint
+      const int64_t cc = chunk_of(0, X.blk_tile0[size_t(b)], local);
+      ech[size_t(4 * q + 2 * side)] = cc;
+      eit[size_t(4 * q + 2 * side)] = ep_item(uint64_t(local), x >> ts, ep);
+      // level B: the keys of tile t in blocks < b precede it
+      const uint32_t t = x >> ts;
+      const int64_t lo = X.tile_tb0[t], hi = X.tile_tb0[size_t(t) + 1];
+      const int64_t idx = std::lower_bound(X.tb_block.begin() + lo, X.tb_block.begin() + hi, b) - X.tb_block.begin();
+      const int64_t pos = idx < hi ? X.tb_pos[size_t(idx)] : X.chain_key0[size_t(t) + 2] - X.chain_key0[size_t(t) + 1];
+      const int64_t cb = chunk_of(1 + int32_t(t), pos, local);
+      ech[size_t(4 * q + 2 * side + 1)] = cb;
+      eit[size_t(4 * q + 2 * side + 1)] = ep_item(uint64_t(local), x & mask, ep);
+    }
+  }
+  // ---- bucket by chunk (stable counting sort), then order each chunk by position
+  const int64_t nchunk = X.nchunk();
+  std::vector<int64_t> ccnt(size_t(nth) * size_t(nchunk), 0);
+  std::vector<int64_t> e0(size_t(nth) + 1);
+  for (int t = 0; t <= nth; ++t) e0[size_t(t)] = nE * t / nth;
+#pragma omp parallel num_threads(nth)
+  {
+    const int t = omp_get_thread_num();
+    int64_t* h = &ccnt[size_t(t) * size_t(nchunk)];
+    for (int64_t i = e0[size_t(t)]; i < e0[size_t(t) + 1]; ++i) h[size_t(ech[size_t(i)])]++;
+  }
+  std::vector<int64_t> cstart(size_t(nchunk) + 1, 0);
+  for (int64_t c = 0; c < nchunk; ++c) {
+    int64_t s = 0;
+    for (int t = 0; t < nth; ++t) s += ccnt[size_t(t) * size_t(nchunk) + size_t(c)];
+    cstart[size_t(c) + 1] = cstart[size_t(c)] + s;
+  }
+  {
+    std::vector<int64_t> pos(cstart.begin(), cstart.end() - 1);
+    for (int t = 0; t < nth; ++t)
+      for (int64_t c = 0; c < nchunk; ++c) {
+        int64_t& slot = ccnt[size_t(t) * size_t(nchunk) + size_t(c)];
+        const int64_t n = slot;
+        slot = pos[size_t(c)];
+        pos[size_t(c)] += n;
+      }
+  }
+  B.eitems.resize(size_t(nE));
+#pragma omp parallel num_threads(nth)
+  {
+    const int t = omp_get_thread_num();
+    int64_t* c = &ccnt[size_t(t) * size_t(nchunk)];
+    for (int64_t i = e0[size_t(t)]; i < e0[size_t(t) + 1]; ++i)
+      B.eitems[size_t(c[size_t(ech[size_t(i)])]++)] = eit[size_t(i)];
+  }
+#pragma omp parallel for schedule(dynamic, 8)
+  for (int64_t c = 0; c < nchunk; ++c)
+    std::stable_sort(B.eitems.begin() + cstart[size_t(c)], B.eitems.begin() + cstart[size_t(c) + 1],
+                     [](uint64_t a, uint64_t b) { return (a >> BWT_EP_POS_SHIFT) < (b >> BWT_EP_POS_SHIFT); });
+  // work items: each chunk's endpoints in runs of at most ep_per_work (a work item replays the
+  // chunk's keys only up to its own last endpoint)
+  B.wk_e0.push_back(0);
+  const int64_t G = X.knobs.ep_per_work;
+  for (int64_t c = 0; c < nchunk; ++c)
+    for (int64_t a = cstart[size_t(c)]; a < cstart[size_t(c) + 1]; a += G) {
+      B.wk_chunk.push_back(c);
+      B.wk_e0.push_back(std::min(a + G, cstart[size_t(c) + 1]));
+    }
+  // in-block ranks of every query's endpoints (aligned with qorder)
+  B.qrank.resize(size_t(M));
+#pragma omp parallel for schedule(static)
+  for (int64_t j = 0; j < M; ++j) {
+    const uint32_t q = B.qorder[size_t(j)];
+    const int32_t b = X.state_block[size_t(qstate[q])];
+    const uint32_t* k0 = X.bkey.data() + X.blk_key0[size_t(b)];
+    const uint32_t* k1 = X.bkey.data() + X.blk_key0[size_t(b) + 1];
+    const uint32_t rl = uint32_t(std::lower_bound(k0, k1, qlo[q]) - k0);
+    const uint32_t rh = uint32_t(std::lower_bound(k0, k1, qhi[q]) - k0);
+    B.qrank[size_t(j)] = rl | (rh << 16);
+  }
   return true;
 }
 

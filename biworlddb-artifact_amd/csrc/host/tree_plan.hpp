@@ -1,7 +1,11 @@
-// tree_plan.hpp -- host plan of the generalised batch path (the BatchEngineV7::build analogue,
+// tree_plan.hpp -- host plans of the generalised batch path (the BatchEngineV7::build analogue,
 // src/batch_engine_v7.hpp:576-704 and build_v7_blocks :707-830, for branching multi-update
-// trees): Euler linearisation, blocks sized to the LDS Fenwick tree, per-block distinct keys
-// and key tiles, and the two carry levels' chains / chunks / scan segments.
+// trees).
+//   TreeIndexPlan  static, update-side: Euler linearisation, blocks sized to the LDS Fenwick
+//                  tree, per-block distinct keys / key tiles / in-block event ranks, and the two
+//                  carry levels' chains, chunks and scan segments.
+//   BatchPlan      per query batch: queries grouped by block and step, endpoint items grouped
+//                  by chunk and position.
 #pragma once
 #include <cstddef>
 #include <cstdint>
@@ -10,42 +14,63 @@
 namespace bwg {
 
 struct PlanKnobs {
-  int rmax = 2048;          // max distinct keys per block (LDS Fenwick size of the block replay)
-  int tile_shift = 11;      // key tile = key >> tile_shift; level-B classes = 1 << tile_shift
-  int64_t chunk_items = 16384;  // items per carry chunk
-  int seg_chunks = 32;      // chunks per carry-scan segment
-  int max_block_steps = 1 << 30;  // optional cap on steps per block
+  int rmax = 2048;              // max distinct keys per block (LDS Fenwick size of the block replay)
+  int tile_shift = 0;           // key tile = key >> tile_shift (level-B classes = 1 << tile_shift);
+                                // 0 = balanced: ceil(log2(NK+1) / 2), so both levels' trees are small
+  int64_t chunk_items = 0;      // key items per carry chunk on level B (0 = auto)
+  int64_t chunk_items_c = 0;    // key items per carry chunk on level C (0 = auto)
+  int64_t ep_per_work = 2048;   // endpoints per chunk work item (batch plans)
+  int seg_chunks = 32;          // chunks per carry-scan segment
+  int max_block_steps = 1 << 30;
 };
 
-struct TreePlan {
+struct TreeIndexPlan {
   int64_t S = 0;
-  int U = 0, Q = 0;
+  int U = 0;
   uint64_t NK = 0;  // coordinates are in [0, NK]
   PlanKnobs knobs;
+  int KT = 0;       // key tiles (level-C classes)
   // Euler blocks
   std::vector<int32_t> steps, blk_step0;
   std::vector<int64_t> blk_key0, blk_tile0;
   std::vector<uint32_t> bkey;
-  std::vector<uint16_t> tile_rank0;
-  std::vector<int32_t> state_block;  // S+1: block of the state's enter step (-1 for the root)
-  // carry chains
-  std::vector<uint64_t> items;
-  std::vector<int64_t> chunk_item0, chunk_cp0, chain_chunk0;
+  std::vector<uint16_t> tile_rank0, evrank;
+  std::vector<int32_t> state_block, state_step;  // S+1 (-1 for the root)
+  // carry chains: chain 0 = level C over all blocks; chain 1+t = level B of tile t
+  std::vector<uint64_t> kitems;
+  std::vector<int64_t> chunk_key0, chunk_cp0, chain_chunk0, chain_key0;
   std::vector<int32_t> chunk_chain, chain_ncls, chain_src, chain_out;
   std::vector<int64_t> seg_chunk0, seg_cp0, chain_seg0;
   int64_t n_cp = 0, n_segsum = 0;
   int max_cls = 0;
+  // endpoint positions on level B: blocks holding keys of tile t and where they start
+  std::vector<int64_t> tile_tb0;
+  std::vector<int32_t> tb_block;
+  std::vector<int64_t> tb_pos;
   int32_t nblk() const { return int32_t(blk_step0.size()) - 1; }
   int64_t nchunk() const { return int64_t(chunk_chain.size()); }
   int64_t nseg() const { return int64_t(seg_chunk0.size()) - 1; }
   int32_t nchain() const { return int32_t(chain_ncls.size()); }
-  size_t host_bytes() const;
 };
 
-// parent: S+1 entries (parent[0] = -1, parent[v] < v); key: S*U coordinates in [0, NK);
-// qlo/qhi: S*Q coordinates in [0, NK].  Returns false with a message on invalid input.
-bool build_tree_plan(int64_t S, int U, int Q, uint64_t NK, const int32_t* parent, const uint32_t* key,
-                     const uint32_t* qlo, const uint32_t* qhi, const PlanKnobs& knobs, TreePlan& plan,
-                     char* err, size_t errlen);
+struct BatchPlan {
+  int64_t M = 0;
+  std::vector<int32_t> qblk;
+  std::vector<int64_t> qblk_q0;
+  std::vector<uint32_t> qorder;
+  std::vector<int32_t> qstep;
+  std::vector<uint32_t> qrank;
+  std::vector<int64_t> wk_chunk, wk_e0;
+  std::vector<uint64_t> eitems;
+};
+
+// parent: S+1 entries (parent[0] = -1, parent[v] < v); key: S*U coordinates in [0, NK).
+bool build_tree_index(int64_t S, int U, uint64_t NK, const int32_t* parent, const uint32_t* key,
+                      const PlanKnobs& knobs, TreeIndexPlan& plan, char* err, size_t errlen);
+
+// M queries (state in [1,S], endpoint coordinates in [0, NK]).  all_blocks lists every block
+// (needed when the same launch also writes the index).
+bool build_batch_plan(const TreeIndexPlan& x, int64_t M, const int32_t* qstate, const uint32_t* qlo,
+                      const uint32_t* qhi, bool all_blocks, BatchPlan& b, char* err, size_t errlen);
 
 }  // namespace bwg

@@ -9,26 +9,27 @@
  * carry, combine; src/batch_engine_v7.hpp:832-968) to that case; they have no counterpart in
  * the reference ABI.  No HIP or torch types appear here: device pointers are plain pointers,
  * a stream is an opaque pointer (NULL = the default stream), every call returns 0 or a
- * hipError_t value, like the reference launchers (src/gpu_v7.cu:98-121).
+ * hipError_t value, like the reference launchers (src/gpu_v7.cu:98-121).  Nothing allocates,
+ * synchronises or memsets, so every sequence of these calls can be captured into a hipGraph.
  *
  * Model.  States 1..S (0 is an empty root); state v applies U point updates (key, delta) on
- * top of its parent and asks Q half-open range sums [lo, hi) over keys, answered after its own
- * updates.  Keys and endpoints are coordinates in [0, NK] (for a sparse 32-bit key space they
- * are ranks, see bwt_rank_remap_launch).  Answers are int64 in canonical order (v-1)*Q + k,
- * exactly the reference oracle's order (src/oracle.cpp:459-466).
+ * top of its parent.  A query (v, lo, hi) asks the sum of the deltas on the root->v path with
+ * lo <= key < hi.  Keys and endpoints are coordinates in [0, NK] (for a sparse 32-bit key space
+ * they are ranks, see bwt_rank_remap_launch).  Query q's answer goes to answers[q]; the
+ * reference's declared workload (Q queries per state, order (v-1)*Q+k, src/oracle.cpp:459-466)
+ * is the batch whose query q belongs to state q/Q+1.
  *
- * Algorithm (the plan is built on the host by bwt_plan_build, the passes run on the GPU):
- *  1. The tree is linearised by its preorder walk into steps (+v enter, -v exit = undo);
- *     consecutive steps form blocks whose distinct keys fit a shared-memory Fenwick tree.
- *  2. Block replay (one workgroup per block): local Fenwick over the block's distinct keys;
- *     for each query endpoint the in-block prefix; at the end the block's net weight per
- *     distinct key (agg) and per key tile (tagg).
- *  3. Carry over earlier blocks, split into two key levels so that every structure fits
- *     shared memory: level C = one chain over all blocks with class = key tile (key >> TS);
- *     level B = one chain per tile with class = key & (2^TS - 1).  Chains are cut into chunks;
- *     a chunk histogram + a segmented column scan gives each chunk its dense carry; a chunk
- *     replay adds the chunk-local Fenwick prefix.  (v7's dV / carrypref / lvals, generalised.)
- *  4. Combine: answer = in-block(hi) - in-block(lo) + carry(hi) - carry(lo).
+ * Algorithm.  The tree is linearised by its preorder walk into steps (+v enter, -v exit = undo);
+ * consecutive steps form blocks whose distinct keys fit an LDS Fenwick tree.  answer(q) =
+ * F(hi) - F(lo), F(x) = sum over the events before q's step with key < x, split as
+ *   in-block part  -- block replay over the block's own distinct keys (local ranks);
+ *   carry level C  -- one chain over all blocks, class = key tile (key >> TS);
+ *   carry level B  -- one chain per key tile, class = key & (2^TS - 1).
+ * Chains are cut into chunks; chunk histograms + a segmented column scan give every chunk its
+ * dense carry; a chunk replay adds the chunk-local prefix at each query endpoint.  (These are
+ * v7's lvals / dV / carrypref / combine, generalised.)  The *index* (block weights, chunk
+ * carries) depends only on the updates and is built once; a *batch* (queries) only replays the
+ * blocks and chunks it touches.
  */
 #ifndef BWGPU_TREE_H
 #define BWGPU_TREE_H
@@ -39,76 +40,99 @@
 extern "C" {
 #endif
 
-/* Item encoding of the carry chains (64 bits): bit 63 = endpoint item, bits 40..51 = class,
- * bits 0..39 = reference (endpoint id 2*((v-1)*Q+k)+side for endpoints; weight index for keys). */
-#define BWT_ITEM_EP (1ull << 63)
+/* Key item of a carry chain: class << 40 | weight index (into agg for level B, tagg for C). */
 #define BWT_CLASS_SHIFT 40
 #define BWT_REF_MASK ((1ull << 40) - 1)
+/* Endpoint item of a batch: local position << 44 | class << 32 | endpoint id (2q + side).
+ * The position is the number of the chunk's key items that precede the endpoint. */
+#define BWT_EP_CLASS_SHIFT 32
+#define BWT_EP_POS_SHIFT 44
 
-typedef struct BwtPassArgs {
-  /* sizes */
-  int64_t S;             /* edited states */
-  int32_t U, Q;          /* updates and queries per state */
-  int32_t nblk;          /* Euler blocks */
-  int32_t block_threads; /* threads of the block-replay workgroup (multiple of 64, <= 1024) */
-  int32_t rmax;          /* max distinct keys of one block (shared-memory Fenwick size) */
-  int32_t max_cls;       /* max classes of any chain (<= 4096) */
-  int64_t nchunk;        /* carry chunks over all chains */
+/* Static, update-side structure (resident across batches). */
+typedef struct BwtIndex {
+  int64_t S;
+  int32_t U;
+  int32_t nblk;
+  int32_t rmax;      /* max distinct keys of one block (LDS Fenwick size of the block replay) */
+  int32_t max_cls;   /* max classes of any chain (<= 4096) */
   int32_t nchain;
-  int32_t chunk_waves;   /* waves per workgroup of the chunk kernels */
-  /* inputs: coordinates (keys / ranks), deltas, query endpoints */
-  const uint32_t* key;   /* S*U */
-  const int64_t* delta;  /* S*U */
-  const uint32_t* qlo;   /* S*Q */
-  const uint32_t* qhi;   /* S*Q */
-  /* plan: Euler blocks */
-  const int32_t* steps;      /* +v enter, -v exit */
-  const int32_t* blk_step0;  /* nblk+1 */
-  const int64_t* blk_key0;   /* nblk+1, offsets into bkey / agg */
-  const uint32_t* bkey;      /* sorted distinct keys of every block */
-  const int64_t* blk_tile0;  /* nblk+1, offsets into tile_rank0 / tagg */
-  const uint16_t* tile_rank0;/* first in-block rank of each key tile present in the block */
-  /* plan: carry chains */
-  const uint64_t* items;     /* all chains' items, chain-major */
-  const int64_t* chunk_item0;/* nchunk+1 */
-  const int32_t* chunk_chain;/* nchunk */
-  const int64_t* chunk_cp0;  /* nchunk, offset of the chunk's dense carry row in cp */
-  const int32_t* chain_ncls; /* nchain */
-  const int32_t* chain_src;  /* nchain: 0 = weights from agg (level B), 1 = from tagg (level C) */
-  const int32_t* chain_out;  /* nchain: 0 or 1, which half of g receives the endpoint values */
-  const int64_t* chain_chunk0; /* nchain+1 */
-  /* plan: carry-scan segments (runs of consecutive chunks of one chain) */
+  int32_t ncls_c;    /* classes of chain 0 (level C: key tiles) */
+  int32_t ncls_b;    /* classes of every other chain (level B: keys within a tile) */
+  int32_t pad0;
+  int64_t nchunk;
+  int64_t nchunk_c;  /* chunks of chain 0; they come first */
   int64_t nseg;
-  const int64_t* seg_chunk0; /* nseg+1 */
-  const int64_t* seg_cp0;    /* nseg, offset of the segment's row in segsum */
-  const int64_t* chain_seg0; /* nchain+1 */
-  int64_t* segsum;           /* scratch: sum over segments of ncls(chain) */
-  /* scratch / outputs */
-  int64_t* agg;    /* blk_key0[nblk] */
-  int64_t* tagg;   /* blk_tile0[nblk] */
-  int64_t* cp;     /* sum over chunks of ncls(chain) */
-  int64_t* g;      /* 2 * 2*S*Q endpoint carries (level C half, level B half) */
-  int64_t* ansr;   /* S*Q in-block differences */
-  int64_t* answers;/* S*Q */
-} BwtPassArgs;
+  const int64_t* delta;       /* S*U update deltas (state-major) */
+  const int32_t* steps;       /* +v enter, -v exit */
+  const int32_t* blk_step0;   /* nblk+1 */
+  const uint16_t* evrank;     /* nsteps*U: in-block rank of the key of update i of step s (s*U+i) */
+  const int64_t* blk_key0;    /* nblk+1, offsets into bkey / agg */
+  const uint32_t* bkey;       /* sorted distinct keys of every block (coordinates) */
+  const int64_t* blk_tile0;   /* nblk+1, offsets into tile_rank0 / tagg */
+  const uint16_t* tile_rank0; /* first in-block rank of each key tile present in the block */
+  const uint64_t* kitems;     /* key items of all chains, chain-major */
+  const int64_t* chunk_key0;  /* nchunk+1 */
+  const int32_t* chunk_chain; /* nchunk */
+  const int64_t* chunk_cp0;   /* nchunk: offset of the chunk's dense carry row in cp */
+  const int32_t* chain_ncls;  /* nchain */
+  const int32_t* chain_src;   /* nchain: 0 = weights from agg (level B), 1 = from tagg (level C) */
+  const int32_t* chain_out;   /* nchain: which half of g receives the endpoint values */
+  const int64_t* chain_chunk0;/* nchain+1 */
+  const int64_t* seg_chunk0;  /* nseg+1: carry-scan segments = runs of chunks of one chain */
+  const int64_t* seg_cp0;     /* nseg */
+  const int64_t* chain_seg0;  /* nchain+1 */
+  int64_t* agg;     /* net weight of every block per distinct key */
+  int64_t* tagg;    /* net weight of every block per key tile */
+  int64_t* cp;      /* chunk carries (dense rows) */
+  int64_t* segsum;  /* scan scratch */
+} BwtIndex;
 
-/* One complete pass (block replay, chunk histogram, carry scan, chunk replay, combine) on
- * `stream`.  Asynchronous; safe to capture into a hipGraph (no host synchronisation, no
- * allocation, no memset). */
-int bwt_pass_launch(const BwtPassArgs* a, void* stream);
+/* One query batch (built on the host by the batch planner; all device pointers). */
+typedef struct BwtBatch {
+  int64_t M;                  /* queries */
+  const uint32_t* qlo;        /* M endpoint coordinates */
+  const uint32_t* qhi;
+  int32_t nqblk;              /* blocks holding queried states */
+  int32_t pad0;
+  const int32_t* qblk;        /* nqblk block ids */
+  const int64_t* qblk_q0;     /* nqblk+1 offsets into qorder / qstep */
+  const uint32_t* qorder;     /* query ids of each listed block, by step */
+  const int32_t* qstep;       /* their (global) enter steps */
+  const uint32_t* qrank;      /* their endpoints' in-block ranks: lo | hi << 16 */
+  int64_t nwork;              /* chunk work items: a chunk and a run of its endpoints */
+  int64_t nwork_c;            /* work items on chain 0; they come first */
+  const int64_t* wk_chunk;    /* nwork chunk ids */
+  const int64_t* wk_e0;       /* nwork+1 offsets into eitems */
+  const uint64_t* eitems;     /* endpoint items per chunk, by position */
+  int64_t* ansr;              /* M: in-block part */
+  int64_t* g;                 /* 2 * 2M: carry per endpoint (level C half, level B half) */
+  int64_t* answers;           /* M */
+} BwtBatch;
 
-/* Individual stages (same arguments), for profiling and the batched form. */
-int bwt_block_replay_launch(const BwtPassArgs* a, void* stream);
-int bwt_chunk_hist_launch(const BwtPassArgs* a, void* stream);
-int bwt_carry_scan_launch(const BwtPassArgs* a, void* stream);
-int bwt_chunk_replay_launch(const BwtPassArgs* a, void* stream);
-int bwt_combine_launch(const BwtPassArgs* a, void* stream);
+typedef struct BwtLaunch {
+  int32_t block_threads;  /* block replay workgroup (multiple of 64, <= 1024) */
+  int32_t chunk_waves;    /* wavefronts per workgroup of the chunk kernels (1..4), one chunk each */
+  int32_t write_index;    /* block replay also writes agg / tagg (needs every block listed) */
+  int32_t pad0;
+} BwtLaunch;
+
+/* Index build: agg/tagg are written by a block replay with write_index set; then: */
+int bwt_chunk_hist_launch(const BwtIndex* x, const BwtLaunch* l, void* stream);
+int bwt_carry_scan_launch(const BwtIndex* x, const BwtLaunch* l, void* stream);
+/* Query side. */
+int bwt_block_replay_launch(const BwtIndex* x, const BwtBatch* b, const BwtLaunch* l, void* stream);
+int bwt_chunk_replay_launch(const BwtIndex* x, const BwtBatch* b, const BwtLaunch* l, void* stream);
+int bwt_combine_launch(const BwtIndex* x, const BwtBatch* b, const BwtLaunch* l, void* stream);
+/* Everything for one batch over a cold index (block replay writing the index, histogram, scan,
+ * chunk replay, combine): the plain "one launch sequence per query batch" form. */
+int bwt_pass_launch(const BwtIndex* x, const BwtBatch* b, const BwtLaunch* l, void* stream);
+/* The query side only, over a resident index (block replay, chunk replay, combine). */
+int bwt_query_launch(const BwtIndex* x, const BwtBatch* b, const BwtLaunch* l, void* stream);
 
 /* CSR rank-LUT over a sorted set of D distinct 32-bit keys (src/sparse_engine.hpp:62-97 is the
  * reference's bitmap+rank-LUT; this is its CSR form): row h = keys whose high 16 bits equal h,
  * row_ptr[h] = number of keys with high bits < h (65537 entries), low16[i] = low 16 bits of
- * key i.  rank(x) = #keys < x = row_ptr[x>>16] + #(low16 in row x>>16 that are < (x & 0xFFFF)).
- * Maps n values in place-free fashion: out[i] = rank(in[i]). */
+ * key i.  out[i] = rank(in[i]) = #keys < in[i]. */
 typedef struct BwtRankArgs {
   int64_t n;
   const uint32_t* in;

@@ -31,18 +31,30 @@ def _cmd(pkg, g, knobs):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["pass", "query"])
 @pytest.mark.parametrize("case,knobs", CASES)
-def test_tree_gpu_matches_golden(pkg, case, knobs):
+def test_tree_gpu_matches_golden(pkg, case, knobs, mode):
+    """pass: index rebuilt inside every pass; query: resident index, only touched blocks/chunks."""
     g = TREE[case]
-    r = run_json(_cmd(pkg, g, knobs), timeout=300)
-    assert r["answers_sha256"] == g["oracle"], (case, knobs)
+    r = run_json(_cmd(pkg, g, knobs + ["--mode", mode]), timeout=300)
+    assert r["answers_sha256"] == g["oracle"], (case, knobs, mode)
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["pass", "query"])
 @pytest.mark.parametrize("case", ["c2_cfg2", "c4_cfg4", "c3_cfg3"])
-def test_tree_gpu_full_size(pkg, case):
+def test_tree_gpu_full_size(pkg, case, mode):
     """The benchmark configs at full size (300,000 states): same complete-answer SHA as the oracle."""
     g = TREE[case]
-    r = run_json(_cmd(pkg, g, []), timeout=600)
+    r = run_json(_cmd(pkg, g, ["--mode", mode]), timeout=600)
     assert r["answers_sha256"] == g["oracle"], case
     assert r["n_answers"] == g["S"] * g["Q"]
+
+
+@pytest.mark.gpu
+def test_rank_remap_matches_host_ranks(pkg):
+    """The device CSR rank-LUT remap of raw 32-bit endpoints equals the host ranks exactly."""
+    g = TREE["c3_cfg3"]
+    r = run_json(_cmd(pkg, g, ["--verify-remap"]), timeout=600)
+    assert r["remap_mismatch"] == 0
+    assert r["answers_sha256"] == g["oracle"]
