@@ -3,7 +3,7 @@
 // generate (load), setup (host plans), upload, execute (device passes), extract (D2H + SHA-256).
 //
 //   bw_tree --cfg 2 [--S n --U n --Q n --N n --seed n] [--mode pass|query]
-//           [--rmax 2048 --tile-shift 11 --chunk 16384 --seg 32 --threads 256 --waves 4]
+//           [--rmax 2048 --tile-shift 10 --chunk 4096 --seg 32 --threads 256 --waves 4]
 //           [--warmup 1 --repeat 5] [--answers FILE] [--json FILE]
 // --mode pass  : the plain form -- every timed pass rebuilds the index (block replay with weights,
 //                chunk histograms, carry scan) and answers the declared queries.
@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "bwgpu_tree.h"
+#include "tree_engine.hpp"
 #include "tree_plan.hpp"
 #include "tree_workload.hpp"
 
@@ -50,23 +51,6 @@ static double ms_between(clk::time_point a, clk::time_point b) {
   return std::chrono::duration<double, std::milli>(b - a).count();
 }
 
-static size_t g_dev_bytes = 0;
-template <typename T>
-static T* upload(const std::vector<T>& v) {
-  T* d = nullptr;
-  const size_t n = std::max<size_t>(v.size(), 1) * sizeof(T);
-  HIPX(hipMalloc(&d, n));
-  if (!v.empty()) HIPX(hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
-  g_dev_bytes += n;
-  return d;
-}
-template <typename T>
-static T* scratch(size_t n) {
-  T* d = nullptr;
-  HIPX(hipMalloc(&d, std::max<size_t>(n, 1) * sizeof(T)));
-  g_dev_bytes += std::max<size_t>(n, 1) * sizeof(T);
-  return d;
-}
 
 static std::string sha_hex(const void* p, size_t n) {
   unsigned char d[32];
@@ -110,7 +94,6 @@ int main(int argc, char** argv) {
     else if (!std::strcmp(k, "--rmax")) kn.rmax = std::atoi(v);
     else if (!std::strcmp(k, "--tile-shift")) kn.tile_shift = std::atoi(v);
     else if (!std::strcmp(k, "--chunk")) kn.chunk_items = std::atoll(v);
-    else if (!std::strcmp(k, "--chunk-c")) kn.chunk_items_c = std::atoll(v);
     else if (!std::strcmp(k, "--ep-per-work")) kn.ep_per_work = std::atoll(v);
     else if (!std::strcmp(k, "--seg")) kn.seg_chunks = std::atoi(v);
     else if (!std::strcmp(k, "--max-block-steps")) kn.max_block_steps = std::atoi(v);
@@ -155,86 +138,52 @@ int main(int argc, char** argv) {
     host_ranks(lut, w.qhi.data(), hc.data(), nq);
     kcoord = kc.data(); lcoord = lc.data(); hcoord = hc.data();
   }
-  TreeIndexPlan P;
   char err[256];
-  if (!build_tree_index(S, spec.U, w.coord_n(), w.parent.data(), kcoord, kn, P, err, sizeof(err))) {
-    std::fprintf(stderr, "bw_tree: index plan: %s\n", err);
-    return 3;
+  HIPX(hipSetDevice(0));
+  HIPX(hipFree(nullptr));
+  TreeEngine E;
+  if (!E.init(S, spec.U, w.coord_n(), w.parent.data(), kcoord, w.delta.data(), kn, BwtLaunch{threads, waves, 0, 0},
+              err, sizeof(err))) {
+    std::fprintf(stderr, "bw_tree: index: %s\n", err);
+    return std::strncmp(err, "device", 6) == 0 ? 5 : 3;
   }
+  const TreeIndexPlan& P = E.plan;
   std::vector<int32_t> qstate(nq);
   for (size_t q = 0; q < nq; ++q) qstate[q] = int32_t(q / size_t(spec.Q) + 1);
-  BatchPlan BP;
-  if (!build_batch_plan(P, int64_t(nq), qstate.data(), lcoord, hcoord, mode == "pass", BP, err, sizeof(err))) {
+  BatchPlan BPlan;
+  if (!build_batch_plan(P, int64_t(nq), qstate.data(), lcoord, hcoord, mode == "pass", BPlan, err, sizeof(err))) {
     std::fprintf(stderr, "bw_tree: batch plan: %s\n", err);
     return 3;
   }
   const auto t2 = clk::now();
 
-  // ---- upload (inputs + plans), allocate scratch
-  HIPX(hipSetDevice(0));
-  HIPX(hipFree(nullptr));
+  // ---- upload the batch inputs (the index arrays went up in init)
   const auto tu0 = clk::now();
-  BwtIndex X{};
-  X.S = S; X.U = spec.U; X.nblk = P.nblk(); X.rmax = kn.rmax; X.max_cls = P.max_cls; X.nchain = P.nchain();
-  X.nchunk = P.nchunk(); X.nseg = P.nseg();
-  X.ncls_c = P.chain_ncls[0];
-  X.ncls_b = P.chain_ncls.size() > 1 ? P.chain_ncls[1] : 1;
-  X.nchunk_c = P.chain_chunk0[1];
-  X.delta = upload(w.delta);
-  X.steps = upload(P.steps);
-  X.blk_step0 = upload(P.blk_step0);
-  X.evrank = upload(P.evrank);
-  X.blk_key0 = upload(P.blk_key0);
-  X.bkey = upload(P.bkey);
-  X.blk_tile0 = upload(P.blk_tile0);
-  X.tile_rank0 = upload(P.tile_rank0);
-  X.kitems = upload(P.kitems);
-  X.chunk_key0 = upload(P.chunk_key0);
-  X.chunk_chain = upload(P.chunk_chain);
-  X.chunk_cp0 = upload(P.chunk_cp0);
-  X.chain_ncls = upload(P.chain_ncls);
-  X.chain_src = upload(P.chain_src);
-  X.chain_out = upload(P.chain_out);
-  X.chain_chunk0 = upload(P.chain_chunk0);
-  X.seg_chunk0 = upload(P.seg_chunk0);
-  X.seg_cp0 = upload(P.seg_cp0);
-  X.chain_seg0 = upload(P.chain_seg0);
-  X.agg = scratch<int64_t>(P.bkey.size());
-  X.tagg = scratch<int64_t>(P.tile_rank0.size());
-  X.cp = scratch<int64_t>(size_t(P.n_cp));
-  X.segsum = scratch<int64_t>(size_t(P.n_segsum));
-  BwtBatch Bt{};
-  Bt.M = int64_t(nq);
+  DeviceArena io;
   uint32_t *d_qlo_raw = nullptr, *d_qhi_raw = nullptr, *d_row = nullptr;
   uint16_t* d_low = nullptr;
   uint32_t* d_qlo;
   uint32_t* d_qhi;
   if (cfg == 3) {  // raw 32-bit endpoints are resolved by rank on the device, inside the pass
-    d_qlo_raw = upload(w.qlo);
-    d_qhi_raw = upload(w.qhi);
-    d_row = upload(lut.row_ptr);
-    d_low = upload(lut.low16);
-    d_qlo = scratch<uint32_t>(nq);
-    d_qhi = scratch<uint32_t>(nq);
+    d_qlo_raw = io.upload(w.qlo);
+    d_qhi_raw = io.upload(w.qhi);
+    d_row = io.upload(lut.row_ptr);
+    d_low = io.upload(lut.low16);
+    d_qlo = io.scratch<uint32_t>(nq);
+    d_qhi = io.scratch<uint32_t>(nq);
   } else {
-    d_qlo = upload(w.qlo);
-    d_qhi = upload(w.qhi);
+    d_qlo = io.upload(w.qlo);
+    d_qhi = io.upload(w.qhi);
   }
-  Bt.qlo = d_qlo; Bt.qhi = d_qhi;
-  Bt.nqblk = int32_t(BP.qblk.size());
-  Bt.qblk = upload(BP.qblk);
-  Bt.qblk_q0 = upload(BP.qblk_q0);
-  Bt.qorder = upload(BP.qorder);
-  Bt.qstep = upload(BP.qstep);
-  Bt.qrank = upload(BP.qrank);
-  Bt.nwork = int64_t(BP.wk_chunk.size());
-  Bt.nwork_c = int64_t(std::lower_bound(BP.wk_chunk.begin(), BP.wk_chunk.end(), P.chain_chunk0[1]) - BP.wk_chunk.begin());
-  Bt.wk_chunk = upload(BP.wk_chunk);
-  Bt.wk_e0 = upload(BP.wk_e0);
-  Bt.eitems = upload(BP.eitems);
-  Bt.ansr = scratch<int64_t>(nq);
-  Bt.g = scratch<int64_t>(4 * nq);
-  Bt.answers = scratch<int64_t>(nq);
+  DeviceBatch DB;
+  if (!io.ok() || !upload_batch(E, std::move(BPlan), d_qlo, d_qhi, DB, err, sizeof(err))) {
+    std::fprintf(stderr, "bw_tree: upload failed\n");
+    return 5;
+  }
+  const BatchPlan& BP = DB.plan;
+  const BwtBatch& Bt = DB.b;
+  BwtIndex& X = E.x;
+  const size_t g_dev_bytes = E.arena.bytes + io.bytes + DB.arena.bytes;
   HIPX(hipDeviceSynchronize());
   const auto tu1 = clk::now();
 
@@ -256,28 +205,12 @@ int main(int argc, char** argv) {
     rk[0] = BwtRankArgs{int64_t(nq), d_qlo_raw, d_qlo, d_row, d_low};
     rk[1] = BwtRankArgs{int64_t(nq), d_qhi_raw, d_qhi, d_row, d_low};
   }
-  // index build (query mode): block replay with weights over every block, no queries
-  std::vector<int32_t> allb(static_cast<size_t>(P.nblk()));
-  for (int32_t b = 0; b < P.nblk(); ++b) allb[size_t(b)] = b;
-  std::vector<int64_t> zq0(size_t(P.nblk()) + 1, 0);
-  BwtBatch Bidx = Bt;
-  Bidx.nqblk = P.nblk();
-  Bidx.qblk = upload(allb);
-  Bidx.qblk_q0 = upload(zq0);
-  Bidx.nwork = 0;
-  Bidx.nwork_c = 0;
   double index_ms = 0;
   // stage events: remap | block | hist | scan | replay | combine
   const int NST = 6;
   hipEvent_t ev[NST + 1];
   for (auto& e : ev) HIPX(hipEventCreate(&e));
-  auto build_index = [&]() {
-    BwtLaunch wl = L;
-    wl.write_index = 1;
-    RUN(bwt_block_replay_launch(&X, &Bidx, &wl, stream));
-    RUN(bwt_chunk_hist_launch(&X, &wl, stream));
-    RUN(bwt_carry_scan_launch(&X, &wl, stream));
-  };
+  auto build_index = [&]() { RUN(E.build_index(stream)); };
   if (mode == "query") {
     build_index();  // warm
     HIPX(hipEventRecord(ev[0], stream));
@@ -354,18 +287,15 @@ int main(int argc, char** argv) {
   // (2 x u32 endpoint + i64 answer) once
   const double bytes_alg = double(nu) * 12.0 + double(nq) * 16.0;
   // algorithmic bytes of each stage (what the stage must read and write once):
-  //  block replay: per walked event u16 rank + i64 delta; per step its id; per query 2 u32
-  //    endpoints + i64 result + u32 order + i32 step; with the index write the block's keys
-  //    (u32 in, i64 weight out) and tiles (u16 in, i64 out)
-  //  chunk histogram: every key item (u64) + its i64 weight, the dense i64 rows out
+  //  block replay: per walked event u16 rank + i64 delta; per step its id; per query its u32
+  //    packed ranks + i64 result; with the index write the block's keys (u32 in, i64 weight
+  //    out) and tiles (u16 in, i64 out)
+  //  chunk histogram: every key item (u64) + its i64 weight, level-B weights out in chain
+  //    order, the dense i64 rows out
   //  carry scan: the rows in and out; chunk replay: walked key items + weights, every endpoint
-  //    item (u64) + its carry (i64) + the result (i64); combine: 5 x i64 in, 1 out per query
-  int64_t steps_run = 0, kit_run = 0;
-  if (mode == "pass") steps_run = int64_t(P.steps.size());
-  else
-    for (size_t i = 0; i < BP.qblk.size(); ++i)
-      if (BP.qblk_q0[i + 1] > BP.qblk_q0[i])
-        steps_run += BP.qstep[size_t(BP.qblk_q0[i + 1] - 1)] + 1 - P.blk_step0[size_t(BP.qblk[i])];
+  //    item (u64) + its carry (i64) + the result (i64); combine per query: order, block, 2
+  //    endpoints, in-block part, 2 level-B carries, 2 level-C row reads, the answer
+  int64_t steps_run = mode == "pass" ? int64_t(P.steps.size()) : BP.steps_run, kit_run = 0;
   for (size_t i = 0; i < BP.wk_chunk.size(); ++i) {
     const int64_t c = BP.wk_chunk[i];
     kit_run += std::min<int64_t>(P.chunk_key0[size_t(c) + 1] - P.chunk_key0[size_t(c)],
@@ -373,12 +303,13 @@ int main(int argc, char** argv) {
   }
   const double nkeys_d = double(P.bkey.size()), ntile_d = double(P.tile_rank0.size());
   const double sb_remap = cfg == 3 ? double(nq) * 16.0 : 0.0;
-  const double sb_block = double(steps_run) * spec.U * 10.0 + double(steps_run) * 4.0 + double(nq) * 20.0 +
+  const double sb_block = double(steps_run) * spec.U * 10.0 + double(steps_run) * 4.0 + double(nq) * 12.0 +
                           (mode == "pass" ? nkeys_d * 12.0 + ntile_d * 10.0 : 0.0);
-  const double sb_hist = mode == "pass" ? double(P.kitems.size()) * 16.0 + double(P.n_cp) * 8.0 : 0.0;
+  const double nkit_b = double(P.kitems.size()) - ntile_d;
+  const double sb_hist = mode == "pass" ? double(P.kitems.size()) * 16.0 + nkit_b * 8.0 + double(P.n_cp) * 8.0 : 0.0;
   const double sb_scan = mode == "pass" ? double(P.n_cp) * 16.0 : 0.0;
   const double sb_replay = double(kit_run) * 16.0 + double(BP.eitems.size()) * 24.0;
-  const double sb_combine = double(nq) * 48.0;
+  const double sb_combine = double(nq) * 64.0;
   char buf[8192];
   int n = std::snprintf(
       buf, sizeof(buf),
@@ -386,7 +317,7 @@ int main(int argc, char** argv) {
       ",\"NK\":%" PRIu64 ",\"seed\":%" PRIu64 ",\"n_answers\":%zu,\"answers_sha256\":\"%s\",\"answers_fn\":\"%016" PRIx64
       "\",\"plan\":{\"nsteps\":%zu,\"nblk\":%d,\"nkeys\":%zu,\"ntiles\":%zu,\"nkitems\":%zu,\"neitems\":%zu,"
       "\"nchunk\":%" PRId64 ",\"nwork\":%zu,\"nchain\":%d,\"nseg\":%" PRId64 ",\"max_cls\":%d,\"rmax\":%d,"
-      "\"tile_shift\":%d,\"chunk_items\":%" PRId64 ",\"chunk_items_c\":%" PRId64 ",\"ep_per_work\":%" PRId64 ",\"seg_chunks\":%d,\"threads\":%d,\"waves\":%d},"
+      "\"tile_shift\":%d,\"chunk_items\":%" PRId64 ",\"ngrp\":%zu,\"ep_per_work\":%" PRId64 ",\"seg_chunks\":%d,\"threads\":%d,\"waves\":%d},"
       "\"phases_ms\":{\"generate\":%.3f,\"setup\":%.3f,\"upload\":%.3f,\"download\":%.3f,\"extract\":%.3f,"
       "\"index_build\":%.4f},"
       "\"pass_ms\":{\"avg\":%.4f,\"min\":%.4f,\"max\":%.4f,\"repeat\":%d,\"warmup\":%d},"
@@ -397,7 +328,7 @@ int main(int argc, char** argv) {
       "\"remap_mismatch\":%" PRId64 ",\"queries_per_s\":%.1f,\"bytes_alg\":%.0f,\"device_bytes\":%zu,\"device\":\"%s\",\"cus\":%d}\n",
       mode.c_str(), cfg, S, spec.U, spec.Q, spec.N, w.coord_n(), spec.seed, nq, sha.c_str(), fn, P.steps.size(),
       P.nblk(), P.bkey.size(), P.tile_rank0.size(), P.kitems.size(), BP.eitems.size(), P.nchunk(), BP.wk_chunk.size(),
-      P.nchain(), P.nseg(), P.max_cls, kn.rmax, P.knobs.tile_shift, P.knobs.chunk_items, P.knobs.chunk_items_c, kn.ep_per_work,
+      P.nchain(), P.nseg(), P.max_cls, kn.rmax, P.knobs.tile_shift, P.knobs.chunk_items, BP.grp_step.size(), kn.ep_per_work,
       kn.seg_chunks, threads, waves,
       ms_between(t0, t1), ms_between(t1, t2), ms_between(tu0, tu1), ms_between(tx0, tx1), ms_between(tx1, tx2),
       index_ms, avg, pass_ms.front(), pass_ms.back(), repeat, warmup, stage_ms[0], stage_ms[1], stage_ms[2],

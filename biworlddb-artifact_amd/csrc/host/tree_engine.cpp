@@ -1,0 +1,136 @@
+// tree_engine.cpp -- see tree_engine.hpp.
+#include "tree_engine.hpp"
+
+#include <algorithm>
+#include <cstdio>
+
+namespace bwg {
+
+void DeviceArena::release() {
+  for (void* p : ptrs_) (void)hipFree(p);
+  ptrs_.clear();
+  bytes = 0;
+}
+
+void* DeviceArena::alloc(size_t n) {
+  if (err != hipSuccess) return nullptr;
+  void* d = nullptr;
+  const hipError_t e = hipMalloc(&d, n);
+  if (e != hipSuccess) {
+    err = e;
+    return nullptr;
+  }
+  ptrs_.push_back(d);
+  bytes += n;
+  return d;
+}
+
+void DeviceArena::copy_in(void* d, const void* s, size_t n) {
+  if (err != hipSuccess) return;
+  const hipError_t e = hipMemcpy(d, s, n, hipMemcpyHostToDevice);
+  if (e != hipSuccess) err = e;
+}
+
+bool TreeEngine::init(int64_t S, int U, uint64_t NK, const int32_t* parent, const uint32_t* key,
+                      const int64_t* delta, const PlanKnobs& knobs, const BwtLaunch& l, char* err, size_t errlen) {
+  if (!build_tree_index(S, U, NK, parent, key, knobs, plan, err, errlen)) return false;
+  launch = l;
+  const TreeIndexPlan& P = plan;
+  x = BwtIndex{};
+  x.S = S;
+  x.U = U;
+  x.nblk = P.nblk();
+  x.rmax = P.knobs.rmax;
+  x.max_cls = P.max_cls;
+  x.nchain = P.nchain();
+  x.ncls_c = P.chain_ncls[0];
+  x.ncls_b = P.chain_ncls.size() > 1 ? P.chain_ncls[1] : 1;
+  x.tile_shift = P.knobs.tile_shift;
+  x.nchunk = P.nchunk();
+  x.nchunk_c = P.chain_chunk0[1];
+  x.nseg = P.nseg();
+  x.delta = arena.upload(delta, size_t(S) * size_t(U));
+  x.steps = arena.upload(P.steps);
+  x.blk_step0 = arena.upload(P.blk_step0);
+  x.evrank = arena.upload(P.evrank);
+  x.blk_key0 = arena.upload(P.blk_key0);
+  x.bkey = arena.upload(P.bkey);
+  x.blk_tile0 = arena.upload(P.blk_tile0);
+  x.tile_rank0 = arena.upload(P.tile_rank0);
+  x.kitems = arena.upload(P.kitems);
+  x.chunk_key0 = arena.upload(P.chunk_key0);
+  x.chunk_chain = arena.upload(P.chunk_chain);
+  x.chunk_cp0 = arena.upload(P.chunk_cp0);
+  x.chain_ncls = arena.upload(P.chain_ncls);
+  x.chain_src = arena.upload(P.chain_src);
+  x.chain_out = arena.upload(P.chain_out);
+  x.chain_chunk0 = arena.upload(P.chain_chunk0);
+  x.seg_chunk0 = arena.upload(P.seg_chunk0);
+  x.seg_cp0 = arena.upload(P.seg_cp0);
+  x.chain_seg0 = arena.upload(P.chain_seg0);
+  x.agg = arena.scratch<int64_t>(P.bkey.size());
+  x.tagg = arena.scratch<int64_t>(P.tile_rank0.size());
+  x.cp = arena.scratch<int64_t>(size_t(P.n_cp));
+  x.segsum = arena.scratch<int64_t>(size_t(P.n_segsum));
+  x.kweight = arena.scratch<int64_t>(P.kitems.size());
+  // the index-build batch: every block, no queries
+  std::vector<int32_t> allb(size_t(P.nblk()));
+  for (int32_t b = 0; b < P.nblk(); ++b) allb[size_t(b)] = b;
+  std::vector<int64_t> zeros(size_t(P.nblk()) + 1, 0);
+  all_blocks = BwtBatch{};
+  all_blocks.nqblk = P.nblk();
+  all_blocks.qblk = arena.upload(allb);
+  all_blocks.qblk_g0 = arena.upload(zeros);
+  all_blocks.grp_q0 = all_blocks.qblk_g0;  // one entry (0) is read
+  if (!arena.ok()) {
+    if (err && errlen) std::snprintf(err, errlen, "device upload: %s", hipGetErrorString(arena.err));
+    return false;
+  }
+  return true;
+}
+
+int TreeEngine::build_index(hipStream_t st) {
+  BwtLaunch w = launch;
+  w.write_index = 1;
+  int rc;
+  if ((rc = bwt_block_replay_launch(&x, &all_blocks, &w, st))) return rc;
+  if ((rc = bwt_chunk_hist_launch(&x, &w, st))) return rc;
+  return bwt_carry_scan_launch(&x, &w, st);
+}
+
+bool upload_batch(const TreeEngine& e, BatchPlan&& plan, const uint32_t* d_qlo, const uint32_t* d_qhi,
+                  DeviceBatch& out, char* err, size_t errlen) {
+  out.arena.release();
+  out.arena.err = hipSuccess;
+  out.plan = std::move(plan);
+  const BatchPlan& B = out.plan;
+  BwtBatch& b = out.b;
+  b = BwtBatch{};
+  b.M = B.M;
+  b.qlo = d_qlo;
+  b.qhi = d_qhi;
+  b.nqblk = int32_t(B.qblk.size());
+  b.qblk = out.arena.upload(B.qblk);
+  b.qblk_g0 = out.arena.upload(B.qblk_g0);
+  b.ngrp = int64_t(B.grp_step.size());
+  b.grp_step = out.arena.upload(B.grp_step);
+  b.grp_q0 = out.arena.upload(B.grp_q0);
+  b.qorder = out.arena.upload(B.qorder);
+  b.qrank = out.arena.upload(B.qrank);
+  b.sblk = out.arena.upload(B.sblk);
+  b.nwork = int64_t(B.wk_chunk.size());
+  b.wk_chunk = out.arena.upload(B.wk_chunk);
+  b.wk_e0 = out.arena.upload(B.wk_e0);
+  b.eitems = out.arena.upload(B.eitems);
+  b.ansr = out.arena.scratch<int64_t>(size_t(B.M));
+  b.g = out.arena.scratch<int64_t>(2 * size_t(B.M));
+  b.answers = out.arena.scratch<int64_t>(size_t(B.M));
+  (void)e;
+  if (!out.arena.ok()) {
+    if (err && errlen) std::snprintf(err, errlen, "batch upload: %s", hipGetErrorString(out.arena.err));
+    return false;
+  }
+  return true;
+}
+
+}  // namespace bwg

@@ -38,7 +38,6 @@ bool build_tree_index(int64_t S, int U, uint64_t NK, const int32_t* parent, cons
   if (KT > 4095) return fail("key space too large for the tile classes (NK >> tile_shift must be < 4095)");
   if (kn.rmax < U || kn.rmax > 8192) return fail("rmax must be in [U, 8192]");
   if ((kn.chunk_items != 0 && (kn.chunk_items < 64 || kn.chunk_items >= (int64_t(1) << 20))) ||
-      (kn.chunk_items_c != 0 && (kn.chunk_items_c < 64 || kn.chunk_items_c >= (int64_t(1) << 20))) ||
       kn.seg_chunks < 1 || kn.ep_per_work < 64)
     return fail("bad chunking");
   for (int64_t v = 1; v <= S; ++v)
@@ -239,27 +238,28 @@ bool build_tree_index(int64_t S, int U, uint64_t NK, const int32_t* parent, cons
     }
   }
 
-  // ---- 5. chunks (at least one per chain) and carry-scan segments.  Auto chunk sizes (knob 0)
-  //          keep each level's dense carry rows near 64 MiB: the smallest power of two >= 1024
-  //          with (key items / chunk) * classes * 8 B <= 64 MiB.
+  // ---- 5. chunks (at least one per chain) and carry-scan segments.  Level C: one chunk per
+  //          block (its tile slots), so every level-C endpoint sits at a chunk start and its
+  //          carry is a plain read of the block's dense row.  Level B: chunks of chunk_items key
+  //          items; the auto size (knob 0) keeps the dense rows near 64 MiB: the smallest power
+  //          of two >= 1024 with (key items / chunk) * classes * 8 B <= 64 MiB.
   auto auto_chunk = [](int64_t items, int64_t ncls) {
     int64_t c = 1024;
     while (c < (int64_t(1) << 19) && (items / c) * ncls * 8 > (int64_t(64) << 20)) c <<= 1;
     return c;
   };
-  if (P.knobs.chunk_items_c == 0) P.knobs.chunk_items_c = auto_chunk(nC, int64_t(KT));
   if (P.knobs.chunk_items == 0) P.knobs.chunk_items = auto_chunk(P.chain_key0[KT + 1] - nC, int64_t(W));
   P.chain_chunk0.push_back(0);
   P.chain_seg0.push_back(0);
   P.seg_chunk0.push_back(0);
   int64_t cp = 0, sg = 0;
   for (int32_t ch = 0; ch < P.nchain(); ++ch) {
-    const int64_t CH = ch == 0 ? P.knobs.chunk_items_c : P.knobs.chunk_items;
+    const int64_t CH = P.knobs.chunk_items;
     const int64_t a = P.chain_key0[size_t(ch)], e = P.chain_key0[size_t(ch) + 1];
     const int ncls = P.chain_ncls[size_t(ch)];
-    const int64_t nck = std::max<int64_t>(1, (e - a + CH - 1) / CH);
+    const int64_t nck = ch == 0 ? int64_t(nblk) : std::max<int64_t>(1, (e - a + CH - 1) / CH);
     for (int64_t k = 0; k < nck; ++k) {
-      P.chunk_key0.push_back(a + k * CH);
+      P.chunk_key0.push_back(ch == 0 ? P.blk_tile0[size_t(k)] : a + k * CH);
       P.chunk_chain.push_back(ch);
       P.chunk_cp0.push_back(cp);
       cp += ncls;
@@ -295,62 +295,68 @@ bool build_batch_plan(const TreeIndexPlan& X, int64_t M, const int32_t* qstate, 
   const int ts = X.knobs.tile_shift;
   const uint32_t mask = (1u << ts) - 1;
   const int nth = std::max(1, omp_get_max_threads());
-  // ---- queries ordered by enter step (stable), hence grouped by block
+  // ---- slots: queries ordered by enter step (stable), hence grouped by block
   std::vector<int64_t> scnt(size_t(nsteps) + 1, 0);
   for (int64_t q = 0; q < M; ++q) scnt[size_t(X.state_step[size_t(qstate[q])]) + 1]++;
   for (int64_t s = 0; s < nsteps; ++s) scnt[size_t(s) + 1] += scnt[size_t(s)];
   B.qorder.resize(size_t(M));
-  B.qstep.resize(size_t(M));
   {
     std::vector<int64_t> f(scnt.begin(), scnt.end() - 1);
-    for (int64_t q = 0; q < M; ++q) {
-      const int32_t s = X.state_step[size_t(qstate[q])];
-      const int64_t at = f[size_t(s)]++;
-      B.qorder[size_t(at)] = uint32_t(q);
-      B.qstep[size_t(at)] = s;
-    }
+    for (int64_t q = 0; q < M; ++q) B.qorder[size_t(f[size_t(X.state_step[size_t(qstate[q])])]++)] = uint32_t(q);
   }
-  // per listed block: [qblk_q0[i], qblk_q0[i+1]) in qorder (listed blocks are ascending, and
-  // the queries of unlisted blocks are none, so consecutive ranges stay contiguous)
-  B.qblk_q0.push_back(0);
+  // ---- listed blocks and their step groups (the queried steps, ascending)
+  B.qblk_g0.push_back(0);
+  B.grp_q0.push_back(0);
   for (int32_t b = 0; b < nblk; ++b) {
-    const int64_t e = scnt[size_t(X.blk_step0[size_t(b) + 1])];
-    if (all_blocks || e > B.qblk_q0.back()) {
-      B.qblk.push_back(b);
-      B.qblk_q0.push_back(e);
-    }
+    const int32_t sa = X.blk_step0[size_t(b)], sb = X.blk_step0[size_t(b) + 1];
+    if (!all_blocks && scnt[size_t(sb)] == scnt[size_t(sa)]) continue;
+    B.qblk.push_back(b);
+    int32_t last = sa - 1;
+    for (int32_t s = sa; s < sb; ++s)
+      if (scnt[size_t(s) + 1] > scnt[size_t(s)]) {
+        B.grp_step.push_back(s);
+        B.grp_q0.push_back(scnt[size_t(s) + 1]);
+        last = s;
+      }
+    B.qblk_g0.push_back(int64_t(B.grp_step.size()));
+    B.steps_run += last + 1 - sa;
   }
-  // ---- endpoint items: (chunk, item) on level C and level B for every endpoint
-  const int64_t nE = 4 * M;
+  // ---- per slot: block and in-block ranks of its endpoints
+  B.qrank.resize(size_t(M));
+  B.sblk.resize(size_t(M));
+#pragma omp parallel for schedule(static)
+  for (int64_t j = 0; j < M; ++j) {
+    const uint32_t q = B.qorder[size_t(j)];
+    const int32_t b = X.state_block[size_t(qstate[q])];
+    const uint32_t* k0 = X.bkey.data() + X.blk_key0[size_t(b)];
+    const uint32_t* k1 = X.bkey.data() + X.blk_key0[size_t(b) + 1];
+    const uint32_t rl = uint32_t(std::lower_bound(k0, k1, qlo[q]) - k0);
+    const uint32_t rh = uint32_t(std::lower_bound(k0, k1, qhi[q]) - k0);
+    B.qrank[size_t(j)] = rl | (rh << 16);
+    B.sblk[size_t(j)] = b;
+  }
+  // ---- level-B endpoint items: (chunk, item) for every endpoint
+  const int64_t nE = 2 * M;
   std::vector<int64_t> ech(static_cast<size_t>(nE));
   std::vector<uint64_t> eit(static_cast<size_t>(nE));
-  auto chunk_of = [&](int32_t chain, int64_t pos, int64_t& local) {
-    const int64_t CH = chain == 0 ? X.knobs.chunk_items_c : X.knobs.chunk_items;
-    const int64_t n = X.chain_chunk0[size_t(chain) + 1] - X.chain_chunk0[size_t(chain)];
-    int64_t k = pos / CH;
-    if (k >= n) k = n - 1;
-    local = pos - k * CH;
-    return X.chain_chunk0[size_t(chain)] + k;
-  };
+  const int64_t CH = X.knobs.chunk_items;
 #pragma omp parallel for schedule(static)
-  for (int64_t q = 0; q < M; ++q) {
-    const int32_t b = X.state_block[size_t(qstate[q])];
+  for (int64_t j = 0; j < M; ++j) {
+    const uint32_t q = B.qorder[size_t(j)];
+    const int32_t b = B.sblk[size_t(j)];
     for (int side = 0; side < 2; ++side) {
       const uint32_t x = side ? qhi[q] : qlo[q];
-      const uint64_t ep = uint64_t(2 * q + side);
-      int64_t local;
-      // level C: the tile slots of blocks < b precede the endpoint
-      const int64_t cc = chunk_of(0, X.blk_tile0[size_t(b)], local);
-      ech[size_t(4 * q + 2 * side)] = cc;
-      eit[size_t(4 * q + 2 * side)] = ep_item(uint64_t(local), x >> ts, ep);
-      // level B: the keys of tile t in blocks < b precede it
+      // the keys of tile t in blocks < b precede the endpoint on chain 1 + t
       const uint32_t t = x >> ts;
       const int64_t lo = X.tile_tb0[t], hi = X.tile_tb0[size_t(t) + 1];
       const int64_t idx = std::lower_bound(X.tb_block.begin() + lo, X.tb_block.begin() + hi, b) - X.tb_block.begin();
       const int64_t pos = idx < hi ? X.tb_pos[size_t(idx)] : X.chain_key0[size_t(t) + 2] - X.chain_key0[size_t(t) + 1];
-      const int64_t cb = chunk_of(1 + int32_t(t), pos, local);
-      ech[size_t(4 * q + 2 * side + 1)] = cb;
-      eit[size_t(4 * q + 2 * side + 1)] = ep_item(uint64_t(local), x & mask, ep);
+      const int32_t chain = 1 + int32_t(t);
+      const int64_t n = X.chain_chunk0[size_t(chain) + 1] - X.chain_chunk0[size_t(chain)];
+      int64_t k = pos / CH;
+      if (k >= n) k = n - 1;
+      ech[size_t(2 * j + side)] = X.chain_chunk0[size_t(chain)] + k;
+      eit[size_t(2 * j + side)] = ep_item(uint64_t(pos - k * CH), x & mask, uint64_t(2 * j + side));
     }
   }
   // ---- bucket by chunk (stable counting sort), then order each chunk by position
@@ -401,18 +407,6 @@ bool build_batch_plan(const TreeIndexPlan& X, int64_t M, const int32_t* qstate, 
       B.wk_chunk.push_back(c);
       B.wk_e0.push_back(std::min(a + G, cstart[size_t(c) + 1]));
     }
-  // in-block ranks of every query's endpoints (aligned with qorder)
-  B.qrank.resize(size_t(M));
-#pragma omp parallel for schedule(static)
-  for (int64_t j = 0; j < M; ++j) {
-    const uint32_t q = B.qorder[size_t(j)];
-    const int32_t b = X.state_block[size_t(qstate[q])];
-    const uint32_t* k0 = X.bkey.data() + X.blk_key0[size_t(b)];
-    const uint32_t* k1 = X.bkey.data() + X.blk_key0[size_t(b) + 1];
-    const uint32_t rl = uint32_t(std::lower_bound(k0, k1, qlo[q]) - k0);
-    const uint32_t rh = uint32_t(std::lower_bound(k0, k1, qhi[q]) - k0);
-    B.qrank[size_t(j)] = rl | (rh << 16);
-  }
   return true;
 }
 

@@ -4,8 +4,8 @@
 //   TreeIndexPlan  static, update-side: Euler linearisation, blocks sized to the LDS Fenwick
 //                  tree, per-block distinct keys / key tiles / in-block event ranks, and the two
 //                  carry levels' chains, chunks and scan segments.
-//   BatchPlan      per query batch: queries grouped by block and step, endpoint items grouped
-//                  by chunk and position.
+//   BatchPlan      per query batch: queries in slots grouped by block and step (step groups),
+//                  level-B endpoint items grouped by chunk and position.
 #pragma once
 #include <cstddef>
 #include <cstdint>
@@ -17,8 +17,8 @@ struct PlanKnobs {
   int rmax = 2048;              // max distinct keys per block (LDS Fenwick size of the block replay)
   int tile_shift = 0;           // key tile = key >> tile_shift (level-B classes = 1 << tile_shift);
                                 // 0 = balanced: ceil(log2(NK+1) / 2), so both levels' trees are small
-  int64_t chunk_items = 0;      // key items per carry chunk on level B (0 = auto)
-  int64_t chunk_items_c = 0;    // key items per carry chunk on level C (0 = auto)
+  int64_t chunk_items = 0;      // key items per carry chunk on level B (0 = auto); level C has
+                                // one chunk per block
   int64_t ep_per_work = 2048;   // endpoints per chunk work item (batch plans)
   int seg_chunks = 32;          // chunks per carry-scan segment
   int max_block_steps = 1 << 30;
@@ -55,13 +55,16 @@ struct TreeIndexPlan {
 
 struct BatchPlan {
   int64_t M = 0;
-  std::vector<int32_t> qblk;
-  std::vector<int64_t> qblk_q0;
-  std::vector<uint32_t> qorder;
-  std::vector<int32_t> qstep;
-  std::vector<uint32_t> qrank;
+  std::vector<int32_t> qblk;      // listed blocks
+  std::vector<int64_t> qblk_g0;   // their step groups
+  std::vector<int32_t> grp_step;  // step of each group
+  std::vector<int64_t> grp_q0;    // its slots
+  std::vector<uint32_t> qorder;   // slot -> query id
+  std::vector<uint32_t> qrank;    // slot -> in-block ranks lo | hi << 16
+  std::vector<int32_t> sblk;      // slot -> block
   std::vector<int64_t> wk_chunk, wk_e0;
-  std::vector<uint64_t> eitems;
+  std::vector<uint64_t> eitems;   // level B only, endpoint id = 2 * slot + side
+  int64_t steps_run = 0;          // block-replay steps the batch needs (statistics)
 };
 
 // parent: S+1 entries (parent[0] = -1, parent[v] < v); key: S*U coordinates in [0, NK).

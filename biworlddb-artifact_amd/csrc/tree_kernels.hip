@@ -3,16 +3,22 @@
 // Stage map onto the reference's v7 algorithm (src/batch_engine_v7.hpp:832-968):
 //   bwt_block_kernel        <- v7_block_kernel / run_blocks phase 1 (src/gpu_v7.cu:10-82): one
 //                              workgroup replays one block from a zero local prefix over the
-//                              block's own distinct keys (so the Fenwick tree always fits LDS),
-//                              undo-aware (Euler steps); in-block prefixes at query endpoints;
-//                              optionally the block's net weights (the dense dV, compacted)
-//   bwt_chunk_hist_kernel   <- dV per carry chunk
+//                              block's own distinct keys, undo-aware (Euler steps).  Instead of
+//                              a Fenwick tree (whose upper nodes serialise LDS atomics) every
+//                              event is one ds_add_u64 into a dense per-rank delta array; at a
+//                              query step a block scan folds the deltas into per-thread rank
+//                              totals (registers) and publishes their exclusive prefix P, which
+//                              the step's endpoints read once each.  Optionally the block's net
+//                              weights (the dense dV, compacted)
+//   bwt_chunk_hist_kernel   <- dV per carry chunk (level C: one chunk per block)
 //   bwt_seg_*_kernel        <- the serial carry pass (src/batch_engine_v7.hpp:936-945) as a
 //                              three-pass segmented column scan over chunk rows
-//   bwt_chunk_replay_kernel <- lvals on the carry levels (src/gpu_v7.cu:72-78): one wavefront
-//                              per chunk merges its static key items with the batch's endpoint
-//                              items, LDS Fenwick tree, in-window corrections by readlane
-//   bwt_combine_kernel      <- gpu_combine_kernel (src/gpu_v7.cu:84-96)
+//   bwt_chunk_replay_kernel <- lvals on carry level B (src/gpu_v7.cu:72-78): one wavefront per
+//                              work item; the chunk's key items and the batch's endpoint items
+//                              are consumed in merged groups of 64; a Fenwick tree seeded from
+//                              the chunk's carry row; pairs inside a group by LDS broadcast
+//   bwt_combine_kernel      <- gpu_combine_kernel (src/gpu_v7.cu:84-96), plus the level-C carry
+//                              read straight from the block's dense carry row
 //   bwt_rank_kernel         <- CsrBase::rank (src/sparse_engine.hpp:62-97), CSR form
 // All arithmetic is int64 adds (LDS ds_add_u64, DPP scans): results are exact and independent
 // of scheduling order.
@@ -28,100 +34,225 @@ using namespace bw;
 
 namespace {
 
-__device__ __forceinline__ int64_t fen_prefix(const int64_t* fen, int y) {
-  int64_t r = 0;
-  for (int j = y; j > 0; j -= j & -j) r += fen[j];
-  return r;
+constexpr int kQCap = 1024;  // query slots staged in LDS by the block replay
+constexpr int kGCap = 64;    // query step groups staged in LDS by the block replay
+constexpr int kEPT = 4;      // block replay: events per thread per staged tile of steps
+
+// Padded LDS slot of in-block rank r when each thread owns RPT consecutive ranks: one spare
+// word per run keeps a wavefront's strided run accesses on distinct banks.
+template <int RPT>
+__device__ __forceinline__ int pslot(int r) {
+  return RPT > 1 ? r + r / RPT : r;
+}
+
+template <int RPT>
+constexpr int block_span(int nt) {
+  return (RPT > 1 ? nt * RPT + nt : nt * RPT) + 1;
 }
 
 // ---------------------------------------------------------------- block replay
 // Events of step s: the U updates of state |steps[s]| with their precomputed in-block ranks.
-// The next step's (rank, delta) pair is loaded before the current one is applied, so the global
-// latency overlaps the LDS Fenwick updates; workgroup barriers only surround query evaluation.
+// Steps are replayed in tiles of T = kEPT*nt/U steps whose events (rank, signed delta) are staged
+// in LDS: while tile k replays, every thread holds its kEPT events of tile k+1 in flight (loaded
+// with step ids fetched one tile earlier, every load unconditional with clamped indices) and the
+// step ids of tile k+2; between tiles the arrived events go to the stage.  Query ranks and query
+// step groups are staged in LDS too, so a query step touches no global memory before its answers
+// are stored.
+template <int RPT>
 __global__ void __launch_bounds__(1024) bwt_block_kernel(const BwtIndex x, const BwtBatch q, const BwtLaunch l) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  int64_t* fen = reinterpret_cast<int64_t*>(smem_raw);  // rmax+1, 1-based
+  const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wv = tid >> 6;
+  const int span = block_span<RPT>(nt);
+  int64_t* w = reinterpret_cast<int64_t*>(smem_raw);          // deltas since the last scan
+  int64_t* P = w + span;                                       // exclusive prefix over ranks
+  int64_t* wtot = P + span;                                    // 16 wavefront totals
+  int64_t* gq0 = wtot + 16;                                    // kGCap + 1 slot offsets
+  int32_t* gstep = reinterpret_cast<int32_t*>(gq0 + kGCap + 1);  // kGCap steps
+  uint32_t* qbuf = reinterpret_cast<uint32_t*>(gstep + kGCap);   // kQCap packed ranks
+  int64_t* sdel = reinterpret_cast<int64_t*>(qbuf + kQCap);      // kEPT*nt staged signed deltas
+  uint16_t* srank = reinterpret_cast<uint16_t*>(sdel + kEPT * nt);  // kEPT*nt staged ranks
   const int b = q.qblk[blockIdx.x];
-  const int tid = threadIdx.x, nt = blockDim.x;
   const int64_t k0 = x.blk_key0[b];
   const int R = (int)(x.blk_key0[b + 1] - k0);
-  const int64_t qa = q.qblk_q0[blockIdx.x], qe = q.qblk_q0[blockIdx.x + 1];
-  for (int i = tid; i <= R; i += nt) fen[i] = 0;
-  __syncthreads();
   const int U = x.U;
   const int s0 = x.blk_step0[b];
+  const int64_t gi0 = q.qblk_g0[blockIdx.x], ge = q.qblk_g0[blockIdx.x + 1];
   // without the index write, nothing after the last queried step matters
-  const int s1 = l.write_index ? x.blk_step0[b + 1] : (qa < qe ? q.qstep[qe - 1] + 1 : s0);
-  int64_t qp = qa;
-  int next_q = (qp < qe) ? q.qstep[qp] : -1;
-  // one event per thread (nt >= U): the next step's (rank, signed delta) is loaded before the
-  // current one enters the Fenwick tree; otherwise a strided loop
-  const bool one = nt >= U;
-  auto load_event = [&](int s, int i, int& r, int64_t& d) {
-    const int st = x.steps[s];
-    const int64_t v = st > 0 ? st : -st;
-    r = x.evrank[(int64_t)s * U + i];
-    const int64_t dv = x.delta[(v - 1) * (int64_t)U + i];
-    d = st > 0 ? dv : -dv;
-  };
-  int r_cur = 0;
-  int64_t d_cur = 0;
-  if (one && tid < U && s0 < s1) load_event(s0, tid, r_cur, d_cur);
-  for (int s = s0; s < s1; ++s) {
-    if (one) {
-      int r_nx = 0;
-      int64_t d_nx = 0;
-      if (tid < U && s + 1 < s1) load_event(s + 1, tid, r_nx, d_nx);
-      if (tid < U)
-        for (int f = r_cur + 1; f <= R; f += f & -f) lds_add64(&fen[f], d_cur);
-      r_cur = r_nx;
-      d_cur = d_nx;
-    } else {
-      for (int i = tid; i < U; i += nt) {
-        int r;
-        int64_t d;
-        load_event(s, i, r, d);
-        for (int f = r + 1; f <= R; f += f & -f) lds_add64(&fen[f], d);
-      }
+  const int s1 = l.write_index ? x.blk_step0[b + 1] : (gi0 < ge ? q.grp_step[ge - 1] + 1 : s0);
+  const int64_t qend = q.grp_q0[ge];
+  auto stage_groups = [&](int64_t g) {
+    for (int i = tid; i <= kGCap; i += nt) {
+      const int64_t gg = g + i;
+      if (i < kGCap) gstep[i] = gg < ge ? q.grp_step[gg] : -1;
+      gq0[i] = gg <= ge ? q.grp_q0[gg] : qend;
     }
-    if (s == next_q) {  // queries of the state entered at step s (uniform branch)
-      int64_t e = qp;
-      while (e < qe && q.qstep[e] == s) ++e;
+  };
+  for (int i = tid; i < span; i += nt) w[i] = 0;
+  stage_groups(gi0);
+  const int64_t qbeg = q.grp_q0[gi0];
+  for (int i = tid; i < kQCap; i += nt) qbuf[i] = (qbeg + i < qend) ? q.qrank[qbeg + i] : 0u;
+  int64_t tot[RPT];
+#pragma unroll
+  for (int j = 0; j < RPT; ++j) tot[j] = 0;
+  int64_t gi = gi0, gwin = gi0, qwin = qbeg;
+  __syncthreads();
+  int next_q = gi < ge ? gstep[0] : -1;
+
+  // w -> rank totals; P[y] = sum of totals of ranks < y.  Two workgroup barriers.
+  auto rebuild = [&]() {
+    const int base = tid * RPT;
+    int64_t loc[RPT];
+    int64_t run = 0;
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) {
+      const int pi = pslot<RPT>(base + j);
+      tot[j] += w[pi];
+      w[pi] = 0;
+      loc[j] = run;
+      run += tot[j];
+    }
+    const int64_t inc = wave_incl_scan(run);
+    if (lane == 63) wtot[wv] = inc;
+    __syncthreads();
+    int64_t off = inc - run;
+    for (int k = 0; k < wv; ++k) off += wtot[k];
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) P[pslot<RPT>(base + j)] = off + loc[j];
+    if (tid == nt - 1) P[pslot<RPT>(nt * RPT)] = off + run;
+    __syncthreads();
+  };
+
+  // answers of the query group gi (the state entered at the current step)
+  auto query = [&]() {
+    __syncthreads();  // every event of the steps so far is in w; previous readers are done
+    const int64_t qa = gq0[gi - gwin], qb = gq0[gi - gwin + 1];
+    const int64_t nq = qb - qa;
+    const bool staged = nq <= kQCap;
+    if (staged && qb > qwin + kQCap) {  // uniform
+      qwin = qa;
+      for (int i = tid; i < kQCap; i += nt) qbuf[i] = (qa + i < qend) ? q.qrank[qa + i] : 0u;
+    }
+    rebuild();  // its barriers also publish qbuf
+    // one endpoint per thread: even = lo, odd = hi of the same slot (nt is a multiple of 64,
+    // so a pair never straddles a wavefront)
+    const int64_t jn = (2 * nq + 63) & ~(int64_t)63;
+    for (int64_t j = tid; j < jn; j += nt) {
+      const int64_t sl = j >> 1;
+      const bool live = j < 2 * nq;
+      uint32_t rk = 0;
+      if (live) rk = staged ? qbuf[qa - qwin + sl] : q.qrank[qa + sl];
+      const int y = (j & 1) ? (int)(rk >> 16) : (int)(rk & 0xFFFFu);
+      const int64_t pv = live ? P[pslot<RPT>(y)] : 0;
+      const int64_t other = __shfl_xor(pv, 1);
+      if (live && !(j & 1)) q.ansr[qa + sl] = other - pv;
+    }
+    ++gi;
+    if (gi < ge && gi - gwin >= kGCap) {  // uniform
       __syncthreads();
-      // one endpoint per lane: even lane = lo, odd lane = hi of the same query (nt is a
-      // multiple of 64 and 2*qp is even, so a pair never straddles a wavefront)
-      for (int64_t j = 2 * qp + tid; j < ((2 * e + 63) & ~63ll); j += nt) {
-        const int64_t ent = j >> 1;
-        const bool live = j < 2 * e;
-        const uint32_t rk = live ? q.qrank[ent] : 0u;
-        const int y = (j & 1) ? (int)(rk >> 16) : (int)(rk & 0xFFFFu);
-        const int64_t pv = live ? fen_prefix(fen, y) : 0;
-        const int64_t other = __shfl_xor(pv, 1);
-        if (live && !(j & 1)) q.ansr[q.qorder[ent]] = other - pv;
+      gwin = gi;
+      stage_groups(gi);
+      __syncthreads();
+    }
+    next_q = gi < ge ? gstep[gi - gwin] : -1;
+  };
+
+  if (U <= kEPT * nt) {
+    const int T = (kEPT * nt) / U;  // steps per tile
+    const int TE = T * U;           // events per tile
+    const int64_t nu = x.S * (int64_t)U;
+    // events of the tile at ts, given its step ids: ranks and raw deltas are loads in flight,
+    // the signs are taken from the step ids at once (so the ids registers are free again)
+    auto events = [&](int ts, const int* st, int* r, int64_t* d, bool* neg) {
+#pragma unroll
+      for (int k = 0; k < kEPT; ++k) {
+        const int e = tid + k * nt;
+        const int s = ts + e / U, i = e - (e / U) * U;
+        const bool ok = e < TE && s < s1;
+        const int64_t v = st[k] > 0 ? st[k] : -st[k];
+        r[k] = x.evrank[ok ? (int64_t)s * U + i : 0];
+        const int64_t at = (v - 1) * (int64_t)U + i;
+        d[k] = x.delta[(ok && at >= 0 && at < nu) ? at : 0];
+        neg[k] = st[k] < 0;
       }
+    };
+    // step id of this thread's k-th event slot in the tile starting at step ts (clamped)
+    auto ids = [&](int ts, int* st) {
+#pragma unroll
+      for (int k = 0; k < kEPT; ++k) {
+        const int e = tid + k * nt;
+        const int s = ts + e / U;
+        st[k] = x.steps[(e < TE && s < s1) ? s : s0];
+      }
+    };
+    auto stage_events = [&](int ts, const int* r, const int64_t* d, const bool* neg) {
+#pragma unroll
+      for (int k = 0; k < kEPT; ++k) {
+        const int e = tid + k * nt;
+        if (e < TE) {
+          const bool ok = ts + e / U < s1;
+          srank[e] = (uint16_t)(ok ? r[k] : 0);
+          sdel[e] = ok ? (neg[k] ? -d[k] : d[k]) : 0;
+        }
+      }
+    };
+    int st[kEPT], r_n[kEPT];
+    int64_t d_n[kEPT];
+    bool neg_n[kEPT];
+    // prologue: tile 0 staged, tile 1's events in flight, tile 2's step ids in flight
+    ids(s0, st);
+    events(s0, st, r_n, d_n, neg_n);
+    stage_events(s0, r_n, d_n, neg_n);
+    ids(s0 + T, st);
+    events(s0 + T, st, r_n, d_n, neg_n);
+    ids(s0 + 2 * T, st);
+    __syncthreads();
+    for (int t0 = s0; t0 < s1; t0 += T) {
+      const int t1 = min(t0 + T, s1);
+      for (int s = t0; s < t1; ++s) {
+        const int base = (s - t0) * U;
+        for (int i = tid; i < U; i += nt) lds_add64(&w[pslot<RPT>(srank[base + i])], sdel[base + i]);
+        if (s == next_q) query();
+      }
+      if (t1 >= s1) break;
+      __syncthreads();  // the stage is free
+      stage_events(t0 + T, r_n, d_n, neg_n);
       __syncthreads();
-      qp = e;
-      next_q = (qp < qe) ? q.qstep[qp] : -1;
+      events(t0 + 2 * T, st, r_n, d_n, neg_n);
+      ids(t0 + 3 * T, st);
+    }
+  } else {
+    for (int s = s0; s < s1; ++s) {
+      const int st = x.steps[s];
+      const int64_t v = st > 0 ? st : -st;
+      for (int i = tid; i < U; i += nt) {
+        const int r = x.evrank[(int64_t)s * U + i];
+        const int64_t dv = x.delta[(v - 1) * (int64_t)U + i];
+        lds_add64(&w[pslot<RPT>(r)], st > 0 ? dv : -dv);
+      }
+      if (s == next_q) query();
     }
   }
   if (!l.write_index) return;
   __syncthreads();
+  rebuild();
   // net weight of the block per distinct key (carry level B) and per key tile (level C)
-  for (int r = tid; r < R; r += nt) x.agg[k0 + r] = fen_prefix(fen, r + 1) - fen_prefix(fen, r);
+  for (int r = tid; r < R; r += nt) x.agg[k0 + r] = P[pslot<RPT>(r + 1)] - P[pslot<RPT>(r)];
   const int64_t t0 = x.blk_tile0[b];
   const int T = (int)(x.blk_tile0[b + 1] - t0);
   for (int s = tid; s < T; s += nt) {
     const int r0 = x.tile_rank0[t0 + s];
     const int r1 = (s + 1 < T) ? (int)x.tile_rank0[t0 + s + 1] : R;
-    x.tagg[t0 + s] = fen_prefix(fen, r1) - fen_prefix(fen, r0);
+    x.tagg[t0 + s] = P[pslot<RPT>(r1)] - P[pslot<RPT>(r0)];
   }
 }
 
 // ---------------------------------------------------------------- chunk histogram
 // One wavefront per chunk: dense class histogram of the chunk's key items, written as its
-// exclusive prefix over classes (cp row = sum of weights with class < y).
+// exclusive prefix over classes (cp row = sum of weights with class < y).  With write_kw the
+// gathered weight of every item is also stored in chain order (kweight), so the chunk replay
+// streams weights instead of gathering them.
 __global__ void __launch_bounds__(256) bwt_chunk_hist_kernel(const BwtIndex x, const BwtLaunch l, int64_t c_begin,
-                                                             int64_t c_end, int stride) {
+                                                             int64_t c_end, int stride, int write_kw) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t c = c_begin + (int64_t)blockIdx.x * l.chunk_waves + wv;
@@ -138,7 +269,9 @@ __global__ void __launch_bounds__(256) bwt_chunk_hist_kernel(const BwtIndex x, c
   while (it < i1) {
     const int64_t nx = it + 64;
     const uint64_t nitem = (nx < i1) ? x.kitems[nx] : 0;  // prefetch
-    lds_add64(&h[(int)((item >> BWT_CLASS_SHIFT) & 0xFFF)], wsrc[item & BWT_REF_MASK]);
+    const int64_t wt = wsrc[item & BWT_REF_MASK];
+    if (write_kw) x.kweight[it] = wt;
+    lds_add64(&h[(int)((item >> BWT_CLASS_SHIFT) & 0xFFF)], wt);
     it = nx;
     item = nitem;
   }
@@ -197,90 +330,123 @@ __global__ void __launch_bounds__(256) bwt_seg_apply_kernel(const BwtIndex x) {
   }
 }
 
-// ---------------------------------------------------------------- chunk replay
-// One wavefront per listed chunk.  Key items are consumed in windows of 64 (one per lane); the
-// endpoints whose position falls before the end of the window are evaluated first:
-//   value = carry[cls] + Fenwick prefix(cls) over earlier windows
-//         + sum of this window's keys that precede the endpoint's position with class < cls
-// (keys broadcast lane by lane with readlane); then the window's keys enter the Fenwick tree.
+// ---------------------------------------------------------------- chunk replay (level B)
+// Per-wavefront LDS of the chunk replay: 64 group key weights, 64 group key classes, the Fenwick
+// tree; an even number of words keeps every wavefront's area 16-byte aligned.
+__host__ __device__ constexpr int replay_wave_words(int stride) { return (96 + stride + 1) & ~1; }
+
+// Fenwick prefix over classes < y for y < 4096 (at most 12 set bits); fen[0] stays 0, so the
+// fixed trip count needs no branch and the 12 LDS reads issue back to back.
+__device__ __forceinline__ int64_t fen_prefix12(const int64_t* fen, int y) {
+  int64_t r = 0;
+#pragma unroll
+  for (int k = 0; k < 12; ++k) {
+    r += fen[y];
+    y &= y - 1;
+  }
+  return r;
+}
+
+// One wavefront per work item (a chunk and a run of its endpoints, sorted by position).  The
+// Fenwick tree starts as the chunk's carry row (fen[i] = row[i] - row[i - lsb(i)]), so a prefix
+// read is the whole carry.  Each group takes the next 64 items of the merged order (endpoint
+// items before key items at equal position): the group's keys go to LDS, every endpoint adds
+// the Fenwick prefix plus the group's keys that precede it with a smaller class (broadcast
+// reads), then the keys enter the tree.  The next group's items are loaded while this one runs
+// (only the high word of a key item, which holds its class).
 __global__ void __launch_bounds__(256) bwt_chunk_replay_kernel(const BwtIndex x, const BwtBatch q, const BwtLaunch l,
-                                                               int64_t w_begin, int64_t w_end, int stride) {
+                                                               int stride) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int64_t ci = w_begin + (int64_t)blockIdx.x * l.chunk_waves + wv;
-  if (ci >= w_end) return;
+  const int64_t ci = (int64_t)blockIdx.x * l.chunk_waves + wv;
+  if (ci >= q.nwork) return;
   const int64_t c = q.wk_chunk[ci];
-  int64_t* fen = reinterpret_cast<int64_t*>(smem_raw) + (int64_t)wv * stride;
-  const int chain = x.chunk_chain[c];
-  const int ncls = x.chain_ncls[chain];
-  const int64_t* wsrc = x.chain_src[chain] ? x.tagg : x.agg;
-  const int64_t* carry = x.cp + x.chunk_cp0[c];
-  int64_t* gout = q.g + (int64_t)x.chain_out[chain] * (2 * q.M);
-  for (int i = lane; i <= ncls; i += 64) fen[i] = 0;
-  wave_lds_sync();
+  int64_t* kws = reinterpret_cast<int64_t*>(smem_raw) + (int64_t)wv * replay_wave_words(stride);  // 64 weights
+  int32_t* kcs = reinterpret_cast<int32_t*>(kws + 64);  // 64 group key classes
+  int64_t* fen = kws + 96;
+  const int ncls = x.chain_ncls[x.chunk_chain[c]];
+  const int64_t* row = x.cp + x.chunk_cp0[c];
+  for (int i = lane; i < stride; i += 64) {
+    int64_t v = 0;
+    if (i >= 1 && i < ncls) v = row[i] - row[i - (i & -i)];
+    fen[i] = v;
+  }
   int64_t ep = q.wk_e0[ci];
   const int64_t e1 = q.wk_e0[ci + 1];
-  // keys after this work item's last endpoint (items are sorted by position) are not needed
   const int64_t k0 = x.chunk_key0[c];
-  const int64_t k1 = min(x.chunk_key0[c + 1], k0 + (int64_t)(q.eitems[e1 - 1] >> BWT_EP_POS_SHIFT));
-  // prefetched key window
-  int64_t kit = k0 + lane;
-  uint64_t kx = (kit < k1) ? x.kitems[kit] : 0;
-  int64_t kw = (kit < k1) ? wsrc[kx & BWT_REF_MASK] : 0;
-  for (int64_t w0 = k0;; w0 += 64) {
-    const int64_t wend = (w0 + 64 < k1) ? w0 + 64 : k1;
-    const bool last = wend == k1;
-    const bool iskey = w0 + lane < k1;
-    const int kcls = (int)((kx >> BWT_CLASS_SHIFT) & 0xFFF);
-    const int64_t kwt = kw;
-    // prefetch the next window while this one is processed
-    const int64_t nkit = w0 + 64 + lane;
-    const uint64_t nkx = (nkit < k1) ? x.kitems[nkit] : 0;
-    const int64_t nkw = (nkit < k1) ? wsrc[nkx & BWT_REF_MASK] : 0;
-    const uint64_t keymask = ballot64(iskey);
-    // endpoints positioned inside this window (or anywhere, at the chunk's last window)
-    const int64_t pend = wend - k0;
-    while (ep < e1) {
-      const int64_t e = ep + lane;
-      const uint64_t it = (e < e1) ? q.eitems[e] : 0;
-      const int64_t pos = (int64_t)(it >> BWT_EP_POS_SHIFT);
-      const bool take = (e < e1) && (last || pos < pend);
-      const uint64_t tm = ballot64(take);
-      if (!tm) break;
-      const int n = __popcll(tm);  // endpoints are sorted by position: a prefix of the lanes
-      // every lane runs the loop (readlane stays in uniform control flow); lanes beyond the
-      // batch hold item 0 (class 0), so every address below stays inside the chunk's rows
-      const int cls = (int)((it >> BWT_EP_CLASS_SHIFT) & 0xFFF);
-      int64_t val = carry[cls] + fen_prefix(fen, cls);
-      const int64_t rel = pos - (w0 - k0);  // keys of this window before the endpoint
-      uint64_t km = keymask;
-      while (km) {
-        const int j = __builtin_ctzll(km);
-        km &= km - 1;
-        const int jc = __builtin_amdgcn_readlane(kcls, j);
-        const int64_t jw = readlane64(kwt, j);
-        if (j < rel && jc < cls) val += jw;
+  // keys after this work item's last endpoint are not needed
+  const int64_t kend = min(x.chunk_key0[c + 1], k0 + (int64_t)(q.eitems[e1 - 1] >> BWT_EP_POS_SHIFT));
+  const uint32_t* khi = reinterpret_cast<const uint32_t*>(x.kitems) + 1;  // little endian
+  int64_t ki = k0;
+  uint64_t it = (ep + lane < e1) ? q.eitems[ep + lane] : 0;
+  uint32_t kx = (ki + lane < kend) ? khi[2 * (ki + lane)] : 0;
+  int64_t kw = (ki + lane < kend) ? x.kweight[ki + lane] : 0;
+  wave_lds_sync();
+  while (true) {
+    const bool ev = ep + lane < e1;
+    // keys of this chunk still ahead of the endpoint (>= 0: earlier ones were consumed)
+    const int rel = ev ? (int)((int64_t)(it >> BWT_EP_POS_SHIFT) - (ki - k0)) : 0;
+    const bool take = ev && lane + rel < 64;  // its merged index falls in this group
+    const int ne = __popcll(ballot64(take));  // positions are sorted: a prefix of the lanes
+    const int nk = (int)min((int64_t)(64 - ne), kend - ki);
+    const int64_t ep2 = ep + ne, ki2 = ki + nk;
+    uint64_t it2 = 0;
+    uint32_t kx2 = 0;
+    int64_t kw2 = 0;
+    if (ep2 < e1) {  // uniform: prefetch the next group
+      if (ep2 + lane < e1) it2 = q.eitems[ep2 + lane];
+      if (ki2 + lane < kend) {
+        kx2 = khi[2 * (ki2 + lane)];
+        kw2 = x.kweight[ki2 + lane];
       }
-      if (take) gout[it & 0xFFFFFFFFull] = val;
-      ep += n;
-      if (n < 64) break;
     }
-    if (iskey)
-      for (int f = kcls + 1; f <= ncls; f += f & -f) lds_add64(&fen[f], kwt);
+    const int kcls = (int)((kx >> (BWT_CLASS_SHIFT - 32)) & 0xFFF);
+    if (lane < nk) {
+      kcs[lane] = kcls;
+      kws[lane] = kw;
+    }
     wave_lds_sync();
-    if (last) break;
-    kx = nkx;
-    kw = nkw;
+    if (ne > 0) {
+      const int cls = (int)((it >> BWT_EP_CLASS_SHIFT) & 0xFFF);
+      // a taken endpoint needs the group's first `rel` keys; rel grows with the lane
+      const int mmax = __builtin_amdgcn_readlane(rel, ne - 1);
+      const int rl = take ? rel : 0;
+      int64_t val = fen_prefix12(fen, take ? cls : 0);
+      // four keys per 16-byte LDS read (entries past nk are excluded by m < rl <= nk)
+      for (int m = 0; m < mmax; m += 4) {
+        const int4 c4 = *reinterpret_cast<const int4*>(kcs + m);
+        const longlong2 w01 = *reinterpret_cast<const longlong2*>(kws + m);
+        const longlong2 w23 = *reinterpret_cast<const longlong2*>(kws + m + 2);
+        if (m < rl && c4.x < cls) val += w01.x;
+        if (m + 1 < rl && c4.y < cls) val += w01.y;
+        if (m + 2 < rl && c4.z < cls) val += w23.x;
+        if (m + 3 < rl && c4.w < cls) val += w23.y;
+      }
+      if (take) q.g[it & 0xFFFFFFFFull] = val;
+    }
+    if (ep2 >= e1) break;
+    if (lane < nk)
+      for (int f = kcls + 1; f <= ncls; f += f & -f) lds_add64(&fen[f], kw);
+    wave_lds_sync();
+    ep = ep2;
+    ki = ki2;
+    it = it2;
+    kx = kx2;
+    kw = kw2;
   }
 }
 
 // ---------------------------------------------------------------- combine
-__global__ void __launch_bounds__(256) bwt_combine_kernel(const BwtBatch q) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= q.M) return;
-  const int64_t* gc = q.g;
-  const int64_t* gb = q.g + 2 * q.M;
-  q.answers[i] = q.ansr[i] + (gc[2 * i + 1] + gb[2 * i + 1]) - (gc[2 * i] + gb[2 * i]);
+// Slot s of the batch: in-block part + level-B carries (by slot) + level-C carry, read from the
+// dense carry row of the slot's block (level-C chunk b = block b) at the endpoints' key tiles.
+__global__ void __launch_bounds__(256) bwt_combine_kernel(const BwtIndex x, const BwtBatch q) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= q.M) return;
+  const uint32_t id = q.qorder[s];
+  const int64_t* row = x.cp + x.chunk_cp0[q.sblk[s]];
+  const int ts = x.tile_shift;
+  const int64_t gc = row[q.qhi[id] >> ts] - row[q.qlo[id] >> ts];
+  q.answers[id] = q.ansr[s] + gc + (q.g[2 * s + 1] - q.g[2 * s]);
 }
 
 // ---------------------------------------------------------------- rank remap
@@ -299,22 +465,43 @@ __global__ void __launch_bounds__(256) bwt_rank_kernel(const BwtRankArgs a) {
 }
 
 // ---------------------------------------------------------------- launch helpers
-size_t block_lds_bytes(const BwtIndex* x) { return (size_t)(x->rmax + 1) * sizeof(int64_t); }
-size_t chunk_lds_bytes(const BwtIndex* x, const BwtLaunch* l) {
-  return (size_t)l->chunk_waves * (size_t)(x->max_cls + 1) * sizeof(int64_t);
+// ranks per thread of the block replay: the smallest power of two with nt * RPT >= rmax
+int block_rpt(const BwtIndex* x, int nt) {
+  const int need = (x->rmax + nt - 1) / nt;
+  int r = 1;
+  while (r < need) r <<= 1;
+  return r;
 }
+
+size_t block_lds_bytes(int rpt, int nt) {
+  const size_t span = (size_t)(rpt > 1 ? nt * rpt + nt : nt * rpt) + 1;
+  return (2 * span + 16 + kGCap + 1) * sizeof(int64_t) + kGCap * sizeof(int32_t) + kQCap * sizeof(uint32_t) +
+         (size_t)kEPT * nt * (sizeof(int64_t) + sizeof(uint16_t));
+}
+
 // LDS stride (entries per wavefront) of a level: its classes + 1, rounded to an odd number of
-// 8-byte words so consecutive wavefronts' trees start in different banks
+// 8-byte words so consecutive wavefronts' rows start in different banks
 int level_stride(int ncls) { return (ncls + 1) | 1; }
+
+size_t replay_lds_bytes(const BwtIndex* x, const BwtLaunch* l) {
+  return (size_t)l->chunk_waves * (size_t)replay_wave_words(level_stride(x->ncls_b)) * sizeof(int64_t);
+}
+
+size_t hist_lds_bytes(const BwtIndex* x, const BwtLaunch* l) {
+  return (size_t)l->chunk_waves * (size_t)level_stride(x->max_cls) * sizeof(int64_t);
+}
 
 int check(const BwtIndex* x, const BwtLaunch* l) {
   if (!x || !l || x->S < 1 || x->U < 1 || x->nblk < 1) return (int)hipErrorInvalidValue;
   if (l->block_threads < 64 || l->block_threads > 1024 || (l->block_threads & 63)) return (int)hipErrorInvalidValue;
   if (x->rmax < 1 || x->rmax > 8192 || x->max_cls < 1 || x->max_cls > 4096) return (int)hipErrorInvalidValue;
   if (x->ncls_c < 1 || x->ncls_c > x->max_cls || x->ncls_b < 1 || x->ncls_b > x->max_cls) return (int)hipErrorInvalidValue;
-  if (x->nchunk_c < 0 || x->nchunk_c > x->nchunk) return (int)hipErrorInvalidValue;
+  if (x->nchunk_c != x->nblk || x->nchunk_c > x->nchunk) return (int)hipErrorInvalidValue;
+  if (x->tile_shift < 1 || x->tile_shift > 12) return (int)hipErrorInvalidValue;
   if (l->chunk_waves < 1 || l->chunk_waves > 4) return (int)hipErrorInvalidValue;
-  if (block_lds_bytes(x) > 160 * 1024 || chunk_lds_bytes(x, l) > 160 * 1024) return (int)hipErrorInvalidValue;
+  const int rpt = block_rpt(x, l->block_threads);
+  if (rpt > 16 || block_lds_bytes(rpt, l->block_threads) > 160 * 1024) return (int)hipErrorInvalidValue;
+  if (replay_lds_bytes(x, l) > 160 * 1024 || hist_lds_bytes(x, l) > 160 * 1024) return (int)hipErrorInvalidValue;
   return 0;
 }
 
@@ -323,7 +510,11 @@ int set_lds_limits() {
   static hipError_t err = hipSuccess;
   std::call_once(once, [] {
     const int lim = 160 * 1024;
-    const void* ks[] = {reinterpret_cast<const void*>(bwt_block_kernel),
+    const void* ks[] = {reinterpret_cast<const void*>(bwt_block_kernel<1>),
+                        reinterpret_cast<const void*>(bwt_block_kernel<2>),
+                        reinterpret_cast<const void*>(bwt_block_kernel<4>),
+                        reinterpret_cast<const void*>(bwt_block_kernel<8>),
+                        reinterpret_cast<const void*>(bwt_block_kernel<16>),
                         reinterpret_cast<const void*>(bwt_chunk_hist_kernel),
                         reinterpret_cast<const void*>(bwt_chunk_replay_kernel)};
     for (const void* k : ks) {
@@ -339,6 +530,12 @@ int report(const char* what, hipError_t e) {
   return (int)e;
 }
 
+template <int RPT>
+void launch_block(const BwtIndex* x, const BwtBatch* b, const BwtLaunch* l, hipStream_t st) {
+  hipLaunchKernelGGL(bwt_block_kernel<RPT>, dim3(b->nqblk), dim3(l->block_threads),
+                     block_lds_bytes(RPT, l->block_threads), st, *x, *b, *l);
+}
+
 }  // namespace
 
 extern "C" int bwt_block_replay_launch(const BwtIndex* x, const BwtBatch* b, const BwtLaunch* l, void* stream) {
@@ -347,8 +544,15 @@ extern "C" int bwt_block_replay_launch(const BwtIndex* x, const BwtBatch* b, con
   if (!b || b->nqblk < 0 || (l->write_index && b->nqblk != x->nblk)) return (int)hipErrorInvalidValue;
   if ((rc = set_lds_limits())) return rc;
   if (b->nqblk == 0) return 0;
-  hipLaunchKernelGGL(bwt_block_kernel, dim3(b->nqblk), dim3(l->block_threads), block_lds_bytes(x),
-                     (hipStream_t)stream, *x, *b, *l);
+  const hipStream_t st = (hipStream_t)stream;
+  switch (block_rpt(x, l->block_threads)) {
+    case 1: launch_block<1>(x, b, l, st); break;
+    case 2: launch_block<2>(x, b, l, st); break;
+    case 4: launch_block<4>(x, b, l, st); break;
+    case 8: launch_block<8>(x, b, l, st); break;
+    case 16: launch_block<16>(x, b, l, st); break;
+    default: return (int)hipErrorInvalidValue;
+  }
   return report("block replay", hipGetLastError());
 }
 
@@ -356,7 +560,8 @@ extern "C" int bwt_chunk_hist_launch(const BwtIndex* x, const BwtLaunch* l, void
   int rc = check(x, l);
   if (rc) return rc;
   if ((rc = set_lds_limits())) return rc;
-  // level C chunks, then level B chunks, each with LDS sized to its own class count
+  // level C chunks (one per block), then level B chunks (which also store their item weights),
+  // each with LDS sized to its own class count
   const int64_t rng[2][2] = {{0, x->nchunk_c}, {x->nchunk_c, x->nchunk}};
   const int ncls[2] = {x->ncls_c, x->ncls_b};
   for (int lv = 0; lv < 2; ++lv) {
@@ -366,7 +571,7 @@ extern "C" int bwt_chunk_hist_launch(const BwtIndex* x, const BwtLaunch* l, void
     const int64_t grid = (n + l->chunk_waves - 1) / l->chunk_waves;
     hipLaunchKernelGGL(bwt_chunk_hist_kernel, dim3((unsigned)grid), dim3(64 * l->chunk_waves),
                        (size_t)l->chunk_waves * stride * sizeof(int64_t), (hipStream_t)stream, *x, *l, rng[lv][0],
-                       rng[lv][1], stride);
+                       rng[lv][1], stride, lv);
     if ((rc = report("chunk histogram", hipGetLastError()))) return rc;
   }
   return 0;
@@ -390,28 +595,20 @@ extern "C" int bwt_chunk_replay_launch(const BwtIndex* x, const BwtBatch* b, con
   if (rc) return rc;
   if (!b || b->nwork < 0) return (int)hipErrorInvalidValue;
   if ((rc = set_lds_limits())) return rc;
-  if (b->nwork_c < 0 || b->nwork_c > b->nwork) return (int)hipErrorInvalidValue;
-  const int64_t rng[2][2] = {{0, b->nwork_c}, {b->nwork_c, b->nwork}};
-  const int ncls[2] = {x->ncls_c, x->ncls_b};
-  for (int lv = 0; lv < 2; ++lv) {
-    const int64_t n = rng[lv][1] - rng[lv][0];
-    if (n <= 0) continue;
-    const int stride = level_stride(ncls[lv]);
-    const int64_t grid = (n + l->chunk_waves - 1) / l->chunk_waves;
-    hipLaunchKernelGGL(bwt_chunk_replay_kernel, dim3((unsigned)grid), dim3(64 * l->chunk_waves),
-                       (size_t)l->chunk_waves * stride * sizeof(int64_t), (hipStream_t)stream, *x, *b, *l, rng[lv][0],
-                       rng[lv][1], stride);
-    if ((rc = report("chunk replay", hipGetLastError()))) return rc;
-  }
-  return 0;
+  if (b->nwork == 0) return 0;
+  const int stride = level_stride(x->ncls_b);
+  const int64_t grid = (b->nwork + l->chunk_waves - 1) / l->chunk_waves;
+  hipLaunchKernelGGL(bwt_chunk_replay_kernel, dim3((unsigned)grid), dim3(64 * l->chunk_waves),
+                     replay_lds_bytes(x, l), (hipStream_t)stream, *x, *b, *l, stride);
+  return report("chunk replay", hipGetLastError());
 }
 
 extern "C" int bwt_combine_launch(const BwtIndex* x, const BwtBatch* b, const BwtLaunch* l, void* stream) {
-  (void)x;
   (void)l;
-  if (!b || b->M < 0) return (int)hipErrorInvalidValue;
+  if (!x || !b || b->M < 0) return (int)hipErrorInvalidValue;
   if (b->M == 0) return 0;
-  hipLaunchKernelGGL(bwt_combine_kernel, dim3((unsigned)((b->M + 255) / 256)), dim3(256), 0, (hipStream_t)stream, *b);
+  hipLaunchKernelGGL(bwt_combine_kernel, dim3((unsigned)((b->M + 255) / 256)), dim3(256), 0, (hipStream_t)stream, *x,
+                     *b);
   return report("combine", hipGetLastError());
 }
 

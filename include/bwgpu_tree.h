@@ -20,16 +20,17 @@
  * is the batch whose query q belongs to state q/Q+1.
  *
  * Algorithm.  The tree is linearised by its preorder walk into steps (+v enter, -v exit = undo);
- * consecutive steps form blocks whose distinct keys fit an LDS Fenwick tree.  answer(q) =
- * F(hi) - F(lo), F(x) = sum over the events before q's step with key < x, split as
+ * consecutive steps form blocks whose distinct keys fit LDS.  answer(q) = F(hi) - F(lo),
+ * F(x) = sum over the events before q's step with key < x, split as
  *   in-block part  -- block replay over the block's own distinct keys (local ranks);
- *   carry level C  -- one chain over all blocks, class = key tile (key >> TS);
- *   carry level B  -- one chain per key tile, class = key & (2^TS - 1).
- * Chains are cut into chunks; chunk histograms + a segmented column scan give every chunk its
- * dense carry; a chunk replay adds the chunk-local prefix at each query endpoint.  (These are
- * v7's lvals / dV / carrypref / combine, generalised.)  The *index* (block weights, chunk
- * carries) depends only on the updates and is built once; a *batch* (queries) only replays the
- * blocks and chunks it touches.
+ *   carry level C  -- one chain over all blocks, class = key tile (key >> TS), cut into one
+ *                     chunk per block, so its carry is a dense row per block read directly;
+ *   carry level B  -- one chain per key tile, class = key & (2^TS - 1), cut into chunks.
+ * Chunk histograms + a segmented column scan give every chunk its dense carry; a level-B chunk
+ * replay adds the chunk-local prefix at each query endpoint.  (These are v7's lvals / dV /
+ * carrypref / combine, generalised.)  The *index* (block weights, chunk carries) depends only
+ * on the updates and is built once; a *batch* (queries) only replays the blocks and chunks it
+ * touches.
  */
 #ifndef BWGPU_TREE_H
 #define BWGPU_TREE_H
@@ -43,7 +44,7 @@ extern "C" {
 /* Key item of a carry chain: class << 40 | weight index (into agg for level B, tagg for C). */
 #define BWT_CLASS_SHIFT 40
 #define BWT_REF_MASK ((1ull << 40) - 1)
-/* Endpoint item of a batch: local position << 44 | class << 32 | endpoint id (2q + side).
+/* Endpoint item of a batch: local position << 44 | class << 32 | endpoint id (2 slot + side).
  * The position is the number of the chunk's key items that precede the endpoint. */
 #define BWT_EP_CLASS_SHIFT 32
 #define BWT_EP_POS_SHIFT 44
@@ -58,9 +59,9 @@ typedef struct BwtIndex {
   int32_t nchain;
   int32_t ncls_c;    /* classes of chain 0 (level C: key tiles) */
   int32_t ncls_b;    /* classes of every other chain (level B: keys within a tile) */
-  int32_t pad0;
+  int32_t tile_shift;/* TS: level-C class of a coordinate x is x >> TS */
   int64_t nchunk;
-  int64_t nchunk_c;  /* chunks of chain 0; they come first */
+  int64_t nchunk_c;  /* chunks of chain 0 = nblk (chunk b = block b); they come first */
   int64_t nseg;
   const int64_t* delta;       /* S*U update deltas (state-major) */
   const int32_t* steps;       /* +v enter, -v exit */
@@ -85,33 +86,39 @@ typedef struct BwtIndex {
   int64_t* tagg;    /* net weight of every block per key tile */
   int64_t* cp;      /* chunk carries (dense rows) */
   int64_t* segsum;  /* scan scratch */
+  int64_t* kweight; /* weight of every key item in chain order (written by the histogram) */
 } BwtIndex;
 
-/* One query batch (built on the host by the batch planner; all device pointers). */
+/* One query batch (built on the host by the batch planner; all device pointers).  Queries are
+ * held in *slots*, grouped by block and, inside a block, by the step of their state. */
 typedef struct BwtBatch {
   int64_t M;                  /* queries */
-  const uint32_t* qlo;        /* M endpoint coordinates */
+  const uint32_t* qlo;        /* M endpoint coordinates, by query id */
   const uint32_t* qhi;
   int32_t nqblk;              /* blocks holding queried states */
   int32_t pad0;
   const int32_t* qblk;        /* nqblk block ids */
-  const int64_t* qblk_q0;     /* nqblk+1 offsets into qorder / qstep */
-  const uint32_t* qorder;     /* query ids of each listed block, by step */
-  const int32_t* qstep;       /* their (global) enter steps */
-  const uint32_t* qrank;      /* their endpoints' in-block ranks: lo | hi << 16 */
-  int64_t nwork;              /* chunk work items: a chunk and a run of its endpoints */
-  int64_t nwork_c;            /* work items on chain 0; they come first */
+  const int64_t* qblk_g0;     /* nqblk+1 offsets into the step groups */
+  int64_t ngrp;               /* step groups: the queries of one state */
+  const int32_t* grp_step;    /* ngrp global enter steps */
+  const int64_t* grp_q0;      /* ngrp+1 offsets into the slots */
+  const uint32_t* qorder;     /* M: query id of each slot */
+  const uint32_t* qrank;      /* M: the slot's endpoints' in-block ranks, lo | hi << 16 */
+  const int32_t* sblk;        /* M: block of each slot */
+  int64_t nwork;              /* level-B work items: a chunk and a run of its endpoints */
   const int64_t* wk_chunk;    /* nwork chunk ids */
   const int64_t* wk_e0;       /* nwork+1 offsets into eitems */
-  const uint64_t* eitems;     /* endpoint items per chunk, by position */
-  int64_t* ansr;              /* M: in-block part */
-  int64_t* g;                 /* 2 * 2M: carry per endpoint (level C half, level B half) */
-  int64_t* answers;           /* M */
+  const uint64_t* eitems;     /* level-B endpoint items per chunk, by position; id = 2*slot+side */
+  int64_t* ansr;              /* M, by slot: in-block part */
+  int64_t* g;                 /* 2M, by endpoint id: level-B carry */
+  int64_t* answers;           /* M, by query id */
 } BwtBatch;
 
 typedef struct BwtLaunch {
   int32_t block_threads;  /* block replay workgroup (multiple of 64, <= 1024) */
   int32_t chunk_waves;    /* wavefronts per workgroup of the chunk kernels (1..4), one chunk each */
+                          /* block replay: each thread owns the smallest power of two >= rmax /
+                           * block_threads of the block's ranks (at most 16) */
   int32_t write_index;    /* block replay also writes agg / tagg (needs every block listed) */
   int32_t pad0;
 } BwtLaunch;
