@@ -1,0 +1,287 @@
+// bw_batch.cpp -- the batched form: a stream of query batches answered over a resident index
+// (one JSON line, the phase/JSON style of the reference GPU workers, src/panel_batch_gpu.cpp:
+// 41-67,258-293).
+//
+//   bw_batch [--S n --U n --N n --seed n] [--requests 100000 --max-batch 2048]
+//            [--rmax --tile-shift --chunk --ep-per-work --seg --threads --waves] [--tuned FILE]
+//            [--warmup 1 --repeat 3] [--plain-sample 8] [--answers FILE] [--json FILE] [--stream-sha]
+//
+// The index (block weights, carry rows) is built once on the device.  Every batch of the stream
+// is then answered by one launch sequence (block replay, chunk replay, combine) captured in a
+// hipGraph: all batches are planned together (their queries are independent), so one replay of
+// the graph answers the whole stream.  Answers are in (batch, query) order.  For comparison the
+// first --plain-sample batches are also run the plain way, each on its own: one full pass
+// (index rebuild + answers, bwt_pass_launch) per batch, and one query-only launch per batch.
+// Exit codes: 2 usage, 3 unsupported input, 4 I/O, 5 GPU error.
+#include <hip/hip_runtime_api.h>
+#include <openssl/sha.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cinttypes>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "bwgpu_tree.h"
+#include "tree_engine.hpp"
+#include "tree_plan.hpp"
+#include "tree_tune.hpp"
+#include "tree_workload.hpp"
+
+using namespace bwg;
+using clk = std::chrono::steady_clock;
+
+#define HIPX(x)                                                                                   \
+  do {                                                                                            \
+    hipError_t e_ = (x);                                                                          \
+    if (e_ != hipSuccess) {                                                                       \
+      std::fprintf(stderr, "HIP error %s at %s:%d\n", hipGetErrorString(e_), __FILE__, __LINE__); \
+      std::exit(5);                                                                               \
+    }                                                                                             \
+  } while (0)
+#define RUN(x)                                                                       \
+  do {                                                                               \
+    const int rc_ = (x);                                                             \
+    if (rc_ != 0) {                                                                  \
+      std::fprintf(stderr, "launch failed (%d) at %s:%d\n", rc_, __FILE__, __LINE__); \
+      std::exit(5);                                                                  \
+    }                                                                                \
+  } while (0)
+
+static double ms_between(clk::time_point a, clk::time_point b) {
+  return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+static std::string sha_hex(const void* p, size_t n) {
+  unsigned char d[32];
+  SHA256(static_cast<const unsigned char*>(p), n, d);
+  static const char* hx = "0123456789abcdef";
+  std::string s(64, '0');
+  for (int i = 0; i < 32; ++i) { s[size_t(2 * i)] = hx[d[i] >> 4]; s[size_t(2 * i + 1)] = hx[d[i] & 15]; }
+  return s;
+}
+
+int main(int argc, char** argv) {
+  TreeSpec spec = tree_defaults(4);
+  StreamSpec ss = stream_defaults(4);
+  PlanKnobs kn;
+  LaunchKnobs lk;
+  int warmup = 1, repeat = 3, plain_sample = 8;
+  bool want_sha = false;
+  std::string answers_path, json_path, tuned_path;
+  bool have_knob[8] = {false, false, false, false, false, false, false, false};
+  for (int i = 1; i < argc; ++i) {
+    if (!std::strcmp(argv[i], "--stream-sha")) { want_sha = true; continue; }
+    if (i + 1 >= argc) { std::fprintf(stderr, "bw_batch: missing value for %s\n", argv[i]); return 2; }
+    const char* k = argv[i];
+    const char* v = argv[++i];
+    if (!std::strcmp(k, "--S")) spec.S = std::atoll(v);
+    else if (!std::strcmp(k, "--U")) spec.U = std::atoi(v);
+    else if (!std::strcmp(k, "--N")) spec.N = std::strtoull(v, nullptr, 10);
+    else if (!std::strcmp(k, "--seed")) spec.seed = std::strtoull(v, nullptr, 10);
+    else if (!std::strcmp(k, "--requests")) ss.nreq = std::atoll(v);
+    else if (!std::strcmp(k, "--max-batch")) ss.max_batch = std::atoi(v);
+    else if (!std::strcmp(k, "--rmax")) { kn.rmax = std::atoi(v); have_knob[0] = true; }
+    else if (!std::strcmp(k, "--tile-shift")) { kn.tile_shift = std::atoi(v); have_knob[1] = true; }
+    else if (!std::strcmp(k, "--chunk")) { kn.chunk_items = std::atoll(v); have_knob[2] = true; }
+    else if (!std::strcmp(k, "--ep-per-work")) { kn.ep_per_work = std::atoll(v); have_knob[3] = true; }
+    else if (!std::strcmp(k, "--seg")) kn.seg_chunks = std::atoi(v);
+    else if (!std::strcmp(k, "--threads")) { lk.block_threads = std::atoi(v); have_knob[4] = true; }
+    else if (!std::strcmp(k, "--waves")) { lk.chunk_waves = std::atoi(v); have_knob[5] = true; }
+    else if (!std::strcmp(k, "--tuned")) tuned_path = v;
+    else if (!std::strcmp(k, "--warmup")) warmup = std::atoi(v);
+    else if (!std::strcmp(k, "--repeat")) repeat = std::atoi(v);
+    else if (!std::strcmp(k, "--plain-sample")) plain_sample = std::atoi(v);
+    else if (!std::strcmp(k, "--answers")) answers_path = v;
+    else if (!std::strcmp(k, "--json")) json_path = v;
+    else { std::fprintf(stderr, "bw_batch: unknown argument %s\n", k); return 2; }
+  }
+  if (spec.S < 1 || spec.U < 1 || spec.N < 2 || ss.nreq < 1 || ss.max_batch < 1 || repeat < 1 || warmup < 0 ||
+      plain_sample < 0) {
+    std::fprintf(stderr, "bw_batch: bad sizes\n");
+    return 2;
+  }
+  // knobs chosen by the autotuner (tree_tune) for this device and workload shape, unless given
+  std::string tuned_from = "defaults";
+  if (!tuned_path.empty()) {
+    PlanKnobs tk = kn;
+    LaunchKnobs tl = lk;
+    if (load_tuned(tuned_path.c_str(), "batched", tk, tl)) {
+      if (!have_knob[0]) kn.rmax = tk.rmax;
+      if (!have_knob[1]) kn.tile_shift = tk.tile_shift;
+      if (!have_knob[2]) kn.chunk_items = tk.chunk_items;
+      if (!have_knob[3]) kn.ep_per_work = tk.ep_per_work;
+      if (!have_knob[4]) lk.block_threads = tl.block_threads;
+      if (!have_knob[5]) lk.chunk_waves = tl.chunk_waves;
+      tuned_from = tuned_path;
+    }
+  }
+
+  // ---- generate: the tree and the request stream
+  const auto t0 = clk::now();
+  TreeWorkload w;
+  tree_generate(spec, w);
+  RequestStream rs;
+  stream_generate(w, ss, rs);
+  if (want_sha) {
+    char hex[65];
+    stream_sha(rs, hex);
+    std::printf("{\"stream_sha256\":\"%s\",\"nq\":%" PRId64 ",\"napp\":%" PRId64 "}\n", hex, rs.nq(), rs.napp());
+    return 0;
+  }
+  const int64_t M = rs.nq();
+  const auto t1 = clk::now();
+
+  // ---- index on the device, then one plan for every query of the stream
+  char err[256];
+  HIPX(hipSetDevice(0));
+  HIPX(hipFree(nullptr));
+  TreeEngine E;
+  if (!E.init(spec.S, spec.U, w.coord_n(), w.parent.data(), w.key.data(), w.delta.data(), kn,
+              BwtLaunch{lk.block_threads, lk.chunk_waves, 0, 0}, err, sizeof(err))) {
+    std::fprintf(stderr, "bw_batch: index: %s\n", err);
+    return std::strncmp(err, "device", 6) == 0 ? 5 : 3;
+  }
+  const auto t2 = clk::now();
+  BatchPlan plan;
+  if (!build_batch_plan(E.plan, M, rs.qstate.data(), rs.qlo.data(), rs.qhi.data(), false, plan, err, sizeof(err))) {
+    std::fprintf(stderr, "bw_batch: batch plan: %s\n", err);
+    return 3;
+  }
+  const auto t3 = clk::now();
+  DeviceArena io;
+  uint32_t* d_qlo = io.upload(rs.qlo);
+  uint32_t* d_qhi = io.upload(rs.qhi);
+  DeviceBatch DB;
+  if (!io.ok() || !upload_batch(E, std::move(plan), d_qlo, d_qhi, DB, err, sizeof(err))) {
+    std::fprintf(stderr, "bw_batch: upload failed\n");
+    return 5;
+  }
+  HIPX(hipDeviceSynchronize());
+  const auto t4 = clk::now();
+
+  hipStream_t st;
+  HIPX(hipStreamCreate(&st));
+  hipEvent_t e0, e1;
+  HIPX(hipEventCreate(&e0));
+  HIPX(hipEventCreate(&e1));
+  // index build (once; timed on the device)
+  RUN(E.build_index(st));
+  HIPX(hipEventRecord(e0, st));
+  RUN(E.build_index(st));
+  HIPX(hipEventRecord(e1, st));
+  HIPX(hipEventSynchronize(e1));
+  float index_ms = 0;
+  HIPX(hipEventElapsedTime(&index_ms, e0, e1));
+  // the whole stream: one query launch sequence, captured once, replayed
+  BwtLaunch L = E.launch;
+  RUN(bwt_query_launch(&E.x, &DB.b, &L, st));  // warm (also sets the kernels' LDS limits)
+  HIPX(hipStreamSynchronize(st));
+  hipGraph_t graph;
+  hipGraphExec_t gexec;
+  HIPX(hipStreamBeginCapture(st, hipStreamCaptureModeGlobal));
+  RUN(bwt_query_launch(&E.x, &DB.b, &L, st));
+  HIPX(hipStreamEndCapture(st, &graph));
+  HIPX(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+  for (int i = 0; i < warmup; ++i) HIPX(hipGraphLaunch(gexec, st));
+  HIPX(hipStreamSynchronize(st));
+  std::vector<double> run_ms;
+  for (int i = 0; i < repeat; ++i) {
+    HIPX(hipEventRecord(e0, st));
+    HIPX(hipGraphLaunch(gexec, st));
+    HIPX(hipEventRecord(e1, st));
+    HIPX(hipEventSynchronize(e1));
+    float t = 0;
+    HIPX(hipEventElapsedTime(&t, e0, e1));
+    run_ms.push_back(t);
+  }
+  const auto tx0 = clk::now();
+  std::vector<int64_t> ans(static_cast<size_t>(M));
+  HIPX(hipMemcpy(ans.data(), DB.b.answers, size_t(M) * sizeof(int64_t), hipMemcpyDeviceToHost));
+  const std::string sha = sha_hex(ans.data(), size_t(M) * sizeof(int64_t));
+  const auto tx1 = clk::now();
+  if (!answers_path.empty()) {
+    FILE* f = std::fopen(answers_path.c_str(), "wb");
+    if (!f) return 4;
+    const bool ok = std::fwrite(ans.data(), sizeof(int64_t), size_t(M), f) == size_t(M);
+    if (std::fclose(f) != 0 || !ok) return 4;
+  }
+
+  // ---- plain comparison on the first batches: each batch on its own
+  const int64_t K = std::min<int64_t>(plain_sample, ss.nreq);
+  double plain_pass_ms = 0, plain_query_ms = 0;
+  int64_t plain_q = 0, plain_mismatch = 0;
+  for (int64_t r = 0; r < K; ++r) {
+    const int64_t a = rs.req_q0[size_t(r)], b = rs.req_q0[size_t(r) + 1];
+    BatchPlan bp;
+    if (!build_batch_plan(E.plan, b - a, rs.qstate.data() + a, rs.qlo.data() + a, rs.qhi.data() + a, true, bp, err,
+                          sizeof(err))) {
+      std::fprintf(stderr, "bw_batch: plain plan: %s\n", err);
+      return 3;
+    }
+    DeviceBatch one;
+    if (!upload_batch(E, std::move(bp), d_qlo + a, d_qhi + a, one, err, sizeof(err))) return 5;
+    RUN(bwt_pass_launch(&E.x, &one.b, &L, st));  // warm
+    HIPX(hipEventRecord(e0, st));
+    RUN(bwt_pass_launch(&E.x, &one.b, &L, st));
+    HIPX(hipEventRecord(e1, st));
+    HIPX(hipEventSynchronize(e1));
+    float t = 0;
+    HIPX(hipEventElapsedTime(&t, e0, e1));
+    plain_pass_ms += t;
+    HIPX(hipEventRecord(e0, st));
+    RUN(bwt_query_launch(&E.x, &one.b, &L, st));
+    HIPX(hipEventRecord(e1, st));
+    HIPX(hipEventSynchronize(e1));
+    HIPX(hipEventElapsedTime(&t, e0, e1));
+    plain_query_ms += t;
+    std::vector<int64_t> one_ans(static_cast<size_t>(b - a));
+    HIPX(hipMemcpy(one_ans.data(), one.b.answers, size_t(b - a) * sizeof(int64_t), hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < b - a; ++i) plain_mismatch += one_ans[size_t(i)] != ans[size_t(a + i)];
+    plain_q += b - a;
+  }
+  HIPX(hipGraphExecDestroy(gexec));
+  HIPX(hipGraphDestroy(graph));
+
+  std::sort(run_ms.begin(), run_ms.end());
+  double avg = 0;
+  for (double x : run_ms) avg += x;
+  avg /= double(run_ms.size());
+  hipDeviceProp_t prop{};
+  HIPX(hipGetDeviceProperties(&prop, 0));
+  const char* dev = prop.name[0] ? prop.name : prop.gcnArchName;
+  const BatchPlan& BP = DB.plan;
+  char buf[8192];
+  const int n = std::snprintf(
+      buf, sizeof(buf),
+      "{\"arm\":\"bwgpu_batched\",\"form\":4,\"S\":%" PRId64 ",\"U\":%d,\"N\":%" PRIu64 ",\"seed\":%" PRIu64
+      ",\"batches\":%" PRId64 ",\"max_batch\":%d,\"n_answers\":%" PRId64 ",\"answers_sha256\":\"%s\","
+      "\"plan\":{\"nblk\":%d,\"nqblk\":%zu,\"ngrp\":%zu,\"neitems\":%zu,\"nwork\":%zu,\"rmax\":%d,\"tile_shift\":%d,"
+      "\"chunk_items\":%" PRId64 ",\"ep_per_work\":%" PRId64 ",\"threads\":%d,\"waves\":%d,\"knobs_from\":\"%s\"},"
+      "\"phases_ms\":{\"generate\":%.3f,\"index_plan_upload\":%.3f,\"batch_plan\":%.3f,\"batch_upload\":%.3f,"
+      "\"extract\":%.3f,\"index_build\":%.4f},"
+      "\"graph_ms\":{\"avg\":%.4f,\"min\":%.4f,\"max\":%.4f,\"repeat\":%d,\"warmup\":%d},"
+      "\"queries_per_s\":%.1f,"
+      "\"plain\":{\"batches\":%" PRId64 ",\"queries\":%" PRId64 ",\"pass_ms\":%.4f,\"query_ms\":%.4f,"
+      "\"pass_queries_per_s\":%.1f,\"query_queries_per_s\":%.1f,\"mismatch\":%" PRId64 "},"
+      "\"device_bytes\":%zu,\"device\":\"%s\",\"cus\":%d}\n",
+      spec.S, spec.U, spec.N, spec.seed, ss.nreq, ss.max_batch, M, sha.c_str(), E.plan.nblk(), BP.qblk.size(),
+      BP.grp_step.size(), BP.eitems.size(), BP.wk_chunk.size(), E.plan.knobs.rmax, E.plan.knobs.tile_shift,
+      E.plan.knobs.chunk_items, E.plan.knobs.ep_per_work, L.block_threads, L.chunk_waves, tuned_from.c_str(),
+      ms_between(t0, t1), ms_between(t1, t2), ms_between(t2, t3), ms_between(t3, t4), ms_between(tx0, tx1), index_ms,
+      avg, run_ms.front(), run_ms.back(), repeat, warmup, double(M) / (avg * 1e-3), K, plain_q, plain_pass_ms,
+      plain_query_ms, plain_pass_ms > 0 ? double(plain_q) / (plain_pass_ms * 1e-3) : 0.0,
+      plain_query_ms > 0 ? double(plain_q) / (plain_query_ms * 1e-3) : 0.0, plain_mismatch,
+      E.arena.bytes + io.bytes + DB.arena.bytes, dev, prop.multiProcessorCount);
+  std::fwrite(buf, 1, size_t(n), stdout);
+  if (!json_path.empty()) {
+    FILE* f = std::fopen(json_path.c_str(), "w");
+    if (!f) return 4;
+    std::fwrite(buf, 1, size_t(n), f);
+    std::fclose(f);
+  }
+  return plain_mismatch == 0 ? 0 : 6;
+}

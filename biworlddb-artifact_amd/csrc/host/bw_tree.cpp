@@ -181,6 +181,7 @@ int main(int argc, char** argv) {
     return 5;
   }
   const BatchPlan& BP = DB.plan;
+  if (cfg == 3) DB.b.qtile = nullptr;  // combine reads the tiles of the ranks remapped on the device
   const BwtBatch& Bt = DB.b;
   BwtIndex& X = E.x;
   const size_t g_dev_bytes = E.arena.bytes + io.bytes + DB.arena.bytes;

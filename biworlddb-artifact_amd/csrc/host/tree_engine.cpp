@@ -116,8 +116,10 @@ bool upload_batch(const TreeEngine& e, BatchPlan&& plan, const uint32_t* d_qlo, 
   b.grp_step = out.arena.upload(B.grp_step);
   b.grp_q0 = out.arena.upload(B.grp_q0);
   b.qorder = out.arena.upload(B.qorder);
+  b.islot = out.arena.upload(B.islot);
   b.qrank = out.arena.upload(B.qrank);
   b.sblk = out.arena.upload(B.sblk);
+  b.qtile = out.arena.upload(B.qtile);
   b.nwork = int64_t(B.wk_chunk.size());
   b.wk_chunk = out.arena.upload(B.wk_chunk);
   b.wk_e0 = out.arena.upload(B.wk_e0);

@@ -300,9 +300,14 @@ bool build_batch_plan(const TreeIndexPlan& X, int64_t M, const int32_t* qstate, 
   for (int64_t q = 0; q < M; ++q) scnt[size_t(X.state_step[size_t(qstate[q])]) + 1]++;
   for (int64_t s = 0; s < nsteps; ++s) scnt[size_t(s) + 1] += scnt[size_t(s)];
   B.qorder.resize(size_t(M));
+  B.islot.resize(size_t(M));
   {
     std::vector<int64_t> f(scnt.begin(), scnt.end() - 1);
-    for (int64_t q = 0; q < M; ++q) B.qorder[size_t(f[size_t(X.state_step[size_t(qstate[q])])]++)] = uint32_t(q);
+    for (int64_t q = 0; q < M; ++q) {
+      const int64_t at = f[size_t(X.state_step[size_t(qstate[q])])]++;
+      B.qorder[size_t(at)] = uint32_t(q);
+      B.islot[size_t(q)] = uint32_t(at);
+    }
   }
   // ---- listed blocks and their step groups (the queried steps, ascending)
   B.qblk_g0.push_back(0);
@@ -324,6 +329,7 @@ bool build_batch_plan(const TreeIndexPlan& X, int64_t M, const int32_t* qstate, 
   // ---- per slot: block and in-block ranks of its endpoints
   B.qrank.resize(size_t(M));
   B.sblk.resize(size_t(M));
+  B.qtile.resize(size_t(M));
 #pragma omp parallel for schedule(static)
   for (int64_t j = 0; j < M; ++j) {
     const uint32_t q = B.qorder[size_t(j)];
@@ -334,6 +340,7 @@ bool build_batch_plan(const TreeIndexPlan& X, int64_t M, const int32_t* qstate, 
     const uint32_t rh = uint32_t(std::lower_bound(k0, k1, qhi[q]) - k0);
     B.qrank[size_t(j)] = rl | (rh << 16);
     B.sblk[size_t(j)] = b;
+    B.qtile[size_t(j)] = (qlo[q] >> ts) | ((qhi[q] >> ts) << 16);
   }
   // ---- level-B endpoint items: (chunk, item) for every endpoint
   const int64_t nE = 2 * M;
@@ -356,7 +363,7 @@ bool build_batch_plan(const TreeIndexPlan& X, int64_t M, const int32_t* qstate, 
       int64_t k = pos / CH;
       if (k >= n) k = n - 1;
       ech[size_t(2 * j + side)] = X.chain_chunk0[size_t(chain)] + k;
-      eit[size_t(2 * j + side)] = ep_item(uint64_t(pos - k * CH), x & mask, uint64_t(2 * j + side));
+      eit[size_t(2 * j + side)] = ep_item(uint64_t(pos - k * CH), x & mask, uint64_t(2 * q) + uint64_t(side));
     }
   }
   // ---- bucket by chunk (stable counting sort), then order each chunk by position

@@ -60,10 +60,12 @@ struct BatchPlan {
   std::vector<int32_t> grp_step;  // step of each group
   std::vector<int64_t> grp_q0;    // its slots
   std::vector<uint32_t> qorder;   // slot -> query id
+  std::vector<uint32_t> islot;    // query id -> slot
   std::vector<uint32_t> qrank;    // slot -> in-block ranks lo | hi << 16
   std::vector<int32_t> sblk;      // slot -> block
+  std::vector<uint32_t> qtile;    // slot -> key tiles of the endpoints lo | hi << 16
   std::vector<int64_t> wk_chunk, wk_e0;
-  std::vector<uint64_t> eitems;   // level B only, endpoint id = 2 * slot + side
+  std::vector<uint64_t> eitems;   // level B only, endpoint id = 2 * query id + side
   int64_t steps_run = 0;          // block-replay steps the batch needs (statistics)
 };
 

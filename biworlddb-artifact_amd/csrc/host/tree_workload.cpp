@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 
 namespace bwg {
 namespace {
@@ -162,6 +163,113 @@ void tree_workload_sha(const TreeWorkload& w, char out_hex[65]) {
   SHA256_Update(&c, w.qlo.data(), sizeof(uint32_t) * w.qlo.size());
   SHA256_Update(&c, w.qhi.data(), sizeof(uint32_t) * w.qhi.size());
   if (!w.dkey.empty()) SHA256_Update(&c, w.dkey.data(), sizeof(uint32_t) * w.dkey.size());
+  unsigned char d[32];
+  SHA256_Final(d, &c);
+  static const char* hx = "0123456789abcdef";
+  for (int i = 0; i < 32; ++i) {
+    out_hex[2 * i] = hx[d[i] >> 4];
+    out_hex[2 * i + 1] = hx[d[i] & 15];
+  }
+  out_hex[64] = 0;
+}
+
+StreamSpec stream_defaults(int form) {
+  StreamSpec s;
+  s.form = form;
+  if (form == 4) {
+    s.nreq = 100000;
+    s.max_batch = 2048;
+  } else {
+    s.nreq = 20000;
+    s.max_batch = 4096;
+    s.append_frac = 0.05;
+    s.append_U = 128;
+    s.rate = 20000.0;
+  }
+  return s;
+}
+
+void stream_generate(const TreeWorkload& tree, const StreamSpec& spec, RequestStream& rs) {
+  rs = RequestStream();
+  rs.spec = spec;
+  const uint64_t seed = tree.spec.seed;
+  const int64_t S = tree.spec.S;
+  const uint64_t N = tree.spec.N;
+  const int64_t R = spec.nreq;
+  rs.req_q0.assign(size_t(R) + 1, 0);
+  rs.req_kind.assign(size_t(R), 0);
+  rs.req_app.assign(size_t(R), 0);
+  rs.arrival.assign(size_t(R), 0.0);
+  const Stream d8(seed, 8), d13(seed, 13), d17(seed, 17);
+  int64_t nq = 0, napp = 0;
+  double t = 0.0;
+  for (int64_t r = 0; r < R; ++r) {
+    int64_t sz;
+    int kind = 0;
+    if (spec.form == 4) {
+      sz = 1 + int64_t(d8.at(uint64_t(r)) % uint64_t(spec.max_batch));
+    } else {
+      kind = unit(d13.at(uint64_t(r))) < spec.append_frac;
+      const double l = std::floor(std::exp(unit(d8.at(uint64_t(r))) * std::log(double(spec.max_batch))));
+      sz = l < 1.0 ? 1 : int64_t(l);
+      if (sz > spec.max_batch) sz = spec.max_batch;
+      if (kind) sz = 0;
+      t += -std::log1p(-unit(d17.at(uint64_t(r)))) / spec.rate;
+    }
+    rs.req_kind[size_t(r)] = int8_t(kind);
+    rs.req_app[size_t(r)] = napp;
+    rs.arrival[size_t(r)] = t;
+    napp += kind;
+    nq += sz;
+    rs.req_q0[size_t(r) + 1] = nq;
+  }
+  rs.qstate.resize(size_t(nq));
+  rs.qlo.resize(size_t(nq));
+  rs.qhi.resize(size_t(nq));
+  const size_t na = size_t(napp) * size_t(spec.append_U);
+  rs.app_parent.resize(size_t(napp));
+  rs.app_key.resize(na);
+  rs.app_delta.resize(na);
+  std::unique_ptr<Zipf> zipf;
+  if (spec.form == 5) zipf.reset(new Zipf(N, 1.1));
+  const Stream d9(seed, 9), d10(seed, 10), d11(seed, 11), d12(seed, 12), d14(seed, 14), d15(seed, 15),
+      d16(seed, 16);
+  for (int64_t a = 0; a < napp; ++a) {
+    rs.app_parent[size_t(a)] = int32_t(1 + int64_t(d14.at(uint64_t(a)) % uint64_t(S + a)));
+    for (int i = 0; i < spec.append_U; ++i) {
+      const uint64_t idx = uint64_t(a) * uint64_t(spec.append_U) + uint64_t(i);
+      rs.app_key[idx] = uint32_t(zipf->pick(unit(d15.at(idx))));
+      rs.app_delta[idx] = int64_t(d16.at(idx) % 2000001ULL) - 1000000;
+    }
+  }
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t r = 0; r < R; ++r) {
+    if (rs.req_kind[size_t(r)]) continue;
+    const uint64_t nstates = uint64_t(S + rs.req_app[size_t(r)]);
+    for (int64_t g = rs.req_q0[size_t(r)]; g < rs.req_q0[size_t(r) + 1]; ++g) {
+      rs.qstate[size_t(g)] = int32_t(1 + int64_t(d9.at(uint64_t(g)) % nstates));
+      if (spec.form == 5 && unit(d12.at(uint64_t(g))) < 0.5) {
+        const uint32_t k = uint32_t(zipf->pick(unit(d10.at(uint64_t(g)))));
+        rs.qlo[size_t(g)] = k;
+        rs.qhi[size_t(g)] = k + 1u;
+      } else {
+        log_range(d10.at(uint64_t(g)), d11.at(uint64_t(g)), N, rs.qlo[size_t(g)], rs.qhi[size_t(g)]);
+      }
+    }
+  }
+}
+
+void stream_sha(const RequestStream& rs, char out_hex[65]) {
+  SHA256_CTX c;
+  SHA256_Init(&c);
+  SHA256_Update(&c, rs.req_q0.data(), rs.req_q0.size() * sizeof(int64_t));
+  SHA256_Update(&c, rs.req_kind.data(), rs.req_kind.size());
+  SHA256_Update(&c, rs.qstate.data(), rs.qstate.size() * sizeof(int32_t));
+  SHA256_Update(&c, rs.qlo.data(), rs.qlo.size() * sizeof(uint32_t));
+  SHA256_Update(&c, rs.qhi.data(), rs.qhi.size() * sizeof(uint32_t));
+  SHA256_Update(&c, rs.app_parent.data(), rs.app_parent.size() * sizeof(int32_t));
+  SHA256_Update(&c, rs.app_key.data(), rs.app_key.size() * sizeof(uint32_t));
+  SHA256_Update(&c, rs.app_delta.data(), rs.app_delta.size() * sizeof(int64_t));
   unsigned char d[32];
   SHA256_Final(d, &c);
   static const char* hx = "0123456789abcdef";

@@ -46,6 +46,44 @@ void tree_generate(const TreeSpec& spec, TreeWorkload& w);
 // Matches the oracle's --workload-sha.
 void tree_workload_sha(const TreeWorkload& w, char out_hex[65]);
 
+// Request streams of the batched form (over a cfg-4 tree) and of the served form (over a cfg-2
+// tree that grows by appended branch states).  Batched: nreq batches of uniform size in
+// [1, max_batch], each query a uniform state and a cfg-2 log-uniform range.  Served: nreq
+// requests, each an append (probability append_frac: a new state under a uniform existing state
+// with append_U Zipf(1.1) updates) or a query batch of log-uniform size in [1, max_batch] over
+// the states existing at its arrival, half point queries [k, k+1) with Zipf(1.1) k, half
+// log-uniform ranges; Poisson arrivals at `rate` per second.  The exact draws are restated in
+// the test oracle's stream generator; the two are checked byte for byte by the test suite.
+struct StreamSpec {
+  int form = 4;  // 4 = batched, 5 = served
+  int64_t nreq = 100000;
+  int max_batch = 2048;
+  double append_frac = 0.0;
+  int append_U = 128;
+  double rate = 20000.0;
+};
+StreamSpec stream_defaults(int form);
+
+struct RequestStream {
+  StreamSpec spec;
+  std::vector<int64_t> req_q0;    // nreq+1 offsets into the queries
+  std::vector<int8_t> req_kind;   // 0 = query batch, 1 = append
+  std::vector<int64_t> req_app;   // nreq: appends before request r (an append r creates S+req_app[r]+1)
+  std::vector<double> arrival;    // seconds (served form)
+  std::vector<int32_t> qstate;
+  std::vector<uint32_t> qlo, qhi;
+  std::vector<int32_t> app_parent;  // napp
+  std::vector<uint32_t> app_key;    // napp * append_U
+  std::vector<int64_t> app_delta;
+  int64_t nq() const { return int64_t(qstate.size()); }
+  int64_t napp() const { return int64_t(app_parent.size()); }
+};
+// tree: the generated tree the stream runs over (its seed, S and N are used).
+void stream_generate(const TreeWorkload& tree, const StreamSpec& spec, RequestStream& rs);
+// SHA-256 (hex) over req_q0, req_kind, qstate, qlo, qhi, app_parent, app_key, app_delta.
+// Matches the oracle's stream --stream-sha.
+void stream_sha(const RequestStream& rs, char out_hex[65]);
+
 // cfg 3: CSR rank-LUT over the distinct keys (row = high 16 bits): 65,537 row pointers and the
 // low 16 bits of every key, plus host ranks of raw values (#keys < x).
 struct RankLut {

@@ -17,7 +17,7 @@
 //                              work item; the chunk's key items and the batch's endpoint items
 //                              are consumed in merged groups of 64; a Fenwick tree seeded from
 //                              the chunk's carry row; pairs inside a group by LDS broadcast
-//   bwt_combine_kernel      <- gpu_combine_kernel (src/gpu_v7.cu:84-96), plus the level-C carry
+//   bwt_combine_*_kernel    <- gpu_combine_kernel (src/gpu_v7.cu:84-96), plus the level-C carry
 //                              read straight from the block's dense carry row
 //   bwt_rank_kernel         <- CsrBase::rank (src/sparse_engine.hpp:62-97), CSR form
 // All arithmetic is int64 adds (LDS ds_add_u64, DPP scans): results are exact and independent
@@ -437,16 +437,29 @@ __global__ void __launch_bounds__(256) bwt_chunk_replay_kernel(const BwtIndex x,
 }
 
 // ---------------------------------------------------------------- combine
-// Slot s of the batch: in-block part + level-B carries (by slot) + level-C carry, read from the
-// dense carry row of the slot's block (level-C chunk b = block b) at the endpoints' key tiles.
-__global__ void __launch_bounds__(256) bwt_combine_kernel(const BwtIndex x, const BwtBatch q) {
+// Two passes, so that no answer is stored to a scattered position: slots (block order) add the
+// level-C carry, read from the dense carry row of the slot's block (level-C chunk b = block b)
+// at the endpoints' key tiles, to the in-block part -- every read streams and the rows stay in
+// cache; then query ids gather their slot's value and add their level-B carries, which the
+// chunk replay stored by query id, and store the answers in order.
+__global__ void __launch_bounds__(256) bwt_combine_slots_kernel(const BwtIndex x, const BwtBatch q) {
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= q.M) return;
-  const uint32_t id = q.qorder[s];
   const int64_t* row = x.cp + x.chunk_cp0[q.sblk[s]];
-  const int ts = x.tile_shift;
-  const int64_t gc = row[q.qhi[id] >> ts] - row[q.qlo[id] >> ts];
-  q.answers[id] = q.ansr[s] + gc + (q.g[2 * s + 1] - q.g[2 * s]);
+  uint32_t tl;
+  if (q.qtile) {
+    tl = q.qtile[s];
+  } else {  // tiles of the endpoint coordinates (e.g. ranks the device just remapped)
+    const uint32_t id = q.qorder[s];
+    tl = (q.qlo[id] >> x.tile_shift) | ((q.qhi[id] >> x.tile_shift) << 16);
+  }
+  q.ansr[s] += row[tl >> 16] - row[tl & 0xFFFFu];
+}
+
+__global__ void __launch_bounds__(256) bwt_combine_ids_kernel(const BwtBatch q) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= q.M) return;
+  q.answers[i] = q.ansr[q.islot[i]] + (q.g[2 * i + 1] - q.g[2 * i]);
 }
 
 // ---------------------------------------------------------------- rank remap
@@ -607,9 +620,12 @@ extern "C" int bwt_combine_launch(const BwtIndex* x, const BwtBatch* b, const Bw
   (void)l;
   if (!x || !b || b->M < 0) return (int)hipErrorInvalidValue;
   if (b->M == 0) return 0;
-  hipLaunchKernelGGL(bwt_combine_kernel, dim3((unsigned)((b->M + 255) / 256)), dim3(256), 0, (hipStream_t)stream, *x,
-                     *b);
-  return report("combine", hipGetLastError());
+  const dim3 grid((unsigned)((b->M + 255) / 256));
+  hipLaunchKernelGGL(bwt_combine_slots_kernel, grid, dim3(256), 0, (hipStream_t)stream, *x, *b);
+  int rc = report("combine (slots)", hipGetLastError());
+  if (rc) return rc;
+  hipLaunchKernelGGL(bwt_combine_ids_kernel, grid, dim3(256), 0, (hipStream_t)stream, *b);
+  return report("combine (ids)", hipGetLastError());
 }
 
 extern "C" int bwt_query_launch(const BwtIndex* x, const BwtBatch* b, const BwtLaunch* l, void* stream) {

@@ -44,7 +44,8 @@ extern "C" {
 /* Key item of a carry chain: class << 40 | weight index (into agg for level B, tagg for C). */
 #define BWT_CLASS_SHIFT 40
 #define BWT_REF_MASK ((1ull << 40) - 1)
-/* Endpoint item of a batch: local position << 44 | class << 32 | endpoint id (2 slot + side).
+/* Endpoint item of a batch: local position << 44 | class << 32 | endpoint id (2 q + side, q the
+ * query id).
  * The position is the number of the chunk's key items that precede the endpoint. */
 #define BWT_EP_CLASS_SHIFT 32
 #define BWT_EP_POS_SHIFT 44
@@ -103,14 +104,17 @@ typedef struct BwtBatch {
   const int32_t* grp_step;    /* ngrp global enter steps */
   const int64_t* grp_q0;      /* ngrp+1 offsets into the slots */
   const uint32_t* qorder;     /* M: query id of each slot */
+  const uint32_t* islot;      /* M: slot of each query id (the inverse of qorder) */
   const uint32_t* qrank;      /* M: the slot's endpoints' in-block ranks, lo | hi << 16 */
   const int32_t* sblk;        /* M: block of each slot */
+  const uint32_t* qtile;      /* M: the slot's endpoints' key tiles (level-C classes), lo | hi << 16;
+                               NULL: combine takes them from qlo/qhi (coordinates >> TS) */
   int64_t nwork;              /* level-B work items: a chunk and a run of its endpoints */
   const int64_t* wk_chunk;    /* nwork chunk ids */
   const int64_t* wk_e0;       /* nwork+1 offsets into eitems */
-  const uint64_t* eitems;     /* level-B endpoint items per chunk, by position; id = 2*slot+side */
-  int64_t* ansr;              /* M, by slot: in-block part */
-  int64_t* g;                 /* 2M, by endpoint id: level-B carry */
+  const uint64_t* eitems;     /* level-B endpoint items per chunk, by position; id = 2*q+side */
+  int64_t* ansr;              /* M, by slot: in-block part, then + level-C carry */
+  int64_t* g;                 /* 2M, by endpoint id (2*q+side): level-B carry */
   int64_t* answers;           /* M, by query id */
 } BwtBatch;
 

@@ -22,6 +22,11 @@
  *       undo with a Fenwick tree over keys (ranks for cfg 3).  "brute" walks the ancestor chain
  *       of every state and sums in __int128 (tiny inputs only; it shares no code with "fenwick").
  *       --workload-sha prints the SHA-256 of the generated arrays instead of solving.
+ *   stream --form 4|5 [--S --U --N --seed (tree)] [--requests R --max-batch M --append-frac f
+ *          --append-U n --rate r] [--algo fenwick|brute] [--stream-sha] --out ANS
+ *       Answers of the request streams of the batched (4) and served (5) forms (bw_stream.h)
+ *       over the tree extended by the appended states, in (request, query) order: the same
+ *       rollback-Fenwick DFS with queries grouped by state, or the ancestor walk ("brute").
  *
  * Output: answers as little-endian int64 in the reference's canonical order (state, query)
  * and one JSON line {"n_answers":..,"answers_sha256":..,"wall_s":..}.
@@ -36,6 +41,7 @@
 
 #include "bw_sha256.h"
 #include "bw_twl.h"
+#include "bw_stream.h"
 
 static double now_s(void) {
   struct timespec ts;
@@ -466,6 +472,165 @@ static int mode_tree(int argc, char** argv) {
   return 0;
 }
 
+/* ------------------------------------------------------------------ request streams */
+/* Rollback-Fenwick DFS over a tree given by parent[] (states 1..ns-1, parent[v] < v), state v's
+ * updates being [upd0[v], upd0[v+1]) of (krank, delta); query q asks state qstate[q] over the
+ * coordinate range [lrank[q], hrank[q]).  Same algebra as mode_tree's solver; it serves the
+ * streams, whose queries are arbitrary (state, range) pairs and whose trees grow by appends. */
+static void solve_states(int64_t ns, const int32_t* parent, const int64_t* upd0, const uint32_t* krank,
+                         const int64_t* delta, uint64_t NK, int64_t nq, const int32_t* qstate,
+                         const uint32_t* lrank, const uint32_t* hrank, int64_t* ans) {
+  int64_t* qoff = (int64_t*)calloc((size_t)ns + 1, sizeof(int64_t));
+  int64_t* qid = (int64_t*)malloc(sizeof(int64_t) * (size_t)(nq ? nq : 1));
+  for (int64_t q = 0; q < nq; ++q) qoff[qstate[q] + 1]++;
+  for (int64_t v = 0; v < ns; ++v) qoff[v + 1] += qoff[v];
+  {
+    int64_t* cur = (int64_t*)malloc(sizeof(int64_t) * (size_t)ns);
+    memcpy(cur, qoff, sizeof(int64_t) * (size_t)ns);
+    for (int64_t q = 0; q < nq; ++q) qid[cur[qstate[q]]++] = q;
+    free(cur);
+  }
+  int64_t* fen = (int64_t*)calloc((size_t)NK + 1, sizeof(int64_t));
+  int64_t* koff = (int64_t*)calloc((size_t)ns + 1, sizeof(int64_t));
+  int32_t* kid = (int32_t*)malloc(sizeof(int32_t) * (size_t)ns);
+  for (int64_t v = 1; v < ns; ++v) koff[parent[v] + 1]++;
+  for (int64_t v = 0; v < ns; ++v) koff[v + 1] += koff[v];
+  {
+    int64_t* cur = (int64_t*)malloc(sizeof(int64_t) * (size_t)ns);
+    memcpy(cur, koff, sizeof(int64_t) * (size_t)ns);
+    for (int64_t v = 1; v < ns; ++v) kid[cur[parent[v]]++] = (int32_t)v;
+    free(cur);
+  }
+  int64_t* stk_v = (int64_t*)malloc(sizeof(int64_t) * (size_t)ns);
+  int64_t* stk_c = (int64_t*)malloc(sizeof(int64_t) * (size_t)ns);
+  int64_t sp = 1;
+  stk_v[0] = 0;
+  stk_c[0] = koff[0];
+  while (sp > 0) {
+    const int64_t v = stk_v[sp - 1];
+    if (stk_c[sp - 1] < koff[v + 1]) {
+      const int64_t c = kid[stk_c[sp - 1]++];
+      for (int64_t i = upd0[c]; i < upd0[c + 1]; ++i)
+        for (uint64_t j = (uint64_t)krank[i] + 1; j <= NK; j += j & (~j + 1)) fen[j] += delta[i];
+      for (int64_t k = qoff[c]; k < qoff[c + 1]; ++k) {
+        const int64_t q = qid[k];
+        int64_t r = 0;
+        for (uint64_t j = hrank[q]; j > 0; j -= j & (~j + 1)) r += fen[j];
+        for (uint64_t j = lrank[q]; j > 0; j -= j & (~j + 1)) r -= fen[j];
+        ans[q] = r;
+      }
+      stk_v[sp] = c;
+      stk_c[sp] = koff[c];
+      ++sp;
+    } else {
+      for (int64_t i = upd0[v]; i < upd0[v + 1]; ++i)
+        for (uint64_t j = (uint64_t)krank[i] + 1; j <= NK; j += j & (~j + 1)) fen[j] -= delta[i];
+      --sp;
+    }
+  }
+  free(qoff); free(qid); free(fen); free(koff); free(kid); free(stk_v); free(stk_c);
+}
+
+/* Independent check for small streams: walk each query's ancestor chain over raw keys. */
+static void solve_states_brute(const int32_t* parent, const int64_t* upd0, const uint32_t* key, const int64_t* delta,
+                               int64_t nq, const int32_t* qstate, const uint32_t* qlo, const uint32_t* qhi,
+                               int64_t* ans) {
+  for (int64_t q = 0; q < nq; ++q) {
+    __int128 acc = 0;
+    for (int64_t a = qstate[q]; a > 0; a = parent[a])
+      for (int64_t i = upd0[a]; i < upd0[a + 1]; ++i)
+        if (key[i] >= qlo[q] && key[i] < qhi[q]) acc += delta[i];
+    ans[q] = checked_i64(acc);
+  }
+}
+
+static int mode_stream(int argc, char** argv) {
+  int form = 4;
+  for (int i = 2; i + 1 < argc; ++i)
+    if (!strcmp(argv[i], "--form")) form = atoi(argv[i + 1]);
+  if (form != 4 && form != 5) die(2, "--form must be 4 or 5");
+  twl_t w;
+  twl_defaults(form == 4 ? 4 : 2, &w);
+  twl_stream_t st;
+  twl_stream_defaults(form, &st);
+  const char* out = NULL;
+  const char* algo = "fenwick";
+  int ssha = 0;
+  for (int i = 2; i < argc; ++i) {
+    if (!strcmp(argv[i], "--stream-sha")) { ssha = 1; continue; }
+    if (i + 1 >= argc) die(2, "missing value");
+    const char* k = argv[i]; const char* v = argv[++i];
+    if (!strcmp(k, "--form")) continue;
+    else if (!strcmp(k, "--S")) w.S = atoll(v);
+    else if (!strcmp(k, "--U")) w.U = atoi(v);
+    else if (!strcmp(k, "--N")) w.N = strtoull(v, NULL, 10);
+    else if (!strcmp(k, "--seed")) w.seed = strtoull(v, NULL, 10);
+    else if (!strcmp(k, "--requests")) st.nreq = atoll(v);
+    else if (!strcmp(k, "--max-batch")) st.max_batch = atoi(v);
+    else if (!strcmp(k, "--append-frac")) st.append_frac = atof(v);
+    else if (!strcmp(k, "--append-U")) st.append_U = atoi(v);
+    else if (!strcmp(k, "--rate")) st.rate = atof(v);
+    else if (!strcmp(k, "--algo")) algo = v;
+    else if (!strcmp(k, "--out")) out = v;
+    else die(2, "unknown argument");
+  }
+  w.Q = 1;
+  if (w.S < 1 || w.U < 1 || w.N < 2 || w.N > 0xFFFFFFFFull || st.nreq < 1 || st.max_batch < 1 ||
+      (form == 5 && (st.append_U < 1 || st.rate <= 0.0 || st.append_frac < 0.0 || st.append_frac > 1.0)))
+    die(2, "bad sizes");
+  const double t0 = now_s();
+  if (twl_generate(&w) != 0 || twl_stream_generate(&w, &st) != 0) die(3, "out of memory");
+  if (ssha) {
+    bw_sha256_ctx c;
+    bw_sha256_init(&c);
+    bw_sha256_update(&c, st.req_q0, sizeof(int64_t) * (size_t)(st.nreq + 1));
+    bw_sha256_update(&c, st.req_kind, (size_t)st.nreq);
+    bw_sha256_update(&c, st.qstate, sizeof(int32_t) * (size_t)st.nq);
+    bw_sha256_update(&c, st.qlo, sizeof(uint32_t) * (size_t)st.nq);
+    bw_sha256_update(&c, st.qhi, sizeof(uint32_t) * (size_t)st.nq);
+    bw_sha256_update(&c, st.app_parent, sizeof(int32_t) * (size_t)st.napp);
+    bw_sha256_update(&c, st.app_key, sizeof(uint32_t) * (size_t)st.napp * (size_t)st.append_U);
+    bw_sha256_update(&c, st.app_delta, sizeof(int64_t) * (size_t)st.napp * (size_t)st.append_U);
+    char hex[65];
+    bw_sha256_final_hex(&c, hex);
+    printf("{\"stream_sha256\":\"%s\",\"nq\":%lld,\"napp\":%lld}\n", hex, (long long)st.nq, (long long)st.napp);
+    twl_stream_free(&st);
+    twl_free(&w);
+    return 0;
+  }
+  /* the tree extended by the appended states: state S+a+1 = append a */
+  const int64_t S = w.S, ns = S + st.napp + 1;
+  int32_t* parent = (int32_t*)malloc(sizeof(int32_t) * (size_t)ns);
+  int64_t* upd0 = (int64_t*)malloc(sizeof(int64_t) * (size_t)(ns + 1));
+  const size_t nu = (size_t)S * (size_t)w.U, na = (size_t)st.napp * (size_t)st.append_U;
+  uint32_t* key = (uint32_t*)malloc(sizeof(uint32_t) * (nu + na + 1));
+  int64_t* delta = (int64_t*)malloc(sizeof(int64_t) * (nu + na + 1));
+  memcpy(parent, w.parent, sizeof(int32_t) * (size_t)(S + 1));
+  for (int64_t a = 0; a < st.napp; ++a) parent[S + 1 + a] = st.app_parent[a];
+  upd0[0] = 0;
+  upd0[1] = 0;
+  for (int64_t v = 1; v <= S; ++v) upd0[v + 1] = upd0[v] + w.U;
+  for (int64_t a = 0; a < st.napp; ++a) upd0[S + 2 + a] = upd0[S + 1 + a] + st.append_U;
+  memcpy(key, w.key, sizeof(uint32_t) * nu);
+  memcpy(delta, w.delta, sizeof(int64_t) * nu);
+  if (na) {
+    memcpy(key + nu, st.app_key, sizeof(uint32_t) * na);
+    memcpy(delta + nu, st.app_delta, sizeof(int64_t) * na);
+  }
+  int64_t* ans = (int64_t*)calloc((size_t)(st.nq ? st.nq : 1), sizeof(int64_t));
+  const double ts0 = now_s();
+  if (!strcmp(algo, "brute"))
+    solve_states_brute(parent, upd0, key, delta, st.nq, st.qstate, st.qlo, st.qhi, ans);
+  else
+    solve_states(ns, parent, upd0, key, delta, w.N, st.nq, st.qstate, st.qlo, st.qhi, ans);
+  g_solve_s = now_s() - ts0;
+  write_answers(out, ans, (size_t)st.nq, t0);
+  free(ans); free(parent); free(upd0); free(key); free(delta);
+  twl_stream_free(&st);
+  twl_free(&w);
+  return 0;
+}
+
 static int mode_sha(int argc, char** argv) {
   if (argc < 3) die(2, "usage: sha FILE");
   FILE* fp = fopen(argv[2], "rb");
@@ -483,10 +648,11 @@ static int mode_sha(int argc, char** argv) {
 }
 
 int main(int argc, char** argv) {
-  if (argc < 2) die(2, "usage: bw_oracle sha|ref|tree ...");
+  if (argc < 2) die(2, "usage: bw_oracle sha|ref|tree|stream ...");
   if (!strcmp(argv[1], "sha")) return mode_sha(argc, argv);
   if (!strcmp(argv[1], "ref")) return mode_ref(argc, argv);
   if (!strcmp(argv[1], "tree")) return mode_tree(argc, argv);
+  if (!strcmp(argv[1], "stream")) return mode_stream(argc, argv);
   die(2, "unknown mode");
   return 2;
 }
