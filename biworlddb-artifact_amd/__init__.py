@@ -20,6 +20,10 @@ LIB_GPU = os.path.join(BUILD_DIR, "libbwgpu.so")
 LIB_HOST = os.path.join(BUILD_DIR, "libbwhost.so")
 BW_TREE = os.path.join(BUILD_DIR, "bw_tree")
 BW_PANEL = os.path.join(BUILD_DIR, "bw_panel")
+BW_BATCH = os.path.join(BUILD_DIR, "bw_batch")
+BW_TUNE = os.path.join(BUILD_DIR, "bw_tune")
+BW_SERVE = os.path.join(BUILD_DIR, "bw_serve")
+TUNED = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned.json")
 INCLUDE_DIR = os.path.join(REPO_DIR, "include")
 
 
@@ -30,7 +34,7 @@ def build(jobs=8):
 
 
 def require_built():
-    missing = [p for p in (LIB_GPU, LIB_HOST, BW_TREE, BW_PANEL) if not os.path.exists(p)]
+    missing = [p for p in (LIB_GPU, LIB_HOST, BW_TREE, BW_PANEL, BW_BATCH, BW_TUNE, BW_SERVE) if not os.path.exists(p)]
     if missing:
         raise RuntimeError("bwgpu native build missing (run make -C biworlddb-artifact_amd): " + ", ".join(missing))
 

@@ -65,8 +65,13 @@ static std::string sha_hex(const void* p, size_t n) {
 }
 
 int main(int argc, char** argv) {
-  TreeSpec spec = tree_defaults(4);
-  StreamSpec ss = stream_defaults(4);
+  // --form 5 only serves --stream-sha (the served form's stream over its cfg-2 tree)
+  int form = 4;
+  for (int i = 1; i + 1 < argc; ++i)
+    if (!std::strcmp(argv[i], "--form")) form = std::atoi(argv[i + 1]);
+  if (form != 4 && form != 5) { std::fprintf(stderr, "bw_batch: --form 4|5\n"); return 2; }
+  TreeSpec spec = tree_defaults(form == 4 ? 4 : 2);
+  StreamSpec ss = stream_defaults(form);
   PlanKnobs kn;
   LaunchKnobs lk;
   int warmup = 1, repeat = 3, plain_sample = 8;
@@ -78,12 +83,16 @@ int main(int argc, char** argv) {
     if (i + 1 >= argc) { std::fprintf(stderr, "bw_batch: missing value for %s\n", argv[i]); return 2; }
     const char* k = argv[i];
     const char* v = argv[++i];
-    if (!std::strcmp(k, "--S")) spec.S = std::atoll(v);
+    if (!std::strcmp(k, "--form")) continue;
+    else if (!std::strcmp(k, "--S")) spec.S = std::atoll(v);
     else if (!std::strcmp(k, "--U")) spec.U = std::atoi(v);
     else if (!std::strcmp(k, "--N")) spec.N = std::strtoull(v, nullptr, 10);
     else if (!std::strcmp(k, "--seed")) spec.seed = std::strtoull(v, nullptr, 10);
     else if (!std::strcmp(k, "--requests")) ss.nreq = std::atoll(v);
     else if (!std::strcmp(k, "--max-batch")) ss.max_batch = std::atoi(v);
+    else if (!std::strcmp(k, "--append-frac")) ss.append_frac = std::atof(v);
+    else if (!std::strcmp(k, "--append-U")) ss.append_U = std::atoi(v);
+    else if (!std::strcmp(k, "--rate")) ss.rate = std::atof(v);
     else if (!std::strcmp(k, "--rmax")) { kn.rmax = std::atoi(v); have_knob[0] = true; }
     else if (!std::strcmp(k, "--tile-shift")) { kn.tile_shift = std::atoi(v); have_knob[1] = true; }
     else if (!std::strcmp(k, "--chunk")) { kn.chunk_items = std::atoll(v); have_knob[2] = true; }
@@ -132,6 +141,7 @@ int main(int argc, char** argv) {
     std::printf("{\"stream_sha256\":\"%s\",\"nq\":%" PRId64 ",\"napp\":%" PRId64 "}\n", hex, rs.nq(), rs.napp());
     return 0;
   }
+  if (form != 4) { std::fprintf(stderr, "bw_batch: the served form runs in bw_serve\n"); return 2; }
   const int64_t M = rs.nq();
   const auto t1 = clk::now();
 

@@ -292,16 +292,30 @@ bool build_batch_plan(const TreeIndexPlan& X, int64_t M, const int32_t* qstate, 
   B.M = M;
   const int32_t nblk = X.nblk();
   const int64_t nsteps = int64_t(X.steps.size());
+  const int64_t nchunk = X.nchunk();
   const int ts = X.knobs.tile_shift;
   const uint32_t mask = (1u << ts) - 1;
   const int nth = std::max(1, omp_get_max_threads());
+  // Small batches (a few queries per thousand steps) are sorted; large ones counting-sorted over
+  // every step and chunk.  Both give the same plan: slots by (enter step, query id), endpoint
+  // items by (chunk, position, query order).
+  const bool sparse = M * 8 < nsteps;
   // ---- slots: queries ordered by enter step (stable), hence grouped by block
-  std::vector<int64_t> scnt(size_t(nsteps) + 1, 0);
-  for (int64_t q = 0; q < M; ++q) scnt[size_t(X.state_step[size_t(qstate[q])]) + 1]++;
-  for (int64_t s = 0; s < nsteps; ++s) scnt[size_t(s) + 1] += scnt[size_t(s)];
   B.qorder.resize(size_t(M));
   B.islot.resize(size_t(M));
-  {
+  if (sparse) {
+    std::vector<uint64_t> key(static_cast<size_t>(M));
+    for (int64_t q = 0; q < M; ++q) key[size_t(q)] = (uint64_t(X.state_step[size_t(qstate[q])]) << 32) | uint64_t(q);
+    std::sort(key.begin(), key.end());
+    for (int64_t at = 0; at < M; ++at) {
+      const uint32_t q = uint32_t(key[size_t(at)] & 0xFFFFFFFFu);
+      B.qorder[size_t(at)] = q;
+      B.islot[size_t(q)] = uint32_t(at);
+    }
+  } else {
+    std::vector<int64_t> scnt(size_t(nsteps) + 1, 0);
+    for (int64_t q = 0; q < M; ++q) scnt[size_t(X.state_step[size_t(qstate[q])]) + 1]++;
+    for (int64_t s = 0; s < nsteps; ++s) scnt[size_t(s) + 1] += scnt[size_t(s)];
     std::vector<int64_t> f(scnt.begin(), scnt.end() - 1);
     for (int64_t q = 0; q < M; ++q) {
       const int64_t at = f[size_t(X.state_step[size_t(qstate[q])])]++;
@@ -309,28 +323,42 @@ bool build_batch_plan(const TreeIndexPlan& X, int64_t M, const int32_t* qstate, 
       B.islot[size_t(q)] = uint32_t(at);
     }
   }
-  // ---- listed blocks and their step groups (the queried steps, ascending)
+  // ---- listed blocks and their step groups (the queried steps, ascending); with all_blocks,
+  //      blocks without queries are listed too (with no groups)
   B.qblk_g0.push_back(0);
   B.grp_q0.push_back(0);
-  for (int32_t b = 0; b < nblk; ++b) {
-    const int32_t sa = X.blk_step0[size_t(b)], sb = X.blk_step0[size_t(b) + 1];
-    if (!all_blocks && scnt[size_t(sb)] == scnt[size_t(sa)]) continue;
-    B.qblk.push_back(b);
-    int32_t last = sa - 1;
-    for (int32_t s = sa; s < sb; ++s)
-      if (scnt[size_t(s) + 1] > scnt[size_t(s)]) {
+  {
+    int64_t j = 0;  // next slot
+    int32_t b = 0;
+    while (true) {
+      const int32_t nb = j < M ? X.state_block[size_t(qstate[B.qorder[size_t(j)]])] : nblk;
+      if (all_blocks)
+        for (; b < nb; ++b) {
+          B.qblk.push_back(b);
+          B.qblk_g0.push_back(int64_t(B.grp_step.size()));
+        }
+      if (j >= M) break;
+      b = nb;
+      B.qblk.push_back(b);
+      const int32_t sa = X.blk_step0[size_t(b)];
+      int32_t last = sa;
+      while (j < M && X.state_block[size_t(qstate[B.qorder[size_t(j)]])] == b) {
+        const int32_t s = X.state_step[size_t(qstate[B.qorder[size_t(j)]])];
+        while (j < M && X.state_step[size_t(qstate[B.qorder[size_t(j)]])] == s) ++j;
         B.grp_step.push_back(s);
-        B.grp_q0.push_back(scnt[size_t(s) + 1]);
+        B.grp_q0.push_back(j);
         last = s;
       }
-    B.qblk_g0.push_back(int64_t(B.grp_step.size()));
-    B.steps_run += last + 1 - sa;
+      B.qblk_g0.push_back(int64_t(B.grp_step.size()));
+      B.steps_run += last + 1 - sa;
+      ++b;
+    }
   }
-  // ---- per slot: block and in-block ranks of its endpoints
+  // ---- per slot: block, in-block ranks and key tiles of its endpoints
   B.qrank.resize(size_t(M));
   B.sblk.resize(size_t(M));
   B.qtile.resize(size_t(M));
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (M > 4096)
   for (int64_t j = 0; j < M; ++j) {
     const uint32_t q = B.qorder[size_t(j)];
     const int32_t b = X.state_block[size_t(qstate[q])];
@@ -342,12 +370,12 @@ bool build_batch_plan(const TreeIndexPlan& X, int64_t M, const int32_t* qstate, 
     B.sblk[size_t(j)] = b;
     B.qtile[size_t(j)] = (qlo[q] >> ts) | ((qhi[q] >> ts) << 16);
   }
-  // ---- level-B endpoint items: (chunk, item) for every endpoint
+  // ---- level-B endpoint items: (chunk, item) for every endpoint, in slot order
   const int64_t nE = 2 * M;
   std::vector<int64_t> ech(static_cast<size_t>(nE));
   std::vector<uint64_t> eit(static_cast<size_t>(nE));
   const int64_t CH = X.knobs.chunk_items;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (M > 4096)
   for (int64_t j = 0; j < M; ++j) {
     const uint32_t q = B.qorder[size_t(j)];
     const int32_t b = B.sblk[size_t(j)];
@@ -366,53 +394,79 @@ bool build_batch_plan(const TreeIndexPlan& X, int64_t M, const int32_t* qstate, 
       eit[size_t(2 * j + side)] = ep_item(uint64_t(pos - k * CH), x & mask, uint64_t(2 * q) + uint64_t(side));
     }
   }
-  // ---- bucket by chunk (stable counting sort), then order each chunk by position
-  const int64_t nchunk = X.nchunk();
-  std::vector<int64_t> ccnt(size_t(nth) * size_t(nchunk), 0);
-  std::vector<int64_t> e0(size_t(nth) + 1);
-  for (int t = 0; t <= nth; ++t) e0[size_t(t)] = nE * t / nth;
-#pragma omp parallel num_threads(nth)
-  {
-    const int t = omp_get_thread_num();
-    int64_t* h = &ccnt[size_t(t) * size_t(nchunk)];
-    for (int64_t i = e0[size_t(t)]; i < e0[size_t(t) + 1]; ++i) h[size_t(ech[size_t(i)])]++;
-  }
-  std::vector<int64_t> cstart(size_t(nchunk) + 1, 0);
-  for (int64_t c = 0; c < nchunk; ++c) {
-    int64_t s = 0;
-    for (int t = 0; t < nth; ++t) s += ccnt[size_t(t) * size_t(nchunk) + size_t(c)];
-    cstart[size_t(c) + 1] = cstart[size_t(c)] + s;
-  }
-  {
-    std::vector<int64_t> pos(cstart.begin(), cstart.end() - 1);
-    for (int t = 0; t < nth; ++t)
-      for (int64_t c = 0; c < nchunk; ++c) {
-        int64_t& slot = ccnt[size_t(t) * size_t(nchunk) + size_t(c)];
-        const int64_t n = slot;
-        slot = pos[size_t(c)];
-        pos[size_t(c)] += n;
-      }
-  }
+  // ---- group by chunk, each chunk's items by position (ties keep slot order)
+  std::vector<int64_t> cfirst;  // chunks holding endpoints, ascending, and their item ranges
+  std::vector<int64_t> cstart;
   B.eitems.resize(size_t(nE));
+  if (sparse) {
+    std::vector<uint32_t> ord(static_cast<size_t>(nE));
+    for (int64_t i = 0; i < nE; ++i) ord[size_t(i)] = uint32_t(i);
+    std::sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) {
+      if (ech[a] != ech[b]) return ech[a] < ech[b];
+      const uint64_t pa = eit[a] >> BWT_EP_POS_SHIFT, pb = eit[b] >> BWT_EP_POS_SHIFT;
+      return pa != pb ? pa < pb : a < b;
+    });
+    for (int64_t i = 0; i < nE; ++i) {
+      const int64_t c = ech[ord[size_t(i)]];
+      if (cfirst.empty() || cfirst.back() != c) {
+        cfirst.push_back(c);
+        cstart.push_back(i);
+      }
+      B.eitems[size_t(i)] = eit[ord[size_t(i)]];
+    }
+    cstart.push_back(nE);
+  } else {
+    std::vector<int64_t> ccnt(size_t(nth) * size_t(nchunk), 0);
+    std::vector<int64_t> e0(size_t(nth) + 1);
+    for (int t = 0; t <= nth; ++t) e0[size_t(t)] = nE * t / nth;
 #pragma omp parallel num_threads(nth)
-  {
-    const int t = omp_get_thread_num();
-    int64_t* c = &ccnt[size_t(t) * size_t(nchunk)];
-    for (int64_t i = e0[size_t(t)]; i < e0[size_t(t) + 1]; ++i)
-      B.eitems[size_t(c[size_t(ech[size_t(i)])]++)] = eit[size_t(i)];
-  }
+    {
+      const int t = omp_get_thread_num();
+      int64_t* h = &ccnt[size_t(t) * size_t(nchunk)];
+      for (int64_t i = e0[size_t(t)]; i < e0[size_t(t) + 1]; ++i) h[size_t(ech[size_t(i)])]++;
+    }
+    std::vector<int64_t> all(size_t(nchunk) + 1, 0);
+    for (int64_t c = 0; c < nchunk; ++c) {
+      int64_t s = 0;
+      for (int t = 0; t < nth; ++t) s += ccnt[size_t(t) * size_t(nchunk) + size_t(c)];
+      all[size_t(c) + 1] = all[size_t(c)] + s;
+    }
+    {
+      std::vector<int64_t> pos(all.begin(), all.end() - 1);
+      for (int t = 0; t < nth; ++t)
+        for (int64_t c = 0; c < nchunk; ++c) {
+          int64_t& slot = ccnt[size_t(t) * size_t(nchunk) + size_t(c)];
+          const int64_t n = slot;
+          slot = pos[size_t(c)];
+          pos[size_t(c)] += n;
+        }
+    }
+#pragma omp parallel num_threads(nth)
+    {
+      const int t = omp_get_thread_num();
+      int64_t* c = &ccnt[size_t(t) * size_t(nchunk)];
+      for (int64_t i = e0[size_t(t)]; i < e0[size_t(t) + 1]; ++i)
+        B.eitems[size_t(c[size_t(ech[size_t(i)])]++)] = eit[size_t(i)];
+    }
 #pragma omp parallel for schedule(dynamic, 8)
-  for (int64_t c = 0; c < nchunk; ++c)
-    std::stable_sort(B.eitems.begin() + cstart[size_t(c)], B.eitems.begin() + cstart[size_t(c) + 1],
-                     [](uint64_t a, uint64_t b) { return (a >> BWT_EP_POS_SHIFT) < (b >> BWT_EP_POS_SHIFT); });
+    for (int64_t c = 0; c < nchunk; ++c)
+      std::stable_sort(B.eitems.begin() + all[size_t(c)], B.eitems.begin() + all[size_t(c) + 1],
+                       [](uint64_t a, uint64_t b) { return (a >> BWT_EP_POS_SHIFT) < (b >> BWT_EP_POS_SHIFT); });
+    for (int64_t c = 0; c < nchunk; ++c)
+      if (all[size_t(c) + 1] > all[size_t(c)]) {
+        cfirst.push_back(c);
+        cstart.push_back(all[size_t(c)]);
+      }
+    cstart.push_back(nE);
+  }
   // work items: each chunk's endpoints in runs of at most ep_per_work (a work item replays the
   // chunk's keys only up to its own last endpoint)
   B.wk_e0.push_back(0);
   const int64_t G = X.knobs.ep_per_work;
-  for (int64_t c = 0; c < nchunk; ++c)
-    for (int64_t a = cstart[size_t(c)]; a < cstart[size_t(c) + 1]; a += G) {
-      B.wk_chunk.push_back(c);
-      B.wk_e0.push_back(std::min(a + G, cstart[size_t(c) + 1]));
+  for (size_t i = 0; i < cfirst.size(); ++i)
+    for (int64_t a = cstart[i]; a < cstart[i + 1]; a += G) {
+      B.wk_chunk.push_back(cfirst[i]);
+      B.wk_e0.push_back(std::min(a + G, cstart[i + 1]));
     }
   return true;
 }

@@ -462,6 +462,27 @@ __global__ void __launch_bounds__(256) bwt_combine_ids_kernel(const BwtBatch q) 
   q.answers[i] = q.ansr[q.islot[i]] + (q.g[2 * i + 1] - q.g[2 * i]);
 }
 
+// ---------------------------------------------------------------- overlay (appended states)
+// One wavefront per overlay query: walk from the appended state up to its first indexed
+// ancestor, every lane summing its share of each appended state's updates in [lo, hi).
+__global__ void __launch_bounds__(256) bwt_overlay_kernel(const BwtOverlay o) {
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t i = (int64_t)blockIdx.x * 4 + wv;
+  if (i >= o.n) return;
+  const uint32_t id = o.qid[i];
+  const uint32_t lo = o.qlo[id], hi = o.qhi[id];
+  int64_t acc = 0;
+  for (int64_t v = o.qstate[i]; v > o.S; v = o.app_parent[v - o.S - 1]) {
+    const int64_t base = (v - o.S - 1) * (int64_t)o.U;
+    for (int u = lane; u < o.U; u += 64) {
+      const uint32_t k = o.app_key[base + u];
+      if (k >= lo && k < hi) acc += o.app_delta[base + u];
+    }
+  }
+  const int64_t tot = readlane64(wave_incl_scan(acc), 63);
+  if (lane == 0) o.answers[id] += tot;
+}
+
 // ---------------------------------------------------------------- rank remap
 __global__ void __launch_bounds__(256) bwt_rank_kernel(const BwtRankArgs a) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -646,6 +667,13 @@ extern "C" int bwt_pass_launch(const BwtIndex* x, const BwtBatch* b, const BwtLa
   if ((rc = bwt_carry_scan_launch(x, &w, stream))) return rc;
   if ((rc = bwt_chunk_replay_launch(x, b, &w, stream))) return rc;
   return bwt_combine_launch(x, b, &w, stream);
+}
+
+extern "C" int bwt_overlay_launch(const BwtOverlay* o, void* stream) {
+  if (!o || o->n < 0 || o->U < 1 || o->S < 1) return (int)hipErrorInvalidValue;
+  if (o->n == 0) return 0;
+  hipLaunchKernelGGL(bwt_overlay_kernel, dim3((unsigned)((o->n + 3) / 4)), dim3(256), 0, (hipStream_t)stream, *o);
+  return report("overlay", hipGetLastError());
 }
 
 extern "C" int bwt_rank_remap_launch(const BwtRankArgs* a, void* stream) {

@@ -140,6 +140,27 @@ int bwt_pass_launch(const BwtIndex* x, const BwtBatch* b, const BwtLaunch* l, vo
 /* The query side only, over a resident index (block replay, chunk replay, combine). */
 int bwt_query_launch(const BwtIndex* x, const BwtBatch* b, const BwtLaunch* l, void* stream);
 
+/* Queries on states appended after the index was built (the served form).  Appended state
+ * S+1+a (a = 0, 1, ...) has parent app_parent[a] and U updates app_key/app_delta[a*U .. a*U+U).
+ * A query on an appended state is answered as its deepest indexed ancestor p (a query of the
+ * batch) plus this overlay: the updates of the appended states on the path from the state up
+ * to p with lo <= key < hi, added to answers[qid].  One wavefront per overlay query. */
+typedef struct BwtOverlay {
+  int64_t S;                  /* indexed states (appended ones are numbered after them) */
+  int32_t U;                  /* updates per appended state */
+  int32_t pad0;
+  int64_t n;                  /* overlay queries */
+  const uint32_t* qid;        /* n: query ids within the batch */
+  const int32_t* qstate;      /* n: their appended states */
+  const uint32_t* qlo;        /* by query id */
+  const uint32_t* qhi;
+  const int32_t* app_parent;  /* appended states' parents (any state numbered lower) */
+  const uint32_t* app_key;    /* appended updates: key coordinates */
+  const int64_t* app_delta;
+  int64_t* answers;           /* by query id: += overlay part */
+} BwtOverlay;
+int bwt_overlay_launch(const BwtOverlay* o, void* stream);
+
 /* CSR rank-LUT over a sorted set of D distinct 32-bit keys (src/sparse_engine.hpp:62-97 is the
  * reference's bitmap+rank-LUT; this is its CSR form): row h = keys whose high 16 bits equal h,
  * row_ptr[h] = number of keys with high bits < h (65537 entries), low16[i] = low 16 bits of

@@ -5,6 +5,10 @@ tree_cases.json: branching multi-update workloads (the benchmark configs' shapes
   * "brute"   the same workload solved by the oracle's independent ancestor-walk (small cases);
   * "ref"     the reference's own Solver (src/oracle.cpp:225-451) run by oracle/_ref/tree_harness_*
               on the same workload (small cases; needs the reference sources to build).
+stream_cases.json: request streams of the batched (form 4) and served (form 5) forms
+  (oracle/bw_stream.h): "oracle" = bw_oracle stream (rollback-Fenwick over the tree extended by
+  the appended states), "brute" = its independent ancestor walk (small cases), "stream" = the
+  SHA-256 of the generated stream (the product's generator must reproduce it).
 ref_cases.json: reference-native workloads (one edit per state, src/common.hpp:279-374) on
   fixtures written by make_fixture.py; "oracle" = oracle/_ref/oracle_K, "compiled" =
   oracle/_ref/compiled_K (the reference's CPU v7 batch engine, BWH_BATCH_FORCE=1), "mine" = the
@@ -35,6 +39,18 @@ TREE_CASES = [
     ("c2_cfg2", 2, 300000, 128, 32, 1 << 20, 20261205, ()),
     ("c3_cfg3", 3, 300000, 256, 32, 4194304, 20261206, ()),
     ("c4_cfg4", 4, 300000, 128, 32, 1 << 20, 20261207, ()),
+]
+
+STREAM_CASES = [
+    # name, form, tree (S, U, N, seed or None for the form's default), stream args, solvers
+    ("s1_batched_small", 4, (300, 8, 4096, None), ["--requests", "50", "--max-batch", "64"], ("brute",)),
+    ("s2_batched_mid", 4, (20000, 16, 1 << 20, None), ["--requests", "2000", "--max-batch", "512"], ()),
+    ("s3_served_small", 5, (300, 8, 4096, None),
+     ["--requests", "200", "--max-batch", "64", "--append-U", "8", "--rate", "1000"], ("brute",)),
+    ("s4_served_mid", 5, (20000, 16, 1 << 20, None),
+     ["--requests", "2000", "--max-batch", "1024", "--append-U", "16", "--rate", "5000"], ()),
+    ("S4_batched_full", 4, None, [], ()),
+    ("S5_served_full", 5, None, [], ()),
 ]
 
 REF_CASES = [
@@ -85,6 +101,35 @@ def tree(only=None):
             rec["ref"] = sha_of([exe, dump, os.path.join(WORK, name + ".ref.bin")])
         shas = {v for k2, v in rec.items() if k2 in ("oracle", "brute", "ref")}
         assert len(shas) == 1, (name, rec)
+        out[name] = rec
+        print(name, rec, flush=True)
+    return out
+
+
+def stream_args(form, tree_, args):
+    cmd = ["stream", "--form", str(form)]
+    if tree_:
+        S, U, N, seed = tree_
+        cmd += ["--S", str(S), "--U", str(U), "--N", str(N)] + (["--seed", str(seed)] if seed else [])
+    return cmd + list(args)
+
+
+def streams(only=None):
+    out = {}
+    for name, form, tree_, args, solvers in STREAM_CASES:
+        if only and name not in only:
+            continue
+        base = [ORACLE] + stream_args(form, tree_, args)
+        rec = {"form": form, "tree": list(tree_) if tree_ else None, "args": list(args)}
+        p = subprocess.run(base + ["--stream-sha"], capture_output=True, text=True, check=True)
+        st = json.loads(p.stdout.strip().splitlines()[-1])
+        rec["stream"] = st["stream_sha256"]
+        rec["nq"] = st["nq"]
+        rec["napp"] = st["napp"]
+        rec["oracle"] = sha_of(base + ["--out", os.path.join(WORK, "s.bin")])
+        if "brute" in solvers:
+            rec["brute"] = sha_of(base + ["--algo", "brute", "--out", os.path.join(WORK, "sb.bin")])
+            assert rec["brute"] == rec["oracle"], (name, rec)
         out[name] = rec
         print(name, rec, flush=True)
     return out
@@ -162,6 +207,10 @@ if __name__ == "__main__":
     if what in ("all", "tree"):
         with open(os.path.join(HERE, "tree_cases.json"), "w") as f:
             json.dump(tree(), f, indent=1, sort_keys=True)
+    if what in ("all", "stream"):
+        os.makedirs(WORK, exist_ok=True)
+        with open(os.path.join(HERE, "stream_cases.json"), "w") as f:
+            json.dump(streams(), f, indent=1, sort_keys=True)
     if what in ("all", "ref"):
         cases = refnative()
         cases.update(refbig())

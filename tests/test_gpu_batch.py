@@ -1,0 +1,41 @@
+"""GPU parity of the batched form (bw_batch: every batch of a request stream answered by one
+hipGraph replay over a resident index) against the stream oracle's golden hashes.  Integer work:
+bit-exact SHA-256 of the whole answer vector in (batch, query) order; the first batches are also
+answered one by one the plain way and must match the batched answers element for element."""
+import pytest
+
+from conftest import golden, run_json
+
+STREAMS = golden("stream_cases.json")
+
+
+def _cmd(pkg, g, extra):
+    cmd = [pkg.BW_BATCH]
+    if g["tree"]:
+        S, U, N, seed = g["tree"]
+        cmd += ["--S", str(S), "--U", str(U), "--N", str(N)] + (["--seed", str(seed)] if seed else [])
+    return cmd + g["args"] + ["--warmup", "0", "--repeat", "2"] + extra
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case,knobs", [
+    ("s1_batched_small", []),
+    ("s1_batched_small", ["--rmax", "64", "--tile-shift", "6", "--chunk", "128", "--threads", "64", "--waves", "1"]),
+    ("s2_batched_mid", []),
+    ("s2_batched_mid", ["--rmax", "1024", "--tile-shift", "9", "--chunk", "2048", "--threads", "128"]),
+])
+def test_batched_matches_oracle(pkg, case, knobs):
+    g = STREAMS[case]
+    r = run_json(_cmd(pkg, g, knobs + ["--plain-sample", "6"]), timeout=300)
+    assert r["answers_sha256"] == g["oracle"], (case, knobs)
+    assert r["n_answers"] == g["nq"]
+    assert r["plain"]["mismatch"] == 0
+
+
+@pytest.mark.gpu
+def test_batched_full_stream(pkg):
+    """100,000 batches (102.5M queries) over the 300,000-state tree, tuned knobs."""
+    g = STREAMS["S4_batched_full"]
+    r = run_json(_cmd(pkg, g, ["--plain-sample", "3", "--tuned", pkg.TUNED]), timeout=900)
+    assert r["answers_sha256"] == g["oracle"]
+    assert r["plain"]["mismatch"] == 0

@@ -1,0 +1,41 @@
+"""GPU parity of the served form (bw_serve: an open-loop Poisson request stream with appended
+branch states, continuous batching over a resident index) against the stream oracle's golden
+hashes, and the served answers of sampled requests against the same requests answered alone."""
+import pytest
+
+from conftest import golden, run_json
+
+STREAMS = golden("stream_cases.json")
+
+
+def _cmd(pkg, g, extra):
+    cmd = [pkg.BW_SERVE]
+    if g["tree"]:
+        S, U, N, seed = g["tree"]
+        cmd += ["--S", str(S), "--U", str(U), "--N", str(N)] + (["--seed", str(seed)] if seed else [])
+    return cmd + g["args"] + extra
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case,extra", [
+    ("s3_served_small", ["--check-alone", "16"]),
+    ("s3_served_small", ["--slots", "1", "--check-alone", "8"]),
+    ("s4_served_mid", ["--check-alone", "32"]),
+    ("s4_served_mid", ["--slots", "4", "--budget-mb", "64", "--rate", "1e9", "--check-alone", "8"]),
+])
+def test_served_matches_oracle_and_alone(pkg, case, extra):
+    g = STREAMS[case]
+    r = run_json(_cmd(pkg, g, extra), timeout=300)
+    assert r["answers_sha256"] == g["oracle"], (case, extra)
+    assert r["n_answers"] == g["nq"] and r["appends"] == g["napp"]
+    assert r["alone"]["mismatch"] == 0 and r["alone"]["requests"] > 0
+    assert r["latency_ms"]["p99"] >= r["latency_ms"]["p50"] > 0
+
+
+@pytest.mark.gpu
+def test_served_full_stream(pkg):
+    """20,000 requests (9.2M queries, 1,031 appended states) over the 300,000-state tree."""
+    g = STREAMS["S5_served_full"]
+    r = run_json(_cmd(pkg, g, ["--check-alone", "32"]), timeout=600)
+    assert r["answers_sha256"] == g["oracle"]
+    assert r["alone"]["mismatch"] == 0

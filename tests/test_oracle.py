@@ -100,6 +100,45 @@ def test_product_generator_matches_oracle_generator(oracle_bin, pkg):
         assert a == b, (cfg, a, b)
 
 
+STREAMS = golden("stream_cases.json")
+
+
+def _stream_cmd(g):
+    cmd = ["--form", str(g["form"])]
+    if g["tree"]:
+        S, U, N, seed = g["tree"]
+        cmd += ["--S", str(S), "--U", str(U), "--N", str(N)] + (["--seed", str(seed)] if seed else [])
+    return cmd + g["args"]
+
+
+@pytest.mark.parametrize("case", [k for k in STREAMS if not k.startswith("S")])
+def test_stream_oracle_matches_golden(oracle_bin, case, tmp_path):
+    """The stream oracle reproduces its golden hash (small streams are also pinned by the brute walk)."""
+    g = STREAMS[case]
+    r = run_json([oracle_bin, "stream"] + _stream_cmd(g) + ["--out", str(tmp_path / "s.bin")])
+    assert r["answers_sha256"] == g["oracle"] and r["n_answers"] == g["nq"]
+    if "brute" in g:
+        assert g["brute"] == g["oracle"]
+
+
+@pytest.mark.parametrize("case", list(STREAMS))
+def test_product_stream_generator_matches_oracle(pkg, case):
+    """The product's request-stream generator is byte-identical to the oracle's (full sizes too)."""
+    g = STREAMS[case]
+    r = run_json([pkg.BW_BATCH] + _stream_cmd(g) + ["--stream-sha"], timeout=300)
+    assert r["stream_sha256"] == g["stream"] and r["nq"] == g["nq"] and r["napp"] == g["napp"]
+
+
+def test_stream_planted_mistake_is_caught(oracle_bin, tmp_path):
+    """A stream solved over a tree whose one update is changed (another seed) must not match."""
+    g = STREAMS["s1_batched_small"]
+    cmd = _stream_cmd(g)
+    S, U, N, _ = g["tree"]
+    bad = ["--form", "4", "--S", str(S), "--U", str(U), "--N", str(N), "--seed", "12345"] + g["args"]
+    assert run_json([oracle_bin, "stream"] + bad + ["--out", str(tmp_path / "b.bin")])["answers_sha256"] != g["oracle"]
+    assert run_json([oracle_bin, "stream"] + cmd + ["--out", str(tmp_path / "a.bin")])["answers_sha256"] == g["oracle"]
+
+
 def test_c_abi_exports_every_declared_symbol(pkg):
     """The library loads (no GPU call is made) and exports every entry point include/*.h declares."""
     names = pkg.declared_symbols()
