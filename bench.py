@@ -1,15 +1,27 @@
-"""Benchmark: exact range-sum queries/s of the GPU batch path on one MI355X.
+"""Benchmark: exact range-sum queries/s at 300,000 states -- the batched form against the plain GPU
+drop-in, with the served form's p50/p99 latency -- on one MI355X.
 
-Workload: the plain GPU drop-in at paper scale (benchmark config 2): N = 2^20 keys, 300,000
-branching states (parent = previous w.p. 0.9, else uniform among the prior 1,024), 128 Zipf(1.1)
-point updates per state with int64 deltas in [-10^6, 10^6], 32 log-uniform range queries per
-state -> 9,600,000 exact int64 answers per pass.  A "step" is one complete pass (block replay,
-carry chunks, carry scan, chunk replay, combine) over inputs already resident in HBM; the worker
-times every pass with HIP events on its stream, bracketed by stream synchronisation.
+Headline (`value`): the batched form.  A request stream of 100,000 query batches (sizes uniform in
+[1, 2048], 102,546,012 queries over uniform states, log-uniform ranges) over the 300,000-state
+tree with uneven chains (geometric runs, mean 64, max 4,096; 128 Zipf(1.1) point updates per state,
+N = 2^20 keys) is answered by ONE replay of a hipGraph (block replay, level-B chunk replay,
+combine) over the index resident in HBM.  A "step" is one such replay over inputs already in HBM;
+bw_batch times each with HIP events on its stream.  The answers' SHA-256 is reported (the test
+suite pins it against the oracle).
 
-Prints one JSON line.  `roofline` is the dominant kernel's algorithmic bytes per launch over its
-average HIP-event duration against the MI355X HBM peak (8 TB/s); `cpu_baseline` is the oracle's
-rollback-Fenwick solver timed on this host (1 thread) on the same workload.
+Beside it, measured in the same run:
+  plain   -- the plain GPU drop-in at paper scale (config 2: 300,000 states, 9.6M queries, one
+             launch sequence that rebuilds the index and answers the batch) and, on the batched
+             stream's first batches, the same plain pass run once per batch;
+  served  -- the served form (bw_serve): 20,000 Poisson requests (log-uniform sizes in [1, 4096],
+             half point / half range queries, 5% appended branch states) by continuous batching,
+             p50/p99 request latency;
+  roofline -- the batched step's dominant kernel: its algorithmic bytes per launch over its
+             average HIP-event duration against the MI355X HBM peak (8 TB/s); `traffic` is the
+             rocprofv3 FETCH_SIZE + WRITE_SIZE of that kernel (profiles/), scaled by the same
+             counters' reading for a copy of known size, when that summary exists;
+  cpu_baseline -- the oracle's rollback-Fenwick stream solver (1 host thread) on a bounded sample
+             of the same stream (its first 10,000 batches).
 """
 import argparse
 import json
@@ -18,85 +30,109 @@ import subprocess
 import sys
 
 REPO = os.path.dirname(os.path.abspath(__file__))
-BW_TREE = os.path.join(REPO, "biworlddb-artifact_amd", "build", "bw_tree")
+BUILD = os.path.join(REPO, "biworlddb-artifact_amd", "build")
+BW_BATCH = os.path.join(BUILD, "bw_batch")
+BW_TREE = os.path.join(BUILD, "bw_tree")
+BW_SERVE = os.path.join(BUILD, "bw_serve")
+TUNED = os.path.join(REPO, "biworlddb-artifact_amd", "tuned.json")
 ORACLE = os.path.join(REPO, "oracle", "bw_oracle")
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_FILE = os.path.join(REPO, "profiles", "traffic_cfg2.json")
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "traffic_batched.json")
+CPU_SAMPLE_BATCHES = 10000
 
 
 def last_json(text):
     return json.loads([ln for ln in text.splitlines() if ln.startswith("{")][-1])
 
 
-def cpu_baseline(cfg):
-    """Oracle (rollback-Fenwick DFS, 1 thread) on the full workload; generation time excluded."""
-    p = subprocess.run([ORACLE, "tree", "--cfg", str(cfg)], capture_output=True, text=True, check=True, timeout=600)
-    r = last_json(p.stdout)
-    return r
+def run(cmd, timeout):
+    print("bench: " + " ".join(os.path.basename(c) if i == 0 else c for i, c in enumerate(cmd)), file=sys.stderr,
+          flush=True)
+    p = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
+    if p.returncode != 0:
+        sys.exit(f"bench.py: {os.path.basename(cmd[0])} failed ({p.returncode}): {p.stderr[-2000:]}")
+    return last_json(p.stdout)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cfg", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-served", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build measures one GPU; the path's sharding is described in DESIGN.md "
+        print("bench.py: this build measures one GPU; how the path would shard is described in DESIGN.md "
               "(no multi-GPU run is made).", file=sys.stderr)
         sys.exit(2)
-    if not os.path.exists(BW_TREE):
-        sys.exit("bench.py: native build missing (make -C biworlddb-artifact_amd)")
-    print(f"bench: bw_tree cfg {a.cfg} warmup {a.warmup} steps {a.steps}", file=sys.stderr, flush=True)
-    p = subprocess.run([BW_TREE, "--cfg", str(a.cfg), "--warmup", str(a.warmup), "--repeat", str(a.steps)],
-                       capture_output=True, text=True, timeout=1200)
-    if p.returncode != 0:
-        sys.exit(f"bench.py: bw_tree failed ({p.returncode}): {p.stderr[-2000:]}")
-    r = last_json(p.stdout)
+    for exe in (BW_BATCH, BW_TREE, BW_SERVE):
+        if not os.path.exists(exe):
+            sys.exit("bench.py: native build missing (python -c 'import __graft_entry__ as g; g.build()')")
+    knobs = ["--tuned", TUNED] if os.path.exists(TUNED) else []
+    r = run([BW_BATCH, "--warmup", str(a.warmup), "--repeat", str(a.steps), "--plain-sample", "4", "--stages"] + knobs,
+            1200)
+    plain = run([BW_TREE, "--cfg", "2", "--mode", "pass", "--warmup", "2", "--repeat", "5"], 600)
     stages = {k: v for k, v in r["stage_ms"].items() if v > 0}
     dom = max(stages, key=stages.get)
-    dom_ms = stages[dom]
-    dom_bytes = r["stage_bytes"][dom]
-    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    achieved = r["stage_bytes"][dom] / (stages[dom] * 1e-3) / 1e9
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
             traffic = json.load(f).get(dom)
     out = {
-        "metric": "exact range-sum queries/s at 300k states (plain GPU drop-in, one launch sequence per query batch)",
+        "metric": "exact range-sum queries/s at 300k states, batched over a resident index (one hipGraph replay "
+                  "per stream of 100,000 query batches) vs the plain GPU drop-in, served p99",
         "value": round(r["queries_per_s"], 1),
         "unit": "queries/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": round(r["pass_ms"]["avg"], 4),
+        "ms_per_step": round(r["graph_ms"]["avg"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "int64",
-        "data": "synthetic (seeded counter-based generator, config 2 shape)",
-        "config": {"workload": f"config {a.cfg}: {r['S']} states x {r['U']} updates, {r['Q']} queries/state, "
-                               f"N={r['N']}", "states": r["S"], "updates_per_state": r["U"],
-                   "queries_per_state": r["Q"], "keys": r["N"], "answers_per_step": r["n_answers"],
-                   "answers_sha256": r["answers_sha256"]},
+        "data": "synthetic (seeded counter-based generators: config-4 tree and its 100,000-batch request stream)",
+        "config": {"workload": f"batched: {r['batches']} query batches (sizes U[1,{r['max_batch']}], "
+                               f"{r['n_answers']} queries) over {r['S']} states x {r['U']} updates, N={r['N']}",
+                   "states": r["S"], "updates_per_state": r["U"], "keys": r["N"], "batches": r["batches"],
+                   "answers_per_step": r["n_answers"], "answers_sha256": r["answers_sha256"], "knobs": r["plan"]},
+        "plain": {
+            "config2_queries_per_s": round(plain["queries_per_s"], 1),
+            "config2_pass_ms": plain["pass_ms"]["avg"],
+            "config2_answers_sha256": plain["answers_sha256"],
+            "per_batch_pass_queries_per_s": round(r["plain"]["pass_queries_per_s"], 1),
+            "per_batch_query_only_queries_per_s": round(r["plain"]["query_queries_per_s"], 1),
+            "per_batch_sample_batches": r["plain"]["batches"],
+        },
+        "batched_vs_plain": round(r["queries_per_s"] / plain["queries_per_s"], 3),
+        "batched_vs_plain_per_batch": round(r["queries_per_s"] / max(1.0, r["plain"]["pass_queries_per_s"]), 1),
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel_ms": round(dom_ms, 4), "kernel_bytes_alg": dom_bytes,
-                     "pass_bytes_alg": r["bytes_alg"],
-                     "pass_effective_GBs": round(r["bytes_alg"] / (r["pass_ms"]["avg"] * 1e-3) / 1e9, 1)},
+                     "kernel_ms": round(stages[dom], 4), "kernel_bytes_alg": r["stage_bytes"][dom]},
         "stage_ms": r["stage_ms"],
-        "setup_ms": r["phases_ms"]["setup"],
+        "index_build_ms": r["phases_ms"]["index_build"],
+        "host_batch_plan_ms": r["phases_ms"]["batch_plan"],
         "device": r["device"],
     }
+    if not a.no_served:
+        s = run([BW_SERVE, "--check-alone", "16"] + knobs, 600)
+        out["served"] = {"queries_per_s": round(s["queries_per_s"], 1),
+                         "offered_queries_per_s": round(s["offered_queries_per_s"], 1),
+                         "latency_ms": s["latency_ms"], "requests": s["requests"], "appends": s["appends"],
+                         "batches": s["batches"], "answers_sha256": s["answers_sha256"],
+                         "alone_mismatch": s["alone"]["mismatch"]}
     if not a.no_cpu:
-        c = cpu_baseline(a.cfg)
-        out["cpu_baseline"] = {"value": round(r["n_answers"] / c["solve_s"], 1), "unit": "queries/s", "cores": 1,
+        p = subprocess.run([ORACLE, "stream", "--form", "4", "--requests", str(CPU_SAMPLE_BATCHES)],
+                           capture_output=True, text=True, check=True, timeout=900)
+        c = last_json(p.stdout)
+        out["cpu_baseline"] = {"value": round(c["n_answers"] / c["solve_s"], 1), "unit": "queries/s", "cores": 1,
                                "kind": "port",
-                               "sample": "the full config-2 workload (9.6M queries), oracle rollback-Fenwick DFS "
-                                         "on 1 host thread, generation excluded; answers SHA "
-                                         + ("equal" if c["answers_sha256"] == r["answers_sha256"] else "DIFFERENT"),
+                               "sample": f"the batched stream's first {CPU_SAMPLE_BATCHES} batches "
+                                         f"({c['n_answers']} queries) over the same 300,000-state tree, oracle "
+                                         "rollback-Fenwick DFS (tree replay + queries) on 1 host thread, generation "
+                                         "excluded",
                                "seconds": round(c["solve_s"], 3)}
     print(json.dumps(out))
 

@@ -4,7 +4,8 @@
 //
 //   bw_batch [--S n --U n --N n --seed n] [--requests 100000 --max-batch 2048]
 //            [--rmax --tile-shift --chunk --ep-per-work --seg --threads --waves] [--tuned FILE]
-//            [--warmup 1 --repeat 3] [--plain-sample 8] [--answers FILE] [--json FILE] [--stream-sha]
+//            [--warmup 1 --repeat 3] [--plain-sample 8] [--stages] [--answers FILE] [--json FILE]
+//            [--stream-sha]
 //
 // The index (block weights, carry rows) is built once on the device.  Every batch of the stream
 // is then answered by one launch sequence (block replay, chunk replay, combine) captured in a
@@ -12,6 +13,8 @@
 // the graph answers the whole stream.  Answers are in (batch, query) order.  For comparison the
 // first --plain-sample batches are also run the plain way, each on its own: one full pass
 // (index rebuild + answers, bwt_pass_launch) per batch, and one query-only launch per batch.
+// --stages also times the launch sequence's kernels one by one (HIP events on the stream they run
+// on, outside the graph) and reports each one's algorithmic bytes.
 // Exit codes: 2 usage, 3 unsupported input, 4 I/O, 5 GPU error.
 #include <hip/hip_runtime_api.h>
 #include <openssl/sha.h>
@@ -75,11 +78,12 @@ int main(int argc, char** argv) {
   PlanKnobs kn;
   LaunchKnobs lk;
   int warmup = 1, repeat = 3, plain_sample = 8;
-  bool want_sha = false;
+  bool want_sha = false, stages = false;
   std::string answers_path, json_path, tuned_path;
   bool have_knob[8] = {false, false, false, false, false, false, false, false};
   for (int i = 1; i < argc; ++i) {
     if (!std::strcmp(argv[i], "--stream-sha")) { want_sha = true; continue; }
+    if (!std::strcmp(argv[i], "--stages")) { stages = true; continue; }
     if (i + 1 >= argc) { std::fprintf(stderr, "bw_batch: missing value for %s\n", argv[i]); return 2; }
     const char* k = argv[i];
     const char* v = argv[++i];
@@ -208,6 +212,48 @@ int main(int argc, char** argv) {
     HIPX(hipEventElapsedTime(&t, e0, e1));
     run_ms.push_back(t);
   }
+  // per-kernel timing of the same sequence (outside the graph), averaged over `repeat` runs
+  double st_ms[3] = {0, 0, 0};
+  if (stages) {
+    hipEvent_t ev[4];
+    for (auto& e : ev) HIPX(hipEventCreate(&e));
+    for (int i = 0; i < repeat; ++i) {
+      HIPX(hipEventRecord(ev[0], st));
+      RUN(bwt_block_replay_launch(&E.x, &DB.b, &L, st));
+      HIPX(hipEventRecord(ev[1], st));
+      RUN(bwt_chunk_replay_launch(&E.x, &DB.b, &L, st));
+      HIPX(hipEventRecord(ev[2], st));
+      RUN(bwt_combine_launch(&E.x, &DB.b, &L, st));
+      HIPX(hipEventRecord(ev[3], st));
+      HIPX(hipEventSynchronize(ev[3]));
+      for (int k = 0; k < 3; ++k) {
+        float t = 0;
+        HIPX(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
+        st_ms[k] += double(t) / repeat;
+      }
+    }
+    for (auto& e : ev) HIPX(hipEventDestroy(e));
+  }
+  // algorithmic bytes per kernel (what each must read and write once):
+  //  block replay: per replayed event its u16 rank + i64 delta, per step its id, per query its
+  //    packed ranks (u32) + in-block result (i64)
+  //  chunk replay: per replayed key item its class word (u32) + weight (i64), per endpoint item
+  //    (u64) its result (i64)
+  //  combine: per query the slot's block and tiles (2 x u32), 2 carry-row reads, the in-block part
+  //    read and written, its slot (u32), the gathered value, 2 level-B carries, the answer
+  int64_t kit_run = 0;
+  {
+    const BatchPlan& P2 = DB.plan;
+    for (size_t i = 0; i < P2.wk_chunk.size(); ++i) {
+      const int64_t c = P2.wk_chunk[i];
+      kit_run += std::min<int64_t>(E.plan.chunk_key0[size_t(c) + 1] - E.plan.chunk_key0[size_t(c)],
+                                   int64_t(P2.eitems[size_t(P2.wk_e0[i + 1] - 1)] >> BWT_EP_POS_SHIFT));
+    }
+  }
+  const double sb_block =
+      double(DB.plan.steps_run) * spec.U * 10.0 + double(DB.plan.steps_run) * 4.0 + double(M) * 12.0;
+  const double sb_chunk = double(kit_run) * 12.0 + double(DB.plan.eitems.size()) * 16.0;
+  const double sb_combine = double(M) * (8.0 + 16.0 + 16.0 + 4.0 + 8.0 + 16.0 + 8.0);
   const auto tx0 = clk::now();
   std::vector<int64_t> ans(static_cast<size_t>(M));
   HIPX(hipMemcpy(ans.data(), DB.b.answers, size_t(M) * sizeof(int64_t), hipMemcpyDeviceToHost));
@@ -275,6 +321,8 @@ int main(int argc, char** argv) {
       "\"extract\":%.3f,\"index_build\":%.4f},"
       "\"graph_ms\":{\"avg\":%.4f,\"min\":%.4f,\"max\":%.4f,\"repeat\":%d,\"warmup\":%d},"
       "\"queries_per_s\":%.1f,"
+      "\"stage_ms\":{\"block_replay\":%.4f,\"chunk_replay\":%.4f,\"combine\":%.4f},"
+      "\"stage_bytes\":{\"block_replay\":%.0f,\"chunk_replay\":%.0f,\"combine\":%.0f},"
       "\"plain\":{\"batches\":%" PRId64 ",\"queries\":%" PRId64 ",\"pass_ms\":%.4f,\"query_ms\":%.4f,"
       "\"pass_queries_per_s\":%.1f,\"query_queries_per_s\":%.1f,\"mismatch\":%" PRId64 "},"
       "\"device_bytes\":%zu,\"device\":\"%s\",\"cus\":%d}\n",
@@ -282,7 +330,8 @@ int main(int argc, char** argv) {
       BP.grp_step.size(), BP.eitems.size(), BP.wk_chunk.size(), E.plan.knobs.rmax, E.plan.knobs.tile_shift,
       E.plan.knobs.chunk_items, E.plan.knobs.ep_per_work, L.block_threads, L.chunk_waves, tuned_from.c_str(),
       ms_between(t0, t1), ms_between(t1, t2), ms_between(t2, t3), ms_between(t3, t4), ms_between(tx0, tx1), index_ms,
-      avg, run_ms.front(), run_ms.back(), repeat, warmup, double(M) / (avg * 1e-3), K, plain_q, plain_pass_ms,
+      avg, run_ms.front(), run_ms.back(), repeat, warmup, double(M) / (avg * 1e-3), st_ms[0], st_ms[1], st_ms[2],
+      sb_block, sb_chunk, sb_combine, K, plain_q, plain_pass_ms,
       plain_query_ms, plain_pass_ms > 0 ? double(plain_q) / (plain_pass_ms * 1e-3) : 0.0,
       plain_query_ms > 0 ? double(plain_q) / (plain_query_ms * 1e-3) : 0.0, plain_mismatch,
       E.arena.bytes + io.bytes + DB.arena.bytes, dev, prop.multiProcessorCount);

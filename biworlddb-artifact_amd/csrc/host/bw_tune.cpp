@@ -1,8 +1,11 @@
 // bw_tune.cpp -- runs the autotuner (tree_tune) for one workload form on this device and stores
 // the chosen knobs in a tuned-knobs file read by bw_batch / bw_serve (--tuned FILE).
 //
-//   bw_tune --form batched|served [--S n --U n --N n --seed n] [--sample-requests 5000]
+//   bw_tune --form batched|served [--S n --U n --N n --seed n] [--sample-requests R]
 //           [--repeat 3] --out FILE
+// The sample is the stream's first R requests (default: the whole batched stream; 400 served
+// requests, about the size of one served batch): knobs are chosen at the density of queries per
+// block the form actually runs at.
 // One JSON line per trial on stderr, the chosen knobs as one JSON line on stdout.
 #include <hip/hip_runtime_api.h>
 
@@ -20,7 +23,7 @@ using namespace bwg;
 
 int main(int argc, char** argv) {
   std::string form = "batched", out;
-  int64_t sample = 5000;
+  int64_t sample = -1;
   int repeat = 3;
   int cfg_form = 4;
   for (int i = 1; i + 1 < argc; ++i)
@@ -42,6 +45,7 @@ int main(int argc, char** argv) {
     else if (!std::strcmp(k, "--out")) out = v;
     else { std::fprintf(stderr, "bw_tune: unknown argument %s\n", k); return 2; }
   }
+  if (sample < 0) sample = cfg_form == 4 ? stream_defaults(4).nreq : 400;
   if (out.empty() || sample < 1 || repeat < 1) { std::fprintf(stderr, "bw_tune: --out FILE required\n"); return 2; }
   TreeWorkload w;
   tree_generate(spec, w);
