@@ -3,7 +3,7 @@
 // 41-67,258-293).
 //
 //   bw_batch [--S n --U n --N n --seed n] [--requests 100000 --max-batch 2048]
-//            [--rmax --tile-shift --chunk --ep-per-work --seg --threads --waves] [--tuned FILE]
+//            [--rmax --tile-shift --chunk --ep-per-work --range-slots --seg --threads --waves] [--tuned FILE]
 //            [--warmup 1 --repeat 3] [--plain-sample 8] [--stages] [--answers FILE] [--json FILE]
 //            [--stream-sha]
 //
@@ -101,6 +101,7 @@ int main(int argc, char** argv) {
     else if (!std::strcmp(k, "--tile-shift")) { kn.tile_shift = std::atoi(v); have_knob[1] = true; }
     else if (!std::strcmp(k, "--chunk")) { kn.chunk_items = std::atoll(v); have_knob[2] = true; }
     else if (!std::strcmp(k, "--ep-per-work")) { kn.ep_per_work = std::atoll(v); have_knob[3] = true; }
+    else if (!std::strcmp(k, "--range-slots")) { kn.range_slots = std::atoll(v); have_knob[6] = true; }
     else if (!std::strcmp(k, "--seg")) kn.seg_chunks = std::atoi(v);
     else if (!std::strcmp(k, "--threads")) { lk.block_threads = std::atoi(v); have_knob[4] = true; }
     else if (!std::strcmp(k, "--waves")) { lk.chunk_waves = std::atoi(v); have_knob[5] = true; }
@@ -127,6 +128,7 @@ int main(int argc, char** argv) {
       if (!have_knob[1]) kn.tile_shift = tk.tile_shift;
       if (!have_knob[2]) kn.chunk_items = tk.chunk_items;
       if (!have_knob[3]) kn.ep_per_work = tk.ep_per_work;
+      if (!have_knob[6]) kn.range_slots = tk.range_slots;
       if (!have_knob[4]) lk.block_threads = tl.block_threads;
       if (!have_knob[5]) lk.chunk_waves = tl.chunk_waves;
       tuned_from = tuned_path;
@@ -237,10 +239,12 @@ int main(int argc, char** argv) {
   // algorithmic bytes per kernel (what each must read and write once):
   //  block replay: per replayed event its u16 rank + i64 delta, per step its id, per query its
   //    packed ranks (u32) + in-block result (i64)
-  //  chunk replay: per replayed key item its class word (u32) + weight (i64), per endpoint item
-  //    (u64) its result (i64)
-  //  combine: per query the slot's block and tiles (2 x u32), 2 carry-row reads, the in-block part
-  //    read and written, its slot (u32), the gathered value, 2 level-B carries, the answer
+  //  chunk replay: per replayed key item its class word (u32) + weight (i64), per work item its
+  //    chunk's carry row, per endpoint item (u64) its read and per stored result (one per
+  //    endpoint, one per same-tile pair) an i64
+  //  combine: per slot its block and tiles (2 x u32), 2 carry-row reads, the in-block part, the
+  //    2 level-B words, its staging position (u32) and staged value; per query id its staging
+  //    position (u32), the staged value and the answer
   int64_t kit_run = 0;
   {
     const BatchPlan& P2 = DB.plan;
@@ -252,8 +256,10 @@ int main(int argc, char** argv) {
   }
   const double sb_block =
       double(DB.plan.steps_run) * spec.U * 10.0 + double(DB.plan.steps_run) * 4.0 + double(M) * 12.0;
-  const double sb_chunk = double(kit_run) * 12.0 + double(DB.plan.eitems.size()) * 16.0;
-  const double sb_combine = double(M) * (8.0 + 16.0 + 16.0 + 4.0 + 8.0 + 16.0 + 8.0);
+  const double sb_chunk = double(kit_run) * 12.0 + double(DB.plan.wk_chunk.size()) * double(E.x.ncls_b) * 8.0 +
+                          double(DB.plan.eitems.size()) * 8.0 +
+                          double(int64_t(DB.plan.eitems.size()) - DB.plan.npairs) * 8.0;
+  const double sb_combine = double(M) * (8.0 + 16.0 + 8.0 + 16.0 + 4.0 + 8.0 + 4.0 + 8.0 + 8.0);
   const auto tx0 = clk::now();
   std::vector<int64_t> ans(static_cast<size_t>(M));
   HIPX(hipMemcpy(ans.data(), DB.b.answers, size_t(M) * sizeof(int64_t), hipMemcpyDeviceToHost));
@@ -316,7 +322,8 @@ int main(int argc, char** argv) {
       "{\"arm\":\"bwgpu_batched\",\"form\":4,\"S\":%" PRId64 ",\"U\":%d,\"N\":%" PRIu64 ",\"seed\":%" PRIu64
       ",\"batches\":%" PRId64 ",\"max_batch\":%d,\"n_answers\":%" PRId64 ",\"answers_sha256\":\"%s\","
       "\"plan\":{\"nblk\":%d,\"nqblk\":%zu,\"ngrp\":%zu,\"neitems\":%zu,\"nwork\":%zu,\"rmax\":%d,\"tile_shift\":%d,"
-      "\"chunk_items\":%" PRId64 ",\"ep_per_work\":%" PRId64 ",\"threads\":%d,\"waves\":%d,\"knobs_from\":\"%s\"},"
+      "\"chunk_items\":%" PRId64 ",\"ep_per_work\":%" PRId64 ",\"range_slots\":%" PRId64 ",\"npairs\":%" PRId64
+      ",\"threads\":%d,\"waves\":%d,\"knobs_from\":\"%s\"},"
       "\"phases_ms\":{\"generate\":%.3f,\"index_plan_upload\":%.3f,\"batch_plan\":%.3f,\"batch_upload\":%.3f,"
       "\"extract\":%.3f,\"index_build\":%.4f},"
       "\"graph_ms\":{\"avg\":%.4f,\"min\":%.4f,\"max\":%.4f,\"repeat\":%d,\"warmup\":%d},"
@@ -328,7 +335,8 @@ int main(int argc, char** argv) {
       "\"device_bytes\":%zu,\"device\":\"%s\",\"cus\":%d}\n",
       spec.S, spec.U, spec.N, spec.seed, ss.nreq, ss.max_batch, M, sha.c_str(), E.plan.nblk(), BP.qblk.size(),
       BP.grp_step.size(), BP.eitems.size(), BP.wk_chunk.size(), E.plan.knobs.rmax, E.plan.knobs.tile_shift,
-      E.plan.knobs.chunk_items, E.plan.knobs.ep_per_work, L.block_threads, L.chunk_waves, tuned_from.c_str(),
+      E.plan.knobs.chunk_items, E.plan.knobs.ep_per_work, E.plan.knobs.range_slots, BP.npairs, L.block_threads,
+      L.chunk_waves, tuned_from.c_str(),
       ms_between(t0, t1), ms_between(t1, t2), ms_between(t2, t3), ms_between(t3, t4), ms_between(tx0, tx1), index_ms,
       avg, run_ms.front(), run_ms.back(), repeat, warmup, double(M) / (avg * 1e-3), st_ms[0], st_ms[1], st_ms[2],
       sb_block, sb_chunk, sb_combine, K, plain_q, plain_pass_ms,

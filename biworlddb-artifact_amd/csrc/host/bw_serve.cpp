@@ -82,8 +82,8 @@ struct Slot {
   unsigned char* host = nullptr;  // pinned: staged inputs, then the answers
   size_t bytes = 0;
   // region offsets
-  size_t o_qlo, o_qhi, o_qblk, o_qblk_g0, o_grp_step, o_grp_q0, o_qorder, o_islot, o_qrank, o_sblk, o_qtile, o_wk_chunk,
-      o_wk_e0, o_eitems, o_ov_qid, o_ov_state, o_ansr, o_g, o_answers;
+  size_t o_qlo, o_qhi, o_qblk, o_qblk_g0, o_grp_step, o_grp_q0, o_qorder, o_spos, o_ipos, o_qrank, o_sblk, o_qtile, o_wk_chunk,
+      o_wk_e0, o_eitems, o_ov_qid, o_ov_state, o_ansr, o_g, o_stage, o_answers;
   // the batch in flight
   bool busy = false;
   std::vector<int64_t> reqs;     // request ids
@@ -109,7 +109,8 @@ size_t layout(Slot& s, int64_t cap, int32_t nblk) {
   s.o_grp_step = take(size_t(cap) * 4);
   s.o_grp_q0 = take((size_t(cap) + 1) * 8);
   s.o_qorder = take(size_t(cap) * 4);
-  s.o_islot = take(size_t(cap) * 4);
+  s.o_spos = take(size_t(cap) * 4);
+  s.o_ipos = take(size_t(cap) * 4);
   s.o_qrank = take(size_t(cap) * 4);
   s.o_sblk = take(size_t(cap) * 4);
   s.o_qtile = take(size_t(cap) * 4);
@@ -120,6 +121,7 @@ size_t layout(Slot& s, int64_t cap, int32_t nblk) {
   s.o_ov_state = take(size_t(cap) * 4);
   s.o_ansr = take(size_t(cap) * 8);
   s.o_g = take(size_t(cap) * 16);
+  s.o_stage = take(size_t(cap) * 8);
   s.o_answers = take(size_t(cap) * 8);
   return o;
 }
@@ -268,7 +270,8 @@ int main(int argc, char** argv) {
     put(s.o_grp_step, bp.grp_step.data(), bp.grp_step.size() * 4);
     put(s.o_grp_q0, bp.grp_q0.data(), bp.grp_q0.size() * 8);
     put(s.o_qorder, bp.qorder.data(), size_t(M) * 4);
-    put(s.o_islot, bp.islot.data(), size_t(M) * 4);
+    put(s.o_spos, bp.spos.data(), size_t(M) * 4);
+    put(s.o_ipos, bp.ipos.data(), size_t(M) * 4);
     put(s.o_qrank, bp.qrank.data(), size_t(M) * 4);
     put(s.o_sblk, bp.sblk.data(), size_t(M) * 4);
     put(s.o_qtile, bp.qtile.data(), size_t(M) * 4);
@@ -288,7 +291,8 @@ int main(int argc, char** argv) {
     b.grp_step = reinterpret_cast<const int32_t*>(s.dev + s.o_grp_step);
     b.grp_q0 = reinterpret_cast<const int64_t*>(s.dev + s.o_grp_q0);
     b.qorder = reinterpret_cast<const uint32_t*>(s.dev + s.o_qorder);
-    b.islot = reinterpret_cast<const uint32_t*>(s.dev + s.o_islot);
+    b.spos = reinterpret_cast<const uint32_t*>(s.dev + s.o_spos);
+    b.ipos = reinterpret_cast<const uint32_t*>(s.dev + s.o_ipos);
     b.qrank = reinterpret_cast<const uint32_t*>(s.dev + s.o_qrank);
     b.sblk = reinterpret_cast<const int32_t*>(s.dev + s.o_sblk);
     b.qtile = reinterpret_cast<const uint32_t*>(s.dev + s.o_qtile);
@@ -298,6 +302,7 @@ int main(int argc, char** argv) {
     b.eitems = reinterpret_cast<const uint64_t*>(s.dev + s.o_eitems);
     b.ansr = reinterpret_cast<int64_t*>(s.dev + s.o_ansr);
     b.g = reinterpret_cast<int64_t*>(s.dev + s.o_g);
+    b.stage = reinterpret_cast<int64_t*>(s.dev + s.o_stage);
     b.answers = reinterpret_cast<int64_t*>(s.dev + s.o_answers);
     HIPX(hipStreamWaitEvent(s.st, app_ready, 0));  // appended states admitted so far are on the device
     BwtLaunch L = E.launch;

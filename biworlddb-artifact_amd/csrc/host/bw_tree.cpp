@@ -95,6 +95,7 @@ int main(int argc, char** argv) {
     else if (!std::strcmp(k, "--tile-shift")) kn.tile_shift = std::atoi(v);
     else if (!std::strcmp(k, "--chunk")) kn.chunk_items = std::atoll(v);
     else if (!std::strcmp(k, "--ep-per-work")) kn.ep_per_work = std::atoll(v);
+    else if (!std::strcmp(k, "--range-slots")) kn.range_slots = std::atoll(v);
     else if (!std::strcmp(k, "--seg")) kn.seg_chunks = std::atoi(v);
     else if (!std::strcmp(k, "--max-block-steps")) kn.max_block_steps = std::atoi(v);
     else if (!std::strcmp(k, "--threads")) threads = std::atoi(v);
@@ -293,9 +294,12 @@ int main(int argc, char** argv) {
   //    out) and tiles (u16 in, i64 out)
   //  chunk histogram: every key item (u64) + its i64 weight, level-B weights out in chain
   //    order, the dense i64 rows out
-  //  carry scan: the rows in and out; chunk replay: walked key items + weights, every endpoint
-  //    item (u64) + its carry (i64) + the result (i64); combine per query: order, block, 2
-  //    endpoints, in-block part, 2 level-B carries, 2 level-C row reads, the answer
+  //  carry scan: the rows in and out; chunk replay: per walked key item its class word (u32) +
+  //    weight (i64), per work item its chunk's carry row, per endpoint item (u64) its read and
+  //    per stored result (one per endpoint, one per same-tile pair) an i64; combine per slot:
+  //    block and tiles (2 x u32), 2 level-C row reads, the in-block part, the 2 level-B words,
+  //    its staging position (u32) and staged value; per query id: its staging position (u32),
+  //    the staged value, the answer
   int64_t steps_run = mode == "pass" ? int64_t(P.steps.size()) : BP.steps_run, kit_run = 0;
   for (size_t i = 0; i < BP.wk_chunk.size(); ++i) {
     const int64_t c = BP.wk_chunk[i];
@@ -303,14 +307,16 @@ int main(int argc, char** argv) {
                                  int64_t(BP.eitems[size_t(BP.wk_e0[i + 1] - 1)] >> BWT_EP_POS_SHIFT));
   }
   const double nkeys_d = double(P.bkey.size()), ntile_d = double(P.tile_rank0.size());
+  const int x_ncls_b = P.chain_ncls.size() > 1 ? P.chain_ncls[1] : 1;
   const double sb_remap = cfg == 3 ? double(nq) * 16.0 : 0.0;
   const double sb_block = double(steps_run) * spec.U * 10.0 + double(steps_run) * 4.0 + double(nq) * 12.0 +
                           (mode == "pass" ? nkeys_d * 12.0 + ntile_d * 10.0 : 0.0);
   const double nkit_b = double(P.kitems.size()) - ntile_d;
   const double sb_hist = mode == "pass" ? double(P.kitems.size()) * 16.0 + nkit_b * 8.0 + double(P.n_cp) * 8.0 : 0.0;
   const double sb_scan = mode == "pass" ? double(P.n_cp) * 16.0 : 0.0;
-  const double sb_replay = double(kit_run) * 16.0 + double(BP.eitems.size()) * 24.0;
-  const double sb_combine = double(nq) * 64.0;
+  const double sb_replay = double(kit_run) * 12.0 + double(BP.wk_chunk.size()) * double(x_ncls_b) * 8.0 +
+                           double(BP.eitems.size()) * 8.0 + double(int64_t(BP.eitems.size()) - BP.npairs) * 8.0;
+  const double sb_combine = double(nq) * 80.0;
   char buf[8192];
   int n = std::snprintf(
       buf, sizeof(buf),

@@ -70,16 +70,17 @@ int main(int argc, char** argv) {
   for (const TuneTrial& t : log)
     std::fprintf(stderr,
                  "{\"rmax\":%d,\"tile_shift\":%d,\"chunk_items\":%" PRId64 ",\"ep_per_work\":%" PRId64
-                 ",\"block_threads\":%d,\"chunk_waves\":%d,\"ms\":%.4f}\n",
-                 t.kn.rmax, t.kn.tile_shift, t.kn.chunk_items, t.kn.ep_per_work, t.lk.block_threads, t.lk.chunk_waves,
-                 t.ms);
+                 ",\"range_slots\":%" PRId64 ",\"block_threads\":%d,\"chunk_waves\":%d,\"ms\":%.4f}\n",
+                 t.kn.rmax, t.kn.tile_shift, t.kn.chunk_items, t.kn.ep_per_work, t.kn.range_slots, t.lk.block_threads,
+                 t.lk.chunk_waves, t.ms);
   if (best.ms <= 0) { std::fprintf(stderr, "bw_tune: no configuration ran\n"); return 5; }
   hipDeviceProp_t prop{};
   (void)hipGetDeviceProperties(&prop, 0);
   if (!save_tuned(out.c_str(), form.c_str(), best.kn, best.lk, prop.gcnArchName, best.ms)) return 4;
   std::printf("{\"form\":\"%s\",\"sample_queries\":%zu,\"trials\":%zu,\"rmax\":%d,\"tile_shift\":%d,\"chunk_items\":%" PRId64
-              ",\"ep_per_work\":%" PRId64 ",\"block_threads\":%d,\"chunk_waves\":%d,\"ms\":%.4f}\n",
+              ",\"ep_per_work\":%" PRId64 ",\"range_slots\":%" PRId64 ",\"block_threads\":%d,\"chunk_waves\":%d,"
+              "\"ms\":%.4f}\n",
               form.c_str(), qs.size(), log.size(), best.kn.rmax, best.kn.tile_shift, best.kn.chunk_items,
-              best.kn.ep_per_work, best.lk.block_threads, best.lk.chunk_waves, best.ms);
+              best.kn.ep_per_work, best.kn.range_slots, best.lk.block_threads, best.lk.chunk_waves, best.ms);
   return 0;
 }

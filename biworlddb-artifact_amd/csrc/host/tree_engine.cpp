@@ -116,7 +116,6 @@ bool upload_batch(const TreeEngine& e, BatchPlan&& plan, const uint32_t* d_qlo, 
   b.grp_step = out.arena.upload(B.grp_step);
   b.grp_q0 = out.arena.upload(B.grp_q0);
   b.qorder = out.arena.upload(B.qorder);
-  b.islot = out.arena.upload(B.islot);
   b.qrank = out.arena.upload(B.qrank);
   b.sblk = out.arena.upload(B.sblk);
   b.qtile = out.arena.upload(B.qtile);
@@ -124,8 +123,11 @@ bool upload_batch(const TreeEngine& e, BatchPlan&& plan, const uint32_t* d_qlo, 
   b.wk_chunk = out.arena.upload(B.wk_chunk);
   b.wk_e0 = out.arena.upload(B.wk_e0);
   b.eitems = out.arena.upload(B.eitems);
+  b.spos = out.arena.upload(B.spos);
+  b.ipos = out.arena.upload(B.ipos);
   b.ansr = out.arena.scratch<int64_t>(size_t(B.M));
   b.g = out.arena.scratch<int64_t>(2 * size_t(B.M));
+  b.stage = out.arena.scratch<int64_t>(size_t(B.M));
   b.answers = out.arena.scratch<int64_t>(size_t(B.M));
   (void)e;
   if (!out.arena.ok()) {

@@ -323,6 +323,26 @@ bool build_batch_plan(const TreeIndexPlan& X, int64_t M, const int32_t* qstate, 
       B.islot[size_t(q)] = uint32_t(at);
     }
   }
+  // ---- staging positions of the combine: grouped by window of query ids, then by the die part
+  //      of the slot (the combine's XCD-major workgroup order), then slot
+  {
+    const int64_t nb = (M + BWT_COMBINE_THREADS - 1) / BWT_COMBINE_THREADS;
+    const int64_t per = std::max<int64_t>(1, (nb + BWT_XCDS - 1) / BWT_XCDS);
+    const int64_t nwin = (M >> kStageShift) + 1;
+    auto seg = [&](int64_t j) {
+      return (int64_t(B.qorder[size_t(j)]) >> kStageShift) * BWT_XCDS + (j / BWT_COMBINE_THREADS) / per;
+    };
+    std::vector<int64_t> base(size_t(nwin * BWT_XCDS) + 1, 0);
+    for (int64_t j = 0; j < M; ++j) base[size_t(seg(j)) + 1]++;
+    for (size_t k = 1; k < base.size(); ++k) base[k] += base[k - 1];
+    B.spos.resize(size_t(M));
+    B.ipos.resize(size_t(M));
+    for (int64_t j = 0; j < M; ++j) {
+      const uint32_t at = uint32_t(base[size_t(seg(j))]++);
+      B.spos[size_t(j)] = at;
+      B.ipos[size_t(B.qorder[size_t(j)])] = at;
+    }
+  }
   // ---- listed blocks and their step groups (the queried steps, ascending); with all_blocks,
   //      blocks without queries are listed too (with no groups)
   B.qblk_g0.push_back(0);
@@ -391,7 +411,7 @@ bool build_batch_plan(const TreeIndexPlan& X, int64_t M, const int32_t* qstate, 
       int64_t k = pos / CH;
       if (k >= n) k = n - 1;
       ech[size_t(2 * j + side)] = X.chain_chunk0[size_t(chain)] + k;
-      eit[size_t(2 * j + side)] = ep_item(uint64_t(pos - k * CH), x & mask, uint64_t(2 * q) + uint64_t(side));
+      eit[size_t(2 * j + side)] = ep_item(uint64_t(pos - k * CH), x & mask, uint64_t(2 * j) + uint64_t(side));
     }
   }
   // ---- group by chunk, each chunk's items by position (ties keep slot order)
@@ -460,14 +480,53 @@ bool build_batch_plan(const TreeIndexPlan& X, int64_t M, const int32_t* qstate, 
     cstart.push_back(nE);
   }
   // work items: each chunk's endpoints in runs of at most ep_per_work (a work item replays the
-  // chunk's keys only up to its own last endpoint)
-  B.wk_e0.push_back(0);
-  const int64_t G = X.knobs.ep_per_work;
-  for (size_t i = 0; i < cfirst.size(); ++i)
-    for (int64_t a = cstart[i]; a < cstart[i + 1]; a += G) {
-      B.wk_chunk.push_back(cfirst[i]);
-      B.wk_e0.push_back(std::min(a + G, cstart[i + 1]));
+  // chunk's keys only up to its own last endpoint), cut where the slot range changes and never
+  // between the two endpoints of one slot; ordered by slot range, then chunk.  A chunk's items
+  // are in slot order (position follows the block, ties keep slot order), so each range is one
+  // run of a chunk's items.
+  const int64_t G = std::max<int64_t>(2, X.knobs.ep_per_work);
+  const int64_t RS = std::max<int64_t>(1, X.knobs.range_slots);
+  auto slot_of = [&](int64_t e) { return int64_t(uint32_t(B.eitems[size_t(e)])) >> 1; };
+  struct Work {
+    int64_t range, chunk, e0, e1;
+  };
+  std::vector<Work> wk;
+  for (size_t i = 0; i < cfirst.size(); ++i) {
+    int64_t a = cstart[i];
+    const int64_t e = cstart[i + 1];
+    while (a < e) {
+      const int64_t r = slot_of(a) / RS;
+      int64_t b = std::min(a + G, e);
+      if (slot_of(b - 1) / RS != r) b = std::partition_point(B.eitems.begin() + a, B.eitems.begin() + b, [&](uint64_t it) {
+                                          return (int64_t(uint32_t(it)) >> 1) / RS == r;
+                                        }) - B.eitems.begin();
+      if (b < e && slot_of(b) == slot_of(b - 1)) ++b;  // keep a slot's pair together
+      wk.push_back(Work{r, cfirst[i], a, b});
+      a = b;
     }
+  }
+  for (const Work& w : wk)  // pairs: a slot's two endpoints on one chain (same tile), adjacent
+    for (int64_t i = w.e0; i + 1 < w.e1; ++i) {
+      const uint32_t id = uint32_t(B.eitems[size_t(i)]);
+      if (!(id & 1u) && uint32_t(B.eitems[size_t(i) + 1]) == id + 1) ++B.npairs;
+    }
+  const bool ranged = !wk.empty() && wk.back().range != wk.front().range;
+  if (ranged) std::stable_sort(wk.begin(), wk.end(), [](const Work& p, const Work& q) { return p.range < q.range; });
+  B.wk_chunk.resize(wk.size());
+  B.wk_e0.resize(wk.size() + 1);
+  B.wk_e0[0] = 0;
+  for (size_t i = 0; i < wk.size(); ++i) {
+    B.wk_chunk[i] = wk[i].chunk;
+    B.wk_e0[i + 1] = B.wk_e0[i] + (wk[i].e1 - wk[i].e0);
+  }
+  if (ranged) {  // lay the items out in work order
+    std::vector<uint64_t> out(B.eitems.size());
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < int64_t(wk.size()); ++i)
+      std::copy(B.eitems.begin() + wk[size_t(i)].e0, B.eitems.begin() + wk[size_t(i)].e1,
+                out.begin() + B.wk_e0[size_t(i)]);
+    B.eitems.swap(out);
+  }
   return true;
 }
 

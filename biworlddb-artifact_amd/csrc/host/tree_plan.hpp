@@ -20,6 +20,9 @@ struct PlanKnobs {
   int64_t chunk_items = 0;      // key items per carry chunk on level B (0 = auto); level C has
                                 // one chunk per block
   int64_t ep_per_work = 2048;   // endpoints per chunk work item (batch plans)
+  int64_t range_slots = 1 << 22;  // batch plans: work items are ordered by ranges of this many
+                                  // query slots, so the level-B results written at any time fall
+                                  // in one window of the slot-ordered result array
   int seg_chunks = 32;          // chunks per carry-scan segment
   int max_block_steps = 1 << 30;
 };
@@ -53,6 +56,9 @@ struct TreeIndexPlan {
   int32_t nchain() const { return int32_t(chain_ncls.size()); }
 };
 
+// Query ids per staging window of the combine (2^18 answers = 2 MiB).
+constexpr int kStageShift = 18;
+
 struct BatchPlan {
   int64_t M = 0;
   std::vector<int32_t> qblk;      // listed blocks
@@ -61,11 +67,15 @@ struct BatchPlan {
   std::vector<int64_t> grp_q0;    // its slots
   std::vector<uint32_t> qorder;   // slot -> query id
   std::vector<uint32_t> islot;    // query id -> slot
+  std::vector<uint32_t> spos;     // slot -> staging position (windows of 2^kStageShift query ids,
+  std::vector<uint32_t> ipos;     //   then die part, then slot); query id -> staging position
   std::vector<uint32_t> qrank;    // slot -> in-block ranks lo | hi << 16
   std::vector<int32_t> sblk;      // slot -> block
   std::vector<uint32_t> qtile;    // slot -> key tiles of the endpoints lo | hi << 16
   std::vector<int64_t> wk_chunk, wk_e0;
-  std::vector<uint64_t> eitems;   // level B only, endpoint id = 2 * query id + side
+  std::vector<uint64_t> eitems;   // level B only, endpoint id = 2 * slot + side
+  int64_t npairs = 0;             // slots whose two endpoints share a key tile (one level-B item
+                                  // pair, adjacent in item order and in one work item)
   int64_t steps_run = 0;          // block-replay steps the batch needs (statistics)
 };
 

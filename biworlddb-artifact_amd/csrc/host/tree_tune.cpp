@@ -52,9 +52,10 @@ std::string entry_text(const PlanKnobs& kn, const LaunchKnobs& lk, const char* d
   char b[512];
   std::snprintf(b, sizeof(b),
                 "{\"rmax\": %d, \"tile_shift\": %d, \"chunk_items\": %lld, \"ep_per_work\": %lld, "
-                "\"block_threads\": %d, \"chunk_waves\": %d, \"ms\": %.4f, \"device\": \"%s\"}",
-                kn.rmax, kn.tile_shift, (long long)kn.chunk_items, (long long)kn.ep_per_work, lk.block_threads,
-                lk.chunk_waves, ms, device ? device : "");
+                "\"range_slots\": %lld, \"block_threads\": %d, \"chunk_waves\": %d, \"ms\": %.4f, "
+                "\"device\": \"%s\"}",
+                kn.rmax, kn.tile_shift, (long long)kn.chunk_items, (long long)kn.ep_per_work,
+                (long long)kn.range_slots, lk.block_threads, lk.chunk_waves, ms, device ? device : "");
   return b;
 }
 
@@ -85,6 +86,7 @@ bool load_tuned(const char* path, const char* form, PlanKnobs& kn, LaunchKnobs& 
   if (number_of(obj, "tile_shift", v)) k.tile_shift = int(v);
   if (number_of(obj, "chunk_items", v)) k.chunk_items = int64_t(v);
   if (number_of(obj, "ep_per_work", v)) k.ep_per_work = int64_t(v);
+  if (number_of(obj, "range_slots", v)) k.range_slots = int64_t(v);
   if (number_of(obj, "block_threads", v)) l.block_threads = int(v);
   if (number_of(obj, "chunk_waves", v)) l.chunk_waves = int(v);
   kn = k;
@@ -137,10 +139,11 @@ TuneTrial autotune(const TreeWorkload& w, const uint32_t* kcoord, int64_t M, con
       eng_kn = k;
       if (eng->build_index(tm.st) != 0) return -1;
     }
-    if (!have_db || k.ep_per_work != db_kn.ep_per_work) {
+    if (!have_db || k.ep_per_work != db_kn.ep_per_work || k.range_slots != db_kn.range_slots) {
       TreeIndexPlan& P = eng->plan;
       const PlanKnobs saved = P.knobs;
       P.knobs.ep_per_work = k.ep_per_work;
+      P.knobs.range_slots = k.range_slots;
       BatchPlan bp;
       const bool ok = build_batch_plan(P, M, qstate, qlo, qhi, false, bp, err, sizeof(err));
       P.knobs = saved;
@@ -173,23 +176,27 @@ TuneTrial autotune(const TreeWorkload& w, const uint32_t* kcoord, int64_t M, con
   }
   log.push_back(cur);
   // candidate values per knob (the current value is always among the tried ones)
-  const int rmax_c[] = {1024, 2048, 4096};
-  const int ts_c[] = {9, 10, 11};
-  const int64_t chunk_c[] = {2048, 4096, 8192};
-  const int64_t epw_c[] = {2048, 8192, 32768};
-  const int thr_c[] = {128, 256, 512};
-  const int wav_c[] = {1, 2, 4};
-  for (int knob = 0; knob < 6; ++knob) {
-    for (int i = 0; i < 3; ++i) {
+  const int rmax_c[] = {1024, 2048, 4096, 0};
+  const int ts_c[] = {9, 10, 11, 0};
+  const int64_t chunk_c[] = {512, 1024, 2048, 4096};
+  const int64_t epw_c[] = {2048, 8192, 32768, 0};
+  const int64_t rs_c[] = {int64_t(1) << 20, int64_t(1) << 21, int64_t(1) << 22, int64_t(1) << 24};
+  const int thr_c[] = {128, 256, 512, 0};
+  const int wav_c[] = {1, 2, 4, 0};
+  for (int knob = 0; knob < 7; ++knob) {
+    for (int i = 0; i < 4; ++i) {
       TuneTrial t = cur;
+      bool none = false;
       switch (knob) {
-        case 0: t.kn.rmax = rmax_c[i]; break;
-        case 1: t.kn.tile_shift = ts_c[i]; break;
+        case 0: none = !rmax_c[i]; t.kn.rmax = rmax_c[i]; break;
+        case 1: none = !ts_c[i]; t.kn.tile_shift = ts_c[i]; break;
         case 2: t.kn.chunk_items = chunk_c[i]; break;
-        case 3: t.kn.ep_per_work = epw_c[i]; break;
-        case 4: t.lk.block_threads = thr_c[i]; break;
-        default: t.lk.chunk_waves = wav_c[i]; break;
+        case 3: none = !epw_c[i]; t.kn.ep_per_work = epw_c[i]; break;
+        case 4: t.kn.range_slots = rs_c[i]; break;
+        case 5: none = !thr_c[i]; t.lk.block_threads = thr_c[i]; break;
+        default: none = !wav_c[i]; t.lk.chunk_waves = wav_c[i]; break;
       }
+      if (none) continue;
       if (std::memcmp(&t.kn, &cur.kn, sizeof(PlanKnobs)) == 0 && t.lk.block_threads == cur.lk.block_threads &&
           t.lk.chunk_waves == cur.lk.chunk_waves)
         continue;

@@ -381,6 +381,9 @@ __global__ void __launch_bounds__(256) bwt_chunk_replay_kernel(const BwtIndex x,
   uint64_t it = (ep + lane < e1) ? q.eitems[ep + lane] : 0;
   uint32_t kx = (ki + lane < kend) ? khi[2 * (ki + lane)] : 0;
   int64_t kw = (ki + lane < kend) ? x.kweight[ki + lane] : 0;
+  // the endpoint consumed last (the previous group's last taken lane): id and value
+  uint32_t last_id = 0xFFFFFFFFu;
+  int64_t last_val = 0;
   wave_lds_sync();
   while (true) {
     const bool ev = ep + lane < e1;
@@ -422,7 +425,23 @@ __global__ void __launch_bounds__(256) bwt_chunk_replay_kernel(const BwtIndex x,
         if (m + 2 < rl && c4.z < cls) val += w23.x;
         if (m + 3 < rl && c4.w < cls) val += w23.y;
       }
-      if (take) q.g[it & 0xFFFFFFFFull] = val;
+      // A slot whose two endpoints lie on this chain (same key tile, hence the same position)
+      // has them as adjacent items, lo first: the hi lane stores hi - lo, the lo lane nothing.
+      const uint32_t id = (uint32_t)it;
+      const uint32_t up_id = (uint32_t)__shfl_up((int)id, 1);
+      const int64_t up_val = (int64_t)(((uint64_t)(uint32_t)__shfl_up((int)(uint32_t)((uint64_t)val >> 32), 1) << 32) |
+                                       (uint64_t)(uint32_t)__shfl_up((int)(uint32_t)(uint64_t)val, 1));
+      const uint32_t dn_id = (uint32_t)__shfl_down((int)id, 1);
+      const uint32_t next_first = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)it2);
+      const uint32_t prev_id = lane > 0 ? up_id : last_id;
+      const int64_t prev_val = lane > 0 ? up_val : last_val;
+      const uint32_t next_id = lane < 63 ? dn_id : next_first;
+      const bool has_next = ep + lane + 1 < e1;
+      const bool lo_pair = !(id & 1u) && has_next && next_id == id + 1;
+      const bool hi_pair = (id & 1u) && prev_id == id - 1;
+      if (take && !lo_pair) q.g[id] = hi_pair ? val - prev_val : val;
+      last_id = (uint32_t)__builtin_amdgcn_readlane((int)id, ne - 1);
+      last_val = readlane64(val, ne - 1);
     }
     if (ep2 >= e1) break;
     if (lane < nk)
@@ -437,13 +456,21 @@ __global__ void __launch_bounds__(256) bwt_chunk_replay_kernel(const BwtIndex x,
 }
 
 // ---------------------------------------------------------------- combine
-// Two passes, so that no answer is stored to a scattered position: slots (block order) add the
-// level-C carry, read from the dense carry row of the slot's block (level-C chunk b = block b)
-// at the endpoints' key tiles, to the in-block part -- every read streams and the rows stay in
-// cache; then query ids gather their slot's value and add their level-B carries, which the
-// chunk replay stored by query id, and store the answers in order.
-__global__ void __launch_bounds__(256) bwt_combine_slots_kernel(const BwtIndex x, const BwtBatch q) {
-  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// Two passes.  Slots (block order) add the level-C carry, read from the dense carry row of the
+// slot's block (level-C chunk b = block b) at the endpoints' key tiles, and the level-B carries,
+// which the chunk replay stored by slot, to the in-block part -- every read streams and the rows
+// stay in cache -- and store the value at the slot's staging position.  The staging buffer is
+// grouped by windows of query ids (then by die part, then slot), so these stores fill each
+// window's lines in order.  Then query ids, each die taking its part, load their value from
+// their window (a few MiB, resident in that die's cache) and store the answers in order.
+// XCD-major block of this workgroup (see BWT_XCDS): grid = BWT_XCDS * per workgroups
+__device__ __forceinline__ int64_t xcd_major_block(int64_t per) {
+  return (int64_t)(blockIdx.x % BWT_XCDS) * per + blockIdx.x / BWT_XCDS;
+}
+
+__global__ void __launch_bounds__(BWT_COMBINE_THREADS) bwt_combine_slots_kernel(const BwtIndex x, const BwtBatch q,
+                                                                                int64_t per) {
+  const int64_t s = xcd_major_block(per) * BWT_COMBINE_THREADS + threadIdx.x;
   if (s >= q.M) return;
   const int64_t* row = x.cp + x.chunk_cp0[q.sblk[s]];
   uint32_t tl;
@@ -453,13 +480,16 @@ __global__ void __launch_bounds__(256) bwt_combine_slots_kernel(const BwtIndex x
     const uint32_t id = q.qorder[s];
     tl = (q.qlo[id] >> x.tile_shift) | ((q.qhi[id] >> x.tile_shift) << 16);
   }
-  q.ansr[s] += row[tl >> 16] - row[tl & 0xFFFFu];
+  // level B, stored by slot; endpoints on one chain stored their difference in the hi word
+  const longlong2 gb = *reinterpret_cast<const longlong2*>(q.g + 2 * s);
+  const int64_t lb = (tl >> 16) == (tl & 0xFFFFu) ? gb.y : gb.y - gb.x;
+  q.stage[q.spos[s]] = q.ansr[s] + (row[tl >> 16] - row[tl & 0xFFFFu]) + lb;
 }
 
-__global__ void __launch_bounds__(256) bwt_combine_ids_kernel(const BwtBatch q) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(BWT_COMBINE_THREADS) bwt_combine_ids_kernel(const BwtBatch q, int64_t per) {
+  const int64_t i = xcd_major_block(per) * BWT_COMBINE_THREADS + threadIdx.x;
   if (i >= q.M) return;
-  q.answers[i] = q.ansr[q.islot[i]] + (q.g[2 * i + 1] - q.g[2 * i]);
+  q.answers[i] = q.stage[q.ipos[i]];
 }
 
 // ---------------------------------------------------------------- overlay (appended states)
@@ -641,11 +671,14 @@ extern "C" int bwt_combine_launch(const BwtIndex* x, const BwtBatch* b, const Bw
   (void)l;
   if (!x || !b || b->M < 0) return (int)hipErrorInvalidValue;
   if (b->M == 0) return 0;
-  const dim3 grid((unsigned)((b->M + 255) / 256));
-  hipLaunchKernelGGL(bwt_combine_slots_kernel, grid, dim3(256), 0, (hipStream_t)stream, *x, *b);
+  if (!b->spos || !b->ipos || !b->stage) return (int)hipErrorInvalidValue;
+  const int64_t nb = (b->M + BWT_COMBINE_THREADS - 1) / BWT_COMBINE_THREADS;
+  const int64_t per = (nb + BWT_XCDS - 1) / BWT_XCDS;
+  const dim3 grid((unsigned)(per * BWT_XCDS));
+  hipLaunchKernelGGL(bwt_combine_slots_kernel, grid, dim3(BWT_COMBINE_THREADS), 0, (hipStream_t)stream, *x, *b, per);
   int rc = report("combine (slots)", hipGetLastError());
   if (rc) return rc;
-  hipLaunchKernelGGL(bwt_combine_ids_kernel, grid, dim3(256), 0, (hipStream_t)stream, *b);
+  hipLaunchKernelGGL(bwt_combine_ids_kernel, grid, dim3(BWT_COMBINE_THREADS), 0, (hipStream_t)stream, *b, per);
   return report("combine (ids)", hipGetLastError());
 }
 

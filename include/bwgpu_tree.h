@@ -104,7 +104,6 @@ typedef struct BwtBatch {
   const int32_t* grp_step;    /* ngrp global enter steps */
   const int64_t* grp_q0;      /* ngrp+1 offsets into the slots */
   const uint32_t* qorder;     /* M: query id of each slot */
-  const uint32_t* islot;      /* M: slot of each query id (the inverse of qorder) */
   const uint32_t* qrank;      /* M: the slot's endpoints' in-block ranks, lo | hi << 16 */
   const int32_t* sblk;        /* M: block of each slot */
   const uint32_t* qtile;      /* M: the slot's endpoints' key tiles (level-C classes), lo | hi << 16;
@@ -112,11 +111,23 @@ typedef struct BwtBatch {
   int64_t nwork;              /* level-B work items: a chunk and a run of its endpoints */
   const int64_t* wk_chunk;    /* nwork chunk ids */
   const int64_t* wk_e0;       /* nwork+1 offsets into eitems */
-  const uint64_t* eitems;     /* level-B endpoint items per chunk, by position; id = 2*q+side */
-  int64_t* ansr;              /* M, by slot: in-block part, then + level-C carry */
-  int64_t* g;                 /* 2M, by endpoint id (2*q+side): level-B carry */
+  const uint64_t* eitems;     /* level-B endpoint items per work item, by position; id = 2*slot+side */
+  int64_t* ansr;              /* M, by slot: in-block part */
+  int64_t* g;                 /* 2M, by endpoint id (2*slot+side): level-B carry; a slot whose
+                                 endpoints share a key tile holds hi - lo at 2*slot+1 */
+  const uint32_t* spos;       /* M: staging position of each slot's finished value */
+  const uint32_t* ipos;       /* M: staging position of each query id's value (= spos of its slot) */
+  int64_t* stage;             /* M staging buffer: finished values grouped by windows of query ids */
   int64_t* answers;           /* M, by query id */
 } BwtBatch;
+
+/* The combine kernels map workgroups to slots / query ids XCD-major: on a part with BWT_XCDS
+ * accelerator dies that take workgroups round robin, die x handles the x-th of BWT_XCDS
+ * contiguous parts of the BWT_COMBINE_THREADS-wide blocks.  The planner groups the staging
+ * buffer by (query-id window, part) so each die's staging stores and loads stay in its own
+ * cache (the mapping affects speed only, never results). */
+#define BWT_XCDS 8
+#define BWT_COMBINE_THREADS 256
 
 typedef struct BwtLaunch {
   int32_t block_threads;  /* block replay workgroup (multiple of 64, <= 1024) */

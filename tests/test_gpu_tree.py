@@ -22,6 +22,10 @@ CASES = [
     ("t6_cfg1_single", []),
     ("t7_cfg2_deep", ["--rmax", "512", "--chunk", "4096"]),
     ("c1_cfg1", []),
+    # wide key tiles (many slots with both endpoints on one level-B chain: paired results), work
+    # items cut at odd sizes and at small slot ranges (range-ordered work)
+    ("c1_cfg1", ["--tile-shift", "12", "--ep-per-work", "65", "--range-slots", "100"]),
+    ("t2_cfg2_small", ["--tile-shift", "11", "--chunk", "64", "--ep-per-work", "67", "--range-slots", "333"]),
 ]
 
 
