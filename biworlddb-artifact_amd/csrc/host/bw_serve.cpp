@@ -82,7 +82,7 @@ struct Slot {
   unsigned char* host = nullptr;  // pinned: staged inputs, then the answers
   size_t bytes = 0;
   // region offsets
-  size_t o_qlo, o_qhi, o_qblk, o_qblk_g0, o_grp_step, o_grp_q0, o_qorder, o_spos, o_ipos, o_qrank, o_sblk, o_qtile, o_wk_chunk,
+  size_t o_qlo, o_qhi, o_qblk, o_qblk_g0, o_grp_step, o_grp_q0, o_qorder, o_spos, o_srank, o_ipos, o_qrank, o_sblk, o_qtile, o_wk_chunk,
       o_wk_e0, o_eitems, o_ov_qid, o_ov_state, o_ansr, o_g, o_stage, o_answers;
   // the batch in flight
   bool busy = false;
@@ -110,6 +110,7 @@ size_t layout(Slot& s, int64_t cap, int32_t nblk) {
   s.o_grp_q0 = take((size_t(cap) + 1) * 8);
   s.o_qorder = take(size_t(cap) * 4);
   s.o_spos = take(size_t(cap) * 4);
+  s.o_srank = take(size_t(cap) * 2);
   s.o_ipos = take(size_t(cap) * 4);
   s.o_qrank = take(size_t(cap) * 4);
   s.o_sblk = take(size_t(cap) * 4);
@@ -271,6 +272,7 @@ int main(int argc, char** argv) {
     put(s.o_grp_q0, bp.grp_q0.data(), bp.grp_q0.size() * 8);
     put(s.o_qorder, bp.qorder.data(), size_t(M) * 4);
     put(s.o_spos, bp.spos.data(), size_t(M) * 4);
+    put(s.o_srank, bp.srank.data(), size_t(M) * 2);
     put(s.o_ipos, bp.ipos.data(), size_t(M) * 4);
     put(s.o_qrank, bp.qrank.data(), size_t(M) * 4);
     put(s.o_sblk, bp.sblk.data(), size_t(M) * 4);
@@ -292,6 +294,7 @@ int main(int argc, char** argv) {
     b.grp_q0 = reinterpret_cast<const int64_t*>(s.dev + s.o_grp_q0);
     b.qorder = reinterpret_cast<const uint32_t*>(s.dev + s.o_qorder);
     b.spos = reinterpret_cast<const uint32_t*>(s.dev + s.o_spos);
+    b.srank = reinterpret_cast<const uint16_t*>(s.dev + s.o_srank);
     b.ipos = reinterpret_cast<const uint32_t*>(s.dev + s.o_ipos);
     b.qrank = reinterpret_cast<const uint32_t*>(s.dev + s.o_qrank);
     b.sblk = reinterpret_cast<const int32_t*>(s.dev + s.o_sblk);

@@ -96,6 +96,7 @@ int main(int argc, char** argv) {
     else if (!std::strcmp(k, "--chunk")) kn.chunk_items = std::atoll(v);
     else if (!std::strcmp(k, "--ep-per-work")) kn.ep_per_work = std::atoll(v);
     else if (!std::strcmp(k, "--range-slots")) kn.range_slots = std::atoll(v);
+    else if (!std::strcmp(k, "--stage-shift")) kn.stage_shift = std::atoi(v);
     else if (!std::strcmp(k, "--seg")) kn.seg_chunks = std::atoi(v);
     else if (!std::strcmp(k, "--max-block-steps")) kn.max_block_steps = std::atoi(v);
     else if (!std::strcmp(k, "--threads")) threads = std::atoi(v);

@@ -124,6 +124,7 @@ bool upload_batch(const TreeEngine& e, BatchPlan&& plan, const uint32_t* d_qlo, 
   b.wk_e0 = out.arena.upload(B.wk_e0);
   b.eitems = out.arena.upload(B.eitems);
   b.spos = out.arena.upload(B.spos);
+  b.srank = out.arena.upload(B.srank);
   b.ipos = out.arena.upload(B.ipos);
   b.ansr = out.arena.scratch<int64_t>(size_t(B.M));
   b.g = out.arena.scratch<int64_t>(2 * size_t(B.M));

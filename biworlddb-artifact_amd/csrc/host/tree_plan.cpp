@@ -326,21 +326,35 @@ bool build_batch_plan(const TreeIndexPlan& X, int64_t M, const int32_t* qstate, 
   // ---- staging positions of the combine: grouped by window of query ids, then by the die part
   //      of the slot (the combine's XCD-major workgroup order), then slot
   {
-    const int64_t nb = (M + BWT_COMBINE_THREADS - 1) / BWT_COMBINE_THREADS;
-    const int64_t per = std::max<int64_t>(1, (nb + BWT_XCDS - 1) / BWT_XCDS);
-    const int64_t nwin = (M >> kStageShift) + 1;
+    const int64_t nt = (M + BWT_COMBINE_TILE - 1) / BWT_COMBINE_TILE;
+    const int64_t per = std::max<int64_t>(1, (nt + BWT_XCDS - 1) / BWT_XCDS);
+    const int sh = std::min(31, std::max(8, X.knobs.stage_shift));
+    const int64_t nwin = (M >> sh) + 1;
     auto seg = [&](int64_t j) {
-      return (int64_t(B.qorder[size_t(j)]) >> kStageShift) * BWT_XCDS + (j / BWT_COMBINE_THREADS) / per;
+      return (int64_t(B.qorder[size_t(j)]) >> sh) * BWT_XCDS + (j / BWT_COMBINE_TILE) / per;
     };
     std::vector<int64_t> base(size_t(nwin * BWT_XCDS) + 1, 0);
     for (int64_t j = 0; j < M; ++j) base[size_t(seg(j)) + 1]++;
     for (size_t k = 1; k < base.size(); ++k) base[k] += base[k - 1];
-    B.spos.resize(size_t(M));
+    std::vector<uint32_t> at(static_cast<size_t>(M));
     B.ipos.resize(size_t(M));
     for (int64_t j = 0; j < M; ++j) {
-      const uint32_t at = uint32_t(base[size_t(seg(j))]++);
-      B.spos[size_t(j)] = at;
-      B.ipos[size_t(B.qorder[size_t(j)])] = at;
+      at[size_t(j)] = uint32_t(base[size_t(seg(j))]++);
+      B.ipos[size_t(B.qorder[size_t(j)])] = at[size_t(j)];
+    }
+    // each tile's positions ascending, and every slot's index in that list
+    B.spos.resize(size_t(M));
+    B.srank.resize(size_t(M));
+#pragma omp parallel for schedule(static) if (nt > 64)
+    for (int64_t t = 0; t < nt; ++t) {
+      const int64_t t0 = t * BWT_COMBINE_TILE, n = std::min<int64_t>(BWT_COMBINE_TILE, M - t0);
+      std::vector<uint64_t> key(static_cast<size_t>(n));
+      for (int64_t k = 0; k < n; ++k) key[size_t(k)] = (uint64_t(at[size_t(t0 + k)]) << 16) | uint64_t(k);
+      std::sort(key.begin(), key.end());
+      for (int64_t r = 0; r < n; ++r) {
+        B.spos[size_t(t0 + r)] = uint32_t(key[size_t(r)] >> 16);
+        B.srank[size_t(t0 + int64_t(key[size_t(r)] & 0xFFFFu))] = uint16_t(r);
+      }
     }
   }
   // ---- listed blocks and their step groups (the queried steps, ascending); with all_blocks,

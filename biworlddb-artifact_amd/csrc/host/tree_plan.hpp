@@ -24,6 +24,8 @@ struct PlanKnobs {
                                   // query slots, so the level-B results written at any time fall
                                   // in one window of the slot-ordered result array
   int seg_chunks = 32;          // chunks per carry-scan segment
+  int stage_shift = 18;         // batch plans: query ids per staging window of the combine
+                                // (2^18 answers = 2 MiB, held in one die's cache)
   int max_block_steps = 1 << 30;
 };
 
@@ -56,9 +58,6 @@ struct TreeIndexPlan {
   int32_t nchain() const { return int32_t(chain_ncls.size()); }
 };
 
-// Query ids per staging window of the combine (2^18 answers = 2 MiB).
-constexpr int kStageShift = 18;
-
 struct BatchPlan {
   int64_t M = 0;
   std::vector<int32_t> qblk;      // listed blocks
@@ -67,8 +66,10 @@ struct BatchPlan {
   std::vector<int64_t> grp_q0;    // its slots
   std::vector<uint32_t> qorder;   // slot -> query id
   std::vector<uint32_t> islot;    // query id -> slot
-  std::vector<uint32_t> spos;     // slot -> staging position (windows of 2^kStageShift query ids,
-  std::vector<uint32_t> ipos;     //   then die part, then slot); query id -> staging position
+  std::vector<uint32_t> spos;     // staging positions (windows of 2^stage_shift query ids, then
+                                  //   die part, then slot), ascending within each slot tile
+  std::vector<uint16_t> srank;    // slot -> index of its staging position in its tile's list
+  std::vector<uint32_t> ipos;     // query id -> staging position
   std::vector<uint32_t> qrank;    // slot -> in-block ranks lo | hi << 16
   std::vector<int32_t> sblk;      // slot -> block
   std::vector<uint32_t> qtile;    // slot -> key tiles of the endpoints lo | hi << 16

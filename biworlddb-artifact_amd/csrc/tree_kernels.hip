@@ -468,22 +468,50 @@ __device__ __forceinline__ int64_t xcd_major_block(int64_t per) {
   return (int64_t)(blockIdx.x % BWT_XCDS) * per + blockIdx.x / BWT_XCDS;
 }
 
+// One tile of BWT_COMBINE_TILE slots per workgroup: the finished values are put in LDS in the
+// order of their staging positions, then stored in that order, so the lanes of a store
+// instruction write runs of consecutive positions (one run per query-id window).
 __global__ void __launch_bounds__(BWT_COMBINE_THREADS) bwt_combine_slots_kernel(const BwtIndex x, const BwtBatch q,
                                                                                 int64_t per) {
-  const int64_t s = xcd_major_block(per) * BWT_COMBINE_THREADS + threadIdx.x;
-  if (s >= q.M) return;
-  const int64_t* row = x.cp + x.chunk_cp0[q.sblk[s]];
-  uint32_t tl;
-  if (q.qtile) {
-    tl = q.qtile[s];
-  } else {  // tiles of the endpoint coordinates (e.g. ranks the device just remapped)
-    const uint32_t id = q.qorder[s];
-    tl = (q.qlo[id] >> x.tile_shift) | ((q.qhi[id] >> x.tile_shift) << 16);
+  constexpr int kPer = BWT_COMBINE_TILE / BWT_COMBINE_THREADS;  // slots per thread
+  __shared__ int64_t sv[BWT_COMBINE_TILE];
+  const int64_t t0 = xcd_major_block(per) * BWT_COMBINE_TILE;
+  if (t0 >= q.M) return;  // the whole workgroup
+  // every load of the thread's slots is issued before any is used (clamped, unconditional)
+  const int64_t last = q.M - 1;
+  int32_t blk[kPer];
+  uint32_t tl[kPer];
+  int64_t part[kPer];
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int64_t s = min(t0 + k * BWT_COMBINE_THREADS + (int64_t)threadIdx.x, last);
+    blk[k] = q.sblk[s];
+    if (q.qtile) {
+      tl[k] = q.qtile[s];
+    } else {  // tiles of the endpoint coordinates (e.g. ranks the device just remapped)
+      const uint32_t id = q.qorder[s];
+      tl[k] = (q.qlo[id] >> x.tile_shift) | ((q.qhi[id] >> x.tile_shift) << 16);
+    }
+    // level B, stored by slot; endpoints on one chain stored their difference in the hi word
+    const longlong2 gb = *reinterpret_cast<const longlong2*>(q.g + 2 * s);
+    part[k] = q.ansr[s] + ((tl[k] >> 16) == (tl[k] & 0xFFFFu) ? gb.y : gb.y - gb.x);
   }
-  // level B, stored by slot; endpoints on one chain stored their difference in the hi word
-  const longlong2 gb = *reinterpret_cast<const longlong2*>(q.g + 2 * s);
-  const int64_t lb = (tl >> 16) == (tl & 0xFFFFu) ? gb.y : gb.y - gb.x;
-  q.stage[q.spos[s]] = q.ansr[s] + (row[tl >> 16] - row[tl & 0xFFFFu]) + lb;
+  int64_t r0[kPer];
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) r0[k] = x.chunk_cp0[blk[k]];
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int64_t s = t0 + k * BWT_COMBINE_THREADS + threadIdx.x;
+    const int64_t* row = x.cp + r0[k];
+    const int64_t v = part[k] + (row[tl[k] >> 16] - row[tl[k] & 0xFFFFu]);
+    if (s <= last) sv[q.srank[s]] = v;
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < BWT_COMBINE_TILE; k += BWT_COMBINE_THREADS) {
+    const int64_t p = t0 + k;
+    if (p >= q.M) break;
+    q.stage[q.spos[p]] = sv[k];
+  }
 }
 
 __global__ void __launch_bounds__(BWT_COMBINE_THREADS) bwt_combine_ids_kernel(const BwtBatch q, int64_t per) {
@@ -671,14 +699,17 @@ extern "C" int bwt_combine_launch(const BwtIndex* x, const BwtBatch* b, const Bw
   (void)l;
   if (!x || !b || b->M < 0) return (int)hipErrorInvalidValue;
   if (b->M == 0) return 0;
-  if (!b->spos || !b->ipos || !b->stage) return (int)hipErrorInvalidValue;
-  const int64_t nb = (b->M + BWT_COMBINE_THREADS - 1) / BWT_COMBINE_THREADS;
-  const int64_t per = (nb + BWT_XCDS - 1) / BWT_XCDS;
-  const dim3 grid((unsigned)(per * BWT_XCDS));
-  hipLaunchKernelGGL(bwt_combine_slots_kernel, grid, dim3(BWT_COMBINE_THREADS), 0, (hipStream_t)stream, *x, *b, per);
+  if (!b->spos || !b->srank || !b->ipos || !b->stage) return (int)hipErrorInvalidValue;
+  const int64_t nt = (b->M + BWT_COMBINE_TILE - 1) / BWT_COMBINE_TILE;
+  const int64_t per_t = (nt + BWT_XCDS - 1) / BWT_XCDS;
+  hipLaunchKernelGGL(bwt_combine_slots_kernel, dim3((unsigned)(per_t * BWT_XCDS)), dim3(BWT_COMBINE_THREADS), 0,
+                     (hipStream_t)stream, *x, *b, per_t);
   int rc = report("combine (slots)", hipGetLastError());
   if (rc) return rc;
-  hipLaunchKernelGGL(bwt_combine_ids_kernel, grid, dim3(BWT_COMBINE_THREADS), 0, (hipStream_t)stream, *b, per);
+  const int64_t nb = (b->M + BWT_COMBINE_THREADS - 1) / BWT_COMBINE_THREADS;
+  const int64_t per = (nb + BWT_XCDS - 1) / BWT_XCDS;
+  hipLaunchKernelGGL(bwt_combine_ids_kernel, dim3((unsigned)(per * BWT_XCDS)), dim3(BWT_COMBINE_THREADS), 0,
+                     (hipStream_t)stream, *b, per);
   return report("combine (ids)", hipGetLastError());
 }
 

@@ -115,19 +115,23 @@ typedef struct BwtBatch {
   int64_t* ansr;              /* M, by slot: in-block part */
   int64_t* g;                 /* 2M, by endpoint id (2*slot+side): level-B carry; a slot whose
                                  endpoints share a key tile holds hi - lo at 2*slot+1 */
-  const uint32_t* spos;       /* M: staging position of each slot's finished value */
+  const uint32_t* spos;       /* M: staging positions of the slots of each tile of
+                                 BWT_COMBINE_TILE slots, ascending within the tile */
+  const uint16_t* srank;      /* M: index of the slot's staging position in its tile's list */
   const uint32_t* ipos;       /* M: staging position of each query id's value (= spos of its slot) */
   int64_t* stage;             /* M staging buffer: finished values grouped by windows of query ids */
   int64_t* answers;           /* M, by query id */
 } BwtBatch;
 
-/* The combine kernels map workgroups to slots / query ids XCD-major: on a part with BWT_XCDS
- * accelerator dies that take workgroups round robin, die x handles the x-th of BWT_XCDS
- * contiguous parts of the BWT_COMBINE_THREADS-wide blocks.  The planner groups the staging
- * buffer by (query-id window, part) so each die's staging stores and loads stay in its own
- * cache (the mapping affects speed only, never results). */
+/* The combine kernels map workgroups to slot tiles / query-id blocks XCD-major: on a part with
+ * BWT_XCDS accelerator dies that take workgroups round robin, die x handles the x-th of BWT_XCDS
+ * contiguous parts of the tiles (slots, BWT_COMBINE_TILE each) or blocks (query ids,
+ * BWT_COMBINE_THREADS each).  The planner groups the staging buffer by (query-id window, slot
+ * part) so each die's staging stores and loads stay in its own cache (the mapping affects speed
+ * only, never results). */
 #define BWT_XCDS 8
 #define BWT_COMBINE_THREADS 256
+#define BWT_COMBINE_TILE 2048
 
 typedef struct BwtLaunch {
   int32_t block_threads;  /* block replay workgroup (multiple of 64, <= 1024) */
