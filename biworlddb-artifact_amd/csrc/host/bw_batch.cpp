@@ -78,12 +78,13 @@ int main(int argc, char** argv) {
   PlanKnobs kn;
   LaunchKnobs lk;
   int warmup = 1, repeat = 3, plain_sample = 8;
-  bool want_sha = false, stages = false;
+  bool want_sha = false, stages = false, dev_plan = false;
   std::string answers_path, json_path, tuned_path;
   bool have_knob[8] = {false, false, false, false, false, false, false, false};
   for (int i = 1; i < argc; ++i) {
     if (!std::strcmp(argv[i], "--stream-sha")) { want_sha = true; continue; }
     if (!std::strcmp(argv[i], "--stages")) { stages = true; continue; }
+    if (!std::strcmp(argv[i], "--device-plan")) { dev_plan = true; continue; }
     if (i + 1 >= argc) { std::fprintf(stderr, "bw_batch: missing value for %s\n", argv[i]); return 2; }
     const char* k = argv[i];
     const char* v = argv[++i];
@@ -193,14 +194,79 @@ int main(int argc, char** argv) {
   HIPX(hipEventSynchronize(e1));
   float index_ms = 0;
   HIPX(hipEventElapsedTime(&index_ms, e0, e1));
+  // the device planner: the same plan built on the GPU from the device copies of the queries,
+  // compared array for array with the host plan; its batch is then the one replayed
+  DevicePlanSlot dslot;
+  BwtBatch* QB = &DB.b;
+  double dplan_ms = -1;
+  int64_t plan_mismatch = -1;
+  if (dev_plan) {
+    DeviceArena qa;
+    const int32_t* d_qstate = qa.upload(rs.qstate);
+    if (!qa.ok() || !alloc_plan_slot(E, M, dslot, err, sizeof(err))) {
+      std::fprintf(stderr, "bw_batch: device plan: %s\n", qa.ok() ? err : "upload");
+      return 5;
+    }
+    RUN(device_plan(E, dslot, M, d_qstate, d_qlo, d_qhi, st));  // warm
+    HIPX(hipStreamSynchronize(st));
+    std::vector<double> pm;
+    for (int i = 0; i < std::max(1, repeat); ++i) {
+      HIPX(hipEventRecord(e0, st));
+      RUN(device_plan(E, dslot, M, d_qstate, d_qlo, d_qhi, st));
+      HIPX(hipEventRecord(e1, st));
+      HIPX(hipEventSynchronize(e1));
+      float t = 0;
+      HIPX(hipEventElapsedTime(&t, e0, e1));
+      pm.push_back(t);
+    }
+    dplan_ms = *std::min_element(pm.begin(), pm.end());
+    int64_t cnt[5];
+    HIPX(hipMemcpy(cnt, dslot.dcounts, sizeof(cnt), hipMemcpyDeviceToHost));
+    if (cnt[4]) { std::fprintf(stderr, "bw_batch: device plan flagged out-of-range queries\n"); return 3; }
+    const BatchPlan& H = DB.plan;
+    plan_mismatch = 0;
+    auto cmp = [&](const char* name, const void* dev, const void* host, size_t n, size_t elem) {
+      std::vector<unsigned char> tmp(n * elem);
+      if (n) HIPX(hipMemcpy(tmp.data(), dev, n * elem, hipMemcpyDeviceToHost));
+      if (n && std::memcmp(tmp.data(), host, n * elem) != 0) {
+        std::fprintf(stderr, "bw_batch: device plan differs in %s\n", name);
+        ++plan_mismatch;
+      }
+      return 0;
+    };
+    if (cnt[0] != int64_t(H.qblk.size()) || cnt[1] != int64_t(H.grp_step.size()) ||
+        cnt[2] != int64_t(H.wk_chunk.size()) || cnt[3] != H.npairs) {
+      std::fprintf(stderr, "bw_batch: device plan counts %ld %ld %ld %ld vs host %zu %zu %zu %ld\n", (long)cnt[0],
+                   (long)cnt[1], (long)cnt[2], (long)cnt[3], H.qblk.size(), H.grp_step.size(), H.wk_chunk.size(),
+                   (long)H.npairs);
+      ++plan_mismatch;
+    } else {
+      const BwtBatch& d = dslot.b;
+      cmp("qblk", d.qblk, H.qblk.data(), H.qblk.size(), 4);
+      cmp("qblk_g0", d.qblk_g0, H.qblk_g0.data(), H.qblk_g0.size(), 8);
+      cmp("grp_step", d.grp_step, H.grp_step.data(), H.grp_step.size(), 4);
+      cmp("grp_q0", d.grp_q0, H.grp_q0.data(), H.grp_q0.size(), 8);
+      cmp("qorder", d.qorder, H.qorder.data(), H.qorder.size(), 4);
+      cmp("qrank", d.qrank, H.qrank.data(), H.qrank.size(), 4);
+      cmp("sblk", d.sblk, H.sblk.data(), H.sblk.size(), 4);
+      cmp("qtile", d.qtile, H.qtile.data(), H.qtile.size(), 4);
+      cmp("wk_chunk", d.wk_chunk, H.wk_chunk.data(), H.wk_chunk.size(), 8);
+      cmp("wk_e0", d.wk_e0, H.wk_e0.data(), H.wk_e0.size(), 8);
+      cmp("eitems", d.eitems, H.eitems.data(), H.eitems.size(), 8);
+      cmp("spos", d.spos, H.spos.data(), H.spos.size(), 4);
+      cmp("srank", d.srank, H.srank.data(), H.srank.size(), 2);
+      cmp("ipos", d.ipos, H.ipos.data(), H.ipos.size(), 4);
+    }
+    QB = &dslot.b;
+  }
   // the whole stream: one query launch sequence, captured once, replayed
   BwtLaunch L = E.launch;
-  RUN(bwt_query_launch(&E.x, &DB.b, &L, st));  // warm (also sets the kernels' LDS limits)
+  RUN(bwt_query_launch(&E.x, QB, &L, st));  // warm (also sets the kernels' LDS limits)
   HIPX(hipStreamSynchronize(st));
   hipGraph_t graph;
   hipGraphExec_t gexec;
   HIPX(hipStreamBeginCapture(st, hipStreamCaptureModeGlobal));
-  RUN(bwt_query_launch(&E.x, &DB.b, &L, st));
+  RUN(bwt_query_launch(&E.x, QB, &L, st));
   HIPX(hipStreamEndCapture(st, &graph));
   HIPX(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
   for (int i = 0; i < warmup; ++i) HIPX(hipGraphLaunch(gexec, st));
@@ -222,11 +288,11 @@ int main(int argc, char** argv) {
     for (auto& e : ev) HIPX(hipEventCreate(&e));
     for (int i = 0; i < repeat; ++i) {
       HIPX(hipEventRecord(ev[0], st));
-      RUN(bwt_block_replay_launch(&E.x, &DB.b, &L, st));
+      RUN(bwt_block_replay_launch(&E.x, QB, &L, st));
       HIPX(hipEventRecord(ev[1], st));
-      RUN(bwt_chunk_replay_launch(&E.x, &DB.b, &L, st));
+      RUN(bwt_chunk_replay_launch(&E.x, QB, &L, st));
       HIPX(hipEventRecord(ev[2], st));
-      RUN(bwt_combine_launch(&E.x, &DB.b, &L, st));
+      RUN(bwt_combine_launch(&E.x, QB, &L, st));
       HIPX(hipEventRecord(ev[3], st));
       HIPX(hipEventSynchronize(ev[3]));
       for (int k = 0; k < 3; ++k) {
@@ -263,7 +329,7 @@ int main(int argc, char** argv) {
   const double sb_combine = double(M) * (8.0 + 16.0 + 8.0 + 16.0 + 4.0 + 8.0 + 4.0 + 8.0 + 8.0);
   const auto tx0 = clk::now();
   std::vector<int64_t> ans(static_cast<size_t>(M));
-  HIPX(hipMemcpy(ans.data(), DB.b.answers, size_t(M) * sizeof(int64_t), hipMemcpyDeviceToHost));
+  HIPX(hipMemcpy(ans.data(), QB->answers, size_t(M) * sizeof(int64_t), hipMemcpyDeviceToHost));
   const std::string sha = sha_hex(ans.data(), size_t(M) * sizeof(int64_t));
   const auto tx1 = clk::now();
   if (!answers_path.empty()) {
@@ -333,6 +399,7 @@ int main(int argc, char** argv) {
       "\"stage_bytes\":{\"block_replay\":%.0f,\"chunk_replay\":%.0f,\"combine\":%.0f},"
       "\"plain\":{\"batches\":%" PRId64 ",\"queries\":%" PRId64 ",\"pass_ms\":%.4f,\"query_ms\":%.4f,"
       "\"pass_queries_per_s\":%.1f,\"query_queries_per_s\":%.1f,\"mismatch\":%" PRId64 "},"
+      "\"device_plan\":{\"used\":%s,\"ms\":%.4f,\"mismatch\":%" PRId64 "},"
       "\"device_bytes\":%zu,\"device\":\"%s\",\"cus\":%d}\n",
       spec.S, spec.U, spec.N, spec.seed, ss.nreq, ss.max_batch, M, sha.c_str(), E.plan.nblk(), BP.qblk.size(),
       BP.grp_step.size(), BP.eitems.size(), BP.wk_chunk.size(), E.plan.knobs.rmax, E.plan.knobs.tile_shift,
@@ -343,7 +410,8 @@ int main(int argc, char** argv) {
       sb_block, sb_chunk, sb_combine, K, plain_q, plain_pass_ms,
       plain_query_ms, plain_pass_ms > 0 ? double(plain_q) / (plain_pass_ms * 1e-3) : 0.0,
       plain_query_ms > 0 ? double(plain_q) / (plain_query_ms * 1e-3) : 0.0, plain_mismatch,
-      E.arena.bytes + io.bytes + DB.arena.bytes, dev, prop.multiProcessorCount);
+      dev_plan ? "true" : "false", dplan_ms, plan_mismatch, E.arena.bytes + io.bytes + DB.arena.bytes + dslot.arena.bytes,
+      dev, prop.multiProcessorCount);
   std::fwrite(buf, 1, size_t(n), stdout);
   if (!json_path.empty()) {
     FILE* f = std::fopen(json_path.c_str(), "w");
@@ -351,5 +419,5 @@ int main(int argc, char** argv) {
     std::fwrite(buf, 1, size_t(n), f);
     std::fclose(f);
   }
-  return plain_mismatch == 0 ? 0 : 6;
+  return plain_mismatch == 0 && plan_mismatch <= 0 ? 0 : 6;
 }

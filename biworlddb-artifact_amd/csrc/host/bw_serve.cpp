@@ -10,10 +10,11 @@
 // Requests arrive at their Poisson times (a clock started when serving starts).  Whenever a slot
 // is free, every request that has arrived and fits the slot's capacity is admitted in arrival
 // order: appends are copied to the device at once (a stream every batch waits on), query
-// requests join the batch.  The batch is planned on the host, copied in, answered by one launch
-// sequence on the slot's own stream and copied out; batches on different slots overlap on the
-// device and with the host planning of the next one.  A query on an appended state is answered
-// at its deepest indexed ancestor plus the overlay of the appended updates on its path.
+// requests join the batch.  The host only stages the batch's raw queries (state, lo, hi) and
+// copies them in; the batch is planned on the device (bwt_plan_launch), answered by one launch
+// sequence and copied out, all on the slot's own stream, so batches on different slots overlap
+// on the device.  A query on an appended state is answered at its deepest indexed ancestor plus
+// the overlay of the appended updates on its path.
 // Latency = time the request's answers are back on the host - its arrival time.
 // --check-alone K: afterwards K requests (evenly spread) are answered again, each alone in its
 // own batch, and must equal their answers from the mix element for element.
@@ -70,20 +71,22 @@ std::string sha_hex(const void* p, size_t n) {
   return s;
 }
 
-// Fixed-capacity device + pinned host buffers of one in-flight batch.  Every array has its own
-// region sized for cap queries; a batch uses a prefix of each.
+// Fixed-capacity device + pinned host buffers of one in-flight batch: the raw queries (pinned
+// staging and device copies), the overlay's query lists, and the device planner's slot (plan
+// outputs, result buffers, workspace).  A batch uses a prefix of each.
 struct Slot {
-  int64_t cap = 0;       // queries
-  int64_t cap_work = 0;  // chunk work items
-  int32_t nblk = 0;
+  int64_t cap = 0;  // queries
   hipStream_t st = nullptr;
   hipEvent_t done = nullptr;
-  unsigned char* dev = nullptr;
-  unsigned char* host = nullptr;  // pinned: staged inputs, then the answers
-  size_t bytes = 0;
-  // region offsets
-  size_t o_qlo, o_qhi, o_qblk, o_qblk_g0, o_grp_step, o_grp_q0, o_qorder, o_spos, o_srank, o_ipos, o_qrank, o_sblk, o_qtile, o_wk_chunk,
-      o_wk_e0, o_eitems, o_ov_qid, o_ov_state, o_ansr, o_g, o_stage, o_answers;
+  unsigned char* host = nullptr;  // pinned: qstate | qlo | qhi | overlay ids | overlay states | answers
+  size_t h_state = 0, h_lo = 0, h_hi = 0, h_ovq = 0, h_ovs = 0, h_ans = 0, host_bytes = 0;
+  DeviceArena in;
+  int32_t* d_state = nullptr;
+  uint32_t* d_lo = nullptr;
+  uint32_t* d_hi = nullptr;
+  uint32_t* d_ovq = nullptr;
+  int32_t* d_ovs = nullptr;
+  DevicePlanSlot plan;
   // the batch in flight
   bool busy = false;
   std::vector<int64_t> reqs;     // request ids
@@ -92,44 +95,31 @@ struct Slot {
   clk::time_point launched;
 };
 
-size_t layout(Slot& s, int64_t cap, int32_t nblk) {
-  s.cap = cap;
-  s.nblk = nblk;
-  s.cap_work = 2 * cap + 1;
+size_t host_layout(Slot& s, int64_t cap) {
   size_t o = 0;
   auto take = [&](size_t n) {
     const size_t at = o;
     o += (n + 255) & ~size_t(255);
     return at;
   };
-  s.o_qlo = take(size_t(cap) * 4);
-  s.o_qhi = take(size_t(cap) * 4);
-  s.o_qblk = take(size_t(nblk) * 4);
-  s.o_qblk_g0 = take((size_t(nblk) + 1) * 8);
-  s.o_grp_step = take(size_t(cap) * 4);
-  s.o_grp_q0 = take((size_t(cap) + 1) * 8);
-  s.o_qorder = take(size_t(cap) * 4);
-  s.o_spos = take(size_t(cap) * 4);
-  s.o_srank = take(size_t(cap) * 2);
-  s.o_ipos = take(size_t(cap) * 4);
-  s.o_qrank = take(size_t(cap) * 4);
-  s.o_sblk = take(size_t(cap) * 4);
-  s.o_qtile = take(size_t(cap) * 4);
-  s.o_wk_chunk = take(size_t(s.cap_work) * 8);
-  s.o_wk_e0 = take((size_t(s.cap_work) + 1) * 8);
-  s.o_eitems = take(size_t(cap) * 16);
-  s.o_ov_qid = take(size_t(cap) * 4);
-  s.o_ov_state = take(size_t(cap) * 4);
-  s.o_ansr = take(size_t(cap) * 8);
-  s.o_g = take(size_t(cap) * 16);
-  s.o_stage = take(size_t(cap) * 8);
-  s.o_answers = take(size_t(cap) * 8);
+  const size_t n = size_t(cap);
+  s.h_state = take(n * 4);
+  s.h_lo = take(n * 4);
+  s.h_hi = take(n * 4);
+  s.h_ovq = take(n * 4);
+  s.h_ovs = take(n * 4);
+  s.h_ans = take(n * 8);
   return o;
 }
 
-size_t bytes_per_query() {
-  Slot s;
-  return layout(s, 1 << 20, 0) >> 20;
+// Device bytes of one slot of cap queries (inputs, overlay lists, plan outputs and results,
+// planner workspace).
+size_t slot_device_bytes(const TreeEngine& E, int64_t cap) {
+  size_t work = 0;
+  if (bwt_plan_workspace(&E.x, &E.px, cap, &work) != 0) return 0;
+  const size_t n = size_t(cap);
+  const size_t outputs = n * (4 + 8 + 4 + 8 + 4 + 4 + 4 + 4 + 16 + 16 + 16 + 4 + 2 + 4 + 8 + 16 + 8 + 8);
+  return work + outputs + n * 20 + 4096;
 }
 
 }  // namespace
@@ -192,7 +182,6 @@ int main(int argc, char** argv) {
   HIPX(hipStreamCreate(&ist));
   RUN(E.build_index(ist));
   HIPX(hipStreamSynchronize(ist));
-  const int32_t nblk = E.plan.nblk();
   // appended states: resident device arrays (capacity = the stream's appends), host mirrors
   DeviceArena app;
   int32_t* d_app_parent = app.scratch<int32_t>(size_t(NA));
@@ -207,128 +196,95 @@ int main(int argc, char** argv) {
   hipEvent_t app_ready;
   HIPX(hipEventCreate(&app_ready));
   HIPX(hipEventRecord(app_ready, ist));
-  // slots within the HBM budget
-  const size_t bpq = bytes_per_query();
+  // slots within the HBM budget: the largest capacity whose device buffers fit the slot's share
   std::vector<Slot> slots(static_cast<size_t>(nslots));
-  const int64_t cap = std::max<int64_t>(ss.max_batch, int64_t(budget_mb * 1048576.0 / nslots) / int64_t(bpq));
+  const size_t share = size_t(budget_mb * 1048576.0 / nslots);
+  int64_t cap = ss.max_batch;
+  {
+    const int64_t probe = int64_t(1) << 20;
+    const size_t pb = slot_device_bytes(E, probe);
+    if (pb == 0) { std::fprintf(stderr, "bw_serve: planner workspace\n"); return 5; }
+    const double per_q = double(pb) / double(probe);
+    cap = std::max<int64_t>(cap, int64_t(double(share) / per_q));
+    while (cap > ss.max_batch && slot_device_bytes(E, cap) > share) cap = cap * 15 / 16;
+    cap = std::max<int64_t>(cap, ss.max_batch);
+  }
   for (Slot& s : slots) {
-    s.bytes = layout(s, cap, nblk);
-    HIPX(hipMalloc(reinterpret_cast<void**>(&s.dev), s.bytes));
-    HIPX(hipHostMalloc(reinterpret_cast<void**>(&s.host), s.bytes, hipHostMallocDefault));
+    s.cap = cap;
+    s.host_bytes = host_layout(s, cap);
+    HIPX(hipHostMalloc(reinterpret_cast<void**>(&s.host), s.host_bytes, hipHostMallocDefault));
+    s.d_state = s.in.scratch<int32_t>(size_t(cap));
+    s.d_lo = s.in.scratch<uint32_t>(size_t(cap));
+    s.d_hi = s.in.scratch<uint32_t>(size_t(cap));
+    s.d_ovq = s.in.scratch<uint32_t>(size_t(cap));
+    s.d_ovs = s.in.scratch<int32_t>(size_t(cap));
+    if (!s.in.ok() || !alloc_plan_slot(E, cap, s.plan, err, sizeof(err))) {
+      std::fprintf(stderr, "bw_serve: slot allocation: %s\n", s.in.ok() ? err : "inputs");
+      return 5;
+    }
     HIPX(hipStreamCreate(&s.st));
     HIPX(hipEventCreate(&s.done));
   }
   const auto g1 = clk::now();
 
-  // ---- one batch: plan, stage, copy in, launch, copy the answers out (all on the slot's stream)
-  std::vector<int32_t> bq_state;
-  std::vector<uint32_t> bq_lo, bq_hi;
-  std::vector<uint32_t> ov_qid;
-  std::vector<int32_t> ov_state;
-  double plan_s = 0;
+  // ---- one batch: stage the raw queries, copy in, plan and answer on the device, copy the
+  //      answers out (all on the slot's stream)
+  double plan_s = 0;  // host time spent staging batches
+  const BwtLaunch L = E.launch;
   auto launch_batch = [&](Slot& s) {
     const auto p0 = clk::now();
-    bq_state.clear();
-    bq_lo.clear();
-    bq_hi.clear();
-    ov_qid.clear();
-    ov_state.clear();
+    int32_t* hs = reinterpret_cast<int32_t*>(s.host + s.h_state);
+    uint32_t* hl = reinterpret_cast<uint32_t*>(s.host + s.h_lo);
+    uint32_t* hh = reinterpret_cast<uint32_t*>(s.host + s.h_hi);
+    uint32_t* hoq = reinterpret_cast<uint32_t*>(s.host + s.h_ovq);
+    int32_t* hos = reinterpret_cast<int32_t*>(s.host + s.h_ovs);
+    int64_t M = 0, nov = 0;
     for (int64_t r : s.reqs)
       for (int64_t q = rs.req_q0[size_t(r)]; q < rs.req_q0[size_t(r) + 1]; ++q) {
         const int32_t v = rs.qstate[size_t(q)];
         if (v > S) {
-          ov_qid.push_back(uint32_t(bq_state.size()));
-          ov_state.push_back(v);
-          bq_state.push_back(app_base[size_t(v - S - 1)]);
+          hoq[nov] = uint32_t(M);
+          hos[nov++] = v;
+          hs[M] = app_base[size_t(v - S - 1)];
         } else {
-          bq_state.push_back(v);
+          hs[M] = v;
         }
-        bq_lo.push_back(rs.qlo[size_t(q)]);
-        bq_hi.push_back(rs.qhi[size_t(q)]);
+        hl[M] = rs.qlo[size_t(q)];
+        hh[M] = rs.qhi[size_t(q)];
+        ++M;
       }
-    const int64_t M = int64_t(bq_state.size());
     s.nq = M;
-    BatchPlan bp;
-    if (!build_batch_plan(E.plan, M, bq_state.data(), bq_lo.data(), bq_hi.data(), false, bp, err, sizeof(err))) {
-      std::fprintf(stderr, "bw_serve: batch plan: %s\n", err);
-      std::exit(3);
+    HIPX(hipMemcpyAsync(s.d_state, hs, size_t(M) * 4, hipMemcpyHostToDevice, s.st));
+    HIPX(hipMemcpyAsync(s.d_lo, hl, size_t(M) * 4, hipMemcpyHostToDevice, s.st));
+    HIPX(hipMemcpyAsync(s.d_hi, hh, size_t(M) * 4, hipMemcpyHostToDevice, s.st));
+    if (nov) {
+      HIPX(hipMemcpyAsync(s.d_ovq, hoq, size_t(nov) * 4, hipMemcpyHostToDevice, s.st));
+      HIPX(hipMemcpyAsync(s.d_ovs, hos, size_t(nov) * 4, hipMemcpyHostToDevice, s.st));
     }
-    if (int64_t(bp.wk_chunk.size()) > s.cap_work) {
-      std::fprintf(stderr, "bw_serve: work items exceed the slot\n");
-      std::exit(3);
-    }
-    plan_s += secs(p0, clk::now());
-    // stage into pinned memory, copy the used prefixes
-    auto put = [&](size_t off, const void* src, size_t n) {
-      if (!n) return;
-      std::memcpy(s.host + off, src, n);
-      HIPX(hipMemcpyAsync(s.dev + off, s.host + off, n, hipMemcpyHostToDevice, s.st));
-    };
-    put(s.o_qlo, bq_lo.data(), size_t(M) * 4);
-    put(s.o_qhi, bq_hi.data(), size_t(M) * 4);
-    put(s.o_qblk, bp.qblk.data(), bp.qblk.size() * 4);
-    put(s.o_qblk_g0, bp.qblk_g0.data(), bp.qblk_g0.size() * 8);
-    put(s.o_grp_step, bp.grp_step.data(), bp.grp_step.size() * 4);
-    put(s.o_grp_q0, bp.grp_q0.data(), bp.grp_q0.size() * 8);
-    put(s.o_qorder, bp.qorder.data(), size_t(M) * 4);
-    put(s.o_spos, bp.spos.data(), size_t(M) * 4);
-    put(s.o_srank, bp.srank.data(), size_t(M) * 2);
-    put(s.o_ipos, bp.ipos.data(), size_t(M) * 4);
-    put(s.o_qrank, bp.qrank.data(), size_t(M) * 4);
-    put(s.o_sblk, bp.sblk.data(), size_t(M) * 4);
-    put(s.o_qtile, bp.qtile.data(), size_t(M) * 4);
-    put(s.o_wk_chunk, bp.wk_chunk.data(), bp.wk_chunk.size() * 8);
-    put(s.o_wk_e0, bp.wk_e0.data(), bp.wk_e0.size() * 8);
-    put(s.o_eitems, bp.eitems.data(), bp.eitems.size() * 8);
-    put(s.o_ov_qid, ov_qid.data(), ov_qid.size() * 4);
-    put(s.o_ov_state, ov_state.data(), ov_state.size() * 4);
-    BwtBatch b{};
-    b.M = M;
-    b.qlo = reinterpret_cast<const uint32_t*>(s.dev + s.o_qlo);
-    b.qhi = reinterpret_cast<const uint32_t*>(s.dev + s.o_qhi);
-    b.nqblk = int32_t(bp.qblk.size());
-    b.qblk = reinterpret_cast<const int32_t*>(s.dev + s.o_qblk);
-    b.qblk_g0 = reinterpret_cast<const int64_t*>(s.dev + s.o_qblk_g0);
-    b.ngrp = int64_t(bp.grp_step.size());
-    b.grp_step = reinterpret_cast<const int32_t*>(s.dev + s.o_grp_step);
-    b.grp_q0 = reinterpret_cast<const int64_t*>(s.dev + s.o_grp_q0);
-    b.qorder = reinterpret_cast<const uint32_t*>(s.dev + s.o_qorder);
-    b.spos = reinterpret_cast<const uint32_t*>(s.dev + s.o_spos);
-    b.srank = reinterpret_cast<const uint16_t*>(s.dev + s.o_srank);
-    b.ipos = reinterpret_cast<const uint32_t*>(s.dev + s.o_ipos);
-    b.qrank = reinterpret_cast<const uint32_t*>(s.dev + s.o_qrank);
-    b.sblk = reinterpret_cast<const int32_t*>(s.dev + s.o_sblk);
-    b.qtile = reinterpret_cast<const uint32_t*>(s.dev + s.o_qtile);
-    b.nwork = int64_t(bp.wk_chunk.size());
-    b.wk_chunk = reinterpret_cast<const int64_t*>(s.dev + s.o_wk_chunk);
-    b.wk_e0 = reinterpret_cast<const int64_t*>(s.dev + s.o_wk_e0);
-    b.eitems = reinterpret_cast<const uint64_t*>(s.dev + s.o_eitems);
-    b.ansr = reinterpret_cast<int64_t*>(s.dev + s.o_ansr);
-    b.g = reinterpret_cast<int64_t*>(s.dev + s.o_g);
-    b.stage = reinterpret_cast<int64_t*>(s.dev + s.o_stage);
-    b.answers = reinterpret_cast<int64_t*>(s.dev + s.o_answers);
     HIPX(hipStreamWaitEvent(s.st, app_ready, 0));  // appended states admitted so far are on the device
-    BwtLaunch L = E.launch;
-    RUN(bwt_query_launch(&E.x, &b, &L, s.st));
-    if (!ov_qid.empty()) {
+    RUN(device_plan(E, s.plan, M, s.d_state, s.d_lo, s.d_hi, s.st));
+    BwtLaunch l = L;
+    RUN(bwt_query_launch(&E.x, &s.plan.b, &l, s.st));
+    if (nov) {
       BwtOverlay o{};
       o.S = S;
       o.U = AU;
-      o.n = int64_t(ov_qid.size());
-      o.qid = reinterpret_cast<const uint32_t*>(s.dev + s.o_ov_qid);
-      o.qstate = reinterpret_cast<const int32_t*>(s.dev + s.o_ov_state);
-      o.qlo = b.qlo;
-      o.qhi = b.qhi;
+      o.n = nov;
+      o.qid = s.d_ovq;
+      o.qstate = s.d_ovs;
+      o.qlo = s.d_lo;
+      o.qhi = s.d_hi;
       o.app_parent = d_app_parent;
       o.app_key = d_app_key;
       o.app_delta = d_app_delta;
-      o.answers = b.answers;
+      o.answers = s.plan.b.answers;
       RUN(bwt_overlay_launch(&o, s.st));
     }
-    HIPX(hipMemcpyAsync(s.host + s.o_answers, s.dev + s.o_answers, size_t(M) * 8, hipMemcpyDeviceToHost, s.st));
+    HIPX(hipMemcpyAsync(s.host + s.h_ans, s.plan.b.answers, size_t(M) * 8, hipMemcpyDeviceToHost, s.st));
     HIPX(hipEventRecord(s.done, s.st));
     s.busy = true;
     s.launched = clk::now();
+    plan_s += secs(p0, s.launched);
   };
   auto admit_append = [&](int64_t r) {
     const int64_t a = rs.req_app[size_t(r)];
@@ -353,7 +309,7 @@ int main(int argc, char** argv) {
     for (Slot& s : slots) {
       if (!s.busy || hipEventQuery(s.done) != hipSuccess) continue;
       const double now = secs(t0, clk::now());
-      const int64_t* out = reinterpret_cast<const int64_t*>(s.host + s.o_answers);
+      const int64_t* out = reinterpret_cast<const int64_t*>(s.host + s.h_ans);
       for (size_t i = 0; i < s.reqs.size(); ++i) {
         const int64_t r = s.reqs[i];
         const int64_t q0 = rs.req_q0[size_t(r)], n = rs.req_q0[size_t(r) + 1] - q0;
@@ -412,7 +368,7 @@ int main(int argc, char** argv) {
     HIPX(hipEventSynchronize(s.done));
     s.busy = false;
     const int64_t q0 = rs.req_q0[size_t(r)], n = rs.req_q0[size_t(r) + 1] - q0;
-    const int64_t* out = reinterpret_cast<const int64_t*>(s.host + s.o_answers);
+    const int64_t* out = reinterpret_cast<const int64_t*>(s.host + s.h_ans);
     for (int64_t i = 0; i < n; ++i) alone_mismatch += out[i] != ans[size_t(q0 + i)];
     ++alone_checked;
   }
@@ -438,12 +394,14 @@ int main(int argc, char** argv) {
       "\"max_batch\":%d,\"n_answers\":%" PRId64 ",\"answers_sha256\":\"%s\",\"slots\":%d,\"budget_mb\":%.0f,"
       "\"slot_capacity_queries\":%" PRId64 ",\"batches\":%" PRId64 ",\"mean_batch_queries\":%.1f,"
       "\"max_batch_queries\":%" PRId64 ",\"wall_s\":%.4f,\"offered_queries_per_s\":%.1f,\"queries_per_s\":%.1f,"
-      "\"latency_ms\":{\"p50\":%.3f,\"p90\":%.3f,\"p99\":%.3f,\"max\":%.3f},\"host_plan_s\":%.4f,"
+      "\"latency_ms\":{\"p50\":%.3f,\"p90\":%.3f,\"p99\":%.3f,\"max\":%.3f},\"planner\":\"device\","
+      "\"host_stage_s\":%.4f,\"slot_device_bytes\":%zu,"
       "\"alone\":{\"requests\":%" PRId64 ",\"mismatch\":%" PRId64 "},\"knobs_from\":\"%s\",\"setup_s\":%.3f,"
       "\"device\":\"%s\",\"cus\":%d}\n",
       S, spec.U, spec.N, spec.seed, R, nquery_req, nappended, ss.rate, ss.max_batch, NQ, sha.c_str(), nslots, budget_mb,
       cap, nbatches, nbatches ? sum_batch_q / double(nbatches) : 0.0, max_batch_q, wall, offered, double(NQ) / wall,
-      pct(0.5) * 1e3, pct(0.9) * 1e3, pct(0.99) * 1e3, lat.empty() ? 0.0 : lat.back() * 1e3, plan_s, alone_checked,
+      pct(0.5) * 1e3, pct(0.9) * 1e3, pct(0.99) * 1e3, lat.empty() ? 0.0 : lat.back() * 1e3, plan_s,
+      slots[0].in.bytes + slots[0].plan.arena.bytes, alone_checked,
       alone_mismatch, tuned_from.c_str(), secs(g0, g1), dev, prop.multiProcessorCount);
   std::fwrite(buf, 1, size_t(n), stdout);
   if (!json_path.empty()) {
@@ -452,9 +410,6 @@ int main(int argc, char** argv) {
     std::fwrite(buf, 1, size_t(n), f);
     std::fclose(f);
   }
-  for (Slot& s : slots) {
-    (void)hipFree(s.dev);
-    (void)hipHostFree(s.host);
-  }
+  for (Slot& s : slots) (void)hipHostFree(s.host);
   return alone_mismatch == 0 ? 0 : 6;
 }

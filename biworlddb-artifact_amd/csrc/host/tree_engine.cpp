@@ -73,6 +73,21 @@ bool TreeEngine::init(int64_t S, int U, uint64_t NK, const int32_t* parent, cons
   x.cp = arena.scratch<int64_t>(size_t(P.n_cp));
   x.segsum = arena.scratch<int64_t>(size_t(P.n_segsum));
   x.kweight = arena.scratch<int64_t>(P.kitems.size());
+  // the device planner's tables
+  px = BwtPlanIndex{};
+  px.nsteps = int64_t(P.steps.size());
+  px.NK = P.NK;
+  px.KT = P.KT;
+  px.stage_shift = P.knobs.stage_shift;
+  px.chunk_items = P.knobs.chunk_items;
+  px.ep_per_work = P.knobs.ep_per_work;
+  px.range_slots = P.knobs.range_slots;
+  px.state_step = arena.upload(P.state_step);
+  px.state_block = arena.upload(P.state_block);
+  px.tile_tb0 = arena.upload(P.tile_tb0);
+  px.tb_block = arena.upload(P.tb_block);
+  px.tb_pos = arena.upload(P.tb_pos);
+  px.chain_key0 = arena.upload(P.chain_key0);
   // the index-build batch: every block, no queries
   std::vector<int32_t> allb(size_t(P.nblk()));
   for (int32_t b = 0; b < P.nblk(); ++b) allb[size_t(b)] = b;
@@ -136,6 +151,55 @@ bool upload_batch(const TreeEngine& e, BatchPlan&& plan, const uint32_t* d_qlo, 
     return false;
   }
   return true;
+}
+
+bool alloc_plan_slot(const TreeEngine& e, int64_t cap, DevicePlanSlot& s, char* err, size_t errlen) {
+  s.arena.release();
+  s.arena.err = hipSuccess;
+  s.cap = cap;
+  s.b = BwtBatch{};
+  const size_t n = size_t(cap);
+  const size_t nq = std::min<size_t>(size_t(e.x.nblk), n);
+  BwtBatch& b = s.b;
+  b.qblk = s.arena.scratch<int32_t>(nq);
+  b.qblk_g0 = s.arena.scratch<int64_t>(nq + 1);
+  b.grp_step = s.arena.scratch<int32_t>(n);
+  b.grp_q0 = s.arena.scratch<int64_t>(n + 1);
+  b.qorder = s.arena.scratch<uint32_t>(n);
+  b.qrank = s.arena.scratch<uint32_t>(n);
+  b.sblk = s.arena.scratch<int32_t>(n);
+  b.qtile = s.arena.scratch<uint32_t>(n);
+  b.wk_chunk = s.arena.scratch<int64_t>(2 * n);
+  b.wk_e0 = s.arena.scratch<int64_t>(2 * n + 1);
+  b.eitems = s.arena.scratch<uint64_t>(2 * n);
+  b.spos = s.arena.scratch<uint32_t>(n);
+  b.srank = s.arena.scratch<uint16_t>(n);
+  b.ipos = s.arena.scratch<uint32_t>(n);
+  b.ansr = s.arena.scratch<int64_t>(n);
+  b.g = s.arena.scratch<int64_t>(2 * n);
+  b.stage = s.arena.scratch<int64_t>(n);
+  b.answers = s.arena.scratch<int64_t>(n);
+  s.dcounts = s.arena.scratch<int64_t>(8);
+  int rc = bwt_plan_workspace(&e.x, &e.px, cap, &s.work_bytes);
+  if (rc) {
+    if (err && errlen) std::snprintf(err, errlen, "plan workspace: error %d", rc);
+    return false;
+  }
+  s.work = s.arena.alloc(s.work_bytes);
+  if (!s.arena.ok()) {
+    if (err && errlen) std::snprintf(err, errlen, "plan slot: %s", hipGetErrorString(s.arena.err));
+    return false;
+  }
+  return true;
+}
+
+int device_plan(const TreeEngine& e, DevicePlanSlot& s, int64_t M, const int32_t* d_qstate, const uint32_t* d_qlo,
+                const uint32_t* d_qhi, hipStream_t st) {
+  if (M < 1 || M > s.cap) return int(hipErrorInvalidValue);
+  s.b.qlo = d_qlo;
+  s.b.qhi = d_qhi;
+  // the workspace was sized for cap queries; a smaller batch needs no more
+  return bwt_plan_launch(&e.x, &e.px, d_qstate, M, &s.b, s.dcounts, s.work, s.work_bytes, st);
 }
 
 }  // namespace bwg

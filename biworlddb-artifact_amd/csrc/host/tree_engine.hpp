@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <vector>
 
+#include "bwgpu_plan.h"
 #include "bwgpu_tree.h"
 #include "tree_plan.hpp"
 
@@ -47,6 +48,7 @@ struct TreeEngine {
   BwtLaunch launch{256, 4, 0, 0};
   DeviceArena arena;
   BwtBatch all_blocks{};  // every block listed, no queries: the index-build batch
+  BwtPlanIndex px{};      // the device planner's tables (uploaded with the index)
 
   // Host index plan + device upload of deltas and plan arrays; agg/tagg/cp are allocated, not
   // yet computed (build_index does that).  key holds coordinates in [0, NK).
@@ -66,5 +68,20 @@ struct DeviceBatch {
 
 bool upload_batch(const TreeEngine& e, BatchPlan&& plan, const uint32_t* d_qlo, const uint32_t* d_qhi,
                   DeviceBatch& out, char* err, size_t errlen);
+
+// Device arrays of a batch planned on the device, sized for up to cap queries: the plan's
+// outputs, the staging/result buffers, the planner's workspace and counts.
+struct DevicePlanSlot {
+  int64_t cap = 0;
+  BwtBatch b{};           // the output pointers (filled in by bwt_plan_launch per batch)
+  int64_t* dcounts = nullptr;
+  void* work = nullptr;
+  size_t work_bytes = 0;
+  DeviceArena arena;
+};
+bool alloc_plan_slot(const TreeEngine& e, int64_t cap, DevicePlanSlot& s, char* err, size_t errlen);
+// Plan M <= cap queries (device qstate / qlo / qhi) on stream st; s.b is then ready to launch.
+int device_plan(const TreeEngine& e, DevicePlanSlot& s, int64_t M, const int32_t* d_qstate, const uint32_t* d_qlo,
+                const uint32_t* d_qhi, hipStream_t st);
 
 }  // namespace bwg

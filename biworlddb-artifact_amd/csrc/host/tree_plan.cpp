@@ -493,11 +493,11 @@ bool build_batch_plan(const TreeIndexPlan& X, int64_t M, const int32_t* qstate, 
       }
     cstart.push_back(nE);
   }
-  // work items: each chunk's endpoints in runs of at most ep_per_work (a work item replays the
-  // chunk's keys only up to its own last endpoint), cut where the slot range changes and never
-  // between the two endpoints of one slot; ordered by slot range, then chunk.  A chunk's items
-  // are in slot order (position follows the block, ties keep slot order), so each range is one
-  // run of a chunk's items.
+  // work items: the endpoints of each (slot range, chunk) run -- a chunk's items are in slot
+  // order (position follows the block, ties keep slot order), so a range is one run of them --
+  // cut at every ep_per_work-th item from the run's start (a work item replays the chunk's keys
+  // only up to its own last endpoint), a cut that would part a slot's two endpoints moved one
+  // item on; ordered by slot range, then chunk.  The device planner makes the same cuts.
   const int64_t G = std::max<int64_t>(2, X.knobs.ep_per_work);
   const int64_t RS = std::max<int64_t>(1, X.knobs.range_slots);
   auto slot_of = [&](int64_t e) { return int64_t(uint32_t(B.eitems[size_t(e)])) >> 1; };
@@ -506,17 +506,21 @@ bool build_batch_plan(const TreeIndexPlan& X, int64_t M, const int32_t* qstate, 
   };
   std::vector<Work> wk;
   for (size_t i = 0; i < cfirst.size(); ++i) {
-    int64_t a = cstart[i];
-    const int64_t e = cstart[i + 1];
-    while (a < e) {
-      const int64_t r = slot_of(a) / RS;
-      int64_t b = std::min(a + G, e);
-      if (slot_of(b - 1) / RS != r) b = std::partition_point(B.eitems.begin() + a, B.eitems.begin() + b, [&](uint64_t it) {
-                                          return (int64_t(uint32_t(it)) >> 1) / RS == r;
-                                        }) - B.eitems.begin();
-      if (b < e && slot_of(b) == slot_of(b - 1)) ++b;  // keep a slot's pair together
-      wk.push_back(Work{r, cfirst[i], a, b});
-      a = b;
+    int64_t a0 = cstart[i];
+    const int64_t ce = cstart[i + 1];
+    while (a0 < ce) {
+      const int64_t r = slot_of(a0) / RS;
+      const int64_t e = std::partition_point(B.eitems.begin() + a0, B.eitems.begin() + ce, [&](uint64_t it) {
+                          return (int64_t(uint32_t(it)) >> 1) / RS == r;
+                        }) - B.eitems.begin();
+      int64_t a = a0;
+      for (int64_t k = 1; a < e; ++k) {
+        const int64_t c = a0 + k * G;
+        const int64_t b = c >= e ? e : c + (slot_of(c) == slot_of(c - 1) ? 1 : 0);
+        wk.push_back(Work{r, cfirst[i], a, b});
+        a = b;
+      }
+      a0 = e;
     }
   }
   for (const Work& w : wk)  // pairs: a slot's two endpoints on one chain (same tile), adjacent

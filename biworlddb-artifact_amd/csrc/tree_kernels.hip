@@ -23,6 +23,8 @@
 // All arithmetic is int64 adds (LDS ds_add_u64, DPP scans): results are exact and independent
 // of scheduling order.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 #include <mutex>
 #include <stdint.h>
 #include <stdio.h>
@@ -37,6 +39,9 @@ namespace {
 constexpr int kQCap = 1024;  // query slots staged in LDS by the block replay
 constexpr int kGCap = 64;    // query step groups staged in LDS by the block replay
 constexpr int kEPT = 4;      // block replay: events per thread per staged tile of steps
+// grid of the replay kernels when the counts are on the device: a few resident workgroups per
+// CU on a 256-CU part, each looping over its share
+constexpr int64_t kDeviceCountGrid = 2048;
 
 // Padded LDS slot of in-block rank r when each thread owns RPT consecutive ranks: one spare
 // word per run keeps a wavefront's strided run accesses on distinct banks.
@@ -59,8 +64,8 @@ constexpr int block_span(int nt) {
 // step groups are staged in LDS too, so a query step touches no global memory before its answers
 // are stored.
 template <int RPT>
-__global__ void __launch_bounds__(1024) bwt_block_kernel(const BwtIndex x, const BwtBatch q, const BwtLaunch l) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
+__device__ __forceinline__ void block_replay_one(const BwtIndex& x, const BwtBatch& q, const BwtLaunch& l,
+                                                 const int64_t bi, unsigned char* smem_raw) {
   const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wv = tid >> 6;
   const int span = block_span<RPT>(nt);
   int64_t* w = reinterpret_cast<int64_t*>(smem_raw);          // deltas since the last scan
@@ -71,12 +76,12 @@ __global__ void __launch_bounds__(1024) bwt_block_kernel(const BwtIndex x, const
   uint32_t* qbuf = reinterpret_cast<uint32_t*>(gstep + kGCap);   // kQCap packed ranks
   int64_t* sdel = reinterpret_cast<int64_t*>(qbuf + kQCap);      // kEPT*nt staged signed deltas
   uint16_t* srank = reinterpret_cast<uint16_t*>(sdel + kEPT * nt);  // kEPT*nt staged ranks
-  const int b = q.qblk[blockIdx.x];
+  const int b = q.qblk[bi];
   const int64_t k0 = x.blk_key0[b];
   const int R = (int)(x.blk_key0[b + 1] - k0);
   const int U = x.U;
   const int s0 = x.blk_step0[b];
-  const int64_t gi0 = q.qblk_g0[blockIdx.x], ge = q.qblk_g0[blockIdx.x + 1];
+  const int64_t gi0 = q.qblk_g0[bi], ge = q.qblk_g0[bi + 1];
   // without the index write, nothing after the last queried step matters
   const int s1 = l.write_index ? x.blk_step0[b + 1] : (gi0 < ge ? q.grp_step[ge - 1] + 1 : s0);
   const int64_t qend = q.grp_q0[ge];
@@ -246,6 +251,18 @@ __global__ void __launch_bounds__(1024) bwt_block_kernel(const BwtIndex x, const
   }
 }
 
+// One workgroup per listed block; with device counts (device planner) the grid is bounded and
+// each workgroup takes every gridDim-th listed block.
+template <int RPT>
+__global__ void __launch_bounds__(1024) bwt_block_kernel(const BwtIndex x, const BwtBatch q, const BwtLaunch l) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int64_t n = q.dcounts ? q.dcounts[0] : (int64_t)q.nqblk;
+  for (int64_t bi = blockIdx.x; bi < n; bi += gridDim.x) {
+    block_replay_one<RPT>(x, q, l, bi, smem_raw);
+    __syncthreads();  // the next block reuses the LDS
+  }
+}
+
 // ---------------------------------------------------------------- chunk histogram
 // One wavefront per chunk: dense class histogram of the chunk's key items, written as its
 // exclusive prefix over classes (cp row = sum of weights with class < y).  With write_kw the
@@ -354,14 +371,10 @@ __device__ __forceinline__ int64_t fen_prefix12(const int64_t* fen, int y) {
 // the Fenwick prefix plus the group's keys that precede it with a smaller class (broadcast
 // reads), then the keys enter the tree.  The next group's items are loaded while this one runs
 // (only the high word of a key item, which holds its class).
-__global__ void __launch_bounds__(256) bwt_chunk_replay_kernel(const BwtIndex x, const BwtBatch q, const BwtLaunch l,
-                                                               int stride) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int64_t ci = (int64_t)blockIdx.x * l.chunk_waves + wv;
-  if (ci >= q.nwork) return;
+__device__ __forceinline__ void chunk_replay_one(const BwtIndex& x, const BwtBatch& q, const int stride,
+                                                 const int64_t ci, int64_t* kws) {
+  const int lane = threadIdx.x & 63;
   const int64_t c = q.wk_chunk[ci];
-  int64_t* kws = reinterpret_cast<int64_t*>(smem_raw) + (int64_t)wv * replay_wave_words(stride);  // 64 weights
   int32_t* kcs = reinterpret_cast<int32_t*>(kws + 64);  // 64 group key classes
   int64_t* fen = kws + 96;
   const int ncls = x.chain_ncls[x.chunk_chain[c]];
@@ -452,6 +465,21 @@ __global__ void __launch_bounds__(256) bwt_chunk_replay_kernel(const BwtIndex x,
     it = it2;
     kx = kx2;
     kw = kw2;
+  }
+}
+
+// One wavefront per work item; with device counts (device planner) the grid is bounded and each
+// wavefront takes every (gridDim * chunk_waves)-th work item.
+__global__ void __launch_bounds__(256) bwt_chunk_replay_kernel(const BwtIndex x, const BwtBatch q, const BwtLaunch l,
+                                                               int stride) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int wv = threadIdx.x >> 6;
+  int64_t* kws = reinterpret_cast<int64_t*>(smem_raw) + (int64_t)wv * replay_wave_words(stride);
+  const int64_t n = q.dcounts ? q.dcounts[2] : q.nwork;
+  const int64_t step = (int64_t)gridDim.x * l.chunk_waves;
+  for (int64_t ci = (int64_t)blockIdx.x * l.chunk_waves + wv; ci < n; ci += step) {
+    chunk_replay_one(x, q, stride, ci, kws);
+    wave_lds_sync();  // the next work item reuses this wavefront's LDS
   }
 }
 
@@ -624,7 +652,8 @@ int report(const char* what, hipError_t e) {
 
 template <int RPT>
 void launch_block(const BwtIndex* x, const BwtBatch* b, const BwtLaunch* l, hipStream_t st) {
-  hipLaunchKernelGGL(bwt_block_kernel<RPT>, dim3(b->nqblk), dim3(l->block_threads),
+  const int64_t grid = b->dcounts ? std::min<int64_t>(b->nqblk, kDeviceCountGrid) : b->nqblk;
+  hipLaunchKernelGGL(bwt_block_kernel<RPT>, dim3((unsigned)grid), dim3(l->block_threads),
                      block_lds_bytes(RPT, l->block_threads), st, *x, *b, *l);
 }
 
@@ -689,7 +718,8 @@ extern "C" int bwt_chunk_replay_launch(const BwtIndex* x, const BwtBatch* b, con
   if ((rc = set_lds_limits())) return rc;
   if (b->nwork == 0) return 0;
   const int stride = level_stride(x->ncls_b);
-  const int64_t grid = (b->nwork + l->chunk_waves - 1) / l->chunk_waves;
+  int64_t grid = (b->nwork + l->chunk_waves - 1) / l->chunk_waves;
+  if (b->dcounts) grid = std::min<int64_t>(grid, kDeviceCountGrid);
   hipLaunchKernelGGL(bwt_chunk_replay_kernel, dim3((unsigned)grid), dim3(64 * l->chunk_waves),
                      replay_lds_bytes(x, l), (hipStream_t)stream, *x, *b, *l, stride);
   return report("chunk replay", hipGetLastError());

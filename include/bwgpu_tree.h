@@ -121,6 +121,10 @@ typedef struct BwtBatch {
   const uint32_t* ipos;       /* M: staging position of each query id's value (= spos of its slot) */
   int64_t* stage;             /* M staging buffer: finished values grouped by windows of query ids */
   int64_t* answers;           /* M, by query id */
+  const int64_t* dcounts;     /* NULL, or device counts [nqblk, ngrp, nwork, ...] written by the
+                                 device planner (include/bwgpu_plan.h); nqblk / ngrp / nwork
+                                 above are then upper bounds, and the replay kernels loop over
+                                 the device counts */
 } BwtBatch;
 
 /* The combine kernels map workgroups to slot tiles / query-id blocks XCD-major: on a part with

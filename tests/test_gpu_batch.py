@@ -39,3 +39,22 @@ def test_batched_full_stream(pkg):
     r = run_json(_cmd(pkg, g, ["--plain-sample", "3", "--tuned", pkg.TUNED]), timeout=900)
     assert r["answers_sha256"] == g["oracle"]
     assert r["plain"]["mismatch"] == 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case,knobs", [
+    ("s1_batched_small", []),
+    ("s1_batched_small", ["--rmax", "64", "--tile-shift", "6", "--chunk", "128", "--ep-per-work", "65",
+                          "--range-slots", "100", "--stage-shift", "8"]),
+    ("s2_batched_mid", ["--range-slots", "65536", "--ep-per-work", "129"]),
+    ("S4_batched_full", ["--tuned", None]),
+])
+def test_device_plan_equals_host_plan(pkg, case, knobs):
+    """The device planner (sorts, scans and searches on the GPU) builds the host planner's plan
+    array for array; the queries replayed through it give the oracle's answers."""
+    g = STREAMS[case]
+    knobs = [pkg.TUNED if k is None else k for k in knobs]
+    r = run_json(_cmd(pkg, g, knobs + ["--plain-sample", "0", "--device-plan"]), timeout=900)
+    assert r["device_plan"]["used"] and r["device_plan"]["mismatch"] == 0, (case, knobs)
+    assert r["answers_sha256"] == g["oracle"], (case, knobs)
+
