@@ -9,13 +9,17 @@ combine) over the index resident in HBM.  A "step" is one such replay over input
 bw_batch times each with HIP events on its stream.  The answers' SHA-256 is reported (the test
 suite pins it against the oracle).
 
+The batch is planned on the GPU (bwt_plan_launch: the host planner's plan, array for array; its
+time is reported beside the replay) and the replay runs over that device-planned batch.
+
 Beside it, measured in the same run:
   plain   -- the plain GPU drop-in at paper scale (config 2: 300,000 states, 9.6M queries, one
              launch sequence that rebuilds the index and answers the batch) and, on the batched
              stream's first batches, the same plain pass run once per batch;
   served  -- the served form (bw_serve): 20,000 Poisson requests (log-uniform sizes in [1, 4096],
-             half point / half range queries, 5% appended branch states) by continuous batching,
-             p50/p99 request latency;
+             half point / half range queries, 5% appended branch states) by continuous batching
+             with every batch planned on the GPU, p50/p99 request latency; and the same stream
+             offered all at once (saturated throughput);
   roofline -- the batched step's dominant kernel: its algorithmic bytes per launch over its
              average HIP-event duration against the MI355X HBM peak (8 TB/s); `traffic` is the
              rocprofv3 FETCH_SIZE + WRITE_SIZE of that kernel (profiles/), scaled by the same
@@ -70,8 +74,8 @@ def main():
         if not os.path.exists(exe):
             sys.exit("bench.py: native build missing (python -c 'import __graft_entry__ as g; g.build()')")
     knobs = ["--tuned", TUNED] if os.path.exists(TUNED) else []
-    r = run([BW_BATCH, "--warmup", str(a.warmup), "--repeat", str(a.steps), "--plain-sample", "4", "--stages"] + knobs,
-            1200)
+    r = run([BW_BATCH, "--warmup", str(a.warmup), "--repeat", str(a.steps), "--plain-sample", "4", "--stages",
+             "--device-plan"] + knobs, 1200)
     plain = run([BW_TREE, "--cfg", "2", "--mode", "pass", "--warmup", "2", "--repeat", "5"], 600)
     stages = {k: v for k, v in r["stage_ms"].items() if v > 0}
     dom = max(stages, key=stages.get)
@@ -113,6 +117,9 @@ def main():
                      "kernel_ms": round(stages[dom], 4), "kernel_bytes_alg": r["stage_bytes"][dom]},
         "stage_ms": r["stage_ms"],
         "index_build_ms": r["phases_ms"]["index_build"],
+        "device_plan": {"ms": r["device_plan"]["ms"], "plan_equals_host_plan": r["device_plan"]["mismatch"] == 0,
+                        "plan_and_query_queries_per_s": round(r["n_answers"] / ((r["device_plan"]["ms"] +
+                                                                                  r["graph_ms"]["avg"]) * 1e-3), 1)},
         "host_batch_plan_ms": r["phases_ms"]["batch_plan"],
         "device": r["device"],
     }
@@ -121,8 +128,12 @@ def main():
         out["served"] = {"queries_per_s": round(s["queries_per_s"], 1),
                          "offered_queries_per_s": round(s["offered_queries_per_s"], 1),
                          "latency_ms": s["latency_ms"], "requests": s["requests"], "appends": s["appends"],
-                         "batches": s["batches"], "answers_sha256": s["answers_sha256"],
-                         "alone_mismatch": s["alone"]["mismatch"]}
+                         "batches": s["batches"], "mean_batch_queries": s["mean_batch_queries"],
+                         "batch_gpu_ms": s["batch_gpu_ms"], "planner": s["planner"],
+                         "answers_sha256": s["answers_sha256"], "alone_mismatch": s["alone"]["mismatch"]}
+        z = run([BW_SERVE, "--check-alone", "0", "--rate", "1e9"] + knobs, 600)
+        out["served_saturated"] = {"queries_per_s": round(z["queries_per_s"], 1), "latency_ms": z["latency_ms"],
+                                   "batches": z["batches"], "answers_sha256": z["answers_sha256"]}
     if not a.no_cpu:
         p = subprocess.run([ORACLE, "stream", "--form", "4", "--requests", str(CPU_SAMPLE_BATCHES)],
                            capture_output=True, text=True, check=True, timeout=900)

@@ -104,6 +104,7 @@ int main(int argc, char** argv) {
     else if (!std::strcmp(k, "--ep-per-work")) { kn.ep_per_work = std::atoll(v); have_knob[3] = true; }
     else if (!std::strcmp(k, "--range-slots")) { kn.range_slots = std::atoll(v); have_knob[6] = true; }
     else if (!std::strcmp(k, "--stage-shift")) kn.stage_shift = std::atoi(v);
+    else if (!std::strcmp(k, "--bt-table-mb")) kn.bt_table_mb = std::atoll(v);
     else if (!std::strcmp(k, "--seg")) kn.seg_chunks = std::atoi(v);
     else if (!std::strcmp(k, "--threads")) { lk.block_threads = std::atoi(v); have_knob[4] = true; }
     else if (!std::strcmp(k, "--waves")) { lk.chunk_waves = std::atoi(v); have_knob[5] = true; }

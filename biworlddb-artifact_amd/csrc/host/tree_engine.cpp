@@ -83,11 +83,33 @@ bool TreeEngine::init(int64_t S, int U, uint64_t NK, const int32_t* parent, cons
   px.ep_per_work = P.knobs.ep_per_work;
   px.range_slots = P.knobs.range_slots;
   px.state_step = arena.upload(P.state_step);
-  px.state_block = arena.upload(P.state_block);
+  {
+    std::vector<int32_t> sb(P.steps.size());
+    for (int32_t b = 0; b < P.nblk(); ++b)
+      for (int32_t s = P.blk_step0[size_t(b)]; s < P.blk_step0[size_t(b) + 1]; ++s) sb[size_t(s)] = b;
+    px.step_block = arena.upload(sb);
+  }
   px.tile_tb0 = arena.upload(P.tile_tb0);
   px.tb_block = arena.upload(P.tb_block);
   px.tb_pos = arena.upload(P.tb_pos);
   px.chain_key0 = arena.upload(P.chain_key0);
+  // the endpoint positions tabulated per (block, tile), when the table fits its budget
+  px.bt_pos = nullptr;
+  const size_t nbt = size_t(P.nblk()) * size_t(P.KT);
+  if (nbt * sizeof(int32_t) <= size_t(std::max<int64_t>(0, P.knobs.bt_table_mb)) << 20) {
+    std::vector<int32_t> bt(nbt);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t t = 0; t < int64_t(P.KT); ++t) {
+      int64_t i = P.tile_tb0[size_t(t)];
+      const int64_t e = P.tile_tb0[size_t(t) + 1];
+      const int32_t end = int32_t(P.chain_key0[size_t(t) + 2] - P.chain_key0[size_t(t) + 1]);
+      for (int32_t b = 0; b < P.nblk(); ++b) {
+        while (i < e && P.tb_block[size_t(i)] < b) ++i;
+        bt[size_t(b) * size_t(P.KT) + size_t(t)] = i < e ? int32_t(P.tb_pos[size_t(i)]) : end;
+      }
+    }
+    px.bt_pos = arena.upload(bt);
+  }
   // the index-build batch: every block, no queries
   std::vector<int32_t> allb(size_t(P.nblk()));
   for (int32_t b = 0; b < P.nblk(); ++b) allb[size_t(b)] = b;

@@ -26,6 +26,8 @@ struct PlanKnobs {
   int seg_chunks = 32;          // chunks per carry-scan segment
   int stage_shift = 18;         // batch plans: query ids per staging window of the combine
                                 // (2^18 answers = 2 MiB, held in one die's cache)
+  int64_t bt_table_mb = 1024;   // device planner: largest (block, key tile) position table to
+                                // build (0: the planner searches instead)
   int max_block_steps = 1 << 30;
 };
 

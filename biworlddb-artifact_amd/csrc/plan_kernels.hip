@@ -5,16 +5,17 @@
 // The reference plans on the host before each launch (qlb/qhb bounds per query,
 // src/batch_engine_v7.hpp:666-698; radix-sort-driven plan build, :1069-1348).  The steps here
 // produce exactly the plan of the host planner build_batch_plan (csrc/host/tree_plan.cpp):
-//   1. slots      sort (enter step, query id) -> qorder; per slot its block, in-block ranks
-//                 (binary search in the block's sorted keys), key tiles;
+//   1. slots      sort (enter step, query id), carrying the endpoints -> qorder; per slot its
+//                 block, in-block ranks (binary search in the block's sorted keys), key tiles;
 //   2. groups     scan of "new step" / "new block" flags -> grp_step/grp_q0, qblk/qblk_g0;
-//   3. endpoints  per endpoint its level-B chain position (binary search over the blocks that
-//                 hold keys of its tile), chunk and item; sort (slot range, chunk, position)
-//                 keeping endpoint order;
+//   3. endpoints  per endpoint its level-B chain position (a dense (block, tile) table, or a
+//                 binary search over the blocks that hold keys of its tile), chunk and item;
+//                 sort (slot range, chunk, position) keeping endpoint order;
 //   4. work       run starts by a max-scan, cuts at every ep_per_work-th item of a run (moved
 //                 on past a slot's pair), scan -> wk_chunk/wk_e0, pair count;
 //   5. staging    sort slots by (query-id window, die part) -> ipos; sort those positions by
 //                 slot tile -> spos (ascending per tile) and srank.
+// Every sort carries what the next step needs, so no step gathers through the permutation.
 // Out-of-range queries are planned as state 1 over [0, 0) and flagged in dcounts[4].
 #include <hip/hip_runtime.h>
 
@@ -72,9 +73,9 @@ bool shape_of(const BwtIndex* x, const BwtPlanIndex* p, int64_t M, Shape& s) {
 }
 
 struct Layout {
-  size_t skey_in, skey_out, sval_in, flags, fscan;           // slots
-  size_t ekey_in, ekey_out, eval_in, wflag, wscan, a0in, a0out;  // endpoints
-  size_t gkey_in, gkey_out, gval_in, gval_out, tkey_in, tkey_out, tval_in;  // staging
+  size_t skey_in, skey_out, sval_in, sval_out, flags, fscan;           // slots
+  size_t ekey_in, ekey_out, eval_in, wflag, wscan, a0in, a0out;        // endpoints
+  size_t gkey_in, gkey_out, gval_in, gval_out, tkey_in, tkey_out, tval_in, tval_out;  // staging
   size_t temp, temp_bytes, total;
 };
 
@@ -88,8 +89,8 @@ hipError_t temp_bytes(const Shape& s, size_t& need) {
   need = 0;
   size_t b = 0;
   hipError_t e;
-  if ((e = rocprim::radix_sort_pairs(nullptr, b, (const uint32_t*)nullptr, (uint32_t*)nullptr,
-                                     (const uint32_t*)nullptr, (uint32_t*)nullptr, size_t(s.M), 0u, s.sbits)))
+  if ((e = rocprim::radix_sort_pairs(nullptr, b, (const uint64_t*)nullptr, (uint64_t*)nullptr,
+                                     (const uint64_t*)nullptr, (uint64_t*)nullptr, size_t(s.M), 32u, 32u + s.sbits)))
     return e;
   need = std::max(need, b);
   if ((e = rocprim::inclusive_scan(nullptr, b, (const uint64_t*)nullptr, (uint64_t*)nullptr, size_t(s.M),
@@ -111,7 +112,7 @@ hipError_t temp_bytes(const Shape& s, size_t& need) {
   need = std::max(need, b);
   for (const unsigned gb : {s.gbits, s.tbits}) {
     if ((e = rocprim::radix_sort_pairs(nullptr, b, (const uint32_t*)nullptr, (uint32_t*)nullptr,
-                                       (const uint32_t*)nullptr, (uint32_t*)nullptr, size_t(s.M), 0u, gb)))
+                                       (const uint64_t*)nullptr, (uint64_t*)nullptr, size_t(s.M), 0u, gb)))
       return e;
     need = std::max(need, b);
   }
@@ -126,9 +127,10 @@ hipError_t layout_of(const Shape& s, Layout& L) {
     return a;
   };
   const size_t M = size_t(s.M), E = size_t(s.nE);
-  L.skey_in = take(M * 4);
-  L.skey_out = take(M * 4);
-  L.sval_in = take(M * 4);
+  L.skey_in = take(M * 8);
+  L.skey_out = take(M * 8);
+  L.sval_in = take(M * 8);
+  L.sval_out = take(M * 8);
   L.flags = take(M * 8);
   L.fscan = take(M * 8);
   L.ekey_in = take(E * 8);
@@ -140,11 +142,12 @@ hipError_t layout_of(const Shape& s, Layout& L) {
   L.a0out = take(E * 4);
   L.gkey_in = take(M * 4);
   L.gkey_out = take(M * 4);
-  L.gval_in = take(M * 4);
-  L.gval_out = take(M * 4);
+  L.gval_in = take(M * 8);
+  L.gval_out = take(M * 8);
   L.tkey_in = take(M * 4);
   L.tkey_out = take(M * 4);
-  L.tval_in = take(M * 4);
+  L.tval_in = take(M * 8);
+  L.tval_out = take(M * 8);
   const hipError_t e = temp_bytes(s, L.temp_bytes);
   if (e) return e;
   L.temp = take(std::max<size_t>(L.temp_bytes, 1));
@@ -169,57 +172,51 @@ __device__ __forceinline__ int64_t lower_bound_i32(const int32_t* a, int64_t lo,
   return lo;
 }
 
-// a query with its out-of-range parts replaced (state 1, [0, 0))
-struct Q {
-  int32_t v;
-  uint32_t lo, hi;
-};
-__device__ __forceinline__ Q query_of(const BwtIndex& x, const BwtPlanIndex& p, const int32_t* qstate,
-                                      const BwtBatch& b, uint32_t q) {
-  Q r{qstate[q], b.qlo[q], b.qhi[q]};
-  if (r.v < 1 || r.v > x.S || r.lo > p.NK || r.hi > p.NK) r = Q{1, 0u, 0u};
-  return r;
-}
-
 __global__ void plan_init_kernel(int64_t* dcounts) {
   if (threadIdx.x < 5) dcounts[threadIdx.x] = 0;
 }
 
-// 1. sort keys of the slots: enter step, query id
+// 1. sort keys of the slots: enter step << 32 | query id; the values carry the endpoints
+//    (an out-of-range query becomes state 1 over [0, 0))
 __global__ void __launch_bounds__(kT) plan_slot_keys_kernel(const BwtIndex x, const BwtPlanIndex p,
                                                             const int32_t* qstate, const BwtBatch b, int64_t M,
-                                                            uint32_t* skey, uint32_t* sval, int64_t* dcounts) {
+                                                            uint64_t* skey, uint64_t* sval, int64_t* dcounts) {
   const int64_t q = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (q >= M) return;
   const int32_t v = qstate[q];
-  const uint32_t lo = b.qlo[q], hi = b.qhi[q];
+  uint32_t lo = b.qlo[q], hi = b.qhi[q];
   const bool bad = v < 1 || v > x.S || lo > p.NK || hi > p.NK;
-  if (bad) dcounts[4] = 1;
-  skey[q] = (uint32_t)p.state_step[bad ? 1 : v];
-  sval[q] = (uint32_t)q;
+  if (bad) {
+    dcounts[4] = 1;
+    lo = hi = 0;
+  }
+  skey[q] = ((uint64_t)(uint32_t)p.state_step[bad ? 1 : v] << 32) | (uint64_t)q;
+  sval[q] = (uint64_t)lo | ((uint64_t)hi << 32);
 }
 
-// per slot: block, in-block ranks, key tiles; flags "new step group" | "new block" << 32
+// per slot: query id, block, in-block ranks, key tiles; flags "new step group" | "new block" << 32
 __global__ void __launch_bounds__(kT) plan_slot_info_kernel(const BwtIndex x, const BwtPlanIndex p,
-                                                            const int32_t* qstate, const BwtBatch b, int64_t M,
-                                                            const uint32_t* sstep, uint64_t* flags) {
+                                                            const BwtBatch b, int64_t M, const uint64_t* skey,
+                                                            const uint64_t* sval, uint64_t* flags) {
   const int64_t j = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (j >= M) return;
-  const Q r = query_of(x, p, qstate, b, b.qorder[j]);
-  const int32_t blk = p.state_block[r.v];
+  const uint64_t key = skey[j], lh = sval[j];
+  const uint32_t s = (uint32_t)(key >> 32), lo = (uint32_t)lh, hi = (uint32_t)(lh >> 32);
+  const int32_t blk = p.step_block[s];
   const int64_t k0 = x.blk_key0[blk], n = x.blk_key0[blk + 1] - k0;
-  const uint32_t rl = lower_bound_u32(x.bkey + k0, n, r.lo), rh = lower_bound_u32(x.bkey + k0, n, r.hi);
+  const uint32_t rl = lower_bound_u32(x.bkey + k0, n, lo), rh = lower_bound_u32(x.bkey + k0, n, hi);
+  const_cast<uint32_t*>(b.qorder)[j] = (uint32_t)key;
   const_cast<int32_t*>(b.sblk)[j] = blk;
   const_cast<uint32_t*>(b.qrank)[j] = rl | (rh << 16);
-  const_cast<uint32_t*>(b.qtile)[j] = (r.lo >> x.tile_shift) | ((r.hi >> x.tile_shift) << 16);
-  const uint32_t s = sstep[j];
-  const bool g = j == 0 || sstep[j - 1] != s;
-  const bool nb = g && (j == 0 || (int64_t)sstep[j - 1] < (int64_t)x.blk_step0[blk]);
+  const_cast<uint32_t*>(b.qtile)[j] = (lo >> x.tile_shift) | ((hi >> x.tile_shift) << 16);
+  const uint32_t ps = j > 0 ? (uint32_t)(skey[j - 1] >> 32) : 0u;
+  const bool g = j == 0 || ps != s;
+  const bool nb = g && (j == 0 || (int64_t)ps < (int64_t)x.blk_step0[blk]);
   flags[j] = (uint64_t)g | ((uint64_t)nb << 32);
 }
 
 // 2. step groups and queried blocks from the scanned flags
-__global__ void __launch_bounds__(kT) plan_groups_kernel(const BwtBatch b, int64_t M, const uint32_t* sstep,
+__global__ void __launch_bounds__(kT) plan_groups_kernel(const BwtBatch b, int64_t M, const uint64_t* skey,
                                                          const uint64_t* flags, const uint64_t* fscan,
                                                          int64_t* dcounts) {
   const int64_t j = (int64_t)blockIdx.x * kT + threadIdx.x;
@@ -227,7 +224,7 @@ __global__ void __launch_bounds__(kT) plan_groups_kernel(const BwtBatch b, int64
   const uint64_t f = flags[j], c = fscan[j];
   const int64_t gi = (int64_t)(uint32_t)c - 1, bi = (int64_t)(c >> 32) - 1;
   if (f & 1u) {
-    const_cast<int32_t*>(b.grp_step)[gi] = (int32_t)sstep[j];
+    const_cast<int32_t*>(b.grp_step)[gi] = (int32_t)(skey[j] >> 32);
     const_cast<int64_t*>(b.grp_q0)[gi] = j;
   }
   if (f >> 32) {
@@ -244,21 +241,26 @@ __global__ void __launch_bounds__(kT) plan_groups_kernel(const BwtBatch b, int64
 }
 
 // 3. endpoint items and their sort keys (slot range, chunk, position)
-__global__ void __launch_bounds__(kT) plan_endpoints_kernel(const BwtIndex x, const BwtPlanIndex p,
-                                                            const int32_t* qstate, const BwtBatch b, int64_t nE,
-                                                            int64_t RS, int64_t CH, unsigned cb, unsigned pb,
-                                                            uint64_t* ekey, uint64_t* eval) {
+__global__ void __launch_bounds__(kT) plan_endpoints_kernel(const BwtIndex x, const BwtPlanIndex p, const BwtBatch b,
+                                                            int64_t nE, int64_t RS, int64_t CH, unsigned cb,
+                                                            unsigned pb, const uint64_t* sval, uint64_t* ekey,
+                                                            uint64_t* eval) {
   const int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (e >= nE) return;
   const int64_t j = e >> 1;
-  const Q r = query_of(x, p, qstate, b, b.qorder[j]);
-  const uint32_t xk = (e & 1) ? r.hi : r.lo;
+  const uint64_t lh = sval[j];
+  const uint32_t xk = (e & 1) ? (uint32_t)(lh >> 32) : (uint32_t)lh;
   const int32_t blk = b.sblk[j];
   // the keys of tile t in blocks < blk precede the endpoint on chain 1 + t
   const uint32_t t = xk >> x.tile_shift;
-  const int64_t lo = p.tile_tb0[t], hi = p.tile_tb0[t + 1];
-  const int64_t idx = lower_bound_i32(p.tb_block, lo, hi, blk);
-  const int64_t pos = idx < hi ? p.tb_pos[idx] : p.chain_key0[t + 2] - p.chain_key0[t + 1];
+  int64_t pos;
+  if (p.bt_pos) {
+    pos = p.bt_pos[(int64_t)blk * p.KT + t];
+  } else {
+    const int64_t lo = p.tile_tb0[t], hi = p.tile_tb0[t + 1];
+    const int64_t idx = lower_bound_i32(p.tb_block, lo, hi, blk);
+    pos = idx < hi ? p.tb_pos[idx] : p.chain_key0[t + 2] - p.chain_key0[t + 1];
+  }
   const int64_t c0 = x.chain_chunk0[1 + t], n = x.chain_chunk0[2 + t] - c0;
   int64_t k = pos / CH;
   if (k >= n) k = n - 1;
@@ -311,31 +313,35 @@ __global__ void __launch_bounds__(kT) plan_work_kernel(const BwtBatch b, int64_t
   }
 }
 
-// 5. staging: sort keys (query-id window, die part of the slot's tile), slot ...
+// 5. staging: sort keys (query-id window, die part of the slot's tile); values slot | id << 32 ...
 __global__ void __launch_bounds__(kT) plan_stage_keys_kernel(const BwtBatch b, int64_t M, int sh, int64_t per_t,
-                                                             uint32_t* gkey, uint32_t* gval) {
+                                                             uint32_t* gkey, uint64_t* gval) {
   const int64_t j = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (j >= M) return;
-  gkey[j] = (uint32_t)(((int64_t)(b.qorder[j] >> sh)) * BWT_XCDS + (j / BWT_COMBINE_TILE) / per_t);
-  gval[j] = (uint32_t)j;
+  const uint32_t q = b.qorder[j];
+  gkey[j] = (uint32_t)(((int64_t)(q >> sh)) * BWT_XCDS + (j / BWT_COMBINE_TILE) / per_t);
+  gval[j] = (uint64_t)j | ((uint64_t)q << 32);
 }
 
-// ... position p of slot gval[p] -> ipos; then sort the positions by slot tile ...
-__global__ void __launch_bounds__(kT) plan_stage_pos_kernel(const BwtBatch b, int64_t M, const uint32_t* gval,
-                                                            uint32_t* tkey, uint32_t* tval) {
+// ... position p of each slot -> ipos of its query; then sort the positions by slot tile ...
+__global__ void __launch_bounds__(kT) plan_stage_pos_kernel(const BwtBatch b, int64_t M, const uint64_t* gval,
+                                                            uint32_t* tkey, uint64_t* tval) {
   const int64_t p = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (p >= M) return;
-  const uint32_t j = gval[p];
-  const_cast<uint32_t*>(b.ipos)[b.qorder[j]] = (uint32_t)p;
+  const uint64_t v = gval[p];
+  const uint32_t j = (uint32_t)v;
+  const_cast<uint32_t*>(b.ipos)[v >> 32] = (uint32_t)p;
   tkey[p] = j / BWT_COMBINE_TILE;
-  tval[p] = (uint32_t)p;
+  tval[p] = (uint64_t)p | ((uint64_t)j << 32);
 }
 
-// ... every tile's positions now ascending in spos; each slot's index in its tile's list
-__global__ void __launch_bounds__(kT) plan_stage_rank_kernel(const BwtBatch b, int64_t M, const uint32_t* gval) {
+// ... every tile's positions now ascending: spos, and each slot's index in its tile's list
+__global__ void __launch_bounds__(kT) plan_stage_rank_kernel(const BwtBatch b, int64_t M, const uint64_t* tval) {
   const int64_t o = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (o >= M) return;
-  const uint32_t j = gval[b.spos[o]];
+  const uint64_t v = tval[o];
+  const uint32_t j = (uint32_t)(v >> 32);
+  const_cast<uint32_t*>(b.spos)[o] = (uint32_t)v;
   const_cast<uint16_t*>(b.srank)[j] = (uint16_t)(o - (int64_t)(j / BWT_COMBINE_TILE) * BWT_COMBINE_TILE);
 }
 
@@ -378,9 +384,10 @@ extern "C" int bwt_plan_launch(const BwtIndex* x, const BwtPlanIndex* p, const i
   const BwtBatch B = *b;
   void* temp = at<void>(work, L.temp);
   size_t tb = L.temp_bytes;
-  uint32_t* skey_in = at<uint32_t>(work, L.skey_in);
-  uint32_t* skey_out = at<uint32_t>(work, L.skey_out);
-  uint32_t* sval_in = at<uint32_t>(work, L.sval_in);
+  uint64_t* skey_in = at<uint64_t>(work, L.skey_in);
+  uint64_t* skey_out = at<uint64_t>(work, L.skey_out);
+  uint64_t* sval_in = at<uint64_t>(work, L.sval_in);
+  uint64_t* sval_out = at<uint64_t>(work, L.sval_out);
   uint64_t* flags = at<uint64_t>(work, L.flags);
   uint64_t* fscan = at<uint64_t>(work, L.fscan);
   uint64_t* ekey_in = at<uint64_t>(work, L.ekey_in);
@@ -392,11 +399,12 @@ extern "C" int bwt_plan_launch(const BwtIndex* x, const BwtPlanIndex* p, const i
   uint32_t* a0out = at<uint32_t>(work, L.a0out);
   uint32_t* gkey_in = at<uint32_t>(work, L.gkey_in);
   uint32_t* gkey_out = at<uint32_t>(work, L.gkey_out);
-  uint32_t* gval_in = at<uint32_t>(work, L.gval_in);
-  uint32_t* gval_out = at<uint32_t>(work, L.gval_out);
+  uint64_t* gval_in = at<uint64_t>(work, L.gval_in);
+  uint64_t* gval_out = at<uint64_t>(work, L.gval_out);
   uint32_t* tkey_in = at<uint32_t>(work, L.tkey_in);
   uint32_t* tkey_out = at<uint32_t>(work, L.tkey_out);
-  uint32_t* tval_in = at<uint32_t>(work, L.tval_in);
+  uint64_t* tval_in = at<uint64_t>(work, L.tval_in);
+  uint64_t* tval_out = at<uint64_t>(work, L.tval_out);
   int rc;
   const dim3 blk(kT);
   hipLaunchKernelGGL(plan_init_kernel, dim3(1), dim3(64), 0, st, dcounts);
@@ -404,11 +412,10 @@ extern "C" int bwt_plan_launch(const BwtIndex* x, const BwtPlanIndex* p, const i
   hipLaunchKernelGGL(plan_slot_keys_kernel, dim3(grid_of(M)), blk, 0, st, *x, *p, qstate, B, M, skey_in, sval_in,
                      dcounts);
   if ((rc = report("slot keys", hipGetLastError()))) return rc;
-  if ((rc = report("slot sort", rocprim::radix_sort_pairs(temp, tb, skey_in, skey_out, sval_in,
-                                                          const_cast<uint32_t*>(B.qorder), size_t(M), 0u, s.sbits,
-                                                          st))))
+  if ((rc = report("slot sort", rocprim::radix_sort_pairs(temp, tb, skey_in, skey_out, sval_in, sval_out, size_t(M),
+                                                          32u, 32u + s.sbits, st))))
     return rc;
-  hipLaunchKernelGGL(plan_slot_info_kernel, dim3(grid_of(M)), blk, 0, st, *x, *p, qstate, B, M, skey_out, flags);
+  hipLaunchKernelGGL(plan_slot_info_kernel, dim3(grid_of(M)), blk, 0, st, *x, *p, B, M, skey_out, sval_out, flags);
   if ((rc = report("slot info", hipGetLastError()))) return rc;
   // 2. groups
   tb = L.temp_bytes;
@@ -418,8 +425,8 @@ extern "C" int bwt_plan_launch(const BwtIndex* x, const BwtPlanIndex* p, const i
   hipLaunchKernelGGL(plan_groups_kernel, dim3(grid_of(M)), blk, 0, st, B, M, skey_out, flags, fscan, dcounts);
   if ((rc = report("groups", hipGetLastError()))) return rc;
   // 3. endpoints
-  hipLaunchKernelGGL(plan_endpoints_kernel, dim3(grid_of(s.nE)), blk, 0, st, *x, *p, qstate, B, s.nE, s.RS, s.CH,
-                     s.cb, s.pb, ekey_in, eval_in);
+  hipLaunchKernelGGL(plan_endpoints_kernel, dim3(grid_of(s.nE)), blk, 0, st, *x, *p, B, s.nE, s.RS, s.CH, s.cb, s.pb,
+                     sval_out, ekey_in, eval_in);
   if ((rc = report("endpoints", hipGetLastError()))) return rc;
   tb = L.temp_bytes;
   if ((rc = report("endpoint sort", rocprim::radix_sort_pairs(temp, tb, ekey_in, ekey_out, eval_in,
@@ -452,10 +459,9 @@ extern "C" int bwt_plan_launch(const BwtIndex* x, const BwtPlanIndex* p, const i
   hipLaunchKernelGGL(plan_stage_pos_kernel, dim3(grid_of(M)), blk, 0, st, B, M, gval_out, tkey_in, tval_in);
   if ((rc = report("stage positions", hipGetLastError()))) return rc;
   tb = L.temp_bytes;
-  if ((rc = report("tile sort", rocprim::radix_sort_pairs(temp, tb, tkey_in, tkey_out, tval_in,
-                                                          const_cast<uint32_t*>(B.spos), size_t(M), 0u, s.tbits,
-                                                          st))))
+  if ((rc = report("tile sort", rocprim::radix_sort_pairs(temp, tb, tkey_in, tkey_out, tval_in, tval_out, size_t(M),
+                                                          0u, s.tbits, st))))
     return rc;
-  hipLaunchKernelGGL(plan_stage_rank_kernel, dim3(grid_of(M)), blk, 0, st, B, M, gval_out);
+  hipLaunchKernelGGL(plan_stage_rank_kernel, dim3(grid_of(M)), blk, 0, st, B, M, tval_out);
   return report("tile ranks", hipGetLastError());
 }

@@ -39,9 +39,10 @@ namespace {
 constexpr int kQCap = 1024;  // query slots staged in LDS by the block replay
 constexpr int kGCap = 64;    // query step groups staged in LDS by the block replay
 constexpr int kEPT = 4;      // block replay: events per thread per staged tile of steps
-// grid of the replay kernels when the counts are on the device: a few resident workgroups per
-// CU on a 256-CU part, each looping over its share
-constexpr int64_t kDeviceCountGrid = 2048;
+// grid cap of the replay kernels when the counts are on the device: enough workgroups that the
+// dispatcher, not a fixed share per workgroup, balances uneven blocks / work items; workgroups
+// past the count leave at once
+constexpr int64_t kDeviceCountGrid = int64_t(1) << 16;
 
 // Padded LDS slot of in-block rank r when each thread owns RPT consecutive ranks: one spare
 // word per run keeps a wavefront's strided run accesses on distinct banks.

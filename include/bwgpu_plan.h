@@ -37,11 +37,13 @@ typedef struct BwtPlanIndex {
   int64_t ep_per_work;         /* endpoint items per work item (>= 64) */
   int64_t range_slots;         /* slots per range of the work order (>= 1) */
   const int32_t* state_step;   /* S+1: enter step of each state */
-  const int32_t* state_block;  /* S+1: block of that step */
+  const int32_t* step_block;   /* nsteps: block of each step */
   const int64_t* tile_tb0;     /* KT+1: offsets into tb_block / tb_pos per key tile */
   const int32_t* tb_block;     /* blocks holding keys of the tile, ascending */
   const int64_t* tb_pos;       /* where those keys start on the tile's level-B chain */
   const int64_t* chain_key0;   /* KT+2: key-item offsets of the chains (chain 1+t = tile t) */
+  const int32_t* bt_pos;       /* NULL, or nblk*KT: [b*KT+t] = position on chain 1+t before which
+                                  lie the tile's keys of blocks < b (the search above, tabulated) */
 } BwtPlanIndex;
 
 /* Workspace bytes for a batch of M queries (M >= 1). */

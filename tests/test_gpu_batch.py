@@ -47,6 +47,7 @@ def test_batched_full_stream(pkg):
     ("s1_batched_small", ["--rmax", "64", "--tile-shift", "6", "--chunk", "128", "--ep-per-work", "65",
                           "--range-slots", "100", "--stage-shift", "8"]),
     ("s2_batched_mid", ["--range-slots", "65536", "--ep-per-work", "129"]),
+    ("s2_batched_mid", ["--bt-table-mb", "0"]),  # positions by binary search instead of the table
     ("S4_batched_full", ["--tuned", None]),
 ])
 def test_device_plan_equals_host_plan(pkg, case, knobs):
