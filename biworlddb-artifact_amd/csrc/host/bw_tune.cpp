@@ -2,7 +2,8 @@
 // the chosen knobs in a tuned-knobs file read by bw_batch / bw_serve (--tuned FILE).
 //
 //   bw_tune --form batched|served [--S n --U n --N n --seed n] [--sample-requests R]
-//           [--repeat 3] --out FILE
+//           [--repeat 3] [--start FILE] --out FILE
+// --start: begin the descent from FILE's knobs for this form (else from the defaults).
 // The sample is the stream's first R requests (default: the whole batched stream; 400 served
 // requests, about the size of one served batch): knobs are chosen at the density of queries per
 // block the form actually runs at.
@@ -22,7 +23,7 @@
 using namespace bwg;
 
 int main(int argc, char** argv) {
-  std::string form = "batched", out;
+  std::string form = "batched", out, start;
   int64_t sample = -1;
   int repeat = 3;
   int cfg_form = 4;
@@ -42,6 +43,7 @@ int main(int argc, char** argv) {
     else if (!std::strcmp(k, "--seed")) spec.seed = std::strtoull(v, nullptr, 10);
     else if (!std::strcmp(k, "--sample-requests")) sample = std::atoll(v);
     else if (!std::strcmp(k, "--repeat")) repeat = std::atoi(v);
+    else if (!std::strcmp(k, "--start")) start = v;
     else if (!std::strcmp(k, "--out")) out = v;
     else { std::fprintf(stderr, "bw_tune: unknown argument %s\n", k); return 2; }
   }
@@ -65,8 +67,14 @@ int main(int argc, char** argv) {
     }
   if (hipSetDevice(0) != hipSuccess) return 5;
   std::vector<TuneTrial> log;
-  const TuneTrial best = autotune(w, w.key.data(), int64_t(qs.size()), qs.data(), lo.data(), hi.data(), PlanKnobs(),
-                                  LaunchKnobs(), repeat, log);
+  PlanKnobs k0;
+  LaunchKnobs l0;
+  if (!start.empty() && !load_tuned(start.c_str(), form.c_str(), k0, l0)) {
+    std::fprintf(stderr, "bw_tune: no %s entry in %s\n", form.c_str(), start.c_str());
+    return 2;
+  }
+  const TuneTrial best = autotune(w, w.key.data(), int64_t(qs.size()), qs.data(), lo.data(), hi.data(), k0, l0, repeat,
+                                  log);
   for (const TuneTrial& t : log)
     std::fprintf(stderr,
                  "{\"rmax\":%d,\"tile_shift\":%d,\"chunk_items\":%" PRId64 ",\"ep_per_work\":%" PRId64
