@@ -1,0 +1,60 @@
+"""Host planner (csrc/host/tree_plan.cpp) checked on the CPU: a lane-level emulation of the bwt_*
+kernels (tests/host/plan_emulator.cpp, test infrastructure) answers the golden tree cases from the
+product's plans; the SHA-256 must equal the oracle's.  Knob variants cut blocks, chunks, segments,
+work items and slot ranges at odd sizes and make many slots pair their level-B endpoints; a
+sampled batch (the planner's sorted path) must reproduce the full batch's answers element for
+element."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import golden
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+TREE = golden("tree_cases.json")
+
+
+@pytest.fixture(scope="module")
+def emulator(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("emu") / "plan_emulator")
+    host = os.path.join(REPO, "biworlddb-artifact_amd", "csrc", "host")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-fopenmp", "-I" + os.path.join(REPO, "include"), "-I" + host,
+                    "-o", out, os.path.join(REPO, "tests", "host", "plan_emulator.cpp"),
+                    os.path.join(host, "tree_plan.cpp"), os.path.join(host, "tree_workload.cpp"), "-lcrypto"],
+                   check=True, capture_output=True, timeout=300)
+    return out
+
+
+def _run(emu, case, knobs, extra=()):
+    g = TREE[case]
+    cmd = [emu, str(g["cfg"]), "--S", str(g["S"]), "--U", str(g["U"]), "--Q", str(g["Q"]), "--N", str(g["N"]),
+           "--seed", str(g["seed"])] + list(knobs) + list(extra)
+    p = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stderr[-2000:]
+    return p.stdout.strip().splitlines()[-1]
+
+
+@pytest.mark.parametrize("case,knobs", [
+    ("c1_cfg1", []),
+    ("c1_cfg1", ["--tile-shift", "12", "--ep-per-work", "65", "--range-slots", "100", "--stage-shift", "8"]),
+    ("t1_cfg1_small", ["--rmax", "64", "--tile-shift", "6", "--chunk", "256", "--seg", "2", "--threads", "64"]),
+    ("t2_cfg2_small", ["--tile-shift", "11", "--chunk", "64", "--ep-per-work", "67", "--range-slots", "333"]),
+    ("t3_cfg3_small", ["--rmax", "48", "--tile-shift", "5", "--chunk", "200", "--seg", "1"]),
+    ("t5_cfg2_dense", ["--rmax", "17", "--tile-shift", "3", "--chunk", "100", "--seg", "4", "--threads", "1024"]),
+    ("t6_cfg1_single", []),
+    ("t1_cfg1_small", ["--all", "0"]),
+])
+def test_emulated_kernels_over_host_plan_match_oracle(emulator, case, knobs):
+    assert _run(emulator, case, knobs) == TREE[case]["oracle"], (case, knobs)
+
+
+def test_sampled_batch_matches_full_batch(emulator, tmp_path):
+    full, part = str(tmp_path / "full.bin"), str(tmp_path / "part.bin")
+    _run(emulator, "c1_cfg1", [], ["--answers", full])
+    _run(emulator, "c1_cfg1", ["--subset", "7", "--all", "0"], ["--answers", part])
+    a = np.fromfile(full, dtype="<i8")
+    b = np.fromfile(part, dtype="<i8")
+    assert len(b) == (len(a) + 6) // 7
+    assert np.array_equal(a[::7], b)
