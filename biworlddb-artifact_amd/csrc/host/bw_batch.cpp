@@ -319,7 +319,7 @@ int main(int argc, char** argv) {
     for (size_t i = 0; i < P2.wk_chunk.size(); ++i) {
       const int64_t c = P2.wk_chunk[i];
       kit_run += std::min<int64_t>(E.plan.chunk_key0[size_t(c) + 1] - E.plan.chunk_key0[size_t(c)],
-                                   int64_t(P2.eitems[size_t(P2.wk_e0[i + 1] - 1)] >> BWT_EP_POS_SHIFT));
+                                   int64_t(BWT_EP_POS(P2.eitems[size_t(P2.wk_e0[i + 1] - 1)])));
     }
   }
   const double sb_block =

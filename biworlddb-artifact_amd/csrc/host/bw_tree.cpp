@@ -305,7 +305,7 @@ int main(int argc, char** argv) {
   for (size_t i = 0; i < BP.wk_chunk.size(); ++i) {
     const int64_t c = BP.wk_chunk[i];
     kit_run += std::min<int64_t>(P.chunk_key0[size_t(c) + 1] - P.chunk_key0[size_t(c)],
-                                 int64_t(BP.eitems[size_t(BP.wk_e0[i + 1] - 1)] >> BWT_EP_POS_SHIFT));
+                                 int64_t(BWT_EP_POS(BP.eitems[size_t(BP.wk_e0[i + 1] - 1)])));
   }
   const double nkeys_d = double(P.bkey.size()), ntile_d = double(P.tile_rank0.size());
   const int x_ncls_b = P.chain_ncls.size() > 1 ? P.chain_ncls[1] : 1;

@@ -37,7 +37,7 @@ bool build_tree_index(int64_t S, int U, uint64_t NK, const int32_t* parent, cons
   const uint64_t KT = (NK >> ts) + 1;
   if (KT > 4095) return fail("key space too large for the tile classes (NK >> tile_shift must be < 4095)");
   if (kn.rmax < U || kn.rmax > 8192) return fail("rmax must be in [U, 8192]");
-  if ((kn.chunk_items != 0 && (kn.chunk_items < 64 || kn.chunk_items >= (int64_t(1) << 20))) ||
+  if ((kn.chunk_items != 0 && (kn.chunk_items < 64 || kn.chunk_items >= (int64_t(1) << 19))) ||
       kn.seg_chunks < 1 || kn.ep_per_work < 64)
     return fail("bad chunking");
   for (int64_t v = 1; v <= S; ++v)
@@ -245,7 +245,7 @@ bool build_tree_index(int64_t S, int U, uint64_t NK, const int32_t* parent, cons
   //          of two >= 1024 with (key items / chunk) * classes * 8 B <= 64 MiB.
   auto auto_chunk = [](int64_t items, int64_t ncls) {
     int64_t c = 1024;
-    while (c < (int64_t(1) << 19) && (items / c) * ncls * 8 > (int64_t(64) << 20)) c <<= 1;
+    while (c < (int64_t(1) << 18) && (items / c) * ncls * 8 > (int64_t(64) << 20)) c <<= 1;
     return c;
   };
   if (P.knobs.chunk_items == 0) P.knobs.chunk_items = auto_chunk(P.chain_key0[KT + 1] - nC, int64_t(W));
@@ -437,7 +437,7 @@ bool build_batch_plan(const TreeIndexPlan& X, int64_t M, const int32_t* qstate, 
     for (int64_t i = 0; i < nE; ++i) ord[size_t(i)] = uint32_t(i);
     std::sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) {
       if (ech[a] != ech[b]) return ech[a] < ech[b];
-      const uint64_t pa = eit[a] >> BWT_EP_POS_SHIFT, pb = eit[b] >> BWT_EP_POS_SHIFT;
+      const uint64_t pa = BWT_EP_POS(eit[a]), pb = BWT_EP_POS(eit[b]);
       return pa != pb ? pa < pb : a < b;
     });
     for (int64_t i = 0; i < nE; ++i) {
@@ -485,7 +485,7 @@ bool build_batch_plan(const TreeIndexPlan& X, int64_t M, const int32_t* qstate, 
 #pragma omp parallel for schedule(dynamic, 8)
     for (int64_t c = 0; c < nchunk; ++c)
       std::stable_sort(B.eitems.begin() + all[size_t(c)], B.eitems.begin() + all[size_t(c) + 1],
-                       [](uint64_t a, uint64_t b) { return (a >> BWT_EP_POS_SHIFT) < (b >> BWT_EP_POS_SHIFT); });
+                       [](uint64_t a, uint64_t b) { return BWT_EP_POS(a) < BWT_EP_POS(b); });
     for (int64_t c = 0; c < nchunk; ++c)
       if (all[size_t(c) + 1] > all[size_t(c)]) {
         cfirst.push_back(c);
@@ -526,7 +526,10 @@ bool build_batch_plan(const TreeIndexPlan& X, int64_t M, const int32_t* qstate, 
   for (const Work& w : wk)  // pairs: a slot's two endpoints on one chain (same tile), adjacent
     for (int64_t i = w.e0; i + 1 < w.e1; ++i) {
       const uint32_t id = uint32_t(B.eitems[size_t(i)]);
-      if (!(id & 1u) && uint32_t(B.eitems[size_t(i) + 1]) == id + 1) ++B.npairs;
+      if (!(id & 1u) && uint32_t(B.eitems[size_t(i) + 1]) == id + 1) {
+        B.eitems[size_t(i)] |= uint64_t(1) << BWT_EP_PAIR_BIT;
+        ++B.npairs;
+      }
     }
   const bool ranged = !wk.empty() && wk.back().range != wk.front().range;
   if (ranged) std::stable_sort(wk.begin(), wk.end(), [](const Work& p, const Work& q) { return p.range < q.range; });

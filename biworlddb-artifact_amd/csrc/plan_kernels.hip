@@ -69,7 +69,7 @@ bool shape_of(const BwtIndex* x, const BwtPlanIndex* p, int64_t M, Shape& s) {
   s.pb = bits_for(uint64_t(s.CH));
   s.gbits = bits_for(uint64_t(s.nwin * BWT_XCDS - 1));
   s.tbits = bits_for(uint64_t(s.ntile - 1));
-  return s.rb + s.cb + s.pb <= 64 && s.pb <= 20;
+  return s.rb + s.cb + s.pb <= 64 && s.pb <= 19;
 }
 
 struct Layout {
@@ -300,6 +300,8 @@ __global__ void __launch_bounds__(kT) plan_work_kernel(const BwtBatch b, int64_t
   const int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (e >= nE) return;
   const uint64_t c = wscan[e];
+  if (wflag[e] >> 32)  // second endpoint of a slot's pair: flag the first
+    const_cast<uint64_t*>(b.eitems)[e - 1] |= uint64_t(1) << BWT_EP_PAIR_BIT;
   if (wflag[e] & 1u) {
     const int64_t w = (int64_t)(uint32_t)c - 1;
     const_cast<int64_t*>(b.wk_chunk)[w] = (int64_t)((ekey[e] >> pb) & ((uint64_t(1) << cb) - 1));

@@ -389,34 +389,34 @@ __device__ __forceinline__ void chunk_replay_one(const BwtIndex& x, const BwtBat
   const int64_t e1 = q.wk_e0[ci + 1];
   const int64_t k0 = x.chunk_key0[c];
   // keys after this work item's last endpoint are not needed
-  const int64_t kend = min(x.chunk_key0[c + 1], k0 + (int64_t)(q.eitems[e1 - 1] >> BWT_EP_POS_SHIFT));
+  const int64_t kend = min(x.chunk_key0[c + 1], k0 + (int64_t)BWT_EP_POS(q.eitems[e1 - 1]));
   const uint32_t* khi = reinterpret_cast<const uint32_t*>(x.kitems) + 1;  // little endian
+  // the items of the group starting at (e, k): every load is issued (an index past the work
+  // item's range is clamped to a valid one and the value ignored), so the number of loads in
+  // flight is the same on every path and a use waits only for its own group's loads
+  auto load_group = [&](int64_t e, int64_t k, uint64_t& it_, uint32_t& kx_, int64_t& kw_) {
+    const int64_t ei = e + lane < e1 ? e + lane : e1 - 1;
+    const int64_t kk = k + lane < kend ? k + lane : 0;
+    it_ = q.eitems[ei];
+    kx_ = khi[2 * kk];
+    kw_ = x.kweight[kk];
+  };
   int64_t ki = k0;
-  uint64_t it = (ep + lane < e1) ? q.eitems[ep + lane] : 0;
-  uint32_t kx = (ki + lane < kend) ? khi[2 * (ki + lane)] : 0;
-  int64_t kw = (ki + lane < kend) ? x.kweight[ki + lane] : 0;
   // the endpoint consumed last (the previous group's last taken lane): id and value
   uint32_t last_id = 0xFFFFFFFFu;
   int64_t last_val = 0;
-  wave_lds_sync();
-  while (true) {
+  // One group of 64 merged items whose items are in (it, kx, kw); the next group's items are
+  // loaded into (it2, kx2, kw2) as soon as its start is known.  Returns false after the last group.
+  auto group = [&](const uint64_t it, const uint32_t kx, const int64_t kw, uint64_t& it2, uint32_t& kx2,
+                   int64_t& kw2) -> bool {
     const bool ev = ep + lane < e1;
     // keys of this chunk still ahead of the endpoint (>= 0: earlier ones were consumed)
-    const int rel = ev ? (int)((int64_t)(it >> BWT_EP_POS_SHIFT) - (ki - k0)) : 0;
+    const int rel = ev ? (int)((int64_t)BWT_EP_POS(it) - (ki - k0)) : 0;
     const bool take = ev && lane + rel < 64;  // its merged index falls in this group
     const int ne = __popcll(ballot64(take));  // positions are sorted: a prefix of the lanes
     const int nk = (int)min((int64_t)(64 - ne), kend - ki);
     const int64_t ep2 = ep + ne, ki2 = ki + nk;
-    uint64_t it2 = 0;
-    uint32_t kx2 = 0;
-    int64_t kw2 = 0;
-    if (ep2 < e1) {  // uniform: prefetch the next group
-      if (ep2 + lane < e1) it2 = q.eitems[ep2 + lane];
-      if (ki2 + lane < kend) {
-        kx2 = khi[2 * (ki2 + lane)];
-        kw2 = x.kweight[ki2 + lane];
-      }
-    }
+    load_group(ep2, ki2, it2, kx2, kw2);
     const int kcls = (int)((kx >> (BWT_CLASS_SHIFT - 32)) & 0xFFF);
     if (lane < nk) {
       kcs[lane] = kcls;
@@ -440,32 +440,35 @@ __device__ __forceinline__ void chunk_replay_one(const BwtIndex& x, const BwtBat
         if (m + 3 < rl && c4.w < cls) val += w23.y;
       }
       // A slot whose two endpoints lie on this chain (same key tile, hence the same position)
-      // has them as adjacent items, lo first: the hi lane stores hi - lo, the lo lane nothing.
+      // has them as adjacent items, lo first (flagged): the hi lane stores hi - lo, the lo lane
+      // nothing.  The lo item is this group's previous lane, or the last endpoint consumed.
       const uint32_t id = (uint32_t)it;
       const uint32_t up_id = (uint32_t)__shfl_up((int)id, 1);
       const int64_t up_val = (int64_t)(((uint64_t)(uint32_t)__shfl_up((int)(uint32_t)((uint64_t)val >> 32), 1) << 32) |
                                        (uint64_t)(uint32_t)__shfl_up((int)(uint32_t)(uint64_t)val, 1));
-      const uint32_t dn_id = (uint32_t)__shfl_down((int)id, 1);
-      const uint32_t next_first = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)it2);
       const uint32_t prev_id = lane > 0 ? up_id : last_id;
       const int64_t prev_val = lane > 0 ? up_val : last_val;
-      const uint32_t next_id = lane < 63 ? dn_id : next_first;
-      const bool has_next = ep + lane + 1 < e1;
-      const bool lo_pair = !(id & 1u) && has_next && next_id == id + 1;
+      const bool lo_pair = (it >> BWT_EP_PAIR_BIT) & 1u;
       const bool hi_pair = (id & 1u) && prev_id == id - 1;
       if (take && !lo_pair) q.g[id] = hi_pair ? val - prev_val : val;
       last_id = (uint32_t)__builtin_amdgcn_readlane((int)id, ne - 1);
       last_val = readlane64(val, ne - 1);
     }
-    if (ep2 >= e1) break;
+    if (ep2 >= e1) return false;
     if (lane < nk)
       for (int f = kcls + 1; f <= ncls; f += f & -f) lds_add64(&fen[f], kw);
     wave_lds_sync();
     ep = ep2;
     ki = ki2;
-    it = it2;
-    kx = kx2;
-    kw = kw2;
+    return true;
+  };
+  // two register sets, alternating, so no loaded value is copied before it is used
+  uint64_t itA, itB;
+  uint32_t kxA, kxB;
+  int64_t kwA, kwB;
+  load_group(ep, ki, itA, kxA, kwA);
+  wave_lds_sync();
+  while (group(itA, kxA, kwA, itB, kxB, kwB) && group(itB, kxB, kwB, itA, kxA, kwA)) {
   }
 }
 

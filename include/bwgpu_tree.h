@@ -44,11 +44,15 @@ extern "C" {
 /* Key item of a carry chain: class << 40 | weight index (into agg for level B, tagg for C). */
 #define BWT_CLASS_SHIFT 40
 #define BWT_REF_MASK ((1ull << 40) - 1)
-/* Endpoint item of a batch: local position << 44 | class << 32 | endpoint id (2 q + side, q the
- * query id).
- * The position is the number of the chunk's key items that precede the endpoint. */
+/* Endpoint item of a batch: pair flag << 63 | local position << 44 | class << 32 | endpoint id
+ * (2 slot + side).  The position (19 bits) is the number of the chunk's key items that precede
+ * the endpoint; the pair flag marks the first of a slot's two endpoints when the second is the
+ * next item of the same work item (both on one chain: the replay stores only their difference). */
 #define BWT_EP_CLASS_SHIFT 32
 #define BWT_EP_POS_SHIFT 44
+#define BWT_EP_POS_MASK 0x7FFFFull
+#define BWT_EP_PAIR_BIT 63
+#define BWT_EP_POS(item) (((item) >> BWT_EP_POS_SHIFT) & BWT_EP_POS_MASK)
 
 /* Static, update-side structure (resident across batches). */
 typedef struct BwtIndex {

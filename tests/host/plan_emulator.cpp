@@ -260,7 +260,7 @@ int main(int argc, char** argv) {
     const int64_t e1 = B.wk_e0[ci + 1];
     CHECK(e1 > ep);
     const int64_t k0 = P.chunk_key0[c];
-    const int64_t kend = std::min<int64_t>(P.chunk_key0[c + 1], k0 + int64_t(B.eitems[e1 - 1] >> BWT_EP_POS_SHIFT));
+    const int64_t kend = std::min<int64_t>(P.chunk_key0[c + 1], k0 + int64_t(BWT_EP_POS(B.eitems[e1 - 1])));
     int64_t ki = k0;
     int64_t kcs[64];
     int64_t kws[64];
@@ -273,7 +273,7 @@ int main(int argc, char** argv) {
       for (int l = 0; l < 64; ++l) {
         const bool ev = ep + l < e1;
         it[l] = ev ? B.eitems[ep + l] : 0;
-        rel[l] = ev ? int(int64_t(it[l] >> BWT_EP_POS_SHIFT) - (ki - k0)) : 0;
+        rel[l] = ev ? int(int64_t(BWT_EP_POS(it[l])) - (ki - k0)) : 0;
         CHECK(rel[l] >= 0);
         take[l] = ev && l + rel[l] < 64;
       }
@@ -311,7 +311,8 @@ int main(int argc, char** argv) {
           const int64_t prev_val = l > 0 ? vals[l - 1] : last_val;
           const bool has_next = ep + l + 1 < e1;
           const uint32_t next_id = has_next ? uint32_t(B.eitems[ep + l + 1]) : 0;
-          const bool lo_pair = !(id & 1u) && has_next && next_id == id + 1;
+          const bool lo_pair = (it[l] >> BWT_EP_PAIR_BIT) & 1u;
+          CHECK(lo_pair == (!(id & 1u) && has_next && next_id == id + 1));
           const bool hi_pair = (id & 1u) && prev_id == id - 1;
           if (!lo_pair) { g[id] = hi_pair ? vals[l] - prev_val : vals[l]; g_set[id]++; }
           else g_set[id]++;
