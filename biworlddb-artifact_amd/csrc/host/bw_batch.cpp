@@ -254,6 +254,7 @@ int main(int argc, char** argv) {
       cmp("wk_chunk", d.wk_chunk, H.wk_chunk.data(), H.wk_chunk.size(), 8);
       cmp("wk_e0", d.wk_e0, H.wk_e0.data(), H.wk_e0.size(), 8);
       cmp("eitems", d.eitems, H.eitems.data(), H.eitems.size(), 8);
+      cmp("epos", d.epos, H.epos.data(), H.epos.size(), 4);
       cmp("spos", d.spos, H.spos.data(), H.spos.size(), 4);
       cmp("srank", d.srank, H.srank.data(), H.srank.size(), 2);
       cmp("ipos", d.ipos, H.ipos.data(), H.ipos.size(), 4);
@@ -286,14 +287,17 @@ int main(int argc, char** argv) {
   double st_ms[3] = {0, 0, 0};
   if (stages) {
     hipEvent_t ev[4];
+    BwtLaunch Lq = L;  // the query sequence's launches, as bwt_query_launch makes them
+    Lq.write_index = 0;
+    Lq.carry_in_block = 1;
     for (auto& e : ev) HIPX(hipEventCreate(&e));
     for (int i = 0; i < repeat; ++i) {
       HIPX(hipEventRecord(ev[0], st));
-      RUN(bwt_block_replay_launch(&E.x, QB, &L, st));
+      RUN(bwt_block_replay_launch(&E.x, QB, &Lq, st));
       HIPX(hipEventRecord(ev[1], st));
-      RUN(bwt_chunk_replay_launch(&E.x, QB, &L, st));
+      RUN(bwt_chunk_replay_launch(&E.x, QB, &Lq, st));
       HIPX(hipEventRecord(ev[2], st));
-      RUN(bwt_combine_launch(&E.x, QB, &L, st));
+      RUN(bwt_combine_launch(&E.x, QB, &Lq, st));
       HIPX(hipEventRecord(ev[3], st));
       HIPX(hipEventSynchronize(ev[3]));
       for (int k = 0; k < 3; ++k) {
@@ -308,11 +312,10 @@ int main(int argc, char** argv) {
   //  block replay: per replayed event its u16 rank + i64 delta, per step its id, per query its
   //    packed ranks (u32) + in-block result (i64)
   //  chunk replay: per replayed key item its class word (u32) + weight (i64), per work item its
-  //    chunk's carry row, per endpoint item (u64) its read and per stored result (one per
-  //    endpoint, one per same-tile pair) an i64
+  //    chunk's carry row, per endpoint item (u64) its read and its result (i64)
   //  combine: per slot its block and tiles (2 x u32), 2 carry-row reads, the in-block part, the
-  //    2 level-B words, its staging position (u32) and staged value; per query id its staging
-  //    position (u32), the staged value and the answer
+  //    2 level-B places (u32) and results, its staging index (u16) + position (u32) and staged
+  //    value; per query id its staging position (u32), the staged value and the answer
   int64_t kit_run = 0;
   {
     const BatchPlan& P2 = DB.plan;
@@ -325,9 +328,8 @@ int main(int argc, char** argv) {
   const double sb_block =
       double(DB.plan.steps_run) * spec.U * 10.0 + double(DB.plan.steps_run) * 4.0 + double(M) * 12.0;
   const double sb_chunk = double(kit_run) * 12.0 + double(DB.plan.wk_chunk.size()) * double(E.x.ncls_b) * 8.0 +
-                          double(DB.plan.eitems.size()) * 8.0 +
-                          double(int64_t(DB.plan.eitems.size()) - DB.plan.npairs) * 8.0;
-  const double sb_combine = double(M) * (8.0 + 16.0 + 8.0 + 16.0 + 4.0 + 8.0 + 4.0 + 8.0 + 8.0);
+                          double(DB.plan.eitems.size()) * 16.0;
+  const double sb_combine = double(M) * (8.0 + 16.0 + 8.0 + 8.0 + 16.0 + 2.0 + 4.0 + 8.0 + 4.0 + 8.0 + 8.0);
   const auto tx0 = clk::now();
   std::vector<int64_t> ans(static_cast<size_t>(M));
   HIPX(hipMemcpy(ans.data(), QB->answers, size_t(M) * sizeof(int64_t), hipMemcpyDeviceToHost));

@@ -119,7 +119,7 @@ size_t slot_device_bytes(const TreeEngine& E, int64_t cap) {
   size_t work = 0;
   if (bwt_plan_workspace(&E.x, &E.px, cap, &work) != 0) return 0;
   const size_t n = size_t(cap);
-  const size_t outputs = n * (4 + 8 + 4 + 8 + 4 + 4 + 4 + 4 + 16 + 16 + 16 + 4 + 2 + 4 + 8 + 16 + 8 + 8);
+  const size_t outputs = n * (4 + 8 + 4 + 8 + 4 + 4 + 4 + 4 + 16 + 16 + 16 + 8 + 4 + 2 + 4 + 8 + 16 + 8 + 8);
   return work + outputs + n * 20 + 4096;
 }
 

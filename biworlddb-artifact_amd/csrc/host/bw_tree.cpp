@@ -227,6 +227,7 @@ int main(int argc, char** argv) {
   auto run_pass = [&](bool timed) {
     BwtLaunch wl = L;
     wl.write_index = mode == "pass";
+    wl.carry_in_block = mode != "pass";  // as bwt_query_launch / bwt_pass_launch do
     if (timed) HIPX(hipEventRecord(ev[0], stream));
     if (cfg == 3)
       for (auto& r : rk) RUN(bwt_rank_remap_launch(&r, stream));
@@ -297,10 +298,10 @@ int main(int argc, char** argv) {
   //    order, the dense i64 rows out
   //  carry scan: the rows in and out; chunk replay: per walked key item its class word (u32) +
   //    weight (i64), per work item its chunk's carry row, per endpoint item (u64) its read and
-  //    per stored result (one per endpoint, one per same-tile pair) an i64; combine per slot:
-  //    block and tiles (2 x u32), 2 level-C row reads, the in-block part, the 2 level-B words,
-  //    its staging position (u32) and staged value; per query id: its staging position (u32),
-  //    the staged value, the answer
+  //    its result (i64); combine per slot: block and tiles (2 x u32), 2 level-C row reads, the
+  //    in-block part, the 2 level-B places (u32) and results, its staging index (u16) + position
+  //    (u32) and staged value; per query id: its staging position (u32), the staged value, the
+  //    answer
   int64_t steps_run = mode == "pass" ? int64_t(P.steps.size()) : BP.steps_run, kit_run = 0;
   for (size_t i = 0; i < BP.wk_chunk.size(); ++i) {
     const int64_t c = BP.wk_chunk[i];
@@ -316,8 +317,8 @@ int main(int argc, char** argv) {
   const double sb_hist = mode == "pass" ? double(P.kitems.size()) * 16.0 + nkit_b * 8.0 + double(P.n_cp) * 8.0 : 0.0;
   const double sb_scan = mode == "pass" ? double(P.n_cp) * 16.0 : 0.0;
   const double sb_replay = double(kit_run) * 12.0 + double(BP.wk_chunk.size()) * double(x_ncls_b) * 8.0 +
-                           double(BP.eitems.size()) * 8.0 + double(int64_t(BP.eitems.size()) - BP.npairs) * 8.0;
-  const double sb_combine = double(nq) * 80.0;
+                           double(BP.eitems.size()) * 16.0;
+  const double sb_combine = double(nq) * 90.0;
   char buf[8192];
   int n = std::snprintf(
       buf, sizeof(buf),

@@ -160,6 +160,7 @@ bool upload_batch(const TreeEngine& e, BatchPlan&& plan, const uint32_t* d_qlo, 
   b.wk_chunk = out.arena.upload(B.wk_chunk);
   b.wk_e0 = out.arena.upload(B.wk_e0);
   b.eitems = out.arena.upload(B.eitems);
+  b.epos = out.arena.upload(B.epos);
   b.spos = out.arena.upload(B.spos);
   b.srank = out.arena.upload(B.srank);
   b.ipos = out.arena.upload(B.ipos);
@@ -194,6 +195,7 @@ bool alloc_plan_slot(const TreeEngine& e, int64_t cap, DevicePlanSlot& s, char* 
   b.wk_chunk = s.arena.scratch<int64_t>(2 * n);
   b.wk_e0 = s.arena.scratch<int64_t>(2 * n + 1);
   b.eitems = s.arena.scratch<uint64_t>(2 * n);
+  b.epos = s.arena.scratch<uint32_t>(2 * n);
   b.spos = s.arena.scratch<uint32_t>(n);
   b.srank = s.arena.scratch<uint16_t>(n);
   b.ipos = s.arena.scratch<uint32_t>(n);

@@ -8,6 +8,7 @@
 #include <cstring>
 
 #include "bwgpu_tree.h"
+#include "plan_common.hpp"
 
 namespace bwg {
 
@@ -499,7 +500,7 @@ bool build_batch_plan(const TreeIndexPlan& X, int64_t M, const int32_t* qstate, 
   // only up to its own last endpoint), a cut that would part a slot's two endpoints moved one
   // item on; ordered by slot range, then chunk.  The device planner makes the same cuts.
   const int64_t G = std::max<int64_t>(2, X.knobs.ep_per_work);
-  const int64_t RS = std::max<int64_t>(1, X.knobs.range_slots);
+  const SlotRanges SR = slot_ranges(M, X.knobs.range_slots);
   auto slot_of = [&](int64_t e) { return int64_t(uint32_t(B.eitems[size_t(e)])) >> 1; };
   struct Work {
     int64_t range, chunk, e0, e1;
@@ -509,9 +510,9 @@ bool build_batch_plan(const TreeIndexPlan& X, int64_t M, const int32_t* qstate, 
     int64_t a0 = cstart[i];
     const int64_t ce = cstart[i + 1];
     while (a0 < ce) {
-      const int64_t r = slot_of(a0) / RS;
+      const int64_t r = range_of(slot_of(a0), SR);
       const int64_t e = std::partition_point(B.eitems.begin() + a0, B.eitems.begin() + ce, [&](uint64_t it) {
-                          return (int64_t(uint32_t(it)) >> 1) / RS == r;
+                          return range_of(int64_t(uint32_t(it)) >> 1, SR) == r;
                         }) - B.eitems.begin();
       int64_t a = a0;
       for (int64_t k = 1; a < e; ++k) {
@@ -526,10 +527,7 @@ bool build_batch_plan(const TreeIndexPlan& X, int64_t M, const int32_t* qstate, 
   for (const Work& w : wk)  // pairs: a slot's two endpoints on one chain (same tile), adjacent
     for (int64_t i = w.e0; i + 1 < w.e1; ++i) {
       const uint32_t id = uint32_t(B.eitems[size_t(i)]);
-      if (!(id & 1u) && uint32_t(B.eitems[size_t(i) + 1]) == id + 1) {
-        B.eitems[size_t(i)] |= uint64_t(1) << BWT_EP_PAIR_BIT;
-        ++B.npairs;
-      }
+      if (!(id & 1u) && uint32_t(B.eitems[size_t(i) + 1]) == id + 1) ++B.npairs;
     }
   const bool ranged = !wk.empty() && wk.back().range != wk.front().range;
   if (ranged) std::stable_sort(wk.begin(), wk.end(), [](const Work& p, const Work& q) { return p.range < q.range; });
@@ -548,6 +546,10 @@ bool build_batch_plan(const TreeIndexPlan& X, int64_t M, const int32_t* qstate, 
                 out.begin() + B.wk_e0[size_t(i)]);
     B.eitems.swap(out);
   }
+  // each endpoint's place in the work order (its level-B result is stored there)
+  B.epos.resize(size_t(nE));
+#pragma omp parallel for schedule(static) if (nE > 8192)
+  for (int64_t i = 0; i < nE; ++i) B.epos[size_t(uint32_t(B.eitems[size_t(i)]))] = uint32_t(i);
   return true;
 }
 

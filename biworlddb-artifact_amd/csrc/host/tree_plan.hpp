@@ -77,6 +77,7 @@ struct BatchPlan {
   std::vector<uint32_t> qtile;    // slot -> key tiles of the endpoints lo | hi << 16
   std::vector<int64_t> wk_chunk, wk_e0;
   std::vector<uint64_t> eitems;   // level B only, endpoint id = 2 * slot + side
+  std::vector<uint32_t> epos;     // endpoint id -> its index in eitems
   int64_t npairs = 0;             // slots whose two endpoints share a key tile (one level-B item
                                   // pair, adjacent in item order and in one work item)
   int64_t steps_run = 0;          // block-replay steps the batch needs (statistics)

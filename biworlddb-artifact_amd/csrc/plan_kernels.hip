@@ -12,7 +12,8 @@
 //                 binary search over the blocks that hold keys of its tile), chunk and item;
 //                 sort (slot range, chunk, position) keeping endpoint order;
 //   4. work       run starts by a max-scan, cuts at every ep_per_work-th item of a run (moved
-//                 on past a slot's pair), scan -> wk_chunk/wk_e0, pair count;
+//                 on past a slot's pair), scan -> wk_chunk/wk_e0, pair count; each endpoint's
+//                 index in the work order -> epos;
 //   5. staging    sort slots by (query-id window, die part) -> ipos; sort those positions by
 //                 slot tile -> spos (ascending per tile) and srank.
 // Every sort carries what the next step needs, so no step gathers through the permutation.
@@ -28,6 +29,9 @@
 
 #include "bwgpu_plan.h"
 #include "bwgpu_tree.h"
+#include "plan_common.hpp"
+
+using bwg::SlotRanges;
 
 namespace {
 
@@ -43,7 +47,8 @@ unsigned bits_for(uint64_t v) {
 struct Shape {
   int64_t M, nE, ntile, per_t, nwin;
   unsigned sbits, rb, cb, pb, gbits, tbits;
-  int64_t G, RS, CH;
+  int64_t G, CH;
+  SlotRanges sr;
   int sh, ts;
 };
 
@@ -59,10 +64,10 @@ bool shape_of(const BwtIndex* x, const BwtPlanIndex* p, int64_t M, Shape& s) {
   s.nwin = (M >> s.sh) + 1;
   s.ts = x->tile_shift;
   s.G = p->ep_per_work;
-  s.RS = p->range_slots;
+  s.sr = bwg::slot_ranges(M, p->range_slots);
   s.CH = p->chunk_items;
   s.sbits = bits_for(uint64_t(p->nsteps - 1));
-  s.rb = bits_for(uint64_t((M - 1) / s.RS));
+  s.rb = bits_for(uint64_t(bwg::range_of(M - 1, s.sr)));
   s.cb = bits_for(uint64_t(x->nchunk - 1));
   // a position within a chunk is at most the chunk length (an endpoint after a chain's last
   // key sits at the end of the chain's last chunk, which is at most one chunk long)
@@ -242,7 +247,7 @@ __global__ void __launch_bounds__(kT) plan_groups_kernel(const BwtBatch b, int64
 
 // 3. endpoint items and their sort keys (slot range, chunk, position)
 __global__ void __launch_bounds__(kT) plan_endpoints_kernel(const BwtIndex x, const BwtPlanIndex p, const BwtBatch b,
-                                                            int64_t nE, int64_t RS, int64_t CH, unsigned cb,
+                                                            int64_t nE, SlotRanges sr, int64_t CH, unsigned cb,
                                                             unsigned pb, const uint64_t* sval, uint64_t* ekey,
                                                             uint64_t* eval) {
   const int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x;
@@ -267,7 +272,7 @@ __global__ void __launch_bounds__(kT) plan_endpoints_kernel(const BwtIndex x, co
   const uint64_t pin = (uint64_t)(pos - k * CH);
   const uint64_t mask = (1u << x.tile_shift) - 1;
   eval[e] = (pin << BWT_EP_POS_SHIFT) | ((uint64_t)(xk & mask) << BWT_EP_CLASS_SHIFT) | (uint64_t)e;
-  ekey[e] = ((uint64_t)(j / RS) << (cb + pb)) | ((uint64_t)(c0 + k) << pb) | pin;
+  ekey[e] = ((uint64_t)bwg::range_of(j, sr) << (cb + pb)) | ((uint64_t)(c0 + k) << pb) | pin;
 }
 
 // 4. work items: run starts (for a max-scan) ...
@@ -300,8 +305,7 @@ __global__ void __launch_bounds__(kT) plan_work_kernel(const BwtBatch b, int64_t
   const int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (e >= nE) return;
   const uint64_t c = wscan[e];
-  if (wflag[e] >> 32)  // second endpoint of a slot's pair: flag the first
-    const_cast<uint64_t*>(b.eitems)[e - 1] |= uint64_t(1) << BWT_EP_PAIR_BIT;
+  const_cast<uint32_t*>(b.epos)[(uint32_t)b.eitems[e]] = (uint32_t)e;  // the endpoint's place in the work order
   if (wflag[e] & 1u) {
     const int64_t w = (int64_t)(uint32_t)c - 1;
     const_cast<int64_t*>(b.wk_chunk)[w] = (int64_t)((ekey[e] >> pb) & ((uint64_t(1) << cb) - 1));
@@ -371,7 +375,8 @@ extern "C" int bwt_plan_launch(const BwtIndex* x, const BwtPlanIndex* p, const i
   Shape s;
   if (!b || !qstate || !dcounts || !work || !shape_of(x, p, M, s)) return (int)hipErrorInvalidValue;
   if (!b->qlo || !b->qhi || !b->qblk || !b->qblk_g0 || !b->grp_step || !b->grp_q0 || !b->qorder || !b->qrank ||
-      !b->sblk || !b->qtile || !b->wk_chunk || !b->wk_e0 || !b->eitems || !b->spos || !b->srank || !b->ipos)
+      !b->sblk || !b->qtile || !b->wk_chunk || !b->wk_e0 || !b->eitems || !b->epos || !b->spos || !b->srank ||
+      !b->ipos)
     return (int)hipErrorInvalidValue;
   Layout L;
   hipError_t e = layout_of(s, L);
@@ -427,7 +432,7 @@ extern "C" int bwt_plan_launch(const BwtIndex* x, const BwtPlanIndex* p, const i
   hipLaunchKernelGGL(plan_groups_kernel, dim3(grid_of(M)), blk, 0, st, B, M, skey_out, flags, fscan, dcounts);
   if ((rc = report("groups", hipGetLastError()))) return rc;
   // 3. endpoints
-  hipLaunchKernelGGL(plan_endpoints_kernel, dim3(grid_of(s.nE)), blk, 0, st, *x, *p, B, s.nE, s.RS, s.CH, s.cb, s.pb,
+  hipLaunchKernelGGL(plan_endpoints_kernel, dim3(grid_of(s.nE)), blk, 0, st, *x, *p, B, s.nE, s.sr, s.CH, s.cb, s.pb,
                      sval_out, ekey_in, eval_in);
   if ((rc = report("endpoints", hipGetLastError()))) return rc;
   tb = L.temp_bytes;

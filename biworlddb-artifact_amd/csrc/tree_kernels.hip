@@ -77,6 +77,16 @@ __device__ __forceinline__ void block_replay_one(const BwtIndex& x, const BwtBat
   uint32_t* qbuf = reinterpret_cast<uint32_t*>(gstep + kGCap);   // kQCap packed ranks
   int64_t* sdel = reinterpret_cast<int64_t*>(qbuf + kQCap);      // kEPT*nt staged signed deltas
   uint16_t* srank = reinterpret_cast<uint16_t*>(sdel + kEPT * nt);  // kEPT*nt staged ranks
+  uint32_t* qtbuf = reinterpret_cast<uint32_t*>(srank + kEPT * nt);  // kQCap key tiles (carry_in_block)
+  const bool add_carry = l.carry_in_block && !l.write_index;
+  // the block's dense level-C row (level-C chunk b = block b); its entries stay in cache
+  const int64_t* crow = add_carry ? x.cp + x.chunk_cp0[q.qblk[bi]] : nullptr;
+  // key tiles of slot s's endpoints, lo | hi << 16
+  auto tiles_of = [&](int64_t s) -> uint32_t {
+    if (q.qtile) return q.qtile[s];
+    const uint32_t id = q.qorder[s];
+    return (q.qlo[id] >> x.tile_shift) | ((q.qhi[id] >> x.tile_shift) << 16);
+  };
   const int b = q.qblk[bi];
   const int64_t k0 = x.blk_key0[b];
   const int R = (int)(x.blk_key0[b + 1] - k0);
@@ -86,6 +96,13 @@ __device__ __forceinline__ void block_replay_one(const BwtIndex& x, const BwtBat
   // without the index write, nothing after the last queried step matters
   const int s1 = l.write_index ? x.blk_step0[b + 1] : (gi0 < ge ? q.grp_step[ge - 1] + 1 : s0);
   const int64_t qend = q.grp_q0[ge];
+  auto stage_queries = [&](int64_t a) {
+    for (int i = tid; i < kQCap; i += nt) {
+      const bool in = a + i < qend;
+      qbuf[i] = in ? q.qrank[a + i] : 0u;
+      if (add_carry) qtbuf[i] = in ? tiles_of(a + i) : 0u;
+    }
+  };
   auto stage_groups = [&](int64_t g) {
     for (int i = tid; i <= kGCap; i += nt) {
       const int64_t gg = g + i;
@@ -96,7 +113,7 @@ __device__ __forceinline__ void block_replay_one(const BwtIndex& x, const BwtBat
   for (int i = tid; i < span; i += nt) w[i] = 0;
   stage_groups(gi0);
   const int64_t qbeg = q.grp_q0[gi0];
-  for (int i = tid; i < kQCap; i += nt) qbuf[i] = (qbeg + i < qend) ? q.qrank[qbeg + i] : 0u;
+  stage_queries(qbeg);
   int64_t tot[RPT];
 #pragma unroll
   for (int j = 0; j < RPT; ++j) tot[j] = 0;
@@ -136,7 +153,7 @@ __device__ __forceinline__ void block_replay_one(const BwtIndex& x, const BwtBat
     const bool staged = nq <= kQCap;
     if (staged && qb > qwin + kQCap) {  // uniform
       qwin = qa;
-      for (int i = tid; i < kQCap; i += nt) qbuf[i] = (qa + i < qend) ? q.qrank[qa + i] : 0u;
+      stage_queries(qa);
     }
     rebuild();  // its barriers also publish qbuf
     // one endpoint per thread: even = lo, odd = hi of the same slot (nt is a multiple of 64,
@@ -148,7 +165,11 @@ __device__ __forceinline__ void block_replay_one(const BwtIndex& x, const BwtBat
       uint32_t rk = 0;
       if (live) rk = staged ? qbuf[qa - qwin + sl] : q.qrank[qa + sl];
       const int y = (j & 1) ? (int)(rk >> 16) : (int)(rk & 0xFFFFu);
-      const int64_t pv = live ? P[pslot<RPT>(y)] : 0;
+      int64_t pv = live ? P[pslot<RPT>(y)] : 0;
+      if (add_carry && live) {  // plus the carry of the earlier blocks' keys in lower tiles
+        const uint32_t tl = staged ? qtbuf[qa - qwin + sl] : tiles_of(qa + sl);
+        pv += crow[(j & 1) ? (tl >> 16) : (tl & 0xFFFFu)];
+      }
       const int64_t other = __shfl_xor(pv, 1);
       if (live && !(j & 1)) q.ansr[qa + sl] = other - pv;
     }
@@ -371,7 +392,8 @@ __device__ __forceinline__ int64_t fen_prefix12(const int64_t* fen, int y) {
 // items before key items at equal position): the group's keys go to LDS, every endpoint adds
 // the Fenwick prefix plus the group's keys that precede it with a smaller class (broadcast
 // reads), then the keys enter the tree.  The next group's items are loaded while this one runs
-// (only the high word of a key item, which holds its class).
+// (only the high word of a key item, which holds its class).  Results are stored in work order
+// (g[i] for eitems[i]): a group's taken lanes write one contiguous run.
 __device__ __forceinline__ void chunk_replay_one(const BwtIndex& x, const BwtBatch& q, const int stride,
                                                  const int64_t ci, int64_t* kws) {
   const int lane = threadIdx.x & 63;
@@ -402,9 +424,6 @@ __device__ __forceinline__ void chunk_replay_one(const BwtIndex& x, const BwtBat
     kw_ = x.kweight[kk];
   };
   int64_t ki = k0;
-  // the endpoint consumed last (the previous group's last taken lane): id and value
-  uint32_t last_id = 0xFFFFFFFFu;
-  int64_t last_val = 0;
   // One group of 64 merged items whose items are in (it, kx, kw); the next group's items are
   // loaded into (it2, kx2, kw2) as soon as its start is known.  Returns false after the last group.
   auto group = [&](const uint64_t it, const uint32_t kx, const int64_t kw, uint64_t& it2, uint32_t& kx2,
@@ -439,20 +458,8 @@ __device__ __forceinline__ void chunk_replay_one(const BwtIndex& x, const BwtBat
         if (m + 2 < rl && c4.z < cls) val += w23.x;
         if (m + 3 < rl && c4.w < cls) val += w23.y;
       }
-      // A slot whose two endpoints lie on this chain (same key tile, hence the same position)
-      // has them as adjacent items, lo first (flagged): the hi lane stores hi - lo, the lo lane
-      // nothing.  The lo item is this group's previous lane, or the last endpoint consumed.
-      const uint32_t id = (uint32_t)it;
-      const uint32_t up_id = (uint32_t)__shfl_up((int)id, 1);
-      const int64_t up_val = (int64_t)(((uint64_t)(uint32_t)__shfl_up((int)(uint32_t)((uint64_t)val >> 32), 1) << 32) |
-                                       (uint64_t)(uint32_t)__shfl_up((int)(uint32_t)(uint64_t)val, 1));
-      const uint32_t prev_id = lane > 0 ? up_id : last_id;
-      const int64_t prev_val = lane > 0 ? up_val : last_val;
-      const bool lo_pair = (it >> BWT_EP_PAIR_BIT) & 1u;
-      const bool hi_pair = (id & 1u) && prev_id == id - 1;
-      if (take && !lo_pair) q.g[id] = hi_pair ? val - prev_val : val;
-      last_id = (uint32_t)__builtin_amdgcn_readlane((int)id, ne - 1);
-      last_val = readlane64(val, ne - 1);
+      // results in work order: the taken lanes store one contiguous run
+      if (take) q.g[ep + lane] = val;
     }
     if (ep2 >= e1) return false;
     if (lane < nk)
@@ -490,8 +497,8 @@ __global__ void __launch_bounds__(256) bwt_chunk_replay_kernel(const BwtIndex x,
 // ---------------------------------------------------------------- combine
 // Two passes.  Slots (block order) add the level-C carry, read from the dense carry row of the
 // slot's block (level-C chunk b = block b) at the endpoints' key tiles, and the level-B carries,
-// which the chunk replay stored by slot, to the in-block part -- every read streams and the rows
-// stay in cache -- and store the value at the slot's staging position.  The staging buffer is
+// which the chunk replay stored in work order (the results of a die's slots lie in that die's
+// ranges of it), to the in-block part, and store the value at the slot's staging position.  The staging buffer is
 // grouped by windows of query ids (then by die part, then slot), so these stores fill each
 // window's lines in order.  Then query ids, each die taking its part, load their value from
 // their window (a few MiB, resident in that die's cache) and store the answers in order.
@@ -502,7 +509,10 @@ __device__ __forceinline__ int64_t xcd_major_block(int64_t per) {
 
 // One tile of BWT_COMBINE_TILE slots per workgroup: the finished values are put in LDS in the
 // order of their staging positions, then stored in that order, so the lanes of a store
-// instruction write runs of consecutive positions (one run per query-id window).
+// instruction write runs of consecutive positions (one run per query-id window).  With
+// kCarry the level-C carry is added here (read from the block's dense row); otherwise the block
+// replay already added it.
+template <bool kCarry>
 __global__ void __launch_bounds__(BWT_COMBINE_THREADS) bwt_combine_slots_kernel(const BwtIndex x, const BwtBatch q,
                                                                                 int64_t per) {
   constexpr int kPer = BWT_COMBINE_TILE / BWT_COMBINE_THREADS;  // slots per thread
@@ -517,26 +527,33 @@ __global__ void __launch_bounds__(BWT_COMBINE_THREADS) bwt_combine_slots_kernel(
 #pragma unroll
   for (int k = 0; k < kPer; ++k) {
     const int64_t s = min(t0 + k * BWT_COMBINE_THREADS + (int64_t)threadIdx.x, last);
-    blk[k] = q.sblk[s];
-    if (q.qtile) {
-      tl[k] = q.qtile[s];
-    } else {  // tiles of the endpoint coordinates (e.g. ranks the device just remapped)
-      const uint32_t id = q.qorder[s];
-      tl[k] = (q.qlo[id] >> x.tile_shift) | ((q.qhi[id] >> x.tile_shift) << 16);
+    if (kCarry) {
+      blk[k] = q.sblk[s];
+      if (q.qtile) {
+        tl[k] = q.qtile[s];
+      } else {  // tiles of the endpoint coordinates (e.g. ranks the device just remapped)
+        const uint32_t id = q.qorder[s];
+        tl[k] = (q.qlo[id] >> x.tile_shift) | ((q.qhi[id] >> x.tile_shift) << 16);
+      }
     }
-    // level B, stored by slot; endpoints on one chain stored their difference in the hi word
-    const longlong2 gb = *reinterpret_cast<const longlong2*>(q.g + 2 * s);
-    part[k] = q.ansr[s] + ((tl[k] >> 16) == (tl[k] & 0xFFFFu) ? gb.y : gb.y - gb.x);
+    // level B: the slot's two results, stored in work order (one die's range of it)
+    const uint2 ep = *reinterpret_cast<const uint2*>(q.epos + 2 * s);
+    part[k] = q.ansr[s] + (q.g[ep.y] - q.g[ep.x]);
   }
-  int64_t r0[kPer];
+  if (kCarry) {
+    int64_t r0[kPer];
 #pragma unroll
-  for (int k = 0; k < kPer; ++k) r0[k] = x.chunk_cp0[blk[k]];
+    for (int k = 0; k < kPer; ++k) r0[k] = x.chunk_cp0[blk[k]];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int64_t* row = x.cp + r0[k];
+      part[k] += row[tl[k] >> 16] - row[tl[k] & 0xFFFFu];
+    }
+  }
 #pragma unroll
   for (int k = 0; k < kPer; ++k) {
     const int64_t s = t0 + k * BWT_COMBINE_THREADS + threadIdx.x;
-    const int64_t* row = x.cp + r0[k];
-    const int64_t v = part[k] + (row[tl[k] >> 16] - row[tl[k] & 0xFFFFu]);
-    if (s <= last) sv[q.srank[s]] = v;
+    if (s <= last) sv[q.srank[s]] = part[k];
   }
   __syncthreads();
   for (int k = threadIdx.x; k < BWT_COMBINE_TILE; k += BWT_COMBINE_THREADS) {
@@ -600,8 +617,9 @@ int block_rpt(const BwtIndex* x, int nt) {
 size_t block_lds_bytes(int rpt, int nt) {
   const size_t span = (size_t)(rpt > 1 ? nt * rpt + nt : nt * rpt) + 1;
   return (2 * span + 16 + kGCap + 1) * sizeof(int64_t) + kGCap * sizeof(int32_t) + kQCap * sizeof(uint32_t) +
-         (size_t)kEPT * nt * (sizeof(int64_t) + sizeof(uint16_t));
+         (size_t)kEPT * nt * (sizeof(int64_t) + sizeof(uint16_t)) + kQCap * sizeof(uint32_t);
 }
+
 
 // LDS stride (entries per wavefront) of a level: its classes + 1, rounded to an odd number of
 // 8-byte words so consecutive wavefronts' rows start in different banks
@@ -730,14 +748,18 @@ extern "C" int bwt_chunk_replay_launch(const BwtIndex* x, const BwtBatch* b, con
 }
 
 extern "C" int bwt_combine_launch(const BwtIndex* x, const BwtBatch* b, const BwtLaunch* l, void* stream) {
-  (void)l;
   if (!x || !b || b->M < 0) return (int)hipErrorInvalidValue;
   if (b->M == 0) return 0;
   if (!b->spos || !b->srank || !b->ipos || !b->stage) return (int)hipErrorInvalidValue;
   const int64_t nt = (b->M + BWT_COMBINE_TILE - 1) / BWT_COMBINE_TILE;
   const int64_t per_t = (nt + BWT_XCDS - 1) / BWT_XCDS;
-  hipLaunchKernelGGL(bwt_combine_slots_kernel, dim3((unsigned)(per_t * BWT_XCDS)), dim3(BWT_COMBINE_THREADS), 0,
-                     (hipStream_t)stream, *x, *b, per_t);
+  const bool carry = !(l && l->carry_in_block && !l->write_index);  // else the block replay added it
+  if (carry)
+    hipLaunchKernelGGL(bwt_combine_slots_kernel<true>, dim3((unsigned)(per_t * BWT_XCDS)), dim3(BWT_COMBINE_THREADS), 0,
+                       (hipStream_t)stream, *x, *b, per_t);
+  else
+    hipLaunchKernelGGL(bwt_combine_slots_kernel<false>, dim3((unsigned)(per_t * BWT_XCDS)), dim3(BWT_COMBINE_THREADS),
+                       0, (hipStream_t)stream, *x, *b, per_t);
   int rc = report("combine (slots)", hipGetLastError());
   if (rc) return rc;
   const int64_t nb = (b->M + BWT_COMBINE_THREADS - 1) / BWT_COMBINE_THREADS;
@@ -750,6 +772,7 @@ extern "C" int bwt_combine_launch(const BwtIndex* x, const BwtBatch* b, const Bw
 extern "C" int bwt_query_launch(const BwtIndex* x, const BwtBatch* b, const BwtLaunch* l, void* stream) {
   BwtLaunch q = *l;
   q.write_index = 0;
+  q.carry_in_block = 1;
   int rc;
   if ((rc = bwt_block_replay_launch(x, b, &q, stream))) return rc;
   if ((rc = bwt_chunk_replay_launch(x, b, &q, stream))) return rc;
@@ -759,6 +782,7 @@ extern "C" int bwt_query_launch(const BwtIndex* x, const BwtBatch* b, const BwtL
 extern "C" int bwt_pass_launch(const BwtIndex* x, const BwtBatch* b, const BwtLaunch* l, void* stream) {
   BwtLaunch w = *l;
   w.write_index = 1;
+  w.carry_in_block = 0;
   int rc;
   if ((rc = bwt_block_replay_launch(x, b, &w, stream))) return rc;
   if ((rc = bwt_chunk_hist_launch(x, &w, stream))) return rc;

@@ -52,7 +52,7 @@ int bwt_plan_workspace(const BwtIndex* x, const BwtPlanIndex* p, int64_t M, size
 /* Plan M queries (1 <= M < 2^31): qstate[M] in [1, S], b->qlo / b->qhi[M] in [0, NK].
  * Output arrays of b, with capacities: qblk min(nblk, M); qblk_g0 min(nblk, M)+1; grp_step M;
  * grp_q0 M+1; qorder, qrank, sblk, qtile, spos, srank, ipos M; wk_chunk 2M; wk_e0 2M+1;
- * eitems 2M (the planner writes through these pointers).  Sets b->M, b->dcounts = dcounts and
+ * eitems, epos 2M (the planner writes through these pointers).  Sets b->M, b->dcounts = dcounts and
  * b->nqblk / b->ngrp / b->nwork to those capacities (upper bounds).  dcounts[5]: nqblk, ngrp,
  * nwork, npairs, and an error flag (nonzero: some query was out of range; it was planned as a
  * query of state 1 over [0, 0), and the batch's answers are not meaningful). */

@@ -44,14 +44,11 @@ extern "C" {
 /* Key item of a carry chain: class << 40 | weight index (into agg for level B, tagg for C). */
 #define BWT_CLASS_SHIFT 40
 #define BWT_REF_MASK ((1ull << 40) - 1)
-/* Endpoint item of a batch: pair flag << 63 | local position << 44 | class << 32 | endpoint id
- * (2 slot + side).  The position (19 bits) is the number of the chunk's key items that precede
- * the endpoint; the pair flag marks the first of a slot's two endpoints when the second is the
- * next item of the same work item (both on one chain: the replay stores only their difference). */
+/* Endpoint item of a batch: local position << 44 | class << 32 | endpoint id (2 slot + side).
+ * The position (19 bits) is the number of the chunk's key items that precede the endpoint. */
 #define BWT_EP_CLASS_SHIFT 32
 #define BWT_EP_POS_SHIFT 44
 #define BWT_EP_POS_MASK 0x7FFFFull
-#define BWT_EP_PAIR_BIT 63
 #define BWT_EP_POS(item) (((item) >> BWT_EP_POS_SHIFT) & BWT_EP_POS_MASK)
 
 /* Static, update-side structure (resident across batches). */
@@ -116,9 +113,9 @@ typedef struct BwtBatch {
   const int64_t* wk_chunk;    /* nwork chunk ids */
   const int64_t* wk_e0;       /* nwork+1 offsets into eitems */
   const uint64_t* eitems;     /* level-B endpoint items per work item, by position; id = 2*slot+side */
+  const uint32_t* epos;       /* 2M, by endpoint id: the endpoint's index in eitems (and in g) */
   int64_t* ansr;              /* M, by slot: in-block part */
-  int64_t* g;                 /* 2M, by endpoint id (2*slot+side): level-B carry; a slot whose
-                                 endpoints share a key tile holds hi - lo at 2*slot+1 */
+  int64_t* g;                 /* 2M, in work order (g[i] belongs to eitems[i]): level-B carry */
   const uint32_t* spos;       /* M: staging positions of the slots of each tile of
                                  BWT_COMBINE_TILE slots, ascending within the tile */
   const uint16_t* srank;      /* M: index of the slot's staging position in its tile's list */
@@ -147,7 +144,10 @@ typedef struct BwtLaunch {
                           /* block replay: each thread owns the smallest power of two >= rmax /
                            * block_threads of the block's ranks (at most 16) */
   int32_t write_index;    /* block replay also writes agg / tagg (needs every block listed) */
-  int32_t pad0;
+  int32_t carry_in_block; /* query side (write_index 0, index built): the block replay also adds
+                             the level-C carry (its block's dense row) to every slot's in-block
+                             part, and the combine does not; bwt_query_launch sets it,
+                             bwt_pass_launch clears it */
 } BwtLaunch;
 
 /* Index build: agg/tagg are written by a block replay with write_index set; then: */

@@ -264,8 +264,6 @@ int main(int argc, char** argv) {
     int64_t ki = k0;
     int64_t kcs[64];
     int64_t kws[64];
-    uint32_t last_id = 0xFFFFFFFFu;
-    int64_t last_val = 0;
     while (true) {
       ++groups;
       int rel[64], take[64], ne = 0;
@@ -307,18 +305,10 @@ int main(int argc, char** argv) {
         for (int l = 0; l < ne; ++l) {
           const uint32_t id = uint32_t(it[l]);
           CHECK(id < 2 * nq);
-          const uint32_t prev_id = l > 0 ? uint32_t(it[l - 1]) : last_id;
-          const int64_t prev_val = l > 0 ? vals[l - 1] : last_val;
-          const bool has_next = ep + l + 1 < e1;
-          const uint32_t next_id = has_next ? uint32_t(B.eitems[ep + l + 1]) : 0;
-          const bool lo_pair = (it[l] >> BWT_EP_PAIR_BIT) & 1u;
-          CHECK(lo_pair == (!(id & 1u) && has_next && next_id == id + 1));
-          const bool hi_pair = (id & 1u) && prev_id == id - 1;
-          if (!lo_pair) { g[id] = hi_pair ? vals[l] - prev_val : vals[l]; g_set[id]++; }
-          else g_set[id]++;
+          CHECK(B.epos[id] == uint64_t(ep + l));  // the endpoint's place in the work order
+          g[ep + l] = vals[l];  // results in work order
+          g_set[ep + l]++;
         }
-        last_id = uint32_t(it[ne - 1]);
-        last_val = vals[ne - 1];
       }
       if (ep + ne >= e1) break;
       for (int l = 0; l < nk; ++l)
@@ -337,7 +327,7 @@ int main(int argc, char** argv) {
     CHECK(B.qtile[sl] == ((lc[id] >> ts) | ((hc[id] >> ts) << 16)));
     CHECK(B.islot[id] == sl);
     const uint32_t tl = B.qtile[sl];
-    const int64_t lb = (tl >> 16) == (tl & 0xFFFFu) ? g[2 * sl + 1] : g[2 * sl + 1] - g[2 * sl];
+    const int64_t lb = g[B.epos[2 * sl + 1]] - g[B.epos[2 * sl]];
     ansr[sl] += row[tl >> 16] - row[tl & 0xFFFF] + lb;
   }
   std::vector<int64_t> stage(nq, 0);
