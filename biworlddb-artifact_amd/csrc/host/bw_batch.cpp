@@ -79,6 +79,7 @@ int main(int argc, char** argv) {
   LaunchKnobs lk;
   int warmup = 1, repeat = 3, plain_sample = 8;
   bool want_sha = false, stages = false, dev_plan = false;
+  int fork = 1;  // replay the block and level-B stages on two streams
   std::string answers_path, json_path, tuned_path;
   bool have_knob[8] = {false, false, false, false, false, false, false, false};
   for (int i = 1; i < argc; ++i) {
@@ -105,6 +106,7 @@ int main(int argc, char** argv) {
     else if (!std::strcmp(k, "--range-slots")) { kn.range_slots = std::atoll(v); have_knob[6] = true; }
     else if (!std::strcmp(k, "--stage-shift")) kn.stage_shift = std::atoi(v);
     else if (!std::strcmp(k, "--bt-table-mb")) kn.bt_table_mb = std::atoll(v);
+    else if (!std::strcmp(k, "--fork")) fork = std::atoi(v);
     else if (!std::strcmp(k, "--seg")) kn.seg_chunks = std::atoi(v);
     else if (!std::strcmp(k, "--threads")) { lk.block_threads = std::atoi(v); have_knob[4] = true; }
     else if (!std::strcmp(k, "--waves")) { lk.chunk_waves = std::atoi(v); have_knob[5] = true; }
@@ -263,12 +265,18 @@ int main(int argc, char** argv) {
   }
   // the whole stream: one query launch sequence, captured once, replayed
   BwtLaunch L = E.launch;
-  RUN(bwt_query_launch(&E.x, QB, &L, st));  // warm (also sets the kernels' LDS limits)
+  hipStream_t side;
+  hipEvent_t fk, jn;
+  HIPX(hipStreamCreate(&side));
+  HIPX(hipEventCreateWithFlags(&fk, hipEventDisableTiming));
+  HIPX(hipEventCreateWithFlags(&jn, hipEventDisableTiming));
+  auto query = [&]() { return fork ? query_forked(E, QB, st, side, fk, jn) : bwt_query_launch(&E.x, QB, &L, st); };
+  RUN(query());  // warm (also sets the kernels' LDS limits)
   HIPX(hipStreamSynchronize(st));
   hipGraph_t graph;
   hipGraphExec_t gexec;
   HIPX(hipStreamBeginCapture(st, hipStreamCaptureModeGlobal));
-  RUN(bwt_query_launch(&E.x, QB, &L, st));
+  RUN(query());
   HIPX(hipStreamEndCapture(st, &graph));
   HIPX(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
   for (int i = 0; i < warmup; ++i) HIPX(hipGraphLaunch(gexec, st));
@@ -402,7 +410,7 @@ int main(int argc, char** argv) {
       "\"stage_bytes\":{\"block_replay\":%.0f,\"chunk_replay\":%.0f,\"combine\":%.0f},"
       "\"plain\":{\"batches\":%" PRId64 ",\"queries\":%" PRId64 ",\"pass_ms\":%.4f,\"query_ms\":%.4f,"
       "\"pass_queries_per_s\":%.1f,\"query_queries_per_s\":%.1f,\"mismatch\":%" PRId64 "},"
-      "\"device_plan\":{\"used\":%s,\"ms\":%.4f,\"mismatch\":%" PRId64 "},"
+      "\"device_plan\":{\"used\":%s,\"ms\":%.4f,\"mismatch\":%" PRId64 "},\"forked\":%d,"
       "\"device_bytes\":%zu,\"device\":\"%s\",\"cus\":%d}\n",
       spec.S, spec.U, spec.N, spec.seed, ss.nreq, ss.max_batch, M, sha.c_str(), E.plan.nblk(), BP.qblk.size(),
       BP.grp_step.size(), BP.eitems.size(), BP.wk_chunk.size(), E.plan.knobs.rmax, E.plan.knobs.tile_shift,
@@ -413,7 +421,7 @@ int main(int argc, char** argv) {
       sb_block, sb_chunk, sb_combine, K, plain_q, plain_pass_ms,
       plain_query_ms, plain_pass_ms > 0 ? double(plain_q) / (plain_pass_ms * 1e-3) : 0.0,
       plain_query_ms > 0 ? double(plain_q) / (plain_query_ms * 1e-3) : 0.0, plain_mismatch,
-      dev_plan ? "true" : "false", dplan_ms, plan_mismatch, E.arena.bytes + io.bytes + DB.arena.bytes + dslot.arena.bytes,
+      dev_plan ? "true" : "false", dplan_ms, plan_mismatch, fork, E.arena.bytes + io.bytes + DB.arena.bytes + dslot.arena.bytes,
       dev, prop.multiProcessorCount);
   std::fwrite(buf, 1, size_t(n), stdout);
   if (!json_path.empty()) {

@@ -176,6 +176,22 @@ bool upload_batch(const TreeEngine& e, BatchPlan&& plan, const uint32_t* d_qlo, 
   return true;
 }
 
+int query_forked(const TreeEngine& e, const BwtBatch* b, hipStream_t st, hipStream_t side, hipEvent_t fork,
+                 hipEvent_t join) {
+  BwtLaunch l = e.launch;  // as bwt_query_launch sets it
+  l.write_index = 0;
+  l.carry_in_block = 1;
+  hipError_t he;
+  int rc;
+  if ((he = hipEventRecord(fork, st)) != hipSuccess || (he = hipStreamWaitEvent(side, fork, 0)) != hipSuccess)
+    return int(he);
+  if ((rc = bwt_chunk_replay_launch(&e.x, b, &l, side))) return rc;
+  if ((he = hipEventRecord(join, side)) != hipSuccess) return int(he);
+  if ((rc = bwt_block_replay_launch(&e.x, b, &l, st))) return rc;
+  if ((he = hipStreamWaitEvent(st, join, 0)) != hipSuccess) return int(he);
+  return bwt_combine_launch(&e.x, b, &l, st);
+}
+
 bool alloc_plan_slot(const TreeEngine& e, int64_t cap, DevicePlanSlot& s, char* err, size_t errlen) {
   s.arena.release();
   s.arena.err = hipSuccess;

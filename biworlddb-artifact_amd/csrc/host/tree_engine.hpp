@@ -69,6 +69,12 @@ struct DeviceBatch {
 bool upload_batch(const TreeEngine& e, BatchPlan&& plan, const uint32_t* d_qlo, const uint32_t* d_qhi,
                   DeviceBatch& out, char* err, size_t errlen);
 
+// The query sequence of bwt_query_launch with its two independent replays forked onto two
+// streams (block replay on st, level-B chunk replay on side; combine on st after both), so their
+// latency-bound phases overlap.  Capturable: fork and join are event record / wait.
+int query_forked(const TreeEngine& e, const BwtBatch* b, hipStream_t st, hipStream_t side, hipEvent_t fork,
+                 hipEvent_t join);
+
 // Device arrays of a batch planned on the device, sized for up to cap queries: the plan's
 // outputs, the staging/result buffers, the planner's workspace and counts.
 struct DevicePlanSlot {
