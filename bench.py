@@ -14,8 +14,9 @@ time is reported beside the replay) and the replay runs over that device-planned
 
 Beside it, measured in the same run:
   plain   -- the plain GPU drop-in at paper scale (config 2: 300,000 states, 9.6M queries, one
-             launch sequence that rebuilds the index and answers the batch) and, on the batched
-             stream's first batches, the same plain pass run once per batch;
+             launch sequence that rebuilds the index and answers the batch, with its own tuned
+             knobs) and, on the batched stream's first batches, the same plain pass run once per
+             batch;
   served  -- the served form (bw_serve): 20,000 Poisson requests (log-uniform sizes in [1, 4096],
              half point / half range queries, 5% appended branch states) by continuous batching
              with every batch planned on the GPU, p50/p99 request latency; and the same stream
@@ -76,7 +77,7 @@ def main():
     knobs = ["--tuned", TUNED] if os.path.exists(TUNED) else []
     r = run([BW_BATCH, "--warmup", str(a.warmup), "--repeat", str(a.steps), "--plain-sample", "4", "--stages",
              "--device-plan"] + knobs, 1200)
-    plain = run([BW_TREE, "--cfg", "2", "--mode", "pass", "--warmup", "2", "--repeat", "5"], 600)
+    plain = run([BW_TREE, "--cfg", "2", "--mode", "pass", "--warmup", "2", "--repeat", "5"] + knobs, 600)
     stages = {k: v for k, v in r["stage_ms"].items() if v > 0}
     dom = max(stages, key=stages.get)
     achieved = r["stage_bytes"][dom] / (stages[dom] * 1e-3) / 1e9

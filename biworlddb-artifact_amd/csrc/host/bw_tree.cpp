@@ -25,6 +25,7 @@
 #include "bwgpu_tree.h"
 #include "tree_engine.hpp"
 #include "tree_plan.hpp"
+#include "tree_tune.hpp"
 #include "tree_workload.hpp"
 
 using namespace bwg;
@@ -78,6 +79,19 @@ int main(int argc, char** argv) {
   std::string answers_path, json_path, mode = "pass";
   bool wsha = false, verify_remap = false;
   int64_t copy_probe = 0;  // bytes of a device-to-device copy run first (HBM counter scale)
+  // --tuned FILE: the "plain" form's knobs from the autotuning file, as defaults the flags override
+  std::string tuned_from = "defaults";
+  for (int i = 1; i + 1 < argc; ++i)
+    if (!std::strcmp(argv[i], "--tuned")) {
+      LaunchKnobs tl;
+      tl.block_threads = threads;
+      tl.chunk_waves = waves;
+      if (load_tuned(argv[i + 1], "plain", kn, tl)) {
+        threads = tl.block_threads;
+        waves = tl.chunk_waves;
+        tuned_from = argv[i + 1];
+      }
+    }
   for (int i = 1; i < argc; ++i) {
     if (!std::strcmp(argv[i], "--workload-sha")) { wsha = true; continue; }
     if (!std::strcmp(argv[i], "--verify-remap")) { verify_remap = true; continue; }
@@ -106,6 +120,7 @@ int main(int argc, char** argv) {
     else if (!std::strcmp(k, "--answers")) answers_path = v;
     else if (!std::strcmp(k, "--json")) json_path = v;
     else if (!std::strcmp(k, "--copy-probe")) copy_probe = std::atoll(v);
+    else if (!std::strcmp(k, "--tuned")) continue;
     else { std::fprintf(stderr, "bw_tree: unknown argument %s\n", k); return 2; }
   }
   if (spec.S < 1 || spec.U < 1 || spec.Q < 1 || spec.N < 1 || repeat < 1 || warmup < 0 ||
@@ -326,7 +341,7 @@ int main(int argc, char** argv) {
       ",\"NK\":%" PRIu64 ",\"seed\":%" PRIu64 ",\"n_answers\":%zu,\"answers_sha256\":\"%s\",\"answers_fn\":\"%016" PRIx64
       "\",\"plan\":{\"nsteps\":%zu,\"nblk\":%d,\"nkeys\":%zu,\"ntiles\":%zu,\"nkitems\":%zu,\"neitems\":%zu,"
       "\"nchunk\":%" PRId64 ",\"nwork\":%zu,\"nchain\":%d,\"nseg\":%" PRId64 ",\"max_cls\":%d,\"rmax\":%d,"
-      "\"tile_shift\":%d,\"chunk_items\":%" PRId64 ",\"ngrp\":%zu,\"ep_per_work\":%" PRId64 ",\"seg_chunks\":%d,\"threads\":%d,\"waves\":%d},"
+      "\"tile_shift\":%d,\"chunk_items\":%" PRId64 ",\"ngrp\":%zu,\"ep_per_work\":%" PRId64 ",\"seg_chunks\":%d,\"threads\":%d,\"waves\":%d,\"knobs_from\":\"%s\"},"
       "\"phases_ms\":{\"generate\":%.3f,\"setup\":%.3f,\"upload\":%.3f,\"download\":%.3f,\"extract\":%.3f,"
       "\"index_build\":%.4f},"
       "\"pass_ms\":{\"avg\":%.4f,\"min\":%.4f,\"max\":%.4f,\"repeat\":%d,\"warmup\":%d},"
@@ -338,7 +353,7 @@ int main(int argc, char** argv) {
       mode.c_str(), cfg, S, spec.U, spec.Q, spec.N, w.coord_n(), spec.seed, nq, sha.c_str(), fn, P.steps.size(),
       P.nblk(), P.bkey.size(), P.tile_rank0.size(), P.kitems.size(), BP.eitems.size(), P.nchunk(), BP.wk_chunk.size(),
       P.nchain(), P.nseg(), P.max_cls, kn.rmax, P.knobs.tile_shift, P.knobs.chunk_items, BP.grp_step.size(), kn.ep_per_work,
-      kn.seg_chunks, threads, waves,
+      kn.seg_chunks, threads, waves, tuned_from.c_str(),
       ms_between(t0, t1), ms_between(t1, t2), ms_between(tu0, tu1), ms_between(tx0, tx1), ms_between(tx1, tx2),
       index_ms, avg, pass_ms.front(), pass_ms.back(), repeat, warmup, stage_ms[0], stage_ms[1], stage_ms[2],
       stage_ms[3], stage_ms[4], stage_ms[5], sb_remap, sb_block, sb_hist, sb_scan, sb_replay, sb_combine,

@@ -614,11 +614,14 @@ int block_rpt(const BwtIndex* x, int nt) {
   return r;
 }
 
-size_t block_lds_bytes(int rpt, int nt) {
+// with_tiles: the query tiles are staged too (the replay adds the level-C carry)
+size_t block_lds_bytes(int rpt, int nt, bool with_tiles) {
   const size_t span = (size_t)(rpt > 1 ? nt * rpt + nt : nt * rpt) + 1;
   return (2 * span + 16 + kGCap + 1) * sizeof(int64_t) + kGCap * sizeof(int32_t) + kQCap * sizeof(uint32_t) +
-         (size_t)kEPT * nt * (sizeof(int64_t) + sizeof(uint16_t)) + kQCap * sizeof(uint32_t);
+         (size_t)kEPT * nt * (sizeof(int64_t) + sizeof(uint16_t)) + (with_tiles ? kQCap * sizeof(uint32_t) : 0);
 }
+
+bool block_adds_carry(const BwtLaunch* l) { return l->carry_in_block && !l->write_index; }
 
 
 // LDS stride (entries per wavefront) of a level: its classes + 1, rounded to an odd number of
@@ -642,7 +645,7 @@ int check(const BwtIndex* x, const BwtLaunch* l) {
   if (x->tile_shift < 1 || x->tile_shift > 12) return (int)hipErrorInvalidValue;
   if (l->chunk_waves < 1 || l->chunk_waves > 4) return (int)hipErrorInvalidValue;
   const int rpt = block_rpt(x, l->block_threads);
-  if (rpt > 16 || block_lds_bytes(rpt, l->block_threads) > 160 * 1024) return (int)hipErrorInvalidValue;
+  if (rpt > 16 || block_lds_bytes(rpt, l->block_threads, block_adds_carry(l)) > 160 * 1024) return (int)hipErrorInvalidValue;
   if (replay_lds_bytes(x, l) > 160 * 1024 || hist_lds_bytes(x, l) > 160 * 1024) return (int)hipErrorInvalidValue;
   return 0;
 }
@@ -676,7 +679,7 @@ template <int RPT>
 void launch_block(const BwtIndex* x, const BwtBatch* b, const BwtLaunch* l, hipStream_t st) {
   const int64_t grid = b->dcounts ? std::min<int64_t>(b->nqblk, kDeviceCountGrid) : b->nqblk;
   hipLaunchKernelGGL(bwt_block_kernel<RPT>, dim3((unsigned)grid), dim3(l->block_threads),
-                     block_lds_bytes(RPT, l->block_threads), st, *x, *b, *l);
+                     block_lds_bytes(RPT, l->block_threads, block_adds_carry(l)), st, *x, *b, *l);
 }
 
 }  // namespace
