@@ -318,12 +318,12 @@ int main(int argc, char** argv) {
   }
   // algorithmic bytes per kernel (what each must read and write once):
   //  block replay: per replayed event its u16 rank + i64 delta, per step its id, per query its
-  //    packed ranks (u32) + in-block result (i64)
+  //    packed ranks and key tiles (2 x u32) + result (i64), per listed block its level-C row
   //  chunk replay: per replayed key item its class word (u32) + weight (i64), per work item its
   //    chunk's carry row, per endpoint item (u64) its read and its result (i64)
-  //  combine: per slot its block and tiles (2 x u32), 2 carry-row reads, the in-block part, the
-  //    2 level-B places (u32) and results, its staging index (u16) + position (u32) and staged
-  //    value; per query id its staging position (u32), the staged value and the answer
+  //  combine (query side: the block replay added the level-C carry): per slot its partial
+  //    result, the 2 level-B places (u32) and results, its staging index (u16) + position (u32)
+  //    and staged value; per query id its staging position (u32), the staged value, the answer
   int64_t kit_run = 0;
   {
     const BatchPlan& P2 = DB.plan;
@@ -333,11 +333,11 @@ int main(int argc, char** argv) {
                                    int64_t(BWT_EP_POS(P2.eitems[size_t(P2.wk_e0[i + 1] - 1)])));
     }
   }
-  const double sb_block =
-      double(DB.plan.steps_run) * spec.U * 10.0 + double(DB.plan.steps_run) * 4.0 + double(M) * 12.0;
+  const double sb_block = double(DB.plan.steps_run) * spec.U * 10.0 + double(DB.plan.steps_run) * 4.0 +
+                          double(M) * 16.0 + double(DB.plan.qblk.size()) * double(E.x.ncls_c) * 8.0;
   const double sb_chunk = double(kit_run) * 12.0 + double(DB.plan.wk_chunk.size()) * double(E.x.ncls_b) * 8.0 +
                           double(DB.plan.eitems.size()) * 16.0;
-  const double sb_combine = double(M) * (8.0 + 16.0 + 8.0 + 8.0 + 16.0 + 2.0 + 4.0 + 8.0 + 4.0 + 8.0 + 8.0);
+  const double sb_combine = double(M) * (8.0 + 8.0 + 16.0 + 2.0 + 4.0 + 8.0 + 4.0 + 8.0 + 8.0);
   const auto tx0 = clk::now();
   std::vector<int64_t> ans(static_cast<size_t>(M));
   HIPX(hipMemcpy(ans.data(), QB->answers, size_t(M) * sizeof(int64_t), hipMemcpyDeviceToHost));

@@ -326,14 +326,17 @@ int main(int argc, char** argv) {
   const double nkeys_d = double(P.bkey.size()), ntile_d = double(P.tile_rank0.size());
   const int x_ncls_b = P.chain_ncls.size() > 1 ? P.chain_ncls[1] : 1;
   const double sb_remap = cfg == 3 ? double(nq) * 16.0 : 0.0;
+  // query mode: the block replay also adds the level-C carry (query tiles + the block's row)
+  const bool carry_in_block = mode != "pass";
   const double sb_block = double(steps_run) * spec.U * 10.0 + double(steps_run) * 4.0 + double(nq) * 12.0 +
-                          (mode == "pass" ? nkeys_d * 12.0 + ntile_d * 10.0 : 0.0);
+                          (mode == "pass" ? nkeys_d * 12.0 + ntile_d * 10.0 : 0.0) +
+                          (carry_in_block ? double(nq) * 4.0 + double(BP.qblk.size()) * double(P.KT) * 8.0 : 0.0);
   const double nkit_b = double(P.kitems.size()) - ntile_d;
   const double sb_hist = mode == "pass" ? double(P.kitems.size()) * 16.0 + nkit_b * 8.0 + double(P.n_cp) * 8.0 : 0.0;
   const double sb_scan = mode == "pass" ? double(P.n_cp) * 16.0 : 0.0;
   const double sb_replay = double(kit_run) * 12.0 + double(BP.wk_chunk.size()) * double(x_ncls_b) * 8.0 +
                            double(BP.eitems.size()) * 16.0;
-  const double sb_combine = double(nq) * 90.0;
+  const double sb_combine = double(nq) * (carry_in_block ? 66.0 : 90.0);
   char buf[8192];
   int n = std::snprintf(
       buf, sizeof(buf),
