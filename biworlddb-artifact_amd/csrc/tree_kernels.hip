@@ -157,21 +157,32 @@ __device__ __forceinline__ void block_replay_one(const BwtIndex& x, const BwtBat
     }
     rebuild();  // its barriers also publish qbuf
     // one endpoint per thread: even = lo, odd = hi of the same slot (nt is a multiple of 64,
-    // so a pair never straddles a wavefront)
+    // so a pair never straddles a wavefront); kUnroll rounds at a time, so the carry-row loads
+    // of all of them are in flight together
+    constexpr int kUnroll = 4;
     const int64_t jn = (2 * nq + 63) & ~(int64_t)63;
-    for (int64_t j = tid; j < jn; j += nt) {
-      const int64_t sl = j >> 1;
-      const bool live = j < 2 * nq;
-      uint32_t rk = 0;
-      if (live) rk = staged ? qbuf[qa - qwin + sl] : q.qrank[qa + sl];
-      const int y = (j & 1) ? (int)(rk >> 16) : (int)(rk & 0xFFFFu);
-      int64_t pv = live ? P[pslot<RPT>(y)] : 0;
-      if (add_carry && live) {  // plus the carry of the earlier blocks' keys in lower tiles
-        const uint32_t tl = staged ? qtbuf[qa - qwin + sl] : tiles_of(qa + sl);
-        pv += crow[(j & 1) ? (tl >> 16) : (tl & 0xFFFFu)];
+    for (int64_t j0 = tid; j0 < jn; j0 += (int64_t)kUnroll * nt) {
+      int64_t pv[kUnroll];
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        const int64_t j = j0 + (int64_t)u * nt;
+        const int64_t sl = j >> 1;
+        const bool live = j < 2 * nq;
+        uint32_t rk = 0;
+        if (live) rk = staged ? qbuf[qa - qwin + sl] : q.qrank[qa + sl];
+        const int y = (j & 1) ? (int)(rk >> 16) : (int)(rk & 0xFFFFu);
+        pv[u] = live ? P[pslot<RPT>(y)] : 0;
+        if (add_carry && live) {  // plus the carry of the earlier blocks' keys in lower tiles
+          const uint32_t tl = staged ? qtbuf[qa - qwin + sl] : tiles_of(qa + sl);
+          pv[u] += crow[(j & 1) ? (tl >> 16) : (tl & 0xFFFFu)];
+        }
       }
-      const int64_t other = __shfl_xor(pv, 1);
-      if (live && !(j & 1)) q.ansr[qa + sl] = other - pv;
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        const int64_t j = j0 + (int64_t)u * nt;
+        const int64_t other = __shfl_xor(pv[u], 1);
+        if (j < 2 * nq && !(j & 1)) q.ansr[qa + (j >> 1)] = other - pv[u];
+      }
     }
     ++gi;
     if (gi < ge && gi - gwin >= kGCap) {  // uniform
