@@ -9,16 +9,20 @@
 //                              query step a block scan folds the deltas into per-thread rank
 //                              totals (registers) and publishes their exclusive prefix P, which
 //                              the step's endpoints read once each.  Optionally the block's net
-//                              weights (the dense dV, compacted)
+//                              weights (the dense dV, compacted); on the query side it also adds
+//                              the level-C carry (the block's dense row at each endpoint's tile)
 //   bwt_chunk_hist_kernel   <- dV per carry chunk (level C: one chunk per block)
 //   bwt_seg_*_kernel        <- the serial carry pass (src/batch_engine_v7.hpp:936-945) as a
 //                              three-pass segmented column scan over chunk rows
 //   bwt_chunk_replay_kernel <- lvals on carry level B (src/gpu_v7.cu:72-78): one wavefront per
 //                              work item; the chunk's key items and the batch's endpoint items
 //                              are consumed in merged groups of 64; a Fenwick tree seeded from
-//                              the chunk's carry row; pairs inside a group by LDS broadcast
-//   bwt_combine_*_kernel    <- gpu_combine_kernel (src/gpu_v7.cu:84-96), plus the level-C carry
-//                              read straight from the block's dense carry row
+//                              the chunk's carry row; pairs inside a group by LDS broadcast;
+//                              results stored in work order
+//   bwt_combine_*_kernel    <- gpu_combine_kernel (src/gpu_v7.cu:84-96): a slot pass gathers each
+//                              slot's level-B results (die-local) and adds them to its partial
+//                              result (plus the level-C carry when the block replay did not add
+//                              it), a staged id pass stores the answers in query order
 //   bwt_rank_kernel         <- CsrBase::rank (src/sparse_engine.hpp:62-97), CSR form
 // All arithmetic is int64 adds (LDS ds_add_u64, DPP scans): results are exact and independent
 // of scheduling order.
