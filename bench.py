@@ -9,8 +9,10 @@ combine) over the index resident in HBM.  A "step" is one such replay over input
 bw_batch times each with HIP events on its stream.  The answers' SHA-256 is reported (the test
 suite pins it against the oracle).
 
-The batch is planned on the GPU (bwt_plan_launch: the host planner's plan, array for array; its
-time is reported beside the replay) and the replay runs over that device-planned batch.
+The stream's queries are materialised on the GPU (bwt_gen_*: the host generator's arrays, element
+for element; its time is reported beside the host generator's), the batch is planned on the GPU
+(bwt_plan_launch: the host planner's plan, array for array; its time is reported beside the
+replay), and the replay runs over that device-generated, device-planned batch.
 
 Beside it, measured in the same run:
   plain   -- the plain GPU drop-in at paper scale (config 2: 300,000 states, 9.6M queries, one
@@ -76,7 +78,7 @@ def main():
             sys.exit("bench.py: native build missing (python -c 'import __graft_entry__ as g; g.build()')")
     knobs = ["--tuned", TUNED] if os.path.exists(TUNED) else []
     r = run([BW_BATCH, "--warmup", str(a.warmup), "--repeat", str(a.steps), "--plain-sample", "4", "--stages",
-             "--device-plan"] + knobs, 1200)
+             "--device-plan", "--device-gen"] + knobs, 1200)
     plain = run([BW_TREE, "--cfg", "2", "--mode", "pass", "--warmup", "2", "--repeat", "5"] + knobs, 600)
     stages = {k: v for k, v in r["stage_ms"].items() if v > 0}
     dom = max(stages, key=stages.get)
@@ -122,6 +124,9 @@ def main():
                         "plan_and_query_queries_per_s": round(r["n_answers"] / ((r["device_plan"]["ms"] +
                                                                                   r["graph_ms"]["avg"]) * 1e-3), 1)},
         "host_batch_plan_ms": r["phases_ms"]["batch_plan"],
+        "device_gen": {"gpu_ms": r["device_gen"]["gpu_ms"], "wall_ms": r["device_gen"]["wall_ms"],
+                       "host_generator_ms": r["device_gen"]["host_ms"],
+                       "equals_host_generator": r["device_gen"]["mismatch"] == 0, "ties": r["device_gen"]["ties"]},
         "device": r["device"],
     }
     if not a.no_served:

@@ -5,7 +5,7 @@
 //   bw_batch [--S n --U n --N n --seed n] [--requests 100000 --max-batch 2048]
 //            [--rmax --tile-shift --chunk --ep-per-work --range-slots --seg --threads --waves] [--tuned FILE]
 //            [--warmup 1 --repeat 3] [--plain-sample 8] [--stages] [--answers FILE] [--json FILE]
-//            [--stream-sha]
+//            [--stream-sha] [--device-plan] [--device-gen]
 //
 // The index (block weights, carry rows) is built once on the device.  Every batch of the stream
 // is then answered by one launch sequence (block replay, chunk replay, combine) captured in a
@@ -15,6 +15,10 @@
 // (index rebuild + answers, bwt_pass_launch) per batch, and one query-only launch per batch.
 // --stages also times the launch sequence's kernels one by one (HIP events on the stream they run
 // on, outside the graph) and reports each one's algorithmic bytes.
+// --device-plan also plans the stream on the device (include/bwgpu_plan.h), compares that plan
+// array for array with the host plan, and replays the device plan.  --device-gen materialises the
+// stream's queries with the device generator (include/bwgpu_gen.h), compares them element for
+// element with the host generator's, and feeds the device arrays to the planner and the replays.
 // Exit codes: 2 usage, 3 unsupported input, 4 I/O, 5 GPU error.
 #include <hip/hip_runtime_api.h>
 #include <openssl/sha.h>
@@ -30,6 +34,7 @@
 
 #include "bwgpu_tree.h"
 #include "tree_engine.hpp"
+#include "tree_gen.hpp"
 #include "tree_plan.hpp"
 #include "tree_tune.hpp"
 #include "tree_workload.hpp"
@@ -78,7 +83,7 @@ int main(int argc, char** argv) {
   PlanKnobs kn;
   LaunchKnobs lk;
   int warmup = 1, repeat = 3, plain_sample = 8;
-  bool want_sha = false, stages = false, dev_plan = false;
+  bool want_sha = false, stages = false, dev_plan = false, dev_gen = false;
   int fork = 1;  // replay the block and level-B stages on two streams
   std::string answers_path, json_path, tuned_path;
   bool have_knob[8] = {false, false, false, false, false, false, false, false};
@@ -86,6 +91,7 @@ int main(int argc, char** argv) {
     if (!std::strcmp(argv[i], "--stream-sha")) { want_sha = true; continue; }
     if (!std::strcmp(argv[i], "--stages")) { stages = true; continue; }
     if (!std::strcmp(argv[i], "--device-plan")) { dev_plan = true; continue; }
+    if (!std::strcmp(argv[i], "--device-gen")) { dev_gen = true; continue; }
     if (i + 1 >= argc) { std::fprintf(stderr, "bw_batch: missing value for %s\n", argv[i]); return 2; }
     const char* k = argv[i];
     const char* v = argv[++i];
@@ -144,12 +150,39 @@ int main(int argc, char** argv) {
   const auto t0 = clk::now();
   TreeWorkload w;
   tree_generate(spec, w);
+  const auto ts0 = clk::now();
   RequestStream rs;
   stream_generate(w, ss, rs);
+  const double stream_host_ms = ms_between(ts0, clk::now());
   if (want_sha) {
+    // with --device-gen the per-query and per-append arrays hashed are the device generator's
+    uint32_t ties = 0;
+    if (dev_gen) {
+      char gerr[256];
+      HIPX(hipSetDevice(0));
+      RequestStream scan;
+      stream_scan(w.spec, ss, scan);
+      DeviceStream G;
+      if (!device_generate_stream(w.spec, scan, G, nullptr, gerr, sizeof(gerr))) {
+        std::fprintf(stderr, "bw_batch: device generation: %s\n", gerr);
+        return 5;
+      }
+      auto down = [](auto& v, const auto* d) {
+        if (!v.empty()) HIPX(hipMemcpy(v.data(), d, v.size() * sizeof(v[0]), hipMemcpyDeviceToHost));
+        return 0;
+      };
+      for (auto* v : {&rs.qlo, &rs.qhi, &rs.app_key}) std::fill(v->begin(), v->end(), 0u);
+      if (down(rs.qstate, G.qstate) || down(rs.qlo, G.qlo) || down(rs.qhi, G.qhi) ||
+          down(rs.app_parent, G.app_parent) || down(rs.app_key, G.app_key) || down(rs.app_delta, G.app_delta))
+        return 5;
+      rs.req_q0 = scan.req_q0;
+      rs.req_kind = scan.req_kind;
+      ties = G.ties;
+    }
     char hex[65];
     stream_sha(rs, hex);
-    std::printf("{\"stream_sha256\":\"%s\",\"nq\":%" PRId64 ",\"napp\":%" PRId64 "}\n", hex, rs.nq(), rs.napp());
+    std::printf("{\"stream_sha256\":\"%s\",\"nq\":%" PRId64 ",\"napp\":%" PRId64 ",\"device_gen\":%s,\"ties\":%u}\n",
+                hex, rs.nq(), rs.napp(), dev_gen ? "true" : "false", ties);
     return 0;
   }
   if (form != 4) { std::fprintf(stderr, "bw_batch: the served form runs in bw_serve\n"); return 2; }
@@ -166,6 +199,26 @@ int main(int argc, char** argv) {
     std::fprintf(stderr, "bw_batch: index: %s\n", err);
     return std::strncmp(err, "device", 6) == 0 ? 5 : 3;
   }
+  // the device generator: the stream's queries materialised on the GPU (timed on the second run:
+  // the first loads code), compared element for element with the host generator's
+  DeviceStream DS;
+  double dgen_gpu_ms = -1, dgen_wall_ms = -1;
+  int64_t dgen_mismatch = -1;
+  if (dev_gen) {
+    for (int rep = 0; rep < 2; ++rep) {
+      const auto g0 = clk::now();
+      RequestStream scan;
+      stream_scan(w.spec, ss, scan);
+      if (!device_generate_stream(w.spec, scan, DS, nullptr, err, sizeof(err))) {
+        std::fprintf(stderr, "bw_batch: device generation: %s\n", err);
+        return 5;
+      }
+      dgen_wall_ms = ms_between(g0, clk::now());
+    }
+    dgen_gpu_ms = DS.gpu_ms;
+    dgen_mismatch = count_device_mismatch(DS.qstate, rs.qstate) + count_device_mismatch(DS.qlo, rs.qlo) +
+                    count_device_mismatch(DS.qhi, rs.qhi);
+  }
   const auto t2 = clk::now();
   BatchPlan plan;
   if (!build_batch_plan(E.plan, M, rs.qstate.data(), rs.qlo.data(), rs.qhi.data(), false, plan, err, sizeof(err))) {
@@ -174,8 +227,8 @@ int main(int argc, char** argv) {
   }
   const auto t3 = clk::now();
   DeviceArena io;
-  uint32_t* d_qlo = io.upload(rs.qlo);
-  uint32_t* d_qhi = io.upload(rs.qhi);
+  uint32_t* d_qlo = dev_gen ? DS.qlo : io.upload(rs.qlo);
+  uint32_t* d_qhi = dev_gen ? DS.qhi : io.upload(rs.qhi);
   DeviceBatch DB;
   if (!io.ok() || !upload_batch(E, std::move(plan), d_qlo, d_qhi, DB, err, sizeof(err))) {
     std::fprintf(stderr, "bw_batch: upload failed\n");
@@ -205,7 +258,7 @@ int main(int argc, char** argv) {
   int64_t plan_mismatch = -1;
   if (dev_plan) {
     DeviceArena qa;
-    const int32_t* d_qstate = qa.upload(rs.qstate);
+    const int32_t* d_qstate = dev_gen ? DS.qstate : qa.upload(rs.qstate);
     if (!qa.ok() || !alloc_plan_slot(E, M, dslot, err, sizeof(err))) {
       std::fprintf(stderr, "bw_batch: device plan: %s\n", qa.ok() ? err : "upload");
       return 5;
@@ -410,7 +463,9 @@ int main(int argc, char** argv) {
       "\"stage_bytes\":{\"block_replay\":%.0f,\"chunk_replay\":%.0f,\"combine\":%.0f},"
       "\"plain\":{\"batches\":%" PRId64 ",\"queries\":%" PRId64 ",\"pass_ms\":%.4f,\"query_ms\":%.4f,"
       "\"pass_queries_per_s\":%.1f,\"query_queries_per_s\":%.1f,\"mismatch\":%" PRId64 "},"
-      "\"device_plan\":{\"used\":%s,\"ms\":%.4f,\"mismatch\":%" PRId64 "},\"forked\":%d,"
+      "\"device_plan\":{\"used\":%s,\"ms\":%.4f,\"mismatch\":%" PRId64 "},"
+      "\"device_gen\":{\"used\":%s,\"gpu_ms\":%.4f,\"wall_ms\":%.3f,\"host_ms\":%.3f,\"ties\":%u,"
+      "\"mismatch\":%" PRId64 "},\"forked\":%d,"
       "\"device_bytes\":%zu,\"device\":\"%s\",\"cus\":%d}\n",
       spec.S, spec.U, spec.N, spec.seed, ss.nreq, ss.max_batch, M, sha.c_str(), E.plan.nblk(), BP.qblk.size(),
       BP.grp_step.size(), BP.eitems.size(), BP.wk_chunk.size(), E.plan.knobs.rmax, E.plan.knobs.tile_shift,
@@ -421,7 +476,8 @@ int main(int argc, char** argv) {
       sb_block, sb_chunk, sb_combine, K, plain_q, plain_pass_ms,
       plain_query_ms, plain_pass_ms > 0 ? double(plain_q) / (plain_pass_ms * 1e-3) : 0.0,
       plain_query_ms > 0 ? double(plain_q) / (plain_query_ms * 1e-3) : 0.0, plain_mismatch,
-      dev_plan ? "true" : "false", dplan_ms, plan_mismatch, fork, E.arena.bytes + io.bytes + DB.arena.bytes + dslot.arena.bytes,
+      dev_plan ? "true" : "false", dplan_ms, plan_mismatch, dev_gen ? "true" : "false", dgen_gpu_ms, dgen_wall_ms,
+      stream_host_ms, DS.ties, dgen_mismatch, fork, E.arena.bytes + io.bytes + DB.arena.bytes + dslot.arena.bytes,
       dev, prop.multiProcessorCount);
   std::fwrite(buf, 1, size_t(n), stdout);
   if (!json_path.empty()) {
@@ -430,5 +486,5 @@ int main(int argc, char** argv) {
     std::fwrite(buf, 1, size_t(n), f);
     std::fclose(f);
   }
-  return plain_mismatch == 0 && plan_mismatch <= 0 ? 0 : 6;
+  return plain_mismatch == 0 && plan_mismatch <= 0 && dgen_mismatch <= 0 ? 0 : 6;
 }

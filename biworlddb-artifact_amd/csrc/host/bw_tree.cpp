@@ -4,11 +4,13 @@
 //
 //   bw_tree --cfg 2 [--S n --U n --Q n --N n --seed n] [--mode pass|query]
 //           [--rmax 2048 --tile-shift 10 --chunk 4096 --seg 32 --threads 256 --waves 4]
-//           [--warmup 1 --repeat 5] [--answers FILE] [--json FILE]
+//           [--warmup 1 --repeat 5] [--answers FILE] [--json FILE] [--device-gen]
 // --mode pass  : the plain form -- every timed pass rebuilds the index (block replay with weights,
 //                chunk histograms, carry scan) and answers the declared queries.
 // --mode query : the index is built once (timed separately), every timed pass answers the
 //                declared queries over the resident index.
+// --device-gen : also materialise the workload's update and query arrays with the device generator
+//                (include/bwgpu_gen.h) and compare them element for element with the host's.
 // Exit codes follow the reference: 2 usage, 3 unsupported input, 4 I/O, 5 GPU error.
 #include <hip/hip_runtime_api.h>
 #include <openssl/sha.h>
@@ -24,6 +26,7 @@
 
 #include "bwgpu_tree.h"
 #include "tree_engine.hpp"
+#include "tree_gen.hpp"
 #include "tree_plan.hpp"
 #include "tree_tune.hpp"
 #include "tree_workload.hpp"
@@ -77,7 +80,7 @@ int main(int argc, char** argv) {
   PlanKnobs kn;
   int threads = 256, waves = 4, warmup = 1, repeat = 5;
   std::string answers_path, json_path, mode = "pass";
-  bool wsha = false, verify_remap = false;
+  bool wsha = false, verify_remap = false, device_gen = false;
   int64_t copy_probe = 0;  // bytes of a device-to-device copy run first (HBM counter scale)
   // --tuned FILE: the "plain" form's knobs from the autotuning file, as defaults the flags override
   std::string tuned_from = "defaults";
@@ -95,6 +98,7 @@ int main(int argc, char** argv) {
   for (int i = 1; i < argc; ++i) {
     if (!std::strcmp(argv[i], "--workload-sha")) { wsha = true; continue; }
     if (!std::strcmp(argv[i], "--verify-remap")) { verify_remap = true; continue; }
+    if (!std::strcmp(argv[i], "--device-gen")) { device_gen = true; continue; }
     if (i + 1 >= argc) { std::fprintf(stderr, "bw_tree: missing value for %s\n", argv[i]); return 2; }
     const char* k = argv[i];
     const char* v = argv[++i];
@@ -134,9 +138,30 @@ int main(int argc, char** argv) {
   TreeWorkload w;
   tree_generate(spec, w);
   if (wsha) {
+    // with --device-gen the update and query arrays hashed are the device generator's
+    uint32_t ties = 0;
+    if (device_gen) {
+      char gerr[256];
+      HIPX(hipSetDevice(0));
+      DeviceTree G;
+      if (!device_generate_tree(spec, w.dkey, G, nullptr, gerr, sizeof(gerr))) {
+        std::fprintf(stderr, "bw_tree: device generation: %s\n", gerr);
+        return 5;
+      }
+      std::fill(w.key.begin(), w.key.end(), 0u);
+      std::fill(w.qlo.begin(), w.qlo.end(), 0u);
+      std::fill(w.qhi.begin(), w.qhi.end(), 0u);
+      std::fill(w.delta.begin(), w.delta.end(), int64_t(0));
+      HIPX(hipMemcpy(w.key.data(), G.key, w.key.size() * 4, hipMemcpyDeviceToHost));
+      HIPX(hipMemcpy(w.delta.data(), G.delta, w.delta.size() * 8, hipMemcpyDeviceToHost));
+      HIPX(hipMemcpy(w.qlo.data(), G.qlo, w.qlo.size() * 4, hipMemcpyDeviceToHost));
+      HIPX(hipMemcpy(w.qhi.data(), G.qhi, w.qhi.size() * 4, hipMemcpyDeviceToHost));
+      ties = G.ties;
+    }
     char hex[65];
     tree_workload_sha(w, hex);
-    std::printf("{\"workload_sha256\":\"%s\"}\n", hex);
+    std::printf("{\"workload_sha256\":\"%s\",\"device_gen\":%s,\"ties\":%u}\n", hex,
+                device_gen ? "true" : "false", ties);
     return 0;
   }
   const auto t1 = clk::now();
@@ -281,6 +306,32 @@ int main(int argc, char** argv) {
     remap_mismatch = 0;
     for (size_t i = 0; i < nq; ++i) remap_mismatch += (dl[i] != lc[i]) + (dh[i] != hc[i]);
   }
+  // ---- the device generator: the same update and query arrays materialised on the GPU, compared
+  // element for element with the host generator's (timed on its second run: the first loads code)
+  double dgen_gpu_ms = -1, dgen_wall_ms = -1;
+  int64_t dgen_mismatch = -1;
+  uint32_t dgen_ties = 0;
+  if (device_gen) {
+    DeviceTree DT;
+    for (int rep = 0; rep < 2; ++rep) {
+      const auto g0 = clk::now();
+      if (!device_generate_tree(spec, w.dkey, DT, nullptr, err, sizeof(err))) {
+        std::fprintf(stderr, "bw_tree: device generation: %s\n", err);
+        return 5;
+      }
+      dgen_wall_ms = ms_between(g0, clk::now());
+    }
+    dgen_gpu_ms = DT.gpu_ms;
+    dgen_ties = DT.ties;
+    dgen_mismatch = count_device_mismatch(DT.key, w.key) + count_device_mismatch(DT.delta, w.delta) + count_device_mismatch(DT.qlo, w.qlo) +
+                    count_device_mismatch(DT.qhi, w.qhi);
+  }
+  char dgen_json[256] = "null";
+  if (device_gen)
+    std::snprintf(dgen_json, sizeof(dgen_json),
+                  "{\"gpu_ms\":%.4f,\"wall_ms\":%.3f,\"host_ms\":%.3f,\"ties\":%u,\"mismatch\":%" PRId64 "}",
+                  dgen_gpu_ms, dgen_wall_ms, ms_between(t0, t1), dgen_ties, dgen_mismatch);
+
   const auto tx0 = clk::now();
   std::vector<int64_t> ans(nq);
   HIPX(hipMemcpy(ans.data(), Bt.answers, nq * sizeof(int64_t), hipMemcpyDeviceToHost));
@@ -352,7 +403,7 @@ int main(int argc, char** argv) {
       "\"chunk_replay\":%.4f,\"combine\":%.4f},"
       "\"stage_bytes\":{\"rank_remap\":%.0f,\"block_replay\":%.0f,\"chunk_hist\":%.0f,\"carry_scan\":%.0f,"
       "\"chunk_replay\":%.0f,\"combine\":%.0f},"
-      "\"remap_mismatch\":%" PRId64 ",\"queries_per_s\":%.1f,\"bytes_alg\":%.0f,\"device_bytes\":%zu,\"device\":\"%s\",\"cus\":%d}\n",
+      "\"remap_mismatch\":%" PRId64 ",\"device_gen\":%s,\"queries_per_s\":%.1f,\"bytes_alg\":%.0f,\"device_bytes\":%zu,\"device\":\"%s\",\"cus\":%d}\n",
       mode.c_str(), cfg, S, spec.U, spec.Q, spec.N, w.coord_n(), spec.seed, nq, sha.c_str(), fn, P.steps.size(),
       P.nblk(), P.bkey.size(), P.tile_rank0.size(), P.kitems.size(), BP.eitems.size(), P.nchunk(), BP.wk_chunk.size(),
       P.nchain(), P.nseg(), P.max_cls, kn.rmax, P.knobs.tile_shift, P.knobs.chunk_items, BP.grp_step.size(), kn.ep_per_work,
@@ -360,7 +411,7 @@ int main(int argc, char** argv) {
       ms_between(t0, t1), ms_between(t1, t2), ms_between(tu0, tu1), ms_between(tx0, tx1), ms_between(tx1, tx2),
       index_ms, avg, pass_ms.front(), pass_ms.back(), repeat, warmup, stage_ms[0], stage_ms[1], stage_ms[2],
       stage_ms[3], stage_ms[4], stage_ms[5], sb_remap, sb_block, sb_hist, sb_scan, sb_replay, sb_combine,
-      remap_mismatch, double(nq) / (avg * 1e-3), bytes_alg, g_dev_bytes, dev, prop.multiProcessorCount);
+      remap_mismatch, dgen_json, double(nq) / (avg * 1e-3), bytes_alg, g_dev_bytes, dev, prop.multiProcessorCount);
   std::fwrite(buf, 1, size_t(n), stdout);
   if (!json_path.empty()) {
     FILE* f = std::fopen((json_path + ".tmp").c_str(), "w");

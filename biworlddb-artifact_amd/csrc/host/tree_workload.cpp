@@ -10,34 +10,15 @@
 #include <cstring>
 #include <memory>
 
+#include "gen_draws.hpp"
+
 namespace bwg {
 namespace {
 
-inline uint64_t mix64(uint64_t z) {
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
-  return z ^ (z >> 31);
-}
-
-// Counter-based draw: element `idx` of the splitmix64 sequence seeded by (seed, stream).
-struct Stream {
-  uint64_t base;
-  Stream(uint64_t seed, uint64_t stream) : base(mix64(seed ^ (stream * 0xD1B54A32D192ED03ULL))) {}
-  uint64_t at(uint64_t idx) const { return mix64(base + (idx + 1) * 0x9E3779B97F4A7C15ULL); }
-};
-
-inline double unit(uint64_t x) { return double(x >> 11) * (1.0 / 9007199254740992.0); }
-
-// Zipf(s) over M ranks by inverse CDF; the cumulative table is summed in rank order.
+// Zipf(s) over M ranks by inverse CDF over zipf_cumulative's table.
 struct Zipf {
   std::vector<double> cum;
-  Zipf(uint64_t m, double s) : cum(m) {
-    double acc = 0.0;
-    for (uint64_t r = 0; r < m; ++r) {
-      acc += std::pow(double(r + 1), -s);
-      cum[r] = acc;
-    }
-  }
+  Zipf(uint64_t m, double s) : cum(zipf_cumulative(m, s)) {}
   uint64_t pick(double u) const {
     const double t = u * cum.back();
     const size_t i = size_t(std::upper_bound(cum.begin(), cum.end(), t) - cum.begin());
@@ -46,15 +27,20 @@ struct Zipf {
 };
 
 void log_range(uint64_t dlen, uint64_t dpos, uint64_t n, uint32_t& lo, uint32_t& hi) {
-  const double l = std::floor(std::exp(unit(dlen) * std::log(double(n))));
-  uint64_t len = l < 1.0 ? 1 : uint64_t(l);
-  if (len > n) len = n;
-  const uint64_t a = dpos % (n - len + 1);
-  lo = uint32_t(a);
-  hi = uint32_t(a + len);
+  range_from_exp(std::exp(unit(dlen) * std::log(double(n))), dpos, n, lo, hi);
 }
 
 }  // namespace
+
+std::vector<double> zipf_cumulative(uint64_t m, double s) {
+  std::vector<double> cum(m);
+  double acc = 0.0;
+  for (uint64_t r = 0; r < m; ++r) {
+    acc += std::pow(double(r + 1), -s);
+    cum[r] = acc;
+  }
+  return cum;
+}
 
 TreeSpec tree_defaults(int cfg) {
   TreeSpec s;
@@ -137,7 +123,7 @@ void tree_generate(const TreeSpec& spec, TreeWorkload& w) {
 #pragma omp parallel for schedule(static)
     for (long long i = 0; i < (long long)nu; ++i) w.key[size_t(i)] = uint32_t(z.pick(unit(sk.at(uint64_t(i)))));
   }
-  const int64_t R = spec.cfg == 1 ? 1000 : (spec.cfg == 3 ? 10000 : 1000000);
+  const int64_t R = delta_bound(spec.cfg);
 #pragma omp parallel for schedule(static)
   for (long long i = 0; i < (long long)nu; ++i)
     w.delta[size_t(i)] = int64_t(sd.at(uint64_t(i)) % uint64_t(2 * R + 1)) - R;
@@ -145,9 +131,7 @@ void tree_generate(const TreeSpec& spec, TreeWorkload& w) {
   for (long long i = 0; i < (long long)nq; ++i) {
     const uint64_t a5 = sq5.at(uint64_t(i)), a6 = sq6.at(uint64_t(i));
     if (spec.cfg == 1) {
-      const uint32_t a = uint32_t(a5 % spec.N), b = uint32_t(a6 % spec.N);
-      w.qlo[size_t(i)] = std::min(a, b);
-      w.qhi[size_t(i)] = std::max(a, b) + 1u;
+      range_uniform(a5, a6, spec.N, w.qlo[size_t(i)], w.qhi[size_t(i)]);
     } else {
       log_range(a5, a6, spec.cfg == 3 ? 0xFFFFFFFFull : spec.N, w.qlo[size_t(i)], w.qhi[size_t(i)]);
     }
@@ -189,12 +173,10 @@ StreamSpec stream_defaults(int form) {
   return s;
 }
 
-void stream_generate(const TreeWorkload& tree, const StreamSpec& spec, RequestStream& rs) {
+void stream_scan(const TreeSpec& tree, const StreamSpec& spec, RequestStream& rs) {
   rs = RequestStream();
   rs.spec = spec;
-  const uint64_t seed = tree.spec.seed;
-  const int64_t S = tree.spec.S;
-  const uint64_t N = tree.spec.N;
+  const uint64_t seed = tree.seed;
   const int64_t R = spec.nreq;
   rs.req_q0.assign(size_t(R) + 1, 0);
   rs.req_kind.assign(size_t(R), 0);
@@ -223,6 +205,15 @@ void stream_generate(const TreeWorkload& tree, const StreamSpec& spec, RequestSt
     nq += sz;
     rs.req_q0[size_t(r) + 1] = nq;
   }
+}
+
+void stream_generate(const TreeWorkload& tree, const StreamSpec& spec, RequestStream& rs) {
+  stream_scan(tree.spec, spec, rs);
+  const uint64_t seed = tree.spec.seed;
+  const int64_t S = tree.spec.S;
+  const uint64_t N = tree.spec.N;
+  const int64_t R = spec.nreq;
+  const int64_t nq = rs.req_q0[size_t(R)], napp = R ? rs.req_app[size_t(R) - 1] + rs.req_kind[size_t(R) - 1] : 0;
   rs.qstate.resize(size_t(nq));
   rs.qlo.resize(size_t(nq));
   rs.qhi.resize(size_t(nq));

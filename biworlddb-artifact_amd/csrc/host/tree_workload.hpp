@@ -40,6 +40,11 @@ struct TreeWorkload {
   uint64_t coord_n() const { return spec.cfg == 3 ? dkey.size() : spec.N; }
 };
 
+// Cumulative Zipf(s) weights of ranks 1..m, summed in rank order: the inverse-CDF table of the
+// Zipf draws (rank r is picked for u when it is the first whose weight exceeds u * total).  Built
+// once on the host in this order, so every generator -- host or device -- reads the same doubles.
+std::vector<double> zipf_cumulative(uint64_t m, double s);
+
 TreeSpec tree_defaults(int cfg);
 void tree_generate(const TreeSpec& spec, TreeWorkload& w);
 // SHA-256 (hex) over parent[1..S], key, delta, qlo, qhi (and dkey for cfg 3), little-endian.
@@ -80,6 +85,11 @@ struct RequestStream {
 };
 // tree: the generated tree the stream runs over (its seed, S and N are used).
 void stream_generate(const TreeWorkload& tree, const StreamSpec& spec, RequestStream& rs);
+// The sequential part of stream_generate alone: request kinds, sizes (req_q0), appends before
+// each request (req_app) and arrival times; qstate / qlo / qhi and the append arrays stay empty.
+// Every element of those is a function of its own index (counter-based draws), so they can be
+// materialised anywhere afterwards (the device generator does).
+void stream_scan(const TreeSpec& tree, const StreamSpec& spec, RequestStream& rs);
 // SHA-256 (hex) over req_q0, req_kind, qstate, qlo, qhi, app_parent, app_key, app_delta.
 // Matches the oracle's stream --stream-sha.
 void stream_sha(const RequestStream& rs, char out_hex[65]);

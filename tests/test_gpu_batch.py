@@ -47,15 +47,19 @@ def test_batched_full_stream(pkg):
     ("s1_batched_small", ["--rmax", "64", "--tile-shift", "6", "--chunk", "128", "--ep-per-work", "65",
                           "--range-slots", "100", "--stage-shift", "8"]),
     ("s2_batched_mid", ["--range-slots", "65536", "--ep-per-work", "129"]),
-    ("s2_batched_mid", ["--bt-table-mb", "0"]),  # positions by binary search instead of the table
-    ("S4_batched_full", ["--tuned", None]),
+    ("s2_batched_mid", ["--bt-table-mb", "0", "--device-gen"]),  # positions by binary search, not the table
+    ("S4_batched_full", ["--tuned", None, "--device-gen"]),
 ])
 def test_device_plan_equals_host_plan(pkg, case, knobs):
     """The device planner (sorts, scans and searches on the GPU) builds the host planner's plan
-    array for array; the queries replayed through it give the oracle's answers."""
+    array for array; the queries replayed through it give the oracle's answers.  With
+    --device-gen the queries planned and replayed are the device generator's (equal to the
+    host's element for element), so the stream never crosses PCIe."""
     g = STREAMS[case]
     knobs = [pkg.TUNED if k is None else k for k in knobs]
     r = run_json(_cmd(pkg, g, knobs + ["--plain-sample", "0", "--device-plan"]), timeout=900)
     assert r["device_plan"]["used"] and r["device_plan"]["mismatch"] == 0, (case, knobs)
     assert r["answers_sha256"] == g["oracle"], (case, knobs)
+    if "--device-gen" in knobs:
+        assert r["device_gen"]["used"] and r["device_gen"]["mismatch"] == 0, r["device_gen"]
 
