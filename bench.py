@@ -27,6 +27,8 @@ Beside it, measured in the same run:
              average HIP-event duration against the MI355X HBM peak (8 TB/s); `traffic` is the
              rocprofv3 FETCH_SIZE + WRITE_SIZE of that kernel (profiles/), scaled by the same
              counters' reading for a copy of known size, when that summary exists;
+  extract -- the answers of the plain config-2 run (9.6M) leaving the device and hashed, one step
+             after another as the reference workers do, and overlapped (bw_extract_answers);
   cpu_baseline -- the oracle's rollback-Fenwick stream solver (1 host thread) on a bounded sample
              of the same stream (its first 10,000 batches).
 """
@@ -79,7 +81,8 @@ def main():
     knobs = ["--tuned", TUNED] if os.path.exists(TUNED) else []
     r = run([BW_BATCH, "--warmup", str(a.warmup), "--repeat", str(a.steps), "--plain-sample", "4", "--stages",
              "--device-plan", "--device-gen"] + knobs, 1200)
-    plain = run([BW_TREE, "--cfg", "2", "--mode", "pass", "--warmup", "2", "--repeat", "5"] + knobs, 600)
+    plain = run([BW_TREE, "--cfg", "2", "--mode", "pass", "--warmup", "2", "--repeat", "5", "--extract", "both"] +
+                knobs, 600)
     stages = {k: v for k, v in r["stage_ms"].items() if v > 0}
     dom = max(stages, key=stages.get)
     achieved = r["stage_bytes"][dom] / (stages[dom] * 1e-3) / 1e9
@@ -113,6 +116,8 @@ def main():
             "per_batch_query_only_queries_per_s": round(r["plain"]["query_queries_per_s"], 1),
             "per_batch_sample_batches": r["plain"]["batches"],
         },
+        "extract_config2": {"answers": plain["n_answers"], "serial_ms": plain["extract"]["serial"],
+                            "overlap_ms": plain["extract"]["overlap"], "same": plain["extract"]["same"]},
         "batched_vs_plain": round(r["queries_per_s"] / plain["queries_per_s"], 3),
         "batched_vs_plain_per_batch": round(r["queries_per_s"] / max(1.0, r["plain"]["pass_queries_per_s"]), 1),
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

@@ -32,6 +32,7 @@
 #include <string>
 #include <vector>
 
+#include "bwgpu_extract.h"
 #include "bwgpu_tree.h"
 #include "tree_engine.hpp"
 #include "tree_gen.hpp"
@@ -391,17 +392,16 @@ int main(int argc, char** argv) {
   const double sb_chunk = double(kit_run) * 12.0 + double(DB.plan.wk_chunk.size()) * double(E.x.ncls_b) * 8.0 +
                           double(DB.plan.eitems.size()) * 16.0;
   const double sb_combine = double(M) * (8.0 + 8.0 + 16.0 + 2.0 + 4.0 + 8.0 + 4.0 + 8.0 + 8.0);
-  const auto tx0 = clk::now();
+  // answers off the device while three host threads write and hash them (bw_extract_answers)
   std::vector<int64_t> ans(static_cast<size_t>(M));
-  HIPX(hipMemcpy(ans.data(), QB->answers, size_t(M) * sizeof(int64_t), hipMemcpyDeviceToHost));
-  const std::string sha = sha_hex(ans.data(), size_t(M) * sizeof(int64_t));
-  const auto tx1 = clk::now();
-  if (!answers_path.empty()) {
-    FILE* f = std::fopen(answers_path.c_str(), "wb");
-    if (!f) return 4;
-    const bool ok = std::fwrite(ans.data(), sizeof(int64_t), size_t(M), f) == size_t(M);
-    if (std::fclose(f) != 0 || !ok) return 4;
+  BwExtractResult xr{};
+  const int xrc = bw_extract_answers(QB->answers, size_t(M), nullptr,
+                                     answers_path.empty() ? nullptr : answers_path.c_str(), ans.data(), &xr);
+  if (xrc) {
+    std::fprintf(stderr, "bw_batch: extract failed (%d)\n", xrc);
+    return xrc < 0 ? 4 : 5;
   }
+  const std::string sha = xr.sha256_hex;
 
   // ---- plain comparison on the first batches: each batch on its own
   const int64_t K = std::min<int64_t>(plain_sample, ss.nreq);
@@ -456,7 +456,7 @@ int main(int argc, char** argv) {
       "\"chunk_items\":%" PRId64 ",\"ep_per_work\":%" PRId64 ",\"range_slots\":%" PRId64 ",\"npairs\":%" PRId64
       ",\"threads\":%d,\"waves\":%d,\"knobs_from\":\"%s\"},"
       "\"phases_ms\":{\"generate\":%.3f,\"index_plan_upload\":%.3f,\"batch_plan\":%.3f,\"batch_upload\":%.3f,"
-      "\"extract\":%.3f,\"index_build\":%.4f},"
+      "\"extract\":%.3f,\"index_build\":%.4f},\"answers_fnv1a\":\"%016" PRIx64 "\","
       "\"graph_ms\":{\"avg\":%.4f,\"min\":%.4f,\"max\":%.4f,\"repeat\":%d,\"warmup\":%d},"
       "\"queries_per_s\":%.1f,"
       "\"stage_ms\":{\"block_replay\":%.4f,\"chunk_replay\":%.4f,\"combine\":%.4f},"
@@ -471,7 +471,7 @@ int main(int argc, char** argv) {
       BP.grp_step.size(), BP.eitems.size(), BP.wk_chunk.size(), E.plan.knobs.rmax, E.plan.knobs.tile_shift,
       E.plan.knobs.chunk_items, E.plan.knobs.ep_per_work, E.plan.knobs.range_slots, BP.npairs, L.block_threads,
       L.chunk_waves, tuned_from.c_str(),
-      ms_between(t0, t1), ms_between(t1, t2), ms_between(t2, t3), ms_between(t3, t4), ms_between(tx0, tx1), index_ms,
+      ms_between(t0, t1), ms_between(t1, t2), ms_between(t2, t3), ms_between(t3, t4), xr.total_ms, index_ms, xr.fnv1a,
       avg, run_ms.front(), run_ms.back(), repeat, warmup, double(M) / (avg * 1e-3), st_ms[0], st_ms[1], st_ms[2],
       sb_block, sb_chunk, sb_combine, K, plain_q, plain_pass_ms,
       plain_query_ms, plain_pass_ms > 0 ? double(plain_q) / (plain_pass_ms * 1e-3) : 0.0,

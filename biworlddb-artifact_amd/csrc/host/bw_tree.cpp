@@ -4,14 +4,18 @@
 //
 //   bw_tree --cfg 2 [--S n --U n --Q n --N n --seed n] [--mode pass|query]
 //           [--rmax 2048 --tile-shift 10 --chunk 4096 --seg 32 --threads 256 --waves 4]
-//           [--warmup 1 --repeat 5] [--answers FILE] [--json FILE] [--device-gen]
+//           [--warmup 1 --repeat 5] [--answers FILE] [--json FILE] [--extract overlap] [--device-gen]
 // --mode pass  : the plain form -- every timed pass rebuilds the index (block replay with weights,
 //                chunk histograms, carry scan) and answers the declared queries.
 // --mode query : the index is built once (timed separately), every timed pass answers the
 //                declared queries over the resident index.
+// --extract overlap|serial|both : how the answers leave the device and are hashed -- overlapped
+//                (bw_extract_answers, the default), one step after another as the reference
+//                workers do, or both in turn (timed side by side, checksums compared).
 // --device-gen : also materialise the workload's update and query arrays with the device generator
 //                (include/bwgpu_gen.h) and compare them element for element with the host's.
-// Exit codes follow the reference: 2 usage, 3 unsupported input, 4 I/O, 5 GPU error.
+// Exit codes follow the reference: 2 usage, 3 unsupported input, 4 I/O, 5 GPU error; 6 when the
+// serial and overlapped extracts disagree.
 #include <hip/hip_runtime_api.h>
 #include <openssl/sha.h>
 
@@ -24,6 +28,7 @@
 #include <string>
 #include <vector>
 
+#include "bwgpu_extract.h"
 #include "bwgpu_tree.h"
 #include "tree_engine.hpp"
 #include "tree_gen.hpp"
@@ -79,7 +84,7 @@ int main(int argc, char** argv) {
   TreeSpec spec = tree_defaults(cfg);
   PlanKnobs kn;
   int threads = 256, waves = 4, warmup = 1, repeat = 5;
-  std::string answers_path, json_path, mode = "pass";
+  std::string answers_path, json_path, mode = "pass", extract_mode = "overlap";
   bool wsha = false, verify_remap = false, device_gen = false;
   int64_t copy_probe = 0;  // bytes of a device-to-device copy run first (HBM counter scale)
   // --tuned FILE: the "plain" form's knobs from the autotuning file, as defaults the flags override
@@ -122,13 +127,15 @@ int main(int argc, char** argv) {
     else if (!std::strcmp(k, "--warmup")) warmup = std::atoi(v);
     else if (!std::strcmp(k, "--repeat")) repeat = std::atoi(v);
     else if (!std::strcmp(k, "--answers")) answers_path = v;
+    else if (!std::strcmp(k, "--extract")) extract_mode = v;
     else if (!std::strcmp(k, "--json")) json_path = v;
     else if (!std::strcmp(k, "--copy-probe")) copy_probe = std::atoll(v);
     else if (!std::strcmp(k, "--tuned")) continue;
     else { std::fprintf(stderr, "bw_tree: unknown argument %s\n", k); return 2; }
   }
   if (spec.S < 1 || spec.U < 1 || spec.Q < 1 || spec.N < 1 || repeat < 1 || warmup < 0 ||
-      (mode != "pass" && mode != "query")) {
+      (mode != "pass" && mode != "query") ||
+      (extract_mode != "overlap" && extract_mode != "serial" && extract_mode != "both")) {
     std::fprintf(stderr, "bw_tree: bad sizes or mode\n");
     return 2;
   }
@@ -332,19 +339,56 @@ int main(int argc, char** argv) {
                   "{\"gpu_ms\":%.4f,\"wall_ms\":%.3f,\"host_ms\":%.3f,\"ties\":%u,\"mismatch\":%" PRId64 "}",
                   dgen_gpu_ms, dgen_wall_ms, ms_between(t0, t1), dgen_ties, dgen_mismatch);
 
-  const auto tx0 = clk::now();
+  // ---- download + extract: the answers off the device, written, hashed (SHA-256, FNV-1a 64).
+  // serial: one step after another, as the reference workers do; overlap: bw_extract_answers
+  // (chunks copied while three threads write and hash them); both: serial, then overlap, and the
+  // two must agree.
   std::vector<int64_t> ans(nq);
-  HIPX(hipMemcpy(ans.data(), Bt.answers, nq * sizeof(int64_t), hipMemcpyDeviceToHost));
-  const auto tx1 = clk::now();
-  if (!answers_path.empty()) {
-    FILE* f = std::fopen(answers_path.c_str(), "wb");
-    if (!f) return 4;
-    const bool ok = std::fwrite(ans.data(), sizeof(int64_t), nq, f) == nq;
-    if (std::fclose(f) != 0 || !ok) return 4;
+  std::string sha;
+  uint64_t fn = 0;
+  double dl_ms = 0, ex_ms = 0, ser[5] = {-1, -1, -1, -1, -1};
+  BwExtractResult xr{};
+  bool extract_same = true;
+  if (extract_mode != "overlap") {
+    const auto a = clk::now();
+    HIPX(hipMemcpy(ans.data(), Bt.answers, nq * sizeof(int64_t), hipMemcpyDeviceToHost));
+    const auto b = clk::now();
+    if (!answers_path.empty()) {
+      FILE* f = std::fopen(answers_path.c_str(), "wb");
+      if (!f) return 4;
+      const bool ok = std::fwrite(ans.data(), sizeof(int64_t), nq, f) == nq;
+      if (std::fclose(f) != 0 || !ok) return 4;
+    }
+    const auto c = clk::now();
+    sha = sha_hex(ans.data(), nq * sizeof(int64_t));
+    const auto d = clk::now();
+    fn = fnv1a(ans.data(), nq * sizeof(int64_t));
+    const auto e = clk::now();
+    const double v[5] = {ms_between(a, b), ms_between(b, c), ms_between(c, d), ms_between(d, e), ms_between(a, e)};
+    std::copy(v, v + 5, ser);
+    dl_ms = ser[0];
+    ex_ms = ser[4] - ser[0];
   }
-  const std::string sha = sha_hex(ans.data(), nq * sizeof(int64_t));
-  const uint64_t fn = fnv1a(ans.data(), nq * sizeof(int64_t));
-  const auto tx2 = clk::now();
+  if (extract_mode != "serial") {
+    const int xrc = bw_extract_answers(Bt.answers, nq, nullptr, answers_path.empty() ? nullptr : answers_path.c_str(),
+                                       ans.data(), &xr);
+    if (xrc) {
+      std::fprintf(stderr, "bw_tree: extract failed (%d)\n", xrc);
+      return xrc < 0 ? 4 : 5;
+    }
+    if (extract_mode == "both") extract_same = sha == xr.sha256_hex && fn == xr.fnv1a;
+    sha = xr.sha256_hex;
+    fn = xr.fnv1a;
+    dl_ms = xr.copy_ms;
+    ex_ms = xr.total_ms - xr.copy_ms;
+  }
+  char ext_json[512];
+  std::snprintf(ext_json, sizeof(ext_json),
+                "{\"mode\":\"%s\",\"serial\":{\"copy\":%.3f,\"write\":%.3f,\"sha256\":%.3f,\"fnv1a\":%.3f,"
+                "\"total\":%.3f},\"overlap\":{\"copy\":%.3f,\"write\":%.3f,\"sha256\":%.3f,\"fnv1a\":%.3f,"
+                "\"total\":%.3f},\"same\":%s}",
+                extract_mode.c_str(), ser[0], ser[1], ser[2], ser[3], ser[4], xr.copy_ms, xr.write_ms, xr.sha_ms,
+                xr.fnv_ms, xr.total_ms, extract_same ? "true" : "false");
 
   std::sort(pass_ms.begin(), pass_ms.end());
   double avg = 0;
@@ -403,15 +447,15 @@ int main(int argc, char** argv) {
       "\"chunk_replay\":%.4f,\"combine\":%.4f},"
       "\"stage_bytes\":{\"rank_remap\":%.0f,\"block_replay\":%.0f,\"chunk_hist\":%.0f,\"carry_scan\":%.0f,"
       "\"chunk_replay\":%.0f,\"combine\":%.0f},"
-      "\"remap_mismatch\":%" PRId64 ",\"device_gen\":%s,\"queries_per_s\":%.1f,\"bytes_alg\":%.0f,\"device_bytes\":%zu,\"device\":\"%s\",\"cus\":%d}\n",
+      "\"remap_mismatch\":%" PRId64 ",\"extract\":%s,\"device_gen\":%s,\"queries_per_s\":%.1f,\"bytes_alg\":%.0f,\"device_bytes\":%zu,\"device\":\"%s\",\"cus\":%d}\n",
       mode.c_str(), cfg, S, spec.U, spec.Q, spec.N, w.coord_n(), spec.seed, nq, sha.c_str(), fn, P.steps.size(),
       P.nblk(), P.bkey.size(), P.tile_rank0.size(), P.kitems.size(), BP.eitems.size(), P.nchunk(), BP.wk_chunk.size(),
       P.nchain(), P.nseg(), P.max_cls, kn.rmax, P.knobs.tile_shift, P.knobs.chunk_items, BP.grp_step.size(), kn.ep_per_work,
       kn.seg_chunks, threads, waves, tuned_from.c_str(),
-      ms_between(t0, t1), ms_between(t1, t2), ms_between(tu0, tu1), ms_between(tx0, tx1), ms_between(tx1, tx2),
+      ms_between(t0, t1), ms_between(t1, t2), ms_between(tu0, tu1), dl_ms, ex_ms,
       index_ms, avg, pass_ms.front(), pass_ms.back(), repeat, warmup, stage_ms[0], stage_ms[1], stage_ms[2],
       stage_ms[3], stage_ms[4], stage_ms[5], sb_remap, sb_block, sb_hist, sb_scan, sb_replay, sb_combine,
-      remap_mismatch, dgen_json, double(nq) / (avg * 1e-3), bytes_alg, g_dev_bytes, dev, prop.multiProcessorCount);
+      remap_mismatch, ext_json, dgen_json, double(nq) / (avg * 1e-3), bytes_alg, g_dev_bytes, dev, prop.multiProcessorCount);
   std::fwrite(buf, 1, size_t(n), stdout);
   if (!json_path.empty()) {
     FILE* f = std::fopen((json_path + ".tmp").c_str(), "w");
@@ -420,5 +464,5 @@ int main(int argc, char** argv) {
     std::fclose(f);
     if (std::rename((json_path + ".tmp").c_str(), json_path.c_str()) != 0) return 4;
   }
-  return 0;
+  return extract_same ? 0 : 6;
 }

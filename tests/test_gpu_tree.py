@@ -1,6 +1,8 @@
 """GPU parity of the generalised tree path: bw_tree (HIP kernels behind bwt_* C-ABI) against the
 oracle's golden hashes, which independent solvers (brute ancestor walk, the reference's own
 Solver) pinned.  Integer work: the bar is bit-exact (SHA-256 equality of the answer vector)."""
+import hashlib
+
 import pytest
 
 from conftest import golden, run_json
@@ -47,12 +49,20 @@ def test_tree_gpu_matches_golden(pkg, case, knobs, mode):
 @pytest.mark.gpu
 @pytest.mark.parametrize("mode", ["pass", "query"])
 @pytest.mark.parametrize("case", ["c2_cfg2", "c4_cfg4", "c3_cfg3"])
-def test_tree_gpu_full_size(pkg, case, mode):
-    """The benchmark configs at full size (300,000 states): same complete-answer SHA as the oracle."""
+def test_tree_gpu_full_size(pkg, case, mode, tmp_path):
+    """The benchmark configs at full size (300,000 states): same complete-answer SHA as the oracle.
+    In pass mode the answers are extracted both ways (serial copy / write / hashes, and the
+    overlapped bw_extract_answers) into the answers file; checksums and file must agree."""
     g = TREE[case]
-    r = run_json(_cmd(pkg, g, ["--mode", mode]), timeout=600)
+    path = str(tmp_path / "answers.bin")
+    extra = ["--extract", "both", "--answers", path] if mode == "pass" else []
+    r = run_json(_cmd(pkg, g, ["--mode", mode] + extra), timeout=600)
     assert r["answers_sha256"] == g["oracle"], case
     assert r["n_answers"] == g["S"] * g["Q"]
+    if mode == "pass":
+        assert r["extract"]["same"], r["extract"]
+        with open(path, "rb") as f:
+            assert hashlib.sha256(f.read()).hexdigest() == g["oracle"]
 
 
 @pytest.mark.gpu
