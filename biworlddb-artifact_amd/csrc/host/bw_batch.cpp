@@ -396,7 +396,8 @@ int main(int argc, char** argv) {
   std::vector<int64_t> ans(static_cast<size_t>(M));
   BwExtractResult xr{};
   const int xrc = bw_extract_answers(QB->answers, size_t(M), nullptr,
-                                     answers_path.empty() ? nullptr : answers_path.c_str(), ans.data(), &xr);
+                                     answers_path.empty() ? nullptr : answers_path.c_str(), ans.data(),
+                                     BW_EXTRACT_FNV_ON_DEVICE, &xr);
   if (xrc) {
     std::fprintf(stderr, "bw_batch: extract failed (%d)\n", xrc);
     return xrc < 0 ? 4 : 5;

@@ -10,8 +10,9 @@
 // --mode query : the index is built once (timed separately), every timed pass answers the
 //                declared queries over the resident index.
 // --extract overlap|serial|both : how the answers leave the device and are hashed -- overlapped
-//                (bw_extract_answers, the default), one step after another as the reference
-//                workers do, or both in turn (timed side by side, checksums compared).
+//                (bw_extract_answers with the FNV-1a on the device, the default), one step after
+//                another on the host as the reference workers do, or both in turn (timed side by
+//                side, checksums compared).
 // --device-gen : also materialise the workload's update and query arrays with the device generator
 //                (include/bwgpu_gen.h) and compare them element for element with the host's.
 // Exit codes follow the reference: 2 usage, 3 unsupported input, 4 I/O, 5 GPU error; 6 when the
@@ -371,7 +372,7 @@ int main(int argc, char** argv) {
   }
   if (extract_mode != "serial") {
     const int xrc = bw_extract_answers(Bt.answers, nq, nullptr, answers_path.empty() ? nullptr : answers_path.c_str(),
-                                       ans.data(), &xr);
+                                       ans.data(), BW_EXTRACT_FNV_ON_DEVICE, &xr);
     if (xrc) {
       std::fprintf(stderr, "bw_tree: extract failed (%d)\n", xrc);
       return xrc < 0 ? 4 : 5;

@@ -1,8 +1,8 @@
 // extract.cpp -- the overlapped extract (include/bwgpu_extract.h).  The calling thread copies the
-// answers off the device chunk by chunk; three threads follow it chunk by chunk: the file writer,
-// SHA-256 and FNV-1a.  Each checksum is one sequential chain over the bytes, so the extract takes
-// as long as its slowest chain (FNV-1a: one dependent multiply per byte) instead of the sum of
-// copy, write and both hashes.
+// answers off the device chunk by chunk; threads follow it chunk by chunk: the file writer,
+// SHA-256 and (unless the device computes it, bw_fnv1a_device, beside the copies) FNV-1a.  Each
+// checksum is one sequential chain over the bytes on the host, so the extract takes as long as
+// its slowest chain instead of the sum of copy, write and both hashes.
 #include "bwgpu_extract.h"
 
 #include <hip/hip_runtime_api.h>
@@ -58,7 +58,7 @@ class Progress {
 }  // namespace
 
 extern "C" int bw_extract_answers(const int64_t* d_answers, size_t n, void* stream, const char* path,
-                                  int64_t* host_out, BwExtractResult* result) {
+                                  int64_t* host_out, int flags, BwExtractResult* result) {
   if (!result || (n && !d_answers)) return int(hipErrorInvalidValue);
   const auto t0 = clk::now();
   *result = BwExtractResult{};
@@ -81,10 +81,32 @@ extern "C" int bw_extract_answers(const int64_t* d_answers, size_t n, void* stre
   const size_t nchunk = (total + kChunkBytes - 1) / kChunkBytes;
   auto chunk_len = [&](size_t k) { return std::min(kChunkBytes, total - k * kChunkBytes); };
 
+  uint64_t fnv = 1469598103934665603ULL;  // the reference's start value (src/common.hpp:396)
+  // FNV-1a on the device: its kernels run on a stream of their own, beside the copies
+  const bool dev_fnv = (flags & BW_EXTRACT_FNV_ON_DEVICE) != 0;
+  hipStream_t fs = nullptr;
+  void* fwork = nullptr;
+  uint64_t* d_fnv = nullptr;
+  if (dev_fnv) {
+    size_t wb = 0;
+    e = hipError_t(bw_fnv1a_workspace(total, &wb));
+    if (e == hipSuccess) e = hipMalloc(&fwork, wb + 256);
+    if (e == hipSuccess) e = hipStreamCreate(&fs);
+    if (e == hipSuccess) {
+      d_fnv = reinterpret_cast<uint64_t*>(static_cast<unsigned char*>(fwork) + wb);
+      e = hipError_t(bw_fnv1a_device(d_answers, total, fnv, fwork, wb, d_fnv, fs));
+    }
+    if (e != hipSuccess) {
+      if (fs) (void)hipStreamDestroy(fs);
+      if (fwork) (void)hipFree(fwork);
+      if (f) std::fclose(f);
+      return int(e);
+    }
+  }
+
   Progress prog;
   bool write_ok = true, sha_ok = true;
   unsigned char digest[32] = {};
-  uint64_t fnv = 1469598103934665603ULL;  // the reference's start value (src/common.hpp:396)
   std::thread writer, sha, fnv_thread;
   if (f)
     writer = std::thread([&] {
@@ -108,16 +130,17 @@ extern "C" int bw_extract_answers(const int64_t* d_answers, size_t n, void* stre
     if (c) EVP_MD_CTX_free(c);
     result->sha_ms = ms_since(t0);
   });
-  fnv_thread = std::thread([&] {
-    uint64_t h = fnv;
-    for (size_t k = 0; k < nchunk; ++k) {
-      if (!prog.wait_for(k + 1)) break;
-      const unsigned char* p = bytes + k * kChunkBytes;
-      for (size_t i = 0, m = chunk_len(k); i < m; ++i) h = (h ^ p[i]) * 0x100000001b3ULL;
-    }
-    fnv = h;
-    result->fnv_ms = ms_since(t0);
-  });
+  if (!dev_fnv)
+    fnv_thread = std::thread([&] {
+      uint64_t h = fnv;
+      for (size_t k = 0; k < nchunk; ++k) {
+        if (!prog.wait_for(k + 1)) break;
+        const unsigned char* p = bytes + k * kChunkBytes;
+        for (size_t i = 0, m = chunk_len(k); i < m; ++i) h = (h ^ p[i]) * 0x100000001b3ULL;
+      }
+      fnv = h;
+      result->fnv_ms = ms_since(t0);
+    });
 
   for (size_t k = 0; k < nchunk; ++k) {
     e = hipMemcpyAsync(reinterpret_cast<unsigned char*>(host) + k * kChunkBytes,
@@ -131,9 +154,17 @@ extern "C" int bw_extract_answers(const int64_t* d_answers, size_t n, void* stre
     prog.publish(k + 1);
   }
   result->copy_ms = ms_since(t0);
+  if (dev_fnv) {
+    hipError_t fe = hipMemcpyAsync(&fnv, d_fnv, sizeof(fnv), hipMemcpyDeviceToHost, fs);
+    if (fe == hipSuccess) fe = hipStreamSynchronize(fs);
+    result->fnv_ms = ms_since(t0);
+    (void)hipStreamDestroy(fs);
+    (void)hipFree(fwork);
+    if (e == hipSuccess) e = fe;
+  }
   if (writer.joinable()) writer.join();
   sha.join();
-  fnv_thread.join();
+  if (fnv_thread.joinable()) fnv_thread.join();
   if (f) std::fclose(f);  // only when the writer never ran to its end
   result->total_ms = ms_since(t0);
   if (e != hipSuccess) return int(e);

@@ -578,10 +578,21 @@ __global__ void __launch_bounds__(BWT_COMBINE_THREADS) bwt_combine_slots_kernel(
   }
 }
 
+// query ids per thread of the id pass: all their loads are issued before any is used
+constexpr int kIdsPer = 8;
+
 __global__ void __launch_bounds__(BWT_COMBINE_THREADS) bwt_combine_ids_kernel(const BwtBatch q, int64_t per) {
-  const int64_t i = xcd_major_block(per) * BWT_COMBINE_THREADS + threadIdx.x;
-  if (i >= q.M) return;
-  q.answers[i] = q.stage[q.ipos[i]];
+  const int64_t i0 = xcd_major_block(per) * (BWT_COMBINE_THREADS * kIdsPer) + threadIdx.x;
+  const int64_t last = q.M - 1;
+  uint32_t p[kIdsPer];
+#pragma unroll
+  for (int k = 0; k < kIdsPer; ++k) p[k] = q.ipos[min(i0 + k * BWT_COMBINE_THREADS, last)];
+  int64_t v[kIdsPer];
+#pragma unroll
+  for (int k = 0; k < kIdsPer; ++k) v[k] = q.stage[p[k]];
+#pragma unroll
+  for (int k = 0; k < kIdsPer; ++k)
+    if (i0 + k * BWT_COMBINE_THREADS <= last) q.answers[i0 + k * BWT_COMBINE_THREADS] = v[k];
 }
 
 // ---------------------------------------------------------------- overlay (appended states)
@@ -780,7 +791,7 @@ extern "C" int bwt_combine_launch(const BwtIndex* x, const BwtBatch* b, const Bw
                        0, (hipStream_t)stream, *x, *b, per_t);
   int rc = report("combine (slots)", hipGetLastError());
   if (rc) return rc;
-  const int64_t nb = (b->M + BWT_COMBINE_THREADS - 1) / BWT_COMBINE_THREADS;
+  const int64_t nb = (b->M + BWT_COMBINE_THREADS * kIdsPer - 1) / (BWT_COMBINE_THREADS * kIdsPer);
   const int64_t per = (nb + BWT_XCDS - 1) / BWT_XCDS;
   hipLaunchKernelGGL(bwt_combine_ids_kernel, dim3((unsigned)(per * BWT_XCDS)), dim3(BWT_COMBINE_THREADS), 0,
                      (hipStream_t)stream, *b, per);

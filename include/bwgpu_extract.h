@@ -32,12 +32,27 @@ typedef struct BwExtractResult {
   double total_ms;
 } BwExtractResult;
 
+/* Flags of bw_extract_answers. */
+#define BW_EXTRACT_FNV_ON_DEVICE 1 /* FNV-1a by bw_fnv1a_device (overlapping the copy) */
+
 /* Copies n answers from d_answers (device) to host_out (n int64, or NULL: an internal buffer) in
  * chunks, writing them to `path` (NULL: no file) and hashing them as the chunks arrive.
  * Returns 0, a hipError_t of a failed copy, or -1 when the file cannot be opened or fully
  * written (the reference workers' exit code 4). */
 int bw_extract_answers(const int64_t* d_answers, size_t n, void* stream, const char* path, int64_t* host_out,
-                       BwExtractResult* result);
+                       int flags, BwExtractResult* result);
+
+/* FNV-1a 64 of n bytes of device memory, from `start` (the reference's start value is
+ * 1469598103934665603), computed on the device: XOR with a byte changes only the low 8 bits of the
+ * state, and the low byte of the next state depends only on low bytes, so the low-byte sequence is
+ * an 8-bit chain whose map over a 4 KiB block can be tabulated for all 256 entry values at once
+ * and composed across blocks; with every block's entry low byte known, the hash is a polynomial
+ * in the prime, h = start * P^n + sum_j S_j * P^(n - end_j), whose block terms S_j are
+ * independent.  Writes the value to d_result (device, 8 bytes); asynchronous on `stream`.
+ * d_bytes: 4-byte aligned.  work: bw_fnv1a_workspace(n) bytes of device memory, 256-byte aligned. */
+int bw_fnv1a_workspace(size_t n, size_t* bytes);
+int bw_fnv1a_device(const void* d_bytes, size_t n, uint64_t start, void* work, size_t work_bytes, uint64_t* d_result,
+                    void* stream);
 
 #ifdef __cplusplus
 }

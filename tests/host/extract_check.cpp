@@ -16,6 +16,13 @@
 #include "bwgpu_extract.h"
 
 extern "C" hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+// the device FNV path is not taken here (flags 0); its calls only need to link
+extern "C" hipError_t hipMalloc(void**, size_t) { return hipErrorOutOfMemory; }
+extern "C" hipError_t hipFree(void*) { return hipSuccess; }
+extern "C" hipError_t hipStreamCreate(hipStream_t*) { return hipErrorNotSupported; }
+extern "C" hipError_t hipStreamDestroy(hipStream_t) { return hipSuccess; }
+extern "C" int bw_fnv1a_workspace(size_t, size_t* b) { *b = 256; return 0; }
+extern "C" int bw_fnv1a_device(const void*, size_t, uint64_t, void*, size_t, uint64_t*, void*) { return 1; }
 extern "C" hipError_t hipMemcpyAsync(void* dst, const void* src, size_t n, hipMemcpyKind, hipStream_t) {
   std::memcpy(dst, src, n);
   return hipSuccess;
@@ -31,7 +38,7 @@ int main(int argc, char** argv) {
     src[i] = int64_t(z) >> (i % 40);
   }
   BwExtractResult r;
-  const int rc = bw_extract_answers(src.data(), n, nullptr, argv[2], dst.data(), &r);
+  const int rc = bw_extract_answers(src.data(), n, nullptr, argv[2], dst.data(), 0, &r);
   unsigned char d[32];
   SHA256(reinterpret_cast<const unsigned char*>(src.data()), n * 8, d);
   char hex[65];

@@ -55,3 +55,33 @@ def test_unwritable_path_is_reported(checker, tmp_path):
     p = subprocess.run([checker, "10", str(tmp_path / "missing_dir" / "a.bin")], capture_output=True, text=True,
                        timeout=60, check=True)
     assert json.loads(p.stdout)["rc"] == -1
+
+
+@pytest.fixture(scope="module")
+def fnv_check(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("fnv") / "fnv_check")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I" + os.path.join(REPO, "biworlddb-artifact_amd", "csrc", "host"),
+                    "-o", out, os.path.join(REPO, "tests", "host", "fnv_check.cpp")], check=True, capture_output=True,
+                   timeout=300)
+    return out
+
+
+@pytest.mark.parametrize("n", [0, 1, 3, 4095, 4096, 4097, 100000, 300001, 17000003])
+def test_fnv_block_split_equals_sequential(fnv_check, n):
+    """The device FNV-1a's block split (low-byte maps for all 256 entries, composed 64 at a time
+    over up to three levels, block polynomials in the prime) restated step by step on the CPU
+    equals the sequential hash, at sizes that end inside a word, a block and a group."""
+    r = json.loads(subprocess.run([fnv_check, str(n), "5"], capture_output=True, text=True, check=True,
+                                  timeout=300).stdout)
+    assert r["split"] == r["sequential"], r
+    assert r["levels"] == (1 if n <= 64 * 4096 else 2 if n <= 64 * 64 * 4096 else 3)
+
+
+def test_fnv_file_mode_pins_the_start_value(fnv_check, tmp_path):
+    path = tmp_path / "b.bin"
+    data = bytes(range(256)) * 3 + b"xyz"
+    path.write_bytes(data)
+    r = json.loads(subprocess.run([fnv_check, "--file", str(path)], capture_output=True, text=True, check=True,
+                                  timeout=60).stdout)
+    assert r["sequential"] == "%016x" % _fnv1a64(data)
+
