@@ -72,12 +72,12 @@ def main():
     ap.add_argument("--no-served", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build measures one GPU; how the path would shard is described in DESIGN.md "
-              "(no multi-GPU run is made).", file=sys.stderr)
+        print("bench.py: measures one GPU only (no multi-GPU run is made); the path would shard by contiguous "
+              "block ranges of the Euler walk with one exclusive scan of per-GPU carry totals.", file=sys.stderr)
         sys.exit(2)
     for exe in (BW_BATCH, BW_TREE, BW_SERVE):
         if not os.path.exists(exe):
-            sys.exit("bench.py: native build missing (python -c 'import __graft_entry__ as g; g.build()')")
+            sys.exit("bench.py: native build missing (make -C biworlddb-artifact_amd)")
     knobs = ["--tuned", TUNED] if os.path.exists(TUNED) else []
     r = run([BW_BATCH, "--warmup", str(a.warmup), "--repeat", str(a.steps), "--plain-sample", "4", "--stages",
              "--device-plan", "--device-gen"] + knobs, 1200)

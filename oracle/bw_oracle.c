@@ -1,7 +1,7 @@
 /* bw_oracle.c -- CPU oracle for the bwgpu batch path.  TEST INFRASTRUCTURE ONLY.
  *
- * This program is the checker.  Only tests/, __graft_entry__.smoke() and bench.py's
- * cpu_baseline leg run it; no product code links or calls it.
+ * The oracle the GPU path is checked against.  Only tests/, the package's smoke check and
+ * bench.py's cpu_baseline leg run it; no product code links or calls it.
  *
  * Modes
  *   sha FILE                    SHA-256 of a file (FIPS 180-4, bw_sha256.h).

@@ -16,7 +16,7 @@ REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 @pytest.fixture(scope="module")
-def checker(tmp_path_factory):
+def extract_check(tmp_path_factory):
     out = str(tmp_path_factory.mktemp("extract") / "extract_check")
     subprocess.run(["g++", "-O2", "-std=c++17", "-Wno-deprecated-declarations", "-D__HIP_PLATFORM_AMD__",
                     "-I/opt/rocm/include", "-I" + os.path.join(REPO, "include"), "-o", out,
@@ -34,9 +34,9 @@ def _fnv1a64(data):
 
 
 @pytest.mark.parametrize("n", [0, 1, 1000, 524288, 524289, 3 * 524288 + 17])
-def test_overlapped_extract_equals_one_shot(checker, tmp_path, n):
+def test_overlapped_extract_equals_one_shot(extract_check, tmp_path, n):
     path = str(tmp_path / "a.bin")
-    p = subprocess.run([checker, str(n), path], capture_output=True, text=True, timeout=120, check=True)
+    p = subprocess.run([extract_check, str(n), path], capture_output=True, text=True, timeout=120, check=True)
     r = json.loads(p.stdout)
     assert r["rc"] == 0 and r["copied"], r
     assert r["sha"] == r["sha_one_shot"] and r["fnv"] == r["fnv_one_shot"], r
@@ -51,8 +51,8 @@ def test_overlapped_extract_equals_one_shot(checker, tmp_path, n):
         assert r["fnv"] == "14650fb0739d0383"
 
 
-def test_unwritable_path_is_reported(checker, tmp_path):
-    p = subprocess.run([checker, "10", str(tmp_path / "missing_dir" / "a.bin")], capture_output=True, text=True,
+def test_unwritable_path_is_reported(extract_check, tmp_path):
+    p = subprocess.run([extract_check, "10", str(tmp_path / "missing_dir" / "a.bin")], capture_output=True, text=True,
                        timeout=60, check=True)
     assert json.loads(p.stdout)["rc"] == -1
 
