@@ -6,7 +6,8 @@
 //   3. walk     from the start value's low byte, each level's group of 64 maps is walked to give
 //               every map its entry low byte, down to the blocks;
 //   4. poly     one thread per block replays its bytes from its entry low byte -> S_j;
-//   5. finish   h = start * P^n + sum_j S_j * P^(n - end_j), one workgroup.
+//   5. finish   h = start * P^n + sum_j S_j * P^(n - end_j), one workgroup (a polynomial in
+//               P^4096, by Horner over ranges of blocks).
 // Step 1 does 256 chain steps per byte (two per VALU op sequence) and dominates: the work is
 // fixed integer ALU per byte, spread over every CU, against one dependent multiply per byte for
 // the sequential hash on one CPU core.
@@ -143,21 +144,25 @@ __global__ void __launch_bounds__(256) fnv_poly_kernel(const unsigned char* byte
   poly[blk] = s;
 }
 
+// h = start * P^n + S_last + P^len_last * sum_{j < last} S_j Q^(last-1-j): each thread takes a
+// range of the full blocks (Horner in Q), the ranges' shares are summed.
 __global__ void __launch_bounds__(256) fnv_finish_kernel(const uint64_t* poly, int64_t nblk, size_t n, uint64_t start,
                                                          uint64_t* out) {
   __shared__ uint64_t part[256];
-  uint64_t acc = 0;
-  for (int64_t j = threadIdx.x; j < nblk; j += 256) {
-    const size_t end = (size_t)(j + 1) * bwg::kFnvBlock < n ? (size_t)(j + 1) * bwg::kFnvBlock : n;
-    acc += poly[j] * bwg::fnv_pow(n - end);
-  }
-  part[threadIdx.x] = acc;
+  const int64_t m = nblk > 0 ? nblk - 1 : 0;  // full blocks before the last one
+  const int64_t per = (m + 255) / 256;
+  const int64_t a = (int64_t)threadIdx.x * per, b = a + per < m ? a + per : m;
+  part[threadIdx.x] = bwg::fnv_range_sum(poly, a, b, m);
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) *out = start * bwg::fnv_pow(n) + part[0];
+  if (threadIdx.x == 0) {
+    uint64_t h = start * bwg::fnv_pow(n);
+    if (nblk > 0) h += poly[m] + bwg::fnv_pow(n - (size_t)m * bwg::kFnvBlock) * part[0];
+    *out = h;
+  }
 }
 
 }  // namespace

@@ -9,7 +9,9 @@
 //   * once a block's entry low byte is known, its d values are, and
 //       h_end = h_start * P^len + S,  S = sum over the block's bytes of d_i * P^(end - i)
 //     (Horner from 0: S <- (S + d_i) * P), so the whole hash is
-//       h_n = h_0 * P^n + sum over blocks j of S_j * P^(n - end_j)   (mod 2^64).
+//       h_n = h_0 * P^n + sum over blocks j of S_j * P^(n - end_j)   (mod 2^64),
+//     where every block but the last is kFnvBlock long, so the sum is a polynomial in
+//     Q = P^kFnvBlock (evaluated by Horner over ranges of blocks, fnv_range_sum).
 #pragma once
 #include <stddef.h>
 #include <stdint.h>
@@ -38,15 +40,29 @@ BWT_HD inline uint32_t fnv_low_step(uint32_t l, uint32_t b) { return ((l ^ b) * 
 // value times 0xB3 stays below 2^16, so one 32-bit multiply serves both halves.
 BWT_HD inline uint32_t fnv_low_step2(uint32_t x, uint32_t b) { return ((x ^ (b * 0x10001u)) * 0xB3u) & 0x00FF00FFu; }
 
-// P^k mod 2^64.
-BWT_HD inline uint64_t fnv_pow(uint64_t k) {
-  uint64_t r = 1, p = kFnvPrime;
+// base^k mod 2^64.
+BWT_HD inline uint64_t pow64(uint64_t base, uint64_t k) {
+  uint64_t r = 1;
   while (k) {
-    if (k & 1) r *= p;
-    p *= p;
+    if (k & 1) r *= base;
+    base *= base;
     k >>= 1;
   }
   return r;
+}
+
+// P^k mod 2^64.
+BWT_HD inline uint64_t fnv_pow(uint64_t k) { return pow64(kFnvPrime, k); }
+
+// The block sum of blocks [a, b) among the m full blocks before the last block, weighted for the
+// total: with Q = P^kFnvBlock, sum_{j<m} S_j P^(n - end_j) = P^len_last * sum_{j<m} S_j Q^(m-1-j),
+// and a range's share of the inner sum is (Horner over the range) * Q^(m - b).
+BWT_HD inline uint64_t fnv_range_sum(const uint64_t* S, int64_t a, int64_t b, int64_t m) {
+  if (a >= b) return 0;
+  const uint64_t Q = fnv_pow(kFnvBlock);
+  uint64_t h = 0;
+  for (int64_t j = a; j < b; ++j) h = h * Q + S[j];
+  return h * pow64(Q, (uint64_t)(m - b));
 }
 
 // The nb low bytes of w (little-endian order) through the chain from entry low byte l, adding

@@ -8,6 +8,7 @@
 #include <cinttypes>
 #include <cstdio>
 #include <cstdlib>
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -76,8 +77,8 @@ int main(int argc, char** argv) {
     }
     parent = st;
   }
-  // 4. block polynomials (words of 4 bytes, then the tail), 5. the sum
-  uint64_t h = kFnvStart * fnv_pow(n);
+  // 4. block polynomials (words of 4 bytes, then the tail)
+  std::vector<uint64_t> S(nblk);
   for (size_t j = 0; j < nblk; ++j) {
     const size_t b0 = j * kFnvBlock, len = std::min(kFnvBlock, n - b0);
     uint32_t l = parent[j];
@@ -88,7 +89,19 @@ int main(int argc, char** argv) {
       fnv_poly_word(w, 4, l, s);
     }
     for (; k < len; ++k) fnv_poly_word(b[b0 + k], 1, l, s);
-    h += s * fnv_pow(n - (b0 + len));
+    S[j] = s;
+  }
+  // 5. the sum as the finishing workgroup forms it: 256 ranges of the full blocks, then the last
+  uint64_t h = kFnvStart * fnv_pow(n);
+  if (nblk > 0) {
+    const int64_t m = int64_t(nblk) - 1, per = (m + 255) / 256;
+    uint64_t inner = 0;
+    for (int64_t t = 0; t < 256; ++t) inner += fnv_range_sum(S.data(), t * per, std::min(t * per + per, m), m);
+    h += S[size_t(m)] + fnv_pow(n - size_t(m) * kFnvBlock) * inner;
+    // and the direct form, block by block
+    uint64_t d = kFnvStart * fnv_pow(n);
+    for (size_t j = 0; j < nblk; ++j) d += S[j] * fnv_pow(n - std::min(n, (j + 1) * kFnvBlock));
+    if (d != h) h = ~h;  // the two forms of the sum must agree
   }
   std::printf("{\"split\": \"%016" PRIx64 "\", \"sequential\": \"%016" PRIx64 "\", \"levels\": %zu}\n", h, seq,
               lv.size());
