@@ -114,34 +114,74 @@ bool build_tree_index(int64_t S, int U, uint64_t NK, const int32_t* parent, cons
   }
   const int32_t nblk = P.nblk();
 
-  // ---- 3. per block: sorted distinct keys, key tiles, in-block ranks of every event
+  // ---- 3. per block: sorted distinct keys, key tiles, in-block ranks of every event.  A block's
+  //          events go through an open-addressing table (tagged with the block, so it is never
+  //          cleared): its distinct keys are collected, sorted, and their ranks stored back in the
+  //          table, from which every event reads its rank.
   std::vector<std::vector<uint32_t>> bk(static_cast<size_t>(nblk));
   std::vector<std::vector<uint16_t>> bt(static_cast<size_t>(nblk));
   P.state_block.assign(size_t(S) + 1, -1);
   P.state_step.assign(size_t(S) + 1, -1);
   P.evrank.resize(size_t(nsteps) * size_t(U));
-#pragma omp parallel for schedule(dynamic, 16)
-  for (int32_t b = 0; b < nblk; ++b) {
-    auto& ks = bk[size_t(b)];
-    for (int32_t s = P.blk_step0[size_t(b)]; s < P.blk_step0[size_t(b) + 1]; ++s) {
-      const int32_t st = P.steps[size_t(s)];
-      const int64_t v = st > 0 ? st : -st;
-      if (st > 0) { P.state_block[size_t(v)] = b; P.state_step[size_t(v)] = s; }
-      const uint32_t* kk = key + size_t(v - 1) * size_t(U);
-      ks.insert(ks.end(), kk, kk + U);
+  int tbits = 1;
+  while ((1 << tbits) < 2 * kn.rmax) ++tbits;
+#pragma omp parallel
+  {
+    const size_t cap = size_t(1) << tbits;
+    std::vector<uint32_t> tkey(cap);
+    std::vector<int32_t> ttag(cap, -1);
+    std::vector<uint16_t> trank(cap);
+    auto slot_of = [&](uint32_t k, int32_t tag, bool& found) {
+      size_t h = size_t((k * 0x9E3779B1u) >> (32 - tbits));
+      while (ttag[h] == tag) {
+        if (tkey[h] == k) {
+          found = true;
+          return h;
+        }
+        h = (h + 1) & (cap - 1);
+      }
+      found = false;
+      return h;
+    };
+#pragma omp for schedule(dynamic, 16)
+    for (int32_t b = 0; b < nblk; ++b) {
+      auto& ks = bk[size_t(b)];
+      bool ok = true;
+      for (int32_t s = P.blk_step0[size_t(b)]; s < P.blk_step0[size_t(b) + 1] && ok; ++s) {
+        const int32_t st = P.steps[size_t(s)];
+        const int64_t v = st > 0 ? st : -st;
+        if (st > 0) { P.state_block[size_t(v)] = b; P.state_step[size_t(v)] = s; }
+        const uint32_t* kk = key + size_t(v - 1) * size_t(U);
+        for (int i = 0; i < U; ++i) {
+          bool found;
+          const size_t h = slot_of(kk[i], b, found);
+          if (found) continue;
+          if (int64_t(ks.size()) >= kn.rmax) { ok = false; break; }  // cannot happen: the cut bounds it
+          tkey[h] = kk[i];
+          ttag[h] = b;
+          ks.push_back(kk[i]);
+        }
+      }
+      if (!ok) { ks.assign(size_t(kn.rmax) + 1, 0); continue; }  // reported below
+      std::sort(ks.begin(), ks.end());
+      for (size_t r = 0; r < ks.size(); ++r) {
+        bool found;
+        trank[slot_of(ks[r], b, found)] = uint16_t(r);
+      }
+      for (int32_t s = P.blk_step0[size_t(b)]; s < P.blk_step0[size_t(b) + 1]; ++s) {
+        const int32_t st = P.steps[size_t(s)];
+        const int64_t v = st > 0 ? st : -st;
+        const uint32_t* kk = key + size_t(v - 1) * size_t(U);
+        uint16_t* er = &P.evrank[size_t(s) * size_t(U)];
+        for (int i = 0; i < U; ++i) {
+          bool found;
+          er[i] = trank[slot_of(kk[i], b, found)];
+        }
+      }
+      auto& tr = bt[size_t(b)];
+      for (size_t r = 0; r < ks.size(); ++r)
+        if (r == 0 || (ks[r] >> ts) != (ks[r - 1] >> ts)) tr.push_back(uint16_t(r));
     }
-    std::sort(ks.begin(), ks.end());
-    ks.erase(std::unique(ks.begin(), ks.end()), ks.end());
-    for (int32_t s = P.blk_step0[size_t(b)]; s < P.blk_step0[size_t(b) + 1]; ++s) {
-      const int32_t st = P.steps[size_t(s)];
-      const int64_t v = st > 0 ? st : -st;
-      const uint32_t* kk = key + size_t(v - 1) * size_t(U);
-      uint16_t* er = &P.evrank[size_t(s) * size_t(U)];
-      for (int i = 0; i < U; ++i) er[i] = uint16_t(std::lower_bound(ks.begin(), ks.end(), kk[i]) - ks.begin());
-    }
-    auto& tr = bt[size_t(b)];
-    for (size_t r = 0; r < ks.size(); ++r)
-      if (r == 0 || (ks[r] >> ts) != (ks[r - 1] >> ts)) tr.push_back(uint16_t(r));
   }
   P.blk_key0.assign(size_t(nblk) + 1, 0);
   P.blk_tile0.assign(size_t(nblk) + 1, 0);
@@ -215,26 +255,54 @@ bool build_tree_index(int64_t S, int U, uint64_t NK, const int32_t* parent, cons
       }
     }
   }
-  // level-B positions: per tile, the blocks holding its keys and where their keys start
+  // level-B positions: per tile, the blocks holding its keys (in block order) and where their
+  // keys start on the tile's chain.  The blocks are split among the threads as for the items
+  // above; a thread's slots of a tile follow those of the threads before it.
   {
-    std::vector<int64_t> cnt(KT + 1, 0);
-    for (int32_t b = 0; b < nblk; ++b)
-      for (int64_t s = P.blk_tile0[size_t(b)]; s < P.blk_tile0[size_t(b) + 1]; ++s)
-        cnt[(P.bkey[size_t(P.blk_key0[size_t(b)] + P.tile_rank0[size_t(s)])] >> ts) + 1]++;
-    for (uint64_t t = 0; t < KT; ++t) cnt[t + 1] += cnt[t];
-    P.tile_tb0 = cnt;
-    P.tb_block.resize(size_t(cnt[KT]));
-    P.tb_pos.resize(size_t(cnt[KT]));
-    std::vector<int64_t> run(KT, 0), fill(cnt.begin(), cnt.end() - 1);
-    for (int32_t b = 0; b < nblk; ++b) {
-      const int64_t t0 = P.blk_tile0[size_t(b)], t1 = P.blk_tile0[size_t(b) + 1];
-      const int64_t R = P.blk_key0[size_t(b) + 1] - P.blk_key0[size_t(b)];
-      for (int64_t s = t0; s < t1; ++s) {
-        const uint32_t tile = P.bkey[size_t(P.blk_key0[size_t(b)] + P.tile_rank0[size_t(s)])] >> ts;
-        const int64_t n = ((s + 1 < t1) ? P.tile_rank0[size_t(s) + 1] : R) - P.tile_rank0[size_t(s)];
-        P.tb_block[size_t(fill[tile])] = b;
-        P.tb_pos[size_t(fill[tile]++)] = run[tile];
-        run[tile] += n;
+    std::vector<int64_t> scnt(size_t(nth) * KT, 0);
+    auto tile_of_slot = [&](int32_t b, int64_t s) {
+      return P.bkey[size_t(P.blk_key0[size_t(b)] + P.tile_rank0[size_t(s)])] >> ts;
+    };
+#pragma omp parallel num_threads(nth)
+    {
+      const int t = omp_get_thread_num();
+      int64_t* c = &scnt[size_t(t) * KT];
+      for (int32_t b = tb0[size_t(t)]; b < tb0[size_t(t) + 1]; ++b)
+        for (int64_t s = P.blk_tile0[size_t(b)]; s < P.blk_tile0[size_t(b) + 1]; ++s) c[tile_of_slot(b, s)]++;
+    }
+    P.tile_tb0.assign(KT + 1, 0);
+    for (uint64_t tile = 0; tile < KT; ++tile) {
+      int64_t n = 0;
+      for (int t = 0; t < nth; ++t) n += scnt[size_t(t) * KT + tile];
+      P.tile_tb0[tile + 1] = P.tile_tb0[tile] + n;
+    }
+    std::vector<int64_t> fo(size_t(nth) * KT), ro(size_t(nth) * KT);
+    for (uint64_t tile = 0; tile < KT; ++tile) {
+      int64_t f = P.tile_tb0[tile], r = 0;
+      for (int t = 0; t < nth; ++t) {
+        fo[size_t(t) * KT + tile] = f;
+        ro[size_t(t) * KT + tile] = r;
+        f += scnt[size_t(t) * KT + tile];
+        r += hist[size_t(t) * KT + tile];
+      }
+    }
+    P.tb_block.resize(size_t(P.tile_tb0[KT]));
+    P.tb_pos.resize(size_t(P.tile_tb0[KT]));
+#pragma omp parallel num_threads(nth)
+    {
+      const int t = omp_get_thread_num();
+      int64_t* f = &fo[size_t(t) * KT];
+      int64_t* r = &ro[size_t(t) * KT];
+      for (int32_t b = tb0[size_t(t)]; b < tb0[size_t(t) + 1]; ++b) {
+        const int64_t t0 = P.blk_tile0[size_t(b)], t1 = P.blk_tile0[size_t(b) + 1];
+        const int64_t R = P.blk_key0[size_t(b) + 1] - P.blk_key0[size_t(b)];
+        for (int64_t s = t0; s < t1; ++s) {
+          const uint32_t tile = tile_of_slot(b, s);
+          const int64_t n = ((s + 1 < t1) ? P.tile_rank0[size_t(s) + 1] : R) - P.tile_rank0[size_t(s)];
+          P.tb_block[size_t(f[tile])] = b;
+          P.tb_pos[size_t(f[tile]++)] = r[tile];
+          r[tile] += n;
+        }
       }
     }
   }
