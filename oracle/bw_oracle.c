@@ -23,7 +23,7 @@
  *       of every state and sums in __int128 (tiny inputs only; it shares no code with "fenwick").
  *       --workload-sha prints the SHA-256 of the generated arrays instead of solving.
  *   stream --form 4|5 [--S --U --N --seed (tree)] [--requests R --max-batch M --append-frac f
- *          --append-U n --rate r] [--algo fenwick|brute] [--stream-sha] --out ANS
+ *          --append-U n --rate r] [--algo fenwick|brute] [--threads T] [--stream-sha] --out ANS
  *       Answers of the request streams of the batched (4) and served (5) forms (bw_stream.h)
  *       over the tree extended by the appended states, in (request, query) order: the same
  *       rollback-Fenwick DFS with queries grouped by state, or the ancestor walk ("brute").
@@ -531,6 +531,123 @@ static void solve_states(int64_t ns, const int32_t* parent, const int64_t* upd0,
   free(qoff); free(qid); free(fen); free(koff); free(kid); free(stk_v); free(stk_c);
 }
 
+/* The same DFS cut into `threads` contiguous pieces of its step sequence, one per OpenMP thread.
+ * A first pass walks the tree without arithmetic and records, at the start of every piece, the
+ * DFS stack (the root path of the state being entered and the next child of each); a piece's
+ * Fenwick tree starts as the prefix sums of its root path's updates (built in O(NK)), then the
+ * piece runs its steps as the sequential solver does.  Every query is answered by exactly one
+ * piece, with the same arithmetic, so the answers equal solve_states' bit for bit. */
+static void solve_states_par(int64_t ns, const int32_t* parent, const int64_t* upd0, const uint32_t* krank,
+                             const int64_t* delta, uint64_t NK, int64_t nq, const int32_t* qstate,
+                             const uint32_t* lrank, const uint32_t* hrank, int64_t* ans, int threads) {
+  int64_t* qoff = (int64_t*)calloc((size_t)ns + 1, sizeof(int64_t));
+  int64_t* qid = (int64_t*)malloc(sizeof(int64_t) * (size_t)(nq ? nq : 1));
+  for (int64_t q = 0; q < nq; ++q) qoff[qstate[q] + 1]++;
+  for (int64_t v = 0; v < ns; ++v) qoff[v + 1] += qoff[v];
+  {
+    int64_t* cur = (int64_t*)malloc(sizeof(int64_t) * (size_t)ns);
+    memcpy(cur, qoff, sizeof(int64_t) * (size_t)ns);
+    for (int64_t q = 0; q < nq; ++q) qid[cur[qstate[q]]++] = q;
+    free(cur);
+  }
+  int64_t* koff = (int64_t*)calloc((size_t)ns + 1, sizeof(int64_t));
+  int32_t* kid = (int32_t*)malloc(sizeof(int32_t) * (size_t)ns);
+  for (int64_t v = 1; v < ns; ++v) koff[parent[v] + 1]++;
+  for (int64_t v = 0; v < ns; ++v) koff[v + 1] += koff[v];
+  {
+    int64_t* cur = (int64_t*)malloc(sizeof(int64_t) * (size_t)ns);
+    memcpy(cur, koff, sizeof(int64_t) * (size_t)ns);
+    for (int64_t v = 1; v < ns; ++v) kid[cur[parent[v]]++] = (int32_t)v;
+    free(cur);
+  }
+  /* the DFS loop runs 2 * (ns - 1) + 1 iterations (an entry per child, an exit per state) */
+  const int64_t iters = 2 * (ns - 1) + 1;
+  if (threads < 1) threads = 1;
+  int64_t* cut = (int64_t*)malloc(sizeof(int64_t) * (size_t)(threads + 1));
+  for (int t = 0; t <= threads; ++t) cut[t] = iters * t / threads;
+  int64_t** snap_v = (int64_t**)calloc((size_t)threads, sizeof(int64_t*));
+  int64_t** snap_c = (int64_t**)calloc((size_t)threads, sizeof(int64_t*));
+  int64_t* snap_sp = (int64_t*)calloc((size_t)threads, sizeof(int64_t));
+  {
+    int64_t* stk_v = (int64_t*)malloc(sizeof(int64_t) * (size_t)ns);
+    int64_t* stk_c = (int64_t*)malloc(sizeof(int64_t) * (size_t)ns);
+    int64_t sp = 1, it = 0;
+    int t = 0;
+    stk_v[0] = 0;
+    stk_c[0] = koff[0];
+    while (sp > 0) {
+      while (t < threads && cut[t] == it) {
+        snap_sp[t] = sp;
+        snap_v[t] = (int64_t*)malloc(sizeof(int64_t) * (size_t)sp);
+        snap_c[t] = (int64_t*)malloc(sizeof(int64_t) * (size_t)sp);
+        memcpy(snap_v[t], stk_v, sizeof(int64_t) * (size_t)sp);
+        memcpy(snap_c[t], stk_c, sizeof(int64_t) * (size_t)sp);
+        ++t;
+      }
+      const int64_t v = stk_v[sp - 1];
+      if (stk_c[sp - 1] < koff[v + 1]) {
+        const int64_t c = kid[stk_c[sp - 1]++];
+        stk_v[sp] = c;
+        stk_c[sp] = koff[c];
+        ++sp;
+      } else {
+        --sp;
+      }
+      ++it;
+    }
+    free(stk_v);
+    free(stk_c);
+  }
+#pragma omp parallel for num_threads(threads) schedule(static, 1)
+  for (int t = 0; t < threads; ++t) {
+    if (!snap_v[t]) continue; /* an empty piece */
+    int64_t* fen = (int64_t*)calloc((size_t)NK + 1, sizeof(int64_t));
+    int64_t* stk_v = (int64_t*)malloc(sizeof(int64_t) * (size_t)ns);
+    int64_t* stk_c = (int64_t*)malloc(sizeof(int64_t) * (size_t)ns);
+    int64_t sp = snap_sp[t];
+    memcpy(stk_v, snap_v[t], sizeof(int64_t) * (size_t)sp);
+    memcpy(stk_c, snap_c[t], sizeof(int64_t) * (size_t)sp);
+    /* the root path's updates as point values, then the Fenwick tree over them in O(NK) */
+    for (int64_t d = 1; d < sp; ++d)
+      for (int64_t i = upd0[stk_v[d]]; i < upd0[stk_v[d] + 1]; ++i) fen[(uint64_t)krank[i] + 1] += delta[i];
+    for (uint64_t j = 1; j <= NK; ++j) {
+      const uint64_t up = j + (j & (~j + 1));
+      if (up <= NK) fen[up] += fen[j];
+    }
+    for (int64_t it = cut[t]; it < cut[t + 1] && sp > 0; ++it) {
+      const int64_t v = stk_v[sp - 1];
+      if (stk_c[sp - 1] < koff[v + 1]) {
+        const int64_t c = kid[stk_c[sp - 1]++];
+        for (int64_t i = upd0[c]; i < upd0[c + 1]; ++i)
+          for (uint64_t j = (uint64_t)krank[i] + 1; j <= NK; j += j & (~j + 1)) fen[j] += delta[i];
+        for (int64_t k = qoff[c]; k < qoff[c + 1]; ++k) {
+          const int64_t q = qid[k];
+          int64_t r = 0;
+          for (uint64_t j = hrank[q]; j > 0; j -= j & (~j + 1)) r += fen[j];
+          for (uint64_t j = lrank[q]; j > 0; j -= j & (~j + 1)) r -= fen[j];
+          ans[q] = r;
+        }
+        stk_v[sp] = c;
+        stk_c[sp] = koff[c];
+        ++sp;
+      } else {
+        for (int64_t i = upd0[v]; i < upd0[v + 1]; ++i)
+          for (uint64_t j = (uint64_t)krank[i] + 1; j <= NK; j += j & (~j + 1)) fen[j] -= delta[i];
+        --sp;
+      }
+    }
+    free(fen);
+    free(stk_v);
+    free(stk_c);
+  }
+  for (int t = 0; t < threads; ++t) {
+    free(snap_v[t]);
+    free(snap_c[t]);
+  }
+  free(snap_v); free(snap_c); free(snap_sp); free(cut);
+  free(qoff); free(qid); free(koff); free(kid);
+}
+
 /* Independent check for small streams: walk each query's ancestor chain over raw keys. */
 static void solve_states_brute(const int32_t* parent, const int64_t* upd0, const uint32_t* key, const int64_t* delta,
                                int64_t nq, const int32_t* qstate, const uint32_t* qlo, const uint32_t* qhi,
@@ -555,7 +672,7 @@ static int mode_stream(int argc, char** argv) {
   twl_stream_defaults(form, &st);
   const char* out = NULL;
   const char* algo = "fenwick";
-  int ssha = 0;
+  int ssha = 0, threads = 1;
   for (int i = 2; i < argc; ++i) {
     if (!strcmp(argv[i], "--stream-sha")) { ssha = 1; continue; }
     if (i + 1 >= argc) die(2, "missing value");
@@ -571,6 +688,7 @@ static int mode_stream(int argc, char** argv) {
     else if (!strcmp(k, "--append-U")) st.append_U = atoi(v);
     else if (!strcmp(k, "--rate")) st.rate = atof(v);
     else if (!strcmp(k, "--algo")) algo = v;
+    else if (!strcmp(k, "--threads")) threads = atoi(v);
     else if (!strcmp(k, "--out")) out = v;
     else die(2, "unknown argument");
   }
@@ -621,6 +739,8 @@ static int mode_stream(int argc, char** argv) {
   const double ts0 = now_s();
   if (!strcmp(algo, "brute"))
     solve_states_brute(parent, upd0, key, delta, st.nq, st.qstate, st.qlo, st.qhi, ans);
+  else if (threads > 1)
+    solve_states_par(ns, parent, upd0, key, delta, w.N, st.nq, st.qstate, st.qlo, st.qhi, ans, threads);
   else
     solve_states(ns, parent, upd0, key, delta, w.N, st.nq, st.qstate, st.qlo, st.qhi, ans);
   g_solve_s = now_s() - ts0;

@@ -129,6 +129,16 @@ def test_product_stream_generator_matches_oracle(pkg, case):
     assert r["stream_sha256"] == g["stream"] and r["nq"] == g["nq"] and r["napp"] == g["napp"]
 
 
+@pytest.mark.parametrize("case", [k for k in STREAMS if not k.startswith("S")])
+@pytest.mark.parametrize("threads", [3, 7])
+def test_stream_oracle_threads_match_golden(oracle_bin, case, threads, tmp_path):
+    """The DFS cut into pieces over OpenMP threads (the CPU baseline's form) answers as the
+    sequential solver does."""
+    g = STREAMS[case]
+    r = run_json([oracle_bin, "stream"] + _stream_cmd(g) + ["--threads", str(threads), "--out", str(tmp_path / "s.bin")])
+    assert r["answers_sha256"] == g["oracle"]
+
+
 def test_stream_planted_mistake_is_caught(oracle_bin, tmp_path):
     """A stream solved over a tree whose one update is changed (another seed) must not match."""
     g = STREAMS["s1_batched_small"]
