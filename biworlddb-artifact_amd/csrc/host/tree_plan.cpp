@@ -19,7 +19,7 @@ inline uint64_t ep_item(uint64_t pos, uint32_t cls, uint64_t ep) {
 }
 }  // namespace
 
-bool build_tree_index(int64_t S, int U, uint64_t NK, const int32_t* parent, const uint32_t* key,
+bool plan_walk_blocks(int64_t S, int U, uint64_t NK, const int32_t* parent, const uint32_t* key,
                       const PlanKnobs& kn, TreeIndexPlan& P, char* err, size_t errlen) {
   auto fail = [&](const char* m) {
     if (err && errlen) std::snprintf(err, errlen, "%s", m);
@@ -49,7 +49,6 @@ bool build_tree_index(int64_t S, int U, uint64_t NK, const int32_t* parent, cons
 
   P = TreeIndexPlan();
   P.S = S; P.U = U; P.NK = NK; P.knobs = kn; P.knobs.tile_shift = ts; P.KT = int(KT);
-  const uint32_t W = 1u << ts, mask = W - 1;
 
   // ---- 1. children (ascending) and the Euler walk: +v on entry, -v after the subtree
   std::vector<int64_t> koff(size_t(S) + 2, 0);
@@ -112,6 +111,21 @@ bool build_tree_index(int64_t S, int U, uint64_t NK, const int32_t* parent, cons
     }
     P.blk_step0.push_back(int32_t(nsteps));
   }
+  return true;
+}
+
+bool plan_block_tables(const uint32_t* key, TreeIndexPlan& P, char* err, size_t errlen) {
+  auto fail = [&](const char* m) {
+    if (err && errlen) std::snprintf(err, errlen, "%s", m);
+    return false;
+  };
+  const int64_t S = P.S;
+  const int U = P.U;
+  const PlanKnobs& kn = P.knobs;
+  const int ts = kn.tile_shift;
+  const uint64_t KT = uint64_t(P.KT);
+  const uint32_t W = 1u << ts, mask = W - 1;
+  const int64_t nsteps = int64_t(P.steps.size());
   const int32_t nblk = P.nblk();
 
   // ---- 3. per block: sorted distinct keys, key tiles, in-block ranks of every event.  A block's
@@ -222,14 +236,6 @@ bool build_tree_index(int64_t S, int U, uint64_t NK, const int32_t* parent, cons
   P.chain_key0.assign(KT + 2, 0);
   P.chain_key0[1] = nC;
   for (uint64_t tile = 0; tile < KT; ++tile) P.chain_key0[tile + 2] = P.chain_key0[tile + 1] + tile_tot[tile];
-  P.chain_ncls.push_back(int32_t(KT));
-  P.chain_src.push_back(1);
-  P.chain_out.push_back(0);
-  for (uint64_t tile = 0; tile < KT; ++tile) {
-    P.chain_ncls.push_back(int32_t(W));
-    P.chain_src.push_back(0);
-    P.chain_out.push_back(1);
-  }
   std::vector<int64_t> cur(size_t(nth) * KT, 0);
   for (uint64_t tile = 0; tile < KT; ++tile) {
     int64_t o = P.chain_key0[tile + 1];
@@ -306,7 +312,25 @@ bool build_tree_index(int64_t S, int U, uint64_t NK, const int32_t* parent, cons
       }
     }
   }
+  (void)fail;
+  return true;
+}
 
+void plan_chunks(TreeIndexPlan& P) {
+  const PlanKnobs& kn = P.knobs;
+  const uint64_t KT = uint64_t(P.KT);
+  const uint32_t W = 1u << kn.tile_shift;
+  const int32_t nblk = P.nblk();
+  const int64_t nC = P.blk_tile0[size_t(nblk)];
+  // the chains: 0 = level C (classes = tiles), 1 + t = level B of tile t (classes = keys in a tile)
+  P.chain_ncls.assign(1, int32_t(KT));
+  P.chain_src.assign(1, 1);
+  P.chain_out.assign(1, 0);
+  for (uint64_t tile = 0; tile < KT; ++tile) {
+    P.chain_ncls.push_back(int32_t(W));
+    P.chain_src.push_back(0);
+    P.chain_out.push_back(1);
+  }
   // ---- 5. chunks (at least one per chain) and carry-scan segments.  Level C: one chunk per
   //          block (its tile slots), so every level-C endpoint sits at a chunk start and its
   //          carry is a plain read of the block's dense row.  Level B: chunks of chunk_items key
@@ -345,6 +369,13 @@ bool build_tree_index(int64_t S, int U, uint64_t NK, const int32_t* parent, cons
   P.n_cp = cp;
   P.n_segsum = sg;
   P.max_cls = int(std::max<uint64_t>(KT, W));
+}
+
+bool build_tree_index(int64_t S, int U, uint64_t NK, const int32_t* parent, const uint32_t* key,
+                      const PlanKnobs& knobs, TreeIndexPlan& P, char* err, size_t errlen) {
+  if (!plan_walk_blocks(S, U, NK, parent, key, knobs, P, err, errlen)) return false;
+  if (!plan_block_tables(key, P, err, errlen)) return false;
+  plan_chunks(P);
   return true;
 }
 
