@@ -5,7 +5,7 @@
 //   bw_batch [--S n --U n --N n --seed n] [--requests 100000 --max-batch 2048]
 //            [--rmax --tile-shift --chunk --ep-per-work --range-slots --seg --threads --waves] [--tuned FILE]
 //            [--warmup 1 --repeat 3] [--plain-sample 8] [--stages] [--answers FILE] [--json FILE]
-//            [--stream-sha] [--device-plan] [--device-gen]
+//            [--stream-sha] [--device-plan] [--device-gen] [--device-index]
 //
 // The index (block weights, carry rows) is built once on the device.  Every batch of the stream
 // is then answered by one launch sequence (block replay, chunk replay, combine) captured in a
@@ -19,6 +19,9 @@
 // array for array with the host plan, and replays the device plan.  --device-gen materialises the
 // stream's queries with the device generator (include/bwgpu_gen.h), compares them element for
 // element with the host generator's, and feeds the device arrays to the planner and the replays.
+// --device-index (implies --device-plan) also builds the index tables on the device
+// (include/bwgpu_index.h), compares them array for array with the host-built ones, and answers the
+// stream once more over them (device plan + query sequence); the answers must be the same.
 // Exit codes: 2 usage, 3 unsupported input, 4 I/O, 5 GPU error.
 #include <hip/hip_runtime_api.h>
 #include <openssl/sha.h>
@@ -84,7 +87,7 @@ int main(int argc, char** argv) {
   PlanKnobs kn;
   LaunchKnobs lk;
   int warmup = 1, repeat = 3, plain_sample = 8;
-  bool want_sha = false, stages = false, dev_plan = false, dev_gen = false;
+  bool want_sha = false, stages = false, dev_plan = false, dev_gen = false, dev_index = false;
   int fork = 1;  // replay the block and level-B stages on two streams
   std::string answers_path, json_path, tuned_path;
   bool have_knob[8] = {false, false, false, false, false, false, false, false};
@@ -93,6 +96,7 @@ int main(int argc, char** argv) {
     if (!std::strcmp(argv[i], "--stages")) { stages = true; continue; }
     if (!std::strcmp(argv[i], "--device-plan")) { dev_plan = true; continue; }
     if (!std::strcmp(argv[i], "--device-gen")) { dev_gen = true; continue; }
+    if (!std::strcmp(argv[i], "--device-index")) { dev_index = dev_plan = true; continue; }
     if (i + 1 >= argc) { std::fprintf(stderr, "bw_batch: missing value for %s\n", argv[i]); return 2; }
     const char* k = argv[i];
     const char* v = argv[++i];
@@ -195,11 +199,13 @@ int main(int argc, char** argv) {
   HIPX(hipSetDevice(0));
   HIPX(hipFree(nullptr));
   TreeEngine E;
+  const auto ti0 = clk::now();
   if (!E.init(spec.S, spec.U, w.coord_n(), w.parent.data(), w.key.data(), w.delta.data(), kn,
               BwtLaunch{lk.block_threads, lk.chunk_waves, 0, 0}, err, sizeof(err))) {
     std::fprintf(stderr, "bw_batch: index: %s\n", err);
     return std::strncmp(err, "device", 6) == 0 ? 5 : 3;
   }
+  const double host_index_ms = ms_between(ti0, clk::now());  // host index plan + its upload
   // the device generator: the stream's queries materialised on the GPU (timed on the second run:
   // the first loads code), compared element for element with the host generator's
   DeviceStream DS;
@@ -254,12 +260,13 @@ int main(int argc, char** argv) {
   // the device planner: the same plan built on the GPU from the device copies of the queries,
   // compared array for array with the host plan; its batch is then the one replayed
   DevicePlanSlot dslot;
+  DeviceArena qa;  // the device copy of the query states (device planner)
+  const int32_t* d_qstate = nullptr;
   BwtBatch* QB = &DB.b;
   double dplan_ms = -1;
   int64_t plan_mismatch = -1;
   if (dev_plan) {
-    DeviceArena qa;
-    const int32_t* d_qstate = dev_gen ? DS.qstate : qa.upload(rs.qstate);
+    d_qstate = dev_gen ? DS.qstate : qa.upload(rs.qstate);
     if (!qa.ok() || !alloc_plan_slot(E, M, dslot, err, sizeof(err))) {
       std::fprintf(stderr, "bw_batch: device plan: %s\n", qa.ok() ? err : "upload");
       return 5;
@@ -404,6 +411,66 @@ int main(int argc, char** argv) {
   }
   const std::string sha = xr.sha256_hex;
 
+  // ---- the index tables built on the device (--device-index): compared with the host-built
+  // tables array for array, then the stream planned and answered over that index
+  char didx_json[768] = "null";
+  int64_t didx_mismatch = -1;
+  if (dev_index) {
+    TreeEngine ED;
+    if (!ED.init_device_index(spec.S, spec.U, w.coord_n(), w.parent.data(), w.key.data(), nullptr, w.delta.data(),
+                              kn, L, st, err, sizeof(err))) {
+      std::fprintf(stderr, "bw_batch: %s\n", err);
+      return 5;
+    }
+    didx_mismatch = 0;
+    auto same = [&](const char* name, const void* a, const void* b, size_t bytes) {
+      std::vector<unsigned char> ha(bytes), hb(bytes);
+      if (bytes && (hipMemcpy(ha.data(), a, bytes, hipMemcpyDeviceToHost) != hipSuccess ||
+                    hipMemcpy(hb.data(), b, bytes, hipMemcpyDeviceToHost) != hipSuccess || ha != hb)) {
+        std::fprintf(stderr, "bw_batch: device index differs in %s\n", name);
+        ++didx_mismatch;
+      }
+    };
+    const TreeIndexPlan& H = E.plan;
+    const size_t nst = H.steps.size(), nbk = size_t(H.nblk()), KT = size_t(H.KT);
+    if (ED.plan.nblk() != H.nblk() || ED.device_index.nkeys != int64_t(H.bkey.size()) ||
+        ED.device_index.nslots != int64_t(H.tile_rank0.size()) || ED.plan.chunk_key0 != H.chunk_key0 ||
+        ED.plan.chunk_cp0 != H.chunk_cp0 || ED.plan.seg_chunk0 != H.seg_chunk0) {
+      std::fprintf(stderr, "bw_batch: device index shape differs\n");
+      ++didx_mismatch;
+    } else {
+      same("evrank", ED.x.evrank, E.x.evrank, nst * size_t(spec.U) * 2);
+      same("blk_key0", ED.x.blk_key0, E.x.blk_key0, (nbk + 1) * 8);
+      same("blk_tile0", ED.x.blk_tile0, E.x.blk_tile0, (nbk + 1) * 8);
+      same("bkey", ED.x.bkey, E.x.bkey, H.bkey.size() * 4);
+      same("tile_rank0", ED.x.tile_rank0, E.x.tile_rank0, H.tile_rank0.size() * 2);
+      same("kitems", ED.x.kitems, E.x.kitems, H.kitems.size() * 8);
+      same("state_step", ED.px.state_step, E.px.state_step, (size_t(spec.S) + 1) * 4);
+      same("step_block", ED.px.step_block, E.px.step_block, nst * 4);
+      same("chain_key0", ED.px.chain_key0, E.px.chain_key0, (KT + 2) * 8);
+      same("tile_tb0", ED.px.tile_tb0, E.px.tile_tb0, (KT + 1) * 8);
+      same("tb_block", ED.px.tb_block, E.px.tb_block, H.tb_block.size() * 4);
+      same("tb_pos", ED.px.tb_pos, E.px.tb_pos, H.tb_pos.size() * 8);
+      if (ED.px.bt_pos && E.px.bt_pos) same("bt_pos", ED.px.bt_pos, E.px.bt_pos, nbk * KT * 4);
+    }
+    // the stream over the device-built index: index weights, device plan, query sequence
+    RUN(ED.build_index(st));
+    RUN(device_plan(ED, dslot, M, d_qstate, d_qlo, d_qhi, st));
+    RUN(bwt_query_launch(&ED.x, &dslot.b, &L, st));
+    BwExtractResult dx{};
+    const int drc = bw_extract_answers(dslot.b.answers, size_t(M), st, nullptr, nullptr, BW_EXTRACT_FNV_ON_DEVICE, &dx);
+    if (drc) return drc < 0 ? 4 : 5;
+    if (sha != dx.sha256_hex) {
+      std::fprintf(stderr, "bw_batch: answers over the device index differ\n");
+      ++didx_mismatch;
+    }
+    std::snprintf(didx_json, sizeof(didx_json),
+                  "{\"walk_cut_ms\":%.3f,\"gpu_ms\":%.3f,\"total_ms\":%.3f,\"host_plan_ms\":%.3f,"
+                  "\"nkeys\":%" PRId64 ",\"nslots\":%" PRId64 ",\"answers_sha256\":\"%s\",\"mismatch\":%" PRId64 "}",
+                  ED.device_index.host_ms, ED.device_index.gpu_ms, ED.device_index.total_ms, host_index_ms,
+                  ED.device_index.nkeys, ED.device_index.nslots, dx.sha256_hex, didx_mismatch);
+  }
+
   // ---- plain comparison on the first batches: each batch on its own
   const int64_t K = std::min<int64_t>(plain_sample, ss.nreq);
   double plain_pass_ms = 0, plain_query_ms = 0;
@@ -465,7 +532,7 @@ int main(int argc, char** argv) {
       "\"plain\":{\"batches\":%" PRId64 ",\"queries\":%" PRId64 ",\"pass_ms\":%.4f,\"query_ms\":%.4f,"
       "\"pass_queries_per_s\":%.1f,\"query_queries_per_s\":%.1f,\"mismatch\":%" PRId64 "},"
       "\"device_plan\":{\"used\":%s,\"ms\":%.4f,\"mismatch\":%" PRId64 "},"
-      "\"device_gen\":{\"used\":%s,\"gpu_ms\":%.4f,\"wall_ms\":%.3f,\"host_ms\":%.3f,\"ties\":%u,"
+      "\"device_index\":%s,\"device_gen\":{\"used\":%s,\"gpu_ms\":%.4f,\"wall_ms\":%.3f,\"host_ms\":%.3f,\"ties\":%u,"
       "\"mismatch\":%" PRId64 "},\"forked\":%d,"
       "\"device_bytes\":%zu,\"device\":\"%s\",\"cus\":%d}\n",
       spec.S, spec.U, spec.N, spec.seed, ss.nreq, ss.max_batch, M, sha.c_str(), E.plan.nblk(), BP.qblk.size(),
@@ -477,7 +544,7 @@ int main(int argc, char** argv) {
       sb_block, sb_chunk, sb_combine, K, plain_q, plain_pass_ms,
       plain_query_ms, plain_pass_ms > 0 ? double(plain_q) / (plain_pass_ms * 1e-3) : 0.0,
       plain_query_ms > 0 ? double(plain_q) / (plain_query_ms * 1e-3) : 0.0, plain_mismatch,
-      dev_plan ? "true" : "false", dplan_ms, plan_mismatch, dev_gen ? "true" : "false", dgen_gpu_ms, dgen_wall_ms,
+      dev_plan ? "true" : "false", dplan_ms, plan_mismatch, didx_json, dev_gen ? "true" : "false", dgen_gpu_ms, dgen_wall_ms,
       stream_host_ms, DS.ties, dgen_mismatch, fork, E.arena.bytes + io.bytes + DB.arena.bytes + dslot.arena.bytes,
       dev, prop.multiProcessorCount);
   std::fwrite(buf, 1, size_t(n), stdout);
@@ -487,5 +554,5 @@ int main(int argc, char** argv) {
     std::fwrite(buf, 1, size_t(n), f);
     std::fclose(f);
   }
-  return plain_mismatch == 0 && plan_mismatch <= 0 && dgen_mismatch <= 0 ? 0 : 6;
+  return plain_mismatch == 0 && plan_mismatch <= 0 && dgen_mismatch <= 0 && didx_mismatch <= 0 ? 0 : 6;
 }

@@ -2,6 +2,7 @@
 #include "tree_engine.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 
 namespace bwg {
@@ -123,6 +124,159 @@ bool TreeEngine::init(int64_t S, int U, uint64_t NK, const int32_t* parent, cons
     if (err && errlen) std::snprintf(err, errlen, "device upload: %s", hipGetErrorString(arena.err));
     return false;
   }
+  return true;
+}
+
+bool TreeEngine::init_device_index(int64_t S, int U, uint64_t NK, const int32_t* parent, const uint32_t* key,
+                                   const uint32_t* d_key, const int64_t* delta, const PlanKnobs& knobs,
+                                   const BwtLaunch& l, hipStream_t st, char* err, size_t errlen) {
+  auto fail = [&](const char* what, int rc) {
+    if (err && errlen) std::snprintf(err, errlen, "device index: %s (%d)", what, rc);
+    return false;
+  };
+  const auto t0 = std::chrono::steady_clock::now();
+  if (!plan_walk_blocks(S, U, NK, parent, key, knobs, plan, err, errlen)) return false;
+  const auto t1 = std::chrono::steady_clock::now();
+  launch = l;
+  TreeIndexPlan& P = plan;
+  const int32_t nblk = P.nblk();
+  const int64_t nsteps = int64_t(P.steps.size());
+  DeviceArena tmp;  // the build's workspace and counts, freed at the end
+  BwtIndexBuild in{};
+  in.S = S;
+  in.U = U;
+  in.rmax = P.knobs.rmax;
+  in.tile_shift = P.knobs.tile_shift;
+  in.KT = P.KT;
+  in.nsteps = nsteps;
+  in.nblk = nblk;
+  x = BwtIndex{};
+  x.steps = arena.upload(P.steps);
+  x.blk_step0 = arena.upload(P.blk_step0);
+  in.steps = x.steps;
+  in.blk_step0 = x.blk_step0;
+  in.key = d_key ? d_key : tmp.upload(key, size_t(S) * size_t(U));
+  size_t wb = 0;
+  int rc = bwt_index_workspace(&in, &wb);
+  if (rc) return fail("workspace", rc);
+  void* work = tmp.alloc(wb);
+  int32_t* nkeys = tmp.scratch<int32_t>(size_t(nblk));
+  int32_t* ntiles = tmp.scratch<int32_t>(size_t(nblk));
+  BwtIndexTables T{};
+  T.evrank = arena.scratch<uint16_t>(size_t(nsteps) * size_t(U));
+  T.state_step = arena.scratch<int32_t>(size_t(S) + 1);
+  T.step_block = arena.scratch<int32_t>(size_t(nsteps));
+  if (!arena.ok() || !tmp.ok()) return fail("allocation", int(arena.ok() ? tmp.err : arena.err));
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return fail("events", 0);
+  (void)hipEventRecord(e0, st);
+  if ((rc = bwt_index_keys_launch(&in, &T, nkeys, ntiles, work, wb, st))) return fail("stage 1", rc);
+  // the two counts' exclusive scans (on the host: nblk entries each)
+  const size_t nb = static_cast<size_t>(nblk);
+  std::vector<int32_t> hk(nb), ht(nb);
+  if (hipMemcpyAsync(hk.data(), nkeys, hk.size() * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipMemcpyAsync(ht.data(), ntiles, ht.size() * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return fail("counts", 0);
+  P.blk_key0.assign(size_t(nblk) + 1, 0);
+  P.blk_tile0.assign(size_t(nblk) + 1, 0);
+  for (int32_t b = 0; b < nblk; ++b) {
+    if (hk[size_t(b)] < 1 || hk[size_t(b)] > P.knobs.rmax) return fail("a block exceeds rmax", hk[size_t(b)]);
+    P.blk_key0[size_t(b) + 1] = P.blk_key0[size_t(b)] + hk[size_t(b)];
+    P.blk_tile0[size_t(b) + 1] = P.blk_tile0[size_t(b)] + ht[size_t(b)];
+  }
+  const int64_t nkeys_total = P.blk_key0[size_t(nblk)], nslots = P.blk_tile0[size_t(nblk)];
+  x.blk_key0 = arena.upload(P.blk_key0);
+  x.blk_tile0 = arena.upload(P.blk_tile0);
+  T.blk_key0 = x.blk_key0;
+  T.blk_tile0 = x.blk_tile0;
+  T.bkey = arena.scratch<uint32_t>(size_t(nkeys_total));
+  T.tile_rank0 = arena.scratch<uint16_t>(size_t(nslots));
+  T.kitems = arena.scratch<uint64_t>(size_t(nslots + nkeys_total));
+  T.chain_key0 = arena.scratch<int64_t>(size_t(P.KT) + 2);
+  T.tile_tb0 = arena.scratch<int64_t>(size_t(P.KT) + 1);
+  T.tb_block = arena.scratch<int32_t>(size_t(nslots));
+  T.tb_pos = arena.scratch<int64_t>(size_t(nslots));
+  const size_t nbt = size_t(nblk) * size_t(P.KT);
+  T.bt_pos = nbt * sizeof(int32_t) <= size_t(std::max<int64_t>(0, P.knobs.bt_table_mb)) << 20
+                 ? arena.scratch<int32_t>(nbt)
+                 : nullptr;
+  if (!arena.ok()) return fail("allocation", int(arena.err));
+  if ((rc = bwt_index_chains_launch(&in, &T, nkeys_total, nslots, work, wb, st))) return fail("stage 2", rc);
+  (void)hipEventRecord(e1, st);
+  P.chain_key0.assign(size_t(P.KT) + 2, 0);
+  if (hipMemcpyAsync(P.chain_key0.data(), T.chain_key0, P.chain_key0.size() * 8, hipMemcpyDeviceToHost, st) !=
+          hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return fail("chain offsets", 0);
+  float gpu_ms = 0;
+  (void)hipEventElapsedTime(&gpu_ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  plan_chunks(P);
+  // the index as init() lays it out
+  x.S = S;
+  x.U = U;
+  x.nblk = nblk;
+  x.rmax = P.knobs.rmax;
+  x.max_cls = P.max_cls;
+  x.nchain = P.nchain();
+  x.ncls_c = P.chain_ncls[0];
+  x.ncls_b = P.chain_ncls.size() > 1 ? P.chain_ncls[1] : 1;
+  x.tile_shift = P.knobs.tile_shift;
+  x.nchunk = P.nchunk();
+  x.nchunk_c = P.chain_chunk0[1];
+  x.nseg = P.nseg();
+  x.delta = arena.upload(delta, size_t(S) * size_t(U));
+  x.evrank = T.evrank;
+  x.bkey = T.bkey;
+  x.tile_rank0 = T.tile_rank0;
+  x.kitems = T.kitems;
+  x.chunk_key0 = arena.upload(P.chunk_key0);
+  x.chunk_chain = arena.upload(P.chunk_chain);
+  x.chunk_cp0 = arena.upload(P.chunk_cp0);
+  x.chain_ncls = arena.upload(P.chain_ncls);
+  x.chain_src = arena.upload(P.chain_src);
+  x.chain_out = arena.upload(P.chain_out);
+  x.chain_chunk0 = arena.upload(P.chain_chunk0);
+  x.seg_chunk0 = arena.upload(P.seg_chunk0);
+  x.seg_cp0 = arena.upload(P.seg_cp0);
+  x.chain_seg0 = arena.upload(P.chain_seg0);
+  x.agg = arena.scratch<int64_t>(size_t(nkeys_total));
+  x.tagg = arena.scratch<int64_t>(size_t(nslots));
+  x.cp = arena.scratch<int64_t>(size_t(P.n_cp));
+  x.segsum = arena.scratch<int64_t>(size_t(P.n_segsum));
+  x.kweight = arena.scratch<int64_t>(size_t(nslots + nkeys_total));
+  px = BwtPlanIndex{};
+  px.nsteps = nsteps;
+  px.NK = P.NK;
+  px.KT = P.KT;
+  px.stage_shift = P.knobs.stage_shift;
+  px.chunk_items = P.knobs.chunk_items;
+  px.ep_per_work = P.knobs.ep_per_work;
+  px.range_slots = P.knobs.range_slots;
+  px.state_step = T.state_step;
+  px.step_block = T.step_block;
+  px.tile_tb0 = T.tile_tb0;
+  px.tb_block = T.tb_block;
+  px.tb_pos = T.tb_pos;
+  px.chain_key0 = T.chain_key0;
+  px.bt_pos = T.bt_pos;
+  std::vector<int32_t> allb(nb);
+  for (int32_t b = 0; b < nblk; ++b) allb[size_t(b)] = b;
+  std::vector<int64_t> zeros(nb + 1, 0);
+  all_blocks = BwtBatch{};
+  all_blocks.nqblk = nblk;
+  all_blocks.qblk = arena.upload(allb);
+  all_blocks.qblk_g0 = arena.upload(zeros);
+  all_blocks.grp_q0 = all_blocks.qblk_g0;
+  if (!arena.ok()) return fail("upload", int(arena.err));
+  if (hipStreamSynchronize(st) != hipSuccess) return fail("sync", 0);
+  device_index.host_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  device_index.gpu_ms = gpu_ms;
+  device_index.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  device_index.nkeys = nkeys_total;
+  device_index.nslots = nslots;
   return true;
 }
 

@@ -63,3 +63,23 @@ def test_device_plan_equals_host_plan(pkg, case, knobs):
     if "--device-gen" in knobs:
         assert r["device_gen"]["used"] and r["device_gen"]["mismatch"] == 0, r["device_gen"]
 
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case,knobs", [
+    ("s1_batched_small", []),
+    ("s1_batched_small", ["--rmax", "64", "--tile-shift", "6", "--chunk", "128", "--ep-per-work", "65",
+                          "--range-slots", "100", "--stage-shift", "8"]),
+    ("s2_batched_mid", ["--bt-table-mb", "0"]),
+    ("S4_batched_full", ["--tuned", None]),
+])
+def test_device_index_equals_host_index(pkg, case, knobs):
+    """The index tables built on the device (per-block keys and ranks by LDS hash set and bitonic
+    sort, chains by a stable radix sort of the tile slots) equal the host-built tables array for
+    array; the stream planned and answered over them gives the oracle's answers."""
+    g = STREAMS[case]
+    knobs = [pkg.TUNED if k is None else k for k in knobs]
+    r = run_json(_cmd(pkg, g, knobs + ["--plain-sample", "0", "--device-index"]), timeout=900)
+    d = r["device_index"]
+    assert d["mismatch"] == 0 and d["answers_sha256"] == g["oracle"], (case, knobs, d)
+    assert r["answers_sha256"] == g["oracle"]
+
