@@ -10,7 +10,9 @@ bw_batch times each with HIP events on its stream.  The answers' SHA-256 is repo
 suite pins it against the oracle).
 
 The stream's queries are materialised on the GPU (bwt_gen_*: the host generator's arrays, element
-for element; its time is reported beside the host generator's), the batch is planned on the GPU
+for element; its time is reported beside the host generator's), the index is also built on the GPU
+once more (bwt_index_*: equal to the host-built index; its build time beside the host's, and the
+stream answered over it must give the same SHA-256), the batch is planned on the GPU
 (bwt_plan_launch: the host planner's plan, array for array; its time is reported beside the
 replay), and the replay runs over that device-generated, device-planned batch.
 
@@ -81,7 +83,7 @@ def main():
             sys.exit("bench.py: native build missing (make -C biworlddb-artifact_amd)")
     knobs = ["--tuned", TUNED] if os.path.exists(TUNED) else []
     r = run([BW_BATCH, "--warmup", str(a.warmup), "--repeat", str(a.steps), "--plain-sample", "4", "--stages",
-             "--device-plan", "--device-gen"] + knobs, 1200)
+             "--device-plan", "--device-gen", "--device-index"] + knobs, 1200)
     plain = run([BW_TREE, "--cfg", "2", "--mode", "pass", "--warmup", "2", "--repeat", "5", "--extract", "both"] +
                 knobs, 600)
     stages = {k: v for k, v in r["stage_ms"].items() if v > 0}
@@ -130,6 +132,10 @@ def main():
                         "plan_and_query_queries_per_s": round(r["n_answers"] / ((r["device_plan"]["ms"] +
                                                                                   r["graph_ms"]["avg"]) * 1e-3), 1)},
         "host_batch_plan_ms": r["phases_ms"]["batch_plan"],
+        "device_index": {"total_ms": r["device_index"]["total_ms"], "device_ms": r["device_index"]["gpu_ms"],
+                         "host_walk_ms": r["device_index"]["walk_ms"],
+                         "host_index_ms": r["device_index"]["host_plan_ms"],
+                         "equals_host_index": r["device_index"]["mismatch"] == 0},
         "device_gen": {"gpu_ms": r["device_gen"]["gpu_ms"], "wall_ms": r["device_gen"]["wall_ms"],
                        "host_generator_ms": r["device_gen"]["host_ms"],
                        "equals_host_generator": r["device_gen"]["mismatch"] == 0, "ties": r["device_gen"]["ties"]},

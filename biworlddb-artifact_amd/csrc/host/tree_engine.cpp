@@ -129,17 +129,16 @@ bool TreeEngine::init(int64_t S, int U, uint64_t NK, const int32_t* parent, cons
 
 bool TreeEngine::init_device_index(int64_t S, int U, uint64_t NK, const int32_t* parent, const uint32_t* key,
                                    const uint32_t* d_key, const int64_t* delta, const PlanKnobs& knobs,
-                                   const BwtLaunch& l, hipStream_t st, char* err, size_t errlen) {
+                                   const BwtLaunch& l, hipStream_t st, char* err, size_t errlen, bool device_cut) {
   auto fail = [&](const char* what, int rc) {
     if (err && errlen) std::snprintf(err, errlen, "device index: %s (%d)", what, rc);
     return false;
   };
   const auto t0 = std::chrono::steady_clock::now();
-  if (!plan_walk_blocks(S, U, NK, parent, key, knobs, plan, err, errlen)) return false;
+  if (!plan_walk_blocks(S, U, NK, parent, key, knobs, plan, err, errlen, !device_cut)) return false;
   const auto t1 = std::chrono::steady_clock::now();
   launch = l;
   TreeIndexPlan& P = plan;
-  const int32_t nblk = P.nblk();
   const int64_t nsteps = int64_t(P.steps.size());
   DeviceArena tmp;  // the build's workspace and counts, freed at the end
   BwtIndexBuild in{};
@@ -149,15 +148,40 @@ bool TreeEngine::init_device_index(int64_t S, int U, uint64_t NK, const int32_t*
   in.tile_shift = P.knobs.tile_shift;
   in.KT = P.KT;
   in.nsteps = nsteps;
-  in.nblk = nblk;
   x = BwtIndex{};
   x.steps = arena.upload(P.steps);
-  x.blk_step0 = arena.upload(P.blk_step0);
   in.steps = x.steps;
-  in.blk_step0 = x.blk_step0;
   in.key = d_key ? d_key : tmp.upload(key, size_t(S) * size_t(U));
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return fail("events", 0);
+  (void)hipEventRecord(e0, st);
+  int rc;
+  if (device_cut) {  // the block cut on the device, then its block starts to the host
+    size_t cb = 0;
+    if ((rc = bwt_index_cut_workspace(&in, &cb))) return fail("cut workspace", rc);
+    void* cwork = tmp.alloc(cb);
+    int32_t* d_b0 = arena.scratch<int32_t>(size_t(nsteps) + 1);
+    int32_t* d_nb = tmp.scratch<int32_t>(1);
+    if (!arena.ok() || !tmp.ok()) return fail("allocation", int(arena.ok() ? tmp.err : arena.err));
+    const int32_t mbs = int32_t(std::min<int64_t>(P.knobs.max_block_steps, int64_t(1) << 30));
+    if ((rc = bwt_index_cut_launch(&in, mbs, d_b0, d_nb, cwork, cb, st))) return fail("cut", rc);
+    int32_t nb = 0;
+    if (hipMemcpyAsync(&nb, d_nb, 4, hipMemcpyDeviceToHost, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
+      return fail("cut count", 0);
+    if (nb < 1 || nb > nsteps) return fail("cut count", nb);
+    P.blk_step0.assign(size_t(nb) + 1, 0);
+    if (hipMemcpyAsync(P.blk_step0.data(), d_b0, P.blk_step0.size() * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return fail("cut download", 0);
+    x.blk_step0 = d_b0;
+  } else {
+    x.blk_step0 = arena.upload(P.blk_step0);
+  }
+  const int32_t nblk = P.nblk();
+  in.nblk = nblk;
+  in.blk_step0 = x.blk_step0;
   size_t wb = 0;
-  int rc = bwt_index_workspace(&in, &wb);
+  rc = bwt_index_workspace(&in, &wb);
   if (rc) return fail("workspace", rc);
   void* work = tmp.alloc(wb);
   int32_t* nkeys = tmp.scratch<int32_t>(size_t(nblk));
@@ -167,9 +191,6 @@ bool TreeEngine::init_device_index(int64_t S, int U, uint64_t NK, const int32_t*
   T.state_step = arena.scratch<int32_t>(size_t(S) + 1);
   T.step_block = arena.scratch<int32_t>(size_t(nsteps));
   if (!arena.ok() || !tmp.ok()) return fail("allocation", int(arena.ok() ? tmp.err : arena.err));
-  hipEvent_t e0, e1;
-  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return fail("events", 0);
-  (void)hipEventRecord(e0, st);
   if ((rc = bwt_index_keys_launch(&in, &T, nkeys, ntiles, work, wb, st))) return fail("stage 1", rc);
   // the two counts' exclusive scans (on the host: nblk entries each)
   const size_t nb = static_cast<size_t>(nblk);

@@ -20,7 +20,7 @@ inline uint64_t ep_item(uint64_t pos, uint32_t cls, uint64_t ep) {
 }  // namespace
 
 bool plan_walk_blocks(int64_t S, int U, uint64_t NK, const int32_t* parent, const uint32_t* key,
-                      const PlanKnobs& kn, TreeIndexPlan& P, char* err, size_t errlen) {
+                      const PlanKnobs& kn, TreeIndexPlan& P, char* err, size_t errlen, bool cut) {
   auto fail = [&](const char* m) {
     if (err && errlen) std::snprintf(err, errlen, "%s", m);
     return false;
@@ -81,7 +81,7 @@ bool plan_walk_blocks(int64_t S, int U, uint64_t NK, const int32_t* parent, cons
   if (nsteps >= (int64_t(1) << 31) / U) return fail("too many steps");
 
   // ---- 2. blocks: greedy, at most rmax distinct keys per block
-  {
+  if (cut) {
     std::vector<int32_t> seen_blk(NK, -1), seen_step(NK, -1);
     P.blk_step0.push_back(0);
     int32_t b = 0;

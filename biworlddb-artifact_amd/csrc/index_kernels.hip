@@ -241,6 +241,106 @@ __global__ void __launch_bounds__(kT) idx_btpos_kernel(int nblk, int KT, BwtInde
   t.bt_pos[x] = lo < e ? (int32_t)t.tb_pos[lo] : (int32_t)(t.chain_key0[tl + 2] - t.chain_key0[tl + 1]);
 }
 
+// ---- stage 0: the block cut.  e(s): where a block starting at step s would end -- the first
+// step t > s whose keys take the block's distinct keys past rmax, or at max_block_steps.
+constexpr int kCutT = 128;
+
+template <int kCap>
+__global__ void __launch_bounds__(kCutT) idx_cut_end_kernel(const BwtIndexBuild in, int32_t mbs, int32_t* e_out) {
+  __shared__ uint32_t tab[kCap];  // key + 1, 0 = empty
+  __shared__ int total;
+  const int tid = threadIdx.x;
+  const int64_t s0 = blockIdx.x;
+  for (int i = tid; i < kCap; i += kCutT) tab[i] = 0;
+  if (tid == 0) total = 0;
+  __syncthreads();
+  const int U = in.U;
+  int64_t e = in.nsteps;
+  for (int64_t t = s0; t < in.nsteps; ++t) {
+    if (t > s0 && t - s0 >= mbs) {
+      e = t;
+      break;
+    }
+    const int st = in.steps[t];
+    const int64_t v = st > 0 ? st : -st;
+    for (int i = tid; i < U; i += kCutT) {
+      const uint32_t k = in.key[(v - 1) * (int64_t)U + i];
+      uint32_t h = (k * 0x9E3779B1u) >> (32 - __builtin_ctz((unsigned)kCap));
+      for (int probe = 0; probe < kCap; ++probe) {
+        const uint32_t old = atomicCAS(&tab[h], 0u, k + 1u);
+        if (old == 0u) {
+          atomicAdd(&total, 1);
+          break;
+        }
+        if (old == k + 1u) break;
+        h = (h + 1) & (kCap - 1);
+      }
+    }
+    __syncthreads();
+    const int d = total;  // distinct keys of steps s0..t
+    __syncthreads();      // every thread has read it before the next step adds to it
+    if (t > s0 && d > in.rmax) {
+      e = t;
+      break;
+    }
+  }
+  if (tid == 0) e_out[s0] = (int32_t)e;
+}
+
+__global__ void __launch_bounds__(kT) idx_cut_init_kernel(int64_t n, int32_t* J, int32_t* mark) {
+  const int64_t s = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (s > n) return;
+  if (s == n) J[s] = (int32_t)n;  // the end maps to itself
+  mark[s] = s == 0 ? 1 : 0;
+}
+
+// one round of pointer doubling: marks J(s) for every marked s, and J <- J o J
+__global__ void __launch_bounds__(kT) idx_cut_mark_kernel(int64_t n, const int32_t* J, int32_t* mark) {
+  const int64_t s = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (s <= n && mark[s]) mark[J[s]] = 1;
+}
+__global__ void __launch_bounds__(kT) idx_cut_jump_kernel(int64_t n, const int32_t* J, int32_t* J2) {
+  const int64_t s = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (s <= n) J2[s] = J[J[s]];
+}
+
+__global__ void __launch_bounds__(kT) idx_cut_scatter_kernel(int64_t n, const int32_t* mark, const int32_t* pos,
+                                                             int32_t* blk_step0, int32_t* nblk) {
+  const int64_t s = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (s < n && mark[s]) blk_step0[pos[s]] = (int32_t)s;
+  if (s == n - 1) {
+    const int32_t nb = pos[s] + mark[s];
+    blk_step0[nb] = (int32_t)n;
+    *nblk = nb;
+  }
+}
+
+struct CutLayout {
+  size_t J, J2, mark, pos, temp, temp_bytes, total;
+};
+
+hipError_t cut_layout_of(const BwtIndexBuild* in, CutLayout& L) {
+  const size_t n = (size_t)in->nsteps + 1;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = off;
+    off += (bytes + 255) & ~size_t(255);
+    return at;
+  };
+  L.J = take(n * 4);
+  L.J2 = take(n * 4);
+  L.mark = take(n * 4);
+  L.pos = take(n * 4);
+  size_t b = 0;
+  const hipError_t e = rocprim::exclusive_scan(nullptr, b, (const int32_t*)nullptr, (int32_t*)nullptr, (int32_t)0,
+                                               (size_t)in->nsteps, rocprim::plus<int32_t>());
+  if (e != hipSuccess) return e;
+  L.temp_bytes = std::max<size_t>(b, 1);
+  L.temp = take(L.temp_bytes);
+  L.total = off;
+  return hipSuccess;
+}
+
 // workspace layout
 struct WsLayout {
   size_t tmp, slot_tile, slot_tile_s, slot_id, slot_id_s, slot_block, slot_n, n_sorted, start, temp, temp_bytes,
@@ -294,6 +394,66 @@ T* at(void* base, size_t off) {
 }
 
 }  // namespace
+
+extern "C" int bwt_index_cut_workspace(const BwtIndexBuild* in, size_t* bytes) {
+  if (!in || !bytes || in->nsteps < 1 || in->nsteps >= (int64_t(1) << 31) - 1) return int(hipErrorInvalidValue);
+  CutLayout L;
+  const hipError_t e = cut_layout_of(in, L);
+  if (e != hipSuccess) return int(e);
+  *bytes = L.total;
+  return 0;
+}
+
+extern "C" int bwt_index_cut_launch(const BwtIndexBuild* in, int32_t max_block_steps, int32_t* blk_step0, int32_t* nblk,
+                                    void* work, size_t work_bytes, void* stream) {
+  if (!in || !in->steps || !in->key || in->U < 1 || in->rmax < in->U || in->rmax > 8192 || in->nsteps < 1 ||
+      in->nsteps >= (int64_t(1) << 31) - 1 || max_block_steps < 1 || !blk_step0 || !nblk)
+    return int(hipErrorInvalidValue);
+  CutLayout L;
+  hipError_t e = cut_layout_of(in, L);
+  if (e != hipSuccess) return int(e);
+  if (!work || work_bytes < L.total) return int(hipErrorInvalidValue);
+  hipStream_t st = hipStream_t(stream);
+  int32_t* J = at<int32_t>(work, L.J);
+  int32_t* J2 = at<int32_t>(work, L.J2);
+  int32_t* mark = at<int32_t>(work, L.mark);
+  int32_t* pos = at<int32_t>(work, L.pos);
+  const int64_t n = in->nsteps;
+  const dim3 gn((unsigned)((n + 1 + kT - 1) / kT)), blk(kT);
+  hipLaunchKernelGGL(idx_cut_init_kernel, gn, blk, 0, st, n, J, mark);
+#define BWT_IDX_CUT(C) \
+  hipLaunchKernelGGL(idx_cut_end_kernel<C>, dim3((unsigned)n), dim3(kCutT), 0, st, *in, max_block_steps, J)
+  int cap = 64;
+  while (cap < 2 * (in->rmax + in->U)) cap <<= 1;
+  switch (cap) {
+    case 64: BWT_IDX_CUT(64); break;
+    case 128: BWT_IDX_CUT(128); break;
+    case 256: BWT_IDX_CUT(256); break;
+    case 512: BWT_IDX_CUT(512); break;
+    case 1024: BWT_IDX_CUT(1024); break;
+    case 2048: BWT_IDX_CUT(2048); break;
+    case 4096: BWT_IDX_CUT(4096); break;
+    case 8192: BWT_IDX_CUT(8192); break;
+    case 16384: BWT_IDX_CUT(16384); break;
+    default: BWT_IDX_CUT(32768); break;
+  }
+#undef BWT_IDX_CUT
+  if ((e = hipGetLastError()) != hipSuccess) return int(e);
+  // pointer doubling: after k rounds every e^i(0), i < 2^k, is marked
+  for (int64_t reach = 1; reach <= n; reach <<= 1) {
+    hipLaunchKernelGGL(idx_cut_mark_kernel, gn, blk, 0, st, n, (const int32_t*)J, mark);
+    hipLaunchKernelGGL(idx_cut_jump_kernel, gn, blk, 0, st, n, (const int32_t*)J, J2);
+    std::swap(J, J2);
+  }
+  hipLaunchKernelGGL(idx_cut_mark_kernel, gn, blk, 0, st, n, (const int32_t*)J, mark);
+  size_t tb = L.temp_bytes;
+  if ((e = rocprim::exclusive_scan(at<void>(work, L.temp), tb, (const int32_t*)mark, pos, (int32_t)0, (size_t)n,
+                                   rocprim::plus<int32_t>(), st)) != hipSuccess)
+    return int(e);
+  hipLaunchKernelGGL(idx_cut_scatter_kernel, dim3((unsigned)((n + kT - 1) / kT)), blk, 0, st, n, (const int32_t*)mark,
+                     (const int32_t*)pos, blk_step0, nblk);
+  return int(hipGetLastError());
+}
 
 extern "C" int bwt_index_workspace(const BwtIndexBuild* in, size_t* bytes) {
   if (!build_ok(in) || !bytes) return int(hipErrorInvalidValue);

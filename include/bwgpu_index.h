@@ -1,13 +1,13 @@
-/* bwgpu_index.h -- C-ABI of the device index build: from the Euler walk of the version tree and
- * its cut into blocks (both made on the host: a walk and a greedy cut are sequential), the
- * per-block and per-chain tables of BwtIndex / BwtPlanIndex (include/bwgpu_tree.h,
- * include/bwgpu_plan.h) on the GPU.
+/* bwgpu_index.h -- C-ABI of the device index build: from the Euler walk of the version tree (made
+ * on the host), its cut into blocks and the per-block and per-chain tables of BwtIndex /
+ * BwtPlanIndex (include/bwgpu_tree.h, include/bwgpu_plan.h) on the GPU.
  *
  * Counterpart in the reference: the host plan build BatchEngineV7::build -- per-row chains by
  * radix sort, supports, base scans (src/batch_engine_v7.hpp:576-704,1069-1415) -- which the
  * reference GPU drivers run on the CPU before the launch (src/panel_batch_gpu.cpp:86-121).  Here
  * the tables are those of the Euler-walk index; they equal the host build (plan_block_tables in
  * csrc/host/tree_plan.hpp) array for array:
+ *   stage 0 (bwt_index_cut_launch)     the greedy block cut (below), or the host's cut;
  *   stage 1 (bwt_index_keys_launch)    one workgroup per block: its events' keys through an LDS
  *            hash set, the distinct keys sorted in LDS (bitonic), every event's rank by binary
  *            search; the block of every step, the enter step of every state; per block its
@@ -68,6 +68,17 @@ int bwt_index_keys_launch(const BwtIndexBuild* in, BwtIndexTables* t, int32_t* n
  * scans of nkeys / ntiles, whose totals are nkeys_total / nslots. */
 int bwt_index_chains_launch(const BwtIndexBuild* in, BwtIndexTables* t, int64_t nkeys_total, int64_t nslots,
                             void* work, size_t work_bytes, void* stream);
+
+/* The greedy block cut on the device (stage 0): the cut the host makes step by step -- a block
+ * takes steps while the distinct keys of its steps stay within rmax and its step count within
+ * max_block_steps -- found in parallel: for every step s, one workgroup computes e(s), the end of
+ * a block that would start at s (an LDS hash set of the keys, step by step); the blocks are then
+ * the path 0, e(0), e(e(0)), ..., marked by pointer doubling (log2(blocks) rounds) and
+ * compacted.  Uses in->steps, in->key, in->U, in->rmax, in->nsteps.  Writes blk_step0 (capacity
+ * nsteps + 1; ends with nsteps) and *nblk (device int32). */
+int bwt_index_cut_workspace(const BwtIndexBuild* in, size_t* bytes);
+int bwt_index_cut_launch(const BwtIndexBuild* in, int32_t max_block_steps, int32_t* blk_step0, int32_t* nblk,
+                         void* work, size_t work_bytes, void* stream);
 
 #ifdef __cplusplus
 }

@@ -73,9 +73,10 @@ def test_device_plan_equals_host_plan(pkg, case, knobs):
     ("S4_batched_full", ["--tuned", None]),
 ])
 def test_device_index_equals_host_index(pkg, case, knobs):
-    """The index tables built on the device (per-block keys and ranks by LDS hash set and bitonic
-    sort, chains by a stable radix sort of the tile slots) equal the host-built tables array for
-    array; the stream planned and answered over them gives the oracle's answers."""
+    """The index built on the device (the greedy block cut by per-step block ends and pointer
+    doubling, per-block keys and ranks by LDS hash set and bitonic sort, chains by a stable radix
+    sort of the tile slots) equals the host-built index array for array; the stream planned and
+    answered over it gives the oracle's answers."""
     g = STREAMS[case]
     knobs = [pkg.TUNED if k is None else k for k in knobs]
     r = run_json(_cmd(pkg, g, knobs + ["--plain-sample", "0", "--device-index"]), timeout=900)
