@@ -4,6 +4,7 @@ product's plans; the SHA-256 must equal the oracle's.  Knob variants cut blocks,
 work items and slot ranges at odd sizes and make many slots pair their level-B endpoints; a
 sampled batch (the planner's sorted path) must reproduce the full batch's answers element for
 element."""
+import json
 import os
 import subprocess
 
@@ -58,3 +59,30 @@ def test_sampled_batch_matches_full_batch(emulator, tmp_path):
     b = np.fromfile(part, dtype="<i8")
     assert len(b) == (len(a) + 6) // 7
     assert np.array_equal(a[::7], b)
+
+
+@pytest.fixture(scope="module")
+def cut_emulator(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("cut") / "cut_emulator")
+    host = os.path.join(REPO, "biworlddb-artifact_amd", "csrc", "host")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-fopenmp", "-Wno-deprecated-declarations",
+                    "-I" + os.path.join(REPO, "include"), "-I" + host, "-o", out,
+                    os.path.join(REPO, "tests", "host", "cut_emulator.cpp"), os.path.join(host, "tree_plan.cpp"),
+                    os.path.join(host, "tree_workload.cpp"), "-lcrypto"], check=True, capture_output=True, timeout=300)
+    return out
+
+
+@pytest.mark.parametrize("args", [
+    ["1", "500", "64", "65536", "5", "128"],
+    ["2", "3000", "16", "1048576", "7", "64"],
+    ["4", "3000", "32", "1048576", "9", "256", "5"],   # max_block_steps cuts too
+    ["1", "2000", "8", "4096", "3", "17"],
+])
+def test_parallel_block_cut_equals_greedy_cut(cut_emulator, args):
+    """The device cut's algorithm (every step's block end, then the block path by pointer doubling)
+    gives the host's sequential greedy cut."""
+    p = subprocess.run([cut_emulator] + args, capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stderr
+    r = json.loads(p.stdout)
+    assert r["equal"] and r["blocks"] > 1, r
+
