@@ -133,7 +133,7 @@ def main():
                                                                                   r["graph_ms"]["avg"]) * 1e-3), 1)},
         "host_batch_plan_ms": r["phases_ms"]["batch_plan"],
         "device_index": {"total_ms": r["device_index"]["total_ms"], "device_ms": r["device_index"]["gpu_ms"],
-                         "host_walk_ms": r["device_index"]["walk_ms"],
+                         "host_checks_ms": r["device_index"]["host_ms"],
                          "host_index_ms": r["device_index"]["host_plan_ms"],
                          "equals_host_index": r["device_index"]["mismatch"] == 0},
         "device_gen": {"gpu_ms": r["device_gen"]["gpu_ms"], "wall_ms": r["device_gen"]["wall_ms"],

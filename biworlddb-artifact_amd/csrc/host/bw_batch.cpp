@@ -20,7 +20,7 @@
 // stream's queries with the device generator (include/bwgpu_gen.h), compares them element for
 // element with the host generator's, and feeds the device arrays to the planner and the replays.
 // --device-index (implies --device-plan) also builds the index on the device (include/bwgpu_index.h:
-// the block cut and every table; the Euler walk stays on the host), compares it array for array
+// the Euler walk, the block cut and every table; the host only checks the inputs), compares it array for array
 // with the host-built index, and answers the stream once more over it (device plan + query
 // sequence); the answers must be the same.
 // Exit codes: 2 usage, 3 unsupported input, 4 I/O, 5 GPU error.
@@ -419,7 +419,7 @@ int main(int argc, char** argv) {
   if (dev_index) {
     TreeEngine ED;
     if (!ED.init_device_index(spec.S, spec.U, w.coord_n(), w.parent.data(), w.key.data(), nullptr, w.delta.data(),
-                              kn, L, st, err, sizeof(err), true)) {
+                              kn, L, st, err, sizeof(err), true, true)) {
       std::fprintf(stderr, "bw_batch: %s\n", err);
       return 5;
     }
@@ -434,7 +434,7 @@ int main(int argc, char** argv) {
     };
     const TreeIndexPlan& H = E.plan;
     const size_t nst = H.steps.size(), nbk = size_t(H.nblk()), KT = size_t(H.KT);
-    if (ED.plan.blk_step0 != H.blk_step0 || ED.device_index.nkeys != int64_t(H.bkey.size()) ||
+    if (ED.plan.steps != H.steps || ED.plan.blk_step0 != H.blk_step0 || ED.device_index.nkeys != int64_t(H.bkey.size()) ||
         ED.device_index.nslots != int64_t(H.tile_rank0.size()) || ED.plan.chunk_key0 != H.chunk_key0 ||
         ED.plan.chunk_cp0 != H.chunk_cp0 || ED.plan.seg_chunk0 != H.seg_chunk0) {
       std::fprintf(stderr, "bw_batch: device index shape differs\n");
@@ -466,7 +466,7 @@ int main(int argc, char** argv) {
       ++didx_mismatch;
     }
     std::snprintf(didx_json, sizeof(didx_json),
-                  "{\"walk_ms\":%.3f,\"gpu_ms\":%.3f,\"total_ms\":%.3f,\"host_plan_ms\":%.3f,"
+                  "{\"host_ms\":%.3f,\"gpu_ms\":%.3f,\"total_ms\":%.3f,\"host_plan_ms\":%.3f,"
                   "\"nkeys\":%" PRId64 ",\"nslots\":%" PRId64 ",\"answers_sha256\":\"%s\",\"mismatch\":%" PRId64 "}",
                   ED.device_index.host_ms, ED.device_index.gpu_ms, ED.device_index.total_ms, host_index_ms,
                   ED.device_index.nkeys, ED.device_index.nslots, dx.sha256_hex, didx_mismatch);

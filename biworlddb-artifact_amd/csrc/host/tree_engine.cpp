@@ -129,18 +129,46 @@ bool TreeEngine::init(int64_t S, int U, uint64_t NK, const int32_t* parent, cons
 
 bool TreeEngine::init_device_index(int64_t S, int U, uint64_t NK, const int32_t* parent, const uint32_t* key,
                                    const uint32_t* d_key, const int64_t* delta, const PlanKnobs& knobs,
-                                   const BwtLaunch& l, hipStream_t st, char* err, size_t errlen, bool device_cut) {
+                                   const BwtLaunch& l, hipStream_t st, char* err, size_t errlen, bool device_cut,
+                                   bool device_walk) {
   auto fail = [&](const char* what, int rc) {
     if (err && errlen) std::snprintf(err, errlen, "device index: %s (%d)", what, rc);
     return false;
   };
   const auto t0 = std::chrono::steady_clock::now();
-  if (!plan_walk_blocks(S, U, NK, parent, key, knobs, plan, err, errlen, !device_cut)) return false;
+  device_cut = device_cut || device_walk;
+  if (!plan_walk_blocks(S, U, NK, parent, key, knobs, plan, err, errlen, !device_cut, !device_walk)) return false;
   const auto t1 = std::chrono::steady_clock::now();
   launch = l;
   TreeIndexPlan& P = plan;
-  const int64_t nsteps = int64_t(P.steps.size());
   DeviceArena tmp;  // the build's workspace and counts, freed at the end
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return fail("events", 0);
+  (void)hipEventRecord(e0, st);
+  int rc;
+  x = BwtIndex{};
+  if (device_walk) {  // the Euler walk on the device, then the walk to the host
+    size_t wwb = 0;
+    if ((rc = bwt_index_walk_workspace(S, &wwb))) return fail("walk workspace", rc);
+    void* wwork = tmp.alloc(wwb);
+    const int32_t* d_parent = tmp.upload(parent, size_t(S) + 1);
+    int32_t* d_steps = arena.scratch<int32_t>(2 * size_t(S));
+    int64_t* d_ns = tmp.scratch<int64_t>(1);
+    if (!arena.ok() || !tmp.ok()) return fail("allocation", int(arena.ok() ? tmp.err : arena.err));
+    if ((rc = bwt_index_walk_launch(S, d_parent, d_steps, d_ns, wwork, wwb, st))) return fail("walk", rc);
+    int64_t ns = 0;
+    if (hipMemcpyAsync(&ns, d_ns, 8, hipMemcpyDeviceToHost, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
+      return fail("walk length", 0);
+    if (ns < 1 || ns > 2 * S || ns >= (int64_t(1) << 31) / U) return fail("walk length", int(ns));
+    P.steps.assign(size_t(ns), 0);
+    if (hipMemcpyAsync(P.steps.data(), d_steps, size_t(ns) * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return fail("walk download", 0);
+    x.steps = d_steps;
+  } else {
+    x.steps = arena.upload(P.steps);
+  }
+  const int64_t nsteps = int64_t(P.steps.size());
   BwtIndexBuild in{};
   in.S = S;
   in.U = U;
@@ -148,14 +176,8 @@ bool TreeEngine::init_device_index(int64_t S, int U, uint64_t NK, const int32_t*
   in.tile_shift = P.knobs.tile_shift;
   in.KT = P.KT;
   in.nsteps = nsteps;
-  x = BwtIndex{};
-  x.steps = arena.upload(P.steps);
   in.steps = x.steps;
   in.key = d_key ? d_key : tmp.upload(key, size_t(S) * size_t(U));
-  hipEvent_t e0, e1;
-  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return fail("events", 0);
-  (void)hipEventRecord(e0, st);
-  int rc;
   if (device_cut) {  // the block cut on the device, then its block starts to the host
     size_t cb = 0;
     if ((rc = bwt_index_cut_workspace(&in, &cb))) return fail("cut workspace", rc);

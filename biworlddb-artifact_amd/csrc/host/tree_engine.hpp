@@ -56,16 +56,17 @@ struct TreeEngine {
   bool init(int64_t S, int U, uint64_t NK, const int32_t* parent, const uint32_t* key, const int64_t* delta,
             const PlanKnobs& knobs, const BwtLaunch& l, char* err, size_t errlen);
   // The same engine with the index tables built on the device (include/bwgpu_index.h): the walk
-  // (and, unless device_cut, the block cut) on the host, then the per-block tables and the
+  // and the block cut on the host (unless device_walk / device_cut), then the per-block tables and the
   // chains on the GPU from d_key (device keys; NULL: uploaded from key), then the chunk tables on
   // the host.  The host plan then holds the walk, blocks, offsets and chunk tables only (no bkey,
   // evrank, kitems or position tables): batches are planned on the device.
-  // device_cut: the greedy block cut on the device too (bwt_index_cut_launch).
+  // device_cut: the greedy block cut on the device too (bwt_index_cut_launch); device_walk: the
+  // Euler walk too (bwt_index_walk_launch; implies device_cut) -- the host then only checks.
   bool init_device_index(int64_t S, int U, uint64_t NK, const int32_t* parent, const uint32_t* key,
                          const uint32_t* d_key, const int64_t* delta, const PlanKnobs& knobs, const BwtLaunch& l,
-                         hipStream_t st, char* err, size_t errlen, bool device_cut = false);
+                         hipStream_t st, char* err, size_t errlen, bool device_cut = false, bool device_walk = false);
   struct {
-    double host_ms = 0, gpu_ms = 0, total_ms = 0;  // walk and cut; device stages; whole build
+    double host_ms = 0, gpu_ms = 0, total_ms = 0;  // host checks (+ walk, cut); device stages; whole build
     int64_t nkeys = 0, nslots = 0;
   } device_index;
   // Block replay with the weight write over every block, chunk histograms, carry scan.

@@ -20,7 +20,7 @@ inline uint64_t ep_item(uint64_t pos, uint32_t cls, uint64_t ep) {
 }  // namespace
 
 bool plan_walk_blocks(int64_t S, int U, uint64_t NK, const int32_t* parent, const uint32_t* key,
-                      const PlanKnobs& kn, TreeIndexPlan& P, char* err, size_t errlen, bool cut) {
+                      const PlanKnobs& kn, TreeIndexPlan& P, char* err, size_t errlen, bool cut, bool walk) {
   auto fail = [&](const char* m) {
     if (err && errlen) std::snprintf(err, errlen, "%s", m);
     return false;
@@ -49,6 +49,7 @@ bool plan_walk_blocks(int64_t S, int U, uint64_t NK, const int32_t* parent, cons
 
   P = TreeIndexPlan();
   P.S = S; P.U = U; P.NK = NK; P.knobs = kn; P.knobs.tile_shift = ts; P.KT = int(KT);
+  if (!walk) return true;  // the checks only
 
   // ---- 1. children (ascending) and the Euler walk: +v on entry, -v after the subtree
   std::vector<int64_t> koff(size_t(S) + 2, 0);

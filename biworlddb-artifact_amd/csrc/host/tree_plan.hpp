@@ -86,12 +86,13 @@ struct BatchPlan {
 // parent: S+1 entries (parent[0] = -1, parent[v] < v); key: S*U coordinates in [0, NK).
 bool build_tree_index(int64_t S, int U, uint64_t NK, const int32_t* parent, const uint32_t* key,
                       const PlanKnobs& knobs, TreeIndexPlan& plan, char* err, size_t errlen);
-// Its three stages: (1) checks, tile width, the Euler walk and (cut = true) the greedy block cut
-// (steps, blk_step0, KT); (2) per block its sorted distinct keys, tiles and event ranks, the chains and the
+// Its three stages: (1) checks and tile width, then (walk = true) the Euler walk and (cut = true)
+// the greedy block cut (steps, blk_step0, KT); (2) per block its sorted distinct keys, tiles and event ranks, the chains and the
 // level-B position tables; (3) the chunk and segment tables, from chain_key0 and blk_tile0 (and
 // the chain class counts of stage 2).  The device index build runs stage 2 on the GPU.
 bool plan_walk_blocks(int64_t S, int U, uint64_t NK, const int32_t* parent, const uint32_t* key,
-                      const PlanKnobs& knobs, TreeIndexPlan& plan, char* err, size_t errlen, bool cut = true);
+                      const PlanKnobs& knobs, TreeIndexPlan& plan, char* err, size_t errlen, bool cut = true,
+                      bool walk = true);
 bool plan_block_tables(const uint32_t* key, TreeIndexPlan& plan, char* err, size_t errlen);
 void plan_chunks(TreeIndexPlan& plan);
 

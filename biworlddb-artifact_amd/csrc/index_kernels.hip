@@ -241,6 +241,112 @@ __global__ void __launch_bounds__(kT) idx_btpos_kernel(int nblk, int KT, BwtInde
   t.bt_pos[x] = lo < e ? (int32_t)t.tb_pos[lo] : (int32_t)(t.chain_key0[tl + 2] - t.chain_key0[tl + 1]);
 }
 
+// ---- the Euler walk: children lists by a stable sort of (parent, state); every tree edge gets
+// its successor in the preorder walk (down an edge to the first child, or back up; up an edge to
+// the next sibling, or further up); list ranking by pointer jumping gives each edge its place.
+// Edge 2(c-1) enters state c, edge 2(c-1)+1 leaves it; edge 2S ends the walk.
+__global__ void __launch_bounds__(kT) walk_keys_kernel(int64_t S, const int32_t* parent, uint32_t* key,
+                                                       uint32_t* val) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= S) return;
+  key[i] = (uint32_t)parent[i + 1];
+  val[i] = (uint32_t)(i + 1);
+}
+
+__global__ void __launch_bounds__(kT) walk_koff_kernel(int64_t S, const uint32_t* sorted_parent,
+                                                       const uint32_t* kids, int64_t* koff, int32_t* pos_of) {
+  const int64_t p = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (p <= S + 1) {  // koff[p]: first child slot of state p (koff[S+1] = S)
+    int64_t lo = 0, hi = S;
+    while (lo < hi) {
+      const int64_t mid = lo + (hi - lo) / 2;
+      if ((int64_t)sorted_parent[mid] < p) lo = mid + 1;
+      else hi = mid;
+    }
+    koff[p] = lo;
+  }
+  if (p < S) pos_of[kids[p]] = (int32_t)p;
+}
+
+__global__ void __launch_bounds__(kT) walk_succ_kernel(int64_t S, const int32_t* parent, const uint32_t* kids,
+                                                       const int64_t* koff, const int32_t* pos_of, int64_t* nxt,
+                                                       int64_t* d) {
+  const int64_t c = (int64_t)blockIdx.x * kT + threadIdx.x + 1;
+  if (c > S + 1) return;
+  const int64_t END = 2 * S;
+  if (c == S + 1) {
+    nxt[END] = END;
+    d[END] = 0;
+    return;
+  }
+  const int64_t dn = 2 * (c - 1), up = dn + 1;
+  const int64_t p = parent[c];
+  const int64_t i = pos_of[c];
+  nxt[dn] = koff[c] < koff[c + 1] ? 2 * ((int64_t)kids[koff[c]] - 1) : up;
+  nxt[up] = i + 1 < koff[p + 1] ? 2 * ((int64_t)kids[i + 1] - 1) : (p == 0 ? END : 2 * (p - 1) + 1);
+  d[dn] = 1;
+  d[up] = 1;
+}
+
+__global__ void __launch_bounds__(kT) walk_rank_kernel(int64_t n, const int64_t* nxt, const int64_t* d, int64_t* nxt2,
+                                                       int64_t* d2) {
+  const int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (e > n) return;
+  const int64_t x = nxt[e];
+  d2[e] = d[e] + d[x];
+  nxt2[e] = nxt[x];
+}
+
+// place every edge at 2S - (its distance to the end); the walk stops after the last entry
+__global__ void __launch_bounds__(kT) walk_place_kernel(int64_t S, const int64_t* d, int32_t* steps,
+                                                        unsigned long long* last) {
+  const int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (e >= 2 * S) return;
+  const int64_t pos = 2 * S - d[e];
+  const int32_t c = (int32_t)(e / 2 + 1);
+  steps[pos] = (e & 1) ? -c : c;
+  if (!(e & 1)) atomicMax(last, (unsigned long long)pos);
+}
+
+__global__ void __launch_bounds__(kT) walk_zero_kernel(unsigned long long* last) { *last = 0; }
+__global__ void __launch_bounds__(kT) walk_nsteps_kernel(const unsigned long long* last, int64_t* nsteps) {
+  *nsteps = (int64_t)*last + 1;
+}
+
+struct WalkLayout {
+  size_t key, key_s, val, kids, koff, pos_of, nxt, d, nxt2, d2, last, temp, temp_bytes, total;
+};
+
+hipError_t walk_layout_of(int64_t S, WalkLayout& L) {
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = off;
+    off += (bytes + 255) & ~size_t(255);
+    return at;
+  };
+  const size_t n = (size_t)S, ne = 2 * (size_t)S + 1;
+  L.key = take(n * 4);
+  L.key_s = take(n * 4);
+  L.val = take(n * 4);
+  L.kids = take(n * 4);
+  L.koff = take((n + 2) * 8);
+  L.pos_of = take((n + 1) * 4);
+  L.nxt = take(ne * 8);
+  L.d = take(ne * 8);
+  L.nxt2 = take(ne * 8);
+  L.d2 = take(ne * 8);
+  L.last = take(8);
+  size_t b = 0;
+  const hipError_t e = rocprim::radix_sort_pairs(nullptr, b, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                                 (const uint32_t*)nullptr, (uint32_t*)nullptr, n, 0u,
+                                                 bits_for((uint64_t)S));
+  if (e != hipSuccess) return e;
+  L.temp_bytes = std::max<size_t>(b, 1);
+  L.temp = take(L.temp_bytes);
+  L.total = off;
+  return hipSuccess;
+}
+
 // ---- stage 0: the block cut.  e(s): where a block starting at step s would end -- the first
 // step t > s whose keys take the block's distinct keys past rmax, or at max_block_steps.
 constexpr int kCutT = 128;
@@ -394,6 +500,60 @@ T* at(void* base, size_t off) {
 }
 
 }  // namespace
+
+extern "C" int bwt_index_walk_workspace(int64_t S, size_t* bytes) {
+  if (S < 1 || S >= (int64_t(1) << 30) || !bytes) return int(hipErrorInvalidValue);
+  WalkLayout L;
+  const hipError_t e = walk_layout_of(S, L);
+  if (e != hipSuccess) return int(e);
+  *bytes = L.total;
+  return 0;
+}
+
+extern "C" int bwt_index_walk_launch(int64_t S, const int32_t* parent, int32_t* steps, int64_t* nsteps, void* work,
+                                     size_t work_bytes, void* stream) {
+  if (S < 1 || S >= (int64_t(1) << 30) || !parent || !steps || !nsteps) return int(hipErrorInvalidValue);
+  WalkLayout L;
+  hipError_t e = walk_layout_of(S, L);
+  if (e != hipSuccess) return int(e);
+  if (!work || work_bytes < L.total) return int(hipErrorInvalidValue);
+  hipStream_t st = hipStream_t(stream);
+  uint32_t* key = at<uint32_t>(work, L.key);
+  uint32_t* key_s = at<uint32_t>(work, L.key_s);
+  uint32_t* val = at<uint32_t>(work, L.val);
+  uint32_t* kids = at<uint32_t>(work, L.kids);
+  int64_t* koff = at<int64_t>(work, L.koff);
+  int32_t* pos_of = at<int32_t>(work, L.pos_of);
+  int64_t* nxt = at<int64_t>(work, L.nxt);
+  int64_t* d = at<int64_t>(work, L.d);
+  int64_t* nxt2 = at<int64_t>(work, L.nxt2);
+  int64_t* d2 = at<int64_t>(work, L.d2);
+  unsigned long long* last = at<unsigned long long>(work, L.last);
+  const dim3 blk(kT);
+  hipLaunchKernelGGL(walk_keys_kernel, dim3((unsigned)((S + kT - 1) / kT)), blk, 0, st, S, parent, key, val);
+  size_t tb = L.temp_bytes;
+  if ((e = rocprim::radix_sort_pairs(at<void>(work, L.temp), tb, key, key_s, val, kids, (size_t)S, 0u,
+                                     bits_for((uint64_t)S), st)) != hipSuccess)
+    return int(e);
+  hipLaunchKernelGGL(walk_koff_kernel, dim3((unsigned)((S + 2 + kT - 1) / kT)), blk, 0, st, S, (const uint32_t*)key_s,
+                     (const uint32_t*)kids, koff, pos_of);
+  hipLaunchKernelGGL(walk_succ_kernel, dim3((unsigned)((S + 1 + kT - 1) / kT)), blk, 0, st, S, parent,
+                     (const uint32_t*)kids, (const int64_t*)koff, (const int32_t*)pos_of, nxt, d);
+  const int64_t n = 2 * S;  // edges 0..2S-1 and the end 2S
+  const dim3 ge((unsigned)((n + 1 + kT - 1) / kT));
+  for (int64_t reach = 1; reach <= n; reach <<= 1) {
+    hipLaunchKernelGGL(walk_rank_kernel, ge, blk, 0, st, n, (const int64_t*)nxt, (const int64_t*)d, nxt2, d2);
+    std::swap(nxt, nxt2);
+    std::swap(d, d2);
+  }
+  hipLaunchKernelGGL(walk_zero_kernel, dim3(1), dim3(1), 0, st, last);
+  hipLaunchKernelGGL(walk_place_kernel, dim3((unsigned)((n + kT - 1) / kT)), blk, 0, st, S, (const int64_t*)d, steps,
+                     last);
+  if ((e = hipGetLastError()) != hipSuccess) return int(e);
+  // nsteps = last entry + 1
+  hipLaunchKernelGGL(walk_nsteps_kernel, dim3(1), dim3(1), 0, st, (const unsigned long long*)last, nsteps);
+  return int(hipGetLastError());
+}
 
 extern "C" int bwt_index_cut_workspace(const BwtIndexBuild* in, size_t* bytes) {
   if (!in || !bytes || in->nsteps < 1 || in->nsteps >= (int64_t(1) << 31) - 1) return int(hipErrorInvalidValue);

@@ -1,6 +1,6 @@
-/* bwgpu_index.h -- C-ABI of the device index build: from the Euler walk of the version tree (made
- * on the host), its cut into blocks and the per-block and per-chain tables of BwtIndex /
- * BwtPlanIndex (include/bwgpu_tree.h, include/bwgpu_plan.h) on the GPU.
+/* bwgpu_index.h -- C-ABI of the device index build: the Euler walk of the version tree, its cut
+ * into blocks and the per-block and per-chain tables of BwtIndex / BwtPlanIndex
+ * (include/bwgpu_tree.h, include/bwgpu_plan.h), on the GPU from the parent array and the keys.
  *
  * Counterpart in the reference: the host plan build BatchEngineV7::build -- per-row chains by
  * radix sort, supports, base scans (src/batch_engine_v7.hpp:576-704,1069-1415) -- which the
@@ -68,6 +68,16 @@ int bwt_index_keys_launch(const BwtIndexBuild* in, BwtIndexTables* t, int32_t* n
  * scans of nkeys / ntiles, whose totals are nkeys_total / nslots. */
 int bwt_index_chains_launch(const BwtIndexBuild* in, BwtIndexTables* t, int64_t nkeys_total, int64_t nslots,
                             void* work, size_t work_bytes, void* stream);
+
+/* The Euler walk on the device: children in ascending order by a stable radix sort of (parent,
+ * state); every tree edge's successor in the preorder walk (down to the first child or back up;
+ * from a child up to its next sibling or further up); the walk's order by list ranking (pointer
+ * jumping over the 2S edges, log2(2S) rounds).  parent: device, S+1 entries, parent[v] in [0, v)
+ * (the caller checks).  Writes steps (capacity 2S; +v enters v, -v leaves it; the walk ends with
+ * the last entry, as the host walk does) and *nsteps (device int64). */
+int bwt_index_walk_workspace(int64_t S, size_t* bytes);
+int bwt_index_walk_launch(int64_t S, const int32_t* parent, int32_t* steps, int64_t* nsteps, void* work,
+                          size_t work_bytes, void* stream);
 
 /* The greedy block cut on the device (stage 0): the cut the host makes step by step -- a block
  * takes steps while the distinct keys of its steps stay within rmax and its step count within

@@ -1,8 +1,12 @@
 // cut_emulator.cpp -- test infrastructure: the device block cut's algorithm (index_kernels.hip,
 // stage 0) restated on the CPU -- e(s), the end of a block starting at step s (distinct keys of its
 // steps past rmax, or max_block_steps), for every step; the path 0, e(0), e(e(0)), ... marked by
-// pointer doubling rounds as the kernels run them -- against the host's sequential greedy cut.
-//   cut_emulator CFG S U N SEED RMAX [MAX_BLOCK_STEPS]   prints {"blocks": n, "equal": true|false}
+// pointer doubling rounds as the kernels run them -- against the host's sequential greedy cut;
+// and the device Euler walk's algorithm (edge successors, list ranking by pointer jumping) against
+// the host's DFS walk.
+//   cut_emulator CFG S U N SEED RMAX [MAX_BLOCK_STEPS]
+//   prints {"blocks": n, "equal": true|false, "walk_equal": true|false}
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <unordered_set>
@@ -59,6 +63,43 @@ int main(int argc, char** argv) {
   for (int64_t s = 0; s < n; ++s)
     if (mark[size_t(s)]) b0.push_back(int32_t(s));
   b0.push_back(int32_t(n));
-  std::printf("{\"blocks\": %zu, \"equal\": %s}\n", b0.size() - 1, b0 == P.blk_step0 ? "true" : "false");
+  // the device walk's algorithm: children by a stable sort on the parent, every edge's successor,
+  // list ranking by pointer-jumping rounds, edges placed at 2S - distance, cut after the last entry
+  const int64_t S = spec.S;
+  std::vector<int64_t> order(static_cast<size_t>(S));
+  for (int64_t i = 0; i < S; ++i) order[size_t(i)] = i + 1;
+  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return w.parent[size_t(a)] < w.parent[size_t(b)]; });
+  std::vector<int64_t> koff(size_t(S) + 2, 0), pos_of(size_t(S) + 1, 0);
+  for (int64_t p = 0; p <= S + 1; ++p)
+    koff[size_t(p)] = std::lower_bound(order.begin(), order.end(), p, [&](int64_t c, int64_t x) { return w.parent[size_t(c)] < x; }) - order.begin();
+  for (int64_t i = 0; i < S; ++i) pos_of[size_t(order[size_t(i)])] = i;
+  const int64_t END = 2 * S;
+  std::vector<int64_t> nxt(size_t(END) + 1), d(size_t(END) + 1), nxt2(nxt.size()), d2(d.size());
+  nxt[size_t(END)] = END;
+  d[size_t(END)] = 0;
+  for (int64_t c = 1; c <= S; ++c) {
+    const int64_t dn = 2 * (c - 1), up = dn + 1, p = w.parent[size_t(c)], i = pos_of[size_t(c)];
+    nxt[size_t(dn)] = koff[size_t(c)] < koff[size_t(c) + 1] ? 2 * (order[size_t(koff[size_t(c)])] - 1) : up;
+    nxt[size_t(up)] = i + 1 < koff[size_t(p) + 1] ? 2 * (order[size_t(i) + 1] - 1) : (p == 0 ? END : 2 * (p - 1) + 1);
+    d[size_t(dn)] = d[size_t(up)] = 1;
+  }
+  for (int64_t reach = 1; reach <= END; reach <<= 1) {
+    for (int64_t e = 0; e <= END; ++e) {
+      d2[size_t(e)] = d[size_t(e)] + d[size_t(nxt[size_t(e)])];
+      nxt2[size_t(e)] = nxt[size_t(nxt[size_t(e)])];
+    }
+    d.swap(d2);
+    nxt.swap(nxt2);
+  }
+  std::vector<int32_t> walk(static_cast<size_t>(END));
+  int64_t last = 0;
+  for (int64_t e = 0; e < END; ++e) {
+    const int64_t pos = END - d[size_t(e)];
+    walk[size_t(pos)] = (e & 1) ? -int32_t(e / 2 + 1) : int32_t(e / 2 + 1);
+    if (!(e & 1)) last = std::max(last, pos);
+  }
+  walk.resize(size_t(last) + 1);
+  std::printf("{\"blocks\": %zu, \"equal\": %s, \"walk_equal\": %s}\n", b0.size() - 1,
+              b0 == P.blk_step0 ? "true" : "false", walk == P.steps ? "true" : "false");
   return 0;
 }

@@ -80,9 +80,10 @@ def cut_emulator(tmp_path_factory):
 ])
 def test_parallel_block_cut_equals_greedy_cut(cut_emulator, args):
     """The device cut's algorithm (every step's block end, then the block path by pointer doubling)
-    gives the host's sequential greedy cut."""
+    gives the host's sequential greedy cut, and the device walk's (edge successors ranked by
+    pointer jumping) the host's DFS walk."""
     p = subprocess.run([cut_emulator] + args, capture_output=True, text=True, timeout=300)
     assert p.returncode == 0, p.stderr
     r = json.loads(p.stdout)
-    assert r["equal"] and r["blocks"] > 1, r
+    assert r["equal"] and r["walk_equal"] and r["blocks"] > 1, r
 
