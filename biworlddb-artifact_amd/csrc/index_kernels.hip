@@ -7,7 +7,7 @@
 //                                block, key count) and their level-C items
 //            radix sort          slots by tile, stable (rocPRIM), then their key counts scanned
 //            idx_tiles_kernel    chain_key0 and tile_tb0 by binary search over the sorted tiles
-//            idx_items_kernel    one wavefront per sorted slot: its level-B items, tb_block, tb_pos
+//            idx_items_kernel    one thread per sorted slot: its level-B items, tb_block, tb_pos
 //            idx_btpos_kernel    the dense (block, tile) table of level-B positions
 // Everything is exact integer work; the results equal the host build array for array.
 #include <hip/hip_runtime.h>
@@ -204,13 +204,12 @@ __global__ void __launch_bounds__(kT) idx_tiles_kernel(int KT, int64_t nslots, i
   if (t == 0) chain_key0[0] = 0;
 }
 
-// one wavefront per sorted slot: its keys' level-B items, and its tb_block / tb_pos
+// one thread per sorted slot (a slot holds a few keys): its keys' level-B items, tb_block, tb_pos
 __global__ void __launch_bounds__(kT) idx_items_kernel(const BwtIndexBuild in, BwtIndexTables t, int64_t nslots,
                                                        int64_t nC, const uint32_t* sorted_tile,
                                                        const uint32_t* sorted_id, const int32_t* slot_block,
                                                        const int64_t* n_sorted, const int64_t* start) {
-  const int lane = threadIdx.x & 63;
-  const int64_t i = (int64_t)blockIdx.x * (kT / 64) + (threadIdx.x >> 6);
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (i >= nslots) return;
   const uint32_t tile = sorted_tile[i];
   const uint32_t slot = sorted_id[i];
@@ -218,11 +217,9 @@ __global__ void __launch_bounds__(kT) idx_items_kernel(const BwtIndexBuild in, B
   const int64_t r0 = t.blk_key0[b] + t.tile_rank0[slot];
   const int64_t n = n_sorted[i], pos0 = nC + start[i];
   const uint32_t mask = (1u << in.tile_shift) - 1;
-  for (int64_t j = lane; j < n; j += 64) t.kitems[pos0 + j] = key_item(t.bkey[r0 + j] & mask, (uint64_t)(r0 + j));
-  if (lane == 0) {
-    t.tb_block[i] = b;
-    t.tb_pos[i] = start[i] - (t.chain_key0[1 + tile] - nC);
-  }
+  for (int64_t j = 0; j < n; ++j) t.kitems[pos0 + j] = key_item(t.bkey[r0 + j] & mask, (uint64_t)(r0 + j));
+  t.tb_block[i] = b;
+  t.tb_pos[i] = start[i] - (t.chain_key0[1 + tile] - nC);
 }
 
 // bt_pos[b*KT + t]: the level-B position of block b's first key of tile t (or of the next block
@@ -693,7 +690,7 @@ extern "C" int bwt_index_chains_launch(const BwtIndexBuild* in, BwtIndexTables* 
   hipLaunchKernelGGL(idx_tiles_kernel, dim3((unsigned)((in->KT + 1 + kT - 1) / kT)), dim3(kT), 0, st, in->KT, nslots,
                      nC, (const uint32_t*)slot_tile_s, (const int64_t*)start, (const int64_t*)n_sorted,
                      t->chain_key0, t->tile_tb0);
-  hipLaunchKernelGGL(idx_items_kernel, dim3((unsigned)((nslots + kT / 64 - 1) / (kT / 64))), dim3(kT), 0, st, *in, *t,
+  hipLaunchKernelGGL(idx_items_kernel, dim3((unsigned)((nslots + kT - 1) / kT)), dim3(kT), 0, st, *in, *t,
                      nslots, nC, (const uint32_t*)slot_tile_s, (const uint32_t*)slot_id_s,
                      (const int32_t*)slot_block, (const int64_t*)n_sorted, (const int64_t*)start);
   if (t->bt_pos) {
