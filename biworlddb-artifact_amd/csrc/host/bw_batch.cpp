@@ -5,7 +5,7 @@
 //   bw_batch [--S n --U n --N n --seed n] [--requests 100000 --max-batch 2048]
 //            [--rmax --tile-shift --chunk --ep-per-work --range-slots --seg --threads --waves] [--tuned FILE]
 //            [--warmup 1 --repeat 3] [--plain-sample 8] [--stages] [--answers FILE] [--json FILE]
-//            [--stream-sha] [--device-plan] [--device-gen] [--device-index]
+//            [--stream-sha] [--device-plan] [--device-gen] [--device-index] [--device-all]
 //
 // The index (block weights, carry rows) is built once on the device.  Every batch of the stream
 // is then answered by one launch sequence (block replay, chunk replay, combine) captured in a
@@ -22,7 +22,9 @@
 // --device-index (implies --device-plan) also builds the index on the device (include/bwgpu_index.h:
 // the Euler walk, the block cut and every table; the host only checks the inputs), compares it array for array
 // with the host-built index, and answers the stream once more over it (device plan + query
-// sequence); the answers must be the same.
+// sequence); the answers must be the same.  --device-all then runs the whole pipeline on the device
+// from the seed -- updates and queries generated, index built, stream planned and answered,
+// answers hashed -- timed phase by phase; its answers must be the same too.
 // Exit codes: 2 usage, 3 unsupported input, 4 I/O, 5 GPU error.
 #include <hip/hip_runtime_api.h>
 #include <openssl/sha.h>
@@ -88,7 +90,7 @@ int main(int argc, char** argv) {
   PlanKnobs kn;
   LaunchKnobs lk;
   int warmup = 1, repeat = 3, plain_sample = 8;
-  bool want_sha = false, stages = false, dev_plan = false, dev_gen = false, dev_index = false;
+  bool want_sha = false, stages = false, dev_plan = false, dev_gen = false, dev_index = false, dev_all = false;
   int fork = 1;  // replay the block and level-B stages on two streams
   std::string answers_path, json_path, tuned_path;
   bool have_knob[8] = {false, false, false, false, false, false, false, false};
@@ -98,6 +100,7 @@ int main(int argc, char** argv) {
     if (!std::strcmp(argv[i], "--device-plan")) { dev_plan = true; continue; }
     if (!std::strcmp(argv[i], "--device-gen")) { dev_gen = true; continue; }
     if (!std::strcmp(argv[i], "--device-index")) { dev_index = dev_plan = true; continue; }
+    if (!std::strcmp(argv[i], "--device-all")) { dev_all = true; continue; }
     if (i + 1 >= argc) { std::fprintf(stderr, "bw_batch: missing value for %s\n", argv[i]); return 2; }
     const char* k = argv[i];
     const char* v = argv[++i];
@@ -472,6 +475,58 @@ int main(int argc, char** argv) {
                   ED.device_index.nkeys, ED.device_index.nslots, dx.sha256_hex, didx_mismatch);
   }
 
+  // ---- everything on the device (--device-all): the tree's updates and the stream's queries
+  // generated, the index built, the stream planned and answered, the answers hashed -- from the
+  // seed and the host's parent array (a sequential recurrence) to the answers' SHA-256
+  char dall_json[512] = "null";
+  int64_t dall_mismatch = -1;
+  if (dev_all) {
+    HIPX(hipStreamSynchronize(st));
+    const auto a0 = clk::now();
+    DeviceTree DT;
+    RequestStream scan;
+    stream_scan(w.spec, ss, scan);
+    DeviceStream DS2;
+    if (!device_generate_tree(spec, w.dkey, DT, st, err, sizeof(err)) ||
+        !device_generate_stream(w.spec, scan, DS2, st, err, sizeof(err))) {
+      std::fprintf(stderr, "bw_batch: device generation: %s\n", err);
+      return 5;
+    }
+    const auto a1 = clk::now();
+    TreeEngine EA;
+    if (!EA.init_device_index(spec.S, spec.U, w.coord_n(), w.parent.data(), nullptr, DT.key, nullptr, kn, L, st, err,
+                              sizeof(err), true, true, DT.delta)) {
+      std::fprintf(stderr, "bw_batch: %s\n", err);
+      return 5;
+    }
+    RUN(EA.build_index(st));
+    HIPX(hipStreamSynchronize(st));
+    const auto a2 = clk::now();
+    DevicePlanSlot slot2;
+    if (!alloc_plan_slot(EA, DS2.nq, slot2, err, sizeof(err))) {
+      std::fprintf(stderr, "bw_batch: %s\n", err);
+      return 5;
+    }
+    HIPX(hipDeviceSynchronize());
+    const auto a3 = clk::now();
+    RUN(device_plan(EA, slot2, DS2.nq, DS2.qstate, DS2.qlo, DS2.qhi, st));
+    RUN(bwt_query_launch(&EA.x, &slot2.b, &L, st));
+    HIPX(hipStreamSynchronize(st));
+    const auto a4 = clk::now();
+    BwExtractResult ax{};
+    const int arc = bw_extract_answers(slot2.b.answers, size_t(DS2.nq), st, nullptr, nullptr, BW_EXTRACT_FNV_ON_DEVICE,
+                                       &ax);
+    if (arc) return arc < 0 ? 4 : 5;
+    const auto a5 = clk::now();
+    dall_mismatch = (DS2.nq == M && sha == ax.sha256_hex) ? 0 : 1;
+    if (dall_mismatch) std::fprintf(stderr, "bw_batch: the all-device run's answers differ\n");
+    std::snprintf(dall_json, sizeof(dall_json),
+                  "{\"generate_ms\":%.3f,\"index_ms\":%.3f,\"alloc_ms\":%.3f,\"plan_answer_ms\":%.3f,"
+                  "\"extract_ms\":%.3f,\"total_ms\":%.3f,\"answers_sha256\":\"%s\",\"mismatch\":%" PRId64 "}",
+                  ms_between(a0, a1), ms_between(a1, a2), ms_between(a2, a3), ms_between(a3, a4), ms_between(a4, a5),
+                  ms_between(a0, a5), ax.sha256_hex, dall_mismatch);
+  }
+
   // ---- plain comparison on the first batches: each batch on its own
   const int64_t K = std::min<int64_t>(plain_sample, ss.nreq);
   double plain_pass_ms = 0, plain_query_ms = 0;
@@ -533,7 +588,7 @@ int main(int argc, char** argv) {
       "\"plain\":{\"batches\":%" PRId64 ",\"queries\":%" PRId64 ",\"pass_ms\":%.4f,\"query_ms\":%.4f,"
       "\"pass_queries_per_s\":%.1f,\"query_queries_per_s\":%.1f,\"mismatch\":%" PRId64 "},"
       "\"device_plan\":{\"used\":%s,\"ms\":%.4f,\"mismatch\":%" PRId64 "},"
-      "\"device_index\":%s,\"device_gen\":{\"used\":%s,\"gpu_ms\":%.4f,\"wall_ms\":%.3f,\"host_ms\":%.3f,\"ties\":%u,"
+      "\"device_index\":%s,\"device_all\":%s,\"device_gen\":{\"used\":%s,\"gpu_ms\":%.4f,\"wall_ms\":%.3f,\"host_ms\":%.3f,\"ties\":%u,"
       "\"mismatch\":%" PRId64 "},\"forked\":%d,"
       "\"device_bytes\":%zu,\"device\":\"%s\",\"cus\":%d}\n",
       spec.S, spec.U, spec.N, spec.seed, ss.nreq, ss.max_batch, M, sha.c_str(), E.plan.nblk(), BP.qblk.size(),
@@ -545,7 +600,7 @@ int main(int argc, char** argv) {
       sb_block, sb_chunk, sb_combine, K, plain_q, plain_pass_ms,
       plain_query_ms, plain_pass_ms > 0 ? double(plain_q) / (plain_pass_ms * 1e-3) : 0.0,
       plain_query_ms > 0 ? double(plain_q) / (plain_query_ms * 1e-3) : 0.0, plain_mismatch,
-      dev_plan ? "true" : "false", dplan_ms, plan_mismatch, didx_json, dev_gen ? "true" : "false", dgen_gpu_ms, dgen_wall_ms,
+      dev_plan ? "true" : "false", dplan_ms, plan_mismatch, didx_json, dall_json, dev_gen ? "true" : "false", dgen_gpu_ms, dgen_wall_ms,
       stream_host_ms, DS.ties, dgen_mismatch, fork, E.arena.bytes + io.bytes + DB.arena.bytes + dslot.arena.bytes,
       dev, prop.multiProcessorCount);
   std::fwrite(buf, 1, size_t(n), stdout);
@@ -555,5 +610,7 @@ int main(int argc, char** argv) {
     std::fwrite(buf, 1, size_t(n), f);
     std::fclose(f);
   }
-  return plain_mismatch == 0 && plan_mismatch <= 0 && dgen_mismatch <= 0 && didx_mismatch <= 0 ? 0 : 6;
+  return plain_mismatch == 0 && plan_mismatch <= 0 && dgen_mismatch <= 0 && didx_mismatch <= 0 && dall_mismatch <= 0
+             ? 0
+             : 6;
 }

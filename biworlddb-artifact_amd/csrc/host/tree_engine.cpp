@@ -130,13 +130,14 @@ bool TreeEngine::init(int64_t S, int U, uint64_t NK, const int32_t* parent, cons
 bool TreeEngine::init_device_index(int64_t S, int U, uint64_t NK, const int32_t* parent, const uint32_t* key,
                                    const uint32_t* d_key, const int64_t* delta, const PlanKnobs& knobs,
                                    const BwtLaunch& l, hipStream_t st, char* err, size_t errlen, bool device_cut,
-                                   bool device_walk) {
+                                   bool device_walk, const int64_t* d_delta) {
   auto fail = [&](const char* what, int rc) {
     if (err && errlen) std::snprintf(err, errlen, "device index: %s (%d)", what, rc);
     return false;
   };
   const auto t0 = std::chrono::steady_clock::now();
-  device_cut = device_cut || device_walk;
+  device_cut = device_cut || device_walk || !key;  // without host keys nothing can be cut on the host
+  if (!key && !d_key) return fail("no keys", 0);
   if (!plan_walk_blocks(S, U, NK, parent, key, knobs, plan, err, errlen, !device_cut, !device_walk)) return false;
   const auto t1 = std::chrono::steady_clock::now();
   launch = l;
@@ -224,7 +225,8 @@ bool TreeEngine::init_device_index(int64_t S, int U, uint64_t NK, const int32_t*
   P.blk_key0.assign(size_t(nblk) + 1, 0);
   P.blk_tile0.assign(size_t(nblk) + 1, 0);
   for (int32_t b = 0; b < nblk; ++b) {
-    if (hk[size_t(b)] < 1 || hk[size_t(b)] > P.knobs.rmax) return fail("a block exceeds rmax", hk[size_t(b)]);
+    if (hk[size_t(b)] < 1 || hk[size_t(b)] > P.knobs.rmax)
+      return fail("a block exceeds rmax or holds a key outside the tiles", hk[size_t(b)]);
     P.blk_key0[size_t(b) + 1] = P.blk_key0[size_t(b)] + hk[size_t(b)];
     P.blk_tile0[size_t(b) + 1] = P.blk_tile0[size_t(b)] + ht[size_t(b)];
   }
@@ -270,7 +272,7 @@ bool TreeEngine::init_device_index(int64_t S, int U, uint64_t NK, const int32_t*
   x.nchunk = P.nchunk();
   x.nchunk_c = P.chain_chunk0[1];
   x.nseg = P.nseg();
-  x.delta = arena.upload(delta, size_t(S) * size_t(U));
+  x.delta = d_delta ? d_delta : arena.upload(delta, size_t(S) * size_t(U));
   x.evrank = T.evrank;
   x.bkey = T.bkey;
   x.tile_rank0 = T.tile_rank0;

@@ -62,9 +62,12 @@ struct TreeEngine {
   // evrank, kitems or position tables): batches are planned on the device.
   // device_cut: the greedy block cut on the device too (bwt_index_cut_launch); device_walk: the
   // Euler walk too (bwt_index_walk_launch; implies device_cut) -- the host then only checks.
+  // key may be NULL when d_key is given (keys only on the device; implies device_cut); d_delta:
+  // the deltas already on the device (else uploaded from delta).
   bool init_device_index(int64_t S, int U, uint64_t NK, const int32_t* parent, const uint32_t* key,
                          const uint32_t* d_key, const int64_t* delta, const PlanKnobs& knobs, const BwtLaunch& l,
-                         hipStream_t st, char* err, size_t errlen, bool device_cut = false, bool device_walk = false);
+                         hipStream_t st, char* err, size_t errlen, bool device_cut = false, bool device_walk = false,
+                         const int64_t* d_delta = nullptr);
   struct {
     double host_ms = 0, gpu_ms = 0, total_ms = 0;  // host checks (+ walk, cut); device stages; whole build
     int64_t nkeys = 0, nslots = 0;

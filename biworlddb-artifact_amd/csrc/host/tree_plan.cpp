@@ -44,8 +44,9 @@ bool plan_walk_blocks(int64_t S, int U, uint64_t NK, const int32_t* parent, cons
   for (int64_t v = 1; v <= S; ++v)
     if (parent[v] < 0 || parent[v] >= v) return fail("parent[v] must be in [0, v)");
   const size_t nu = size_t(S) * size_t(U);
-  for (size_t i = 0; i < nu; ++i)
-    if (key[i] >= NK) return fail("key coordinate out of range");
+  if (key)  // (keys held only on the device are checked there)
+    for (size_t i = 0; i < nu; ++i)
+      if (key[i] >= NK) return fail("key coordinate out of range");
 
   P = TreeIndexPlan();
   P.S = S; P.U = U; P.NK = NK; P.knobs = kn; P.knobs.tile_shift = ts; P.KT = int(KT);

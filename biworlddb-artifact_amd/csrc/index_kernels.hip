@@ -74,6 +74,10 @@ __global__ void __launch_bounds__(kT) idx_keys_kernel(const BwtIndexBuild in, ui
       if (st > 0) state_step[v] = s;
     }
     const uint32_t k = in.key[(v - 1) * (int64_t)U + i];
+    if ((k >> in.tile_shift) >= (uint32_t)in.KT) {  // a key outside the tiles: reported as an overfull block
+      atomicAdd(&cnt, kCap);
+      continue;
+    }
     uint32_t h = (k * 0x9E3779B1u) >> (32 - __builtin_ctz((unsigned)kCap));
     // at most kCap probes: a full table (more distinct keys than the cut allows) ends the insert
     for (int probe = 0; probe < kCap; ++probe) {

@@ -59,8 +59,8 @@ typedef struct BwtIndexTables {  /* device arrays; capacities in brackets */
 int bwt_index_workspace(const BwtIndexBuild* in, size_t* bytes);
 
 /* Stage 1: writes t->evrank, t->state_step, t->step_block and, per block, nkeys[b] / ntiles[b]
- * (device, nblk each).  A block with more than rmax distinct keys is an error the caller finds
- * in nkeys (the host cut never makes one). */
+ * (device, nblk each).  A block with more than rmax distinct keys (the cut never makes one) or a
+ * key whose tile is >= KT is an error the caller finds in nkeys (> rmax); stage 2 must not run. */
 int bwt_index_keys_launch(const BwtIndexBuild* in, BwtIndexTables* t, int32_t* nkeys, int32_t* ntiles, void* work,
                           size_t work_bytes, void* stream);
 

@@ -84,3 +84,20 @@ def test_device_index_equals_host_index(pkg, case, knobs):
     assert d["mismatch"] == 0 and d["answers_sha256"] == g["oracle"], (case, knobs, d)
     assert r["answers_sha256"] == g["oracle"]
 
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case,knobs", [
+    ("s1_batched_small", []),
+    ("s2_batched_mid", ["--range-slots", "65536"]),
+    ("S4_batched_full", ["--tuned", None]),
+])
+def test_all_device_pipeline(pkg, case, knobs):
+    """From the seed to the answers on the device: the tree's updates and the stream's queries
+    generated, the index built (walk, cut, tables), the stream planned and answered, the answers
+    hashed -- the oracle's answers."""
+    g = STREAMS[case]
+    knobs = [pkg.TUNED if k is None else k for k in knobs]
+    r = run_json(_cmd(pkg, g, knobs + ["--plain-sample", "0", "--device-all"]), timeout=900)
+    d = r["device_all"]
+    assert d["mismatch"] == 0 and d["answers_sha256"] == g["oracle"], (case, d)
+
