@@ -12,7 +12,8 @@ suite pins it against the oracle).
 The stream's queries are materialised on the GPU (bwt_gen_*: the host generator's arrays, element
 for element; its time is reported beside the host generator's), the index is also built on the GPU
 once more (bwt_index_*: equal to the host-built index; its build time beside the host's, and the
-stream answered over it must give the same SHA-256), the batch is planned on the GPU
+stream answered over it must give the same SHA-256; and once more the whole pipeline runs on the
+GPU from the seed, timed phase by phase: `all_on_device`), the batch is planned on the GPU
 (bwt_plan_launch: the host planner's plan, array for array; its time is reported beside the
 replay), and the replay runs over that device-generated, device-planned batch.
 
@@ -83,7 +84,7 @@ def main():
             sys.exit("bench.py: native build missing (make -C biworlddb-artifact_amd)")
     knobs = ["--tuned", TUNED] if os.path.exists(TUNED) else []
     r = run([BW_BATCH, "--warmup", str(a.warmup), "--repeat", str(a.steps), "--plain-sample", "4", "--stages",
-             "--device-plan", "--device-gen", "--device-index"] + knobs, 1200)
+             "--device-plan", "--device-gen", "--device-index", "--device-all"] + knobs, 1200)
     plain = run([BW_TREE, "--cfg", "2", "--mode", "pass", "--warmup", "2", "--repeat", "5", "--extract", "both"] +
                 knobs, 600)
     stages = {k: v for k, v in r["stage_ms"].items() if v > 0}
@@ -136,6 +137,9 @@ def main():
                          "host_checks_ms": r["device_index"]["host_ms"],
                          "host_index_ms": r["device_index"]["host_plan_ms"],
                          "equals_host_index": r["device_index"]["mismatch"] == 0},
+        "all_on_device": {k: r["device_all"][k] for k in ("generate_ms", "index_ms", "alloc_ms", "plan_answer_ms",
+                                                          "extract_ms", "total_ms")}
+        | {"same_answers": r["device_all"]["mismatch"] == 0},
         "device_gen": {"gpu_ms": r["device_gen"]["gpu_ms"], "wall_ms": r["device_gen"]["wall_ms"],
                        "host_generator_ms": r["device_gen"]["host_ms"],
                        "equals_host_generator": r["device_gen"]["mismatch"] == 0, "ties": r["device_gen"]["ties"]},
