@@ -143,8 +143,15 @@ bool TreeEngine::init_device_index(int64_t S, int U, uint64_t NK, const int32_t*
   launch = l;
   TreeIndexPlan& P = plan;
   DeviceArena tmp;  // the build's workspace and counts, freed at the end
-  hipEvent_t e0, e1;
-  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return fail("events", 0);
+  struct Events {  // destroyed on every return
+    hipEvent_t a = nullptr, b = nullptr;
+    ~Events() {
+      if (a) (void)hipEventDestroy(a);
+      if (b) (void)hipEventDestroy(b);
+    }
+  } ev;
+  if (hipEventCreate(&ev.a) != hipSuccess || hipEventCreate(&ev.b) != hipSuccess) return fail("events", 0);
+  hipEvent_t e0 = ev.a, e1 = ev.b;
   (void)hipEventRecord(e0, st);
   int rc;
   x = BwtIndex{};
@@ -256,8 +263,6 @@ bool TreeEngine::init_device_index(int64_t S, int U, uint64_t NK, const int32_t*
     return fail("chain offsets", 0);
   float gpu_ms = 0;
   (void)hipEventElapsedTime(&gpu_ms, e0, e1);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   plan_chunks(P);
   // the index as init() lays it out
   x.S = S;
