@@ -6,6 +6,12 @@
 //   bw_panel --fixture F --topology 0 --seed S --factor-period P --q Q --answers A --json J
 //            [--base-id ID] [--repeat R] [--arm v7|v7b|sess|sess_global] [--states K]
 //            [--blocks B] [--threads T] [--carry gpu|cpu] [--per-state]
+//   bw_panel --fixture BASE --spec SPEC --answers A --json J [--arm ...] [--plan-only]
+//       a BWHHIST1 declared history (src/real_history_spec.hpp, src/real_branch_spec.hpp) over its
+//       version-0 base fixture instead of the generated workload: chain histories as they are,
+//       branching ones as the chain of their depth-first walk (entries apply an edit, exits undo
+//       it); the answers are every version's, in (version, range) order.  --plan-only stops after
+//       the host plan and prints its shape (no GPU call).
 // BWH_RANGE_SCOPE=per_state selects per-state ranges as in the reference.
 // Exit codes: 2 usage, 3 unsupported input, 4 I/O, 5 GPU error (src/panel_batch_gpu.cpp:32-39).
 #include <hip/hip_runtime_api.h>
@@ -66,11 +72,12 @@ static int dev_alloc(T** d, size_t n) {
 #define AL(d, n) do { if (dev_alloc(&(d), (n))) return 5; } while (0)
 
 int main(int argc, char** argv) {
-  std::string fixture, base_id, answers_path, json_path, arm = "v7b", carry = "gpu";
+  std::string fixture, base_id, answers_path, json_path, arm = "v7b", carry = "gpu", spec_path;
   int topology = 0, seed = 7, fp = 4, q = 2, repeat = 0, K = 2049, B = 256, threads = 256;
-  bool per_state = false;
+  bool per_state = false, plan_only = false;
   for (int i = 1; i < argc; ++i) {
     if (!std::strcmp(argv[i], "--per-state")) { per_state = true; continue; }
+    if (!std::strcmp(argv[i], "--plan-only")) { plan_only = true; continue; }
     if (i + 1 >= argc) { std::fprintf(stderr, "missing value for %s\n", argv[i]); return 2; }
     const char* k = argv[i];
     const char* v = argv[++i];
@@ -88,6 +95,7 @@ int main(int argc, char** argv) {
     else if (!std::strcmp(k, "--blocks")) B = std::atoi(v);
     else if (!std::strcmp(k, "--threads")) threads = std::atoi(v);
     else if (!std::strcmp(k, "--carry")) carry = v;
+    else if (!std::strcmp(k, "--spec")) spec_path = v;
     else { std::fprintf(stderr, "unknown argument %s\n", k); return 2; }
   }
   if (fixture.empty() || answers_path.empty() || json_path.empty()) {
@@ -108,10 +116,36 @@ int main(int argc, char** argv) {
   if (!load_ref_fixture(fixture, f)) { std::fprintf(stderr, "cannot load fixture\n"); return 3; }
   const auto t1 = clk::now();
   RefWorkload w;
-  make_ref_workload(f, K, topology, seed, fp, q, per_state, -1, w);
-  RefPlan p;
   std::string err;
+  HistorySpec spec;
+  std::vector<int32_t> entry;  // spec: chain state of every version
+  const bool from_spec = !spec_path.empty();
+  if (from_spec) {
+    if (!load_history_spec(spec_path, f, spec, err)) { std::fprintf(stderr, "cannot use spec: %s\n", err.c_str()); return 3; }
+    spec_workload(f, spec, w, entry);
+    K = w.K;
+    q = spec.q;
+    fp = spec.fp;
+    per_state = true;
+    topology = spec.chain() ? 0 : 1;
+  } else {
+    make_ref_workload(f, K, topology, seed, fp, q, per_state, -1, w);
+  }
+  RefPlan p;
   if (!build_ref_plan(f, w, B, arm == "v7", p, err)) { std::fprintf(stderr, "unsupported case: %s\n", err.c_str()); return 3; }
+  char spec_json[512] = "";
+  if (from_spec)
+    std::snprintf(spec_json, sizeof(spec_json),
+                  ",\"spec\":{\"versions\":%d,\"q\":%d,\"branching\":%s,\"header_fields\":%d,\"walk_states\":%d,"
+                  "\"inserts\":%" PRId64 ",\"deletes\":%" PRId64 ",\"credits\":%" PRId64 ",\"factor_edits\":%" PRId64
+                  ",\"n_ins\":%" PRId64 "}",
+                  spec.K, spec.q, spec.chain() ? "false" : "true", spec.header_fields, w.K, spec.n_inserts,
+                  spec.n_deletes, spec.n_credits, spec.n_factor_edits, spec.n_ins);
+  if (plan_only) {
+    std::printf("{\"plan_only\":true,\"n_states\":%d,\"q\":%d,\"blocks\":%d,\"ncls\":%d,\"G\":%d,\"factor_groups\":%d%s}\n",
+                K, q, p.B, p.ncls, p.G, p.n_active, spec_json);
+    return 0;
+  }
   B = p.B;
   const int G = p.G, Cpfx = p.Cpfx, V = K - 1;
   std::vector<GpuV7Node> n72;
@@ -257,6 +291,14 @@ int main(int argc, char** argv) {
   }
   std::vector<int64_t> answers(ans_n);
   HIPX(hipMemcpy(answers.data(), d_ans, ans_n * 8, hipMemcpyDeviceToHost));
+  if (from_spec) {  // every version's answers, from the chain state of its entry
+    std::vector<int64_t> by_version(size_t(spec.K - 1) * q);
+    for (int v = 1; v < spec.K; ++v)
+      for (int k = 0; k < q; ++k)
+        by_version[size_t(v - 1) * q + k] = answers[size_t(entry[size_t(v)] - 1) * q + k];
+    answers.swap(by_version);
+  }
+  const size_t out_n = answers.size();
   const auto t3 = clk::now();
 
   // ---- extract
@@ -267,9 +309,9 @@ int main(int argc, char** argv) {
   if (!wok) return 4;
   uint64_t fnv = 1469598103934665603ULL;
   const uint8_t* bp = reinterpret_cast<const uint8_t*>(answers.data());
-  for (size_t i = 0; i < ans_n * 8; ++i) fnv = (fnv ^ bp[i]) * 1099511628211ULL;
+  for (size_t i = 0; i < out_n * 8; ++i) fnv = (fnv ^ bp[i]) * 1099511628211ULL;
   unsigned char dg[32];
-  SHA256(bp, ans_n * 8, dg);
+  SHA256(bp, out_n * 8, dg);
   char sha[65];
   for (int i = 0; i < 32; ++i) std::snprintf(sha + 2 * i, 3, "%02x", dg[i]);
   const auto t4 = clk::now();
@@ -294,12 +336,13 @@ int main(int argc, char** argv) {
       ",\"extract\":%" PRId64 ",\"cleanup\":%" PRId64 "},\"gpu\":{\"device\":\"%s\",\"blocks\":%d,\"threads\":%d,"
       "\"Cpfx\":%d,\"G\":%d,\"carry\":\"%s\",\"upload_ns\":%" PRId64 ",\"snap_ns\":%" PRId64 ",\"kernel_ns\":%" PRId64
       ",\"carry_ns\":%" PRId64 ",\"combine_ns\":%" PRId64 ",\"download_ns\":%" PRId64 ",\"steady_batch_ns\":%" PRId64
-      "},\"full_ns\":%" PRId64 "}\n",
-      arm.c_str(), base_id.c_str(), fixture.c_str(), topology, seed, fp, q, repeat, K, ans_n,
+      "},\"full_ns\":%" PRId64 "%s}\n",
+      arm.c_str(), base_id.c_str(), fixture.c_str(), topology, seed, fp, q, repeat, from_spec ? spec.K : K, out_n,
       per_state ? "per_state" : "shared", fnv, sha, load_ns, setup_ns, execute_ns, extract_ns, cleanup_ns, prop.name, B,
       threads, Cpfx, G, carry.c_str(), upload_ns, int64_t(snap_ms * 1e6), int64_t(kernel_ms * 1e6),
       int64_t(carry_ms * 1e6), int64_t(combine_ms * 1e6), download_ns, steady_ns,
-      load_ns + setup_ns + execute_ns + extract_ns + cleanup_ns);
+      load_ns + setup_ns + execute_ns + extract_ns + cleanup_ns, spec_json);
+  if (n < 0 || size_t(n) >= sizeof(buf)) return 4;
   std::fwrite(buf, 1, size_t(n), stdout);
   FILE* jf = std::fopen((json_path + ".tmp").c_str(), "w");
   if (!jf) return 4;

@@ -136,6 +136,173 @@ void make_ref_workload(const RefFixture& f, int K, int topology, int seed, int f
   }
 }
 
+bool HistorySpec::chain() const {
+  for (int v = 1; v < K; ++v)
+    if (parent[size_t(v)] != v - 1) return false;
+  return true;
+}
+
+bool load_history_spec(const std::string& path, const RefFixture& f, HistorySpec& s, std::string& err) {
+  s = HistorySpec();
+  FILE* fp = std::fopen(path.c_str(), "rb");
+  if (!fp) { err = "cannot open spec"; return false; }
+  std::fseek(fp, 0, SEEK_END);
+  const long size = std::ftell(fp);
+  std::fseek(fp, 0, SEEK_SET);
+  unsigned char h[124];
+  if (size < 124 || std::fread(h, 1, 124, fp) != 124 || std::memcmp(h, "BWHHIST1", 8) != 0) {
+    std::fclose(fp);
+    err = "not a BWHHIST1 spec";
+    return false;
+  }
+  for (int nf = 4; nf <= 5 && !s.header_fields; ++nf) {
+    int32_t i4[5] = {0, 0, 0, 0, 0};
+    int64_t i8[12];
+    std::memcpy(i4, h + 8, 4 * size_t(nf));
+    std::memcpy(i8, h + 8 + 4 * nf, sizeof(i8));
+    const int64_t K = i4[0], q = i4[1], nins = i8[1], nep = i8[4];
+    if (K < 2 || K > (1 << 28) || q < 1 || q > 128 || i4[3] < 1 || nins < 0 || nep < 0 || nins > (int64_t(1) << 32) ||
+        nep > (int64_t(1) << 32))
+      continue;
+    if (8 + 4 * nf + 96 + K * 4 + K * 40 + K * q * 16 + nins * 12 + nep * 8 != int64_t(size)) continue;
+    s.header_fields = nf;
+    s.K = i4[0]; s.q = i4[1]; s.fp = i4[2]; s.G = i4[3]; s.topology = nf == 5 ? i4[4] : 0;
+    s.n_base = i8[0]; s.n_ins = i8[1]; s.t_min = i8[2]; s.t_max = i8[3]; s.n_ep = i8[4]; s.period_lo = i8[5];
+    s.period_hi = i8[6]; s.bucket = i8[7]; s.n_inserts = i8[8]; s.n_deletes = i8[9]; s.n_credits = i8[10];
+    s.n_factor_edits = i8[11];
+  }
+  if (!s.header_fields) {
+    std::fclose(fp);
+    err = "spec size matches neither header layout";
+    return false;
+  }
+  std::fseek(fp, 8 + 4 * s.header_fields + 96, SEEK_SET);
+  const size_t K = size_t(s.K), nq = K * size_t(s.q);
+  struct RawEdit {  // the reference's Edit (src/common.hpp:166-175)
+    uint8_t op, pad[3];
+    int32_t g, pad2;
+    int64_t id, val, t;
+  };
+  static_assert(sizeof(RawEdit) == 40, "edit record");
+  std::vector<RawEdit> raw(K);
+  std::vector<int64_t> qr(2 * nq);
+  s.parent.resize(K);
+  s.ins_t.resize(size_t(s.n_ins));
+  s.ins_g.resize(size_t(s.n_ins));
+  s.endpoints.resize(size_t(s.n_ep));
+  const bool ok = std::fread(s.parent.data(), 4, K, fp) == K && std::fread(raw.data(), 40, K, fp) == K &&
+                  std::fread(qr.data(), 16, nq, fp) == nq &&
+                  std::fread(s.ins_t.data(), 8, s.ins_t.size(), fp) == s.ins_t.size() &&
+                  std::fread(s.ins_g.data(), 4, s.ins_g.size(), fp) == s.ins_g.size() &&
+                  std::fread(s.endpoints.data(), 8, s.endpoints.size(), fp) == s.endpoints.size();
+  std::fclose(fp);
+  if (!ok) { err = "short spec"; return false; }
+  s.edit.resize(K);
+  s.qlo.resize(nq);
+  s.qhi.resize(nq);
+  for (size_t i = 0; i < nq; ++i) { s.qlo[i] = qr[2 * i]; s.qhi[i] = qr[2 * i + 1]; }
+  if (s.n_base != f.n || s.G != f.G) { err = "spec and base fixture disagree (rows or groups)"; return false; }
+  const int64_t M = f.n + s.n_ins;
+  for (int64_t k = 0; k < s.n_ins; ++k)
+    if (s.ins_g[size_t(k)] < 0 || s.ins_g[size_t(k)] >= f.G) { err = "inserted row group out of range"; return false; }
+  for (size_t v = 1; v < K; ++v) {
+    const RawEdit& r = raw[v];
+    RefEdit& e = s.edit[v];
+    e.op = r.op; e.g = r.g; e.id = r.id; e.val = r.val; e.t = r.t;
+    if (s.parent[v] < 0 || s.parent[v] >= int32_t(v)) { err = "spec parents must satisfy 0 <= parent[v] < v"; return false; }
+    if (r.op > kFactor) { err = "unknown edit op"; return false; }
+    if (r.op == kFactor ? (r.g < 0 || r.g >= f.G) : (r.id < 0 || r.id >= M)) { err = "edit target out of range"; return false; }
+    if ((r.op == kInsert || r.op == kMove) && (r.g < 0 || r.g >= f.G)) { err = "edit group out of range"; return false; }
+    for (int k = 0; k < s.q; ++k)
+      if (s.qlo[v * size_t(s.q) + size_t(k)] > s.qhi[v * size_t(s.q) + size_t(k)]) { err = "range lo > hi"; return false; }
+  }
+  s.parent[0] = -1;
+  return true;
+}
+
+void spec_workload(const RefFixture& f, const HistorySpec& s, RefWorkload& w, std::vector<int32_t>& entry) {
+  const int K = s.K, q = s.q;
+  w = RefWorkload();
+  w.topology = 0;
+  w.factor_period = s.fp;
+  w.q = q;
+  w.per_state = true;
+  w.ins_t = s.ins_t;
+  w.ins_g = s.ins_g;
+  // children in ascending order (CSR)
+  std::vector<int32_t> off(static_cast<size_t>(K) + 1, 0), kid(static_cast<size_t>(K));
+  for (int v = 1; v < K; ++v) ++off[size_t(s.parent[size_t(v)]) + 1];
+  for (int v = 0; v < K; ++v) off[size_t(v) + 1] += off[size_t(v)];
+  {
+    std::vector<int32_t> cur(off.begin(), off.end() - 1);
+    for (int v = 1; v < K; ++v) kid[size_t(cur[size_t(s.parent[size_t(v)])]++)] = v;
+  }
+  // current row / factor state, to record what every edit overwrites
+  const int64_t N = f.n;
+  struct Row { uint8_t ex; int32_t g; int64_t t, amt; };
+  std::vector<Row> row(size_t(N + s.n_ins));
+  for (int64_t i = 0; i < N; ++i) row[size_t(i)] = {1, f.rows[size_t(i)].g, f.rows[size_t(i)].t, f.rows[size_t(i)].amount};
+  for (int64_t k = 0; k < s.n_ins; ++k) row[size_t(N + k)] = {0, s.ins_g[size_t(k)], s.ins_t[size_t(k)], 0};
+  std::vector<int64_t> fac(f.base_factor);
+  std::vector<RefEdit> undo(static_cast<size_t>(K));
+  std::vector<int32_t> state_of;  // chain state -> version whose ranges it carries
+  w.edit.assign(1, RefEdit{});
+  state_of.push_back(0);
+  entry.assign(size_t(K), 0);
+  size_t last_entry = 0;
+  std::vector<std::pair<int32_t, int32_t>> stk{{0, off[0]}};
+  while (!stk.empty()) {
+    auto& [v, next] = stk.back();
+    if (next < off[size_t(v) + 1]) {
+      const int32_t c = kid[size_t(next++)];
+      const RefEdit& e = s.edit[size_t(c)];
+      RefEdit& u = undo[size_t(c)];
+      if (e.op == kFactor) {
+        u.op = kFactor; u.g = e.g; u.val = fac[size_t(e.g)];
+        fac[size_t(e.g)] = e.val;
+      } else {
+        Row& r = row[size_t(e.id)];
+        u.op = kRestore; u.id = e.id; u.ex = r.ex; u.g = r.g; u.t = r.t; u.val = r.amt;
+        switch (e.op) {
+          case kReplace: r.ex = 1; r.amt = e.val; break;
+          case kDelete: r.ex = 0; break;
+          default: r.ex = 1; r.g = e.g; r.t = e.t; r.amt = e.val; break;  // insert / move
+        }
+      }
+      w.edit.push_back(e);
+      state_of.push_back(c);
+      entry[size_t(c)] = int32_t(w.edit.size() - 1);
+      last_entry = w.edit.size();
+      stk.push_back({c, off[size_t(c)]});
+    } else {
+      if (v > 0) {
+        const RefEdit& u = undo[size_t(v)];
+        if (u.op == kFactor) {
+          fac[size_t(u.g)] = u.val;
+        } else {
+          Row& r = row[size_t(u.id)];
+          r.ex = u.ex; r.g = u.g; r.t = u.t; r.amt = u.val;
+        }
+        w.edit.push_back(u);
+        state_of.push_back(v);
+      }
+      stk.pop_back();
+    }
+  }
+  w.edit.resize(last_entry);
+  state_of.resize(last_entry);
+  w.K = int(last_entry);
+  w.parent.resize(size_t(w.K));
+  for (int v = 0; v < w.K; ++v) w.parent[size_t(v)] = v - 1;
+  w.qlo.resize(size_t(w.K) * q);
+  w.qhi.resize(size_t(w.K) * q);
+  for (int v = 0; v < w.K; ++v)
+    for (int k = 0; k < q; ++k) {
+      w.qlo[size_t(v) * q + k] = s.qlo[size_t(state_of[size_t(v)]) * q + k];
+      w.qhi[size_t(v) * q + k] = s.qhi[size_t(state_of[size_t(v)]) * q + k];
+    }
+}
+
 bool build_ref_plan(const RefFixture& f, const RefWorkload& w, int B, bool cpu_snapshots, RefPlan& p,
                     std::string& err) {
   if (w.topology != 0) { err = "the v7 block path needs a chain version tree (topology 0)"; return false; }
@@ -198,6 +365,7 @@ bool build_ref_plan(const RefFixture& f, const RefWorkload& w, int B, bool cpu_s
     switch (e.op) {
       case kReplace: r.ex = 1; r.amt = e.val; break;
       case kDelete: r.ex = 0; break;
+      case kRestore: r.ex = e.ex; r.g = e.g; r.c = class_of(e.t); r.amt = e.val; break;
       default: r.ex = 1; r.g = e.g; r.c = class_of(e.t); r.amt = e.val; break;  // insert / move
     }
     n.ex_new = r.ex; n.g_new = r.g; n.c_new = r.c; n.amt_new = r.amt;

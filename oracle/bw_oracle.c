@@ -13,6 +13,10 @@
  *       by a direct scan over all base + inserted rows accumulating amount*factor in __int128,
  *       checked into int64 (exit 6 on overflow, src/oracle.cpp:203-211).  K is the reference's
  *       compile-time kStates (src/common.hpp:184-192) given at run time.
+ *   spec --fixture BASE --spec SPEC --out ANS
+ *       Answers of a BWHHIST1 declared history spec (chain or branching; both header layouts)
+ *       over its version-0 base fixture, by the same DFS (the reference's spec oracles,
+ *       src/panel_real_history_oracle.cpp, src/panel_real_branch_oracle.cpp).
  *   tree --cfg C [--S n --U n --Q n --N n --seed n] [--algo fenwick|brute] --out ANS
  *       Oracle for the branching multi-update point-update workloads of the benchmark configs
  *       (bw_twl.h).  A point update (key, d) is the reference's OP_INSERT of a fresh row with
@@ -149,6 +153,99 @@ static int ref_parent(int topology, int32_t v) {
   return 0;
 }
 
+/* The reference oracle's solve (src/oracle.cpp:225-451): iterative DFS over the version tree
+ * (children in ascending order), apply a version's edit on entry, answer its q ranges by a direct
+ * scan of every base + inserted row accumulating amount*factor in __int128 (checked into int64),
+ * restore the edit on exit.  Inserted row k (id N + k) starts absent at (ins_t[k], ins_g[k],
+ * amount 0), or at (0, 0, 0) when ins_t is NULL.  Ranges of version v at (v*q + k) when
+ * per_state, else shared.  Writes ans[(v-1)*q + k] for v = 1..K-1. */
+static void ref_solve(int64_t K, const int32_t* parent, const edit_t* ed, int64_t q, const int64_t* qlo,
+                      const int64_t* qhi, int per_state, const fixture_t* fx, int64_t next_ins, const int64_t* ins_t,
+                      const int32_t* ins_g, int64_t* ans) {
+  const int64_t N = fx->n;
+  const int32_t G = fx->G;
+  /* mutable row state (src/oracle.cpp:240-278) */
+  const int64_t M = N + next_ins;
+  int64_t* t = (int64_t*)malloc(sizeof(int64_t) * (size_t)M);
+  int64_t* amt = (int64_t*)malloc(sizeof(int64_t) * (size_t)M);
+  int32_t* g = (int32_t*)malloc(sizeof(int32_t) * (size_t)M);
+  uint8_t* ex = (uint8_t*)malloc((size_t)M);
+  for (int64_t i = 0; i < N; ++i) { t[i] = fx->rows[i].t; amt[i] = fx->rows[i].amount; g[i] = fx->rows[i].g; ex[i] = 1; }
+  for (int64_t k = 0; k < next_ins; ++k) {
+    t[N + k] = ins_t ? ins_t[k] : 0; g[N + k] = ins_g ? ins_g[k] : 0; amt[N + k] = 0; ex[N + k] = 0;
+  }
+  int64_t* fac = (int64_t*)malloc(sizeof(int64_t) * (size_t)G);
+  for (int32_t gg = 0; gg < G; ++gg) fac[gg] = fx->factor[gg];
+  /* children in ascending order (CSR) */
+  int64_t* koff = (int64_t*)calloc((size_t)K + 1, sizeof(int64_t));
+  int32_t* kid = (int32_t*)malloc(sizeof(int32_t) * (size_t)K);
+  for (int64_t v = 1; v < K; ++v) koff[parent[v] + 1]++;
+  for (int64_t v = 0; v < K; ++v) koff[v + 1] += koff[v];
+  {
+    int64_t* cur = (int64_t*)malloc(sizeof(int64_t) * (size_t)K);
+    memcpy(cur, koff, sizeof(int64_t) * (size_t)K);
+    for (int64_t v = 1; v < K; ++v) kid[cur[parent[v]]++] = (int32_t)v;
+    free(cur);
+  }
+  /* undo records */
+  typedef struct { int64_t amt, t, fac; int32_t g; uint8_t ex; } undo_t;
+  undo_t* un = (undo_t*)calloc((size_t)K, sizeof(undo_t));
+  int64_t* stk_v = (int64_t*)malloc(sizeof(int64_t) * (size_t)K);
+  int64_t* stk_c = (int64_t*)malloc(sizeof(int64_t) * (size_t)K);
+  __int128* acc = (__int128*)malloc(sizeof(__int128) * (size_t)q);
+  int64_t sp = 0;
+  stk_v[0] = 0; stk_c[0] = koff[0]; sp = 1;
+  while (sp > 0) {
+    const int64_t v = stk_v[sp - 1];
+    if (stk_c[sp - 1] < koff[v + 1]) {
+      const int32_t c = kid[stk_c[sp - 1]++];
+      /* apply (src/oracle.cpp:280-330) */
+      const edit_t* e = &ed[c];
+      undo_t* u = &un[c];
+      switch (e->op) {
+        case OP_REPLACE: u->amt = amt[e->id]; u->ex = ex[e->id]; amt[e->id] = e->val; ex[e->id] = 1; break;
+        case OP_INSERT:
+        case OP_MOVE:
+          u->amt = amt[e->id]; u->t = t[e->id]; u->g = g[e->id]; u->ex = ex[e->id];
+          t[e->id] = e->t; g[e->id] = e->g; amt[e->id] = e->val; ex[e->id] = 1; break;
+        case OP_DELETE: u->ex = ex[e->id]; ex[e->id] = 0; break;
+        case OP_FACTOR: u->fac = fac[e->g]; fac[e->g] = e->val; break;
+        default: die(7, "unknown op");
+      }
+      /* answer: direct scan of every row (src/oracle.cpp:359-422) */
+      const int64_t* lo = per_state ? &qlo[(size_t)c * q] : qlo;
+      const int64_t* hi = per_state ? &qhi[(size_t)c * q] : qhi;
+      for (int64_t k = 0; k < q; ++k) acc[k] = 0;
+      for (int64_t i = 0; i < M; ++i) {
+        if (!ex[i]) continue;
+        const int64_t ti = t[i];
+        const __int128 w = (__int128)amt[i] * (__int128)fac[g[i]];
+        for (int64_t k = 0; k < q; ++k)
+          if (ti >= lo[k] && ti < hi[k]) acc[k] += w;
+      }
+      for (int64_t k = 0; k < q; ++k) ans[(size_t)(c - 1) * q + k] = checked_i64(acc[k]);
+      stk_v[sp] = c; stk_c[sp] = koff[c]; ++sp;
+    } else {
+      if (v > 0) {
+        /* revert (src/oracle.cpp:332-355) */
+        const edit_t* e = &ed[v];
+        const undo_t* u = &un[v];
+        switch (e->op) {
+          case OP_REPLACE: amt[e->id] = u->amt; ex[e->id] = u->ex; break;
+          case OP_INSERT:
+          case OP_MOVE: t[e->id] = u->t; g[e->id] = u->g; amt[e->id] = u->amt; ex[e->id] = u->ex; break;
+          case OP_DELETE: ex[e->id] = u->ex; break;
+          case OP_FACTOR: fac[e->g] = u->fac; break;
+          default: break;
+        }
+      }
+      --sp;
+    }
+  }
+  free(t); free(amt); free(g); free(ex); free(fac); free(koff); free(kid); free(un); free(stk_v); free(stk_c);
+  free(acc);
+}
+
 static int mode_ref(int argc, char** argv) {
   const char* fx = NULL; const char* out = NULL;
   int64_t topo = -1, seed = 0, fp = -1, q = -1, K = 2049, fpct = -1;
@@ -240,84 +337,114 @@ static int mode_ref(int argc, char** argv) {
     }
     ed[v] = e;
   }
-  /* mutable row state (src/oracle.cpp:240-278) */
-  const int64_t M = N + next_ins;
-  int64_t* t = (int64_t*)malloc(sizeof(int64_t) * (size_t)M);
-  int64_t* amt = (int64_t*)malloc(sizeof(int64_t) * (size_t)M);
-  int32_t* g = (int32_t*)malloc(sizeof(int32_t) * (size_t)M);
-  uint8_t* ex = (uint8_t*)malloc((size_t)M);
-  for (int64_t i = 0; i < N; ++i) { t[i] = f.rows[i].t; amt[i] = f.rows[i].amount; g[i] = f.rows[i].g; ex[i] = 1; }
-  for (int64_t k = 0; k < next_ins; ++k) { t[N + k] = ins_t[k]; g[N + k] = ins_g[k]; amt[N + k] = 0; ex[N + k] = 0; }
-  int64_t* fac = (int64_t*)malloc(sizeof(int64_t) * (size_t)G);
-  for (int32_t gg = 0; gg < G; ++gg) fac[gg] = f.factor[gg];
-  /* children in ascending order (CSR) */
-  int64_t* koff = (int64_t*)calloc((size_t)K + 1, sizeof(int64_t));
-  int32_t* kid = (int32_t*)malloc(sizeof(int32_t) * (size_t)K);
-  for (int64_t v = 1; v < K; ++v) koff[ref_parent((int)topo, (int32_t)v) + 1]++;
-  for (int64_t v = 0; v < K; ++v) koff[v + 1] += koff[v];
-  {
-    int64_t* cur = (int64_t*)malloc(sizeof(int64_t) * (size_t)K);
-    memcpy(cur, koff, sizeof(int64_t) * (size_t)K);
-    for (int64_t v = 1; v < K; ++v) kid[cur[ref_parent((int)topo, (int32_t)v)]++] = (int32_t)v;
-    free(cur);
-  }
-  /* undo records */
-  typedef struct { int64_t amt, t, fac; int32_t g; uint8_t ex; } undo_t;
-  undo_t* un = (undo_t*)calloc((size_t)K, sizeof(undo_t));
+  int32_t* par = (int32_t*)malloc(sizeof(int32_t) * (size_t)K);
+  par[0] = -1;
+  for (int64_t v = 1; v < K; ++v) par[v] = ref_parent((int)topo, (int32_t)v);
   const size_t nans = (size_t)(K - 1) * (size_t)q;
   int64_t* ans = (int64_t*)calloc(nans ? nans : 1, sizeof(int64_t));
-  int64_t* stk_v = (int64_t*)malloc(sizeof(int64_t) * (size_t)K);
-  int64_t* stk_c = (int64_t*)malloc(sizeof(int64_t) * (size_t)K);
-  __int128* acc = (__int128*)malloc(sizeof(__int128) * (size_t)q);
-  int64_t sp = 0;
-  stk_v[0] = 0; stk_c[0] = koff[0]; sp = 1;
-  while (sp > 0) {
-    const int64_t v = stk_v[sp - 1];
-    if (stk_c[sp - 1] < koff[v + 1]) {
-      const int32_t c = kid[stk_c[sp - 1]++];
-      /* apply (src/oracle.cpp:280-330) */
-      const edit_t* e = &ed[c];
-      undo_t* u = &un[c];
-      switch (e->op) {
-        case OP_REPLACE: u->amt = amt[e->id]; u->ex = ex[e->id]; amt[e->id] = e->val; ex[e->id] = 1; break;
-        case OP_INSERT:
-        case OP_MOVE:
-          u->amt = amt[e->id]; u->t = t[e->id]; u->g = g[e->id]; u->ex = ex[e->id];
-          t[e->id] = e->t; g[e->id] = e->g; amt[e->id] = e->val; ex[e->id] = 1; break;
-        case OP_DELETE: u->ex = ex[e->id]; ex[e->id] = 0; break;
-        case OP_FACTOR: u->fac = fac[e->g]; fac[e->g] = e->val; break;
-        default: die(7, "unknown op");
-      }
-      /* answer: direct scan of every row (src/oracle.cpp:359-422) */
-      const int64_t* lo = per_state ? &qlo[(size_t)c * q] : qlo;
-      const int64_t* hi = per_state ? &qhi[(size_t)c * q] : qhi;
-      for (int64_t k = 0; k < q; ++k) acc[k] = 0;
-      for (int64_t i = 0; i < M; ++i) {
-        if (!ex[i]) continue;
-        const int64_t ti = t[i];
-        const __int128 w = (__int128)amt[i] * (__int128)fac[g[i]];
-        for (int64_t k = 0; k < q; ++k)
-          if (ti >= lo[k] && ti < hi[k]) acc[k] += w;
-      }
-      for (int64_t k = 0; k < q; ++k) ans[(size_t)(c - 1) * q + k] = checked_i64(acc[k]);
-      stk_v[sp] = c; stk_c[sp] = koff[c]; ++sp;
-    } else {
-      if (v > 0) {
-        /* revert (src/oracle.cpp:332-355) */
-        const edit_t* e = &ed[v];
-        const undo_t* u = &un[v];
-        switch (e->op) {
-          case OP_REPLACE: amt[e->id] = u->amt; ex[e->id] = u->ex; break;
-          case OP_INSERT:
-          case OP_MOVE: t[e->id] = u->t; g[e->id] = u->g; amt[e->id] = u->amt; ex[e->id] = u->ex; break;
-          case OP_DELETE: ex[e->id] = u->ex; break;
-          case OP_FACTOR: fac[e->g] = u->fac; break;
-          default: break;
-        }
-      }
-      --sp;
-    }
+  ref_solve(K, par, ed, q, qlo, qhi, per_state, &f, next_ins, ins_t, ins_g, ans);
+  write_answers(out, ans, nans, t0);
+  return 0;
+}
+
+/* ------------------------------------------------------------------ BWHHIST1 specs */
+/* A declared history spec (chain histories: src/real_history_spec.hpp:41-71, 4-field header;
+ * branching histories: src/real_branch_spec.hpp:45-71, 5-field header with a topology flag):
+ * "BWHHIST1", i32 kstates, q, fp, G [, topology], 12 i64 (n_base, n_ins, t_min, t_max, n_ep,
+ * period_lo, period_hi, bucket, n_inserts, n_deletes, n_credits, n_factor_edits), parent[K] i32,
+ * edit[K] (40-byte records: op u8, pad[3], g i32, pad i32, id i64, val i64, t i64), queries[K*q]
+ * (lo, hi i64), ins_t[n_ins] i64, ins_g[n_ins] i32, endpoints[n_ep] i64.  Both layouts share the
+ * magic: the one whose arrays add up to the file's size is taken. */
+typedef struct {
+  int32_t K, q, fp, G, topology, header_fields;
+  int64_t n_base, n_ins, n_ep;
+  int32_t* parent;
+  edit_t* ed;
+  int64_t *qlo, *qhi;
+} spec_t;
+
+static int load_spec(const char* path, spec_t* s) {
+  FILE* fp = fopen(path, "rb");
+  if (!fp) return 0;
+  fseek(fp, 0, SEEK_END);
+  const long size = ftell(fp);
+  fseek(fp, 0, SEEK_SET);
+  unsigned char h[128];
+  if (size < 124 || fread(h, 1, 124, fp) != 124 || memcmp(h, "BWHHIST1", 8) != 0) { fclose(fp); return 0; }
+  int found = 0;
+  for (int nf = 4; nf <= 5 && !found; ++nf) {
+    int32_t i4[5] = {0, 0, 0, 0, 0};
+    int64_t i8[12];
+    memcpy(i4, h + 8, 4 * (size_t)nf);
+    memcpy(i8, h + 8 + 4 * nf, 96);
+    const int64_t K = i4[0], q = i4[1], nins = i8[1], nep = i8[4];
+    if (K < 2 || q < 1 || q > 1024 || nins < 0 || nep < 0 || i4[3] < 1) continue;
+    const int64_t want = 8 + 4 * nf + 96 + K * 4 + K * 40 + K * q * 16 + nins * 12 + nep * 8;
+    if (want != size) continue;
+    found = 1;
+    s->K = i4[0]; s->q = i4[1]; s->fp = i4[2]; s->G = i4[3]; s->topology = nf == 5 ? i4[4] : 0;
+    s->header_fields = nf;
+    s->n_base = i8[0]; s->n_ins = nins; s->n_ep = nep;
   }
+  if (!found) { fclose(fp); return 0; }
+  fseek(fp, 8 + 4 * s->header_fields + 96, SEEK_SET);
+  const size_t K = (size_t)s->K, nq = K * (size_t)s->q;
+  s->parent = (int32_t*)malloc(4 * K);
+  s->ed = (edit_t*)calloc(K, sizeof(edit_t));
+  s->qlo = (int64_t*)malloc(8 * nq);
+  s->qhi = (int64_t*)malloc(8 * nq);
+  unsigned char* raw = (unsigned char*)malloc(40 * K);
+  int64_t* qr = (int64_t*)malloc(16 * nq);
+  int ok = fread(s->parent, 4, K, fp) == K && fread(raw, 40, K, fp) == K && fread(qr, 16, nq, fp) == nq;
+  fclose(fp);
+  for (size_t v = 0; ok && v < K; ++v) {
+    const unsigned char* r = raw + 40 * v;
+    s->ed[v].op = r[0];
+    memcpy(&s->ed[v].g, r + 4, 4);
+    memcpy(&s->ed[v].id, r + 16, 8);
+    memcpy(&s->ed[v].val, r + 24, 8);
+    memcpy(&s->ed[v].t, r + 32, 8);
+  }
+  for (size_t i = 0; ok && i < nq; ++i) { s->qlo[i] = qr[2 * i]; s->qhi[i] = qr[2 * i + 1]; }
+  free(raw);
+  free(qr);
+  return ok;
+}
+
+/* spec --fixture BASE --spec SPEC --out ANS: the answers of every version of a declared history
+ * (both layouts), the reference's spec oracles' semantics (src/panel_real_history_oracle.cpp:
+ * 63-110 replays the chain; src/panel_real_branch_oracle.cpp:36-83,120-170 the enter/exit DFS):
+ * inserted rows start absent at (0, 0, 0); a version's value is the sum over present rows with
+ * lo <= t < hi of factor[g] * amount. */
+static int mode_spec(int argc, char** argv) {
+  const char* fx = NULL; const char* sp = NULL; const char* out = NULL;
+  for (int i = 2; i < argc; ++i) {
+    if (i + 1 >= argc) die(2, "missing value");
+    const char* k = argv[i]; const char* v = argv[++i];
+    if (!strcmp(k, "--fixture")) fx = v;
+    else if (!strcmp(k, "--spec")) sp = v;
+    else if (!strcmp(k, "--out")) out = v;
+    else die(2, "unknown argument");
+  }
+  if (!fx || !sp) die(2, "usage: spec --fixture BASE --spec SPEC --out ANS");
+  const double t0 = now_s();
+  fixture_t f;
+  spec_t s;
+  memset(&s, 0, sizeof(s));
+  if (!load_fixture(fx, &f)) die(1, "cannot load fixture");
+  if (!load_spec(sp, &s)) die(1, "cannot load spec");
+  if (s.n_base != f.n || s.G != f.G) die(3, "spec/fixture base mismatch");
+  const int64_t M = f.n + s.n_ins;
+  for (int64_t v = 1; v < s.K; ++v) {
+    if (s.parent[v] < 0 || s.parent[v] >= v) die(3, "spec parent out of order");
+    const edit_t* e = &s.ed[v];
+    if (e->op > OP_FACTOR) die(3, "spec edit op");
+    if (e->op == OP_FACTOR ? (e->g < 0 || e->g >= f.G) : (e->id < 0 || e->id >= M)) die(3, "spec edit target");
+    if ((e->op == OP_INSERT || e->op == OP_MOVE) && (e->g < 0 || e->g >= f.G)) die(3, "spec edit group");
+  }
+  const size_t nans = (size_t)(s.K - 1) * (size_t)s.q;
+  int64_t* ans = (int64_t*)calloc(nans ? nans : 1, sizeof(int64_t));
+  ref_solve(s.K, s.parent, s.ed, s.q, s.qlo, s.qhi, 1, &f, s.n_ins, NULL, NULL, ans);
   write_answers(out, ans, nans, t0);
   return 0;
 }
@@ -771,6 +898,7 @@ int main(int argc, char** argv) {
   if (argc < 2) die(2, "usage: bw_oracle sha|ref|tree|stream ...");
   if (!strcmp(argv[1], "sha")) return mode_sha(argc, argv);
   if (!strcmp(argv[1], "ref")) return mode_ref(argc, argv);
+  if (!strcmp(argv[1], "spec")) return mode_spec(argc, argv);
   if (!strcmp(argv[1], "tree")) return mode_tree(argc, argv);
   if (!strcmp(argv[1], "stream")) return mode_stream(argc, argv);
   die(2, "unknown mode");

@@ -58,3 +58,17 @@ def fixture_file(params):
     from make_golden import fixture_path
 
     return fixture_path(params)
+
+
+def spec_files(case, tmp_path):
+    """(base fixture, spec) of a committed BWHHIST1 case, decompressed into tmp_path."""
+    import gzip
+    import shutil
+
+    out = []
+    for part in ("base", "spec"):
+        dst = os.path.join(str(tmp_path), f"{case}.{part}")
+        with gzip.open(os.path.join(GOLDEN, "specs", f"{case}.{part}.gz"), "rb") as fi, open(dst, "wb") as fo:
+            shutil.copyfileobj(fi, fo)
+        out.append(dst)
+    return out
