@@ -389,12 +389,13 @@ __global__ void __launch_bounds__(256) bwt_seg_apply_kernel(const BwtIndex x) {
 // tree; an even number of words keeps every wavefront's area 16-byte aligned.
 __host__ __device__ constexpr int replay_wave_words(int stride) { return (96 + stride + 1) & ~1; }
 
-// Fenwick prefix over classes < y for y < 4096 (at most 12 set bits); fen[0] stays 0, so the
-// fixed trip count needs no branch and the 12 LDS reads issue back to back.
-__device__ __forceinline__ int64_t fen_prefix12(const int64_t* fen, int y) {
+// Fenwick prefix over classes < y for y < 2^kBits (at most kBits set bits); fen[0] stays 0, so
+// the fixed trip count needs no branch and the LDS reads issue back to back.
+template <int kBits>
+__device__ __forceinline__ int64_t fen_prefix(const int64_t* fen, int y) {
   int64_t r = 0;
 #pragma unroll
-  for (int k = 0; k < 12; ++k) {
+  for (int k = 0; k < kBits; ++k) {
     r += fen[y];
     y &= y - 1;
   }
@@ -408,7 +409,8 @@ __device__ __forceinline__ int64_t fen_prefix12(const int64_t* fen, int y) {
 // the Fenwick prefix plus the group's keys that precede it with a smaller class (broadcast
 // reads), then the keys enter the tree.  The next group's items are loaded while this one runs
 // (only the high word of a key item, which holds its class).  Results are stored in work order
-// (g[i] for eitems[i]): a group's taken lanes write one contiguous run.
+// (g[i] for eitems[i]): a group's taken lanes write one contiguous run.  Classes < 2^kBits.
+template <int kBits>
 __device__ __forceinline__ void chunk_replay_one(const BwtIndex& x, const BwtBatch& q, const int stride,
                                                  const int64_t ci, int64_t* kws) {
   const int lane = threadIdx.x & 63;
@@ -462,7 +464,7 @@ __device__ __forceinline__ void chunk_replay_one(const BwtIndex& x, const BwtBat
       // a taken endpoint needs the group's first `rel` keys; rel grows with the lane
       const int mmax = __builtin_amdgcn_readlane(rel, ne - 1);
       const int rl = take ? rel : 0;
-      int64_t val = fen_prefix12(fen, take ? cls : 0);
+      int64_t val = fen_prefix<kBits>(fen, take ? cls : 0);
       // four keys per 16-byte LDS read (entries past nk are excluded by m < rl <= nk)
       for (int m = 0; m < mmax; m += 4) {
         const int4 c4 = *reinterpret_cast<const int4*>(kcs + m);
@@ -496,6 +498,7 @@ __device__ __forceinline__ void chunk_replay_one(const BwtIndex& x, const BwtBat
 
 // One wavefront per work item; with device counts (device planner) the grid is bounded and each
 // wavefront takes every (gridDim * chunk_waves)-th work item.
+template <int kBits>
 __global__ void __launch_bounds__(256) bwt_chunk_replay_kernel(const BwtIndex x, const BwtBatch q, const BwtLaunch l,
                                                                int stride) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -504,7 +507,7 @@ __global__ void __launch_bounds__(256) bwt_chunk_replay_kernel(const BwtIndex x,
   const int64_t n = q.dcounts ? q.dcounts[2] : q.nwork;
   const int64_t step = (int64_t)gridDim.x * l.chunk_waves;
   for (int64_t ci = (int64_t)blockIdx.x * l.chunk_waves + wv; ci < n; ci += step) {
-    chunk_replay_one(x, q, stride, ci, kws);
+    chunk_replay_one<kBits>(x, q, stride, ci, kws);
     wave_lds_sync();  // the next work item reuses this wavefront's LDS
   }
 }
@@ -687,7 +690,8 @@ int set_lds_limits() {
                         reinterpret_cast<const void*>(bwt_block_kernel<8>),
                         reinterpret_cast<const void*>(bwt_block_kernel<16>),
                         reinterpret_cast<const void*>(bwt_chunk_hist_kernel),
-                        reinterpret_cast<const void*>(bwt_chunk_replay_kernel)};
+                        reinterpret_cast<const void*>(bwt_chunk_replay_kernel<9>),
+                        reinterpret_cast<const void*>(bwt_chunk_replay_kernel<12>)};
     for (const void* k : ks) {
       const hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
       if (e != hipSuccess && err == hipSuccess) err = e;
@@ -771,8 +775,13 @@ extern "C" int bwt_chunk_replay_launch(const BwtIndex* x, const BwtBatch* b, con
   const int stride = level_stride(x->ncls_b);
   int64_t grid = (b->nwork + l->chunk_waves - 1) / l->chunk_waves;
   if (b->dcounts) grid = std::min<int64_t>(grid, kDeviceCountGrid);
-  hipLaunchKernelGGL(bwt_chunk_replay_kernel, dim3((unsigned)grid), dim3(64 * l->chunk_waves),
-                     replay_lds_bytes(x, l), (hipStream_t)stream, *x, *b, *l, stride);
+  // classes of a level-B chain: key offsets within a tile, < ncls_b <= 4096
+  if (x->ncls_b <= 512)
+    hipLaunchKernelGGL(bwt_chunk_replay_kernel<9>, dim3((unsigned)grid), dim3(64 * l->chunk_waves),
+                       replay_lds_bytes(x, l), (hipStream_t)stream, *x, *b, *l, stride);
+  else
+    hipLaunchKernelGGL(bwt_chunk_replay_kernel<12>, dim3((unsigned)grid), dim3(64 * l->chunk_waves),
+                       replay_lds_bytes(x, l), (hipStream_t)stream, *x, *b, *l, stride);
   return report("chunk replay", hipGetLastError());
 }
 
