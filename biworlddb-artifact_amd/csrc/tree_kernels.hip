@@ -555,8 +555,9 @@ __global__ void __launch_bounds__(BWT_COMBINE_THREADS) bwt_combine_slots_kernel(
       }
     }
     // level B: the slot's two results, stored in work order (one die's range of it)
-    const uint2 ep = *reinterpret_cast<const uint2*>(q.epos + 2 * s);
-    part[k] = q.ansr[s] + (q.g[ep.y] - q.g[ep.x]);
+    // epos and ansr are read once: non-temporal, so they do not evict the gathered g lines
+    const uint64_t epp = __builtin_nontemporal_load(reinterpret_cast<const uint64_t*>(q.epos) + s);
+    part[k] = __builtin_nontemporal_load(q.ansr + s) + (q.g[(uint32_t)(epp >> 32)] - q.g[(uint32_t)epp]);
   }
   if (kCarry) {
     int64_t r0[kPer];
@@ -589,13 +590,13 @@ __global__ void __launch_bounds__(BWT_COMBINE_THREADS) bwt_combine_ids_kernel(co
   const int64_t last = q.M - 1;
   uint32_t p[kIdsPer];
 #pragma unroll
-  for (int k = 0; k < kIdsPer; ++k) p[k] = q.ipos[min(i0 + k * BWT_COMBINE_THREADS, last)];
+  for (int k = 0; k < kIdsPer; ++k) p[k] = __builtin_nontemporal_load(q.ipos + min(i0 + k * BWT_COMBINE_THREADS, last));
   int64_t v[kIdsPer];
 #pragma unroll
   for (int k = 0; k < kIdsPer; ++k) v[k] = q.stage[p[k]];
 #pragma unroll
   for (int k = 0; k < kIdsPer; ++k)
-    if (i0 + k * BWT_COMBINE_THREADS <= last) q.answers[i0 + k * BWT_COMBINE_THREADS] = v[k];
+    if (i0 + k * BWT_COMBINE_THREADS <= last) __builtin_nontemporal_store(v[k], q.answers + i0 + k * BWT_COMBINE_THREADS);
 }
 
 // ---------------------------------------------------------------- overlay (appended states)
