@@ -72,6 +72,67 @@ def make_source(path, n_pos, groups, seed, cancel_every=6, credit_every=11):
         f.write(np.arange(groups, dtype="<i4").tobytes())
 
 
+def write_synthetic_spec(base_path, spec_path, K, q, seed, n_base=600, groups=5, window=24):
+    """A branching BWHHIST1 spec (5-field layout) of this project's own definition, with every
+    reference op: inserts of new rows, deletes, replaces and moves of base rows, group-factor
+    edits; parent[v] uniform over the `window` versions before v.  Replaces and moves touch base
+    rows only (a replace keeps the row's hidden time and group, so it must not meet a row whose
+    hidden values differ between readers' initialisations).  The base fixture is written too."""
+    d = _splitmix(seed, 3 * n_base + K * (8 + 2 * q)).astype(np.uint64)
+    it = iter(d.tolist())
+    nxt = lambda: next(it)  # noqa: E731
+    span = 1_000_000
+    base = [(nxt() % span, nxt() % 90000 + 1, nxt() % groups) for _ in range(n_base)]
+    rec = np.zeros(n_base, dtype=[("t", "<i8"), ("a", "<i8"), ("g", "<i4"), ("p", "<i4")])
+    rec["t"], rec["a"], rec["g"] = [b[0] for b in base], [b[1] for b in base], [b[2] for b in base]
+    hdr = struct.pack("<8sqiiqqqqq", b"BWBASE02", n_base, groups, 0, n_base, 0, 0, span - 1, 2)
+    with open(base_path, "wb") as f:
+        f.write(hdr + rec.tobytes() + np.full(groups, 1000, dtype="<i8").tobytes() +
+                np.arange(groups, dtype="<i4").tobytes())
+    parent = [-1] + [max(0, v - 1 - int(nxt() % window)) for v in range(1, K)]
+    edits = [(0, 0, 0, 0, 0)]  # (op, g, id, val, t)
+    ins_t, ins_g = [], []
+    for v in range(1, K):
+        roll = nxt() % 100
+        if roll < 30:
+            t, g = nxt() % span, nxt() % groups
+            edits.append((1, g, n_base + len(ins_t), int(nxt() % 200001) - 100000, t))
+            ins_t.append(t)
+            ins_g.append(g)
+        elif roll < 50:
+            # delete a base row or a row inserted earlier (it may be absent on this path)
+            nrow = n_base + len(ins_t)
+            edits.append((2, 0, int(nxt() % nrow), 0, 0))
+        elif roll < 70:
+            edits.append((0, 0, int(nxt() % n_base), int(nxt() % 200001) - 100000, 0))
+        elif roll < 88:
+            edits.append((3, int(nxt() % groups), int(nxt() % n_base), int(nxt() % 200001) - 100000, int(nxt() % span)))
+        else:
+            edits.append((4, int(nxt() % groups), 0, 900 + int(nxt() % 201), 0))
+    qs = []
+    for v in range(K):
+        for _ in range(q):
+            # endpoints on a 1,000-wide grid: about a thousand classes, so the shared-memory
+            # arms (v7, v7b) take the case too
+            a, b = sorted((int(nxt() % (span + 2000)) // 1000 * 1000 - 1000, int(nxt() % (span + 2000)) // 1000 * 1000 - 1000))
+            qs.append((a, b))
+    eps = sorted({x for ab in qs[q:] for x in ab})
+    with open(spec_path, "wb") as f:
+        f.write(b"BWHHIST1" + struct.pack("<5i", K, q, 0, groups, 1))
+        n_ins = len(ins_t)
+        nins = sum(1 for e in edits[1:] if e[0] == 1)
+        ndel = sum(1 for e in edits[1:] if e[0] == 2)
+        nfac = sum(1 for e in edits[1:] if e[0] == 4)
+        f.write(struct.pack("<12q", n_base, n_ins, 0, span - 1, len(eps), 0, 0, 0, nins, ndel, 0, nfac))
+        f.write(struct.pack(f"<{K}i", *parent))
+        for op, g, i, val, t in edits:
+            f.write(struct.pack("<B3xii4xqqq", op, g, 0, i, val, t))  # 40 bytes, id at 16
+        for a, b in qs:
+            f.write(struct.pack("<qq", a, b))
+        f.write(struct.pack(f"<{n_ins}q", *ins_t) + struct.pack(f"<{n_ins}i", *ins_g))
+        f.write(struct.pack(f"<{len(eps)}q", *eps))
+
+
 def _run(cmd):
     p = subprocess.run(cmd, capture_output=True, text=True)
     if p.returncode != 0:

@@ -8,7 +8,7 @@ import subprocess
 
 import pytest
 
-from conftest import fixture_file, golden, run_json, spec_files
+from conftest import GOLDEN, fixture_file, golden, run_json, spec_files
 
 REF = golden("ref_cases.json")
 SMALL = [k for k, v in REF.items() if v["states"] <= 9001]
@@ -109,3 +109,24 @@ def test_dropin_spec_matches_reference(pkg, tmp_path, case, arm):
     r = json.loads([ln for ln in p.stdout.splitlines() if ln.startswith("{")][-1])
     assert r["n_answers"] == (SPECS[case]["states"] - 1) * SPECS[case]["q"]
     assert r["answers_sha256"] == SPECS[case]["oracle"], (case, arm)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("arm", ["v7", "v7b", "sess", "sess_global"])
+@pytest.mark.parametrize("seed,blocks", [(77, 64), (78, 1), (79, 256)])
+def test_dropin_synthetic_spec_all_ops(pkg, oracle_bin, tmp_path, arm, seed, blocks):
+    """A branching history with every reference op (inserts, deletes, replaces, moves, factor
+    edits) through the depth-first walk: the undo states must restore every row's hidden values
+    exactly.  Checked against the oracle's spec mode on the same files."""
+    import sys
+
+    sys.path.insert(0, GOLDEN)
+    from make_spec import write_synthetic_spec
+
+    base, spec = str(tmp_path / "syn.base"), str(tmp_path / "syn.spec")
+    write_synthetic_spec(base, spec, 1500, 3, seed)
+    ref = run_json([oracle_bin, "spec", "--fixture", base, "--spec", spec, "--out", str(tmp_path / "o.bin")])
+    r = run_json([pkg.BW_PANEL, "--fixture", base, "--spec", spec, "--arm", arm, "--blocks", str(blocks),
+                  "--answers", str(tmp_path / "a.bin"), "--json", str(tmp_path / "a.json")], timeout=300)
+    assert r["spec"]["branching"] and r["n_answers"] == ref["n_answers"]
+    assert r["answers_sha256"] == ref["answers_sha256"], (seed, arm, blocks)

@@ -7,10 +7,14 @@ import json
 import os
 import struct
 import subprocess
+import sys
 
 import pytest
 
-from conftest import fixture_file, golden, spec_files
+from conftest import GOLDEN, fixture_file, golden, spec_files
+
+sys.path.insert(0, GOLDEN)
+from make_spec import write_synthetic_spec  # noqa: E402
 
 SPECS = golden("spec_cases.json")
 
@@ -92,3 +96,23 @@ def test_panel_spec_rejects_bad_input(pkg, tmp_path):
                             "--json", str(tmp_path / "a.json"), "--plan-only"], capture_output=True, text=True,
                            timeout=120)
         assert p.returncode == 3 and msg in p.stderr, (p.returncode, p.stderr)
+
+
+def synthetic_case(tmp_path, K=1500, q=3, seed=77):
+    base, spec = str(tmp_path / "syn.base"), str(tmp_path / "syn.spec")
+    write_synthetic_spec(base, spec, K, q, seed)
+    return base, spec
+
+
+def test_synthetic_spec_all_ops(oracle_bin, pkg, tmp_path):
+    """A branching spec with every op (moves and replaces whose undo must restore hidden row
+    values): the oracle's spec mode answers it, the driver walks it (entries + exits)."""
+    base, spec = synthetic_case(tmp_path)
+    out = subprocess.run([oracle_bin, "spec", "--fixture", base, "--spec", spec, "--out", str(tmp_path / "o.bin")],
+                         capture_output=True, text=True, check=True).stdout
+    assert json.loads(out.strip().splitlines()[-1])["n_answers"] == 1499 * 3
+    p = subprocess.run([pkg.BW_PANEL, "--fixture", base, "--spec", spec, "--answers", str(tmp_path / "a.bin"),
+                        "--json", str(tmp_path / "a.json"), "--plan-only"], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    s = json.loads(p.stdout.strip().splitlines()[-1])["spec"]
+    assert s["branching"] and s["header_fields"] == 5 and 1500 < s["walk_states"] < 3000
