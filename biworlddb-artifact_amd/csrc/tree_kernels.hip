@@ -414,44 +414,53 @@ template <int kBits>
 __device__ __forceinline__ void chunk_replay_one(const BwtIndex& x, const BwtBatch& q, const int stride,
                                                  const int64_t ci, int64_t* kws) {
   const int lane = threadIdx.x & 63;
-  const int64_t c = q.wk_chunk[ci];
+  // the work item's fields are the same in every lane: marked uniform, they and the group
+  // cursors below stay in scalar registers
+  const int64_t c = uniform64(q.wk_chunk[ci]);
   int32_t* kcs = reinterpret_cast<int32_t*>(kws + 64);  // 64 group key classes
   int64_t* fen = kws + 96;
-  const int ncls = x.chain_ncls[x.chunk_chain[c]];
+  const int ncls = __builtin_amdgcn_readfirstlane(x.chain_ncls[x.chunk_chain[c]]);
   const int64_t* row = x.cp + x.chunk_cp0[c];
   for (int i = lane; i < stride; i += 64) {
     int64_t v = 0;
     if (i >= 1 && i < ncls) v = row[i] - row[i - (i & -i)];
     fen[i] = v;
   }
-  int64_t ep = q.wk_e0[ci];
-  const int64_t e1 = q.wk_e0[ci + 1];
-  const int64_t k0 = x.chunk_key0[c];
-  // keys after this work item's last endpoint are not needed
-  const int64_t kend = min(x.chunk_key0[c + 1], k0 + (int64_t)BWT_EP_POS(q.eitems[e1 - 1]));
-  const uint32_t* khi = reinterpret_cast<const uint32_t*>(x.kitems) + 1;  // little endian
-  // the items of the group starting at (e, k): every load is issued (an index past the work
+  // the work item's endpoints e0 .. e0+ne_all-1 and keys k0 .. k0+nk_all-1 (keys after its last
+  // endpoint are not needed); the cursors below are 32-bit offsets from them
+  const int64_t e0 = uniform64(q.wk_e0[ci]);
+  const int ne_all = (int)(uniform64(q.wk_e0[ci + 1]) - e0);
+  const int64_t k0 = uniform64(x.chunk_key0[c]);
+  const int nk_all = (int)(uniform64(min(x.chunk_key0[c + 1], k0 + (int64_t)BWT_EP_POS(q.eitems[e0 + ne_all - 1]))) - k0);
+  const uint64_t* eit = q.eitems + e0;
+  // offset 0 must stay readable when the work item needs no key (its chunk may end the array)
+  const int64_t kb = nk_all > 0 ? k0 : 0;
+  const uint32_t* khi = reinterpret_cast<const uint32_t*>(x.kitems + kb) + 1;  // little endian
+  const int64_t* kwt = x.kweight + kb;
+  int64_t* gout = q.g + e0;
+  // the items of the group starting at (e, k): every load is issued (an offset past the work
   // item's range is clamped to a valid one and the value ignored), so the number of loads in
   // flight is the same on every path and a use waits only for its own group's loads
-  auto load_group = [&](int64_t e, int64_t k, uint64_t& it_, uint32_t& kx_, int64_t& kw_) {
-    const int64_t ei = e + lane < e1 ? e + lane : e1 - 1;
-    const int64_t kk = k + lane < kend ? k + lane : 0;
-    it_ = q.eitems[ei];
-    kx_ = khi[2 * kk];
-    kw_ = x.kweight[kk];
+  auto load_group = [&](int e, int k, uint64_t& it_, uint32_t& kx_, int64_t& kw_) {
+    // unsigned 32-bit offsets from uniform bases: scalar-base loads, no 64-bit address math
+    const uint32_t ei = (uint32_t)min(e + lane, ne_all - 1);
+    const uint32_t kk = k + lane < nk_all ? (uint32_t)(k + lane) : 0u;
+    it_ = eit[ei];
+    kx_ = khi[2u * kk];
+    kw_ = kwt[kk];
   };
-  int64_t ki = k0;
+  int ep = 0, ki = 0;
   // One group of 64 merged items whose items are in (it, kx, kw); the next group's items are
   // loaded into (it2, kx2, kw2) as soon as its start is known.  Returns false after the last group.
   auto group = [&](const uint64_t it, const uint32_t kx, const int64_t kw, uint64_t& it2, uint32_t& kx2,
                    int64_t& kw2) -> bool {
-    const bool ev = ep + lane < e1;
+    const bool ev = ep + lane < ne_all;
     // keys of this chunk still ahead of the endpoint (>= 0: earlier ones were consumed)
-    const int rel = ev ? (int)((int64_t)BWT_EP_POS(it) - (ki - k0)) : 0;
+    const int rel = ev ? (int)BWT_EP_POS(it) - ki : 0;
     const bool take = ev && lane + rel < 64;  // its merged index falls in this group
     const int ne = __popcll(ballot64(take));  // positions are sorted: a prefix of the lanes
-    const int nk = (int)min((int64_t)(64 - ne), kend - ki);
-    const int64_t ep2 = ep + ne, ki2 = ki + nk;
+    const int nk = min(64 - ne, nk_all - ki);
+    const int ep2 = __builtin_amdgcn_readfirstlane(ep + ne), ki2 = __builtin_amdgcn_readfirstlane(ki + nk);
     load_group(ep2, ki2, it2, kx2, kw2);
     const int kcls = (int)((kx >> (BWT_CLASS_SHIFT - 32)) & 0xFFF);
     if (lane < nk) {
@@ -476,9 +485,9 @@ __device__ __forceinline__ void chunk_replay_one(const BwtIndex& x, const BwtBat
         if (m + 3 < rl && c4.w < cls) val += w23.y;
       }
       // results in work order: the taken lanes store one contiguous run
-      if (take) q.g[ep + lane] = val;
+      if (take) gout[(uint32_t)(ep + lane)] = val;
     }
-    if (ep2 >= e1) return false;
+    if (ep2 >= ne_all) return false;
     if (lane < nk)
       for (int f = kcls + 1; f <= ncls; f += f & -f) lds_add64(&fen[f], kw);
     wave_lds_sync();
@@ -502,7 +511,9 @@ template <int kBits>
 __global__ void __launch_bounds__(256) bwt_chunk_replay_kernel(const BwtIndex x, const BwtBatch q, const BwtLaunch l,
                                                                int stride) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int wv = threadIdx.x >> 6;
+  // the wavefront's index, marked uniform: the work item and everything derived from it (its
+  // chunk, cursors and bounds) then live in scalar registers and load through scalar loads
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   int64_t* kws = reinterpret_cast<int64_t*>(smem_raw) + (int64_t)wv * replay_wave_words(stride);
   const int64_t n = q.dcounts ? q.dcounts[2] : q.nwork;
   const int64_t step = (int64_t)gridDim.x * l.chunk_waves;

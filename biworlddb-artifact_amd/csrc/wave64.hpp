@@ -43,6 +43,15 @@ __device__ __forceinline__ int64_t readlane64(int64_t x, int lane) {
 
 __device__ __forceinline__ uint64_t ballot64(bool p) { return __ballot(p); }
 
+// A value every active lane holds alike (e.g. loaded from one address by the whole wavefront),
+// marked uniform so the compiler keeps it, and what is computed from it, in scalar registers.
+__device__ __forceinline__ int64_t uniform64(int64_t x) {
+  const uint64_t u = (uint64_t)x;
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)u);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(u >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
 // Orders this wavefront's LDS traffic: every LDS access issued before completes before any
 // issued after (the kernels that use it keep all cross-lane LDS dependencies inside one wave).
 __device__ __forceinline__ void wave_lds_sync() {
