@@ -84,22 +84,24 @@ __device__ __forceinline__ void block_replay_one(const BwtIndex& x, const BwtBat
   uint32_t* qtbuf = reinterpret_cast<uint32_t*>(srank + kEPT * nt);  // kQCap key tiles (carry_in_block)
   const bool add_carry = l.carry_in_block && !l.write_index;
   // the block's dense level-C row (level-C chunk b = block b); its entries stay in cache
-  const int64_t* crow = add_carry ? x.cp + x.chunk_cp0[q.qblk[bi]] : nullptr;
+  const int64_t* crow = add_carry ? x.cp + uniform64(x.chunk_cp0[q.qblk[bi]]) : nullptr;
   // key tiles of slot s's endpoints, lo | hi << 16
   auto tiles_of = [&](int64_t s) -> uint32_t {
     if (q.qtile) return q.qtile[s];
     const uint32_t id = q.qorder[s];
     return (q.qlo[id] >> x.tile_shift) | ((q.qhi[id] >> x.tile_shift) << 16);
   };
-  const int b = q.qblk[bi];
-  const int64_t k0 = x.blk_key0[b];
-  const int R = (int)(x.blk_key0[b + 1] - k0);
+  // the block's fields are the same in every thread: marked uniform, they stay in scalar registers
+  const int b = __builtin_amdgcn_readfirstlane(q.qblk[bi]);
+  const int64_t k0 = uniform64(x.blk_key0[b]);
+  const int R = (int)(uniform64(x.blk_key0[b + 1]) - k0);
   const int U = x.U;
-  const int s0 = x.blk_step0[b];
-  const int64_t gi0 = q.qblk_g0[bi], ge = q.qblk_g0[bi + 1];
+  const int s0 = __builtin_amdgcn_readfirstlane(x.blk_step0[b]);
+  const int64_t gi0 = uniform64(q.qblk_g0[bi]), ge = uniform64(q.qblk_g0[bi + 1]);
   // without the index write, nothing after the last queried step matters
-  const int s1 = l.write_index ? x.blk_step0[b + 1] : (gi0 < ge ? q.grp_step[ge - 1] + 1 : s0);
-  const int64_t qend = q.grp_q0[ge];
+  const int s1 = __builtin_amdgcn_readfirstlane(
+      l.write_index ? x.blk_step0[b + 1] : (gi0 < ge ? q.grp_step[ge - 1] + 1 : s0));
+  const int64_t qend = uniform64(q.grp_q0[ge]);
   auto stage_queries = [&](int64_t a) {
     for (int i = tid; i < kQCap; i += nt) {
       const bool in = a + i < qend;
@@ -116,14 +118,14 @@ __device__ __forceinline__ void block_replay_one(const BwtIndex& x, const BwtBat
   };
   for (int i = tid; i < span; i += nt) w[i] = 0;
   stage_groups(gi0);
-  const int64_t qbeg = q.grp_q0[gi0];
+  const int64_t qbeg = uniform64(q.grp_q0[gi0]);
   stage_queries(qbeg);
   int64_t tot[RPT];
 #pragma unroll
   for (int j = 0; j < RPT; ++j) tot[j] = 0;
   int64_t gi = gi0, gwin = gi0, qwin = qbeg;
   __syncthreads();
-  int next_q = gi < ge ? gstep[0] : -1;
+  int next_q = __builtin_amdgcn_readfirstlane(gi < ge ? gstep[0] : -1);
 
   // w -> rank totals; P[y] = sum of totals of ranks < y.  Two workgroup barriers.
   auto rebuild = [&]() {
@@ -152,7 +154,7 @@ __device__ __forceinline__ void block_replay_one(const BwtIndex& x, const BwtBat
   // answers of the query group gi (the state entered at the current step)
   auto query = [&]() {
     __syncthreads();  // every event of the steps so far is in w; previous readers are done
-    const int64_t qa = gq0[gi - gwin], qb = gq0[gi - gwin + 1];
+    const int64_t qa = uniform64(gq0[gi - gwin]), qb = uniform64(gq0[gi - gwin + 1]);
     const int64_t nq = qb - qa;
     const bool staged = nq <= kQCap;
     if (staged && qb > qwin + kQCap) {  // uniform
@@ -195,7 +197,7 @@ __device__ __forceinline__ void block_replay_one(const BwtIndex& x, const BwtBat
       stage_groups(gi);
       __syncthreads();
     }
-    next_q = gi < ge ? gstep[gi - gwin] : -1;
+    next_q = __builtin_amdgcn_readfirstlane(gi < ge ? gstep[gi - gwin] : -1);
   };
 
   if (U <= kEPT * nt) {
