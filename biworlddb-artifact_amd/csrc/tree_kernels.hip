@@ -157,15 +157,31 @@ __device__ __forceinline__ void block_replay_one(const BwtIndex& x, const BwtBat
     const int64_t qa = uniform64(gq0[gi - gwin]), qb = uniform64(gq0[gi - gwin + 1]);
     const int64_t nq = qb - qa;
     const bool staged = nq <= kQCap;
-    if (staged && qb > qwin + kQCap) {  // uniform
+    const bool fresh = staged && qb > qwin + kQCap;  // uniform
+    if (fresh) {
       qwin = qa;
       stage_queries(qa);
     }
-    rebuild();  // its barriers also publish qbuf
     // one endpoint per thread: even = lo, odd = hi of the same slot (nt is a multiple of 64,
     // so a pair never straddles a wavefront); kUnroll rounds at a time, so the carry-row loads
-    // of all of them are in flight together
+    // of all of them are in flight together.  When the group's tiles were staged by an earlier
+    // step, the first round's carry-row loads (which do not depend on the replay) are issued
+    // before the rebuild, whose barriers then hide them.  (A restaging step would have to load
+    // its tiles from global memory first: that chain, waited on before the barriers, costs more
+    // than it hides.)
     constexpr int kUnroll = 4;
+    const bool pre = add_carry && staged && !fresh;  // uniform
+    int64_t cpre[kUnroll];
+    if (pre) {
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        const int64_t j = tid + (int64_t)u * nt;
+        const bool live = j < 2 * nq;
+        const uint32_t tl = live ? qtbuf[qa - qwin + (j >> 1)] : 0u;
+        cpre[u] = live ? crow[(j & 1) ? (tl >> 16) : (tl & 0xFFFFu)] : 0;
+      }
+    }
+    rebuild();  // its barriers also publish qbuf
     const int64_t jn = (2 * nq + 63) & ~(int64_t)63;
     for (int64_t j0 = tid; j0 < jn; j0 += (int64_t)kUnroll * nt) {
       int64_t pv[kUnroll];
@@ -179,8 +195,12 @@ __device__ __forceinline__ void block_replay_one(const BwtIndex& x, const BwtBat
         const int y = (j & 1) ? (int)(rk >> 16) : (int)(rk & 0xFFFFu);
         pv[u] = live ? P[pslot<RPT>(y)] : 0;
         if (add_carry && live) {  // plus the carry of the earlier blocks' keys in lower tiles
-          const uint32_t tl = staged ? qtbuf[qa - qwin + sl] : tiles_of(qa + sl);
-          pv[u] += crow[(j & 1) ? (tl >> 16) : (tl & 0xFFFFu)];
+          if (pre && j0 == tid) {
+            pv[u] += cpre[u];
+          } else {
+            const uint32_t tl = staged ? qtbuf[qa - qwin + sl] : tiles_of(qa + sl);
+            pv[u] += crow[(j & 1) ? (tl >> 16) : (tl & 0xFFFFu)];
+          }
         }
       }
 #pragma unroll
