@@ -272,10 +272,14 @@ __device__ __forceinline__ void block_replay_one(const BwtIndex& x, const BwtBat
     __syncthreads();
     for (int t0 = s0; t0 < s1; t0 += T) {
       const int t1 = min(t0 + T, s1);
-      for (int s = t0; s < t1; ++s) {
-        const int base = (s - t0) * U;
-        for (int i = tid; i < U; i += nt) lds_add64(&w[pslot<RPT>(srank[base + i])], sdel[base + i]);
-        if (s == next_q) query();
+      // the staged events of every step up to the next query step (or the tile's end) in one
+      // pass: the adds commute, and only a query needs the steps before it complete
+      for (int s = t0; s < t1;) {
+        const int se = (next_q >= s && next_q < t1) ? next_q + 1 : t1;
+        const int base = (s - t0) * U, nev = (se - s) * U;
+        for (int i = tid; i < nev; i += nt) lds_add64(&w[pslot<RPT>(srank[base + i])], sdel[base + i]);
+        s = se;
+        if (se - 1 == next_q) query();
       }
       if (t1 >= s1) break;
       __syncthreads();  // the stage is free
