@@ -476,6 +476,9 @@ __device__ __forceinline__ void chunk_replay_one(const BwtIndex& x, const BwtBat
     kw_ = kwt[kk];
   };
   int ep = 0, ki = 0;
+  // lane -> (key kSub of a pass, Fenwick level kLev) of the tree update below
+  constexpr int kLevels = kBits + 1, kKeysPer = 64 / kLevels;
+  const int kSub = lane / kLevels, kLev = lane - kSub * kLevels;
   // One group of 64 merged items whose items are in (it, kx, kw); the next group's items are
   // loaded into (it2, kx2, kw2) as soon as its start is known.  Returns false after the last group.
   auto group = [&](const uint64_t it, const uint32_t kx, const int64_t kw, uint64_t& it2, uint32_t& kx2,
@@ -514,8 +517,17 @@ __device__ __forceinline__ void chunk_replay_one(const BwtIndex& x, const BwtBat
       if (take) gout[(uint32_t)(ep + lane)] = val;
     }
     if (ep2 >= ne_all) return false;
-    if (lane < nk)
-      for (int f = kcls + 1; f <= ncls; f += f & -f) lds_add64(&fen[f], kw);
+    // the group's keys enter the tree, one lane per (key, level): the update path of class c is
+    // (c | (2^t - 1)) + 1 for every level t with bit t of c clear (up to ncls), so kPer keys of
+    // kLevels levels each go in one pass instead of one level per pass for every key
+    for (int m0 = 0; m0 < nk; m0 += kKeysPer) {
+      const int m = m0 + kSub;
+      if (kSub < kKeysPer && m < nk) {
+        const int cm = kcs[m];
+        const int f = (cm | ((1 << kLev) - 1)) + 1;
+        if (!((cm >> kLev) & 1) && f <= ncls) lds_add64(&fen[f], kws[m]);
+      }
+    }
     wave_lds_sync();
     ep = ep2;
     ki = ki2;
