@@ -621,13 +621,13 @@ __global__ void __launch_bounds__(BWT_COMBINE_THREADS) bwt_combine_slots_kernel(
 #pragma unroll
   for (int k = 0; k < kPer; ++k) {
     const int64_t s = t0 + k * BWT_COMBINE_THREADS + threadIdx.x;
-    if (s <= last) sv[q.srank[s]] = part[k];
+    if (s <= last) sv[__builtin_nontemporal_load(q.srank + s)] = part[k];
   }
   __syncthreads();
   for (int k = threadIdx.x; k < BWT_COMBINE_TILE; k += BWT_COMBINE_THREADS) {
     const int64_t p = t0 + k;
     if (p >= q.M) break;
-    q.stage[q.spos[p]] = sv[k];
+    q.stage[__builtin_nontemporal_load(q.spos + p)] = sv[k];
   }
 }
 
