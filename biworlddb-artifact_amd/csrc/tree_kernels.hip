@@ -518,7 +518,7 @@ __device__ __forceinline__ void chunk_replay_one(const BwtIndex& x, const BwtBat
     }
     if (ep2 >= ne_all) return false;
     // the group's keys enter the tree, one lane per (key, level): the update path of class c is
-    // (c | (2^t - 1)) + 1 for every level t with bit t of c clear (up to ncls), so kPer keys of
+    // (c | (2^t - 1)) + 1 for every level t with bit t of c clear (up to ncls), so kKeysPer keys of
     // kLevels levels each go in one pass instead of one level per pass for every key
     for (int m0 = 0; m0 < nk; m0 += kKeysPer) {
       const int m = m0 + kSub;
