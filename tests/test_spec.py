@@ -116,3 +116,18 @@ def test_synthetic_spec_all_ops(oracle_bin, pkg, tmp_path):
     assert p.returncode == 0, p.stderr
     s = json.loads(p.stdout.strip().splitlines()[-1])["spec"]
     assert s["branching"] and s["header_fields"] == 5 and 1500 < s["walk_states"] < 3000
+
+
+@pytest.mark.parametrize("K,q", [(2, 1), (7, 5)])
+def test_tiny_specs(oracle_bin, pkg, tmp_path, K, q):
+    """The smallest history (one version) and a short ragged one: the oracle answers them and the
+    driver plans them."""
+    base, spec = synthetic_case(tmp_path, K=K, q=q, seed=80 + K)
+    out = subprocess.run([oracle_bin, "spec", "--fixture", base, "--spec", spec, "--out", str(tmp_path / "o.bin")],
+                         capture_output=True, text=True, check=True).stdout
+    assert json.loads(out.strip().splitlines()[-1])["n_answers"] == (K - 1) * q
+    p = subprocess.run([pkg.BW_PANEL, "--fixture", base, "--spec", spec, "--answers", str(tmp_path / "a.bin"),
+                        "--json", str(tmp_path / "a.json"), "--plan-only"], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    s = json.loads(p.stdout.strip().splitlines()[-1])["spec"]
+    assert s["versions"] == K and s["q"] == q and 1 <= s["walk_states"] - 1 <= 2 * (K - 1)
