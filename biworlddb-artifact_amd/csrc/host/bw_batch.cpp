@@ -80,12 +80,20 @@ static std::string sha_hex(const void* p, size_t n) {
 }
 
 int main(int argc, char** argv) {
-  // --form 5 only serves --stream-sha (the served form's stream over its cfg-2 tree)
-  int form = 4;
-  for (int i = 1; i + 1 < argc; ++i)
+  // --form 5 only serves --stream-sha (the served form's stream over its cfg-2 tree, or with
+  // --cfg 3 over a cfg-3 tree)
+  int form = 4, tcfg = 0;
+  for (int i = 1; i + 1 < argc; ++i) {
     if (!std::strcmp(argv[i], "--form")) form = std::atoi(argv[i + 1]);
+    if (!std::strcmp(argv[i], "--cfg")) tcfg = std::atoi(argv[i + 1]);
+  }
   if (form != 4 && form != 5) { std::fprintf(stderr, "bw_batch: --form 4|5\n"); return 2; }
-  TreeSpec spec = tree_defaults(form == 4 ? 4 : 2);
+  if (tcfg == 0) tcfg = form == 4 ? 4 : 2;
+  if ((form == 4 && tcfg != 4) || (form == 5 && tcfg != 2 && tcfg != 3)) {
+    std::fprintf(stderr, "bw_batch: --cfg 4 (batched), 2 or 3 (served stream)\n");
+    return 2;
+  }
+  TreeSpec spec = tree_defaults(tcfg);
   StreamSpec ss = stream_defaults(form);
   PlanKnobs kn;
   LaunchKnobs lk;
@@ -104,7 +112,7 @@ int main(int argc, char** argv) {
     if (i + 1 >= argc) { std::fprintf(stderr, "bw_batch: missing value for %s\n", argv[i]); return 2; }
     const char* k = argv[i];
     const char* v = argv[++i];
-    if (!std::strcmp(k, "--form")) continue;
+    if (!std::strcmp(k, "--form") || !std::strcmp(k, "--cfg")) continue;
     else if (!std::strcmp(k, "--S")) spec.S = std::atoll(v);
     else if (!std::strcmp(k, "--U")) spec.U = std::atoi(v);
     else if (!std::strcmp(k, "--N")) spec.N = std::strtoull(v, nullptr, 10);
@@ -166,6 +174,10 @@ int main(int argc, char** argv) {
   if (want_sha) {
     // with --device-gen the per-query and per-append arrays hashed are the device generator's
     uint32_t ties = 0;
+    if (dev_gen && tcfg == 3) {
+      std::fprintf(stderr, "bw_batch: the device generator makes the served stream over a config-2 tree only\n");
+      return 3;
+    }
     if (dev_gen) {
       char gerr[256];
       HIPX(hipSetDevice(0));

@@ -1,7 +1,7 @@
 // bw_serve.cpp -- the served form: an open-loop request stream answered by continuous batching
 // over a resident index, with appended branch states (one JSON line).
 //
-//   bw_serve [--S n --U n --N n --seed n] [--requests 20000 --max-batch 4096 --append-frac 0.05
+//   bw_serve [--cfg 2|3] [--S n --U n --N n --seed n] [--requests 20000 --max-batch 4096 --append-frac 0.05
 //            --append-U 128 --rate 20000] [--tuned FILE] [--slots 2] [--budget-mb 1024]
 //            [--check-alone 64] [--answers FILE] [--json FILE]
 //
@@ -126,7 +126,14 @@ size_t slot_device_bytes(const TreeEngine& E, int64_t cap) {
 }  // namespace
 
 int main(int argc, char** argv) {
-  TreeSpec spec = tree_defaults(2);
+  // the tree: config 2 (keys are their own ranks), or --cfg 3 (distinct 32-bit keys with the
+  // rank-LUT resident: every batch's raw endpoints and every appended key are ranked)
+  int tcfg = 2;
+  for (int i = 1; i + 1 < argc; ++i)
+    if (!std::strcmp(argv[i], "--cfg")) tcfg = std::atoi(argv[i + 1]);
+  if (tcfg != 2 && tcfg != 3) { std::fprintf(stderr, "bw_serve: --cfg 2|3\n"); return 2; }
+  const bool ranked = tcfg == 3;
+  TreeSpec spec = tree_defaults(tcfg);
   StreamSpec ss = stream_defaults(5);
   PlanKnobs kn;
   LaunchKnobs lk;
@@ -137,7 +144,8 @@ int main(int argc, char** argv) {
     if (i + 1 >= argc) { std::fprintf(stderr, "bw_serve: missing value for %s\n", argv[i]); return 2; }
     const char* k = argv[i];
     const char* v = argv[++i];
-    if (!std::strcmp(k, "--S")) spec.S = std::atoll(v);
+    if (!std::strcmp(k, "--cfg")) continue;
+    else if (!std::strcmp(k, "--S")) spec.S = std::atoll(v);
     else if (!std::strcmp(k, "--U")) spec.U = std::atoi(v);
     else if (!std::strcmp(k, "--N")) spec.N = std::strtoull(v, nullptr, 10);
     else if (!std::strcmp(k, "--seed")) spec.seed = std::strtoull(v, nullptr, 10);
@@ -161,6 +169,9 @@ int main(int argc, char** argv) {
   }
   std::string tuned_from = "defaults";
   if (!tuned_path.empty() && load_tuned(tuned_path.c_str(), "served", kn, lk)) tuned_from = tuned_path;
+  // the served knobs were tuned on config 2's 2^20 keys; over config 3's ranks the key-tile width
+  // is derived from the rank space instead
+  if (ranked) kn.tile_shift = 0;
 
   // ---- the tree, the stream, the resident index
   const auto g0 = clk::now();
@@ -173,8 +184,25 @@ int main(int argc, char** argv) {
   char err[256];
   HIPX(hipSetDevice(0));
   HIPX(hipFree(nullptr));
+  // ranked: the index is built over the keys' ranks; the LUT (65,537 row pointers, the low 16
+  // bits of every distinct key) stays on the device for the batches
+  RankLut lut;
+  std::vector<uint32_t> kranks;
+  const uint32_t* kcoord = w.key.data();
+  DeviceArena lutmem;
+  const uint32_t* d_row = nullptr;
+  const uint16_t* d_low = nullptr;
+  if (ranked) {
+    lut = build_rank_lut(w.dkey);
+    kranks.resize(w.key.size());
+    host_ranks(lut, w.key.data(), kranks.data(), w.key.size());
+    kcoord = kranks.data();
+    d_row = lutmem.upload(lut.row_ptr);
+    d_low = lutmem.upload(lut.low16);
+    if (!lutmem.ok()) return 5;
+  }
   TreeEngine E;
-  if (!E.init(S, spec.U, w.coord_n(), w.parent.data(), w.key.data(), w.delta.data(), kn,
+  if (!E.init(S, spec.U, w.coord_n(), w.parent.data(), kcoord, w.delta.data(), kn,
               BwtLaunch{lk.block_threads, lk.chunk_waves, 0, 0}, err, sizeof(err))) {
     std::fprintf(stderr, "bw_serve: index: %s\n", err);
     return std::strncmp(err, "device", 6) == 0 ? 5 : 3;
@@ -264,6 +292,11 @@ int main(int argc, char** argv) {
       HIPX(hipMemcpyAsync(s.d_ovq, hoq, size_t(nov) * 4, hipMemcpyHostToDevice, s.st));
       HIPX(hipMemcpyAsync(s.d_ovs, hos, size_t(nov) * 4, hipMemcpyHostToDevice, s.st));
     }
+    if (ranked) {  // raw endpoints -> ranks (#keys < x), in place
+      BwtRankArgs rl{M, s.d_lo, s.d_lo, d_row, d_low}, rh{M, s.d_hi, s.d_hi, d_row, d_low};
+      RUN(bwt_rank_remap_launch(&rl, s.st));
+      RUN(bwt_rank_remap_launch(&rh, s.st));
+    }
     HIPX(hipStreamWaitEvent(s.st, app_ready, 0));  // appended states admitted so far are on the device
     RUN(device_plan(E, s.plan, M, s.d_state, s.d_lo, s.d_hi, s.st));
     BwtLaunch l = L;
@@ -289,10 +322,21 @@ int main(int argc, char** argv) {
     s.launched = clk::now();
     plan_s += secs(p0, s.launched);
   };
+  std::vector<uint32_t> app_rank(ranked ? size_t(NA) * size_t(AU) : 0);  // ranked appended keys
   auto admit_append = [&](int64_t r) {
     const int64_t a = rs.req_app[size_t(r)];
+    const uint32_t* keys = &rs.app_key[size_t(a * AU)];
+    if (ranked) {  // an appended key is a distinct key: its rank, from the LUT row of its high bits
+      for (int i = 0; i < AU; ++i) {
+        const uint32_t x = keys[i];
+        const uint16_t* b = lut.low16.data() + lut.row_ptr[x >> 16];
+        const uint16_t* e = lut.low16.data() + lut.row_ptr[(x >> 16) + 1];
+        app_rank[size_t(a * AU) + size_t(i)] = uint32_t(std::lower_bound(b, e, uint16_t(x & 0xFFFFu)) - lut.low16.data());
+      }
+      keys = &app_rank[size_t(a * AU)];
+    }
     HIPX(hipMemcpyAsync(d_app_parent + a, &rs.app_parent[size_t(a)], 4, hipMemcpyHostToDevice, ist));
-    HIPX(hipMemcpyAsync(d_app_key + a * AU, &rs.app_key[size_t(a * AU)], size_t(AU) * 4, hipMemcpyHostToDevice, ist));
+    HIPX(hipMemcpyAsync(d_app_key + a * AU, keys, size_t(AU) * 4, hipMemcpyHostToDevice, ist));
     HIPX(hipMemcpyAsync(d_app_delta + a * AU, &rs.app_delta[size_t(a * AU)], size_t(AU) * 8, hipMemcpyHostToDevice,
                         ist));
     HIPX(hipEventRecord(app_ready, ist));
@@ -421,7 +465,7 @@ int main(int argc, char** argv) {
   char buf[4096];
   const int n = std::snprintf(
       buf, sizeof(buf),
-      "{\"arm\":\"bwgpu_served\",\"form\":5,\"S\":%" PRId64 ",\"U\":%d,\"N\":%" PRIu64 ",\"seed\":%" PRIu64
+      "{\"arm\":\"bwgpu_served\",\"form\":5,\"tree_cfg\":%d,\"rank_lut\":%s,\"S\":%" PRId64 ",\"U\":%d,\"N\":%" PRIu64 ",\"seed\":%" PRIu64
       ",\"requests\":%" PRId64 ",\"query_requests\":%" PRId64 ",\"appends\":%" PRId64 ",\"rate\":%.1f,"
       "\"max_batch\":%d,\"n_answers\":%" PRId64 ",\"answers_sha256\":\"%s\",\"slots\":%d,\"budget_mb\":%.0f,"
       "\"slot_capacity_queries\":%" PRId64 ",\"batches\":%" PRId64 ",\"mean_batch_queries\":%.1f,"
@@ -431,7 +475,7 @@ int main(int argc, char** argv) {
       "\"batch_gpu_ms\":{\"mean\":%.4f,\"p99\":%.4f},"
       "\"alone\":{\"requests\":%" PRId64 ",\"mismatch\":%" PRId64 "},\"knobs_from\":\"%s\",\"setup_s\":%.3f,"
       "\"device\":\"%s\",\"cus\":%d}\n",
-      S, spec.U, spec.N, spec.seed, R, nquery_req, nappended, ss.rate, ss.max_batch, NQ, sha.c_str(), nslots, budget_mb,
+      tcfg, ranked ? "true" : "false", S, spec.U, spec.N, spec.seed, R, nquery_req, nappended, ss.rate, ss.max_batch, NQ, sha.c_str(), nslots, budget_mb,
       cap, nbatches, nbatches ? sum_batch_q / double(nbatches) : 0.0, max_batch_q, wall, offered, double(NQ) / wall,
       pct(0.5) * 1e3, pct(0.9) * 1e3, pct(0.99) * 1e3, lat.empty() ? 0.0 : lat.back() * 1e3, plan_s,
       slots[0].in.bytes + slots[0].plan.arena.bytes, p99_late * 1e3, gpu_mean, gpu_p99, alone_checked,

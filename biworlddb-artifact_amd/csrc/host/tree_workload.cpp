@@ -221,15 +221,20 @@ void stream_generate(const TreeWorkload& tree, const StreamSpec& spec, RequestSt
   rs.app_parent.resize(size_t(napp));
   rs.app_key.resize(na);
   rs.app_delta.resize(na);
+  // over a cfg-3 tree (ranked keys): appended keys and point queries are distinct keys drawn by
+  // the tree's own Zipf(0.9) law, ranges span the 32-bit domain; else keys are their own ranks
+  const bool ranked = tree.spec.cfg == 3;
+  const uint64_t ND = ranked ? uint64_t(tree.dkey.size()) : N;
+  auto key_of = [&](uint64_t r) { return ranked ? tree.dkey[size_t(r)] : uint32_t(r); };
   std::unique_ptr<Zipf> zipf;
-  if (spec.form == 5) zipf.reset(new Zipf(N, 1.1));
+  if (spec.form == 5) zipf.reset(new Zipf(ND, ranked ? 0.9 : 1.1));
   const Stream d9(seed, 9), d10(seed, 10), d11(seed, 11), d12(seed, 12), d14(seed, 14), d15(seed, 15),
       d16(seed, 16);
   for (int64_t a = 0; a < napp; ++a) {
     rs.app_parent[size_t(a)] = int32_t(1 + int64_t(d14.at(uint64_t(a)) % uint64_t(S + a)));
     for (int i = 0; i < spec.append_U; ++i) {
       const uint64_t idx = uint64_t(a) * uint64_t(spec.append_U) + uint64_t(i);
-      rs.app_key[idx] = uint32_t(zipf->pick(unit(d15.at(idx))));
+      rs.app_key[idx] = key_of(zipf->pick(unit(d15.at(idx))));
       rs.app_delta[idx] = int64_t(d16.at(idx) % 2000001ULL) - 1000000;
     }
   }
@@ -240,11 +245,12 @@ void stream_generate(const TreeWorkload& tree, const StreamSpec& spec, RequestSt
     for (int64_t g = rs.req_q0[size_t(r)]; g < rs.req_q0[size_t(r) + 1]; ++g) {
       rs.qstate[size_t(g)] = int32_t(1 + int64_t(d9.at(uint64_t(g)) % nstates));
       if (spec.form == 5 && unit(d12.at(uint64_t(g))) < 0.5) {
-        const uint32_t k = uint32_t(zipf->pick(unit(d10.at(uint64_t(g)))));
+        const uint32_t k = key_of(zipf->pick(unit(d10.at(uint64_t(g)))));
         rs.qlo[size_t(g)] = k;
-        rs.qhi[size_t(g)] = k + 1u;
+        rs.qhi[size_t(g)] = k == 0xFFFFFFFFu ? k : k + 1u;  // [k, k+1), empty at the top value
       } else {
-        log_range(d10.at(uint64_t(g)), d11.at(uint64_t(g)), N, rs.qlo[size_t(g)], rs.qhi[size_t(g)]);
+        log_range(d10.at(uint64_t(g)), d11.at(uint64_t(g)), ranked ? 0xFFFFFFFFull : N, rs.qlo[size_t(g)],
+                  rs.qhi[size_t(g)]);
       }
     }
   }

@@ -57,7 +57,9 @@ void tree_workload_sha(const TreeWorkload& w, char out_hex[65]);
 // requests, each an append (probability append_frac: a new state under a uniform existing state
 // with append_U Zipf(1.1) updates) or a query batch of log-uniform size in [1, max_batch] over
 // the states existing at its arrival, half point queries [k, k+1) with Zipf(1.1) k, half
-// log-uniform ranges; Poisson arrivals at `rate` per second.  The exact draws are restated in
+// log-uniform ranges; Poisson arrivals at `rate` per second.  Over a cfg-3 tree the appended keys
+// and point queries are distinct keys drawn by Zipf(0.9) over the distinct set, and the ranges
+// span the 32-bit domain (raw endpoints, resolved by rank).  The exact draws are restated in
 // the test oracle's stream generator; the two are checked byte for byte by the test suite.
 struct StreamSpec {
   int form = 4;  // 4 = batched, 5 = served

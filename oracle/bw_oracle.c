@@ -26,11 +26,13 @@
  *       undo with a Fenwick tree over keys (ranks for cfg 3).  "brute" walks the ancestor chain
  *       of every state and sums in __int128 (tiny inputs only; it shares no code with "fenwick").
  *       --workload-sha prints the SHA-256 of the generated arrays instead of solving.
- *   stream --form 4|5 [--S --U --N --seed (tree)] [--requests R --max-batch M --append-frac f
+ *   stream --form 4|5 [--cfg 3] [--S --U --N --seed (tree)] [--requests R --max-batch M --append-frac f
  *          --append-U n --rate r] [--algo fenwick|brute] [--threads T] [--stream-sha] --out ANS
  *       Answers of the request streams of the batched (4) and served (5) forms (bw_stream.h)
  *       over the tree extended by the appended states, in (request, query) order: the same
- *       rollback-Fenwick DFS with queries grouped by state, or the ancestor walk ("brute").
+ *       rollback-Fenwick DFS with queries grouped by state, or the ancestor walk ("brute").  The
+ *       served form runs over a config-2 tree, or a config-3 tree with --cfg 3 (raw 32-bit keys
+ *       and endpoints; the Fenwick solvers rank them, the ancestor walk compares them raw).
  *
  * Output: answers as little-endian int64 in the reference's canonical order (state, query)
  * and one JSON line {"n_answers":..,"answers_sha256":..,"wall_s":..}.
@@ -793,8 +795,14 @@ static int mode_stream(int argc, char** argv) {
   for (int i = 2; i + 1 < argc; ++i)
     if (!strcmp(argv[i], "--form")) form = atoi(argv[i + 1]);
   if (form != 4 && form != 5) die(2, "--form must be 4 or 5");
+  /* the served form's tree: config 2 (keys are their own ranks) or config 3 (--cfg 3: distinct
+   * 32-bit keys, answered through ranks) */
+  int tcfg = form == 4 ? 4 : 2;
+  for (int i = 2; i + 1 < argc; ++i)
+    if (!strcmp(argv[i], "--cfg")) tcfg = atoi(argv[i + 1]);
+  if ((form == 4 && tcfg != 4) || (form == 5 && tcfg != 2 && tcfg != 3)) die(2, "--cfg: 4 (batched), 2 or 3 (served)");
   twl_t w;
-  twl_defaults(form == 4 ? 4 : 2, &w);
+  twl_defaults(tcfg, &w);
   twl_stream_t st;
   twl_stream_defaults(form, &st);
   const char* out = NULL;
@@ -804,7 +812,7 @@ static int mode_stream(int argc, char** argv) {
     if (!strcmp(argv[i], "--stream-sha")) { ssha = 1; continue; }
     if (i + 1 >= argc) die(2, "missing value");
     const char* k = argv[i]; const char* v = argv[++i];
-    if (!strcmp(k, "--form")) continue;
+    if (!strcmp(k, "--form") || !strcmp(k, "--cfg")) continue;
     else if (!strcmp(k, "--S")) w.S = atoll(v);
     else if (!strcmp(k, "--U")) w.U = atoi(v);
     else if (!strcmp(k, "--N")) w.N = strtoull(v, NULL, 10);
@@ -863,6 +871,15 @@ static int mode_stream(int argc, char** argv) {
     memcpy(delta + nu, st.app_delta, sizeof(int64_t) * na);
   }
   int64_t* ans = (int64_t*)calloc((size_t)(st.nq ? st.nq : 1), sizeof(int64_t));
+  if (tcfg == 3 && strcmp(algo, "brute")) {
+    /* the Fenwick solvers work on ranks: every key is a distinct key (its index), an endpoint x
+     * becomes #keys < x; the brute walk compares raw values and needs neither */
+    for (size_t i = 0; i < nu + na; ++i) key[i] = (uint32_t)lower_bound_u32(w.dkey, w.N, key[i]);
+    for (int64_t g = 0; g < st.nq; ++g) {
+      st.qlo[g] = (uint32_t)lower_bound_u32(w.dkey, w.N, st.qlo[g]);
+      st.qhi[g] = (uint32_t)lower_bound_u32(w.dkey, w.N, st.qhi[g]);
+    }
+  }
   const double ts0 = now_s();
   if (!strcmp(algo, "brute"))
     solve_states_brute(parent, upd0, key, delta, st.nq, st.qstate, st.qlo, st.qhi, ans);

@@ -10,7 +10,7 @@
  *   size_b = 1 + bounded(draw(8,b), max_batch)                       (uniform in [1, max_batch])
  * queries; query g (global index, batches in order) asks
  *   state = 1 + bounded(draw(9,g), S),   range = loglen(draw(10,g), draw(11,g), N)  (as cfg 2).
- * Served form (over a config-2 tree): nreq requests; request r is an append when
+ * Served form (over a config-2 tree, or a config-3 tree: see twl_stream_generate): nreq requests; request r is an append when
  *   u01(draw(13,r)) < append_frac, else a query batch of log-uniform size
  *   size_r = clamp(floor(exp(u01(draw(8,r)) * ln(max_batch))), 1, max_batch).
  * Append a (a-th append) creates state S+a+1 with parent 1 + bounded(draw(14,a), S+a) and
@@ -104,15 +104,19 @@ static int twl_stream_generate(const twl_t* w, twl_stream_t* s) {
   s->app_key = (uint32_t*)malloc(sizeof(uint32_t) * (na ? na : 1));
   s->app_delta = (int64_t*)malloc(sizeof(int64_t) * (na ? na : 1));
   if (!s->qstate || !s->qlo || !s->qhi || !s->app_parent || !s->app_key || !s->app_delta) return -1;
+  /* over a cfg-3 tree: appended keys and point queries are distinct keys by the tree's Zipf(0.9),
+   * ranges over the 32-bit domain; else keys are their own ranks */
+  const int ranked = w->cfg == 3;
   double* cum = NULL;
-  if (s->form == 5) cum = twl_zipf_table(N, 1.1);
+  if (s->form == 5) cum = twl_zipf_table(N, ranked ? 0.9 : 1.1);
   int64_t a = 0;
   for (int64_t r = 0; r < s->nreq; ++r) {
     if (s->req_kind[r]) {
       s->app_parent[a] = (int32_t)(1 + (int64_t)(twl_draw(seed, 14, (uint64_t)a) % (uint64_t)(S + a)));
       for (int i = 0; i < s->append_U; ++i) {
         const uint64_t idx = (uint64_t)a * (uint64_t)s->append_U + (uint64_t)i;
-        s->app_key[idx] = (uint32_t)twl_zipf_pick(cum, N, twl_u01(twl_draw(seed, 15, idx)));
+        const uint64_t pk = twl_zipf_pick(cum, N, twl_u01(twl_draw(seed, 15, idx)));
+        s->app_key[idx] = ranked ? w->dkey[pk] : (uint32_t)pk;
         s->app_delta[idx] = (int64_t)(twl_draw(seed, 16, idx) % 2000001ULL) - 1000000;
       }
       ++a;
@@ -122,11 +126,13 @@ static int twl_stream_generate(const twl_t* w, twl_stream_t* s) {
     for (int64_t g = s->req_q0[r]; g < s->req_q0[r + 1]; ++g) {
       s->qstate[g] = (int32_t)(1 + (int64_t)(twl_draw(seed, 9, (uint64_t)g) % nstates));
       if (s->form == 5 && twl_u01(twl_draw(seed, 12, (uint64_t)g)) < 0.5) {
-        const uint32_t k = (uint32_t)twl_zipf_pick(cum, N, twl_u01(twl_draw(seed, 10, (uint64_t)g)));
+        const uint64_t pk = twl_zipf_pick(cum, N, twl_u01(twl_draw(seed, 10, (uint64_t)g)));
+        const uint32_t k = ranked ? w->dkey[pk] : (uint32_t)pk;
         s->qlo[g] = k;
-        s->qhi[g] = k + 1u;
+        s->qhi[g] = k == 0xFFFFFFFFu ? k : k + 1u; /* [k, k+1), empty at the top value */
       } else {
-        twl_loglen_range(twl_draw(seed, 10, (uint64_t)g), twl_draw(seed, 11, (uint64_t)g), N, &s->qlo[g], &s->qhi[g]);
+        twl_loglen_range(twl_draw(seed, 10, (uint64_t)g), twl_draw(seed, 11, (uint64_t)g),
+                         ranked ? 0xFFFFFFFFull : N, &s->qlo[g], &s->qhi[g]);
       }
     }
   }

@@ -51,6 +51,12 @@ STREAM_CASES = [
      ["--requests", "2000", "--max-batch", "1024", "--append-U", "16", "--rate", "5000"], ()),
     ("S4_batched_full", 4, None, [], ()),
     ("S5_served_full", 5, None, [], ()),
+    # the served form over a config-3 tree: raw 32-bit keys and endpoints through the rank-LUT
+    ("s6_served_lut_small", 5, (300, 8, 4096, None),
+     ["--cfg", "3", "--requests", "200", "--max-batch", "64", "--append-U", "8", "--rate", "1000"], ("brute",)),
+    ("s7_served_lut_mid", 5, (20000, 16, 65536, None),
+     ["--cfg", "3", "--requests", "2000", "--max-batch", "1024", "--append-U", "16", "--rate", "5000"], ()),
+    ("S6_served_lut_full", 5, None, ["--cfg", "3"], ()),
 ]
 
 REF_CASES = [

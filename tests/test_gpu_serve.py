@@ -22,11 +22,17 @@ def _cmd(pkg, g, extra):
     ("s3_served_small", ["--slots", "1", "--check-alone", "8"]),
     ("s4_served_mid", ["--check-alone", "32"]),
     ("s4_served_mid", ["--slots", "4", "--budget-mb", "64", "--rate", "1e9", "--check-alone", "8"]),
+    # over a config-3 tree: raw 32-bit endpoints ranked on the device per batch, appended keys at
+    # admission
+    ("s6_served_lut_small", ["--check-alone", "16"]),
+    ("s7_served_lut_mid", ["--check-alone", "32"]),
+    ("s7_served_lut_mid", ["--slots", "3", "--budget-mb", "64", "--rate", "1e9", "--check-alone", "8"]),
 ])
 def test_served_matches_oracle_and_alone(pkg, case, extra):
     g = STREAMS[case]
     r = run_json(_cmd(pkg, g, extra), timeout=300)
     assert r["answers_sha256"] == g["oracle"], (case, extra)
+    assert r["rank_lut"] == ("--cfg" in g["args"])
     assert r["n_answers"] == g["nq"] and r["appends"] == g["napp"]
     assert r["alone"]["mismatch"] == 0 and r["alone"]["requests"] > 0
     assert r["latency_ms"]["p99"] >= r["latency_ms"]["p50"] > 0
@@ -38,4 +44,14 @@ def test_served_full_stream(pkg):
     g = STREAMS["S5_served_full"]
     r = run_json(_cmd(pkg, g, ["--check-alone", "32"]), timeout=600)
     assert r["answers_sha256"] == g["oracle"]
+    assert r["alone"]["mismatch"] == 0
+
+
+@pytest.mark.gpu
+def test_served_full_stream_rank_lut(pkg):
+    """The served stream over the config-3 tree (300,000 states x 256 updates, 4,194,304 distinct
+    32-bit keys; 9.2M queries, 1,005 appended states), its endpoints ranked on the device."""
+    g = STREAMS["S6_served_lut_full"]
+    r = run_json(_cmd(pkg, g, ["--check-alone", "32"]), timeout=600)
+    assert r["answers_sha256"] == g["oracle"] and r["rank_lut"]
     assert r["alone"]["mismatch"] == 0
