@@ -96,7 +96,11 @@ bool zipf_ok(const BwtGenSpec* g) {
 }
 
 // Range bound and draw streams of the log-uniform ranges of a query launch.
-uint64_t range_bound(const BwtGenSpec* g, int what) { return what == 0 && g->cfg == 3 ? 0xFFFFFFFFull : g->N; }
+// ranges over a cfg-3 tree (its own queries and its streams') span the 32-bit domain
+uint64_t range_bound(const BwtGenSpec* g, int what) {
+  (void)what;
+  return g->cfg == 3 ? 0xFFFFFFFFull : g->N;
+}
 
 }  // namespace
 
@@ -140,7 +144,7 @@ extern "C" int bwt_gen_stream_launch(const BwtGenSpec* g, int32_t form, const in
     return int(hipErrorInvalidValue);
   if (nq >= (int64_t(1) << 32) || (nq > 0 && (!qstate || !qlo || !qhi))) return int(hipErrorInvalidValue);
   if (form == 5 && (!req_q0 || !req_app || !zipf_ok(g))) return int(hipErrorInvalidValue);
-  const double log_n = std::log(double(g->N));
+  const double log_n = std::log(double(range_bound(g, 1)));
   hipStream_t st = hipStream_t(stream);
   hipLaunchKernelGGL(gen_zero_kernel, dim3(1), dim3(1), 0, st, ties.count);
   if (nq > 0)

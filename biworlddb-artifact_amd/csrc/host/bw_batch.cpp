@@ -174,17 +174,13 @@ int main(int argc, char** argv) {
   if (want_sha) {
     // with --device-gen the per-query and per-append arrays hashed are the device generator's
     uint32_t ties = 0;
-    if (dev_gen && tcfg == 3) {
-      std::fprintf(stderr, "bw_batch: the device generator makes the served stream over a config-2 tree only\n");
-      return 3;
-    }
     if (dev_gen) {
       char gerr[256];
       HIPX(hipSetDevice(0));
       RequestStream scan;
       stream_scan(w.spec, ss, scan);
       DeviceStream G;
-      if (!device_generate_stream(w.spec, scan, G, nullptr, gerr, sizeof(gerr))) {
+      if (!device_generate_stream(w.spec, scan, G, nullptr, gerr, sizeof(gerr), w.dkey)) {
         std::fprintf(stderr, "bw_batch: device generation: %s\n", gerr);
         return 5;
       }
@@ -232,7 +228,7 @@ int main(int argc, char** argv) {
       const auto g0 = clk::now();
       RequestStream scan;
       stream_scan(w.spec, ss, scan);
-      if (!device_generate_stream(w.spec, scan, DS, nullptr, err, sizeof(err))) {
+      if (!device_generate_stream(w.spec, scan, DS, nullptr, err, sizeof(err), w.dkey)) {
         std::fprintf(stderr, "bw_batch: device generation: %s\n", err);
         return 5;
       }
@@ -500,7 +496,7 @@ int main(int argc, char** argv) {
     stream_scan(w.spec, ss, scan);
     DeviceStream DS2;
     if (!device_generate_tree(spec, w.dkey, DT, st, err, sizeof(err)) ||
-        !device_generate_stream(w.spec, scan, DS2, st, err, sizeof(err))) {
+        !device_generate_stream(w.spec, scan, DS2, st, err, sizeof(err), w.dkey)) {
       std::fprintf(stderr, "bw_batch: device generation: %s\n", err);
       return 5;
     }

@@ -153,12 +153,14 @@ BWT_HD inline bool gen_stream_query_at(const BwtGenSpec& g, const GenStreams& s,
   qstate = int32_t(1 + int64_t(s.s9.at(uint64_t(q)) % nstates));
   const uint64_t a10 = s.s10.at(uint64_t(q));
   if (form == 5 && unit(s.s12.at(uint64_t(q))) < 0.5) {
-    lo = uint32_t(zipf_pick(g.zipf_cum, g.zipf_n, g.zipf_guide, a10));
-    hi = lo + 1u;
+    // a point query [k, k+1): over a cfg-3 tree k is a distinct key (empty at the top value)
+    const uint64_t r = zipf_pick(g.zipf_cum, g.zipf_n, g.zipf_guide, a10);
+    lo = g.cfg == 3 ? g.dkey[r] : uint32_t(r);
+    hi = lo == 0xFFFFFFFFu ? lo : lo + 1u;
     return false;
   }
   const double e = ex(unit(a10) * log_n);
-  range_from_exp(e, s.s11.at(uint64_t(q)), g.N, lo, hi);
+  range_from_exp(e, s.s11.at(uint64_t(q)), g.cfg == 3 ? 0xFFFFFFFFull : g.N, lo, hi);
   return near_integer(e);
 }
 
@@ -166,7 +168,8 @@ BWT_HD inline bool gen_stream_query_at(const BwtGenSpec& g, const GenStreams& s,
 BWT_HD inline void gen_append_at(const BwtGenSpec& g, const GenStreams& s, int64_t napp, int64_t i,
                                  int32_t* app_parent, uint32_t* app_key, int64_t* app_delta) {
   if (i < napp) app_parent[i] = int32_t(1 + int64_t(s.a14.at(uint64_t(i)) % uint64_t(g.S + i)));
-  app_key[i] = uint32_t(zipf_pick(g.zipf_cum, g.zipf_n, g.zipf_guide, s.a15.at(uint64_t(i))));
+  const uint64_t r = zipf_pick(g.zipf_cum, g.zipf_n, g.zipf_guide, s.a15.at(uint64_t(i)));
+  app_key[i] = g.cfg == 3 ? g.dkey[r] : uint32_t(r);  // over a cfg-3 tree: a distinct key
   app_delta[i] = int64_t(s.a16.at(uint64_t(i)) % 2000001ULL) - 1000000;
 }
 

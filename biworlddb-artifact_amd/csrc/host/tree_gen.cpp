@@ -85,7 +85,8 @@ bool device_generate_tree(const TreeSpec& spec, const std::vector<uint32_t>& dke
 }
 
 bool device_generate_stream(const TreeSpec& tree, const RequestStream& scan, DeviceStream& out, hipStream_t st,
-                            char* err, size_t errlen) {
+                            char* err, size_t errlen,
+                            const std::vector<uint32_t>& dkey) {
   out.arena.release();
   out.arena.err = hipSuccess;
   const StreamSpec& ss = scan.spec;
@@ -105,10 +106,17 @@ bool device_generate_stream(const TreeSpec& tree, const RequestStream& scan, Dev
   g.N = tree.N;
   g.seed = tree.seed;
   if (ss.form == 5) {
-    if (!device_zipf(tree.N, 1.1, z, st, err, errlen)) return false;
+    // over a cfg-3 tree the stream draws distinct keys by the tree's own Zipf(0.9) law
+    const bool ranked = tree.cfg == 3;
+    if (ranked && uint64_t(dkey.size()) != tree.N) {
+      if (err && errlen) std::snprintf(err, errlen, "stream generation: the cfg-3 tree's distinct keys are missing");
+      return false;
+    }
+    if (!device_zipf(tree.N, ranked ? 0.9 : 1.1, z, st, err, errlen)) return false;
     g.zipf_cum = z.cum;
     g.zipf_n = tree.N;
     g.zipf_guide = z.guide;
+    if (ranked) g.dkey = out.arena.upload(dkey);
   }
   const int64_t* req_q0 = ss.form == 5 ? out.arena.upload(scan.req_q0) : nullptr;
   const int64_t* req_app = ss.form == 5 ? out.arena.upload(scan.req_app) : nullptr;

@@ -49,7 +49,8 @@ struct DeviceStream {
   DeviceArena arena;
 };
 bool device_generate_stream(const TreeSpec& tree, const RequestStream& scan, DeviceStream& out, hipStream_t st,
-                            char* err, size_t errlen);
+                            char* err, size_t errlen,
+                            const std::vector<uint32_t>& dkey = {});
 
 // Elements of a device array that differ from the host vector (all of them if the copy fails).
 template <typename T>

@@ -18,8 +18,7 @@ def _stream_cmd(g):
 
 
 @pytest.mark.gpu
-# the device generator makes the served stream over a config-2 tree only (bw_batch refuses --cfg 3)
-@pytest.mark.parametrize("case", [k for k in STREAMS if "--cfg" not in STREAMS[k]["args"]])
+@pytest.mark.parametrize("case", list(STREAMS))
 def test_device_generated_stream_hashes_to_golden(pkg, case):
     g = STREAMS[case]
     r = run_json([pkg.BW_BATCH] + _stream_cmd(g) + ["--stream-sha", "--device-gen"], timeout=300)
