@@ -61,7 +61,9 @@ int bwt_gen_state_queries_launch(const BwtGenSpec* g, uint32_t* qlo, uint32_t* q
 
 /* The queries of a request stream over the tree (form 4 batched, 5 served): req_q0[nreq+1] and,
  * for the served form, req_app[nreq] come from the host scan (stream_scan); nq = req_q0[nreq].
- * Writes qstate, qlo, qhi[nq]. */
+ * Writes qstate, qlo, qhi[nq].  For the served form g->zipf_* is the key law of its points and
+ * appended keys: Zipf(1.1) over the N keys of a cfg-2 tree, or over a cfg-3 tree the tree's own
+ * Zipf(0.9) over its distinct keys (g->dkey), its ranges then spanning the 32-bit domain. */
 int bwt_gen_stream_launch(const BwtGenSpec* g, int32_t form, const int64_t* req_q0, const int64_t* req_app,
                           int64_t nreq, int64_t nq, int32_t* qstate, uint32_t* qlo, uint32_t* qhi, BwtGenTies ties,
                           void* stream);
