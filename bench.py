@@ -24,8 +24,10 @@ Beside it, measured in the same run:
              batch;
   served  -- the served form (bw_serve): 20,000 Poisson requests (log-uniform sizes in [1, 4096],
              half point / half range queries, 5% appended branch states) by continuous batching
-             with every batch planned on the GPU, p50/p99 request latency; and the same stream
-             offered all at once (saturated throughput);
+             with every batch planned on the GPU, p50/p99 request latency; the same stream
+             offered all at once (saturated throughput); and the same request law over the
+             config-3 tree (4,194,304 distinct 32-bit keys) with its rank-LUT resident, every
+             batch's raw endpoints ranked on the GPU (served_rank_lut);
   roofline -- the batched step's dominant kernel: its algorithmic bytes per launch over its
              average HIP-event duration against the MI355X HBM peak (8 TB/s); `traffic` is the
              rocprofv3 FETCH_SIZE + WRITE_SIZE of that kernel (profiles/), scaled by the same
@@ -156,6 +158,12 @@ def main():
         z = run([BW_SERVE, "--check-alone", "0", "--rate", "1e9"] + knobs, 600)
         out["served_saturated"] = {"queries_per_s": round(z["queries_per_s"], 1), "latency_ms": z["latency_ms"],
                                    "batches": z["batches"], "answers_sha256": z["answers_sha256"]}
+        # the same request law over the config-3 tree, its raw endpoints ranked through the LUT
+        u = run([BW_SERVE, "--cfg", "3", "--check-alone", "16"] + knobs, 600)
+        out["served_rank_lut"] = {"queries_per_s": round(u["queries_per_s"], 1),
+                                  "offered_queries_per_s": round(u["offered_queries_per_s"], 1),
+                                  "latency_ms": u["latency_ms"], "requests": u["requests"], "appends": u["appends"],
+                                  "answers_sha256": u["answers_sha256"], "alone_mismatch": u["alone"]["mismatch"]}
     if not a.no_cpu:
         try:
             threads = len(os.sched_getaffinity(0))
