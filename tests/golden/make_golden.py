@@ -19,12 +19,15 @@ import json
 import os
 import subprocess
 import sys
+import tempfile
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(os.path.dirname(HERE))
 ORACLE = os.path.join(REPO, "oracle", "bw_oracle")
 REF = os.path.join(REPO, "oracle", "_ref")
-WORK = os.path.join(REPO, "_work", "golden")
+# scratch files (solver dumps, answer files) and the regenerated fixtures live in the system temp dir
+WORK = os.path.join(tempfile.gettempdir(), "bwgpu_golden")
+FIXTURES = os.path.join(tempfile.gettempdir(), "bwgpu_fixtures")
 
 TREE_CASES = [
     # name, cfg, S, U, Q, N, seed, solvers
@@ -146,7 +149,7 @@ def fixture_path(fx):
     sys.path.insert(0, HERE)
     from make_fixture import make_fixture
 
-    d = os.path.join(REPO, "_work", "fixtures") if os.path.isdir(os.path.join(REPO, "_work")) else "/tmp/bwgpu_fixtures"
+    d = FIXTURES
     os.makedirs(d, exist_ok=True)
     p = os.path.join(d, "fx_%d_%d_%d_%d.bin" % tuple(fx))
     if not os.path.exists(p):
