@@ -1,72 +1,147 @@
 """Benchmark: exact range-sum queries/s at 300,000 states -- the batched form against the plain GPU
 drop-in, with the served form's p50/p99 latency -- on one MI355X.
 
-Headline (`value`): the batched form.  A request stream of 100,000 query batches (sizes uniform in
-[1, 2048], 102,546,012 queries over uniform states, log-uniform ranges) over the 300,000-state
-tree with uneven chains (geometric runs, mean 64, max 4,096; 128 Zipf(1.1) point updates per state,
-N = 2^20 keys) is answered by ONE replay of a hipGraph (block replay, level-B chunk replay,
-combine) over the index resident in HBM.  A "step" is one such replay over inputs already in HBM;
-bw_batch times each with HIP events on its stream.  The answers' SHA-256 is reported (the test
-suite pins it against the oracle).
+Headline (`value`): the batched form, planning included.  A request stream of 100,000 query
+batches (sizes uniform in [1, 2048], 102,546,012 queries over uniform states, log-uniform ranges)
+over the 300,000-state tree with uneven chains (geometric runs, mean 64, max 4,096; 128 Zipf(1.1)
+point updates per state, N = 2^20 keys).  A "step" is one pass of the hot path over the whole
+stream, its queries already in HBM: the stream is planned on the device (one radix sort of the
+queries by the preorder rank of their state) and answered by ONE launch of a persistent
+block-replay kernel (captured in a hipGraph) whose workgroups take blocks from a device work
+queue, over the dense-carry index resident in HBM (one carry row of 2^20+1 int64 per block of up to
+4,096 keys, built once; its build time is reported beside).  bw_batch times each step with one
+pair of HIP events on its stream; the answers' SHA-256 must equal the two-level form's (which the
+test suite pins against the oracle).
 
-The stream's queries are materialised on the GPU (bwt_gen_*: the host generator's arrays, element
-for element; its time is reported beside the host generator's), the index is also built on the GPU
-once more (bwt_index_*: equal to the host-built index; its build time beside the host's, and the
-stream answered over it must give the same SHA-256; and once more the whole pipeline runs on the
-GPU from the seed, timed phase by phase: `all_on_device`), the batch is planned on the GPU
-(bwt_plan_launch: the host planner's plan, array for array; its time is reported beside the
-replay), and the replay runs over that device-generated, device-planned batch.
-
-Beside it, measured in the same run:
-  plain   -- the plain GPU drop-in at paper scale (config 2: 300,000 states, 9.6M queries, one
-             launch sequence that rebuilds the index and answers the batch, with its own tuned
-             knobs) and, on the batched stream's first batches, the same plain pass run once per
-             batch;
-  served  -- the served form (bw_serve): 20,000 Poisson requests (log-uniform sizes in [1, 4096],
-             half point / half range queries, 5% appended branch states) by continuous batching
-             with every batch planned on the GPU, p50/p99 request latency; the same stream
-             offered all at once (saturated throughput); and the same request law over the
-             config-3 tree (4,194,304 distinct 32-bit keys) with its rank-LUT resident, every
-             batch's raw endpoints ranked on the GPU (served_rank_lut);
-  roofline -- the batched step's dominant kernel: its algorithmic bytes per launch over its
-             average HIP-event duration against the MI355X HBM peak (8 TB/s); `traffic` is the
-             rocprofv3 FETCH_SIZE + WRITE_SIZE of that kernel (profiles/), scaled by the same
-             counters' reading for a copy of known size, when that summary exists;
-  extract -- the answers of the plain config-2 run (9.6M) leaving the device and hashed, one step
-             after another as the reference workers do, and overlapped (bw_extract_answers);
-  cpu_baseline -- the oracle's rollback-Fenwick stream solver on the host's cores (the DFS cut into
-             one contiguous piece per OpenMP thread, up to 16) over the same whole stream; its
-             answers' SHA-256 must equal the GPU's.
+Beside it, measured in the same run (every figure planning-inclusive unless named otherwise):
+  two_level -- the same stream over the two-level carry index (blocks of 1,024 keys, level-C rows,
+               level-B chunk chains): device plan + one hipGraph replay of block replay, chunk
+               replay and combine, per step; its replay alone as a sub-field;
+  plain     -- the plain GPU drop-in at paper scale (config 2: 300,000 states, 9.6M queries, one
+               launch sequence that plans the batch on the device, rebuilds the index and
+               answers), the same for config 3 (4,194,304 distinct 32-bit keys through the
+               rank-LUT), and on the batched stream's first 64 batches the plain pass once per
+               batch (host plan, outside the timed region);
+  dropin    -- the reference-ABI kernels (gpu_v7b.h / gpu_session.h arms of bw_panel) on the
+               reference's own 300,000-state workload R1 (chain, factor period 37, q = 2 per
+               state: 599,998 answers), kernel time and execute phase, SHA-256 = the reference
+               CPU engine's;
+  reference_cpu -- that CPU engine itself (src/panel_batch_v7fix.cpp built from the reference
+               sources into oracle/_ref/, OpenMP, B = threads) on R1, on the host's cores;
+  served    -- the served form (bw_serve): 20,000 Poisson requests (log-uniform sizes in [1, 4096],
+               half point / half range queries, 5% appended branch states) by continuous batching,
+               p50/p99 request latency, at several offered rates up to saturation; and the same
+               request law over the config-3 tree with its rank-LUT resident (served_rank_lut);
+  roofline  -- the compulsory bytes of a step (20 B per query: state, two endpoints, answer; 12 B
+               per update: key, delta) over the persistent replay's average HIP-event time, against
+               the MI355X HBM peak (8 TB/s); the whole step's figure beside it; `traffic` is the
+               rocprofv3 FETCH_SIZE + WRITE_SIZE of that kernel per launch (profiles/), scaled by the
+               same counters' reading for a copy of known size, when that summary exists;
+  cpu_baseline -- the oracle's rollback-Fenwick stream solver (a port, not the reference engine:
+               that engine cannot take branching multi-update trees) on the host's cores over the
+               same whole stream; its answers' SHA-256 must equal the GPU's.
 """
 import argparse
+import importlib.util
 import json
 import os
 import subprocess
 import sys
+import tempfile
 
 REPO = os.path.dirname(os.path.abspath(__file__))
 BUILD = os.path.join(REPO, "biworlddb-artifact_amd", "build")
 BW_BATCH = os.path.join(BUILD, "bw_batch")
 BW_TREE = os.path.join(BUILD, "bw_tree")
 BW_SERVE = os.path.join(BUILD, "bw_serve")
+BW_PANEL = os.path.join(BUILD, "bw_panel")
 TUNED = os.path.join(REPO, "biworlddb-artifact_amd", "tuned.json")
 ORACLE = os.path.join(REPO, "oracle", "bw_oracle")
+REF_ENGINE = os.path.join(REPO, "oracle", "_ref", "compiled_300000")
+REF_CASES = os.path.join(REPO, "tests", "golden", "ref_cases.json")
+MAKE_FIXTURE = os.path.join(REPO, "tests", "golden", "make_fixture.py")
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_FILE = os.path.join(REPO, "profiles", "traffic_batched.json")
-CPU_MAX_THREADS = 16  # the host share of one GPU
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "traffic_dense.json")
+SERVED_RATES = (20000.0, 80000.0, 320000.0)  # offered request rates (req/s), then saturation
 
 
 def last_json(text):
     return json.loads([ln for ln in text.splitlines() if ln.startswith("{")][-1])
 
 
-def run(cmd, timeout):
+def run(cmd, timeout, env=None):
     print("bench: " + " ".join(os.path.basename(c) if i == 0 else c for i, c in enumerate(cmd)), file=sys.stderr,
           flush=True)
-    p = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
+    p = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout, env=env)
     if p.returncode != 0:
         sys.exit(f"bench.py: {os.path.basename(cmd[0])} failed ({p.returncode}): {p.stderr[-2000:]}")
     return last_json(p.stdout)
+
+
+def host_threads():
+    """Host cores this process may use: its CPU affinity, bounded by OMP_NUM_THREADS when set."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def r1_case(tmp):
+    """The reference's own 300,000-state workload R1 and its fixture (regenerated from parameters)."""
+    with open(REF_CASES) as f:
+        g = json.load(f)["R1_chain_300k_fp37_q2"]
+    spec = importlib.util.spec_from_file_location("make_fixture", MAKE_FIXTURE)
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    path = os.path.join(tmp, "r1_fixture.bin")
+    mod.make_fixture(path, *g["fixture_params"])
+    args = ["--fixture", path, "--topology", str(g["topology"]), "--seed", str(g["seed"]), "--factor-period",
+            str(g["factor_period"]), "--q", str(g["q"])]
+    return g, args
+
+
+def dropin_legs(tmp, threads):
+    g, args = r1_case(tmp)
+    out = {"workload": "R1: chain of 300,000 states, one edit each, factor period 37, q = 2 per-state ranges "
+                       "(599,998 answers), reference-native"}
+    for arm in ("v7b", "sess"):
+        j = os.path.join(tmp, f"p_{arm}.json")
+        r = run([BW_PANEL] + args + ["--states", str(g["states"]), "--arm", arm, "--carry", "gpu", "--blocks", "64",
+                                     "--threads", "256", "--per-state", "--answers", os.path.join(tmp, "p.bin"),
+                                     "--json", j], 600)
+        if r["answers_sha256"] != g["compiled"]:
+            sys.exit(f"bench.py: drop-in {arm} answers {r['answers_sha256']} != reference {g['compiled']}")
+        ph, gp = r["phases_ns"], r["gpu"]
+        out[arm] = {"kernel_ms": round(gp["kernel_ns"] / 1e6, 4), "execute_ms": round(ph["execute"] / 1e6, 4),
+                    "upload_ms": round(gp["upload_ns"] / 1e6, 4),
+                    "kernel_queries_per_s": round(r["n_answers"] / (gp["kernel_ns"] * 1e-9), 1),
+                    "execute_queries_per_s": round(r["n_answers"] / (ph["execute"] * 1e-9), 1),
+                    "full_ms": round(r["full_ns"] / 1e6, 3), "blocks": 64, "answers_sha256": r["answers_sha256"]}
+    ref = None
+    if os.path.exists(REF_ENGINE):
+        j = os.path.join(tmp, "ref.json")
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), BWH_RANGE_SCOPE="per_state")
+        print(f"bench: compiled_300000 (OMP_NUM_THREADS={threads})", file=sys.stderr, flush=True)
+        p = subprocess.run([REF_ENGINE] + args + ["--answers", os.path.join(tmp, "ref.bin"), "--json", j],
+                           capture_output=True, text=True, timeout=600, env=env)
+        if p.returncode != 0:
+            sys.exit(f"bench.py: reference engine failed: {p.stderr[-2000:]}")
+        with open(j) as f:
+            c = json.load(f)
+        if c["answers_sha256"] != g["compiled"]:
+            sys.exit("bench.py: reference engine answers differ from its golden hash")
+        ph = c["phases_ns"]
+        ref = {"kind": "reference", "binary": "oracle/_ref/compiled_300000 (src/panel_batch_v7fix.cpp)",
+               "mode": c["mode"], "threads": threads, "execute_ms": round(ph["execute"] / 1e6, 3),
+               "setup_ms": round(ph["setup"] / 1e6, 3), "full_ms": round(c["full_ns"] / 1e6, 3),
+               "execute_queries_per_s": round(c["n_answers"] / (ph["execute"] * 1e-9), 1),
+               "full_queries_per_s": round(c["n_answers"] / (c["full_ns"] * 1e-9), 1),
+               "answers_sha256": c["answers_sha256"]}
+    return out, ref
 
 
 def main():
@@ -81,60 +156,84 @@ def main():
         print("bench.py: measures one GPU only (no multi-GPU run is made); the path would shard by contiguous "
               "block ranges of the Euler walk with one exclusive scan of per-GPU carry totals.", file=sys.stderr)
         sys.exit(2)
-    for exe in (BW_BATCH, BW_TREE, BW_SERVE):
+    for exe in (BW_BATCH, BW_TREE, BW_SERVE, BW_PANEL):
         if not os.path.exists(exe):
             sys.exit("bench.py: native build missing (make -C biworlddb-artifact_amd)")
     knobs = ["--tuned", TUNED] if os.path.exists(TUNED) else []
-    r = run([BW_BATCH, "--warmup", str(a.warmup), "--repeat", str(a.steps), "--plain-sample", "4", "--stages",
-             "--device-plan", "--device-gen", "--device-index", "--device-all"] + knobs, 1200)
-    plain = run([BW_TREE, "--cfg", "2", "--mode", "pass", "--warmup", "2", "--repeat", "5", "--extract", "both"] +
-                knobs, 600)
-    stages = {k: v for k, v in r["stage_ms"].items() if v > 0}
-    dom = max(stages, key=stages.get)
-    achieved = r["stage_bytes"][dom] / (stages[dom] * 1e-3) / 1e9
+    threads = host_threads()
+    r = run([BW_BATCH, "--warmup", str(a.warmup), "--repeat", str(a.steps), "--plain-sample", "64", "--stages",
+             "--dense", "--device-gen", "--device-index", "--device-all"] + knobs, 1200)
+    d = r["dense"]
+    if d["mismatch"] != 0 or d["answers_sha256"] != r["answers_sha256"]:
+        sys.exit("bench.py: dense-carry answers differ from the two-level form's")
+    plain = run([BW_TREE, "--cfg", "2", "--mode", "pass", "--device-plan", "--warmup", "2", "--repeat", "5",
+                 "--extract", "both"] + knobs, 600)
+    # config 3 on its own default knobs (the tuned ones are config 2's 2^20-key tiles)
+    plain3 = run([BW_TREE, "--cfg", "3", "--mode", "pass", "--device-plan", "--warmup", "2", "--repeat", "5"], 600)
+    M, S, U = r["n_answers"], r["S"], r["U"]
+    alg_bytes = 20.0 * M + 12.0 * S * U  # compulsory bytes of one step
+    step_ms, replay_ms = d["step_ms"]["avg"], d["replay_ms"]
+    achieved = alg_bytes / (replay_ms * 1e-3) / 1e9
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
-            traffic = json.load(f).get(dom)
+            traffic = json.load(f).get("bwd_block_kernel")
+    tl = r["device_plan"]
+    value = d["queries_per_s"]
     out = {
-        "metric": "exact range-sum queries/s at 300k states, batched over a resident index (one hipGraph replay "
-                  "per stream of 100,000 query batches) vs the plain GPU drop-in, served p99",
-        "value": round(r["queries_per_s"], 1),
+        "metric": "exact range-sum queries/s at 300k states, batched (device plan + one persistent-kernel launch per "
+                  "stream of 100,000 query batches) vs the plain GPU drop-in, served p99",
+        "value": round(value, 1),
         "unit": "queries/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": round(r["graph_ms"]["avg"], 4),
+        "ms_per_step": round(step_ms, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "int64",
         "data": "synthetic (seeded counter-based generators: config-4 tree and its 100,000-batch request stream)",
         "config": {"workload": f"batched: {r['batches']} query batches (sizes U[1,{r['max_batch']}], "
-                               f"{r['n_answers']} queries) over {r['S']} states x {r['U']} updates, N={r['N']}",
-                   "states": r["S"], "updates_per_state": r["U"], "keys": r["N"], "batches": r["batches"],
-                   "answers_per_step": r["n_answers"], "answers_sha256": r["answers_sha256"], "knobs": r["plan"]},
+                               f"{M} queries) over {S} states x {U} updates, N={r['N']}; device plan + persistent "
+                               "replay over the dense-carry index",
+                   "states": S, "updates_per_state": U, "keys": r["N"], "batches": r["batches"],
+                   "answers_per_step": M, "answers_sha256": d["answers_sha256"],
+                   "dense_index": {"rmax": d["rmax"], "blocks": d["nblk"], "carry_bytes": d["carry_bytes"],
+                                   "carry_build_ms": d["carry_build_ms"], "threads": d["threads"]}},
+        "step": {"plan_ms": d["plan_ms"], "replay_ms": d["replay_ms"],
+                 "min_ms": d["step_ms"]["min"], "max_ms": d["step_ms"]["max"]},
+        "two_level": {"step_queries_per_s": tl["step_queries_per_s"], "step_ms": tl["step_ms"],
+                      "plan_ms": tl["ms"], "replay_ms": r["graph_ms"]["avg"],
+                      "replay_queries_per_s": round(r["queries_per_s"], 1), "stage_ms": r["stage_ms"],
+                      "plan_equals_host_plan": tl["mismatch"] == 0, "index_build_ms": r["phases_ms"]["index_build"],
+                      "knobs": r["plan"]},
         "plain": {
             "config2_queries_per_s": round(plain["queries_per_s"], 1),
             "config2_pass_ms": plain["pass_ms"]["avg"],
+            "config2_stage_ms": plain["stage_ms"],
             "config2_answers_sha256": plain["answers_sha256"],
+            "config3_queries_per_s": round(plain3["queries_per_s"], 1),
+            "config3_pass_ms": plain3["pass_ms"]["avg"],
+            "config3_stage_ms": plain3["stage_ms"],
+            "config3_answers_sha256": plain3["answers_sha256"],
             "per_batch_pass_queries_per_s": round(r["plain"]["pass_queries_per_s"], 1),
             "per_batch_query_only_queries_per_s": round(r["plain"]["query_queries_per_s"], 1),
             "per_batch_sample_batches": r["plain"]["batches"],
+            "per_batch_note": "each of the first batches alone: index rebuild + answers (host plan, outside the clock)",
         },
+        "batched_vs_plain": round(value / plain["queries_per_s"], 3),
+        "batched_vs_plain_per_batch": round(value / max(1.0, r["plain"]["pass_queries_per_s"]), 1),
+        "two_level_vs_plain": round(tl["step_queries_per_s"] / plain["queries_per_s"], 3),
         "extract_config2": {"answers": plain["n_answers"], "serial_ms": plain["extract"]["serial"],
                             "overlap_ms": plain["extract"]["overlap"], "same": plain["extract"]["same"]},
-        "batched_vs_plain": round(r["queries_per_s"] / plain["queries_per_s"], 3),
-        "batched_vs_plain_per_batch": round(r["queries_per_s"] / max(1.0, r["plain"]["pass_queries_per_s"]), 1),
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel_ms": round(stages[dom], 4), "kernel_bytes_alg": r["stage_bytes"][dom]},
-        "stage_ms": r["stage_ms"],
-        "index_build_ms": r["phases_ms"]["index_build"],
-        "device_plan": {"ms": r["device_plan"]["ms"], "plan_equals_host_plan": r["device_plan"]["mismatch"] == 0,
-                        "plan_and_query_queries_per_s": round(r["n_answers"] / ((r["device_plan"]["ms"] +
-                                                                                  r["graph_ms"]["avg"]) * 1e-3), 1)},
-        "host_batch_plan_ms": r["phases_ms"]["batch_plan"],
+        "roofline": {"bound": "hbm", "kernel": "bwd_block_kernel", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "kernel_ms": round(replay_ms, 4), "kernel_bytes_alg": alg_bytes,
+                     "bytes_alg_rule": "20 B per query (state, lo, hi, answer) + 12 B per update (key, delta)",
+                     "step_achieved": round(alg_bytes / (step_ms * 1e-3) / 1e9, 1),
+                     "step_frac": round(alg_bytes / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic_over_alg": round(traffic / alg_bytes, 2) if traffic else None},
         "device_index": {"total_ms": r["device_index"]["total_ms"], "device_ms": r["device_index"]["gpu_ms"],
                          "host_checks_ms": r["device_index"]["host_ms"],
                          "host_index_ms": r["device_index"]["host_plan_ms"],
@@ -147,40 +246,42 @@ def main():
                        "equals_host_generator": r["device_gen"]["mismatch"] == 0, "ties": r["device_gen"]["ties"]},
         "device": r["device"],
     }
+    with tempfile.TemporaryDirectory() as tmp:
+        out["dropin"], ref = dropin_legs(tmp, threads)
+        if ref:
+            out["reference_cpu"] = ref
     if not a.no_served:
-        s = run([BW_SERVE, "--check-alone", "16"] + knobs, 600)
-        out["served"] = {"queries_per_s": round(s["queries_per_s"], 1),
-                         "offered_queries_per_s": round(s["offered_queries_per_s"], 1),
-                         "latency_ms": s["latency_ms"], "requests": s["requests"], "appends": s["appends"],
-                         "batches": s["batches"], "mean_batch_queries": s["mean_batch_queries"],
-                         "batch_gpu_ms": s["batch_gpu_ms"], "planner": s["planner"],
-                         "answers_sha256": s["answers_sha256"], "alone_mismatch": s["alone"]["mismatch"]}
-        z = run([BW_SERVE, "--check-alone", "0", "--rate", "1e9"] + knobs, 600)
-        out["served_saturated"] = {"queries_per_s": round(z["queries_per_s"], 1), "latency_ms": z["latency_ms"],
-                                   "batches": z["batches"], "answers_sha256": z["answers_sha256"]}
-        # the same request law over the config-3 tree, its raw endpoints ranked through the LUT
+        curve = []
+        for rate in SERVED_RATES + (1e9,):
+            s = run([BW_SERVE, "--check-alone", "16" if rate == SERVED_RATES[0] else "0", "--rate", str(rate)] + knobs,
+                    600)
+            curve.append({"offered_requests_per_s": rate if rate < 1e9 else "saturated",
+                          "offered_queries_per_s": round(s["offered_queries_per_s"], 1),
+                          "queries_per_s": round(s["queries_per_s"], 1), "latency_ms": s["latency_ms"],
+                          "batches": s["batches"], "mean_batch_queries": s["mean_batch_queries"],
+                          "batch_gpu_ms": s["batch_gpu_ms"], "answers_sha256": s["answers_sha256"],
+                          "alone_mismatch": s["alone"]["mismatch"]})
+        if len({c["answers_sha256"] for c in curve}) != 1:
+            sys.exit("bench.py: served answers depend on the offered rate")
+        out["served"] = curve[0] | {"requests": s["requests"], "appends": s["appends"], "planner": s["planner"]}
+        out["served_curve"] = curve
         u = run([BW_SERVE, "--cfg", "3", "--check-alone", "16"] + knobs, 600)
         out["served_rank_lut"] = {"queries_per_s": round(u["queries_per_s"], 1),
                                   "offered_queries_per_s": round(u["offered_queries_per_s"], 1),
                                   "latency_ms": u["latency_ms"], "requests": u["requests"], "appends": u["appends"],
                                   "answers_sha256": u["answers_sha256"], "alone_mismatch": u["alone"]["mismatch"]}
     if not a.no_cpu:
-        try:
-            threads = len(os.sched_getaffinity(0))
-        except AttributeError:
-            threads = os.cpu_count() or 1
-        threads = max(1, min(CPU_MAX_THREADS, threads))
         p = subprocess.run([ORACLE, "stream", "--form", "4", "--threads", str(threads)],
                            capture_output=True, text=True, check=True, timeout=900)
         c = last_json(p.stdout)
-        if c["answers_sha256"] != r["answers_sha256"]:
-            sys.exit(f"bench.py: CPU baseline answers {c['answers_sha256']} != GPU {r['answers_sha256']}")
+        if c["answers_sha256"] != d["answers_sha256"]:
+            sys.exit(f"bench.py: CPU baseline answers {c['answers_sha256']} != GPU {d['answers_sha256']}")
         out["cpu_baseline"] = {"value": round(c["n_answers"] / c["solve_s"], 1), "unit": "queries/s",
                                "cores": threads, "kind": "port",
                                "sample": f"the whole batched stream ({c['n_answers']} queries, {r['batches']} batches) "
                                          "over the same 300,000-state tree: the oracle's rollback-Fenwick DFS (tree "
-                                         f"replay + queries) cut into {threads} pieces on {threads} host threads, "
-                                         "generation excluded; answers' SHA-256 equal to the GPU's",
+                                         f"replay + queries) cut into {threads} pieces on {threads} host threads (the "
+                                         "host's cores), generation excluded; answers' SHA-256 equal to the GPU's",
                                "seconds": round(c["solve_s"], 3)}
     print(json.dumps(out))
 

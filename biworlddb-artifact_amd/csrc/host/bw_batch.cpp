@@ -38,8 +38,10 @@
 #include <string>
 #include <vector>
 
+#include "bwgpu_dense.h"
 #include "bwgpu_extract.h"
 #include "bwgpu_tree.h"
+#include "dense_engine.hpp"
 #include "tree_engine.hpp"
 #include "tree_gen.hpp"
 #include "tree_plan.hpp"
@@ -99,6 +101,8 @@ int main(int argc, char** argv) {
   LaunchKnobs lk;
   int warmup = 1, repeat = 3, plain_sample = 8;
   bool want_sha = false, stages = false, dev_plan = false, dev_gen = false, dev_index = false, dev_all = false;
+  bool dense = false;
+  int dense_rmax = 4096, dense_threads = 1024;
   int fork = 1;  // replay the block and level-B stages on two streams
   std::string answers_path, json_path, tuned_path;
   bool have_knob[8] = {false, false, false, false, false, false, false, false};
@@ -109,6 +113,7 @@ int main(int argc, char** argv) {
     if (!std::strcmp(argv[i], "--device-gen")) { dev_gen = true; continue; }
     if (!std::strcmp(argv[i], "--device-index")) { dev_index = dev_plan = true; continue; }
     if (!std::strcmp(argv[i], "--device-all")) { dev_all = true; continue; }
+    if (!std::strcmp(argv[i], "--dense")) { dense = dev_plan = true; continue; }
     if (i + 1 >= argc) { std::fprintf(stderr, "bw_batch: missing value for %s\n", argv[i]); return 2; }
     const char* k = argv[i];
     const char* v = argv[++i];
@@ -130,6 +135,8 @@ int main(int argc, char** argv) {
     else if (!std::strcmp(k, "--stage-shift")) kn.stage_shift = std::atoi(v);
     else if (!std::strcmp(k, "--bt-table-mb")) kn.bt_table_mb = std::atoll(v);
     else if (!std::strcmp(k, "--fork")) fork = std::atoi(v);
+    else if (!std::strcmp(k, "--dense-rmax")) dense_rmax = std::atoi(v);
+    else if (!std::strcmp(k, "--dense-threads")) dense_threads = std::atoi(v);
     else if (!std::strcmp(k, "--seg")) kn.seg_chunks = std::atoi(v);
     else if (!std::strcmp(k, "--threads")) { lk.block_threads = std::atoi(v); have_knob[4] = true; }
     else if (!std::strcmp(k, "--waves")) { lk.chunk_waves = std::atoi(v); have_knob[5] = true; }
@@ -295,7 +302,8 @@ int main(int argc, char** argv) {
       HIPX(hipEventElapsedTime(&t, e0, e1));
       pm.push_back(t);
     }
-    dplan_ms = *std::min_element(pm.begin(), pm.end());
+    dplan_ms = 0;
+    for (double t : pm) dplan_ms += t / double(pm.size());  // average, like every other timing here
     int64_t cnt[5];
     HIPX(hipMemcpy(cnt, dslot.dcounts, sizeof(cnt), hipMemcpyDeviceToHost));
     if (cnt[4]) { std::fprintf(stderr, "bw_batch: device plan flagged out-of-range queries\n"); return 3; }
@@ -364,6 +372,22 @@ int main(int argc, char** argv) {
     HIPX(hipEventElapsedTime(&t, e0, e1));
     run_ms.push_back(t);
   }
+  // the two-level form's whole step with its device plan: plan + graph replay in one event pair
+  std::vector<double> step2_ms;
+  if (dev_plan) {
+    for (int i = 0; i < repeat; ++i) {
+      HIPX(hipEventRecord(e0, st));
+      RUN(device_plan(E, dslot, M, d_qstate, d_qlo, d_qhi, st));
+      HIPX(hipGraphLaunch(gexec, st));
+      HIPX(hipEventRecord(e1, st));
+      HIPX(hipEventSynchronize(e1));
+      float t = 0;
+      HIPX(hipEventElapsedTime(&t, e0, e1));
+      step2_ms.push_back(t);
+    }
+  }
+  double step2_avg = 0;
+  for (double t : step2_ms) step2_avg += t / double(step2_ms.size());
   // per-kernel timing of the same sequence (outside the graph), averaged over `repeat` runs
   double st_ms[3] = {0, 0, 0};
   if (stages) {
@@ -422,6 +446,95 @@ int main(int argc, char** argv) {
     return xrc < 0 ? 4 : 5;
   }
   const std::string sha = xr.sha256_hex;
+
+  // ---- the dense-carry form (--dense, include/bwgpu_dense.h): an engine cut into blocks of up
+  // to dense_rmax keys, one dense carry row per block (built once), then per step the whole
+  // stream planned (one radix sort) and answered (one persistent block-replay launch, captured
+  // in a hipGraph); its answers must equal the two-level form's
+  char dense_json[1024] = "null";
+  int64_t dense_mismatch = -1;
+  if (dense) {
+    PlanKnobs dk = kn;
+    dk.rmax = dense_rmax;
+    TreeEngine E4;
+    const auto d0 = clk::now();
+    if (!E4.init(spec.S, spec.U, w.coord_n(), w.parent.data(), w.key.data(), w.delta.data(), dk,
+                 BwtLaunch{1024, lk.chunk_waves, 0, 0}, err, sizeof(err))) {
+      std::fprintf(stderr, "bw_batch: dense index: %s\n", err);
+      return std::strncmp(err, "device", 6) == 0 ? 5 : 3;
+    }
+    RUN(E4.build_index(st));
+    DenseEngine D;
+    D.launch = BwdLaunch{dense_threads, 1};
+    if (!D.init(E4, st, err, sizeof(err))) {
+      std::fprintf(stderr, "bw_batch: %s\n", err);
+      return 5;
+    }
+    const double dense_index_wall = ms_between(d0, clk::now());
+    DenseBatch DBn;
+    int64_t* d_ans = nullptr;
+    if (!alloc_dense_batch(D, M, DBn, err, sizeof(err)) || !(d_ans = DBn.arena.scratch<int64_t>(size_t(M)))) {
+      std::fprintf(stderr, "bw_batch: %s\n", err);
+      return 5;
+    }
+    auto plan_d = [&]() { return bwd_plan_launch(&D.x, M, d_qstate, d_qlo, d_qhi, DBn.flags, DBn.work, DBn.work_bytes, st); };
+    RUN(plan_d());
+    RUN(bwd_replay_launch(&D.x, M, d_ans, &D.launch, DBn.work, DBn.work_bytes, st));  // warm
+    HIPX(hipStreamSynchronize(st));
+    int64_t flag = 0;
+    HIPX(hipMemcpy(&flag, DBn.flags, sizeof(flag), hipMemcpyDeviceToHost));
+    if (flag) { std::fprintf(stderr, "bw_batch: dense plan flagged out-of-range queries\n"); return 3; }
+    hipGraph_t dg;
+    hipGraphExec_t dgx;
+    HIPX(hipStreamBeginCapture(st, hipStreamCaptureModeGlobal));
+    RUN(bwd_replay_launch(&D.x, M, d_ans, &D.launch, DBn.work, DBn.work_bytes, st));
+    HIPX(hipStreamEndCapture(st, &dg));
+    HIPX(hipGraphInstantiate(&dgx, dg, nullptr, nullptr, 0));
+    hipEvent_t em;
+    HIPX(hipEventCreate(&em));
+    for (int i = 0; i < warmup; ++i) {
+      RUN(plan_d());
+      HIPX(hipGraphLaunch(dgx, st));
+    }
+    HIPX(hipStreamSynchronize(st));
+    std::vector<double> step_ms, dplan_v, drep_v;
+    for (int i = 0; i < repeat; ++i) {
+      HIPX(hipEventRecord(e0, st));
+      RUN(plan_d());
+      HIPX(hipEventRecord(em, st));
+      HIPX(hipGraphLaunch(dgx, st));
+      HIPX(hipEventRecord(e1, st));
+      HIPX(hipEventSynchronize(e1));
+      float t = 0, tp = 0;
+      HIPX(hipEventElapsedTime(&t, e0, e1));
+      HIPX(hipEventElapsedTime(&tp, e0, em));
+      step_ms.push_back(t);
+      dplan_v.push_back(tp);
+      drep_v.push_back(double(t) - double(tp));
+    }
+    BwExtractResult dxr{};
+    const int dxrc = bw_extract_answers(d_ans, size_t(M), st, nullptr, nullptr, BW_EXTRACT_FNV_ON_DEVICE, &dxr);
+    if (dxrc) return dxrc < 0 ? 4 : 5;
+    dense_mismatch = sha == dxr.sha256_hex ? 0 : 1;
+    if (dense_mismatch) std::fprintf(stderr, "bw_batch: dense-carry answers differ from the two-level form's\n");
+    auto mean = [](const std::vector<double>& v) {
+      double a = 0;
+      for (double x : v) a += x;
+      return a / double(v.size());
+    };
+    const double sm = mean(step_ms);
+    std::snprintf(dense_json, sizeof(dense_json),
+                  "{\"rmax\":%d,\"nblk\":%d,\"threads\":%d,\"carry_bytes\":%zu,\"carry_build_ms\":%.3f,"
+                  "\"index_wall_ms\":%.3f,\"step_ms\":{\"avg\":%.4f,\"min\":%.4f,\"max\":%.4f,\"repeat\":%d},"
+                  "\"plan_ms\":%.4f,\"replay_ms\":%.4f,\"queries_per_s\":%.1f,\"answers_sha256\":\"%s\","
+                  "\"mismatch\":%" PRId64 "}",
+                  dense_rmax, E4.x.nblk, dense_threads, D.carry_bytes, D.build_ms, dense_index_wall, sm,
+                  *std::min_element(step_ms.begin(), step_ms.end()), *std::max_element(step_ms.begin(), step_ms.end()),
+                  repeat, mean(dplan_v), mean(drep_v), double(M) / (sm * 1e-3), dxr.sha256_hex, dense_mismatch);
+    HIPX(hipGraphExecDestroy(dgx));
+    HIPX(hipGraphDestroy(dg));
+    HIPX(hipEventDestroy(em));
+  }
 
   // ---- the index tables built on the device (--device-index): compared with the host-built
   // tables array for array, then the stream planned and answered over that index
@@ -595,8 +708,8 @@ int main(int argc, char** argv) {
       "\"stage_bytes\":{\"block_replay\":%.0f,\"chunk_replay\":%.0f,\"combine\":%.0f},"
       "\"plain\":{\"batches\":%" PRId64 ",\"queries\":%" PRId64 ",\"pass_ms\":%.4f,\"query_ms\":%.4f,"
       "\"pass_queries_per_s\":%.1f,\"query_queries_per_s\":%.1f,\"mismatch\":%" PRId64 "},"
-      "\"device_plan\":{\"used\":%s,\"ms\":%.4f,\"mismatch\":%" PRId64 "},"
-      "\"device_index\":%s,\"device_all\":%s,\"device_gen\":{\"used\":%s,\"gpu_ms\":%.4f,\"wall_ms\":%.3f,\"host_ms\":%.3f,\"ties\":%u,"
+      "\"device_plan\":{\"used\":%s,\"ms\":%.4f,\"mismatch\":%" PRId64 ",\"step_ms\":%.4f,\"step_queries_per_s\":%.1f},"
+      "\"device_index\":%s,\"device_all\":%s,\"dense\":%s,\"device_gen\":{\"used\":%s,\"gpu_ms\":%.4f,\"wall_ms\":%.3f,\"host_ms\":%.3f,\"ties\":%u,"
       "\"mismatch\":%" PRId64 "},\"forked\":%d,"
       "\"device_bytes\":%zu,\"device\":\"%s\",\"cus\":%d}\n",
       spec.S, spec.U, spec.N, spec.seed, ss.nreq, ss.max_batch, M, sha.c_str(), E.plan.nblk(), BP.qblk.size(),
@@ -608,7 +721,8 @@ int main(int argc, char** argv) {
       sb_block, sb_chunk, sb_combine, K, plain_q, plain_pass_ms,
       plain_query_ms, plain_pass_ms > 0 ? double(plain_q) / (plain_pass_ms * 1e-3) : 0.0,
       plain_query_ms > 0 ? double(plain_q) / (plain_query_ms * 1e-3) : 0.0, plain_mismatch,
-      dev_plan ? "true" : "false", dplan_ms, plan_mismatch, didx_json, dall_json, dev_gen ? "true" : "false", dgen_gpu_ms, dgen_wall_ms,
+      dev_plan ? "true" : "false", dplan_ms, plan_mismatch, step2_avg,
+      step2_avg > 0 ? double(M) / (step2_avg * 1e-3) : 0.0, didx_json, dall_json, dense_json, dev_gen ? "true" : "false", dgen_gpu_ms, dgen_wall_ms,
       stream_host_ms, DS.ties, dgen_mismatch, fork, E.arena.bytes + io.bytes + DB.arena.bytes + dslot.arena.bytes,
       dev, prop.multiProcessorCount);
   std::fwrite(buf, 1, size_t(n), stdout);
@@ -618,7 +732,8 @@ int main(int argc, char** argv) {
     std::fwrite(buf, 1, size_t(n), f);
     std::fclose(f);
   }
-  return plain_mismatch == 0 && plan_mismatch <= 0 && dgen_mismatch <= 0 && didx_mismatch <= 0 && dall_mismatch <= 0
+  return plain_mismatch == 0 && plan_mismatch <= 0 && dgen_mismatch <= 0 && didx_mismatch <= 0 && dall_mismatch <= 0 &&
+                     dense_mismatch <= 0
              ? 0
              : 6;
 }

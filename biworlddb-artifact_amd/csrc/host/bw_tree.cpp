@@ -15,6 +15,9 @@
 //                side, checksums compared).
 // --device-gen : also materialise the workload's update and query arrays with the device generator
 //                (include/bwgpu_gen.h) and compare them element for element with the host's.
+// --device-plan : every timed pass also plans its batch on the device (include/bwgpu_plan.h; in pass
+//                mode with every block listed, bwt_plan_all_blocks_launch) instead of using the host
+//                plan built before the clock starts.
 // Exit codes follow the reference: 2 usage, 3 unsupported input, 4 I/O, 5 GPU error; 6 when the
 // serial and overlapped extracts disagree.
 #include <hip/hip_runtime_api.h>
@@ -86,7 +89,7 @@ int main(int argc, char** argv) {
   PlanKnobs kn;
   int threads = 256, waves = 4, warmup = 1, repeat = 5;
   std::string answers_path, json_path, mode = "pass", extract_mode = "overlap";
-  bool wsha = false, verify_remap = false, device_gen = false;
+  bool wsha = false, verify_remap = false, device_gen = false, dplan = false;
   int64_t copy_probe = 0;  // bytes of a device-to-device copy run first (HBM counter scale)
   // --tuned FILE: the "plain" form's knobs from the autotuning file, as defaults the flags override
   std::string tuned_from = "defaults";
@@ -105,6 +108,7 @@ int main(int argc, char** argv) {
     if (!std::strcmp(argv[i], "--workload-sha")) { wsha = true; continue; }
     if (!std::strcmp(argv[i], "--verify-remap")) { verify_remap = true; continue; }
     if (!std::strcmp(argv[i], "--device-gen")) { device_gen = true; continue; }
+    if (!std::strcmp(argv[i], "--device-plan")) { dplan = true; continue; }
     if (i + 1 >= argc) { std::fprintf(stderr, "bw_tree: missing value for %s\n", argv[i]); return 2; }
     const char* k = argv[i];
     const char* v = argv[++i];
@@ -232,7 +236,19 @@ int main(int argc, char** argv) {
   }
   const BatchPlan& BP = DB.plan;
   if (cfg == 3) DB.b.qtile = nullptr;  // combine reads the tiles of the ranks remapped on the device
-  const BwtBatch& Bt = DB.b;
+  // --device-plan: the pass plans its batch on the device (include/bwgpu_plan.h) inside the
+  // timed region, from the device copies of the query states and (remapped) endpoints
+  DevicePlanSlot dslot;
+  DeviceArena qsa;
+  int32_t* d_qs = nullptr;
+  if (dplan) {
+    d_qs = qsa.upload(qstate);
+    if (!qsa.ok() || !alloc_plan_slot(E, int64_t(nq), dslot, err, sizeof(err))) {
+      std::fprintf(stderr, "bw_tree: device plan: %s\n", qsa.ok() ? err : "upload");
+      return 5;
+    }
+  }
+  const BwtBatch& Bt = dplan ? dslot.b : DB.b;
   BwtIndex& X = E.x;
   const size_t g_dev_bytes = E.arena.bytes + io.bytes + DB.arena.bytes;
   HIPX(hipDeviceSynchronize());
@@ -261,6 +277,9 @@ int main(int argc, char** argv) {
   const int NST = 6;
   hipEvent_t ev[NST + 1];
   for (auto& e : ev) HIPX(hipEventCreate(&e));
+  hipEvent_t evp;
+  HIPX(hipEventCreate(&evp));
+  double plan_stage_ms = 0;
   auto build_index = [&]() { RUN(E.build_index(stream)); };
   if (mode == "query") {
     build_index();  // warm
@@ -280,6 +299,11 @@ int main(int argc, char** argv) {
     if (cfg == 3)
       for (auto& r : rk) RUN(bwt_rank_remap_launch(&r, stream));
     if (timed) HIPX(hipEventRecord(ev[1], stream));
+    if (dplan) {
+      RUN(device_plan(E, dslot, int64_t(nq), d_qs, d_qlo, d_qhi, stream));
+      if (mode == "pass") RUN(bwt_plan_all_blocks_launch(&X, &dslot.b, stream));
+    }
+    if (timed) HIPX(hipEventRecord(evp, stream));
     RUN(bwt_block_replay_launch(&X, &Bt, &wl, stream));
     if (timed) HIPX(hipEventRecord(ev[2], stream));
     if (mode == "pass") RUN(bwt_chunk_hist_launch(&X, &wl, stream));
@@ -305,6 +329,9 @@ int main(int argc, char** argv) {
       HIPX(hipEventElapsedTime(&t, ev[s], ev[s + 1]));
       stage_ms[s] += t / repeat;
     }
+    HIPX(hipEventElapsedTime(&t, ev[1], evp));  // the device plan, inside the block stage's events
+    plan_stage_ms += t / repeat;
+    stage_ms[1] -= t / repeat;
   }
   int64_t remap_mismatch = -1;
   if (cfg == 3 && verify_remap) {  // device CSR rank-LUT output against the host ranks
@@ -444,19 +471,19 @@ int main(int argc, char** argv) {
       "\"phases_ms\":{\"generate\":%.3f,\"setup\":%.3f,\"upload\":%.3f,\"download\":%.3f,\"extract\":%.3f,"
       "\"index_build\":%.4f},"
       "\"pass_ms\":{\"avg\":%.4f,\"min\":%.4f,\"max\":%.4f,\"repeat\":%d,\"warmup\":%d},"
-      "\"stage_ms\":{\"rank_remap\":%.4f,\"block_replay\":%.4f,\"chunk_hist\":%.4f,\"carry_scan\":%.4f,"
+      "\"stage_ms\":{\"rank_remap\":%.4f,\"device_plan\":%.4f,\"block_replay\":%.4f,\"chunk_hist\":%.4f,\"carry_scan\":%.4f,"
       "\"chunk_replay\":%.4f,\"combine\":%.4f},"
       "\"stage_bytes\":{\"rank_remap\":%.0f,\"block_replay\":%.0f,\"chunk_hist\":%.0f,\"carry_scan\":%.0f,"
       "\"chunk_replay\":%.0f,\"combine\":%.0f},"
-      "\"remap_mismatch\":%" PRId64 ",\"extract\":%s,\"device_gen\":%s,\"queries_per_s\":%.1f,\"bytes_alg\":%.0f,\"device_bytes\":%zu,\"device\":\"%s\",\"cus\":%d}\n",
+      "\"device_plan\":%s,\"remap_mismatch\":%" PRId64 ",\"extract\":%s,\"device_gen\":%s,\"queries_per_s\":%.1f,\"bytes_alg\":%.0f,\"device_bytes\":%zu,\"device\":\"%s\",\"cus\":%d}\n",
       mode.c_str(), cfg, S, spec.U, spec.Q, spec.N, w.coord_n(), spec.seed, nq, sha.c_str(), fn, P.steps.size(),
       P.nblk(), P.bkey.size(), P.tile_rank0.size(), P.kitems.size(), BP.eitems.size(), P.nchunk(), BP.wk_chunk.size(),
       P.nchain(), P.nseg(), P.max_cls, kn.rmax, P.knobs.tile_shift, P.knobs.chunk_items, BP.grp_step.size(), kn.ep_per_work,
       kn.seg_chunks, threads, waves, tuned_from.c_str(),
       ms_between(t0, t1), ms_between(t1, t2), ms_between(tu0, tu1), dl_ms, ex_ms,
-      index_ms, avg, pass_ms.front(), pass_ms.back(), repeat, warmup, stage_ms[0], stage_ms[1], stage_ms[2],
+      index_ms, avg, pass_ms.front(), pass_ms.back(), repeat, warmup, stage_ms[0], plan_stage_ms, stage_ms[1], stage_ms[2],
       stage_ms[3], stage_ms[4], stage_ms[5], sb_remap, sb_block, sb_hist, sb_scan, sb_replay, sb_combine,
-      remap_mismatch, ext_json, dgen_json, double(nq) / (avg * 1e-3), bytes_alg, g_dev_bytes, dev, prop.multiProcessorCount);
+      dplan ? "true" : "false", remap_mismatch, ext_json, dgen_json, double(nq) / (avg * 1e-3), bytes_alg, g_dev_bytes, dev, prop.multiProcessorCount);
   std::fwrite(buf, 1, size_t(n), stdout);
   if (!json_path.empty()) {
     FILE* f = std::fopen((json_path + ".tmp").c_str(), "w");

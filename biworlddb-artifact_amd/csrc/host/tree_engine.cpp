@@ -402,7 +402,7 @@ bool alloc_plan_slot(const TreeEngine& e, int64_t cap, DevicePlanSlot& s, char* 
   s.cap = cap;
   s.b = BwtBatch{};
   const size_t n = size_t(cap);
-  const size_t nq = std::min<size_t>(size_t(e.x.nblk), n);
+  const size_t nq = size_t(e.x.nblk);  // every block can be listed (bwt_plan_all_blocks_launch)
   BwtBatch& b = s.b;
   b.qblk = s.arena.scratch<int32_t>(nq);
   b.qblk_g0 = s.arena.scratch<int64_t>(nq + 1);

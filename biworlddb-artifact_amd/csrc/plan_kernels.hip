@@ -353,6 +353,22 @@ __global__ void __launch_bounds__(kT) plan_stage_rank_kernel(const BwtBatch b, i
 
 unsigned grid_of(int64_t n) { return (unsigned)((n + kT - 1) / kT); }
 
+// every block listed (a pass that also writes the index replays every block): qblk[b] = b and
+// qblk_g0[b] = the first step group whose step is in block b or later (grp_step ascending)
+__global__ void __launch_bounds__(kT) plan_all_blocks_kernel(const BwtIndex x, const BwtBatch b, int64_t* dcounts) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i > x.nblk) return;
+  const int64_t ngrp = dcounts[1];
+  if (i == x.nblk) {
+    const_cast<int64_t*>(b.qblk_g0)[i] = ngrp;
+    dcounts[0] = x.nblk;
+    return;
+  }
+  const_cast<int32_t*>(b.qblk)[i] = (int32_t)i;
+  const_cast<int64_t*>(b.qblk_g0)[i] = lower_bound_i32(b.grp_step, 0, ngrp, x.blk_step0[i]);
+}
+
+
 int report(const char* what, hipError_t e) {
   if (e != hipSuccess) std::fprintf(stderr, "bwt_plan: %s: %s\n", what, hipGetErrorString(e));
   return (int)e;
@@ -471,4 +487,14 @@ extern "C" int bwt_plan_launch(const BwtIndex* x, const BwtPlanIndex* p, const i
     return rc;
   hipLaunchKernelGGL(plan_stage_rank_kernel, dim3(grid_of(M)), blk, 0, st, B, M, tval_out);
   return report("tile ranks", hipGetLastError());
+}
+
+extern "C" int bwt_plan_all_blocks_launch(const BwtIndex* x, BwtBatch* b, void* stream) {
+  if (!x || !b || x->nblk < 1 || !b->dcounts || !b->qblk || !b->qblk_g0 || !b->grp_step) return (int)hipErrorInvalidValue;
+  // dcounts[1] (step groups) is read on the device: this launch follows bwt_plan_launch on the stream
+  hipLaunchKernelGGL(plan_all_blocks_kernel, dim3(grid_of((int64_t)x->nblk + 1)), dim3(kT), 0, (hipStream_t)stream, *x,
+                     *b, const_cast<int64_t*>(b->dcounts));
+  const int rc = report("all blocks", hipGetLastError());
+  if (rc == 0) b->nqblk = x->nblk;
+  return rc;
 }

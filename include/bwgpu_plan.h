@@ -59,6 +59,13 @@ int bwt_plan_workspace(const BwtIndex* x, const BwtPlanIndex* p, int64_t M, size
 int bwt_plan_launch(const BwtIndex* x, const BwtPlanIndex* p, const int32_t* qstate, int64_t M, BwtBatch* b,
                     int64_t* dcounts, void* work, size_t work_bytes, void* stream);
 
+/* After bwt_plan_launch on the same stream: list every block of the index in b (qblk[i] = i, with
+ * its step groups), as a pass that also writes the index needs (bwt_pass_launch); sets b->nqblk =
+ * nblk and dcounts[0].  Capacities: qblk nblk, qblk_g0 nblk+1.  Counterpart: the host planner's
+ * all_blocks option (build_batch_plan); the reference's drivers always launch every block
+ * (src/panel_batch_gpu.cpp:172-178). */
+int bwt_plan_all_blocks_launch(const BwtIndex* x, BwtBatch* b, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
