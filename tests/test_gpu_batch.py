@@ -101,3 +101,25 @@ def test_all_device_pipeline(pkg, case, knobs):
     d = r["device_all"]
     assert d["mismatch"] == 0 and d["answers_sha256"] == g["oracle"], (case, d)
 
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case,knobs", [
+    ("s1_batched_small", []),
+    ("s1_batched_small", ["--dense-rmax", "64", "--dense-threads", "256"]),
+    ("s2_batched_mid", []),
+    ("s2_batched_mid", ["--dense-rmax", "1024", "--dense-threads", "512"]),
+    ("s2_batched_mid", ["--dense-rmax", "2048", "--dense-threads", "256", "--device-gen"]),
+    ("S4_batched_full", ["--tuned", None]),
+])
+def test_dense_form_matches_oracle(pkg, case, knobs):
+    """The dense-carry form (include/bwgpu_dense.h): carry rows built from the blocks' weights, the
+    stream sorted by state rank on the device, one persistent work-queue replay launch (replayed
+    from a hipGraph) -- the oracle's answers at every block size and workgroup shape, small trees
+    (one block, or blocks of 64 keys) to the full 102.5M-query stream."""
+    g = STREAMS[case]
+    knobs = [pkg.TUNED if k is None else k for k in knobs]
+    r = run_json(_cmd(pkg, g, knobs + ["--plain-sample", "0", "--dense"]), timeout=900)
+    d = r["dense"]
+    assert d["mismatch"] == 0 and d["answers_sha256"] == g["oracle"], (case, knobs, d)
+    assert r["answers_sha256"] == g["oracle"]
