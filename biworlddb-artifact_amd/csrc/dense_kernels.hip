@@ -94,6 +94,7 @@ __global__ void __launch_bounds__(kCT) bwd_carry_range_kernel(const BwdIndex x, 
                                                               const uint16_t* rb, int nr, int64_t* T, int64_t ldT) {
   __shared__ uint32_t skey[kChunk];
   __shared__ int64_t sap[kChunk + 1];
+  __shared__ int64_t srow[kW];  // the range's values, stored lane-contiguous from here
   const int j = blockIdx.x, g = blockIdx.y;
   const int64_t NKP = (int64_t)x.NK + 1;
   const int64_t xb = (int64_t)j * kW + (int64_t)threadIdx.x * kXPT;  // this thread's first coordinate
@@ -101,13 +102,23 @@ __global__ void __launch_bounds__(kCT) bwd_carry_range_kernel(const BwdIndex x, 
   int64_t acc[kXPT];
 #pragma unroll
   for (int i = 0; i < kXPT; ++i) acc[i] = (kRows && xb + i < NKP) ? T[(int64_t)g * ldT + xb + i] : 0;
-  for (int b = b0; b < b1; ++b) {
-    if (kRows) {
-      int64_t* row = x.carry + (int64_t)b * x.carry_ld;
+  // a thread's values cover kXPT consecutive coordinates; they go through LDS so that each store
+  // instruction writes one contiguous run (a direct store would leave every 64-byte piece of the
+  // row partly written per instruction)
+  auto store_range = [&](int64_t* dst) {
+    __syncthreads();  // the previous range's readers are done
 #pragma unroll
-      for (int i = 0; i < kXPT; ++i)
-        if (xb + i < NKP) row[xb + i] = acc[i];
+    for (int i = 0; i < kXPT; ++i) srow[threadIdx.x * kXPT + i] = acc[i];
+    __syncthreads();
+    const int64_t x0 = (int64_t)j * kW;
+#pragma unroll
+    for (int i = 0; i < kXPT; ++i) {
+      const int k = i * kCT + threadIdx.x;
+      if (x0 + k < NKP) dst[x0 + k] = srow[k];
     }
+  };
+  for (int b = b0; b < b1; ++b) {
+    if (kRows) store_range(x.carry + (int64_t)b * x.carry_ld);
     const int64_t k0 = x.blk_key0[b];
     const int64_t pbase = k0 + b;  // aggp offset of block b
     const int r0 = rb[(int64_t)b * (nr + 1) + j], r1 = rb[(int64_t)b * (nr + 1) + j + 1];
@@ -139,11 +150,7 @@ __global__ void __launch_bounds__(kCT) bwd_carry_range_kernel(const BwdIndex x, 
 #pragma unroll
     for (int i = 0; i < kXPT; ++i) acc[i] += d[i];
   }
-  if (!kRows) {
-#pragma unroll
-    for (int i = 0; i < kXPT; ++i)
-      if (xb + i < NKP) T[(int64_t)g * ldT + xb + i] = acc[i];
-  }
+  if (!kRows) store_range(T + (int64_t)g * ldT);
 }
 
 // exclusive scan of T over segments, per coordinate

@@ -2,20 +2,27 @@
 // over a resident index, with appended branch states (one JSON line).
 //
 //   bw_serve [--cfg 2|3] [--S n --U n --N n --seed n] [--requests 20000 --max-batch 4096 --append-frac 0.05
-//            --append-U 128 --rate 20000] [--tuned FILE] [--slots 2] [--budget-mb 1024]
-//            [--check-alone 64] [--answers FILE] [--json FILE]
+//            --append-U 128 --rate 20000] [--tuned FILE] [--slots 2] [--budget-mb 8192] [--slot-queries 16777216]
+//            [--check-alone 64] [--answers FILE] [--json FILE] [--timeline]
 //
 // Resident in HBM: the version tree's index (TreeEngine), the appended states' updates, and per
-// in-flight batch one fixed buffer ("slot") whose size is the HBM budget divided among the slots.
+// in-flight batch one fixed buffer ("slot") whose size is the HBM budget divided among the slots
+// (at most --slot-queries queries).
 // Requests arrive at their Poisson times (a clock started when serving starts).  Whenever a slot
 // is free, every request that has arrived and fits the slot's capacity is admitted in arrival
-// order: appends are copied to the device at once (a stream every batch waits on), query
-// requests join the batch.  The host only stages the batch's raw queries (state, lo, hi) and
-// copies them in; the batch is planned on the device (bwt_plan_launch), answered by one launch
-// sequence and copied out, all on the slot's own stream, so batches on different slots overlap
-// on the device.  A query on an appended state is answered at its deepest indexed ancestor plus
+// order: appends join the round's contiguous range of appended states, copied to the device
+// (from pinned receive buffers, one copy per array) before the round's batch launches, on a
+// stream every batch waits on; query requests join the batch.  Requests land in pinned receive buffers (state, lo, hi per query, in
+// arrival order), so a batch's queries are one contiguous slice of them: the host only replaces
+// appended states by their deepest indexed ancestor (listing those queries for the overlay) and
+// copies the slice in; the batch is planned on the device (bwt_plan_launch), answered by one
+// launch sequence and its answers copied straight into the pinned answer array, all on the slot's
+// own stream, so batches on different slots overlap on the device and on the copy engines.  A query on an appended state is answered at its deepest indexed ancestor plus
 // the overlay of the appended updates on its path.
 // Latency = time the request's answers are back on the host - its arrival time.
+// Before the clock starts, one small batch per slot loads the kernels, and copies in both
+// directions on every stream warm up the copy engines (their first use costs milliseconds).
+// --timeline prints each batch's host launch steps and completion times on stderr.
 // --check-alone K: afterwards K requests (evenly spread) are answered again, each alone in its
 // own batch, and must equal their answers from the mix element for element.
 // Exit codes: 2 usage, 3 unsupported input, 4 I/O, 5 GPU error, 6 alone/mix mismatch.
@@ -31,6 +38,8 @@
 #include <string>
 #include <thread>
 #include <vector>
+
+#include <omp.h>
 
 #include "bwgpu_tree.h"
 #include "tree_engine.hpp"
@@ -79,8 +88,8 @@ struct Slot {
   hipStream_t st = nullptr;
   hipEvent_t done = nullptr;
   hipEvent_t start = nullptr;
-  unsigned char* host = nullptr;  // pinned: qstate | qlo | qhi | overlay ids | overlay states | answers
-  size_t h_state = 0, h_lo = 0, h_hi = 0, h_ovq = 0, h_ovs = 0, h_ans = 0, host_bytes = 0;
+  unsigned char* host = nullptr;  // pinned: overlay ids | overlay states | answers of a batch run alone
+  size_t h_ovq = 0, h_ovs = 0, h_ans = 0, host_bytes = 0;
   DeviceArena in;
   int32_t* d_state = nullptr;
   uint32_t* d_lo = nullptr;
@@ -93,6 +102,7 @@ struct Slot {
   std::vector<int64_t> reqs;     // request ids
   std::vector<int64_t> req_off;  // their first query in the batch
   int64_t nq = 0;
+  int64_t q0 = 0;                // the batch's first query in the stream
   clk::time_point launched;
 };
 
@@ -104,9 +114,6 @@ size_t host_layout(Slot& s, int64_t cap) {
     return at;
   };
   const size_t n = size_t(cap);
-  s.h_state = take(n * 4);
-  s.h_lo = take(n * 4);
-  s.h_hi = take(n * 4);
   s.h_ovq = take(n * 4);
   s.h_ovs = take(n * 4);
   s.h_ans = take(n * 8);
@@ -138,9 +145,12 @@ int main(int argc, char** argv) {
   PlanKnobs kn;
   LaunchKnobs lk;
   int nslots = 2, check_alone = 64;
-  double budget_mb = 1024;
+  int64_t slot_queries = int64_t(1) << 24;  // most queries one batch takes
+  bool timeline = false;  // --timeline: per-batch host times on stderr  // most queries one batch takes (batches pipeline across slots)
+  double budget_mb = 8192;
   std::string answers_path, json_path, tuned_path;
   for (int i = 1; i < argc; ++i) {
+    if (!std::strcmp(argv[i], "--timeline")) { timeline = true; continue; }
     if (i + 1 >= argc) { std::fprintf(stderr, "bw_serve: missing value for %s\n", argv[i]); return 2; }
     const char* k = argv[i];
     const char* v = argv[++i];
@@ -157,6 +167,7 @@ int main(int argc, char** argv) {
     else if (!std::strcmp(k, "--tuned")) tuned_path = v;
     else if (!std::strcmp(k, "--slots")) nslots = std::atoi(v);
     else if (!std::strcmp(k, "--budget-mb")) budget_mb = std::atof(v);
+    else if (!std::strcmp(k, "--slot-queries")) slot_queries = std::atoll(v);
     else if (!std::strcmp(k, "--check-alone")) check_alone = std::atoi(v);
     else if (!std::strcmp(k, "--answers")) answers_path = v;
     else if (!std::strcmp(k, "--json")) json_path = v;
@@ -222,9 +233,34 @@ int main(int argc, char** argv) {
     int64_t p = rs.app_parent[size_t(a)];
     app_base[size_t(a)] = int32_t(p <= S ? p : app_base[size_t(p - S - 1)]);
   }
+  // pinned receive buffers: every request's queries land here in arrival order, and the answers
+  // come back into p_ans at the same offsets
+  int32_t* p_state = nullptr;
+  uint32_t *p_lo = nullptr, *p_hi = nullptr;
+  int64_t* p_ans = nullptr;
+  HIPX(hipHostMalloc(reinterpret_cast<void**>(&p_state), size_t(std::max<int64_t>(NQ, 1)) * 4, hipHostMallocDefault));
+  HIPX(hipHostMalloc(reinterpret_cast<void**>(&p_lo), size_t(std::max<int64_t>(NQ, 1)) * 4, hipHostMallocDefault));
+  HIPX(hipHostMalloc(reinterpret_cast<void**>(&p_hi), size_t(std::max<int64_t>(NQ, 1)) * 4, hipHostMallocDefault));
+  HIPX(hipHostMalloc(reinterpret_cast<void**>(&p_ans), size_t(std::max<int64_t>(NQ, 1)) * 8, hipHostMallocDefault));
+  std::memcpy(p_state, rs.qstate.data(), size_t(NQ) * 4);
+  std::memcpy(p_lo, rs.qlo.data(), size_t(NQ) * 4);
+  std::memcpy(p_hi, rs.qhi.data(), size_t(NQ) * 4);
+  // appended states land in pinned receive buffers too (keys ranked at admission over a ranked tree)
+  int32_t* p_app_parent = nullptr;
+  uint32_t* p_app_key = nullptr;
+  int64_t* p_app_delta = nullptr;
+  const size_t nap = size_t(std::max<int64_t>(NA, 1));
+  HIPX(hipHostMalloc(reinterpret_cast<void**>(&p_app_parent), nap * 4, hipHostMallocDefault));
+  HIPX(hipHostMalloc(reinterpret_cast<void**>(&p_app_key), nap * size_t(AU) * 4, hipHostMallocDefault));
+  HIPX(hipHostMalloc(reinterpret_cast<void**>(&p_app_delta), nap * size_t(AU) * 8, hipHostMallocDefault));
+  std::memcpy(p_app_parent, rs.app_parent.data(), size_t(NA) * 4);
+  std::memcpy(p_app_key, rs.app_key.data(), size_t(NA) * size_t(AU) * 4);
+  std::memcpy(p_app_delta, rs.app_delta.data(), size_t(NA) * size_t(AU) * 8);
   hipEvent_t app_ready;
   HIPX(hipEventCreate(&app_ready));
   HIPX(hipEventRecord(app_ready, ist));
+  hipStream_t ast;  // the appended states' copies
+  HIPX(hipStreamCreateWithFlags(&ast, hipStreamNonBlocking));
   // slots within the HBM budget: the largest capacity whose device buffers fit the slot's share
   std::vector<Slot> slots(static_cast<size_t>(nslots));
   const size_t share = size_t(budget_mb * 1048576.0 / nslots);
@@ -236,7 +272,7 @@ int main(int argc, char** argv) {
     const double per_q = double(pb) / double(probe);
     cap = std::max<int64_t>(cap, int64_t(double(share) / per_q));
     while (cap > ss.max_batch && slot_device_bytes(E, cap) > share) cap = cap * 15 / 16;
-    cap = std::max<int64_t>(cap, ss.max_batch);
+    cap = std::max<int64_t>(std::min(cap, std::max<int64_t>(slot_queries, 1)), ss.max_batch);
   }
   for (Slot& s : slots) {
     s.cap = cap;
@@ -261,28 +297,48 @@ int main(int argc, char** argv) {
   //      answers out (all on the slot's stream)
   double plan_s = 0;  // host time spent staging batches
   const BwtLaunch L = E.launch;
-  auto launch_batch = [&](Slot& s) {
+  // dst: pinned host array receiving the batch's answers (the stream's answer array at the
+  // batch's offset, or the slot's own buffer for a request re-answered alone)
+  auto launch_batch = [&](Slot& s, int64_t* dst) {
     const auto p0 = clk::now();
-    int32_t* hs = reinterpret_cast<int32_t*>(s.host + s.h_state);
-    uint32_t* hl = reinterpret_cast<uint32_t*>(s.host + s.h_lo);
-    uint32_t* hh = reinterpret_cast<uint32_t*>(s.host + s.h_hi);
     uint32_t* hoq = reinterpret_cast<uint32_t*>(s.host + s.h_ovq);
     int32_t* hos = reinterpret_cast<int32_t*>(s.host + s.h_ovs);
-    int64_t M = 0, nov = 0;
-    for (int64_t r : s.reqs)
-      for (int64_t q = rs.req_q0[size_t(r)]; q < rs.req_q0[size_t(r) + 1]; ++q) {
-        const int32_t v = rs.qstate[size_t(q)];
-        if (v > S) {
-          hoq[nov] = uint32_t(M);
-          hos[nov++] = v;
-          hs[M] = app_base[size_t(v - S - 1)];
-        } else {
-          hs[M] = v;
+    // the batch's requests are consecutive query requests (appends carry no queries), so their
+    // queries are one contiguous slice of the receive buffers
+    const int64_t q0 = rs.req_q0[size_t(s.reqs.front())];
+    const int64_t M = rs.req_q0[size_t(s.reqs.back()) + 1] - q0;
+    // queries on appended states: answered at their deepest indexed ancestor plus the overlay;
+    // the slice is scanned in parallel pieces (a count pass, then each piece fills its part)
+    int64_t nov = 0;
+    const int npc = M >= (int64_t(1) << 16) ? std::max(1, omp_get_max_threads()) : 1;
+    std::vector<int64_t> pcnt(size_t(npc) + 1, 0);
+#pragma omp parallel for num_threads(npc) schedule(static, 1) if (npc > 1)
+    for (int t = 0; t < npc; ++t) {
+      int64_t c = 0;
+      for (int64_t q = q0 + M * t / npc; q < q0 + M * (t + 1) / npc; ++q) c += rs.qstate[size_t(q)] > S;
+      pcnt[size_t(t) + 1] = c;
+    }
+    for (int t = 0; t < npc; ++t) pcnt[size_t(t) + 1] += pcnt[size_t(t)];
+    nov = pcnt[size_t(npc)];
+    if (nov) {
+#pragma omp parallel for num_threads(npc) schedule(static, 1) if (npc > 1)
+      for (int t = 0; t < npc; ++t) {
+        int64_t o = pcnt[size_t(t)];
+        for (int64_t q = q0 + M * t / npc; q < q0 + M * (t + 1) / npc; ++q) {
+          const int32_t v = rs.qstate[size_t(q)];
+          if (v > S) {
+            hoq[o] = uint32_t(q - q0);
+            hos[o++] = v;
+            p_state[q] = app_base[size_t(v - S - 1)];
+          }
         }
-        hl[M] = rs.qlo[size_t(q)];
-        hh[M] = rs.qhi[size_t(q)];
-        ++M;
       }
+    }
+    const auto p1 = clk::now();
+    int32_t* hs = p_state + q0;
+    uint32_t* hl = p_lo + q0;
+    uint32_t* hh = p_hi + q0;
+    s.q0 = q0;
     s.nq = M;
     HIPX(hipEventRecord(s.start, s.st));
     HIPX(hipMemcpyAsync(s.d_state, hs, size_t(M) * 4, hipMemcpyHostToDevice, s.st));
@@ -298,7 +354,9 @@ int main(int argc, char** argv) {
       RUN(bwt_rank_remap_launch(&rh, s.st));
     }
     HIPX(hipStreamWaitEvent(s.st, app_ready, 0));  // appended states admitted so far are on the device
+    const auto p2 = clk::now();
     RUN(device_plan(E, s.plan, M, s.d_state, s.d_lo, s.d_hi, s.st));
+    const auto p3 = clk::now();
     BwtLaunch l = L;
     RUN(bwt_query_launch(&E.x, &s.plan.b, &l, s.st));
     if (nov) {
@@ -316,30 +374,47 @@ int main(int argc, char** argv) {
       o.answers = s.plan.b.answers;
       RUN(bwt_overlay_launch(&o, s.st));
     }
-    HIPX(hipMemcpyAsync(s.host + s.h_ans, s.plan.b.answers, size_t(M) * 8, hipMemcpyDeviceToHost, s.st));
+    HIPX(hipMemcpyAsync(dst, s.plan.b.answers, size_t(M) * 8, hipMemcpyDeviceToHost, s.st));
     HIPX(hipEventRecord(s.done, s.st));
     s.busy = true;
     s.launched = clk::now();
+    if (timeline)
+      std::fprintf(stderr, "launch %" PRId64 " queries: scan %.3f ms, copies-in %.3f ms, plan launches %.3f ms, rest %.3f ms\n",
+                   M, secs(p0, p1) * 1e3, secs(p1, p2) * 1e3, secs(p2, p3) * 1e3, secs(p3, s.launched) * 1e3);
     plan_s += secs(p0, s.launched);
   };
-  std::vector<uint32_t> app_rank(ranked ? size_t(NA) * size_t(AU) : 0);  // ranked appended keys
+  // appends admitted but not yet copied: the contiguous range [app_lo, app_hi) of appended states
+  int64_t app_lo = 0, app_hi = 0;
   auto admit_append = [&](int64_t r) {
     const int64_t a = rs.req_app[size_t(r)];
-    const uint32_t* keys = &rs.app_key[size_t(a * AU)];
     if (ranked) {  // an appended key is a distinct key: its rank, from the LUT row of its high bits
+      uint32_t* keys = p_app_key + size_t(a * AU);
       for (int i = 0; i < AU; ++i) {
-        const uint32_t x = keys[i];
+        const uint32_t x = rs.app_key[size_t(a * AU) + size_t(i)];
         const uint16_t* b = lut.low16.data() + lut.row_ptr[x >> 16];
         const uint16_t* e = lut.low16.data() + lut.row_ptr[(x >> 16) + 1];
-        app_rank[size_t(a * AU) + size_t(i)] = uint32_t(std::lower_bound(b, e, uint16_t(x & 0xFFFFu)) - lut.low16.data());
+        keys[i] = uint32_t(std::lower_bound(b, e, uint16_t(x & 0xFFFFu)) - lut.low16.data());
       }
-      keys = &app_rank[size_t(a * AU)];
     }
-    HIPX(hipMemcpyAsync(d_app_parent + a, &rs.app_parent[size_t(a)], 4, hipMemcpyHostToDevice, ist));
-    HIPX(hipMemcpyAsync(d_app_key + a * AU, keys, size_t(AU) * 4, hipMemcpyHostToDevice, ist));
-    HIPX(hipMemcpyAsync(d_app_delta + a * AU, &rs.app_delta[size_t(a * AU)], size_t(AU) * 8, hipMemcpyHostToDevice,
-                        ist));
-    HIPX(hipEventRecord(app_ready, ist));
+    if (app_hi == app_lo) app_lo = app_hi = a;
+    app_hi = a + 1;  // appends are admitted in order
+  };
+  auto flush_appends = [&]() {
+    if (app_hi == app_lo) return;
+    const size_t n = size_t(app_hi - app_lo);
+    const auto f0 = clk::now();
+    HIPX(hipMemcpyAsync(d_app_parent + app_lo, p_app_parent + app_lo, n * 4, hipMemcpyHostToDevice, ast));
+    const auto f1 = clk::now();
+    HIPX(hipMemcpyAsync(d_app_key + app_lo * AU, p_app_key + app_lo * AU, n * size_t(AU) * 4, hipMemcpyHostToDevice, ast));
+    const auto f2 = clk::now();
+    HIPX(hipMemcpyAsync(d_app_delta + app_lo * AU, p_app_delta + app_lo * AU, n * size_t(AU) * 8, hipMemcpyHostToDevice,
+                        ast));
+    const auto f3 = clk::now();
+    HIPX(hipEventRecord(app_ready, ast));
+    if (timeline)
+      std::fprintf(stderr, "appends %zu: copies %.3f %.3f %.3f ms, record %.3f ms\n", n, secs(f0, f1) * 1e3,
+                   secs(f1, f2) * 1e3, secs(f2, f3) * 1e3, secs(f3, clk::now()) * 1e3);
+    app_lo = app_hi;
   };
 
   // ---- warm-up: one small batch per slot (first launches load the kernels), not timed
@@ -350,15 +425,42 @@ int main(int argc, char** argv) {
       for (Slot& s : slots) {
         s.reqs.assign(1, r0);
         s.req_off.assign(1, 0);
-        launch_batch(s);
+        launch_batch(s, reinterpret_cast<int64_t*>(s.host + s.h_ans));
         HIPX(hipEventSynchronize(s.done));
         s.busy = false;
       }
+    // copies: the runtime's first copies through each copy engine pay a one-time set-up
+    // (measured ~5-6 ms, once per engine), so before the clock starts copies go in both
+    // directions on every stream at once, at several sizes
+    for (size_t n = size_t(4) << 10; n <= (size_t(16) << 20); n *= 16) {
+      for (Slot& s : slots) {
+        const size_t m = std::min({n, size_t(NQ) * 4, size_t(s.cap) * 4});
+        HIPX(hipMemcpyAsync(s.d_lo, p_lo, m, hipMemcpyHostToDevice, s.st));
+        HIPX(hipMemcpyAsync(s.host + s.h_ans, s.plan.b.answers, m, hipMemcpyDeviceToHost, s.st));
+      }
+      if (NA > 0) {
+        const size_t m = std::min(n, size_t(NA) * size_t(AU) * 8);
+        HIPX(hipMemcpyAsync(d_app_delta, p_app_delta, m, hipMemcpyHostToDevice, ast));
+      }
+      for (Slot& s : slots) HIPX(hipStreamSynchronize(s.st));
+      HIPX(hipStreamSynchronize(ast));
+    }
+    // and every append array at one append's size, 64 KiB and whole (the device arrays are
+    // overwritten at admission)
+    if (NA > 0) {
+      const size_t whole[3] = {size_t(NA) * 4, size_t(NA) * size_t(AU) * 4, size_t(NA) * size_t(AU) * 8};
+      const size_t one[3] = {4, size_t(AU) * 4, size_t(AU) * 8};
+      void* dst[3] = {d_app_parent, d_app_key, d_app_delta};
+      const void* src[3] = {p_app_parent, p_app_key, p_app_delta};
+      for (int k = 0; k < 3; ++k)
+        for (const size_t n : {one[k], std::min<size_t>(whole[k], size_t(64) << 10), whole[k]})
+          HIPX(hipMemcpyAsync(dst[k], src[k], n, hipMemcpyHostToDevice, ast));
+      HIPX(hipStreamSynchronize(ast));
+    }
     plan_s = 0;
   }
   // ---- serving loop (open loop: arrivals follow the stream's clock from now on)
   std::vector<double> gpu_ms;  // per batch: stream time from its first copy in to its answers out
-  std::vector<int64_t> ans(static_cast<size_t>(NQ), 0);
   std::vector<double> lat;  // seconds, query requests
   lat.reserve(size_t(R));
   std::vector<double> done_at(static_cast<size_t>(R), -1.0);
@@ -371,17 +473,17 @@ int main(int argc, char** argv) {
     for (Slot& s : slots) {
       if (!s.busy || hipEventQuery(s.done) != hipSuccess) continue;
       const double now = secs(t0, clk::now());
-      const int64_t* out = reinterpret_cast<const int64_t*>(s.host + s.h_ans);
-      for (size_t i = 0; i < s.reqs.size(); ++i) {
+      for (size_t i = 0; i < s.reqs.size(); ++i) {  // the answers are in p_ans already
         const int64_t r = s.reqs[i];
-        const int64_t q0 = rs.req_q0[size_t(r)], n = rs.req_q0[size_t(r) + 1] - q0;
-        std::memcpy(&ans[size_t(q0)], out + s.req_off[i], size_t(n) * 8);
         done_at[size_t(r)] = now;
         lat.push_back(now - rs.arrival[size_t(r)]);
       }
       float t = 0;
       HIPX(hipEventElapsedTime(&t, s.start, s.done));
       gpu_ms.push_back(double(t));
+      if (timeline)
+        std::fprintf(stderr, "batch %" PRId64 " queries: launched %.3f ms, harvested %.3f ms, stream %.3f ms\n", s.nq,
+                     secs(t0, s.launched) * 1e3, now * 1e3, double(t));
       s.busy = false;
       progressed = true;
     }
@@ -389,6 +491,7 @@ int main(int argc, char** argv) {
     const double now = secs(t0, clk::now());
     for (Slot& s : slots) {
       if (s.busy || next >= R || rs.arrival[size_t(next)] > now) continue;
+      const auto a0 = clk::now();
       s.reqs.clear();
       s.req_off.clear();
       int64_t nq = 0;
@@ -407,8 +510,13 @@ int main(int argc, char** argv) {
         nq += n;
         ++next;
       }
+      const auto a1 = clk::now();
+      flush_appends();  // the round's appended states reach the device before its batch
+      if (timeline)
+        std::fprintf(stderr, "admit at %.3f ms: %zu requests %.3f ms, appends copy %.3f ms\n", secs(t0, a0) * 1e3,
+                     s.reqs.size(), secs(a0, a1) * 1e3, secs(a1, clk::now()) * 1e3);
       if (s.reqs.empty()) continue;
-      launch_batch(s);
+      launch_batch(s, p_ans + rs.req_q0[size_t(s.reqs.front())]);
       ++nbatches;
       sum_batch_q += double(nq);
       max_batch_q = std::max(max_batch_q, nq);
@@ -429,12 +537,12 @@ int main(int argc, char** argv) {
     Slot& s = slots[0];
     s.reqs.assign(1, r);
     s.req_off.assign(1, 0);
-    launch_batch(s);
+    launch_batch(s, reinterpret_cast<int64_t*>(s.host + s.h_ans));
     HIPX(hipEventSynchronize(s.done));
     s.busy = false;
     const int64_t q0 = rs.req_q0[size_t(r)], n = rs.req_q0[size_t(r) + 1] - q0;
     const int64_t* out = reinterpret_cast<const int64_t*>(s.host + s.h_ans);
-    for (int64_t i = 0; i < n; ++i) alone_mismatch += out[i] != ans[size_t(q0 + i)];
+    for (int64_t i = 0; i < n; ++i) alone_mismatch += out[i] != p_ans[size_t(q0 + i)];
     ++alone_checked;
   }
 
@@ -451,11 +559,11 @@ int main(int argc, char** argv) {
   const double gpu_p99 = gpu_ms.empty() ? 0.0 : gpu_ms[std::min(gpu_ms.size() - 1, size_t(0.99 * double(gpu_ms.size())))];
   std::sort(lat.begin(), lat.end());
   auto pct = [&](double p) { return lat.empty() ? 0.0 : lat[std::min(lat.size() - 1, size_t(p * double(lat.size())))]; };
-  const std::string sha = sha_hex(ans.data(), size_t(NQ) * 8);
+  const std::string sha = sha_hex(p_ans, size_t(NQ) * 8);
   if (!answers_path.empty()) {
     FILE* f = std::fopen(answers_path.c_str(), "wb");
     if (!f) return 4;
-    const bool ok = std::fwrite(ans.data(), 8, size_t(NQ), f) == size_t(NQ);
+    const bool ok = std::fwrite(p_ans, 8, size_t(NQ), f) == size_t(NQ);
     if (std::fclose(f) != 0 || !ok) return 4;
   }
   hipDeviceProp_t prop{};
@@ -488,5 +596,8 @@ int main(int argc, char** argv) {
     std::fclose(f);
   }
   for (Slot& s : slots) (void)hipHostFree(s.host);
+  for (void* p : {static_cast<void*>(p_state), static_cast<void*>(p_lo), static_cast<void*>(p_hi), static_cast<void*>(p_ans),
+                  static_cast<void*>(p_app_parent), static_cast<void*>(p_app_key), static_cast<void*>(p_app_delta)})
+    (void)hipHostFree(p);
   return alone_mismatch == 0 ? 0 : 6;
 }
