@@ -168,6 +168,49 @@ __device__ inline void block_incl_scan(const int64_t* leaf, int64_t* pref, int n
   __syncthreads();
 }
 
+// The same scan over global memory that other workgroups' atomics also touch: agent-scope loads
+// of `leaf`, agent-scope stores of the inclusive prefix into `pref` (both this block's own rows).
+__device__ inline void block_incl_scan_agent(const int64_t* leaf, int64_t* pref, int n, int64_t* tmp) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int lane = tid & 63, wv = tid >> 6, nw = nt >> 6;
+  const int per = (n + nt - 1) / nt;
+  const int b0 = min(n, tid * per), b1 = min(n, b0 + per);
+  int64_t s = 0;
+  for (int i = b0; i < b1; ++i) s += ld_agent(&leaf[i]);
+  const int64_t inc = wave_incl_scan(s);
+  if (lane == 63) tmp[wv] = inc;
+  __syncthreads();
+  if (tid < 64) {
+    const int64_t t = (tid < nw) ? tmp[tid] : 0;
+    const int64_t ti = wave_incl_scan(t);
+    if (tid < nw) tmp[tid] = ti - t;
+  }
+  __syncthreads();
+  int64_t run = tmp[wv] + inc - s;
+  for (int i = b0; i < b1; ++i) {
+    run += ld_agent(&leaf[i]);
+    __hip_atomic_store(&pref[i], run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+}
+
+// A factor edit moves group g's factor from e.b (fac_prev) to e.a (fac_val).  Every class of the
+// group's merged support -- one row of the ms_* lists -- holds mass base + the group's deviation
+// in that class (read from the block-start snapshot at `snap`), so the class's leaf changes by
+// (e.a - e.b) * mass.  The workgroup shares the row; `apply(leaf index, change)` receives every
+// nonzero change.  A row lists each class once, so callers may add without atomics.
+template <class Apply>
+__device__ __forceinline__ void support_shift(const Replay& a, const Node& e, int64_t snap, int row, Apply apply) {
+  const int64_t step = e.a - e.b;
+  const int end = a.ms_off[row + 1];
+  for (int k = a.ms_off[row] + (int)threadIdx.x; k < end; k += (int)blockDim.x) {
+    const int slot = a.ms_dev[k];
+    const int64_t mass = a.ms_base[k] + (slot < 0 ? 0 : ld_agent(&a.devbuf[snap + slot]));
+    const int64_t change = step * mass;
+    if (change != 0) apply((int)a.ms_cls[k] + 1, change);
+  }
+}
+
 // ------------------------------------------------------------- LDS block replay (v7, v7b, session)
 template <int FMT>
 __global__ void __launch_bounds__(1024) ref_block_lds_kernel(const Replay a) {
@@ -187,23 +230,18 @@ __global__ void __launch_bounds__(1024) ref_block_lds_kernel(const Replay a) {
   int v = a.bv0[b];
   while (v < v1) {
     if (node_kind<FMT>(a.nodes, v) == 2) {
-      const Node p = load_node<FMT>(a.nodes, v);
-      const int g = p.g_old;
-      const int row = a.g2row[g];
-      const int64_t off = a.need_off[(size_t)b * G + g];
-      if (row >= 0 && off >= 0) {
-        const int64_t ds = p.a - p.b;
-        const int kb = a.ms_off[row], ke = a.ms_off[row + 1];
-        for (int k = kb + tid; k < ke; k += nt) {
-          const int di = a.ms_dev[k];
-          const int64_t dev = (di >= 0) ? ld_agent(&a.devbuf[off + di]) : 0;
-          const int64_t d = ds * (a.ms_base[k] + dev);
-          if (d) leaf[(int)a.ms_cls[k] + 1] += d;  // support classes are distinct
-        }
+      // a factor edit: its support's leaf changes, then the prefix is rescanned
+      const Node e = load_node<FMT>(a.nodes, v);
+      const int row = a.g2row[e.g_old];
+      const int64_t snap = a.need_off[(size_t)b * G + e.g_old];
+      if (row >= 0 && snap >= 0) {
+        support_shift(a, e, snap, row, [&](int c, int64_t change) { leaf[c] += change; });
         __syncthreads();
         block_incl_scan(leaf, pref, Cpfx, tmp);
       }
-      if (tid == 0) fac[g] = p.a;
+      if (tid == 0) fac[e.g_old] = e.a;
+      const int g = e.g_old;
+      (void)g;
       for (int e = tid; e < 2 * q; e += nt) {
         const int idx = (e & 1) ? a.qhb[v * q + (e >> 1)] : a.qlb[v * q + (e >> 1)];
         a.lvals[(size_t)(v - 1) * (2 * q) + e] = pref[idx];
@@ -245,6 +283,7 @@ __global__ void __launch_bounds__(1024) ref_block_global_kernel(const Replay a) 
   int64_t* fac = reinterpret_cast<int64_t*>(smem_raw);
   int64_t* ww = fac + G;
   int32_t* wc = reinterpret_cast<int32_t*>(ww + 2 * kMaxWin);
+  int64_t* tmpg = reinterpret_cast<int64_t*>(wc + 2 * kMaxWin);  // 17 wavefront sums of the block scan
   const int b = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
   int64_t* fen = a.gfen + (size_t)b * Cpfx;
   int64_t* leaf = a.gleaf + (size_t)b * Cpfx;
@@ -258,41 +297,35 @@ __global__ void __launch_bounds__(1024) ref_block_global_kernel(const Replay a) 
   int v = a.bv0[b];
   while (v < v1) {
     if (node_kind<1>(a.nodes, v) == 2) {
-      const Node p = load_node<1>(a.nodes, v);
-      const int g = p.g_old;
-      const int row = a.g2row[g];
-      const int64_t off = a.need_off[(size_t)b * G + g];
-      if (row >= 0 && off >= 0) {
-        const int64_t ds = p.a - p.b;
-        const int kb = a.ms_off[row], ke = a.ms_off[row + 1];
-        // a support much smaller than the class domain goes in as Fenwick point updates; a large
-        // one is swept into the leaf and the tree is rebuilt (the reference always rebuilds)
+      const Node e = load_node<1>(a.nodes, v);
+      const int row = a.g2row[e.g_old];
+      const int64_t snap = a.need_off[(size_t)b * G + e.g_old];
+      if (row >= 0 && snap >= 0) {
+        // a support much smaller than the class domain goes into the tree as point updates; a
+        // large one only into the leaf, and the tree is then rebuilt from the leaf's prefix
         const int lg = 32 - __builtin_clz((unsigned)Cpfx);
-        const bool point = (int64_t)(ke - kb) * lg * 2 < (int64_t)Cpfx;
-        for (int k = kb + tid; k < ke; k += nt) {
-          const int di = a.ms_dev[k];
-          const int64_t dev = (di >= 0) ? ld_agent(&a.devbuf[off + di]) : 0;
-          const int64_t d = ds * (a.ms_base[k] + dev);
-          if (!d) continue;
-          const int c1 = (int)a.ms_cls[k] + 1;
-          add_agent(&leaf[c1], d);
+        const bool point = (int64_t)(a.ms_off[row + 1] - a.ms_off[row]) * lg * 2 < (int64_t)Cpfx;
+        support_shift(a, e, snap, row, [&](int c, int64_t change) {
+          add_agent(&leaf[c], change);
           if (point)
-            for (int i = c1; i <= ncls; i += i & -i) add_agent(&fen[i], d);
-        }
+            for (int i = c; i <= ncls; i += i & -i) add_agent(&fen[i], change);
+        });
         __syncthreads();
         if (!point) {
-          // rebuild the Fenwick tree from the leaf: copy, then one level per lowest-set-bit s
-          for (int i = tid; i < Cpfx; i += nt)
-            __hip_atomic_store(&fen[i], ld_agent(&leaf[i]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          // node i of the tree holds leaf (i - lsb(i), i], i.e. P[i] - P[i - lsb(i)] with P the
+          // leaf's inclusive prefix: one block scan into this block's dV row (free until the end,
+          // when it receives the leaf), then every node from two prefix values
+          int64_t* P = a.dV + (size_t)b * Cpfx;
+          block_incl_scan_agent(leaf, P, Cpfx, tmpg);
+          for (int i = tid + 1; i < Cpfx; i += nt)
+            __hip_atomic_store(&fen[i], ld_agent(&P[i]) - ld_agent(&P[i - (i & -i)]), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
           __syncthreads();
-          for (int s = 1; s < Cpfx; s <<= 1) {
-            for (int64_t i = (int64_t)s * (2 * tid + 1); i + s < Cpfx; i += 2 * (int64_t)s * nt)
-              add_agent(&fen[i + s], ld_agent(&fen[i]));
-            __syncthreads();
-          }
         }
       }
-      if (tid == 0) fac[g] = p.a;
+      if (tid == 0) fac[e.g_old] = e.a;
+      const int g = e.g_old;
+      (void)g;
       for (int e = tid; e < 2 * q; e += nt) {
         const int idx = (e & 1) ? a.qhb[v * q + (e >> 1)] : a.qlb[v * q + (e >> 1)];
         a.lvals[(size_t)(v - 1) * (2 * q) + e] = gfen_prefix(fen, idx);
@@ -415,7 +448,9 @@ int window_for(int q, int threads) {
 size_t lds_dense_bytes(int Cpfx, int G) {
   return (2 * (size_t)Cpfx + (size_t)G + 2 * kMaxWin + 17) * sizeof(int64_t) + 2 * kMaxWin * sizeof(int32_t);
 }
-size_t lds_global_bytes(int G) { return ((size_t)G + 2 * kMaxWin) * sizeof(int64_t) + 2 * kMaxWin * sizeof(int32_t); }
+size_t lds_global_bytes(int G) {
+  return ((size_t)G + 2 * kMaxWin + 17) * sizeof(int64_t) + 2 * kMaxWin * sizeof(int32_t);
+}
 
 int report(const char* what, hipError_t e) {
   if (e != hipSuccess) fprintf(stderr, "bwgpu: %s: %s\n", what, hipGetErrorString(e));
