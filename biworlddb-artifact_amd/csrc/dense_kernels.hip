@@ -33,16 +33,18 @@ using namespace bw;
 namespace {
 
 // ---------------------------------------------------------------- sort key layout
-// key = rank << 45 | query id << 18 | lo & 0x3FFFF;  val = hi | (lo >> 18) << 21
-constexpr int kRankShift = 45;
-constexpr int kIdShift = 18;
+// key = rank << 44 | query id << 17 | lo & 0x1FFFF;  val = hi | (lo >> 17) << 21.  The sorted bits
+// [44, 44 + 19) end below bit 64: rocPRIM's path for inputs under 2^20 items (merge sort) left keys
+// sorted on bits [45, 64) out of order here, while its onesweep path sorted them.
+constexpr int kRankShift = 44;
+constexpr int kIdShift = 17;
 constexpr uint64_t kIdMask = (uint64_t(1) << 27) - 1;
-constexpr uint32_t kLoLowMask = (1u << 18) - 1;
+constexpr uint32_t kLoLowMask = (1u << 17) - 1;
 constexpr uint32_t kHiMask = (1u << 21) - 1;
 
 __device__ __forceinline__ void unpack(uint64_t key, uint32_t val, uint32_t& qid, uint32_t& lo, uint32_t& hi) {
   qid = (uint32_t)((key >> kIdShift) & kIdMask);
-  lo = ((uint32_t)key & kLoLowMask) | ((val >> 21) << 18);
+  lo = ((uint32_t)key & kLoLowMask) | ((val >> 21) << 17);
   hi = val & kHiMask;
 }
 
@@ -188,7 +190,7 @@ __global__ void __launch_bounds__(256) bwd_keys_kernel(const BwdIndex x, int64_t
   }
   const uint64_t e = (uint64_t)(uint32_t)x.state_erank[v];
   keys[q] = (e << kRankShift) | ((uint64_t)q << kIdShift) | (uint64_t)(lo & kLoLowMask);
-  vals[q] = hi | ((lo >> 18) << 21);
+  vals[q] = hi | ((lo >> 17) << 21);
 }
 
 // grp[e] = first sorted position whose rank is >= e, for e in [0, S]; grp[S] = M

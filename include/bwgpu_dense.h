@@ -32,8 +32,8 @@
 extern "C" {
 #endif
 
-/* Limits of this form: endpoints are packed into the sort key (20 bits of lo, 21 of hi), state
- * ranks take 19 bits and query ids 27 bits. */
+/* Limits of this form: a query's state rank (19 bits), id (27 bits) and endpoints (up to 2^20
+ * each) are packed into one 64-bit sort key and one 32-bit value. */
 #define BWD_MAX_NK (1u << 20)
 #define BWD_MAX_STATES ((1 << 19) - 1)
 #define BWD_MAX_QUERIES (1ll << 27)
