@@ -30,7 +30,19 @@
 
 using namespace bw;
 
+// ---------------------------------------------------------------- the allocation-free entry's scratch
+// bwd_answer_noalloc_launch plans and answers a batch in this array instead of a caller's
+// workspace: every GPU the library runs on holds one copy, at its own address, until the process
+// ends.  Host code must not pass `scratch` itself to a kernel (on the host the name is only a
+// stand-in); its device address comes from hipGetSymbolAddress, once per device.
+static constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_at[kMaxDevices] = {};
 
 // ---------------------------------------------------------------- sort key layout
 // key = rank << 44 | query id << 17 | lo & 0x1FFFF;  val = hi | (lo >> 17) << 21.  The sorted bits
@@ -643,10 +655,55 @@ extern "C" int bwd_replay_launch(const BwdIndex* x, int64_t M, int64_t* answers,
   return report("dense block replay", hipGetLastError());
 }
 
+// the current device's address of `scratch` (looked up on the first call on each device)
+hipError_t scratch_address(void** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_at[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e) return e;
+    g_scratch_at[dev] = p;
+  }
+  *out = g_scratch_at[dev];
+  return hipSuccess;
+}
+
 extern "C" int bwd_answer_launch(const BwdIndex* x, int64_t M, const int32_t* qstate, const uint32_t* qlo,
                                  const uint32_t* qhi, int64_t* answers, int64_t* flags, const BwdLaunch* l, void* work,
                                  size_t work_bytes, void* stream) {
   const int rc = bwd_plan_launch(x, M, qstate, qlo, qhi, flags, work, work_bytes, stream);
   if (rc) return rc;
   return bwd_replay_launch(x, M, answers, l, work, work_bytes, stream);
+}
+
+extern "C" int bwd_answer_noalloc_launch(const BwdIndex* x, int64_t M, const int32_t* qstate, const uint32_t* qlo,
+                                         const uint32_t* qhi, int64_t* answers, int64_t* flags, const BwdLaunch* l,
+                                         void* stream) {
+  if (!index_ok(x) || M < 1 || M > BWD_MAX_QUERIES) return (int)hipErrorInvalidValue;
+  AnswerLayout L;
+  hipError_t e = answer_layout(x, M, L);
+  if (e) return report("answer workspace", e);
+  if (L.total > kScratchBytes) {
+    fprintf(stderr, "bwd_answer_noalloc_launch: a batch of %lld queries over %lld states needs %zu bytes of scratch; "
+                    "this entry point holds %zu (use bwd_answer_launch with a workspace)\n",
+            (long long)M, (long long)x->S, L.total, kScratchBytes);
+    return (int)hipErrorInvalidValue;
+  }
+  void* work = nullptr;
+  if ((e = scratch_address(&work))) return report("scratch address", e);
+  return bwd_answer_launch(x, M, qstate, qlo, qhi, answers, flags, l, work, kScratchBytes, stream);
+}
+
+extern "C" int bwd_answer_noalloc_capacity(const BwdIndex* x, int64_t M, size_t* bytes_needed, size_t* bytes_held) {
+  if (!bytes_needed || !bytes_held || !index_ok(x) || M < 1 || M > BWD_MAX_QUERIES) return (int)hipErrorInvalidValue;
+  AnswerLayout L;
+  const hipError_t e = answer_layout(x, M, L);
+  if (e) return report("answer workspace", e);
+  *bytes_needed = L.total;
+  *bytes_held = kScratchBytes;
+  return 0;
 }

@@ -101,7 +101,7 @@ int main(int argc, char** argv) {
   LaunchKnobs lk;
   int warmup = 1, repeat = 3, plain_sample = 8;
   bool want_sha = false, stages = false, dev_plan = false, dev_gen = false, dev_index = false, dev_all = false;
-  bool dense = false;
+  bool dense = false, noalloc_check = false;
   int dense_rmax = 4096, dense_threads = 1024;
   int fork = 1;  // replay the block and level-B stages on two streams
   std::string answers_path, json_path, tuned_path;
@@ -114,6 +114,7 @@ int main(int argc, char** argv) {
     if (!std::strcmp(argv[i], "--device-index")) { dev_index = dev_plan = true; continue; }
     if (!std::strcmp(argv[i], "--device-all")) { dev_all = true; continue; }
     if (!std::strcmp(argv[i], "--dense")) { dense = dev_plan = true; continue; }
+    if (!std::strcmp(argv[i], "--noalloc-check")) { noalloc_check = dense = dev_plan = true; continue; }
     if (i + 1 >= argc) { std::fprintf(stderr, "bw_batch: missing value for %s\n", argv[i]); return 2; }
     const char* k = argv[i];
     const char* v = argv[++i];
@@ -451,8 +452,8 @@ int main(int argc, char** argv) {
   // to dense_rmax keys, one dense carry row per block (built once), then per step the whole
   // stream planned (one radix sort) and answered (one persistent block-replay launch, captured
   // in a hipGraph); its answers must equal the two-level form's
-  char dense_json[1024] = "null";
-  int64_t dense_mismatch = -1;
+  char dense_json[1024] = "null", noalloc_json[512] = "null";
+  int64_t dense_mismatch = -1, noalloc_mismatch = -1;
   if (dense) {
     PlanKnobs dk = kn;
     dk.rmax = dense_rmax;
@@ -534,6 +535,52 @@ int main(int argc, char** argv) {
     HIPX(hipGraphExecDestroy(dgx));
     HIPX(hipGraphDestroy(dg));
     HIPX(hipEventDestroy(em));
+    // --noalloc-check: the allocation-free entry point (library-held scratch) against the
+    // workspace entry point on two prefixes of the stream in a row, the second smaller; both must
+    // also equal the whole stream's answers.  A batch beyond its scratch must be refused.
+    if (noalloc_check) {
+      size_t need = 0, held = 0;
+      int64_t m1 = std::min<int64_t>(M, int64_t(1) << 21);
+      while (m1 > 1 && (bwd_answer_noalloc_capacity(&D.x, m1, &need, &held) != 0 || need > held)) m1 = m1 * 3 / 4;
+      const int64_t m2 = std::max<int64_t>(1, m1 / 3 + 7 < m1 ? m1 / 3 + 7 : m1 - 1);
+      DeviceArena na;
+      int64_t* a_ref = na.scratch<int64_t>(size_t(m1));
+      int64_t* a_na = na.scratch<int64_t>(size_t(m1));
+      int64_t* f_na = na.scratch<int64_t>(1);
+      if (!na.ok()) return 5;
+      noalloc_mismatch = 0;
+      std::vector<int64_t> hr(static_cast<size_t>(m1)), hn(static_cast<size_t>(m1));
+      int64_t diff[2][3] = {{0, 0, 0}, {0, 0, 0}};  // per call: ref vs noalloc, ref vs stream, noalloc vs stream
+      int call = 0;
+      for (const int64_t m : {m1, m2}) {
+        RUN(bwd_answer_launch(&D.x, m, d_qstate, d_qlo, d_qhi, a_ref, DBn.flags, &D.launch, DBn.work, DBn.work_bytes,
+                              st));
+        RUN(bwd_answer_noalloc_launch(&D.x, m, d_qstate, d_qlo, d_qhi, a_na, f_na, &D.launch, st));
+        HIPX(hipStreamSynchronize(st));
+        HIPX(hipMemcpy(hr.data(), a_ref, size_t(m) * 8, hipMemcpyDeviceToHost));
+        HIPX(hipMemcpy(hn.data(), a_na, size_t(m) * 8, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < m; ++i) {
+          diff[call][0] += hr[size_t(i)] != hn[size_t(i)];
+          diff[call][1] += hr[size_t(i)] != ans[size_t(i)];
+          diff[call][2] += hn[size_t(i)] != ans[size_t(i)];
+        }
+        noalloc_mismatch += diff[call][0] + diff[call][2];
+        ++call;
+      }
+      int64_t fl = 0;
+      HIPX(hipMemcpy(&fl, f_na, sizeof(fl), hipMemcpyDeviceToHost));
+      noalloc_mismatch += fl != 0;
+      // the whole stream needs far more than the scratch: refused before any device work
+      const bool refused = M > m1 && bwd_answer_noalloc_launch(&D.x, M, d_qstate, d_qlo, d_qhi, a_na, f_na, &D.launch,
+                                                               st) != 0;
+      if (M > m1 && !refused) ++noalloc_mismatch;
+      std::snprintf(noalloc_json, sizeof(noalloc_json),
+                    "{\"m1\":%" PRId64 ",\"m2\":%" PRId64 ",\"scratch_bytes\":%zu,\"m1_needs_bytes\":%zu,"
+                    "\"refuses_oversize\":%s,\"diff\":[[%" PRId64 ",%" PRId64 ",%" PRId64 "],[%" PRId64 ",%" PRId64
+                    ",%" PRId64 "]],\"mismatch\":%" PRId64 "}",
+                    m1, m2, held, need, refused ? "true" : "false", diff[0][0], diff[0][1], diff[0][2], diff[1][0],
+                    diff[1][1], diff[1][2], noalloc_mismatch);
+    }
   }
 
   // ---- the index tables built on the device (--device-index): compared with the host-built
@@ -709,7 +756,7 @@ int main(int argc, char** argv) {
       "\"plain\":{\"batches\":%" PRId64 ",\"queries\":%" PRId64 ",\"pass_ms\":%.4f,\"query_ms\":%.4f,"
       "\"pass_queries_per_s\":%.1f,\"query_queries_per_s\":%.1f,\"mismatch\":%" PRId64 "},"
       "\"device_plan\":{\"used\":%s,\"ms\":%.4f,\"mismatch\":%" PRId64 ",\"step_ms\":%.4f,\"step_queries_per_s\":%.1f},"
-      "\"device_index\":%s,\"device_all\":%s,\"dense\":%s,\"device_gen\":{\"used\":%s,\"gpu_ms\":%.4f,\"wall_ms\":%.3f,\"host_ms\":%.3f,\"ties\":%u,"
+      "\"device_index\":%s,\"device_all\":%s,\"dense\":%s,\"noalloc\":%s,\"device_gen\":{\"used\":%s,\"gpu_ms\":%.4f,\"wall_ms\":%.3f,\"host_ms\":%.3f,\"ties\":%u,"
       "\"mismatch\":%" PRId64 "},\"forked\":%d,"
       "\"device_bytes\":%zu,\"device\":\"%s\",\"cus\":%d}\n",
       spec.S, spec.U, spec.N, spec.seed, ss.nreq, ss.max_batch, M, sha.c_str(), E.plan.nblk(), BP.qblk.size(),
@@ -722,7 +769,7 @@ int main(int argc, char** argv) {
       plain_query_ms, plain_pass_ms > 0 ? double(plain_q) / (plain_pass_ms * 1e-3) : 0.0,
       plain_query_ms > 0 ? double(plain_q) / (plain_query_ms * 1e-3) : 0.0, plain_mismatch,
       dev_plan ? "true" : "false", dplan_ms, plan_mismatch, step2_avg,
-      step2_avg > 0 ? double(M) / (step2_avg * 1e-3) : 0.0, didx_json, dall_json, dense_json, dev_gen ? "true" : "false", dgen_gpu_ms, dgen_wall_ms,
+      step2_avg > 0 ? double(M) / (step2_avg * 1e-3) : 0.0, didx_json, dall_json, dense_json, noalloc_json, dev_gen ? "true" : "false", dgen_gpu_ms, dgen_wall_ms,
       stream_host_ms, DS.ties, dgen_mismatch, fork, E.arena.bytes + io.bytes + DB.arena.bytes + dslot.arena.bytes,
       dev, prop.multiProcessorCount);
   std::fwrite(buf, 1, size_t(n), stdout);
@@ -733,7 +780,7 @@ int main(int argc, char** argv) {
     std::fclose(f);
   }
   return plain_mismatch == 0 && plan_mismatch <= 0 && dgen_mismatch <= 0 && didx_mismatch <= 0 && dall_mismatch <= 0 &&
-                     dense_mismatch <= 0
+                     dense_mismatch <= 0 && noalloc_mismatch <= 0
              ? 0
              : 6;
 }

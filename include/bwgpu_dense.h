@@ -96,6 +96,20 @@ int bwd_answer_launch(const BwdIndex* x, int64_t M, const int32_t* qstate, const
                       int64_t* answers, int64_t* flags, const BwdLaunch* l, void* work, size_t work_bytes,
                       void* stream);
 
+/* Allocation-free variant of bwd_answer_launch, for callers that cannot allocate device memory
+ * (tight serving loops, harnesses that forbid hipMalloc).  No counterpart in the reference.  It plans and answers in a library-held scratch array of 64 MiB
+ * per GPU (held until the process ends) instead of a caller's workspace, and calls no hipMalloc,
+ * hipFree or allocating library routine; the caller still passes inputs, answers and flags.
+ * A batch whose plan needs more scratch than that (about 2.5M queries over 300,000 states; see
+ * bwd_answer_noalloc_capacity) is rejected with hipErrorInvalidValue and a message on stderr.
+ * Every call on a GPU shares that GPU's array and leaves its contents behind: calls that overlap
+ * on one GPU (two streams, two threads) race and must be serialised by the caller.  Like
+ * bwd_plan_launch it issues device memsets (the radix sort's), so it is not for graph capture. */
+int bwd_answer_noalloc_launch(const BwdIndex* x, int64_t M, const int32_t* qstate, const uint32_t* qlo,
+                              const uint32_t* qhi, int64_t* answers, int64_t* flags, const BwdLaunch* l, void* stream);
+/* Scratch bytes a batch of M queries needs, and the bytes the allocation-free entry point holds. */
+int bwd_answer_noalloc_capacity(const BwdIndex* x, int64_t M, size_t* bytes_needed, size_t* bytes_held);
+
 #ifdef __cplusplus
 }
 #endif
