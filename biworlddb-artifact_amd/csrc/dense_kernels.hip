@@ -8,7 +8,8 @@
 //                       running sum over blocks, computed per (coordinate range, block segment)
 //                       with a scan over segments in between, so ~45k workgroups share the work.
 //   plan             <- qlb/qhb (src/batch_engine_v7.hpp:666-698): queries sorted by the preorder
-//                       rank of their state (rocPRIM radix sort), group offsets per rank.
+//                       rank of their state (rocPRIM radix sort in two 10-bit passes), group offsets
+//                       per rank.
 //   persistent block <- v7_block_kernel + gpu_combine_kernel (src/gpu_v7.cu:10-96): workgroups
 //                       take blocks from a device queue; each replays its block's steps (events
 //                       are ds_add_u64 into a dense per-rank array; at every enter step a DPP
@@ -188,21 +189,47 @@ __global__ void bwd_init_kernel(int64_t* flags, uint32_t* counter) {
   }
 }
 
-__global__ void __launch_bounds__(256) bwd_keys_kernel(const BwdIndex x, int64_t M, const int32_t* qstate,
-                                                       const uint32_t* qlo, const uint32_t* qhi, uint64_t* keys,
-                                                       uint32_t* vals, int64_t* flags) {
-  const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (q >= M) return;
-  int32_t v = qstate[q];
-  uint32_t lo = qlo[q], hi = qhi[q];
-  if (v < 1 || v > x.S || lo > x.NK || hi > x.NK) {
-    flags[0] = 1;
-    v = 1;
-    lo = hi = 0;
+// The sort's input: key = rank << 44 | query id << 17 | lo & 0x1FFFF, val = hi | (lo >> 17) << 21.
+// An out-of-range query sets flags[0] and is planned as a query of state 1 over [0, 0).  Each
+// thread packs kKeysPer queries a workgroup apart (coalesced), their loads all issued before the
+// rank gathers, so each wave has several loads in flight.
+constexpr int kKeysPer = 4;
+__global__ void __launch_bounds__(256) bwd_keys_kernel(const BwdIndex x, int64_t M, const int32_t* __restrict__ qstate,
+                                                       const uint32_t* __restrict__ qlo,
+                                                       const uint32_t* __restrict__ qhi, uint64_t* __restrict__ keys,
+                                                       uint32_t* __restrict__ vals, int64_t* flags) {
+  const int64_t q0 = (int64_t)blockIdx.x * (256 * kKeysPer) + threadIdx.x;
+  int32_t v[kKeysPer];
+  uint32_t lo[kKeysPer], hi[kKeysPer];
+#pragma unroll
+  for (int k = 0; k < kKeysPer; ++k) {
+    const int64_t q = q0 + k * 256;
+    const bool in = q < M;
+    v[k] = in ? qstate[q] : 1;
+    lo[k] = in ? qlo[q] : 0;
+    hi[k] = in ? qhi[q] : 0;
   }
-  const uint64_t e = (uint64_t)(uint32_t)x.state_erank[v];
-  keys[q] = (e << kRankShift) | ((uint64_t)q << kIdShift) | (uint64_t)(lo & kLoLowMask);
-  vals[q] = hi | ((lo >> 17) << 21);
+  bool bad = false;
+#pragma unroll
+  for (int k = 0; k < kKeysPer; ++k) {
+    if (v[k] < 1 || v[k] > x.S || lo[k] > x.NK || hi[k] > x.NK) {
+      bad = true;
+      v[k] = 1;
+      lo[k] = hi[k] = 0;
+    }
+  }
+  uint32_t e[kKeysPer];
+#pragma unroll
+  for (int k = 0; k < kKeysPer; ++k) e[k] = (uint32_t)x.state_erank[v[k]];
+#pragma unroll
+  for (int k = 0; k < kKeysPer; ++k) {
+    const int64_t q = q0 + k * 256;
+    if (q < M) {
+      keys[q] = ((uint64_t)e[k] << kRankShift) | ((uint64_t)q << kIdShift) | (uint64_t)(lo[k] & kLoLowMask);
+      vals[q] = hi[k] | ((lo[k] >> 17) << 21);
+    }
+  }
+  if (bad) flags[0] = 1;
 }
 
 // grp[e] = first sorted position whose rank is >= e, for e in [0, S]; grp[S] = M
@@ -486,6 +513,19 @@ CarryLayout carry_layout(const BwdIndex* x, int64_t nkeys) {
   return L;
 }
 
+// The plan's sort: 10-bit digits, so the 19 rank bits of a 300,000-state tree take two passes
+// (rocPRIM's default for 64-bit keys takes three 8-bit passes); 1,024-thread workgroups of 12 items.
+using PlanSortConfig = rocprim::radix_sort_config<
+    rocprim::default_config, rocprim::default_config,
+    rocprim::radix_sort_onesweep_config<rocprim::kernel_config<1024, 16>, rocprim::kernel_config<1024, 12>, 10,
+                                        rocprim::block_radix_rank_algorithm::match>>;
+
+hipError_t plan_sort(void* temp, size_t& temp_bytes, const uint64_t* kin, uint64_t* kout, const uint32_t* vin,
+                     uint32_t* vout, int64_t M, unsigned end_bit, hipStream_t st) {
+  return rocprim::radix_sort_pairs<PlanSortConfig>(temp, temp_bytes, kin, kout, vin, vout, (size_t)M,
+                                                   (unsigned)kRankShift, end_bit, st);
+}
+
 struct AnswerLayout {
   size_t kin, kout, vin, vout, grp, counter, temp, temp_bytes, total;
   unsigned end_bit;
@@ -494,9 +534,7 @@ struct AnswerLayout {
 hipError_t answer_layout(const BwdIndex* x, int64_t M, AnswerLayout& L) {
   L.end_bit = kRankShift + bits_for((uint64_t)(x->S > 1 ? x->S - 1 : 1));
   L.temp_bytes = 0;
-  const hipError_t e = rocprim::radix_sort_pairs(nullptr, L.temp_bytes, (const uint64_t*)nullptr, (uint64_t*)nullptr,
-                                                 (const uint32_t*)nullptr, (uint32_t*)nullptr, (size_t)M,
-                                                 (unsigned)kRankShift, L.end_bit);
+  const hipError_t e = plan_sort(nullptr, L.temp_bytes, nullptr, nullptr, nullptr, nullptr, M, L.end_bit, nullptr);
   if (e) return e;
   size_t o = 0;
   auto take = [&](size_t n) {
@@ -599,13 +637,13 @@ extern "C" int bwd_plan_launch(const BwdIndex* x, int64_t M, const int32_t* qsta
   uint32_t* grp = at<uint32_t>(work, L.grp);
   int rc;
   hipLaunchKernelGGL(bwd_init_kernel, dim3(1), dim3(64), 0, st, flags, (uint32_t*)nullptr);
-  const unsigned grid = (unsigned)((M + 255) / 256);
-  hipLaunchKernelGGL(bwd_keys_kernel, dim3(grid), dim3(256), 0, st, *x, M, qstate, qlo, qhi, kin, vin, flags);
+  const unsigned kgrid = (unsigned)((M + 256 * kKeysPer - 1) / (256 * kKeysPer));
+  hipLaunchKernelGGL(bwd_keys_kernel, dim3(kgrid), dim3(256), 0, st, *x, M, qstate, qlo, qhi, kin, vin, flags);
   if ((rc = report("plan keys", hipGetLastError()))) return rc;
   size_t tb = L.temp_bytes;
-  if ((rc = report("plan sort", rocprim::radix_sort_pairs(at<void>(work, L.temp), tb, kin, kout, vin, vout, (size_t)M,
-                                                          (unsigned)kRankShift, L.end_bit, st))))
+  if ((rc = report("plan sort", plan_sort(at<void>(work, L.temp), tb, kin, kout, vin, vout, M, L.end_bit, st))))
     return rc;
+  const unsigned grid = (unsigned)((M + 255) / 256);
   hipLaunchKernelGGL(bwd_groups_kernel, dim3(grid), dim3(256), 0, st, x->S, M, kout, grp);
   return report("plan groups", hipGetLastError());
 }
