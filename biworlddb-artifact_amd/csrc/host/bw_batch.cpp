@@ -25,7 +25,11 @@
 // sequence); the answers must be the same.  --device-all then runs the whole pipeline on the device
 // from the seed -- updates and queries generated, index built, stream planned and answered,
 // answers hashed -- timed phase by phase; its answers must be the same too.
-// Exit codes: 2 usage, 3 unsupported input, 4 I/O, 5 GPU error.
+// --dense [--dense-rmax 4096 --dense-threads 1024 --dense-wg 1] also answers the stream with the
+// dense-carry form (include/bwgpu_dense.h): blocks of up to --dense-rmax keys, the persistent
+// replay at --dense-threads threads and --dense-wg workgroups per CU; its answers must be the same.
+// --noalloc-check also compares its allocation-free entry point with the workspace entry point.
+// Exit codes: 2 usage, 3 unsupported input, 4 I/O, 5 GPU error, 6 answers differ.
 #include <hip/hip_runtime_api.h>
 #include <openssl/sha.h>
 
@@ -102,7 +106,7 @@ int main(int argc, char** argv) {
   int warmup = 1, repeat = 3, plain_sample = 8;
   bool want_sha = false, stages = false, dev_plan = false, dev_gen = false, dev_index = false, dev_all = false;
   bool dense = false, noalloc_check = false;
-  int dense_rmax = 4096, dense_threads = 1024;
+  int dense_rmax = 4096, dense_threads = 1024, dense_wg = 1;
   int fork = 1;  // replay the block and level-B stages on two streams
   std::string answers_path, json_path, tuned_path;
   bool have_knob[8] = {false, false, false, false, false, false, false, false};
@@ -138,6 +142,7 @@ int main(int argc, char** argv) {
     else if (!std::strcmp(k, "--fork")) fork = std::atoi(v);
     else if (!std::strcmp(k, "--dense-rmax")) dense_rmax = std::atoi(v);
     else if (!std::strcmp(k, "--dense-threads")) dense_threads = std::atoi(v);
+    else if (!std::strcmp(k, "--dense-wg")) dense_wg = std::atoi(v);
     else if (!std::strcmp(k, "--seg")) kn.seg_chunks = std::atoi(v);
     else if (!std::strcmp(k, "--threads")) { lk.block_threads = std::atoi(v); have_knob[4] = true; }
     else if (!std::strcmp(k, "--waves")) { lk.chunk_waves = std::atoi(v); have_knob[5] = true; }
@@ -466,7 +471,7 @@ int main(int argc, char** argv) {
     }
     RUN(E4.build_index(st));
     DenseEngine D;
-    D.launch = BwdLaunch{dense_threads, 1};
+    D.launch = BwdLaunch{dense_threads, dense_wg};
     if (!D.init(E4, st, err, sizeof(err))) {
       std::fprintf(stderr, "bw_batch: %s\n", err);
       return 5;
@@ -525,11 +530,11 @@ int main(int argc, char** argv) {
     };
     const double sm = mean(step_ms);
     std::snprintf(dense_json, sizeof(dense_json),
-                  "{\"rmax\":%d,\"nblk\":%d,\"threads\":%d,\"carry_bytes\":%zu,\"carry_build_ms\":%.3f,"
+                  "{\"rmax\":%d,\"nblk\":%d,\"threads\":%d,\"wg_per_cu\":%d,\"carry_bytes\":%zu,\"carry_build_ms\":%.3f,"
                   "\"index_wall_ms\":%.3f,\"step_ms\":{\"avg\":%.4f,\"min\":%.4f,\"max\":%.4f,\"repeat\":%d},"
                   "\"plan_ms\":%.4f,\"replay_ms\":%.4f,\"queries_per_s\":%.1f,\"answers_sha256\":\"%s\","
                   "\"mismatch\":%" PRId64 "}",
-                  dense_rmax, E4.x.nblk, dense_threads, D.carry_bytes, D.build_ms, dense_index_wall, sm,
+                  dense_rmax, E4.x.nblk, dense_threads, dense_wg, D.carry_bytes, D.build_ms, dense_index_wall, sm,
                   *std::min_element(step_ms.begin(), step_ms.end()), *std::max_element(step_ms.begin(), step_ms.end()),
                   repeat, mean(dplan_v), mean(drep_v), double(M) / (sm * 1e-3), dxr.sha256_hex, dense_mismatch);
     HIPX(hipGraphExecDestroy(dgx));
