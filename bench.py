@@ -36,7 +36,10 @@ Beside it, measured in the same run (every figure planning-inclusive unless name
                per update: key, delta) over the persistent replay's average HIP-event time, against
                the MI355X HBM peak (8 TB/s); the whole step's figure beside it; `traffic` is the
                rocprofv3 FETCH_SIZE + WRITE_SIZE of that kernel per launch (profiles/), scaled by the
-               same counters' reading for a copy of known size, when that summary exists;
+               same counters' reading for a copy of known size, when that summary exists; and the
+               replay's random-access floor: its 2 carry-row reads and 1 answer store per query at
+               the rates bw_floor measures on the device (random 8-byte gathers over 8 MiB rows,
+               random 8-byte stores), against the replay's time;
   cpu_baseline -- the oracle's rollback-Fenwick stream solver (a port, not the reference engine:
                that engine cannot take branching multi-update trees) on the host's cores over the
                same whole stream; its answers' SHA-256 must equal the GPU's.
@@ -55,6 +58,7 @@ BW_BATCH = os.path.join(BUILD, "bw_batch")
 BW_TREE = os.path.join(BUILD, "bw_tree")
 BW_SERVE = os.path.join(BUILD, "bw_serve")
 BW_PANEL = os.path.join(BUILD, "bw_panel")
+BW_FLOOR = os.path.join(BUILD, "bw_floor")
 TUNED = os.path.join(REPO, "biworlddb-artifact_amd", "tuned.json")
 ORACLE = os.path.join(REPO, "oracle", "bw_oracle")
 REF_ENGINE = os.path.join(REPO, "oracle", "_ref", "compiled_300000")
@@ -156,7 +160,7 @@ def main():
         print("bench.py: measures one GPU only (no multi-GPU run is made); the path would shard by contiguous "
               "block ranges of the Euler walk with one exclusive scan of per-GPU carry totals.", file=sys.stderr)
         sys.exit(2)
-    for exe in (BW_BATCH, BW_TREE, BW_SERVE, BW_PANEL):
+    for exe in (BW_BATCH, BW_TREE, BW_SERVE, BW_PANEL, BW_FLOOR):
         if not os.path.exists(exe):
             sys.exit("bench.py: native build missing (make -C biworlddb-artifact_amd)")
     knobs = ["--tuned", TUNED] if os.path.exists(TUNED) else []
@@ -178,6 +182,10 @@ def main():
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
             traffic = json.load(f).get("bwd_block_kernel")
+    # the replay's own floor: its two random streams (two carry-row reads per query, one answer store per
+    # query) at the device's measured random-access rates
+    fl = run([BW_FLOOR, "--gathers", str(2 * M), "--scatters", str(M)], 300)
+    floor_ms = (2 * M / fl["gather"]["per_s"] + M / fl["scatter"]["per_s"]) * 1e3
     tl = r["device_plan"]
     value = d["queries_per_s"]
     out = {
@@ -233,7 +241,15 @@ def main():
                      "bytes_alg_rule": "20 B per query (state, lo, hi, answer) + 12 B per update (key, delta)",
                      "step_achieved": round(alg_bytes / (step_ms * 1e-3) / 1e9, 1),
                      "step_frac": round(alg_bytes / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                     "traffic_over_alg": round(traffic / alg_bytes, 2) if traffic else None},
+                     "traffic_over_alg": round(traffic / alg_bytes, 2) if traffic else None,
+                     "random_access": {
+                         "rule": "2 random 8-byte carry-row reads + 1 random 8-byte answer store per query; on "
+                                 "gfx950 each such read fetches a 128-byte line",
+                         "gathers": 2 * M, "gathers_per_s": round(fl["gather"]["per_s"], 1),
+                         "scatters": M, "scatters_per_s": round(fl["scatter"]["per_s"], 1),
+                         "floor_ms": round(floor_ms, 4), "replay_ms": round(replay_ms, 4),
+                         "floor_over_replay": round(floor_ms / replay_ms, 3),
+                         "copy_GBps": fl["copy"]["GBps"]}},
         "device_index": {"total_ms": r["device_index"]["total_ms"], "device_ms": r["device_index"]["gpu_ms"],
                          "host_checks_ms": r["device_index"]["host_ms"],
                          "host_index_ms": r["device_index"]["host_plan_ms"],
