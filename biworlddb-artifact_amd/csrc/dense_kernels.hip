@@ -191,23 +191,38 @@ __global__ void bwd_init_kernel(int64_t* flags, uint32_t* counter) {
 
 // The sort's input: key = rank << 44 | query id << 17 | lo & 0x1FFFF, val = hi | (lo >> 17) << 21.
 // An out-of-range query sets flags[0] and is planned as a query of state 1 over [0, 0).  Each
-// thread packs kKeysPer queries a workgroup apart (coalesced), their loads all issued before the
-// rank gathers, so each wave has several loads in flight.
+// thread packs kKeysPer queries: with kVec four consecutive ones, read and written with 16-byte
+// accesses (the caller's arrays 16-byte aligned), else four a workgroup apart; either way all of a
+// thread's loads are issued before its rank gathers.
 constexpr int kKeysPer = 4;
+template <bool kVec>
 __global__ void __launch_bounds__(256) bwd_keys_kernel(const BwdIndex x, int64_t M, const int32_t* __restrict__ qstate,
                                                        const uint32_t* __restrict__ qlo,
                                                        const uint32_t* __restrict__ qhi, uint64_t* __restrict__ keys,
                                                        uint32_t* __restrict__ vals, int64_t* flags) {
-  const int64_t q0 = (int64_t)blockIdx.x * (256 * kKeysPer) + threadIdx.x;
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t qv = t * kKeysPer;  // kVec: the thread's first query
+  const bool whole = kVec && qv + kKeysPer <= M;
+  int64_t q[kKeysPer];
   int32_t v[kKeysPer];
   uint32_t lo[kKeysPer], hi[kKeysPer];
 #pragma unroll
-  for (int k = 0; k < kKeysPer; ++k) {
-    const int64_t q = q0 + k * 256;
-    const bool in = q < M;
-    v[k] = in ? qstate[q] : 1;
-    lo[k] = in ? qlo[q] : 0;
-    hi[k] = in ? qhi[q] : 0;
+  for (int k = 0; k < kKeysPer; ++k) q[k] = kVec ? qv + k : (int64_t)blockIdx.x * (256 * kKeysPer) + threadIdx.x + k * 256;
+  if (whole) {
+    const int4 a = *reinterpret_cast<const int4*>(qstate + qv);
+    const uint4 l = *reinterpret_cast<const uint4*>(qlo + qv);
+    const uint4 h = *reinterpret_cast<const uint4*>(qhi + qv);
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    lo[0] = l.x; lo[1] = l.y; lo[2] = l.z; lo[3] = l.w;
+    hi[0] = h.x; hi[1] = h.y; hi[2] = h.z; hi[3] = h.w;
+  } else {
+#pragma unroll
+    for (int k = 0; k < kKeysPer; ++k) {
+      const bool in = q[k] < M;
+      v[k] = in ? qstate[q[k]] : 1;
+      lo[k] = in ? qlo[q[k]] : 0;
+      hi[k] = in ? qhi[q[k]] : 0;
+    }
   }
   bool bad = false;
 #pragma unroll
@@ -218,15 +233,26 @@ __global__ void __launch_bounds__(256) bwd_keys_kernel(const BwdIndex x, int64_t
       lo[k] = hi[k] = 0;
     }
   }
-  uint32_t e[kKeysPer];
-#pragma unroll
-  for (int k = 0; k < kKeysPer; ++k) e[k] = (uint32_t)x.state_erank[v[k]];
+  uint64_t key[kKeysPer];
+  uint32_t val[kKeysPer];
 #pragma unroll
   for (int k = 0; k < kKeysPer; ++k) {
-    const int64_t q = q0 + k * 256;
-    if (q < M) {
-      keys[q] = ((uint64_t)e[k] << kRankShift) | ((uint64_t)q << kIdShift) | (uint64_t)(lo[k] & kLoLowMask);
-      vals[q] = hi[k] | ((lo[k] >> 17) << 21);
+    const uint64_t e = (uint64_t)(uint32_t)x.state_erank[v[k]];
+    key[k] = (e << kRankShift) | ((uint64_t)q[k] << kIdShift) | (uint64_t)(lo[k] & kLoLowMask);
+    val[k] = hi[k] | ((lo[k] >> 17) << 21);
+  }
+  if (whole) {
+    ulonglong2* kp = reinterpret_cast<ulonglong2*>(keys + qv);
+    kp[0] = make_ulonglong2(key[0], key[1]);
+    kp[1] = make_ulonglong2(key[2], key[3]);
+    *reinterpret_cast<uint4*>(vals + qv) = make_uint4(val[0], val[1], val[2], val[3]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < kKeysPer; ++k) {
+      if (q[k] < M) {
+        keys[q[k]] = key[k];
+        vals[q[k]] = val[k];
+      }
     }
   }
   if (bad) flags[0] = 1;
@@ -638,7 +664,11 @@ extern "C" int bwd_plan_launch(const BwdIndex* x, int64_t M, const int32_t* qsta
   int rc;
   hipLaunchKernelGGL(bwd_init_kernel, dim3(1), dim3(64), 0, st, flags, (uint32_t*)nullptr);
   const unsigned kgrid = (unsigned)((M + 256 * kKeysPer - 1) / (256 * kKeysPer));
-  hipLaunchKernelGGL(bwd_keys_kernel, dim3(kgrid), dim3(256), 0, st, *x, M, qstate, qlo, qhi, kin, vin, flags);
+  // the 16-byte form when the caller's query arrays allow it (the workspace arrays are 256-byte aligned)
+  if ((((uintptr_t)qstate | (uintptr_t)qlo | (uintptr_t)qhi) & 15) == 0)
+    hipLaunchKernelGGL(bwd_keys_kernel<true>, dim3(kgrid), dim3(256), 0, st, *x, M, qstate, qlo, qhi, kin, vin, flags);
+  else
+    hipLaunchKernelGGL(bwd_keys_kernel<false>, dim3(kgrid), dim3(256), 0, st, *x, M, qstate, qlo, qhi, kin, vin, flags);
   if ((rc = report("plan keys", hipGetLastError()))) return rc;
   size_t tb = L.temp_bytes;
   if ((rc = report("plan sort", plan_sort(at<void>(work, L.temp), tb, kin, kout, vin, vout, M, L.end_bit, st))))

@@ -523,6 +523,27 @@ int main(int argc, char** argv) {
     if (dxrc) return dxrc < 0 ? 4 : 5;
     dense_mismatch = sha == dxr.sha256_hex ? 0 : 1;
     if (dense_mismatch) std::fprintf(stderr, "bw_batch: dense-carry answers differ from the two-level form's\n");
+    // the stream shifted by one query (every query array 4 bytes past a 16-byte boundary, so the
+    // plan packs keys with its unaligned form): its answers are the whole stream's, shifted by one
+    int64_t offset_mismatch = 0;
+    {
+      int64_t* d_ans2 = DBn.arena.scratch<int64_t>(size_t(M));
+      if (!d_ans2) {
+        std::fprintf(stderr, "bw_batch: device: offset check allocation\n");
+        return 5;
+      }
+      RUN(bwd_answer_launch(&D.x, M - 1, d_qstate + 1, d_qlo + 1, d_qhi + 1, d_ans2, DBn.flags, &D.launch, DBn.work,
+                            DBn.work_bytes, st));
+      HIPX(hipStreamSynchronize(st));
+      std::vector<int64_t> full(static_cast<size_t>(M)), shifted(static_cast<size_t>(M - 1));
+      HIPX(hipMemcpy(full.data(), d_ans, sizeof(int64_t) * size_t(M), hipMemcpyDeviceToHost));
+      HIPX(hipMemcpy(shifted.data(), d_ans2, sizeof(int64_t) * size_t(M - 1), hipMemcpyDeviceToHost));
+      HIPX(hipMemcpy(&flag, DBn.flags, sizeof(flag), hipMemcpyDeviceToHost));
+      for (int64_t i = 0; i + 1 < M; ++i) offset_mismatch += shifted[size_t(i)] != full[size_t(i + 1)];
+      offset_mismatch += flag != 0;
+      if (offset_mismatch) std::fprintf(stderr, "bw_batch: the shifted stream's answers differ (%" PRId64 ")\n", offset_mismatch);
+      dense_mismatch += offset_mismatch;
+    }
     auto mean = [](const std::vector<double>& v) {
       double a = 0;
       for (double x : v) a += x;
@@ -533,10 +554,11 @@ int main(int argc, char** argv) {
                   "{\"rmax\":%d,\"nblk\":%d,\"threads\":%d,\"wg_per_cu\":%d,\"carry_bytes\":%zu,\"carry_build_ms\":%.3f,"
                   "\"index_wall_ms\":%.3f,\"step_ms\":{\"avg\":%.4f,\"min\":%.4f,\"max\":%.4f,\"repeat\":%d},"
                   "\"plan_ms\":%.4f,\"replay_ms\":%.4f,\"queries_per_s\":%.1f,\"answers_sha256\":\"%s\","
-                  "\"mismatch\":%" PRId64 "}",
+                  "\"offset_mismatch\":%" PRId64 ",\"mismatch\":%" PRId64 "}",
                   dense_rmax, E4.x.nblk, dense_threads, dense_wg, D.carry_bytes, D.build_ms, dense_index_wall, sm,
                   *std::min_element(step_ms.begin(), step_ms.end()), *std::max_element(step_ms.begin(), step_ms.end()),
-                  repeat, mean(dplan_v), mean(drep_v), double(M) / (sm * 1e-3), dxr.sha256_hex, dense_mismatch);
+                  repeat, mean(dplan_v), mean(drep_v), double(M) / (sm * 1e-3), dxr.sha256_hex, offset_mismatch,
+                  dense_mismatch);
     HIPX(hipGraphExecDestroy(dgx));
     HIPX(hipGraphDestroy(dg));
     HIPX(hipEventDestroy(em));
