@@ -15,7 +15,7 @@ namespace bwg {
 namespace {
 inline uint64_t key_item(uint32_t cls, uint64_t ref) { return (uint64_t(cls) << BWT_CLASS_SHIFT) | (ref & BWT_REF_MASK); }
 inline uint64_t ep_item(uint64_t pos, uint32_t cls, uint64_t ep) {
-  return (pos << BWT_EP_POS_SHIFT) | (uint64_t(cls) << BWT_EP_CLASS_SHIFT) | (ep & 0xFFFFFFFFull);
+  return (pos << BWT_EP_POS_SHIFT) | (uint64_t(cls) << BWT_EP_CLASS_SHIFT) | (ep & BWT_EP_ID_MASK);
 }
 }  // namespace
 
@@ -36,7 +36,8 @@ bool plan_walk_blocks(int64_t S, int U, uint64_t NK, const int32_t* parent, cons
   }
   if (ts < 1 || ts > 12) return fail("tile_shift must be in [1,12]");
   const uint64_t KT = (NK >> ts) + 1;
-  if (KT > 4095) return fail("key space too large for the tile classes (NK >> tile_shift must be < 4095)");
+  if (KT > BWT_MAX_CLASSES - 1)
+    return fail("key space too large for the tile classes (NK >> tile_shift must be < BWT_MAX_CLASSES - 2)");
   if (kn.rmax < U || kn.rmax > 8192) return fail("rmax must be in [U, 8192]");
   if ((kn.chunk_items != 0 && (kn.chunk_items < 64 || kn.chunk_items >= (int64_t(1) << 19))) ||
       kn.seg_chunks < 1 || kn.ep_per_work < 64)
@@ -387,7 +388,7 @@ bool build_batch_plan(const TreeIndexPlan& X, int64_t M, const int32_t* qstate, 
     if (err && errlen) std::snprintf(err, errlen, "%s", m);
     return false;
   };
-  if (M < 0 || 2 * M > int64_t(0xFFFFFFFF)) return fail("bad batch size");
+  if (M < 0 || M > BWT_MAX_BATCH) return fail("bad batch size");
   for (int64_t q = 0; q < M; ++q)
     if (qstate[q] < 1 || qstate[q] > X.S || qlo[q] > X.NK || qhi[q] > X.NK) return fail("query out of range");
   B = BatchPlan();
@@ -602,7 +603,7 @@ bool build_batch_plan(const TreeIndexPlan& X, int64_t M, const int32_t* qstate, 
   // item on; ordered by slot range, then chunk.  The device planner makes the same cuts.
   const int64_t G = std::max<int64_t>(2, X.knobs.ep_per_work);
   const SlotRanges SR = slot_ranges(M, X.knobs.range_slots);
-  auto slot_of = [&](int64_t e) { return int64_t(uint32_t(B.eitems[size_t(e)])) >> 1; };
+  auto slot_of = [&](int64_t e) { return int64_t(BWT_EP_ID(B.eitems[size_t(e)])) >> 1; };
   struct Work {
     int64_t range, chunk, e0, e1;
   };
@@ -613,7 +614,7 @@ bool build_batch_plan(const TreeIndexPlan& X, int64_t M, const int32_t* qstate, 
     while (a0 < ce) {
       const int64_t r = range_of(slot_of(a0), SR);
       const int64_t e = std::partition_point(B.eitems.begin() + a0, B.eitems.begin() + ce, [&](uint64_t it) {
-                          return range_of(int64_t(uint32_t(it)) >> 1, SR) == r;
+                          return range_of(int64_t(BWT_EP_ID(it)) >> 1, SR) == r;
                         }) - B.eitems.begin();
       int64_t a = a0;
       for (int64_t k = 1; a < e; ++k) {
@@ -627,8 +628,8 @@ bool build_batch_plan(const TreeIndexPlan& X, int64_t M, const int32_t* qstate, 
   }
   for (const Work& w : wk)  // pairs: a slot's two endpoints on one chain (same tile), adjacent
     for (int64_t i = w.e0; i + 1 < w.e1; ++i) {
-      const uint32_t id = uint32_t(B.eitems[size_t(i)]);
-      if (!(id & 1u) && uint32_t(B.eitems[size_t(i) + 1]) == id + 1) ++B.npairs;
+      const uint32_t id = BWT_EP_ID(B.eitems[size_t(i)]);
+      if (!(id & 1u) && BWT_EP_ID(B.eitems[size_t(i) + 1]) == id + 1) ++B.npairs;
     }
   const bool ranged = !wk.empty() && wk.back().range != wk.front().range;
   if (ranged) std::stable_sort(wk.begin(), wk.end(), [](const Work& p, const Work& q) { return p.range < q.range; });
@@ -650,7 +651,7 @@ bool build_batch_plan(const TreeIndexPlan& X, int64_t M, const int32_t* qstate, 
   // each endpoint's place in the work order (its level-B result is stored there)
   B.epos.resize(size_t(nE));
 #pragma omp parallel for schedule(static) if (nE > 8192)
-  for (int64_t i = 0; i < nE; ++i) B.epos[size_t(uint32_t(B.eitems[size_t(i)]))] = uint32_t(i);
+  for (int64_t i = 0; i < nE; ++i) B.epos[size_t(BWT_EP_ID(B.eitems[size_t(i)]))] = uint32_t(i);
   return true;
 }
 

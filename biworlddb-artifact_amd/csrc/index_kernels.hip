@@ -491,7 +491,7 @@ hipError_t layout_of(const BwtIndexBuild* in, WsLayout& L) {
 
 bool build_ok(const BwtIndexBuild* in) {
   return in && in->S >= 1 && in->U >= 1 && in->rmax >= in->U && in->rmax >= 8 && in->rmax <= 8192 &&
-         in->tile_shift >= 1 && in->tile_shift <= 12 && in->KT >= 1 && in->KT < 4096 && in->nsteps >= 1 &&
+         in->tile_shift >= 1 && in->tile_shift <= 12 && in->KT >= 1 && in->KT < BWT_MAX_CLASSES && in->nsteps >= 1 &&
          in->nblk >= 1 && in->steps && in->blk_step0 && in->key;
 }
 

@@ -53,7 +53,7 @@ struct Shape {
 };
 
 bool shape_of(const BwtIndex* x, const BwtPlanIndex* p, int64_t M, Shape& s) {
-  if (!x || !p || M < 1 || M >= (int64_t(1) << 31) || p->nsteps < 1 || p->chunk_items < 1 || p->ep_per_work < 64 ||
+  if (!x || !p || M < 1 || M > BWT_MAX_BATCH || p->nsteps < 1 || p->chunk_items < 1 || p->ep_per_work < 64 ||
       p->range_slots < 1 || x->tile_shift < 1 || x->tile_shift > 12)
     return false;
   s.M = M;
@@ -271,7 +271,7 @@ __global__ void __launch_bounds__(kT) plan_endpoints_kernel(const BwtIndex x, co
   if (k >= n) k = n - 1;
   const uint64_t pin = (uint64_t)(pos - k * CH);
   const uint64_t mask = (1u << x.tile_shift) - 1;
-  eval[e] = (pin << BWT_EP_POS_SHIFT) | ((uint64_t)(xk & mask) << BWT_EP_CLASS_SHIFT) | (uint64_t)e;
+  eval[e] = (pin << BWT_EP_POS_SHIFT) | ((uint64_t)(xk & mask) << BWT_EP_CLASS_SHIFT) | ((uint64_t)e & BWT_EP_ID_MASK);
   ekey[e] = ((uint64_t)bwg::range_of(j, sr) << (cb + pb)) | ((uint64_t)(c0 + k) << pb) | pin;
 }
 
@@ -290,7 +290,7 @@ __global__ void __launch_bounds__(kT) plan_work_flags_kernel(const BwtBatch b, i
   const int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (e >= nE) return;
   const int64_t off = e - (int64_t)a0[e];
-  auto slot = [&](int64_t i) { return (uint32_t)b.eitems[i] >> 1; };
+  auto slot = [&](int64_t i) { return BWT_EP_ID(b.eitems[i]) >> 1; };
   bool start = off == 0;
   if (off > 0 && off % G == 0) start = slot(e) != slot(e - 1);
   else if (off > G && off % G == 1) start = slot(e - 1) == slot(e - 2);
@@ -305,7 +305,7 @@ __global__ void __launch_bounds__(kT) plan_work_kernel(const BwtBatch b, int64_t
   const int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (e >= nE) return;
   const uint64_t c = wscan[e];
-  const_cast<uint32_t*>(b.epos)[(uint32_t)b.eitems[e]] = (uint32_t)e;  // the endpoint's place in the work order
+  const_cast<uint32_t*>(b.epos)[BWT_EP_ID(b.eitems[e])] = (uint32_t)e;  // the endpoint's place in the work order
   if (wflag[e] & 1u) {
     const int64_t w = (int64_t)(uint32_t)c - 1;
     const_cast<int64_t*>(b.wk_chunk)[w] = (int64_t)((ekey[e] >> pb) & ((uint64_t(1) << cb) - 1));

@@ -351,7 +351,7 @@ __global__ void __launch_bounds__(256) bwt_chunk_hist_kernel(const BwtIndex x, c
     const uint64_t nitem = (nx < i1) ? x.kitems[nx] : 0;  // prefetch
     const int64_t wt = wsrc[item & BWT_REF_MASK];
     if (write_kw) x.kweight[it] = wt;
-    lds_add64(&h[(int)((item >> BWT_CLASS_SHIFT) & 0xFFF)], wt);
+    lds_add64(&h[(int)BWT_KEY_CLASS(item)], wt);
     it = nx;
     item = nitem;
   }
@@ -367,7 +367,56 @@ __global__ void __launch_bounds__(256) bwt_chunk_hist_kernel(const BwtIndex x, c
   }
 }
 
+// Level C, one workgroup per block: the block's tile slots are its distinct key tiles in
+// ascending order (one key item each, weight tagg[slot]), so its row is a step function over the
+// tiles: row[y] = sum of the weights of its slots with tile < y.  The slots' tiles and inclusive
+// weight prefix go to LDS; each row entry is then one short search and one coalesced store.  (The
+// per-wavefront histogram below would scan the whole row in one wavefront.)
+constexpr int kRowCThreads = 256;
+__global__ void __launch_bounds__(kRowCThreads) bwt_row_c_kernel(const BwtIndex x, int ncap) {
+  // dynamic LDS only (the launch raises its limit to the whole 160 KiB): wave totals, then
+  // ncap inclusive prefix values, then ncap tiles
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  int64_t* wtot = reinterpret_cast<int64_t*>(smem_raw);
+  int64_t* pre = wtot + kRowCThreads / 64;
+  uint32_t* tl = reinterpret_cast<uint32_t*>(pre + ncap);
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int64_t c = blockIdx.x;  // level-C chunk c = block c
+  const int64_t i0 = x.chunk_key0[c];
+  // a block has at most rmax distinct keys, so at most ncap = rmax tile slots
+  const int n = min((int)(x.chunk_key0[c + 1] - i0), ncap);
+  const int ncls = x.chain_ncls[0];
+  const int64_t* wsrc = x.chain_src[0] ? x.tagg : x.agg;
+  // per thread a run of consecutive slots: its sum, a block scan of the sums, then the prefix
+  const int per = (n + kRowCThreads - 1) / kRowCThreads;
+  const int a = min(tid * per, n), b = min(a + per, n);
+  int64_t sum = 0;
+  for (int i = a; i < b; ++i) {
+    const uint64_t item = x.kitems[i0 + i];
+    tl[i] = BWT_KEY_CLASS(item);
+    sum += wsrc[item & BWT_REF_MASK];
+    pre[i] = sum;
+  }
+  const int64_t inc = wave_incl_scan(sum);
+  if (lane == 63) wtot[wv] = inc;
+  __syncthreads();
+  int64_t off = inc - sum;
+  for (int k = 0; k < wv; ++k) off += wtot[k];
+  for (int i = a; i < b; ++i) pre[i] += off;
+  __syncthreads();
+  int64_t* row = x.cp + x.chunk_cp0[c];
+  for (int y = tid; y < ncls; y += kRowCThreads) {
+    int lo = 0, hi = n;  // slots with tile < y
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (tl[mid] < (uint32_t)y) lo = mid + 1; else hi = mid;
+    }
+    row[y] = lo > 0 ? pre[lo - 1] : 0;
+  }
+}
+
 // ---------------------------------------------------------------- segmented carry scan
+constexpr int kSegBatch = 8;  // rows whose loads a thread has in flight at once
 // cp rows of one chain form a matrix [chunks][ncls]; the carry of chunk r is the exclusive
 // column sum over rows < r.  Pass A: segment sums; pass B: scan over segments; pass C: rows.
 __global__ void __launch_bounds__(256) bwt_seg_sum_kernel(const BwtIndex x) {
@@ -376,8 +425,17 @@ __global__ void __launch_bounds__(256) bwt_seg_sum_kernel(const BwtIndex x) {
   const int64_t c0 = x.seg_chunk0[s], c1 = x.seg_chunk0[s + 1];
   const int ncls = x.chain_ncls[x.chunk_chain[c0]];
   if (y >= ncls) return;
+  // kSegBatch rows' loads in flight at a time (their offsets are uniform: scalar loads)
   int64_t acc = 0;
-  for (int64_t c = c0; c < c1; ++c) acc += x.cp[x.chunk_cp0[c] + y];
+  int64_t c = c0;
+  for (; c + kSegBatch <= c1; c += kSegBatch) {
+    int64_t v[kSegBatch];
+#pragma unroll
+    for (int k = 0; k < kSegBatch; ++k) v[k] = x.cp[x.chunk_cp0[c + k] + y];
+#pragma unroll
+    for (int k = 0; k < kSegBatch; ++k) acc += v[k];
+  }
+  for (; c < c1; ++c) acc += x.cp[x.chunk_cp0[c] + y];
   x.segsum[x.seg_cp0[s] + y] = acc;
 }
 
@@ -387,7 +445,18 @@ __global__ void __launch_bounds__(256) bwt_seg_scan_kernel(const BwtIndex x) {
   if (y >= x.chain_ncls[chain]) return;
   int64_t run = 0;
   const int64_t e = x.chain_seg0[chain + 1];
-  for (int64_t s = x.chain_seg0[chain]; s < e; ++s) {
+  int64_t s = x.chain_seg0[chain];
+  for (; s + kSegBatch <= e; s += kSegBatch) {  // the batch's loads before its stores
+    int64_t v[kSegBatch];
+#pragma unroll
+    for (int k = 0; k < kSegBatch; ++k) v[k] = x.segsum[x.seg_cp0[s + k] + y];
+#pragma unroll
+    for (int k = 0; k < kSegBatch; ++k) {
+      x.segsum[x.seg_cp0[s + k] + y] = run;
+      run += v[k];
+    }
+  }
+  for (; s < e; ++s) {
     int64_t* p = x.segsum + x.seg_cp0[s] + y;
     const int64_t t = *p;
     *p = run;
@@ -402,7 +471,18 @@ __global__ void __launch_bounds__(256) bwt_seg_apply_kernel(const BwtIndex x) {
   const int ncls = x.chain_ncls[x.chunk_chain[c0]];
   if (y >= ncls) return;
   int64_t run = x.segsum[x.seg_cp0[s] + y];
-  for (int64_t c = c0; c < c1; ++c) {
+  int64_t c = c0;
+  for (; c + kSegBatch <= c1; c += kSegBatch) {  // the batch's loads before its stores
+    int64_t v[kSegBatch];
+#pragma unroll
+    for (int k = 0; k < kSegBatch; ++k) v[k] = x.cp[x.chunk_cp0[c + k] + y];
+#pragma unroll
+    for (int k = 0; k < kSegBatch; ++k) {
+      x.cp[x.chunk_cp0[c + k] + y] = run;
+      run += v[k];
+    }
+  }
+  for (; c < c1; ++c) {
     int64_t* p = x.cp + x.chunk_cp0[c] + y;
     const int64_t t = *p;
     *p = run;
@@ -491,14 +571,14 @@ __device__ __forceinline__ void chunk_replay_one(const BwtIndex& x, const BwtBat
     const int nk = min(64 - ne, nk_all - ki);
     const int ep2 = __builtin_amdgcn_readfirstlane(ep + ne), ki2 = __builtin_amdgcn_readfirstlane(ki + nk);
     load_group(ep2, ki2, it2, kx2, kw2);
-    const int kcls = (int)((kx >> (BWT_CLASS_SHIFT - 32)) & 0xFFF);
+    const int kcls = (int)((kx >> (BWT_CLASS_SHIFT - 32)) & BWT_CLASS_MASK);
     if (lane < nk) {
       kcs[lane] = kcls;
       kws[lane] = kw;
     }
     wave_lds_sync();
     if (ne > 0) {
-      const int cls = (int)((it >> BWT_EP_CLASS_SHIFT) & 0xFFF);
+      const int cls = (int)BWT_EP_CLASS(it);
       // a taken endpoint needs the group's first `rel` keys; rel grows with the lane
       const int mmax = __builtin_amdgcn_readlane(rel, ne - 1);
       const int rl = take ? rel : 0;
@@ -711,21 +791,27 @@ size_t replay_lds_bytes(const BwtIndex* x, const BwtLaunch* l) {
   return (size_t)l->chunk_waves * (size_t)replay_wave_words(level_stride(x->ncls_b)) * sizeof(int64_t);
 }
 
-size_t hist_lds_bytes(const BwtIndex* x, const BwtLaunch* l) {
-  return (size_t)l->chunk_waves * (size_t)level_stride(x->max_cls) * sizeof(int64_t);
+// wavefronts per workgroup of the chunk histogram for a level of ncls classes: the launch's, fewer
+// when that many wavefronts' class rows would not fit the workgroup's LDS
+int hist_waves(int ncls, const BwtLaunch* l) {
+  int w = l->chunk_waves;
+  while (w > 1 && (size_t)w * (size_t)level_stride(ncls) * sizeof(int64_t) > 160 * 1024) --w;
+  return w;
 }
 
 int check(const BwtIndex* x, const BwtLaunch* l) {
   if (!x || !l || x->S < 1 || x->U < 1 || x->nblk < 1) return (int)hipErrorInvalidValue;
   if (l->block_threads < 64 || l->block_threads > 1024 || (l->block_threads & 63)) return (int)hipErrorInvalidValue;
-  if (x->rmax < 1 || x->rmax > 8192 || x->max_cls < 1 || x->max_cls > 4096) return (int)hipErrorInvalidValue;
+  if (x->rmax < 1 || x->rmax > 8192 || x->max_cls < 1 || x->max_cls > BWT_MAX_CLASSES) return (int)hipErrorInvalidValue;
   if (x->ncls_c < 1 || x->ncls_c > x->max_cls || x->ncls_b < 1 || x->ncls_b > x->max_cls) return (int)hipErrorInvalidValue;
+  if (x->ncls_b > 4096) return (int)hipErrorInvalidValue;  // level-B classes are tile offsets (tile_shift <= 12)
   if (x->nchunk_c != x->nblk || x->nchunk_c > x->nchunk) return (int)hipErrorInvalidValue;
   if (x->tile_shift < 1 || x->tile_shift > 12) return (int)hipErrorInvalidValue;
   if (l->chunk_waves < 1 || l->chunk_waves > 4) return (int)hipErrorInvalidValue;
   const int rpt = block_rpt(x, l->block_threads);
   if (rpt > 16 || block_lds_bytes(rpt, l->block_threads, block_adds_carry(l)) > 160 * 1024) return (int)hipErrorInvalidValue;
-  if (replay_lds_bytes(x, l) > 160 * 1024 || hist_lds_bytes(x, l) > 160 * 1024) return (int)hipErrorInvalidValue;
+  if (replay_lds_bytes(x, l) > 160 * 1024 || (size_t)level_stride(x->max_cls) * sizeof(int64_t) > 160 * 1024)
+    return (int)hipErrorInvalidValue;
   return 0;
 }
 
@@ -740,7 +826,9 @@ int set_lds_limits() {
                         reinterpret_cast<const void*>(bwt_block_kernel<8>),
                         reinterpret_cast<const void*>(bwt_block_kernel<16>),
                         reinterpret_cast<const void*>(bwt_chunk_hist_kernel),
+                        reinterpret_cast<const void*>(bwt_row_c_kernel),
                         reinterpret_cast<const void*>(bwt_chunk_replay_kernel<9>),
+                        reinterpret_cast<const void*>(bwt_chunk_replay_kernel<10>),
                         reinterpret_cast<const void*>(bwt_chunk_replay_kernel<12>)};
     for (const void* k : ks) {
       const hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
@@ -786,17 +874,25 @@ extern "C" int bwt_chunk_hist_launch(const BwtIndex* x, const BwtLaunch* l, void
   int rc = check(x, l);
   if (rc) return rc;
   if ((rc = set_lds_limits())) return rc;
-  // level C chunks (one per block), then level B chunks (which also store their item weights),
-  // each with LDS sized to its own class count
+  // level C: one workgroup per block writes its row from the block's sorted tile slots
+  if (x->nchunk_c > 0) {
+    hipLaunchKernelGGL(bwt_row_c_kernel, dim3((unsigned)x->nchunk_c), dim3(kRowCThreads),
+                       (kRowCThreads / 64) * sizeof(int64_t) + (size_t)x->rmax * (sizeof(int64_t) + sizeof(uint32_t)),
+                       (hipStream_t)stream, *x, x->rmax);
+    if ((rc = report("level-C rows", hipGetLastError()))) return rc;
+  }
+  // level B chunks (which also store their item weights), LDS sized to the class count
   const int64_t rng[2][2] = {{0, x->nchunk_c}, {x->nchunk_c, x->nchunk}};
   const int ncls[2] = {x->ncls_c, x->ncls_b};
-  for (int lv = 0; lv < 2; ++lv) {
+  for (int lv = 1; lv < 2; ++lv) {
     const int64_t n = rng[lv][1] - rng[lv][0];
     if (n <= 0) continue;
     const int stride = level_stride(ncls[lv]);
-    const int64_t grid = (n + l->chunk_waves - 1) / l->chunk_waves;
-    hipLaunchKernelGGL(bwt_chunk_hist_kernel, dim3((unsigned)grid), dim3(64 * l->chunk_waves),
-                       (size_t)l->chunk_waves * stride * sizeof(int64_t), (hipStream_t)stream, *x, *l, rng[lv][0],
+    BwtLaunch lh = *l;
+    lh.chunk_waves = hist_waves(ncls[lv], l);
+    const int64_t grid = (n + lh.chunk_waves - 1) / lh.chunk_waves;
+    hipLaunchKernelGGL(bwt_chunk_hist_kernel, dim3((unsigned)grid), dim3(64 * lh.chunk_waves),
+                       (size_t)lh.chunk_waves * stride * sizeof(int64_t), (hipStream_t)stream, *x, lh, rng[lv][0],
                        rng[lv][1], stride, lv);
     if ((rc = report("chunk histogram", hipGetLastError()))) return rc;
   }
@@ -828,6 +924,9 @@ extern "C" int bwt_chunk_replay_launch(const BwtIndex* x, const BwtBatch* b, con
   // classes of a level-B chain: key offsets within a tile, < ncls_b <= 4096
   if (x->ncls_b <= 512)
     hipLaunchKernelGGL(bwt_chunk_replay_kernel<9>, dim3((unsigned)grid), dim3(64 * l->chunk_waves),
+                       replay_lds_bytes(x, l), (hipStream_t)stream, *x, *b, *l, stride);
+  else if (x->ncls_b <= 1024)
+    hipLaunchKernelGGL(bwt_chunk_replay_kernel<10>, dim3((unsigned)grid), dim3(64 * l->chunk_waves),
                        replay_lds_bytes(x, l), (hipStream_t)stream, *x, *b, *l, stride);
   else
     hipLaunchKernelGGL(bwt_chunk_replay_kernel<12>, dim3((unsigned)grid), dim3(64 * l->chunk_waves),

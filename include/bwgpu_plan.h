@@ -49,7 +49,7 @@ typedef struct BwtPlanIndex {
 /* Workspace bytes for a batch of M queries (M >= 1). */
 int bwt_plan_workspace(const BwtIndex* x, const BwtPlanIndex* p, int64_t M, size_t* bytes);
 
-/* Plan M queries (1 <= M < 2^31): qstate[M] in [1, S], b->qlo / b->qhi[M] in [0, NK].
+/* Plan M queries (1 <= M <= BWT_MAX_BATCH): qstate[M] in [1, S], b->qlo / b->qhi[M] in [0, NK].
  * Output arrays of b, with capacities: qblk min(nblk, M); qblk_g0 min(nblk, M)+1; grp_step M;
  * grp_q0 M+1; qorder, qrank, sblk, qtile, spos, srank, ipos M; wk_chunk 2M; wk_e0 2M+1;
  * eitems, epos 2M (the planner writes through these pointers).  Sets b->M, b->dcounts = dcounts and

@@ -41,15 +41,26 @@
 extern "C" {
 #endif
 
+/* Classes are 14-bit: a chain has at most BWT_MAX_CLASSES classes (level C: key tiles, so
+ * NK >> tile_shift < BWT_MAX_CLASSES - 1; level B: offsets within a tile). */
+#define BWT_CLASS_BITS 14
+#define BWT_CLASS_MASK ((1u << BWT_CLASS_BITS) - 1)
+#define BWT_MAX_CLASSES (1 << BWT_CLASS_BITS)
 /* Key item of a carry chain: class << 40 | weight index (into agg for level B, tagg for C). */
 #define BWT_CLASS_SHIFT 40
 #define BWT_REF_MASK ((1ull << 40) - 1)
-/* Endpoint item of a batch: local position << 44 | class << 32 | endpoint id (2 slot + side).
- * The position (19 bits) is the number of the chunk's key items that precede the endpoint. */
-#define BWT_EP_CLASS_SHIFT 32
+#define BWT_KEY_CLASS(item) ((uint32_t)((item) >> BWT_CLASS_SHIFT) & BWT_CLASS_MASK)
+/* Endpoint item of a batch: local position << 44 | class << 30 | endpoint id (2 slot + side).
+ * The position (19 bits) is the number of the chunk's key items that precede the endpoint; the
+ * id (30 bits) bounds a batch to BWT_MAX_BATCH queries. */
+#define BWT_EP_CLASS_SHIFT 30
+#define BWT_EP_ID_MASK ((1ull << 30) - 1)
 #define BWT_EP_POS_SHIFT 44
 #define BWT_EP_POS_MASK 0x7FFFFull
 #define BWT_EP_POS(item) (((item) >> BWT_EP_POS_SHIFT) & BWT_EP_POS_MASK)
+#define BWT_EP_CLASS(item) ((uint32_t)((item) >> BWT_EP_CLASS_SHIFT) & BWT_CLASS_MASK)
+#define BWT_EP_ID(item) ((uint32_t)((item) & BWT_EP_ID_MASK))
+#define BWT_MAX_BATCH ((1ll << 29) - 1)
 
 /* Static, update-side structure (resident across batches). */
 typedef struct BwtIndex {
@@ -57,7 +68,7 @@ typedef struct BwtIndex {
   int32_t U;
   int32_t nblk;
   int32_t rmax;      /* max distinct keys of one block (LDS Fenwick size of the block replay) */
-  int32_t max_cls;   /* max classes of any chain (<= 4096) */
+  int32_t max_cls;   /* max classes of any chain (<= BWT_MAX_CLASSES) */
   int32_t nchain;
   int32_t ncls_c;    /* classes of chain 0 (level C: key tiles) */
   int32_t ncls_b;    /* classes of every other chain (level B: keys within a tile) */

@@ -221,7 +221,7 @@ int main(int argc, char** argv) {
     std::vector<int64_t> h(n, 0);
     for (int64_t it = P.chunk_key0[c]; it < P.chunk_key0[c + 1]; ++it) {
       const uint64_t x = P.kitems[it];
-      const int cl = int((x >> 40) & 0xFFF);
+      const int cl = int(BWT_KEY_CLASS(x));
       CHECK(cl < n);
       const int64_t wt = src[x & BWT_REF_MASK];
       if (c >= P.chain_chunk0[1]) kweight[it] = wt;
@@ -284,7 +284,7 @@ int main(int argc, char** argv) {
       for (int l = 0; l < nk; ++l) {
         CHECK(ki + l < P.chunk_key0[c + 1]);
         const uint64_t kx = P.kitems[ki + l];
-        kcs[l] = int((kx >> 40) & 0xFFF);
+        kcs[l] = int(BWT_KEY_CLASS(kx));
         kws[l] = kweight[ki + l];
       }
       if (ne > 0) {
@@ -293,7 +293,7 @@ int main(int argc, char** argv) {
         corr_iters += mmax;
         int64_t vals[64];
         for (int l = 0; l < ne; ++l) {
-          const int cls = int((it[l] >> 32) & 0xFFF);
+          const int cls = int(BWT_EP_CLASS(it[l]));
           CHECK(cls < ncls);
           int64_t val = 0;
           int y = cls;
@@ -303,7 +303,7 @@ int main(int argc, char** argv) {
           vals[l] = val;
         }
         for (int l = 0; l < ne; ++l) {
-          const uint32_t id = uint32_t(it[l]);
+          const uint32_t id = BWT_EP_ID(it[l]);
           CHECK(id < 2 * nq);
           CHECK(B.epos[id] == uint64_t(ep + l));  // the endpoint's place in the work order
           g[ep + l] = vals[l];  // results in work order

@@ -28,6 +28,9 @@ CASES = [
     # items cut at odd sizes and at small slot ranges (range-ordered work)
     ("c1_cfg1", ["--tile-shift", "12", "--ep-per-work", "65", "--range-slots", "100"]),
     ("t2_cfg2_small", ["--tile-shift", "11", "--chunk", "64", "--ep-per-work", "67", "--range-slots", "333"]),
+    # 65,536 keys in tiles of 8 and 16: 8,193 and 4,097 key-tile classes (beyond 12 bits)
+    ("c1_cfg1", ["--tile-shift", "3"]),
+    ("c1_cfg1", ["--tile-shift", "4", "--chunk", "128"]),
 ]
 
 
@@ -63,6 +66,16 @@ def test_tree_gpu_full_size(pkg, case, mode, tmp_path):
         assert r["extract"]["same"], r["extract"]
         with open(path, "rb") as f:
             assert hashlib.sha256(f.read()).hexdigest() == g["oracle"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ts", ["9", "10"])
+def test_tree_gpu_cfg3_fine_tiles(pkg, ts):
+    """Config 3 at full size over 4,194,304 ranks in key tiles of 512 or 1,024 (8,193 or 4,097
+    key-tile classes), planned on the device: the oracle's complete-answer SHA."""
+    g = TREE["c3_cfg3"]
+    r = run_json(_cmd(pkg, g, ["--mode", "pass", "--device-plan", "--tile-shift", ts]), timeout=600)
+    assert r["answers_sha256"] == g["oracle"], ts
 
 
 @pytest.mark.gpu

@@ -40,6 +40,9 @@ def _run(emu, case, knobs, extra=()):
 @pytest.mark.parametrize("case,knobs", [
     ("c1_cfg1", []),
     ("c1_cfg1", ["--tile-shift", "12", "--ep-per-work", "65", "--range-slots", "100", "--stage-shift", "8"]),
+    # 65,536 keys in tiles of 8 and 16: 8,193 and 4,097 level-C classes, past a 12-bit class field
+    ("c1_cfg1", ["--tile-shift", "3"]),
+    ("c1_cfg1", ["--tile-shift", "4", "--chunk", "128"]),
     ("t1_cfg1_small", ["--rmax", "64", "--tile-shift", "6", "--chunk", "256", "--seg", "2", "--threads", "64"]),
     ("t2_cfg2_small", ["--tile-shift", "11", "--chunk", "64", "--ep-per-work", "67", "--range-slots", "333"]),
     ("t3_cfg3_small", ["--rmax", "48", "--tile-shift", "5", "--chunk", "200", "--seg", "1"]),
