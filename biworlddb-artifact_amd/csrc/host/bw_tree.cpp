@@ -91,14 +91,15 @@ int main(int argc, char** argv) {
   std::string answers_path, json_path, mode = "pass", extract_mode = "overlap";
   bool wsha = false, verify_remap = false, device_gen = false, dplan = false;
   int64_t copy_probe = 0;  // bytes of a device-to-device copy run first (HBM counter scale)
-  // --tuned FILE: the "plain" form's knobs from the autotuning file, as defaults the flags override
+  // --tuned FILE: the plain form's knobs from the autotuning file ("plain3" for config 3's 2^22
+  // ranks, "plain" otherwise), as defaults the flags override
   std::string tuned_from = "defaults";
   for (int i = 1; i + 1 < argc; ++i)
     if (!std::strcmp(argv[i], "--tuned")) {
       LaunchKnobs tl;
       tl.block_threads = threads;
       tl.chunk_waves = waves;
-      if (load_tuned(argv[i + 1], "plain", kn, tl)) {
+      if (load_tuned(argv[i + 1], cfg == 3 ? "plain3" : "plain", kn, tl)) {
         threads = tl.block_threads;
         waves = tl.chunk_waves;
         tuned_from = argv[i + 1];

@@ -98,7 +98,7 @@ bool save_tuned(const char* path, const char* form, const PlanKnobs& kn, const L
                 double ms) {
   const std::string text = read_file(path);
   // keep the other forms' entries
-  const char* forms[] = {"batched", "served", "plain"};
+  const char* forms[] = {"batched", "served", "plain", "plain3"};
   std::string out = "{\n";
   bool first = true;
   for (const char* f : forms) {
