@@ -283,12 +283,20 @@ def main():
         for rate in SERVED_RATES + (1e9,):
             s = run([BW_SERVE, "--check-alone", "16" if rate == SERVED_RATES[0] else "0", "--rate", str(rate)] + knobs,
                     600)
+            runs = None
+            if rate >= 1e9:
+                # offered at once the stream is two batches and ~10 ms of wall time: three runs, the
+                # median one reported, all three rates listed
+                tries = [s] + [run([BW_SERVE, "--check-alone", "0", "--rate", str(rate)] + knobs, 600) for _ in range(2)]
+                tries.sort(key=lambda t: t["queries_per_s"])
+                s = tries[1]
+                runs = [round(t["queries_per_s"], 1) for t in tries]
             curve.append({"offered_requests_per_s": rate if rate < 1e9 else "saturated",
                           "offered_queries_per_s": round(s["offered_queries_per_s"], 1),
                           "queries_per_s": round(s["queries_per_s"], 1), "latency_ms": s["latency_ms"],
                           "batches": s["batches"], "mean_batch_queries": s["mean_batch_queries"],
                           "batch_gpu_ms": s["batch_gpu_ms"], "answers_sha256": s["answers_sha256"],
-                          "alone_mismatch": s["alone"]["mismatch"]})
+                          "alone_mismatch": s["alone"]["mismatch"]} | ({"runs_queries_per_s": runs} if runs else {}))
         if len({c["answers_sha256"] for c in curve}) != 1:
             sys.exit("bench.py: served answers depend on the offered rate")
         out["served"] = curve[0] | {"requests": s["requests"], "appends": s["appends"], "planner": s["planner"]}
