@@ -570,6 +570,13 @@ int main(int argc, char** argv) {
       int64_t m1 = std::min<int64_t>(M, int64_t(1) << 21);
       while (m1 > 1 && (bwd_answer_noalloc_capacity(&D.x, m1, &need, &held) != 0 || need > held)) m1 = m1 * 3 / 4;
       const int64_t m2 = std::max<int64_t>(1, m1 / 3 + 7 < m1 ? m1 / 3 + 7 : m1 - 1);
+      // the largest batch the scratch holds over this index
+      int64_t cap_lo = 0, cap_hi = int64_t(1) << 24;
+      while (cap_lo < cap_hi) {
+        const int64_t mid = (cap_lo + cap_hi + 1) / 2;
+        size_t cn = 0, ch = 0;
+        if (bwd_answer_noalloc_capacity(&D.x, mid, &cn, &ch) == 0 && cn <= ch) cap_lo = mid; else cap_hi = mid - 1;
+      }
       DeviceArena na;
       int64_t* a_ref = na.scratch<int64_t>(size_t(m1));
       int64_t* a_na = na.scratch<int64_t>(size_t(m1));
@@ -603,10 +610,10 @@ int main(int argc, char** argv) {
       if (M > m1 && !refused) ++noalloc_mismatch;
       std::snprintf(noalloc_json, sizeof(noalloc_json),
                     "{\"m1\":%" PRId64 ",\"m2\":%" PRId64 ",\"scratch_bytes\":%zu,\"m1_needs_bytes\":%zu,"
-                    "\"refuses_oversize\":%s,\"diff\":[[%" PRId64 ",%" PRId64 ",%" PRId64 "],[%" PRId64 ",%" PRId64
-                    ",%" PRId64 "]],\"mismatch\":%" PRId64 "}",
-                    m1, m2, held, need, refused ? "true" : "false", diff[0][0], diff[0][1], diff[0][2], diff[1][0],
-                    diff[1][1], diff[1][2], noalloc_mismatch);
+                    "\"capacity_queries\":%" PRId64 ",\"refuses_oversize\":%s,\"diff\":[[%" PRId64 ",%" PRId64
+                    ",%" PRId64 "],[%" PRId64 ",%" PRId64 ",%" PRId64 "]],\"mismatch\":%" PRId64 "}",
+                    m1, m2, held, need, cap_lo, refused ? "true" : "false", diff[0][0], diff[0][1], diff[0][2],
+                    diff[1][0], diff[1][1], diff[1][2], noalloc_mismatch);
     }
   }
 
