@@ -17,11 +17,11 @@ Beside it, measured in the same run (every figure planning-inclusive unless name
   two_level -- the same stream over the two-level carry index (blocks of 1,024 keys, level-C rows,
                level-B chunk chains): device plan + one hipGraph replay of block replay, chunk
                replay and combine, per step; its replay alone as a sub-field;
-  plain     -- the plain GPU drop-in at paper scale (config 2: 300,000 states, 9.6M queries, one
-               launch sequence that plans the batch on the device, rebuilds the index and
+  plain     -- the generalised plain pass at paper scale (config 2: 300,000 states, 9.6M queries,
+               one launch sequence that plans the batch on the device, rebuilds the index and
                answers), the same for config 3 (4,194,304 distinct 32-bit keys through the
-               rank-LUT), and on the batched stream's first 64 batches the plain pass once per
-               batch (host plan, outside the timed region);
+               rank-LUT, on its own tuned knobs), and on the batched stream's first 64 batches the
+               plain pass once per batch (host plan, outside the timed region);
   dropin    -- the reference-ABI kernels (gpu_v7b.h / gpu_session.h arms of bw_panel) on the
                reference's own 300,000-state workload R1 (chain, factor period 37, q = 2 per
                state: 599,998 answers), kernel time and execute phase, SHA-256 = the reference
@@ -224,6 +224,8 @@ def main():
                       "plan_equals_host_plan": tl["mismatch"] == 0, "index_build_ms": r["phases_ms"]["index_build"],
                       "knobs": r["plan"]},
         "plain": {
+            "form": "the generalised plain pass (bwt_pass_launch: one launch sequence per batch that plans the "
+                    "batch on the device, rebuilds the index and answers); the reference-ABI kernels are `dropin`",
             "config2_queries_per_s": round(plain["queries_per_s"], 1),
             "config2_pass_ms": plain["pass_ms"]["avg"],
             "config2_stage_ms": plain["stage_ms"],
