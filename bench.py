@@ -122,6 +122,9 @@ def dropin_legs(tmp, threads):
         ph, gp = r["phases_ns"], r["gpu"]
         out[arm] = {"kernel_ms": round(gp["kernel_ns"] / 1e6, 4), "execute_ms": round(ph["execute"] / 1e6, 4),
                     "upload_ms": round(gp["upload_ns"] / 1e6, 4),
+                    # outside the execute phase, with the host plan: the device arena, the pinned
+                    # staging buffer and the copy engines' and host threads' start
+                    "setup_alloc_ms": round(gp["alloc_ns"] / 1e6, 4),
                     "kernel_queries_per_s": round(r["n_answers"] / (gp["kernel_ns"] * 1e-9), 1),
                     "execute_queries_per_s": round(r["n_answers"] / (ph["execute"] * 1e-9), 1),
                     "full_ms": round(r["full_ns"] / 1e6, 3), "blocks": 64, "answers_sha256": r["answers_sha256"]}
