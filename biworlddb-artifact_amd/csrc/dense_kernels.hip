@@ -683,7 +683,7 @@ extern "C" int bwd_replay_launch(const BwdIndex* x, int64_t M, int64_t* answers,
   if (!index_ok(x) || M < 1 || M > BWD_MAX_QUERIES || !answers || !l || !work) return (int)hipErrorInvalidValue;
   const int nt = l->threads;
   if (nt != 256 && nt != 512 && nt != 1024) return (int)hipErrorInvalidValue;
-  if (l->wg_per_cu < 1 || l->wg_per_cu > 8) return (int)hipErrorInvalidValue;
+  if (l->wg_per_cu < 1 || l->wg_per_cu > 8 || l->max_workgroups < 0) return (int)hipErrorInvalidValue;
   // ranks per thread: the smallest of 4, 8, 16 with rpt * nt >= rmax
   const int need = (x->rmax + nt - 1) / nt;
   const int rpt = need <= 4 ? 4 : need <= 8 ? 8 : 16;
@@ -702,7 +702,9 @@ extern "C" int bwd_replay_launch(const BwdIndex* x, int64_t M, int64_t* answers,
   const hipStream_t st = (hipStream_t)stream;
   uint32_t* counter = at<uint32_t>(work, L.counter);
   hipLaunchKernelGGL(bwd_init_kernel, dim3(1), dim3(64), 0, st, (int64_t*)nullptr, counter);
-  const dim3 grid((unsigned)(ncu * l->wg_per_cu));
+  int nwg = ncu * l->wg_per_cu;
+  if (l->max_workgroups > 0 && l->max_workgroups < nwg) nwg = l->max_workgroups;
+  const dim3 grid((unsigned)nwg);
   const uint64_t* kout = at<uint64_t>(work, L.kout);
   const uint32_t* vout = at<uint32_t>(work, L.vout);
   const uint32_t* grp = at<uint32_t>(work, L.grp);

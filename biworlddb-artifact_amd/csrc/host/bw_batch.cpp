@@ -106,7 +106,8 @@ int main(int argc, char** argv) {
   int warmup = 1, repeat = 3, plain_sample = 8;
   bool want_sha = false, stages = false, dev_plan = false, dev_gen = false, dev_index = false, dev_all = false;
   bool dense = false, noalloc_check = false;
-  int dense_rmax = 4096, dense_threads = 1024, dense_wg = 1;
+  int dense_rmax = 4096, dense_threads = 1024, dense_wg = 1, dense_maxwg = 0;
+  bool dense_overlap = false;
   int fork = 1;  // replay the block and level-B stages on two streams
   std::string answers_path, json_path, tuned_path;
   bool have_knob[8] = {false, false, false, false, false, false, false, false};
@@ -119,6 +120,7 @@ int main(int argc, char** argv) {
     if (!std::strcmp(argv[i], "--device-all")) { dev_all = true; continue; }
     if (!std::strcmp(argv[i], "--dense")) { dense = dev_plan = true; continue; }
     if (!std::strcmp(argv[i], "--noalloc-check")) { noalloc_check = dense = dev_plan = true; continue; }
+    if (!std::strcmp(argv[i], "--dense-overlap")) { dense_overlap = dense = dev_plan = true; continue; }
     if (i + 1 >= argc) { std::fprintf(stderr, "bw_batch: missing value for %s\n", argv[i]); return 2; }
     const char* k = argv[i];
     const char* v = argv[++i];
@@ -143,6 +145,7 @@ int main(int argc, char** argv) {
     else if (!std::strcmp(k, "--dense-rmax")) dense_rmax = std::atoi(v);
     else if (!std::strcmp(k, "--dense-threads")) dense_threads = std::atoi(v);
     else if (!std::strcmp(k, "--dense-wg")) dense_wg = std::atoi(v);
+    else if (!std::strcmp(k, "--dense-max-wg")) dense_maxwg = std::atoi(v);
     else if (!std::strcmp(k, "--seg")) kn.seg_chunks = std::atoi(v);
     else if (!std::strcmp(k, "--threads")) { lk.block_threads = std::atoi(v); have_knob[4] = true; }
     else if (!std::strcmp(k, "--waves")) { lk.chunk_waves = std::atoi(v); have_knob[5] = true; }
@@ -457,7 +460,7 @@ int main(int argc, char** argv) {
   // to dense_rmax keys, one dense carry row per block (built once), then per step the whole
   // stream planned (one radix sort) and answered (one persistent block-replay launch, captured
   // in a hipGraph); its answers must equal the two-level form's
-  char dense_json[1024] = "null", noalloc_json[512] = "null";
+  char dense_json[1536] = "null", noalloc_json[512] = "null";
   int64_t dense_mismatch = -1, noalloc_mismatch = -1;
   if (dense) {
     PlanKnobs dk = kn;
@@ -471,7 +474,7 @@ int main(int argc, char** argv) {
     }
     RUN(E4.build_index(st));
     DenseEngine D;
-    D.launch = BwdLaunch{dense_threads, dense_wg};
+    D.launch = BwdLaunch{dense_threads, dense_wg, dense_overlap ? 0 : dense_maxwg, 0};
     if (!D.init(E4, st, err, sizeof(err))) {
       std::fprintf(stderr, "bw_batch: %s\n", err);
       return 5;
@@ -544,6 +547,83 @@ int main(int argc, char** argv) {
       if (offset_mismatch) std::fprintf(stderr, "bw_batch: the shifted stream's answers differ (%" PRId64 ")\n", offset_mismatch);
       dense_mismatch += offset_mismatch;
     }
+    // --dense-overlap: consecutive steps pipelined.  Step i+1's plan runs on a second stream, into
+    // a second workspace, beside step i's replay, which is capped at --dense-max-wg workgroups so
+    // that compute units stay free for the plan; a plan waits until the replay two steps back has
+    // left its workspace, a replay waits for its plan.  The clock brackets all `repeat` steps,
+    // the first plan (not overlapped) included.
+    char overlap_json[384] = "null";
+    if (dense_overlap) {
+      DenseBatch DB2;
+      if (!alloc_dense_batch(D, M, DB2, err, sizeof(err))) {
+        std::fprintf(stderr, "bw_batch: %s\n", err);
+        return 5;
+      }
+      DenseBatch* wsp[2] = {&DBn, &DB2};
+      BwdLaunch lo = D.launch;
+      lo.max_workgroups = dense_maxwg;
+      hipStream_t sp;
+      HIPX(hipStreamCreate(&sp));
+      hipEvent_t ep[2], er[2], o0, o1;
+      for (int k = 0; k < 2; ++k) {
+        HIPX(hipEventCreateWithFlags(&ep[k], hipEventDisableTiming));
+        HIPX(hipEventCreateWithFlags(&er[k], hipEventDisableTiming));
+      }
+      HIPX(hipEventCreate(&o0));
+      HIPX(hipEventCreate(&o1));
+      auto plan_on = [&](int w) {
+        return bwd_plan_launch(&D.x, M, d_qstate, d_qlo, d_qhi, wsp[w]->flags, wsp[w]->work, wsp[w]->work_bytes, sp);
+      };
+      auto run_steps = [&](int n) {
+        RUN(plan_on(0));
+        HIPX(hipEventRecord(ep[0], sp));
+        for (int i = 0; i < n; ++i) {
+          const int w = i & 1;
+          HIPX(hipStreamWaitEvent(st, ep[w], 0));
+          RUN(bwd_replay_launch(&D.x, M, d_ans, &lo, wsp[w]->work, wsp[w]->work_bytes, st));
+          HIPX(hipEventRecord(er[w], st));
+          if (i + 1 < n) {
+            const int w2 = (i + 1) & 1;
+            if (i >= 1) HIPX(hipStreamWaitEvent(sp, er[w2], 0));  // replay i-1 has left workspace w2
+            RUN(plan_on(w2));
+            HIPX(hipEventRecord(ep[w2], sp));
+          }
+        }
+        return 0;
+      };
+      HIPX(hipDeviceSynchronize());
+      if (run_steps(std::max(2, warmup))) return 5;
+      HIPX(hipDeviceSynchronize());
+      HIPX(hipEventRecord(o0, st));
+      HIPX(hipStreamWaitEvent(sp, o0, 0));
+      if (run_steps(repeat)) return 5;
+      HIPX(hipEventRecord(o1, st));
+      HIPX(hipEventSynchronize(o1));
+      HIPX(hipDeviceSynchronize());
+      float tov = 0;
+      HIPX(hipEventElapsedTime(&tov, o0, o1));
+      int64_t fo = 0, f2 = 0;
+      HIPX(hipMemcpy(&fo, DBn.flags, sizeof(fo), hipMemcpyDeviceToHost));
+      HIPX(hipMemcpy(&f2, DB2.flags, sizeof(f2), hipMemcpyDeviceToHost));
+      BwExtractResult oxr{};
+      const int oxrc = bw_extract_answers(d_ans, size_t(M), st, nullptr, nullptr, BW_EXTRACT_FNV_ON_DEVICE, &oxr);
+      if (oxrc) return oxrc < 0 ? 4 : 5;
+      const int64_t omis = (sha == oxr.sha256_hex ? 0 : 1) + (fo != 0) + (f2 != 0);
+      if (omis) std::fprintf(stderr, "bw_batch: the overlapped steps' answers differ\n");
+      dense_mismatch += omis;
+      const double ostep = double(tov) / repeat;
+      std::snprintf(overlap_json, sizeof(overlap_json),
+                    "{\"max_workgroups\":%d,\"step_ms\":%.4f,\"queries_per_s\":%.1f,\"repeat\":%d,\"mismatch\":%" PRId64
+                    "}",
+                    dense_maxwg, ostep, double(M) / (ostep * 1e-3), repeat, omis);
+      for (int k = 0; k < 2; ++k) {
+        HIPX(hipEventDestroy(ep[k]));
+        HIPX(hipEventDestroy(er[k]));
+      }
+      HIPX(hipEventDestroy(o0));
+      HIPX(hipEventDestroy(o1));
+      HIPX(hipStreamDestroy(sp));
+    }
     auto mean = [](const std::vector<double>& v) {
       double a = 0;
       for (double x : v) a += x;
@@ -554,11 +634,11 @@ int main(int argc, char** argv) {
                   "{\"rmax\":%d,\"nblk\":%d,\"threads\":%d,\"wg_per_cu\":%d,\"carry_bytes\":%zu,\"carry_build_ms\":%.3f,"
                   "\"index_wall_ms\":%.3f,\"step_ms\":{\"avg\":%.4f,\"min\":%.4f,\"max\":%.4f,\"repeat\":%d},"
                   "\"plan_ms\":%.4f,\"replay_ms\":%.4f,\"queries_per_s\":%.1f,\"answers_sha256\":\"%s\","
-                  "\"offset_mismatch\":%" PRId64 ",\"mismatch\":%" PRId64 "}",
+                  "\"offset_mismatch\":%" PRId64 ",\"overlap\":%s,\"mismatch\":%" PRId64 "}",
                   dense_rmax, E4.x.nblk, dense_threads, dense_wg, D.carry_bytes, D.build_ms, dense_index_wall, sm,
                   *std::min_element(step_ms.begin(), step_ms.end()), *std::max_element(step_ms.begin(), step_ms.end()),
                   repeat, mean(dplan_v), mean(drep_v), double(M) / (sm * 1e-3), dxr.sha256_hex, offset_mismatch,
-                  dense_mismatch);
+                  overlap_json, dense_mismatch);
     HIPX(hipGraphExecDestroy(dgx));
     HIPX(hipGraphDestroy(dg));
     HIPX(hipEventDestroy(em));

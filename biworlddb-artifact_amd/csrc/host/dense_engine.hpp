@@ -15,7 +15,7 @@ namespace bwg {
 
 struct DenseEngine {
   BwdIndex x{};
-  BwdLaunch launch{1024, 1};
+  BwdLaunch launch{1024, 1, 0, 0};
   DeviceArena arena;   // the rank tables and the carry rows
   double build_ms = 0;  // device time of the carry build
   size_t carry_bytes = 0;

@@ -70,10 +70,14 @@ int bwd_carry_workspace(const BwdIndex* x, size_t* bytes);
 int bwd_carry_build_launch(const BwdIndex* x, const int64_t* agg, void* work, size_t work_bytes, void* stream);
 
 /* Launch shape of the persistent block replay: workgroup threads (256, 512 or 1024; each thread
- * owns rmax/threads in-block ranks, at most 16) and workgroups per compute unit. */
+ * owns rmax/threads in-block ranks, at most 16), workgroups per compute unit, and a cap on the
+ * workgroups (0: none) that leaves compute units free for work on another stream, such as the
+ * next batch's plan. */
 typedef struct BwdLaunch {
   int32_t threads;
   int32_t wg_per_cu;
+  int32_t max_workgroups;
+  int32_t pad0;
 } BwdLaunch;
 
 /* A planned batch of M queries: the queries sorted by (state rank, query id), with the group of
