@@ -25,9 +25,12 @@
 // sequence); the answers must be the same.  --device-all then runs the whole pipeline on the device
 // from the seed -- updates and queries generated, index built, stream planned and answered,
 // answers hashed -- timed phase by phase; its answers must be the same too.
-// --dense [--dense-rmax 4096 --dense-threads 1024 --dense-wg 1] also answers the stream with the
-// dense-carry form (include/bwgpu_dense.h): blocks of up to --dense-rmax keys, the persistent
-// replay at --dense-threads threads and --dense-wg workgroups per CU; its answers must be the same.
+// --dense [--dense-rmax 4096 --dense-threads 1024 --dense-wg 1 --dense-max-wg 0] also answers the
+// stream with the dense-carry form (include/bwgpu_dense.h): blocks of up to --dense-rmax keys, the
+// persistent replay at --dense-threads threads, --dense-wg workgroups per CU and at most
+// --dense-max-wg workgroups in all (0: no cap); its answers must be the same.  --dense-overlap
+// also times the steps pipelined (the next plan on a second stream beside a replay capped at
+// --dense-max-wg workgroups).
 // --noalloc-check also compares its allocation-free entry point with the workspace entry point.
 // Exit codes: 2 usage, 3 unsupported input, 4 I/O, 5 GPU error, 6 answers differ.
 #include <hip/hip_runtime_api.h>
