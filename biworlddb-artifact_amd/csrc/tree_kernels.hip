@@ -764,6 +764,51 @@ __global__ void __launch_bounds__(256) bwt_rank_kernel(const BwtRankArgs a) {
   a.out[i] = lo;
 }
 
+// The same remap with the LUT's row offsets staged in LDS: each workgroup stages the 65,537 row
+// offsets as a 32-bit base per 64 rows plus a 16-bit offset per row (132 KiB), then takes
+// endpoints a grid apart, so a lookup reads its row's bounds from LDS and only its row's keys
+// (one or two 128-byte lines) from global memory.  A workgroup whose rows do not fit 16-bit
+// offsets (more than 65,535 keys under 64 consecutive rows) reads the bounds from global memory.
+constexpr int kRankLdsThreads = 1024;
+constexpr int64_t kRankLdsMin = int64_t(1) << 20;  // endpoints from which the staged form is used
+constexpr int kRankRows = 65536, kRankGroup = 64;
+constexpr size_t kRankLdsBytes = 16 + (kRankRows / kRankGroup + 1) * 4 + (kRankRows + 1) * 2;
+__global__ void __launch_bounds__(kRankLdsThreads) bwt_rank_lds_kernel(const BwtRankArgs a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  int* overflow = reinterpret_cast<int*>(smem_raw);
+  uint32_t* base = reinterpret_cast<uint32_t*>(smem_raw + 16);
+  uint16_t* off16 = reinterpret_cast<uint16_t*>(base + kRankRows / kRankGroup + 1);
+  const int tid = threadIdx.x;
+  if (tid == 0) overflow[0] = 0;
+  for (int g = tid; g <= kRankRows / kRankGroup; g += kRankLdsThreads) base[g] = a.row_ptr[g * kRankGroup];
+  __syncthreads();
+  for (int h = tid; h <= kRankRows; h += kRankLdsThreads) {
+    const uint32_t d = a.row_ptr[h] - base[h / kRankGroup];
+    if (d > 0xFFFFu) atomicOr(overflow, 1);
+    off16[h] = (uint16_t)d;
+  }
+  __syncthreads();
+  const bool staged = overflow[0] == 0;  // uniform
+  for (int64_t i = (int64_t)blockIdx.x * kRankLdsThreads + tid; i < a.n; i += (int64_t)gridDim.x * kRankLdsThreads) {
+    const uint32_t v = a.in[i];
+    const uint32_t h = v >> 16;
+    const uint16_t lo16 = (uint16_t)(v & 0xFFFFu);
+    uint32_t lo, hi;
+    if (staged) {
+      lo = base[h / kRankGroup] + off16[h];
+      hi = base[(h + 1) / kRankGroup] + off16[h + 1];
+    } else {
+      lo = a.row_ptr[h];
+      hi = a.row_ptr[h + 1];
+    }
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (a.low16[mid] < lo16) lo = mid + 1; else hi = mid;
+    }
+    a.out[i] = lo;
+  }
+}
+
 // ---------------------------------------------------------------- launch helpers
 // ranks per thread of the block replay: the smallest power of two with nt * RPT >= rmax
 int block_rpt(const BwtIndex* x, int nt) {
@@ -827,6 +872,7 @@ int set_lds_limits() {
                         reinterpret_cast<const void*>(bwt_block_kernel<16>),
                         reinterpret_cast<const void*>(bwt_chunk_hist_kernel),
                         reinterpret_cast<const void*>(bwt_row_c_kernel),
+                        reinterpret_cast<const void*>(bwt_rank_lds_kernel),
                         reinterpret_cast<const void*>(bwt_chunk_replay_kernel<9>),
                         reinterpret_cast<const void*>(bwt_chunk_replay_kernel<10>),
                         reinterpret_cast<const void*>(bwt_chunk_replay_kernel<12>)};
@@ -988,6 +1034,20 @@ extern "C" int bwt_overlay_launch(const BwtOverlay* o, void* stream) {
 extern "C" int bwt_rank_remap_launch(const BwtRankArgs* a, void* stream) {
   if (!a || a->n < 0) return (int)hipErrorInvalidValue;
   if (a->n == 0) return 0;
-  hipLaunchKernelGGL(bwt_rank_kernel, dim3((unsigned)((a->n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, *a);
+  // large remaps stage the row offsets in LDS (one workgroup per CU, each staging 256 KiB of
+  // offsets once); small ones (a served batch's endpoints) read them from global memory
+  if (a->n >= kRankLdsMin) {
+    int rc = set_lds_limits();
+    if (rc) return rc;
+    int dev = 0, ncu = 0;
+    hipError_t e;
+    if ((e = hipGetDevice(&dev)) || (e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev)))
+      return report("device query", e);
+    const int64_t want = (a->n + kRankLdsThreads - 1) / kRankLdsThreads;
+    const unsigned grid = (unsigned)std::min<int64_t>(ncu, want);
+    hipLaunchKernelGGL(bwt_rank_lds_kernel, dim3(grid), dim3(kRankLdsThreads), kRankLdsBytes, (hipStream_t)stream, *a);
+  } else {
+    hipLaunchKernelGGL(bwt_rank_kernel, dim3((unsigned)((a->n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, *a);
+  }
   return report("rank remap", hipGetLastError());
 }
