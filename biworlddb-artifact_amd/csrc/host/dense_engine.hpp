@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "bwgpu_dense.h"
+#include "dense_tables.hpp"
 #include "tree_engine.hpp"
 
 namespace bwg {
@@ -24,14 +25,6 @@ struct DenseEngine {
   // weight per distinct key).  The engine must outlive this one (its arrays are shared).
   bool init(const TreeEngine& e, hipStream_t st, char* err, size_t errlen);
 };
-
-// Host restatement of the rank tables (also used by the tests' emulation): for the walk `steps`
-// cut at blk_step0, state_erank[v] = index of v's enter step among the enter steps, erank_step[e]
-// = the step of the e-th enter, blk_e0[b] = enter steps before block b.  False if some state is
-// not entered exactly once.
-bool dense_rank_tables(int64_t S, const std::vector<int32_t>& steps, const std::vector<int32_t>& blk_step0,
-                       std::vector<int32_t>& state_erank, std::vector<int32_t>& erank_step,
-                       std::vector<int32_t>& blk_e0);
 
 // A batch's workspace (plan + replay) for up to cap queries.
 struct DenseBatch {

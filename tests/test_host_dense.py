@@ -1,0 +1,44 @@
+"""CPU parity of the dense-carry form's algorithm: tests/host/dense_emulator.cpp (test
+infrastructure) follows the dense kernels step by step over the product's host index plan and rank
+tables -- carry rows from the blocks' net weights, the plan's packed 64-bit keys and 32-bit values
+(the low endpoint split across them), the sort, the per-rank groups, the block replay -- and must
+give the stream oracle's golden answer hashes at several block sizes, 2^20-key trees included
+(endpoints past 2^17 take the split path)."""
+import json
+import os
+import subprocess
+
+import pytest
+
+from conftest import golden
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+STREAMS = golden("stream_cases.json")
+
+
+@pytest.fixture(scope="module")
+def emulator(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("demu") / "dense_emulator")
+    host = os.path.join(REPO, "biworlddb-artifact_amd", "csrc", "host")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-fopenmp", "-I" + os.path.join(REPO, "include"), "-I" + host,
+                    "-o", out, os.path.join(REPO, "tests", "host", "dense_emulator.cpp"),
+                    os.path.join(host, "tree_plan.cpp"), os.path.join(host, "tree_workload.cpp"),
+                    os.path.join(host, "dense_tables.cpp"), "-lcrypto"],
+                   check=True, capture_output=True, timeout=300)
+    return out
+
+
+@pytest.mark.parametrize("case,rmax", [("s1_batched_small", 4096), ("s1_batched_small", 64),
+                                       ("s1_batched_small", 8), ("s2_batched_mid", 4096),
+                                       ("s2_batched_mid", 1024)])
+def test_dense_emulation_matches_oracle(emulator, case, rmax):
+    g = STREAMS[case]
+    S, U, N, seed = g["tree"]
+    cmd = [emulator, "--S", str(S), "--U", str(U), "--N", str(N), "--rmax", str(rmax)] + g["args"]
+    if seed:
+        cmd += ["--seed", str(seed)]
+    p = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+    assert p.returncode == 0, p.stderr[-2000:]
+    r = json.loads(p.stdout.strip().splitlines()[-1])
+    assert r["answers_sha256"] == g["oracle"], (case, rmax, r)
+    assert r["n_answers"] == g["nq"]
