@@ -3,7 +3,7 @@
 //
 //   bw_serve [--cfg 2|3] [--S n --U n --N n --seed n] [--requests 20000 --max-batch 4096 --append-frac 0.05
 //            --append-U 128 --rate 20000] [--tuned FILE] [--slots 2] [--budget-mb 8192] [--slot-queries 16777216]
-//            [--check-alone 64] [--answers FILE] [--json FILE] [--timeline]
+//            [--check-alone 64] [--answers FILE] [--json FILE] [--timeline] [--tile-shift T]
 //
 // Resident in HBM: the version tree's index (TreeEngine), the appended states' updates, and per
 // in-flight batch one fixed buffer ("slot") whose size is the HBM budget divided among the slots
@@ -22,7 +22,8 @@
 // Latency = time the request's answers are back on the host - its arrival time.
 // Before the clock starts, one small batch per slot loads the kernels, and copies in both
 // directions on every stream warm up the copy engines (their first use costs milliseconds).
-// --timeline prints each batch's host launch steps and completion times on stderr.
+// --timeline prints each batch's host launch steps and completion times on stderr.  --tile-shift
+// overrides the tuned key-tile width (0: the balanced rule).
 // --check-alone K: afterwards K requests (evenly spread) are answered again, each alone in its
 // own batch, and must equal their answers from the mix element for element.
 // Exit codes: 2 usage, 3 unsupported input, 4 I/O, 5 GPU error, 6 alone/mix mismatch.
@@ -146,6 +147,7 @@ int main(int argc, char** argv) {
   LaunchKnobs lk;
   int nslots = 2, check_alone = 64;
   int64_t slot_queries = int64_t(1) << 24;  // most queries one batch takes
+  int tile_shift_arg = -1;  // --tile-shift: key-tile width override (0: the balanced rule)
   bool timeline = false;  // --timeline: per-batch host times on stderr  // most queries one batch takes (batches pipeline across slots)
   double budget_mb = 8192;
   std::string answers_path, json_path, tuned_path;
@@ -169,6 +171,7 @@ int main(int argc, char** argv) {
     else if (!std::strcmp(k, "--budget-mb")) budget_mb = std::atof(v);
     else if (!std::strcmp(k, "--slot-queries")) slot_queries = std::atoll(v);
     else if (!std::strcmp(k, "--check-alone")) check_alone = std::atoi(v);
+    else if (!std::strcmp(k, "--tile-shift")) tile_shift_arg = std::atoi(v);
     else if (!std::strcmp(k, "--answers")) answers_path = v;
     else if (!std::strcmp(k, "--json")) json_path = v;
     else { std::fprintf(stderr, "bw_serve: unknown argument %s\n", k); return 2; }
@@ -180,9 +183,14 @@ int main(int argc, char** argv) {
   }
   std::string tuned_from = "defaults";
   if (!tuned_path.empty() && load_tuned(tuned_path.c_str(), "served", kn, lk)) tuned_from = tuned_path;
-  // the served knobs were tuned on config 2's 2^20 keys; over config 3's ranks the key-tile width
-  // is derived from the rank space instead
-  if (ranked) kn.tile_shift = 0;
+  // the served knobs were tuned on config 2's 2^20 keys; over config 3's 2^22 ranks their key
+  // tiles (2^9 keys: 8,193 tiles) still fit the 14-bit class field and measured best (p99 0.93 ms
+  // against 1.27 ms with the balanced rule's 2^11 tiles); a rank space the field cannot hold in
+  // tiles that narrow falls back to the balanced rule
+  if (ranked && kn.tile_shift > 0 &&
+      (uint64_t(spec.N) >> kn.tile_shift) + 1 > uint64_t(BWT_MAX_CLASSES - 1))
+    kn.tile_shift = 0;
+  if (tile_shift_arg >= 0) kn.tile_shift = tile_shift_arg;
 
   // ---- the tree, the stream, the resident index
   const auto g0 = clk::now();
