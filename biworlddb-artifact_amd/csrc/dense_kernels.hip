@@ -304,9 +304,11 @@ __host__ __device__ inline BlockLds block_lds_layout(int span, int nt, int rmax)
   return L;
 }
 
-template <int RPT>
+// kSorted: each answer goes to its query's sorted position (out[M], runs of consecutive positions
+// per state; the unsort kernels below put them in query-id order); otherwise to out[query id].
+template <int RPT, bool kSorted>
 __device__ void dense_block(const BwdIndex& x, const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals,
-                            const uint32_t* __restrict__ grp, int64_t* __restrict__ answers, const int b,
+                            const uint32_t* __restrict__ grp, int64_t* __restrict__ out, const int b,
                             unsigned char* smem) {
   const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wv = tid >> 6;
   const int span = span_of<RPT>(nt);
@@ -393,12 +395,12 @@ __device__ void dense_block(const BwdIndex& x, const uint64_t* __restrict__ keys
     rebuild();
     if (live) {
       const int rl = lower_bound_lds<uint32_t>(skey, R, lo), rh = lower_bound_lds<uint32_t>(skey, R, hi);
-      answers[qid] = (chi - clo) + (P[pslot<RPT>(rh)] - P[pslot<RPT>(rl)]);
+      out[kSorted ? qa + tid : qid] = (chi - clo) + (P[pslot<RPT>(rh)] - P[pslot<RPT>(rl)]);
     }
     for (int j = tid + nt; j < nq; j += nt) {  // groups larger than the workgroup
       unpack(keys[qa + j], vals[qa + j], qid, lo, hi);
       const int rl = lower_bound_lds<uint32_t>(skey, R, lo), rh = lower_bound_lds<uint32_t>(skey, R, hi);
-      answers[qid] = (crow[hi] - crow[lo]) + (P[pslot<RPT>(rh)] - P[pslot<RPT>(rl)]);
+      out[kSorted ? qa + j : qid] = (crow[hi] - crow[lo]) + (P[pslot<RPT>(rh)] - P[pslot<RPT>(rl)]);
     }
   };
 
@@ -479,9 +481,9 @@ __device__ void dense_block(const BwdIndex& x, const uint64_t* __restrict__ keys
   }
 }
 
-template <int RPT>
+template <int RPT, bool kSorted>
 __global__ void __launch_bounds__(1024) bwd_block_kernel(const BwdIndex x, const uint64_t* keys, const uint32_t* vals,
-                                                         const uint32_t* grp, int64_t* answers, uint32_t* counter) {
+                                                         const uint32_t* grp, int64_t* out, uint32_t* counter) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ int next;
   for (;;) {
@@ -490,9 +492,111 @@ __global__ void __launch_bounds__(1024) bwd_block_kernel(const BwdIndex x, const
     const int b = __builtin_amdgcn_readfirstlane(next);
     __syncthreads();  // every thread has read `next` before it is taken again
     if (b >= x.nblk) break;
-    dense_block<RPT>(x, keys, vals, grp, answers, b, smem_raw);
+    dense_block<RPT, kSorted>(x, keys, vals, grp, out, b, smem_raw);
     __syncthreads();  // the next block reuses the LDS
   }
+}
+
+// ---------------------------------------------------------------- unsort: answers to query-id order
+// A random 8-byte store costs a 32-byte write request and a partial line in some die's cache, so
+// the replay (kSorted) stores its answers in sorted order, and two streaming passes put them in
+// query-id order through windows of 2^kWinBits ids (2 MiB of answers each):
+//   partition -- a tile of sorted positions is grouped by window in LDS, and each window's run is
+//                appended to that window's region of a staging buffer (region w = staging positions
+//                [w << kWinBits, ...): every window but the last holds exactly 2^kWinBits ids, so the
+//                staging buffer is dense over [0, M)); a per-window cursor hands out the runs, so
+//                the order inside a region is free (the next pass stores by id);
+//   scatter   -- the staging buffer is cut into kDies contiguous parts, part d taken by the
+//                workgroups that run on die d (workgroup g runs on die g mod kDies), so each
+//                window's stores come from one die and its lines fill in that die's L2 before they
+//                are written back.
+constexpr int kWinBits = 18;
+constexpr int kMaxWin = (int)(BWD_MAX_QUERIES >> kWinBits);  // 512 windows cover every query id
+constexpr int kPartT = 1024, kPartE = 8, kPartTile = kPartT * kPartE;
+constexpr int kScatT = 256, kScatE = 8, kScatTile = kScatT * kScatE;
+constexpr int kDies = 8;
+static_assert(kMaxWin <= kPartT && kMaxWin % 64 == 0, "one thread per window in the partition's scan");
+
+__global__ void __launch_bounds__(512) bwd_replay_init_kernel(uint32_t* counter, uint32_t* cursor, int nw) {
+  if (threadIdx.x == 0) counter[0] = 0;
+  if (cursor && (int)threadIdx.x < nw) cursor[threadIdx.x] = 0;
+}
+
+__global__ void __launch_bounds__(kPartT) bwd_part_kernel(int64_t M, const uint64_t* __restrict__ keys,
+                                                          const int64_t* __restrict__ sans, uint32_t* __restrict__ cursor,
+                                                          int64_t* __restrict__ stage_ans, uint32_t* __restrict__ stage_id) {
+  __shared__ uint32_t cnt[kMaxWin], lstart[kMaxWin], gbase[kMaxWin];
+  __shared__ uint32_t wtot[kMaxWin / 64];
+  __shared__ int64_t sv[kPartTile];
+  __shared__ uint32_t si[kPartTile];
+  const int tid = threadIdx.x;
+  const int64_t t0 = (int64_t)blockIdx.x * kPartTile;
+  const int n = (int)min<int64_t>(kPartTile, M - t0);
+  if (tid < kMaxWin) cnt[tid] = 0;
+  // every load is issued before any is used (clamped to the tile's last position)
+  uint32_t id[kPartE];
+  int64_t a[kPartE];
+#pragma unroll
+  for (int k = 0; k < kPartE; ++k) {
+    const int64_t p = t0 + min(k * kPartT + tid, n - 1);
+    id[k] = (uint32_t)((__builtin_nontemporal_load(keys + p) >> kIdShift) & kIdMask);
+    a[k] = __builtin_nontemporal_load(sans + p);
+  }
+  __syncthreads();  // counts are zero
+  uint32_t off[kPartE];
+#pragma unroll
+  for (int k = 0; k < kPartE; ++k) off[k] = k * kPartT + tid < n ? atomicAdd(&cnt[id[k] >> kWinBits], 1u) : 0u;
+  __syncthreads();
+  // thread w: window w's start in the tile (exclusive scan of the counts) and its run's place in
+  // the staging region
+  if (tid < kMaxWin) {
+    const uint32_t c = cnt[tid];
+    const uint32_t inc = (uint32_t)wave_incl_scan((int64_t)c);
+    if ((tid & 63) == 63) wtot[tid >> 6] = inc;
+    lstart[tid] = inc - c;
+    if (c) gbase[tid] = atomicAdd(cursor + tid, c);
+  }
+  __syncthreads();
+  if (tid < kMaxWin) {
+    uint32_t o = 0;
+    for (int w = 0; w < (tid >> 6); ++w) o += wtot[w];
+    lstart[tid] += o;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kPartE; ++k) {
+    if (k * kPartT + tid < n) {
+      const int j = (int)(lstart[id[k] >> kWinBits] + off[k]);
+      sv[j] = a[k];
+      si[j] = id[k];
+    }
+  }
+  __syncthreads();
+  // consecutive tile places of one window go to consecutive staging positions
+  for (int j = tid; j < n; j += kPartT) {
+    const uint32_t q = si[j];
+    const int w = (int)(q >> kWinBits);
+    const int64_t g = ((int64_t)w << kWinBits) + gbase[w] + (uint32_t)(j - (int)lstart[w]);
+    stage_ans[g] = sv[j];
+    stage_id[g] = q;
+  }
+}
+
+__global__ void __launch_bounds__(kScatT) bwd_scatter_kernel(int64_t M, int64_t per, const int64_t* __restrict__ stage_ans,
+                                                             const uint32_t* __restrict__ stage_id,
+                                                             int64_t* __restrict__ answers) {
+  const int64_t i0 = ((int64_t)(blockIdx.x % kDies) * per + blockIdx.x / kDies) * kScatTile + threadIdx.x;
+  uint32_t q[kScatE];
+  int64_t v[kScatE];
+#pragma unroll
+  for (int k = 0; k < kScatE; ++k) {
+    const int64_t i = min(i0 + k * kScatT, M - 1);
+    q[k] = __builtin_nontemporal_load(stage_id + i);
+    v[k] = __builtin_nontemporal_load(stage_ans + i);
+  }
+#pragma unroll
+  for (int k = 0; k < kScatE; ++k)
+    if (i0 + k * kScatT < M) answers[q[k]] = v[k];
 }
 
 // ---------------------------------------------------------------- host helpers
@@ -552,8 +656,10 @@ hipError_t plan_sort(void* temp, size_t& temp_bytes, const uint64_t* kin, uint64
                                                    (unsigned)kRankShift, end_bit, st);
 }
 
+// kin / vin (the sort's input) are free once the plan has run: the unsort's staging buffer reuses
+// them (stage_ans = kin, stage_id = vin); sans holds the replay's answers in sorted order.
 struct AnswerLayout {
-  size_t kin, kout, vin, vout, grp, counter, temp, temp_bytes, total;
+  size_t kin, kout, vin, vout, grp, counter, cursor, sans, temp, temp_bytes, total;
   unsigned end_bit;
 };
 
@@ -574,6 +680,8 @@ hipError_t answer_layout(const BwdIndex* x, int64_t M, AnswerLayout& L) {
   L.vout = take((size_t)M * 4);
   L.grp = take((size_t)(x->S + 1) * 4);
   L.counter = take(4);
+  L.cursor = take((size_t)kMaxWin * 4);
+  L.sans = take((size_t)M * 8);
   L.temp = take(std::max<size_t>(L.temp_bytes, 1));
   L.total = o;
   return hipSuccess;
@@ -588,9 +696,12 @@ int set_lds_limits() {
   static std::once_flag once;
   static hipError_t err = hipSuccess;
   std::call_once(once, [] {
-    const void* ks[] = {reinterpret_cast<const void*>(bwd_block_kernel<4>),
-                        reinterpret_cast<const void*>(bwd_block_kernel<8>),
-                        reinterpret_cast<const void*>(bwd_block_kernel<16>)};
+    const void* ks[] = {reinterpret_cast<const void*>(bwd_block_kernel<4, false>),
+                        reinterpret_cast<const void*>(bwd_block_kernel<8, false>),
+                        reinterpret_cast<const void*>(bwd_block_kernel<16, false>),
+                        reinterpret_cast<const void*>(bwd_block_kernel<4, true>),
+                        reinterpret_cast<const void*>(bwd_block_kernel<8, true>),
+                        reinterpret_cast<const void*>(bwd_block_kernel<16, true>)};
     for (const void* k : ks) {
       const hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, kBlockLdsLimit);
       if (e != hipSuccess && err == hipSuccess) err = e;
@@ -683,7 +794,9 @@ extern "C" int bwd_replay_launch(const BwdIndex* x, int64_t M, int64_t* answers,
   if (!index_ok(x) || M < 1 || M > BWD_MAX_QUERIES || !answers || !l || !work) return (int)hipErrorInvalidValue;
   const int nt = l->threads;
   if (nt != 256 && nt != 512 && nt != 1024) return (int)hipErrorInvalidValue;
-  if (l->wg_per_cu < 1 || l->wg_per_cu > 8 || l->max_workgroups < 0) return (int)hipErrorInvalidValue;
+  if (l->wg_per_cu < 1 || l->wg_per_cu > 8 || l->max_workgroups < 0 || l->store < BWD_STORE_AUTO ||
+      l->store > BWD_STORE_STAGED)
+    return (int)hipErrorInvalidValue;
   // ranks per thread: the smallest of 4, 8, 16 with rpt * nt >= rmax
   const int need = (x->rmax + nt - 1) / nt;
   const int rpt = need <= 4 ? 4 : need <= 8 ? 8 : 16;
@@ -700,29 +813,39 @@ extern "C" int bwd_replay_launch(const BwdIndex* x, int64_t M, int64_t* answers,
   if ((e = hipGetDevice(&dev)) || (e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev)))
     return report("device query", e);
   const hipStream_t st = (hipStream_t)stream;
+  // staged stores pay off once the ids span more than one window
+  const bool staged = l->store == BWD_STORE_STAGED || (l->store == BWD_STORE_AUTO && M > (int64_t(1) << kWinBits));
+  const int nwin = (int)((M + (int64_t(1) << kWinBits) - 1) >> kWinBits);
   uint32_t* counter = at<uint32_t>(work, L.counter);
-  hipLaunchKernelGGL(bwd_init_kernel, dim3(1), dim3(64), 0, st, (int64_t*)nullptr, counter);
+  uint32_t* cursor = at<uint32_t>(work, L.cursor);
+  hipLaunchKernelGGL(bwd_replay_init_kernel, dim3(1), dim3(512), 0, st, counter, staged ? cursor : nullptr, nwin);
   int nwg = ncu * l->wg_per_cu;
   if (l->max_workgroups > 0 && l->max_workgroups < nwg) nwg = l->max_workgroups;
   const dim3 grid((unsigned)nwg);
   const uint64_t* kout = at<uint64_t>(work, L.kout);
   const uint32_t* vout = at<uint32_t>(work, L.vout);
   const uint32_t* grp = at<uint32_t>(work, L.grp);
-  switch (rpt) {
-    case 4:
-      hipLaunchKernelGGL(bwd_block_kernel<4>, grid, dim3(nt), block_lds_layout(span_of<4>(nt), nt, x->rmax).total, st, *x, kout, vout, grp,
-                         answers, counter);
-      break;
-    case 8:
-      hipLaunchKernelGGL(bwd_block_kernel<8>, grid, dim3(nt), block_lds_layout(span_of<8>(nt), nt, x->rmax).total, st, *x, kout, vout, grp,
-                         answers, counter);
-      break;
-    default:
-      hipLaunchKernelGGL(bwd_block_kernel<16>, grid, dim3(nt), block_lds_layout(span_of<16>(nt), nt, x->rmax).total, st, *x, kout, vout, grp,
-                         answers, counter);
-      break;
+  int64_t* out = staged ? at<int64_t>(work, L.sans) : answers;
+  const size_t lds = block_lds_layout(spn, nt, x->rmax).total;
+  auto launch = [&](auto kernel) { hipLaunchKernelGGL(kernel, grid, dim3(nt), lds, st, *x, kout, vout, grp, out, counter); };
+  if (staged) {
+    if (rpt == 4) launch(bwd_block_kernel<4, true>);
+    else if (rpt == 8) launch(bwd_block_kernel<8, true>);
+    else launch(bwd_block_kernel<16, true>);
+  } else {
+    if (rpt == 4) launch(bwd_block_kernel<4, false>);
+    else if (rpt == 8) launch(bwd_block_kernel<8, false>);
+    else launch(bwd_block_kernel<16, false>);
   }
-  return report("dense block replay", hipGetLastError());
+  if ((rc = report("dense block replay", hipGetLastError()))) return rc;
+  if (!staged) return 0;
+  hipLaunchKernelGGL(bwd_part_kernel, dim3((unsigned)((M + kPartTile - 1) / kPartTile)), dim3(kPartT), 0, st, M, kout,
+                     at<const int64_t>(work, L.sans), cursor, at<int64_t>(work, L.kin), at<uint32_t>(work, L.vin));
+  if ((rc = report("answer partition", hipGetLastError()))) return rc;
+  const int64_t per = ((M + kScatTile - 1) / kScatTile + kDies - 1) / kDies;
+  hipLaunchKernelGGL(bwd_scatter_kernel, dim3((unsigned)(per * kDies)), dim3(kScatT), 0, st, M, per,
+                     at<const int64_t>(work, L.kin), at<const uint32_t>(work, L.vin), answers);
+  return report("answer scatter", hipGetLastError());
 }
 
 // the current device's address of `scratch` (looked up on the first call on each device)

@@ -28,7 +28,9 @@
 // --dense [--dense-rmax 4096 --dense-threads 1024 --dense-wg 1 --dense-max-wg 0] also answers the
 // stream with the dense-carry form (include/bwgpu_dense.h): blocks of up to --dense-rmax keys, the
 // persistent replay at --dense-threads threads, --dense-wg workgroups per CU and at most
-// --dense-max-wg workgroups in all (0: no cap); its answers must be the same.  --dense-overlap
+// --dense-max-wg workgroups in all (0: no cap), answers stored by --dense-store (0 auto, 1 direct
+// random stores, 2 staged: sorted order, then partition and die-local window scatter; the
+// BWD_STORE_* modes); its answers must be the same.  --dense-overlap
 // also times the steps pipelined (the next plan on a second stream beside a replay capped at
 // --dense-max-wg workgroups).
 // --noalloc-check also compares its allocation-free entry point with the workspace entry point.
@@ -109,7 +111,7 @@ int main(int argc, char** argv) {
   int warmup = 1, repeat = 3, plain_sample = 8;
   bool want_sha = false, stages = false, dev_plan = false, dev_gen = false, dev_index = false, dev_all = false;
   bool dense = false, noalloc_check = false;
-  int dense_rmax = 4096, dense_threads = 1024, dense_wg = 1, dense_maxwg = 0;
+  int dense_rmax = 4096, dense_threads = 1024, dense_wg = 1, dense_maxwg = 0, dense_store = BWD_STORE_AUTO;
   bool dense_overlap = false;
   int fork = 1;  // replay the block and level-B stages on two streams
   std::string answers_path, json_path, tuned_path;
@@ -149,6 +151,7 @@ int main(int argc, char** argv) {
     else if (!std::strcmp(k, "--dense-threads")) dense_threads = std::atoi(v);
     else if (!std::strcmp(k, "--dense-wg")) dense_wg = std::atoi(v);
     else if (!std::strcmp(k, "--dense-max-wg")) dense_maxwg = std::atoi(v);
+    else if (!std::strcmp(k, "--dense-store")) dense_store = std::atoi(v);
     else if (!std::strcmp(k, "--seg")) kn.seg_chunks = std::atoi(v);
     else if (!std::strcmp(k, "--threads")) { lk.block_threads = std::atoi(v); have_knob[4] = true; }
     else if (!std::strcmp(k, "--waves")) { lk.chunk_waves = std::atoi(v); have_knob[5] = true; }
@@ -477,7 +480,7 @@ int main(int argc, char** argv) {
     }
     RUN(E4.build_index(st));
     DenseEngine D;
-    D.launch = BwdLaunch{dense_threads, dense_wg, dense_overlap ? 0 : dense_maxwg, 0};
+    D.launch = BwdLaunch{dense_threads, dense_wg, dense_overlap ? 0 : dense_maxwg, dense_store};
     if (!D.init(E4, st, err, sizeof(err))) {
       std::fprintf(stderr, "bw_batch: %s\n", err);
       return 5;
@@ -634,11 +637,11 @@ int main(int argc, char** argv) {
     };
     const double sm = mean(step_ms);
     std::snprintf(dense_json, sizeof(dense_json),
-                  "{\"rmax\":%d,\"nblk\":%d,\"threads\":%d,\"wg_per_cu\":%d,\"carry_bytes\":%zu,\"carry_build_ms\":%.3f,"
+                  "{\"rmax\":%d,\"nblk\":%d,\"threads\":%d,\"wg_per_cu\":%d,\"store\":%d,\"carry_bytes\":%zu,\"carry_build_ms\":%.3f,"
                   "\"index_wall_ms\":%.3f,\"step_ms\":{\"avg\":%.4f,\"min\":%.4f,\"max\":%.4f,\"repeat\":%d},"
                   "\"plan_ms\":%.4f,\"replay_ms\":%.4f,\"queries_per_s\":%.1f,\"answers_sha256\":\"%s\","
                   "\"offset_mismatch\":%" PRId64 ",\"overlap\":%s,\"mismatch\":%" PRId64 "}",
-                  dense_rmax, E4.x.nblk, dense_threads, dense_wg, D.carry_bytes, D.build_ms, dense_index_wall, sm,
+                  dense_rmax, E4.x.nblk, dense_threads, dense_wg, dense_store, D.carry_bytes, D.build_ms, dense_index_wall, sm,
                   *std::min_element(step_ms.begin(), step_ms.end()), *std::max_element(step_ms.begin(), step_ms.end()),
                   repeat, mean(dplan_v), mean(drep_v), double(M) / (sm * 1e-3), dxr.sha256_hex, offset_mismatch,
                   overlap_json, dense_mismatch);

@@ -70,14 +70,22 @@ int bwd_carry_workspace(const BwdIndex* x, size_t* bytes);
 int bwd_carry_build_launch(const BwdIndex* x, const int64_t* agg, void* work, size_t work_bytes, void* stream);
 
 /* Launch shape of the persistent block replay: workgroup threads (256, 512 or 1024; each thread
- * owns rmax/threads in-block ranks, at most 16), workgroups per compute unit, and a cap on the
+ * owns rmax/threads in-block ranks, at most 16), workgroups per compute unit, a cap on the
  * workgroups (0: none) that leaves compute units free for work on another stream, such as the
- * next batch's plan. */
+ * next batch's plan, and how answers reach their query ids:
+ *   BWD_STORE_DIRECT  the replay stores each answer at its query id (one random 8-byte store each);
+ *   BWD_STORE_STAGED  the replay stores answers in sorted order, then a partition pass groups them
+ *                     by windows of 2^18 query ids and a scatter pass stores each window from one
+ *                     die (two streaming passes instead of random stores);
+ *   BWD_STORE_AUTO    staged when the batch's ids span more than one window. */
+#define BWD_STORE_AUTO 0
+#define BWD_STORE_DIRECT 1
+#define BWD_STORE_STAGED 2
 typedef struct BwdLaunch {
   int32_t threads;
   int32_t wg_per_cu;
   int32_t max_workgroups;
-  int32_t pad0;
+  int32_t store;             /* BWD_STORE_* */
 } BwdLaunch;
 
 /* A planned batch of M queries: the queries sorted by (state rank, query id), with the group of

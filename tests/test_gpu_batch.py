@@ -107,7 +107,9 @@ def test_all_device_pipeline(pkg, case, knobs):
 @pytest.mark.parametrize("case,knobs", [
     ("s1_batched_small", []),
     ("s1_batched_small", ["--dense-rmax", "64", "--dense-threads", "256"]),
+    ("s1_batched_small", ["--dense-store", "2"]),
     ("s2_batched_mid", []),
+    ("s2_batched_mid", ["--dense-store", "1"]),
     ("s2_batched_mid", ["--dense-rmax", "1024", "--dense-threads", "512"]),
     ("s2_batched_mid", ["--dense-rmax", "2048", "--dense-threads", "256", "--device-gen"]),
     ("S4_batched_full", ["--tuned", None]),
@@ -116,7 +118,9 @@ def test_dense_form_matches_oracle(pkg, case, knobs):
     """The dense-carry form (include/bwgpu_dense.h): carry rows built from the blocks' weights, the
     stream sorted by state rank on the device, one persistent work-queue replay launch (replayed
     from a hipGraph) -- the oracle's answers at every block size and workgroup shape, small trees
-    (one block, or blocks of 64 keys) to the full 102.5M-query stream."""
+    (one block, or blocks of 64 keys) to the full 102.5M-query stream, with the answers stored at
+    their ids directly or staged (sorted order, partition by id window, die-local scatter; the
+    default once the ids span more than one window, so from s2 on)."""
     g = STREAMS[case]
     knobs = [pkg.TUNED if k is None else k for k in knobs]
     r = run_json(_cmd(pkg, g, knobs + ["--plain-sample", "0", "--dense"]), timeout=900)
