@@ -7,9 +7,9 @@
 //                       weights (agg) become per-block prefix functions D_b(x); rows are their
 //                       running sum over blocks, computed per (coordinate range, block segment)
 //                       with a scan over segments in between, so ~45k workgroups share the work.
-//   plan             <- qlb/qhb (src/batch_engine_v7.hpp:666-698): queries sorted by the preorder
-//                       rank of their state (rocPRIM radix sort in two 10-bit passes), group offsets
-//                       per rank.
+//   plan             <- qlb/qhb (src/batch_engine_v7.hpp:666-698): queries grouped by the preorder
+//                       rank of their state (a two-level bucket sort: high rank digit, then low
+//                       digit inside each bucket), group offsets per rank.
 //   persistent block <- v7_block_kernel + gpu_combine_kernel (src/gpu_v7.cu:10-96): workgroups
 //                       take blocks from a device queue; each replays its block's steps (events
 //                       are ds_add_u64 into a dense per-rank array; at every enter step a DPP
@@ -22,7 +22,6 @@
 
 #include <algorithm>
 #include <mutex>
-#include <rocprim/device/device_radix_sort.hpp>
 #include <stdint.h>
 #include <stdio.h>
 
@@ -181,92 +180,349 @@ __global__ void __launch_bounds__(256) bwd_seg_scan_kernel(int64_t NKP, int nseg
   }
 }
 
-// ---------------------------------------------------------------- plan
-__global__ void bwd_init_kernel(int64_t* flags, uint32_t* counter) {
-  if (threadIdx.x == 0) {
-    if (flags) flags[0] = 0;
-    if (counter) counter[0] = 0;
+// ---------------------------------------------------------------- plan: queries grouped by state rank
+// The replay needs the queries grouped by the preorder rank e of their state, in any order inside a
+// rank (answers are stored by sorted position and then routed to query ids, or stored by id).  A
+// two-level bucket sort -- the high digit e >> kLoBits, then the low digit e & (kLoBins - 1) inside
+// each high bucket -- so no pass needs a stable order and runs are placed by counts and cursors:
+//   count    per tile of queries: LDS counts of the high digit, added to the global counts;
+//   layout   one workgroup: bucket starts (also the buckets' cursors), the second level's tiles
+//            per bucket (tile j of bucket h covers positions [hbase[h] + j*kPTile, ...) of it),
+//            grp[S] = M;
+//   bucket   per tile of queries: the sort key packed (key = rank << 44 | id << 17 | lo & 0x1FFFF,
+//            val = hi | (lo >> 17) << 21), grouped by high digit in LDS, each bucket's run appended
+//            at the bucket's cursor (16-byte loads when the query arrays allow);
+//   tile     per second-level tile: LDS counts of the low digit;
+//   offsets  per bucket: each tile's start inside every rank of the bucket, and grp[e], the first
+//            sorted position of rank e;
+//   rank     per second-level tile: grouped by low digit in LDS, each rank's run stored at grp[e]
+//            plus the tile's start.
+// A run goes through LDS so that each store instruction writes consecutive positions.  Counts and
+// placements come from the same rank of the same queries, so every position in [0, M) is written
+// once; a position outside it (which the counts rule out) would set flags[0] to 2 and not be
+// written.  An out-of-range query (state outside [1, S], an endpoint above NK) sets flags[0] to 1
+// and is planned over [0, 0) (answer 0), at state 1's rank when its state is out of range.
+constexpr int kLoBits = 9, kLoBins = 1 << kLoBits;       // 512 ranks per bucket
+constexpr int kMaxHi = (BWD_MAX_STATES >> kLoBits) + 1;  // 1,024 buckets
+constexpr int kPT = 1024, kPE = 16, kPTile = kPT * kPE;  // 16,384 queries per tile, both levels
+constexpr int kHiShift = kRankShift + kLoBits;           // key >> kHiShift = the high digit
+static_assert(kMaxHi <= kPT && kLoBins <= kPT, "one thread per bucket or digit in the scans");
+
+// Exclusive scan of one value per thread over the workgroup (every thread calls it); wtot is LDS
+// scratch of blockDim.x / 64 entries.  *total receives the sum.
+__device__ __forceinline__ uint32_t block_excl_scan_u32(uint32_t v, uint32_t* wtot, uint32_t* total) {
+  const int tid = threadIdx.x, wv = tid >> 6, nw = blockDim.x >> 6;
+  const uint32_t inc = (uint32_t)wave_incl_scan((int64_t)v);
+  if ((tid & 63) == 63) wtot[wv] = inc;
+  __syncthreads();
+  uint32_t off = 0, tot = 0;
+  for (int k = 0; k < nw; ++k) {
+    const uint32_t t = wtot[k];
+    off += k < wv ? t : 0u;
+    tot += t;
+  }
+  __syncthreads();  // wtot is free for the next scan
+  *total = tot;
+  return off + inc - v;
+}
+
+__device__ __forceinline__ uint32_t state_rank(const BwdIndex& x, int32_t v) {
+  return (uint32_t)x.state_erank[(v >= 1 && v <= x.S) ? v : 1];
+}
+
+__global__ void __launch_bounds__(1024) bwd_plan_init_kernel(int64_t* flags, uint32_t* hcount, int nh) {
+  if (threadIdx.x == 0) flags[0] = 0;
+  for (int i = threadIdx.x; i < nh; i += 1024) hcount[i] = 0;
+}
+
+__global__ void __launch_bounds__(kPT) bwd_count_kernel(const BwdIndex x, int64_t M, const int32_t* __restrict__ qstate,
+                                                        int nh, uint32_t* __restrict__ hcount) {
+  __shared__ uint32_t c[kMaxHi];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < nh; i += kPT) c[i] = 0;
+  const int64_t t0 = (int64_t)blockIdx.x * kPTile;
+  int32_t v[kPE];
+#pragma unroll
+  for (int k = 0; k < kPE; ++k) {
+    const int64_t i = t0 + k * kPT + tid;
+    v[k] = i < M ? qstate[i] : 1;
+  }
+  uint32_t e[kPE];
+#pragma unroll
+  for (int k = 0; k < kPE; ++k) e[k] = state_rank(x, v[k]);
+  __syncthreads();  // counts are zero
+#pragma unroll
+  for (int k = 0; k < kPE; ++k)
+    if (t0 + k * kPT + tid < M) atomicAdd(&c[e[k] >> kLoBits], 1u);
+  __syncthreads();
+  for (int h = tid; h < nh; h += kPT)
+    if (c[h]) atomicAdd(hcount + h, c[h]);
+}
+
+// hbase[h] (and the cursor hcur[h]) = first position of bucket h, hbase[nh] = M; tstart[h] = first
+// second-level tile of bucket h, tstart[nh] = their number; grp[S] = M.
+__global__ void __launch_bounds__(kPT) bwd_layout_kernel(int64_t S, int64_t M, int nh, const uint32_t* __restrict__ hcount,
+                                                         uint32_t* __restrict__ hbase, uint32_t* __restrict__ hcur,
+                                                         uint32_t* __restrict__ tstart, uint32_t* __restrict__ grp) {
+  __shared__ uint32_t wtot[kPT / 64];
+  const int h = threadIdx.x;
+  const uint32_t c = h < nh ? hcount[h] : 0u;
+  const uint32_t t = (c + kPTile - 1) / kPTile;
+  uint32_t ctot, ttot;
+  const uint32_t cb = block_excl_scan_u32(c, wtot, &ctot);
+  const uint32_t tb = block_excl_scan_u32(t, wtot, &ttot);
+  if (h < nh) {
+    hbase[h] = cb;
+    hcur[h] = cb;
+    tstart[h] = tb;
+  }
+  if (h == 0) {
+    hbase[nh] = (uint32_t)M;
+    tstart[nh] = ttot;
+    grp[S] = (uint32_t)M;
   }
 }
 
-// The sort's input: key = rank << 44 | query id << 17 | lo & 0x1FFFF, val = hi | (lo >> 17) << 21.
-// An out-of-range query sets flags[0] and is planned as a query of state 1 over [0, 0).  Each
-// thread packs kKeysPer queries: with kVec four consecutive ones, read and written with 16-byte
-// accesses (the caller's arrays 16-byte aligned), else four a workgroup apart; either way all of a
-// thread's loads are issued before its rank gathers.
-constexpr int kKeysPer = 4;
 template <bool kVec>
-__global__ void __launch_bounds__(256) bwd_keys_kernel(const BwdIndex x, int64_t M, const int32_t* __restrict__ qstate,
-                                                       const uint32_t* __restrict__ qlo,
-                                                       const uint32_t* __restrict__ qhi, uint64_t* __restrict__ keys,
-                                                       uint32_t* __restrict__ vals, int64_t* flags) {
-  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  const int64_t qv = t * kKeysPer;  // kVec: the thread's first query
-  const bool whole = kVec && qv + kKeysPer <= M;
-  int64_t q[kKeysPer];
-  int32_t v[kKeysPer];
-  uint32_t lo[kKeysPer], hi[kKeysPer];
+__global__ void __launch_bounds__(kPT) bwd_bucket_kernel(const BwdIndex x, int64_t M, const int32_t* __restrict__ qstate,
+                                                         const uint32_t* __restrict__ qlo, const uint32_t* __restrict__ qhi,
+                                                         int nh, uint32_t* __restrict__ hcur, uint64_t* __restrict__ keys,
+                                                         uint32_t* __restrict__ vals, int64_t* flags) {
+  __shared__ uint32_t cnt[kMaxHi], lstart[kMaxHi], gbase[kMaxHi];
+  __shared__ uint32_t wtot[kPT / 64];
+  __shared__ uint64_t stage[kPTile];  // the keys, then (as 32-bit words) the values
+  const int tid = threadIdx.x;
+  const int64_t t0 = (int64_t)blockIdx.x * kPTile;
+  const int n = (int)min<int64_t>(kPTile, M - t0);
+  for (int i = tid; i < nh; i += kPT) cnt[i] = 0;
+  // the thread's k-th query: with kVec four consecutive ones per 16-byte load, in four rounds a
+  // workgroup of quadruples apart; otherwise one a workgroup apart
+  auto qid = [&](int k) -> int64_t {
+    return kVec ? t0 + (int64_t)(k >> 2) * (kPT * 4) + 4 * tid + (k & 3) : t0 + (int64_t)k * kPT + tid;
+  };
+  int32_t v[kPE];
+  uint32_t lo[kPE], hi[kPE];
+  if (kVec) {
 #pragma unroll
-  for (int k = 0; k < kKeysPer; ++k) q[k] = kVec ? qv + k : (int64_t)blockIdx.x * (256 * kKeysPer) + threadIdx.x + k * 256;
-  if (whole) {
-    const int4 a = *reinterpret_cast<const int4*>(qstate + qv);
-    const uint4 l = *reinterpret_cast<const uint4*>(qlo + qv);
-    const uint4 h = *reinterpret_cast<const uint4*>(qhi + qv);
-    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
-    lo[0] = l.x; lo[1] = l.y; lo[2] = l.z; lo[3] = l.w;
-    hi[0] = h.x; hi[1] = h.y; hi[2] = h.z; hi[3] = h.w;
+    for (int r = 0; r < kPE / 4; ++r) {
+      const int64_t b = qid(4 * r);
+      if (b + 4 <= M) {
+        const int4 a = *reinterpret_cast<const int4*>(qstate + b);
+        const uint4 l = *reinterpret_cast<const uint4*>(qlo + b);
+        const uint4 h = *reinterpret_cast<const uint4*>(qhi + b);
+        v[4 * r] = a.x; v[4 * r + 1] = a.y; v[4 * r + 2] = a.z; v[4 * r + 3] = a.w;
+        lo[4 * r] = l.x; lo[4 * r + 1] = l.y; lo[4 * r + 2] = l.z; lo[4 * r + 3] = l.w;
+        hi[4 * r] = h.x; hi[4 * r + 1] = h.y; hi[4 * r + 2] = h.z; hi[4 * r + 3] = h.w;
+      } else {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const bool in = b + c < M;
+          v[4 * r + c] = in ? qstate[b + c] : 1;
+          lo[4 * r + c] = in ? qlo[b + c] : 0;
+          hi[4 * r + c] = in ? qhi[b + c] : 0;
+        }
+      }
+    }
   } else {
 #pragma unroll
-    for (int k = 0; k < kKeysPer; ++k) {
-      const bool in = q[k] < M;
-      v[k] = in ? qstate[q[k]] : 1;
-      lo[k] = in ? qlo[q[k]] : 0;
-      hi[k] = in ? qhi[q[k]] : 0;
+    for (int k = 0; k < kPE; ++k) {
+      const int64_t i = qid(k);
+      const bool in = i < M;
+      v[k] = in ? qstate[i] : 1;
+      lo[k] = in ? qlo[i] : 0;
+      hi[k] = in ? qhi[i] : 0;
     }
   }
   bool bad = false;
+  uint64_t key[kPE];
+  uint32_t val[kPE];
 #pragma unroll
-  for (int k = 0; k < kKeysPer; ++k) {
-    if (v[k] < 1 || v[k] > x.S || lo[k] > x.NK || hi[k] > x.NK) {
+  for (int k = 0; k < kPE; ++k) {
+    uint32_t l = lo[k], h = hi[k];
+    if (qid(k) < M && (v[k] < 1 || v[k] > x.S || l > x.NK || h > x.NK)) {
       bad = true;
-      v[k] = 1;
-      lo[k] = hi[k] = 0;
+      l = h = 0;
     }
+    key[k] = ((uint64_t)state_rank(x, v[k]) << kRankShift) | ((uint64_t)qid(k) << kIdShift) | (uint64_t)(l & kLoLowMask);
+    val[k] = h | ((l >> 17) << 21);
   }
-  uint64_t key[kKeysPer];
-  uint32_t val[kKeysPer];
+  __syncthreads();  // counts are zero
+  uint32_t off[kPE];
 #pragma unroll
-  for (int k = 0; k < kKeysPer; ++k) {
-    const uint64_t e = (uint64_t)(uint32_t)x.state_erank[v[k]];
-    key[k] = (e << kRankShift) | ((uint64_t)q[k] << kIdShift) | (uint64_t)(lo[k] & kLoLowMask);
-    val[k] = hi[k] | ((lo[k] >> 17) << 21);
+  for (int k = 0; k < kPE; ++k) off[k] = qid(k) < M ? atomicAdd(&cnt[key[k] >> kHiShift], 1u) : 0u;
+  __syncthreads();
+  uint32_t tot;
+  const uint32_t c = tid < nh ? cnt[tid] : 0u;
+  const uint32_t ls = block_excl_scan_u32(c, wtot, &tot);
+  if (tid < nh) {
+    lstart[tid] = ls;
+    if (c) gbase[tid] = atomicAdd(hcur + tid, c);
   }
-  if (whole) {
-    ulonglong2* kp = reinterpret_cast<ulonglong2*>(keys + qv);
-    kp[0] = make_ulonglong2(key[0], key[1]);
-    kp[1] = make_ulonglong2(key[2], key[3]);
-    *reinterpret_cast<uint4*>(vals + qv) = make_uint4(val[0], val[1], val[2], val[3]);
-  } else {
+  __syncthreads();
 #pragma unroll
-    for (int k = 0; k < kKeysPer; ++k) {
-      if (q[k] < M) {
-        keys[q[k]] = key[k];
-        vals[q[k]] = val[k];
+  for (int k = 0; k < kPE; ++k)
+    if (qid(k) < M) stage[lstart[key[k] >> kHiShift] + off[k]] = key[k];
+  __syncthreads();
+  uint32_t dst[kPE];
+  bool lost = false;
+#pragma unroll
+  for (int i = 0; i < kPE; ++i) {
+    const int j = tid + i * kPT;
+    dst[i] = 0xFFFFFFFFu;
+    if (j < n) {
+      const uint64_t kk = stage[j];
+      const int h = (int)(kk >> kHiShift);
+      const uint32_t d = gbase[h] + (uint32_t)(j - (int)lstart[h]);
+      if (d < (uint64_t)M) {
+        dst[i] = d;
+        keys[d] = kk;
+      } else {
+        lost = true;
       }
     }
   }
+  __syncthreads();  // the keys are out of the stage
+  uint32_t* sv = reinterpret_cast<uint32_t*>(stage);
+#pragma unroll
+  for (int k = 0; k < kPE; ++k)
+    if (qid(k) < M) sv[lstart[key[k] >> kHiShift] + off[k]] = val[k];
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < kPE; ++i)
+    if (dst[i] != 0xFFFFFFFFu) vals[dst[i]] = sv[tid + i * kPT];
   if (bad) flags[0] = 1;
+  if (lost) flags[0] = 2;
 }
 
-// grp[e] = first sorted position whose rank is >= e, for e in [0, S]; grp[S] = M
-__global__ void __launch_bounds__(256) bwd_groups_kernel(int64_t S, int64_t M, const uint64_t* keys, uint32_t* grp) {
-  const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (j >= M) return;
-  const int64_t e = (int64_t)(keys[j] >> kRankShift);
-  const int64_t pe = j > 0 ? (int64_t)(keys[j - 1] >> kRankShift) : -1;
-  for (int64_t t = pe + 1; t <= e; ++t) grp[t] = (uint32_t)j;
-  if (j == M - 1)
-    for (int64_t t = e + 1; t <= S; ++t) grp[t] = (uint32_t)M;
+// The second-level tile of workgroup g: bucket h (the last one whose tiles start at or before g) and
+// its positions [p0, p1).  False for the workgroups beyond the last tile (the grid is a bound).
+__device__ __forceinline__ bool tile_of(int g, int nh, const uint32_t* tstart, const uint32_t* hbase, int& h,
+                                        uint32_t& p0, uint32_t& p1) {
+  if ((uint32_t)g >= tstart[nh]) return false;
+  int a = 0, b = nh;  // tstart[a] <= g < tstart[b]
+  while (b - a > 1) {
+    const int mid = (a + b) >> 1;
+    if (tstart[mid] <= (uint32_t)g) a = mid; else b = mid;
+  }
+  h = a;
+  p0 = hbase[h] + (uint32_t)(g - (int)tstart[h]) * kPTile;
+  p1 = min(p0 + (uint32_t)kPTile, hbase[h + 1]);
+  return true;
+}
+
+__global__ void __launch_bounds__(kPT) bwd_tile_count_kernel(int nh, const uint32_t* __restrict__ hbase,
+                                                             const uint32_t* __restrict__ tstart,
+                                                             const uint64_t* __restrict__ keys, uint32_t* __restrict__ thist) {
+  __shared__ uint32_t c[kLoBins];
+  int h;
+  uint32_t p0, p1;
+  if (!tile_of(blockIdx.x, nh, tstart, hbase, h, p0, p1)) return;  // the whole workgroup
+  const int tid = threadIdx.x;
+  if (tid < kLoBins) c[tid] = 0;
+  uint32_t e[kPE];
+#pragma unroll
+  for (int k = 0; k < kPE; ++k) {
+    const uint32_t p = p0 + k * kPT + tid;
+    e[k] = p < p1 ? (uint32_t)(keys[p] >> kRankShift) : 0xFFFFFFFFu;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kPE; ++k)
+    if (e[k] != 0xFFFFFFFFu) atomicAdd(&c[e[k] & (kLoBins - 1)], 1u);
+  __syncthreads();
+  if (tid < kLoBins) thist[(size_t)blockIdx.x * kLoBins + tid] = c[tid];
+}
+
+// Per bucket h, thread l: thist[g][l] becomes tile g's start inside rank h*kLoBins + l (the counts
+// of the bucket's earlier tiles), and grp[e] = the rank's first sorted position.
+__global__ void __launch_bounds__(kLoBins) bwd_tile_offsets_kernel(int64_t S, const uint32_t* __restrict__ hbase,
+                                                                   const uint32_t* __restrict__ tstart,
+                                                                   uint32_t* __restrict__ thist, uint32_t* __restrict__ grp) {
+  __shared__ uint32_t wtot[kLoBins / 64];
+  const int h = blockIdx.x, l = threadIdx.x;
+  const uint32_t g0 = tstart[h], g1 = tstart[h + 1];
+  uint32_t s = 0;
+  for (uint32_t g = g0; g < g1; ++g) {
+    uint32_t* p = thist + (size_t)g * kLoBins + l;
+    const uint32_t c = *p;
+    *p = s;
+    s += c;
+  }
+  uint32_t tot;
+  const uint32_t base = block_excl_scan_u32(s, wtot, &tot);
+  const int64_t e = (int64_t)h * kLoBins + l;
+  if (e < S) grp[e] = hbase[h] + base;
+}
+
+__global__ void __launch_bounds__(kPT) bwd_rank_kernel(int64_t M, int nh, const uint32_t* __restrict__ hbase,
+                                                       const uint32_t* __restrict__ tstart, const uint32_t* __restrict__ thist,
+                                                       const uint32_t* __restrict__ grp, const uint64_t* __restrict__ kin,
+                                                       const uint32_t* __restrict__ vin, uint64_t* __restrict__ kout,
+                                                       uint32_t* __restrict__ vout, int64_t* flags) {
+  __shared__ uint32_t cnt[kLoBins], lstart[kLoBins], gdst[kLoBins];
+  __shared__ uint32_t wtot[kPT / 64];
+  __shared__ uint64_t stage[kPTile];
+  int h;
+  uint32_t p0, p1;
+  if (!tile_of(blockIdx.x, nh, tstart, hbase, h, p0, p1)) return;  // the whole workgroup
+  const int tid = threadIdx.x;
+  const int n = (int)(p1 - p0);
+  if (tid < kLoBins) cnt[tid] = 0;
+  // every load is issued before any is used (clamped to the tile's last position)
+  uint64_t key[kPE];
+  uint32_t val[kPE];
+#pragma unroll
+  for (int k = 0; k < kPE; ++k) {
+    const uint32_t p = p0 + (uint32_t)min(k * kPT + tid, n - 1);
+    key[k] = __builtin_nontemporal_load(kin + p);
+    val[k] = __builtin_nontemporal_load(vin + p);
+  }
+  __syncthreads();  // counts are zero
+  uint32_t off[kPE];
+#pragma unroll
+  for (int k = 0; k < kPE; ++k)
+    off[k] = k * kPT + tid < n ? atomicAdd(&cnt[(key[k] >> kRankShift) & (kLoBins - 1)], 1u) : 0u;
+  __syncthreads();
+  uint32_t tot;
+  const uint32_t c = tid < kLoBins ? cnt[tid] : 0u;
+  const uint32_t ls = block_excl_scan_u32(c, wtot, &tot);
+  if (tid < kLoBins) {
+    lstart[tid] = ls;
+    gdst[tid] = c ? grp[(int64_t)h * kLoBins + tid] + thist[(size_t)blockIdx.x * kLoBins + tid] : 0u;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kPE; ++k)
+    if (k * kPT + tid < n) stage[lstart[(key[k] >> kRankShift) & (kLoBins - 1)] + off[k]] = key[k];
+  __syncthreads();
+  uint32_t dst[kPE];
+  bool lost = false;
+#pragma unroll
+  for (int i = 0; i < kPE; ++i) {
+    const int j = tid + i * kPT;
+    dst[i] = 0xFFFFFFFFu;
+    if (j < n) {
+      const uint64_t kk = stage[j];
+      const int l = (int)((kk >> kRankShift) & (kLoBins - 1));
+      const uint32_t d = gdst[l] + (uint32_t)(j - (int)lstart[l]);
+      if (d < (uint64_t)M) {
+        dst[i] = d;
+        kout[d] = kk;
+      } else {
+        lost = true;
+      }
+    }
+  }
+  __syncthreads();  // the keys are out of the stage
+  uint32_t* sv = reinterpret_cast<uint32_t*>(stage);
+#pragma unroll
+  for (int k = 0; k < kPE; ++k)
+    if (k * kPT + tid < n) sv[lstart[(key[k] >> kRankShift) & (kLoBins - 1)] + off[k]] = val[k];
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < kPE; ++i)
+    if (dst[i] != 0xFFFFFFFFu) vout[dst[i]] = sv[tid + i * kPT];
+  if (lost) flags[0] = 2;
 }
 
 // ---------------------------------------------------------------- persistent block replay
@@ -517,9 +773,10 @@ constexpr int kScatT = 256, kScatE = 8, kScatTile = kScatT * kScatE;
 constexpr int kDies = 8;
 static_assert(kMaxWin <= kPartT && kMaxWin % 64 == 0, "one thread per window in the partition's scan");
 
-__global__ void __launch_bounds__(512) bwd_replay_init_kernel(uint32_t* counter, uint32_t* cursor, int nw) {
+__global__ void __launch_bounds__(256) bwd_replay_init_kernel(uint32_t* counter, uint32_t* cursor, int nw) {
   if (threadIdx.x == 0) counter[0] = 0;
-  if (cursor && (int)threadIdx.x < nw) cursor[threadIdx.x] = 0;
+  if (cursor)
+    for (int i = threadIdx.x; i < nw; i += 256) cursor[i] = 0;
 }
 
 __global__ void __launch_bounds__(kPartT) bwd_part_kernel(int64_t M, const uint64_t* __restrict__ keys,
@@ -605,12 +862,6 @@ int report(const char* what, hipError_t e) {
   return (int)e;
 }
 
-unsigned bits_for(uint64_t v) {
-  unsigned b = 1;
-  while (b < 64 && (v >> b) != 0) ++b;
-  return b;
-}
-
 bool index_ok(const BwdIndex* x) {
   return x && x->S >= 1 && x->S <= BWD_MAX_STATES && x->U >= 1 && x->nblk >= 1 && x->rmax >= 1 &&
          x->rmax <= BWD_MAX_RMAX && x->NK >= 1 && x->NK <= BWD_MAX_NK && x->carry_ld >= (int64_t)x->NK + 1 &&
@@ -643,31 +894,18 @@ CarryLayout carry_layout(const BwdIndex* x, int64_t nkeys) {
   return L;
 }
 
-// The plan's sort: 10-bit digits, so the 19 rank bits of a 300,000-state tree take two passes
-// (rocPRIM's default for 64-bit keys takes three 8-bit passes); 1,024-thread workgroups of 12 items.
-using PlanSortConfig = rocprim::radix_sort_config<
-    rocprim::default_config, rocprim::default_config,
-    rocprim::radix_sort_onesweep_config<rocprim::kernel_config<1024, 16>, rocprim::kernel_config<1024, 12>, 10,
-                                        rocprim::block_radix_rank_algorithm::match>>;
-
-hipError_t plan_sort(void* temp, size_t& temp_bytes, const uint64_t* kin, uint64_t* kout, const uint32_t* vin,
-                     uint32_t* vout, int64_t M, unsigned end_bit, hipStream_t st) {
-  return rocprim::radix_sort_pairs<PlanSortConfig>(temp, temp_bytes, kin, kout, vin, vout, (size_t)M,
-                                                   (unsigned)kRankShift, end_bit, st);
-}
-
-// kin / vin (the sort's input) are free once the plan has run: the unsort's staging buffer reuses
-// them (stage_ans = kin, stage_id = vin); sans holds the replay's answers in sorted order.
+// The planned batch: keys/vals (the bucket pass's output: queries grouped by high rank digit; then,
+// once the plan has run, the unsort's staging buffer: stage_ans = kin, stage_id = vin), kout/vout
+// (queries grouped by rank), grp, the bucket tables; sans holds the replay's answers in sorted order.
 struct AnswerLayout {
-  size_t kin, kout, vin, vout, grp, counter, cursor, sans, temp, temp_bytes, total;
-  unsigned end_bit;
+  size_t kin, kout, vin, vout, grp, counter, cursor, sans, hcount, hcur, hbase, tstart, thist, total;
+  int nh;
+  unsigned tiles;  // bound on the second level's tiles (its grid)
 };
 
 hipError_t answer_layout(const BwdIndex* x, int64_t M, AnswerLayout& L) {
-  L.end_bit = kRankShift + bits_for((uint64_t)(x->S > 1 ? x->S - 1 : 1));
-  L.temp_bytes = 0;
-  const hipError_t e = plan_sort(nullptr, L.temp_bytes, nullptr, nullptr, nullptr, nullptr, M, L.end_bit, nullptr);
-  if (e) return e;
+  L.nh = (int)(((x->S - 1) >> kLoBits) + 1);
+  L.tiles = (unsigned)((M + kPTile - 1) / kPTile + L.nh);
   size_t o = 0;
   auto take = [&](size_t n) {
     const size_t a = o;
@@ -682,7 +920,11 @@ hipError_t answer_layout(const BwdIndex* x, int64_t M, AnswerLayout& L) {
   L.counter = take(4);
   L.cursor = take((size_t)kMaxWin * 4);
   L.sans = take((size_t)M * 8);
-  L.temp = take(std::max<size_t>(L.temp_bytes, 1));
+  L.hcount = take((size_t)kMaxHi * 4);
+  L.hcur = take((size_t)kMaxHi * 4);
+  L.hbase = take((size_t)(kMaxHi + 1) * 4);
+  L.tstart = take((size_t)(kMaxHi + 1) * 4);
+  L.thist = take((size_t)L.tiles * kLoBins * 4);
   L.total = o;
   return hipSuccess;
 }
@@ -772,21 +1014,31 @@ extern "C" int bwd_plan_launch(const BwdIndex* x, int64_t M, const int32_t* qsta
   uint32_t* vin = at<uint32_t>(work, L.vin);
   uint32_t* vout = at<uint32_t>(work, L.vout);
   uint32_t* grp = at<uint32_t>(work, L.grp);
+  uint32_t* hcount = at<uint32_t>(work, L.hcount);
+  uint32_t* hcur = at<uint32_t>(work, L.hcur);
+  uint32_t* hbase = at<uint32_t>(work, L.hbase);
+  uint32_t* tstart = at<uint32_t>(work, L.tstart);
+  uint32_t* thist = at<uint32_t>(work, L.thist);
+  const int nh = L.nh;
+  const unsigned qgrid = (unsigned)((M + kPTile - 1) / kPTile);
   int rc;
-  hipLaunchKernelGGL(bwd_init_kernel, dim3(1), dim3(64), 0, st, flags, (uint32_t*)nullptr);
-  const unsigned kgrid = (unsigned)((M + 256 * kKeysPer - 1) / (256 * kKeysPer));
-  // the 16-byte form when the caller's query arrays allow it (the workspace arrays are 256-byte aligned)
+  hipLaunchKernelGGL(bwd_plan_init_kernel, dim3(1), dim3(1024), 0, st, flags, hcount, nh);
+  hipLaunchKernelGGL(bwd_count_kernel, dim3(qgrid), dim3(kPT), 0, st, *x, M, qstate, nh, hcount);
+  hipLaunchKernelGGL(bwd_layout_kernel, dim3(1), dim3(kPT), 0, st, x->S, M, nh, hcount, hbase, hcur, tstart, grp);
+  if ((rc = report("plan counts", hipGetLastError()))) return rc;
+  // the 16-byte form when the caller's query arrays allow it
   if ((((uintptr_t)qstate | (uintptr_t)qlo | (uintptr_t)qhi) & 15) == 0)
-    hipLaunchKernelGGL(bwd_keys_kernel<true>, dim3(kgrid), dim3(256), 0, st, *x, M, qstate, qlo, qhi, kin, vin, flags);
+    hipLaunchKernelGGL(bwd_bucket_kernel<true>, dim3(qgrid), dim3(kPT), 0, st, *x, M, qstate, qlo, qhi, nh, hcur, kin,
+                       vin, flags);
   else
-    hipLaunchKernelGGL(bwd_keys_kernel<false>, dim3(kgrid), dim3(256), 0, st, *x, M, qstate, qlo, qhi, kin, vin, flags);
-  if ((rc = report("plan keys", hipGetLastError()))) return rc;
-  size_t tb = L.temp_bytes;
-  if ((rc = report("plan sort", plan_sort(at<void>(work, L.temp), tb, kin, kout, vin, vout, M, L.end_bit, st))))
-    return rc;
-  const unsigned grid = (unsigned)((M + 255) / 256);
-  hipLaunchKernelGGL(bwd_groups_kernel, dim3(grid), dim3(256), 0, st, x->S, M, kout, grp);
-  return report("plan groups", hipGetLastError());
+    hipLaunchKernelGGL(bwd_bucket_kernel<false>, dim3(qgrid), dim3(kPT), 0, st, *x, M, qstate, qlo, qhi, nh, hcur, kin,
+                       vin, flags);
+  if ((rc = report("plan buckets", hipGetLastError()))) return rc;
+  hipLaunchKernelGGL(bwd_tile_count_kernel, dim3(L.tiles), dim3(kPT), 0, st, nh, hbase, tstart, kin, thist);
+  hipLaunchKernelGGL(bwd_tile_offsets_kernel, dim3((unsigned)nh), dim3(kLoBins), 0, st, x->S, hbase, tstart, thist, grp);
+  hipLaunchKernelGGL(bwd_rank_kernel, dim3(L.tiles), dim3(kPT), 0, st, M, nh, hbase, tstart, thist, grp, kin, vin, kout,
+                     vout, flags);
+  return report("plan ranks", hipGetLastError());
 }
 
 extern "C" int bwd_replay_launch(const BwdIndex* x, int64_t M, int64_t* answers, const BwdLaunch* l, void* work,
@@ -818,7 +1070,7 @@ extern "C" int bwd_replay_launch(const BwdIndex* x, int64_t M, int64_t* answers,
   const int nwin = (int)((M + (int64_t(1) << kWinBits) - 1) >> kWinBits);
   uint32_t* counter = at<uint32_t>(work, L.counter);
   uint32_t* cursor = at<uint32_t>(work, L.cursor);
-  hipLaunchKernelGGL(bwd_replay_init_kernel, dim3(1), dim3(512), 0, st, counter, staged ? cursor : nullptr, nwin);
+  hipLaunchKernelGGL(bwd_replay_init_kernel, dim3(1), dim3(256), 0, st, counter, staged ? cursor : nullptr, nwin);
   int nwg = ncu * l->wg_per_cu;
   if (l->max_workgroups > 0 && l->max_workgroups < nwg) nwg = l->max_workgroups;
   const dim3 grid((unsigned)nwg);
