@@ -44,11 +44,10 @@ constexpr int kMaxDevices = 64;
 std::mutex g_scratch_mu;
 void* g_scratch_at[kMaxDevices] = {};
 
-// ---------------------------------------------------------------- sort key layout
-// key = rank << 44 | query id << 17 | lo & 0x1FFFF;  val = hi | (lo >> 17) << 21.  The sorted bits
-// [44, 44 + 19) end below bit 64: rocPRIM's path for inputs under 2^20 items (merge sort) left keys
-// sorted on bits [45, 64) out of order here, while its onesweep path sorted them.
-constexpr int kRankShift = 44;
+// ---------------------------------------------------------------- planned query layout
+// key = state << 44 | query id << 17 | lo & 0x1FFFF;  val = hi | (lo >> 17) << 21 (the state in 19
+// bits, the id in 27, the endpoints in 21 each).
+constexpr int kStateShift = 44;
 constexpr int kIdShift = 17;
 constexpr uint64_t kIdMask = (uint64_t(1) << 27) - 1;
 constexpr uint32_t kLoLowMask = (1u << 17) - 1;
@@ -180,32 +179,37 @@ __global__ void __launch_bounds__(256) bwd_seg_scan_kernel(int64_t NKP, int nseg
   }
 }
 
-// ---------------------------------------------------------------- plan: queries grouped by state rank
-// The replay needs the queries grouped by the preorder rank e of their state, in any order inside a
-// rank (answers are stored by sorted position and then routed to query ids, or stored by id).  A
-// two-level bucket sort -- the high digit e >> kLoBits, then the low digit e & (kLoBins - 1) inside
-// each high bucket -- so no pass needs a stable order and runs are placed by counts and cursors:
+// ---------------------------------------------------------------- plan: queries grouped by state
+// The replay needs the queries of each state together, in any order inside a state (answers are
+// stored by sorted position and then routed to query ids, or stored by id), and per preorder rank
+// of a state where its group lies.  The queries are grouped by state id (no lookup per query: a
+// random 4-byte lookup costs one L2 request, ~0.4 ms per 10^8), and one small pass per block maps
+// ranks to groups.  A two-level bucket sort -- the high digit v >> kLoBits of state v, then the low
+// digit v & (kLoBins - 1) inside each high bucket -- so no pass needs a stable order and runs are
+// placed by counts and cursors:
 //   count    per tile of queries: LDS counts of the high digit, added to the global counts;
 //   layout   one workgroup: bucket starts (also the buckets' cursors), the second level's tiles
 //            per bucket (tile j of bucket h covers positions [hbase[h] + j*kPTile, ...) of it),
-//            grp[S] = M;
-//   bucket   per tile of queries: the sort key packed (key = rank << 44 | id << 17 | lo & 0x1FFFF,
+//            grp[S + 1] = M;
+//   bucket   per tile of queries: the key packed (key = state << 44 | id << 17 | lo & 0x1FFFF,
 //            val = hi | (lo >> 17) << 21), grouped by high digit in LDS, each bucket's run appended
 //            at the bucket's cursor (16-byte loads when the query arrays allow);
 //   tile     per second-level tile: LDS counts of the low digit;
-//   offsets  per bucket: each tile's start inside every rank of the bucket, and grp[e], the first
-//            sorted position of rank e;
-//   rank     per second-level tile: grouped by low digit in LDS, each rank's run stored at grp[e]
-//            plus the tile's start.
+//   offsets  per bucket: each tile's start inside every state of the bucket, and grp[v], the first
+//            sorted position of state v;
+//   state    per second-level tile: grouped by low digit in LDS, each state's run stored at grp[v]
+//            plus the tile's start;
+//   ranks    per block: the group [first, end) of the state of each of its ranks, and the block's
+//            last rank with queries (-1: none), which ends its replay.
 // A run goes through LDS so that each store instruction writes consecutive positions.  Counts and
 // placements come from the same rank of the same queries, so every position in [0, M) is written
 // once; a position outside it (which the counts rule out) would set flags[0] to 2 and not be
 // written.  An out-of-range query (state outside [1, S], an endpoint above NK) sets flags[0] to 1
-// and is planned over [0, 0) (answer 0), at state 1's rank when its state is out of range.
-constexpr int kLoBits = 9, kLoBins = 1 << kLoBits;       // 512 ranks per bucket
+// and is planned over [0, 0) (answer 0), as a query of state 1 when its state is out of range.
+constexpr int kLoBits = 9, kLoBins = 1 << kLoBits;       // 512 states per bucket
 constexpr int kMaxHi = (BWD_MAX_STATES >> kLoBits) + 1;  // 1,024 buckets
 constexpr int kPT = 1024, kPE = 16, kPTile = kPT * kPE;  // 16,384 queries per tile, both levels
-constexpr int kHiShift = kRankShift + kLoBits;           // key >> kHiShift = the high digit
+constexpr int kHiShift = kStateShift + kLoBits;           // key >> kHiShift = the high digit
 static_assert(kMaxHi <= kPT && kLoBins <= kPT, "one thread per bucket or digit in the scans");
 
 // Exclusive scan of one value per thread over the workgroup (every thread calls it); wtot is LDS
@@ -226,17 +230,15 @@ __device__ __forceinline__ uint32_t block_excl_scan_u32(uint32_t v, uint32_t* wt
   return off + inc - v;
 }
 
-__device__ __forceinline__ uint32_t state_rank(const BwdIndex& x, int32_t v) {
-  return (uint32_t)x.state_erank[(v >= 1 && v <= x.S) ? v : 1];
-}
+__device__ __forceinline__ uint32_t plan_state(int64_t S, int32_t v) { return (v >= 1 && v <= S) ? (uint32_t)v : 1u; }
 
 __global__ void __launch_bounds__(1024) bwd_plan_init_kernel(int64_t* flags, uint32_t* hcount, int nh) {
   if (threadIdx.x == 0) flags[0] = 0;
   for (int i = threadIdx.x; i < nh; i += 1024) hcount[i] = 0;
 }
 
-__global__ void __launch_bounds__(kPT) bwd_count_kernel(const BwdIndex x, int64_t M, const int32_t* __restrict__ qstate,
-                                                        int nh, uint32_t* __restrict__ hcount) {
+__global__ void __launch_bounds__(kPT) bwd_count_kernel(int64_t S, int64_t M, const int32_t* __restrict__ qstate, int nh,
+                                                        uint32_t* __restrict__ hcount) {
   __shared__ uint32_t c[kMaxHi];
   const int tid = threadIdx.x;
   for (int i = tid; i < nh; i += kPT) c[i] = 0;
@@ -247,20 +249,17 @@ __global__ void __launch_bounds__(kPT) bwd_count_kernel(const BwdIndex x, int64_
     const int64_t i = t0 + k * kPT + tid;
     v[k] = i < M ? qstate[i] : 1;
   }
-  uint32_t e[kPE];
-#pragma unroll
-  for (int k = 0; k < kPE; ++k) e[k] = state_rank(x, v[k]);
   __syncthreads();  // counts are zero
 #pragma unroll
   for (int k = 0; k < kPE; ++k)
-    if (t0 + k * kPT + tid < M) atomicAdd(&c[e[k] >> kLoBits], 1u);
+    if (t0 + k * kPT + tid < M) atomicAdd(&c[plan_state(S, v[k]) >> kLoBits], 1u);
   __syncthreads();
   for (int h = tid; h < nh; h += kPT)
     if (c[h]) atomicAdd(hcount + h, c[h]);
 }
 
 // hbase[h] (and the cursor hcur[h]) = first position of bucket h, hbase[nh] = M; tstart[h] = first
-// second-level tile of bucket h, tstart[nh] = their number; grp[S] = M.
+// second-level tile of bucket h, tstart[nh] = their number; grp[S + 1] = M.
 __global__ void __launch_bounds__(kPT) bwd_layout_kernel(int64_t S, int64_t M, int nh, const uint32_t* __restrict__ hcount,
                                                          uint32_t* __restrict__ hbase, uint32_t* __restrict__ hcur,
                                                          uint32_t* __restrict__ tstart, uint32_t* __restrict__ grp) {
@@ -279,9 +278,12 @@ __global__ void __launch_bounds__(kPT) bwd_layout_kernel(int64_t S, int64_t M, i
   if (h == 0) {
     hbase[nh] = (uint32_t)M;
     tstart[nh] = ttot;
-    grp[S] = (uint32_t)M;
+    grp[S + 1] = (uint32_t)M;
   }
 }
+
+// The bucket pass's tiles: kBE queries per thread, staged keys and values both in LDS (156 KiB)
+constexpr int kBE = 12, kBTile = kPT * kBE;
 
 template <bool kVec>
 __global__ void __launch_bounds__(kPT) bwd_bucket_kernel(const BwdIndex x, int64_t M, const int32_t* __restrict__ qstate,
@@ -290,21 +292,22 @@ __global__ void __launch_bounds__(kPT) bwd_bucket_kernel(const BwdIndex x, int64
                                                          uint32_t* __restrict__ vals, int64_t* flags) {
   __shared__ uint32_t cnt[kMaxHi], lstart[kMaxHi], gbase[kMaxHi];
   __shared__ uint32_t wtot[kPT / 64];
-  __shared__ uint64_t stage[kPTile];  // the keys, then (as 32-bit words) the values
+  __shared__ uint64_t skey[kBTile];
+  __shared__ uint32_t sval[kBTile];
   const int tid = threadIdx.x;
-  const int64_t t0 = (int64_t)blockIdx.x * kPTile;
-  const int n = (int)min<int64_t>(kPTile, M - t0);
+  const int64_t t0 = (int64_t)blockIdx.x * kBTile;
+  const int n = (int)min<int64_t>(kBTile, M - t0);
   for (int i = tid; i < nh; i += kPT) cnt[i] = 0;
-  // the thread's k-th query: with kVec four consecutive ones per 16-byte load, in four rounds a
+  // the thread's k-th query: with kVec four consecutive ones per 16-byte load, in rounds a
   // workgroup of quadruples apart; otherwise one a workgroup apart
   auto qid = [&](int k) -> int64_t {
     return kVec ? t0 + (int64_t)(k >> 2) * (kPT * 4) + 4 * tid + (k & 3) : t0 + (int64_t)k * kPT + tid;
   };
-  int32_t v[kPE];
-  uint32_t lo[kPE], hi[kPE];
+  int32_t v[kBE];
+  uint32_t lo[kBE], hi[kBE];
   if (kVec) {
 #pragma unroll
-    for (int r = 0; r < kPE / 4; ++r) {
+    for (int r = 0; r < kBE / 4; ++r) {
       const int64_t b = qid(4 * r);
       if (b + 4 <= M) {
         const int4 a = *reinterpret_cast<const int4*>(qstate + b);
@@ -325,7 +328,7 @@ __global__ void __launch_bounds__(kPT) bwd_bucket_kernel(const BwdIndex x, int64
     }
   } else {
 #pragma unroll
-    for (int k = 0; k < kPE; ++k) {
+    for (int k = 0; k < kBE; ++k) {
       const int64_t i = qid(k);
       const bool in = i < M;
       v[k] = in ? qstate[i] : 1;
@@ -334,22 +337,22 @@ __global__ void __launch_bounds__(kPT) bwd_bucket_kernel(const BwdIndex x, int64
     }
   }
   bool bad = false;
-  uint64_t key[kPE];
-  uint32_t val[kPE];
+  uint64_t key[kBE];
+  uint32_t val[kBE];
 #pragma unroll
-  for (int k = 0; k < kPE; ++k) {
+  for (int k = 0; k < kBE; ++k) {
     uint32_t l = lo[k], h = hi[k];
     if (qid(k) < M && (v[k] < 1 || v[k] > x.S || l > x.NK || h > x.NK)) {
       bad = true;
       l = h = 0;
     }
-    key[k] = ((uint64_t)state_rank(x, v[k]) << kRankShift) | ((uint64_t)qid(k) << kIdShift) | (uint64_t)(l & kLoLowMask);
+    key[k] = ((uint64_t)plan_state(x.S, v[k]) << kStateShift) | ((uint64_t)qid(k) << kIdShift) | (uint64_t)(l & kLoLowMask);
     val[k] = h | ((l >> 17) << 21);
   }
   __syncthreads();  // counts are zero
-  uint32_t off[kPE];
+  uint32_t off[kBE];
 #pragma unroll
-  for (int k = 0; k < kPE; ++k) off[k] = qid(k) < M ? atomicAdd(&cnt[key[k] >> kHiShift], 1u) : 0u;
+  for (int k = 0; k < kBE; ++k) off[k] = qid(k) < M ? atomicAdd(&cnt[key[k] >> kHiShift], 1u) : 0u;
   __syncthreads();
   uint32_t tot;
   const uint32_t c = tid < nh ? cnt[tid] : 0u;
@@ -360,36 +363,26 @@ __global__ void __launch_bounds__(kPT) bwd_bucket_kernel(const BwdIndex x, int64
   }
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < kPE; ++k)
-    if (qid(k) < M) stage[lstart[key[k] >> kHiShift] + off[k]] = key[k];
-  __syncthreads();
-  uint32_t dst[kPE];
-  bool lost = false;
-#pragma unroll
-  for (int i = 0; i < kPE; ++i) {
-    const int j = tid + i * kPT;
-    dst[i] = 0xFFFFFFFFu;
-    if (j < n) {
-      const uint64_t kk = stage[j];
-      const int h = (int)(kk >> kHiShift);
-      const uint32_t d = gbase[h] + (uint32_t)(j - (int)lstart[h]);
-      if (d < (uint64_t)M) {
-        dst[i] = d;
-        keys[d] = kk;
-      } else {
-        lost = true;
-      }
+  for (int k = 0; k < kBE; ++k) {
+    if (qid(k) < M) {
+      const uint32_t j = lstart[key[k] >> kHiShift] + off[k];
+      skey[j] = key[k];
+      sval[j] = val[k];
     }
   }
-  __syncthreads();  // the keys are out of the stage
-  uint32_t* sv = reinterpret_cast<uint32_t*>(stage);
-#pragma unroll
-  for (int k = 0; k < kPE; ++k)
-    if (qid(k) < M) sv[lstart[key[k] >> kHiShift] + off[k]] = val[k];
   __syncthreads();
-#pragma unroll
-  for (int i = 0; i < kPE; ++i)
-    if (dst[i] != 0xFFFFFFFFu) vals[dst[i]] = sv[tid + i * kPT];
+  bool lost = false;
+  for (int j = tid; j < n; j += kPT) {
+    const uint64_t kk = skey[j];
+    const int h = (int)(kk >> kHiShift);
+    const uint32_t d = gbase[h] + (uint32_t)(j - (int)lstart[h]);
+    if (d < (uint64_t)M) {
+      keys[d] = kk;
+      vals[d] = sval[j];
+    } else {
+      lost = true;
+    }
+  }
   if (bad) flags[0] = 1;
   if (lost) flags[0] = 2;
 }
@@ -423,7 +416,7 @@ __global__ void __launch_bounds__(kPT) bwd_tile_count_kernel(int nh, const uint3
 #pragma unroll
   for (int k = 0; k < kPE; ++k) {
     const uint32_t p = p0 + k * kPT + tid;
-    e[k] = p < p1 ? (uint32_t)(keys[p] >> kRankShift) : 0xFFFFFFFFu;
+    e[k] = p < p1 ? (uint32_t)(keys[p] >> kStateShift) : 0xFFFFFFFFu;
   }
   __syncthreads();
 #pragma unroll
@@ -433,8 +426,8 @@ __global__ void __launch_bounds__(kPT) bwd_tile_count_kernel(int nh, const uint3
   if (tid < kLoBins) thist[(size_t)blockIdx.x * kLoBins + tid] = c[tid];
 }
 
-// Per bucket h, thread l: thist[g][l] becomes tile g's start inside rank h*kLoBins + l (the counts
-// of the bucket's earlier tiles), and grp[e] = the rank's first sorted position.
+// Per bucket h, thread l: thist[g][l] becomes tile g's start inside state h*kLoBins + l (the counts
+// of the bucket's earlier tiles), and grp[v] = the state's first sorted position.
 __global__ void __launch_bounds__(kLoBins) bwd_tile_offsets_kernel(int64_t S, const uint32_t* __restrict__ hbase,
                                                                    const uint32_t* __restrict__ tstart,
                                                                    uint32_t* __restrict__ thist, uint32_t* __restrict__ grp) {
@@ -451,10 +444,10 @@ __global__ void __launch_bounds__(kLoBins) bwd_tile_offsets_kernel(int64_t S, co
   uint32_t tot;
   const uint32_t base = block_excl_scan_u32(s, wtot, &tot);
   const int64_t e = (int64_t)h * kLoBins + l;
-  if (e < S) grp[e] = hbase[h] + base;
+  if (e <= S) grp[e] = hbase[h] + base;
 }
 
-__global__ void __launch_bounds__(kPT) bwd_rank_kernel(int64_t M, int nh, const uint32_t* __restrict__ hbase,
+__global__ void __launch_bounds__(kPT) bwd_state_kernel(int64_t M, int nh, const uint32_t* __restrict__ hbase,
                                                        const uint32_t* __restrict__ tstart, const uint32_t* __restrict__ thist,
                                                        const uint32_t* __restrict__ grp, const uint64_t* __restrict__ kin,
                                                        const uint32_t* __restrict__ vin, uint64_t* __restrict__ kout,
@@ -481,7 +474,7 @@ __global__ void __launch_bounds__(kPT) bwd_rank_kernel(int64_t M, int nh, const 
   uint32_t off[kPE];
 #pragma unroll
   for (int k = 0; k < kPE; ++k)
-    off[k] = k * kPT + tid < n ? atomicAdd(&cnt[(key[k] >> kRankShift) & (kLoBins - 1)], 1u) : 0u;
+    off[k] = k * kPT + tid < n ? atomicAdd(&cnt[(key[k] >> kStateShift) & (kLoBins - 1)], 1u) : 0u;
   __syncthreads();
   uint32_t tot;
   const uint32_t c = tid < kLoBins ? cnt[tid] : 0u;
@@ -493,7 +486,7 @@ __global__ void __launch_bounds__(kPT) bwd_rank_kernel(int64_t M, int nh, const 
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < kPE; ++k)
-    if (k * kPT + tid < n) stage[lstart[(key[k] >> kRankShift) & (kLoBins - 1)] + off[k]] = key[k];
+    if (k * kPT + tid < n) stage[lstart[(key[k] >> kStateShift) & (kLoBins - 1)] + off[k]] = key[k];
   __syncthreads();
   uint32_t dst[kPE];
   bool lost = false;
@@ -503,7 +496,7 @@ __global__ void __launch_bounds__(kPT) bwd_rank_kernel(int64_t M, int nh, const 
     dst[i] = 0xFFFFFFFFu;
     if (j < n) {
       const uint64_t kk = stage[j];
-      const int l = (int)((kk >> kRankShift) & (kLoBins - 1));
+      const int l = (int)((kk >> kStateShift) & (kLoBins - 1));
       const uint32_t d = gdst[l] + (uint32_t)(j - (int)lstart[l]);
       if (d < (uint64_t)M) {
         dst[i] = d;
@@ -517,12 +510,33 @@ __global__ void __launch_bounds__(kPT) bwd_rank_kernel(int64_t M, int nh, const 
   uint32_t* sv = reinterpret_cast<uint32_t*>(stage);
 #pragma unroll
   for (int k = 0; k < kPE; ++k)
-    if (k * kPT + tid < n) sv[lstart[(key[k] >> kRankShift) & (kLoBins - 1)] + off[k]] = val[k];
+    if (k * kPT + tid < n) sv[lstart[(key[k] >> kStateShift) & (kLoBins - 1)] + off[k]] = val[k];
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < kPE; ++i)
     if (dst[i] != 0xFFFFFFFFu) vout[dst[i]] = sv[tid + i * kPT];
   if (lost) flags[0] = 2;
+}
+
+// Per block, one workgroup: rg[e] = [first, end) of the sorted positions of the queries of the state
+// of rank e, for the block's ranks, and blast[b] = its last rank with queries (-1: none).
+__global__ void __launch_bounds__(256) bwd_rank_groups_kernel(const BwdIndex x, const uint32_t* __restrict__ grp,
+                                                              uint2* __restrict__ rg, int32_t* __restrict__ blast) {
+  __shared__ int last;
+  const int b = blockIdx.x;
+  const int e0 = x.blk_e0[b], e1 = x.blk_e0[b + 1];
+  if (threadIdx.x == 0) last = -1;
+  __syncthreads();
+  int mx = -1;
+  for (int e = e0 + (int)threadIdx.x; e < e1; e += 256) {
+    const int v = x.steps[x.erank_step[e]];
+    const uint2 r = make_uint2(grp[v], grp[v + 1]);
+    rg[e] = r;
+    if (r.y > r.x) mx = e;
+  }
+  if (mx >= 0) atomicMax(&last, mx);
+  __syncthreads();
+  if (threadIdx.x == 0) blast[b] = last;
 }
 
 // ---------------------------------------------------------------- persistent block replay
@@ -553,7 +567,7 @@ __host__ __device__ inline BlockLds block_lds_layout(int span, int nt, int rmax)
   L.wtot = al16(L.P + (size_t)span * 8);
   L.skey = al16(L.wtot + 16 * 8);
   L.gq = al16(L.skey + (size_t)rmax * 4);
-  L.sdel = al16(L.gq + (kGCap + 1) * 4);
+  L.sdel = al16(L.gq + 2 * kGCap * 4);
   L.srank = al16(L.sdel + (size_t)kEPT * nt * 8);
   L.sstep = al16(L.srank + (size_t)kEPT * nt * 2);
   L.total = al16(L.sstep + (size_t)kEPT * nt * 4);
@@ -564,7 +578,8 @@ __host__ __device__ inline BlockLds block_lds_layout(int span, int nt, int rmax)
 // per state; the unsort kernels below put them in query-id order); otherwise to out[query id].
 template <int RPT, bool kSorted>
 __device__ void dense_block(const BwdIndex& x, const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals,
-                            const uint32_t* __restrict__ grp, int64_t* __restrict__ out, const int b,
+                            const uint2* __restrict__ rg, const int32_t* __restrict__ blast,
+                            int64_t* __restrict__ out, const int b,
                             unsigned char* smem) {
   const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wv = tid >> 6;
   const int span = span_of<RPT>(nt);
@@ -573,7 +588,7 @@ __device__ void dense_block(const BwdIndex& x, const uint64_t* __restrict__ keys
   int64_t* P = reinterpret_cast<int64_t*>(smem + lay.P);          // exclusive prefix over ranks
   int64_t* wtot = reinterpret_cast<int64_t*>(smem + lay.wtot);    // 16 wavefront totals
   uint32_t* skey = reinterpret_cast<uint32_t*>(smem + lay.skey);  // the block's sorted keys
-  uint32_t* gq = reinterpret_cast<uint32_t*>(smem + lay.gq);      // kGCap + 1 group offsets
+  uint32_t* gq = reinterpret_cast<uint32_t*>(smem + lay.gq);      // kGCap group starts, then ends
   int64_t* sdel = reinterpret_cast<int64_t*>(smem + lay.sdel);    // staged signed deltas
   uint16_t* srank = reinterpret_cast<uint16_t*>(smem + lay.srank);  // staged ranks
   int32_t* sstep = reinterpret_cast<int32_t*>(smem + lay.sstep);    // staged step ids (one per step)
@@ -581,10 +596,9 @@ __device__ void dense_block(const BwdIndex& x, const uint64_t* __restrict__ keys
   const int s0 = __builtin_amdgcn_readfirstlane(x.blk_step0[b]);
   const int e0 = __builtin_amdgcn_readfirstlane(x.blk_e0[b]);
   const int e1 = __builtin_amdgcn_readfirstlane(x.blk_e0[b + 1]);
-  const uint32_t q0 = __builtin_amdgcn_readfirstlane(grp[e0]), q1 = __builtin_amdgcn_readfirstlane(grp[e1]);
-  if (q1 == q0) return;  // no query in this block: nothing to replay
-  // replay up to the enter step of the last queried state
-  const int el = (int)(uniform64((int64_t)(keys[q1 - 1] >> kRankShift)));
+  // replay up to the enter step of the last queried state (none: nothing to replay)
+  const int el = __builtin_amdgcn_readfirstlane(blast[b]);
+  if (el < 0) return;
   const int s1 = __builtin_amdgcn_readfirstlane(x.erank_step[el]) + 1;
   const int64_t k0 = uniform64(x.blk_key0[b]);
   const int R = (int)(uniform64(x.blk_key0[b + 1]) - k0);
@@ -593,9 +607,13 @@ __device__ void dense_block(const BwdIndex& x, const uint64_t* __restrict__ keys
 
   for (int i = tid; i < span; i += nt) w[i] = 0;
   for (int i = tid; i < R; i += nt) skey[i] = x.bkey[k0 + i];
-  int gwin = e0;  // gq[i] = grp[gwin + i]
+  int gwin = e0;  // gq[i], gq[kGCap + i] = the group of rank gwin + i
   auto stage_groups = [&](int g) {
-    for (int i = tid; i <= kGCap; i += nt) gq[i] = grp[min(g + i, e1)];
+    for (int i = tid; i < kGCap; i += nt) {
+      const uint2 r = rg[min(g + i, e1 - 1)];
+      gq[i] = r.x;
+      gq[kGCap + i] = r.y;
+    }
   };
   stage_groups(e0);
   int64_t tot[RPT];
@@ -634,7 +652,7 @@ __device__ void dense_block(const BwdIndex& x, const uint64_t* __restrict__ keys
       stage_groups(e);
       __syncthreads();
     }
-    const uint32_t qa = gq[e - gwin], qb = gq[e - gwin + 1];
+    const uint32_t qa = gq[e - gwin], qb = gq[kGCap + e - gwin];
     const int nq = (int)(qb - qa);
     if (nq == 0) return;
     // first round: the queries' items and carry-row reads are issued before the scan, whose
@@ -739,7 +757,8 @@ __device__ void dense_block(const BwdIndex& x, const uint64_t* __restrict__ keys
 
 template <int RPT, bool kSorted>
 __global__ void __launch_bounds__(1024) bwd_block_kernel(const BwdIndex x, const uint64_t* keys, const uint32_t* vals,
-                                                         const uint32_t* grp, int64_t* out, uint32_t* counter) {
+                                                         const uint2* rg, const int32_t* blast, int64_t* out,
+                                                         uint32_t* counter) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ int next;
   for (;;) {
@@ -748,7 +767,7 @@ __global__ void __launch_bounds__(1024) bwd_block_kernel(const BwdIndex x, const
     const int b = __builtin_amdgcn_readfirstlane(next);
     __syncthreads();  // every thread has read `next` before it is taken again
     if (b >= x.nblk) break;
-    dense_block<RPT, kSorted>(x, keys, vals, grp, out, b, smem_raw);
+    dense_block<RPT, kSorted>(x, keys, vals, rg, blast, out, b, smem_raw);
     __syncthreads();  // the next block reuses the LDS
   }
 }
@@ -896,15 +915,16 @@ CarryLayout carry_layout(const BwdIndex* x, int64_t nkeys) {
 
 // The planned batch: keys/vals (the bucket pass's output: queries grouped by high rank digit; then,
 // once the plan has run, the unsort's staging buffer: stage_ans = kin, stage_id = vin), kout/vout
-// (queries grouped by rank), grp, the bucket tables; sans holds the replay's answers in sorted order.
+// (queries grouped by state), grp, the bucket tables, per rank its state's group (rg) and per block
+// its last queried rank (blast); sans holds the replay's answers in sorted order.
 struct AnswerLayout {
-  size_t kin, kout, vin, vout, grp, counter, cursor, sans, hcount, hcur, hbase, tstart, thist, total;
+  size_t kin, kout, vin, vout, grp, rg, blast, counter, cursor, sans, hcount, hcur, hbase, tstart, thist, total;
   int nh;
   unsigned tiles;  // bound on the second level's tiles (its grid)
 };
 
 hipError_t answer_layout(const BwdIndex* x, int64_t M, AnswerLayout& L) {
-  L.nh = (int)(((x->S - 1) >> kLoBits) + 1);
+  L.nh = (int)((x->S >> kLoBits) + 1);
   L.tiles = (unsigned)((M + kPTile - 1) / kPTile + L.nh);
   size_t o = 0;
   auto take = [&](size_t n) {
@@ -916,7 +936,9 @@ hipError_t answer_layout(const BwdIndex* x, int64_t M, AnswerLayout& L) {
   L.kout = take((size_t)M * 8);
   L.vin = take((size_t)M * 4);
   L.vout = take((size_t)M * 4);
-  L.grp = take((size_t)(x->S + 1) * 4);
+  L.grp = take((size_t)(x->S + 2) * 4);
+  L.rg = take((size_t)x->S * 8);
+  L.blast = take((size_t)x->nblk * 4);
   L.counter = take(4);
   L.cursor = take((size_t)kMaxWin * 4);
   L.sans = take((size_t)M * 8);
@@ -1020,25 +1042,28 @@ extern "C" int bwd_plan_launch(const BwdIndex* x, int64_t M, const int32_t* qsta
   uint32_t* tstart = at<uint32_t>(work, L.tstart);
   uint32_t* thist = at<uint32_t>(work, L.thist);
   const int nh = L.nh;
-  const unsigned qgrid = (unsigned)((M + kPTile - 1) / kPTile);
+  const unsigned qgrid = (unsigned)((M + kPTile - 1) / kPTile), bgrid = (unsigned)((M + kBTile - 1) / kBTile);
   int rc;
   hipLaunchKernelGGL(bwd_plan_init_kernel, dim3(1), dim3(1024), 0, st, flags, hcount, nh);
-  hipLaunchKernelGGL(bwd_count_kernel, dim3(qgrid), dim3(kPT), 0, st, *x, M, qstate, nh, hcount);
+  hipLaunchKernelGGL(bwd_count_kernel, dim3(qgrid), dim3(kPT), 0, st, x->S, M, qstate, nh, hcount);
   hipLaunchKernelGGL(bwd_layout_kernel, dim3(1), dim3(kPT), 0, st, x->S, M, nh, hcount, hbase, hcur, tstart, grp);
   if ((rc = report("plan counts", hipGetLastError()))) return rc;
   // the 16-byte form when the caller's query arrays allow it
   if ((((uintptr_t)qstate | (uintptr_t)qlo | (uintptr_t)qhi) & 15) == 0)
-    hipLaunchKernelGGL(bwd_bucket_kernel<true>, dim3(qgrid), dim3(kPT), 0, st, *x, M, qstate, qlo, qhi, nh, hcur, kin,
+    hipLaunchKernelGGL(bwd_bucket_kernel<true>, dim3(bgrid), dim3(kPT), 0, st, *x, M, qstate, qlo, qhi, nh, hcur, kin,
                        vin, flags);
   else
-    hipLaunchKernelGGL(bwd_bucket_kernel<false>, dim3(qgrid), dim3(kPT), 0, st, *x, M, qstate, qlo, qhi, nh, hcur, kin,
+    hipLaunchKernelGGL(bwd_bucket_kernel<false>, dim3(bgrid), dim3(kPT), 0, st, *x, M, qstate, qlo, qhi, nh, hcur, kin,
                        vin, flags);
   if ((rc = report("plan buckets", hipGetLastError()))) return rc;
   hipLaunchKernelGGL(bwd_tile_count_kernel, dim3(L.tiles), dim3(kPT), 0, st, nh, hbase, tstart, kin, thist);
   hipLaunchKernelGGL(bwd_tile_offsets_kernel, dim3((unsigned)nh), dim3(kLoBins), 0, st, x->S, hbase, tstart, thist, grp);
-  hipLaunchKernelGGL(bwd_rank_kernel, dim3(L.tiles), dim3(kPT), 0, st, M, nh, hbase, tstart, thist, grp, kin, vin, kout,
+  hipLaunchKernelGGL(bwd_state_kernel, dim3(L.tiles), dim3(kPT), 0, st, M, nh, hbase, tstart, thist, grp, kin, vin, kout,
                      vout, flags);
-  return report("plan ranks", hipGetLastError());
+  if ((rc = report("plan states", hipGetLastError()))) return rc;
+  hipLaunchKernelGGL(bwd_rank_groups_kernel, dim3((unsigned)x->nblk), dim3(256), 0, st, *x, grp, at<uint2>(work, L.rg),
+                     at<int32_t>(work, L.blast));
+  return report("plan rank groups", hipGetLastError());
 }
 
 extern "C" int bwd_replay_launch(const BwdIndex* x, int64_t M, int64_t* answers, const BwdLaunch* l, void* work,
@@ -1076,10 +1101,13 @@ extern "C" int bwd_replay_launch(const BwdIndex* x, int64_t M, int64_t* answers,
   const dim3 grid((unsigned)nwg);
   const uint64_t* kout = at<uint64_t>(work, L.kout);
   const uint32_t* vout = at<uint32_t>(work, L.vout);
-  const uint32_t* grp = at<uint32_t>(work, L.grp);
+  const uint2* rg = at<uint2>(work, L.rg);
+  const int32_t* blast = at<int32_t>(work, L.blast);
   int64_t* out = staged ? at<int64_t>(work, L.sans) : answers;
   const size_t lds = block_lds_layout(spn, nt, x->rmax).total;
-  auto launch = [&](auto kernel) { hipLaunchKernelGGL(kernel, grid, dim3(nt), lds, st, *x, kout, vout, grp, out, counter); };
+  auto launch = [&](auto kernel) {
+    hipLaunchKernelGGL(kernel, grid, dim3(nt), lds, st, *x, kout, vout, rg, blast, out, counter);
+  };
   if (staged) {
     if (rpt == 4) launch(bwd_block_kernel<4, true>);
     else if (rpt == 8) launch(bwd_block_kernel<8, true>);

@@ -1,6 +1,6 @@
 /* bwgpu_dense.h -- C-ABI of the dense-carry batched form: a batch of exact int64 range-sum
- * queries over a branching version tree, answered by one radix sort of the queries and one
- * persistent block-replay kernel over a resident index whose carry is stored dense.
+ * queries over a branching version tree, answered by one bucket sort of the queries by state and
+ * one persistent block-replay kernel over a resident index whose carry is stored dense.
  *
  * Counterparts in the reference.  The v7 algorithm splits every answer into a block-local part
  * (lvals: block replay from a zero prefix, src/gpu_v7.cu:10-82) and the carry of the blocks
@@ -12,7 +12,7 @@
  * tree cut into ~5,600 blocks take ~47 GB and are built once.  A query then needs one read of
  * its block's row at each endpoint, no second carry level, no endpoint sort and no combine.
  * The per-query planning the reference does on the host (qlb/qhb, src/batch_engine_v7.hpp:
- * 666-698) is one device radix sort of the queries by the preorder rank of their state.
+ * 666-698) is one device bucket sort of the queries by their state.
  *
  * No HIP or torch types appear here: device pointers are plain pointers, a stream is an opaque
  * pointer (NULL = the default stream), every call returns 0 or a hipError_t value.  Nothing
@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-/* Limits of this form: a query's state rank (19 bits), id (27 bits) and endpoints (up to 2^20
+/* Limits of this form: a query's state (19 bits), id (27 bits) and endpoints (up to 2^20
  * each) are packed into one 64-bit sort key and one 32-bit value. */
 #define BWD_MAX_NK (1u << 20)
 #define BWD_MAX_STATES ((1 << 19) - 1)
@@ -88,14 +88,15 @@ typedef struct BwdLaunch {
   int32_t store;             /* BWD_STORE_* */
 } BwdLaunch;
 
-/* A planned batch of M queries: the queries sorted by (state rank, query id), with the group of
- * every state rank.  Lives in the answer workspace. */
+/* A planned batch of M queries: the queries grouped by state (any order inside a state), the group
+ * of the state of every preorder rank, and every block's last rank with queries.  Lives in the
+ * answer workspace. */
 int bwd_answer_workspace(const BwdIndex* x, int64_t M, size_t* bytes);
 
-/* Plan: sort M queries (qstate[M] in [1, S], qlo / qhi[M] in [0, NK]) into the workspace.  An
- * out-of-range query sets flags[0] to 1 and is planned as a query of state 1 over [0, 0)
- * (answer 0); flags is one int64 on the device, written by this call.  The sort issues device
- * memsets (rocPRIM), so this call is not for graph capture. */
+/* Plan: group M queries (qstate[M] in [1, S], qlo / qhi[M] in [0, NK]) by state into the
+ * workspace.  An out-of-range query sets flags[0] to 1 and is answered 0 (planned over [0, 0));
+ * flags is one int64 on the device, written by this call (2 would report an inconsistent plan,
+ * which the plan's own counts rule out).  Kernels only: capturable in a hipGraph. */
 int bwd_plan_launch(const BwdIndex* x, int64_t M, const int32_t* qstate, const uint32_t* qlo, const uint32_t* qhi,
                     int64_t* flags, void* work, size_t work_bytes, void* stream);
 /* Replay: one launch of the persistent block kernel answers the planned batch (answers[M], by
@@ -116,8 +117,8 @@ int bwd_answer_launch(const BwdIndex* x, int64_t M, const int32_t* qstate, const
  * A batch whose plan needs more scratch than that (about 1.8M queries over 300,000 states; see
  * bwd_answer_noalloc_capacity) is rejected with hipErrorInvalidValue and a message on stderr.
  * Every call on a GPU shares that GPU's array and leaves its contents behind: calls that overlap
- * on one GPU (two streams, two threads) race and must be serialised by the caller.  Like
- * bwd_plan_launch it issues device memsets (the radix sort's), so it is not for graph capture. */
+ * on one GPU (two streams, two threads) race and must be serialised by the caller.  Kernels only,
+ * like the workspace entry points. */
 int bwd_answer_noalloc_launch(const BwdIndex* x, int64_t M, const int32_t* qstate, const uint32_t* qlo,
                               const uint32_t* qhi, int64_t* answers, int64_t* flags, const BwdLaunch* l, void* stream);
 /* Scratch bytes a batch of M queries needs, and the bytes the allocation-free entry point holds. */
