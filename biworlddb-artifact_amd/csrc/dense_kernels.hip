@@ -784,12 +784,16 @@ __global__ void __launch_bounds__(1024) bwd_block_kernel(const BwdIndex x, const
 //   scatter   -- the staging buffer is cut into kDies contiguous parts, part d taken by the
 //                workgroups that run on die d (workgroup g runs on die g mod kDies), so each
 //                window's stores come from one die and its lines fill in that die's L2 before they
-//                are written back.
+//                are written back.  The launch reserves kScatLds bytes of LDS it does not use, so
+//                one workgroup runs per compute unit: with a quarter window in flight per die the
+//                partly written lines stay in its L2 until complete (measured over 102.5M answers:
+//                0.69 ms and 2.1 GB written, against 1.12 ms and 3.5 GB at full occupancy).
 constexpr int kWinBits = 18;
 constexpr int kMaxWin = (int)(BWD_MAX_QUERIES >> kWinBits);  // 512 windows cover every query id
 constexpr int kPartT = 1024, kPartE = 8, kPartTile = kPartT * kPartE;
 constexpr int kScatT = 256, kScatE = 8, kScatTile = kScatT * kScatE;
 constexpr int kDies = 8;
+constexpr int kScatLds = 100 * 1024;
 static_assert(kMaxWin <= kPartT && kMaxWin % 64 == 0, "one thread per window in the partition's scan");
 
 __global__ void __launch_bounds__(256) bwd_replay_init_kernel(uint32_t* counter, uint32_t* cursor, int nw) {
@@ -970,6 +974,9 @@ int set_lds_limits() {
       const hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, kBlockLdsLimit);
       if (e != hipSuccess && err == hipSuccess) err = e;
     }
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(bwd_scatter_kernel),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, kScatLds);
+    if (e != hipSuccess && err == hipSuccess) err = e;
   });
   return (int)err;
 }
@@ -1123,7 +1130,7 @@ extern "C" int bwd_replay_launch(const BwdIndex* x, int64_t M, int64_t* answers,
                      at<const int64_t>(work, L.sans), cursor, at<int64_t>(work, L.kin), at<uint32_t>(work, L.vin));
   if ((rc = report("answer partition", hipGetLastError()))) return rc;
   const int64_t per = ((M + kScatTile - 1) / kScatTile + kDies - 1) / kDies;
-  hipLaunchKernelGGL(bwd_scatter_kernel, dim3((unsigned)(per * kDies)), dim3(kScatT), 0, st, M, per,
+  hipLaunchKernelGGL(bwd_scatter_kernel, dim3((unsigned)(per * kDies)), dim3(kScatT), kScatLds, st, M, per,
                      at<const int64_t>(work, L.kin), at<const uint32_t>(work, L.vin), answers);
   return report("answer scatter", hipGetLastError());
 }
