@@ -5,13 +5,15 @@ Headline (`value`): the batched form, planning included.  A request stream of 10
 batches (sizes uniform in [1, 2048], 102,546,012 queries over uniform states, log-uniform ranges)
 over the 300,000-state tree with uneven chains (geometric runs, mean 64, max 4,096; 128 Zipf(1.1)
 point updates per state, N = 2^20 keys).  A "step" is one pass of the hot path over the whole
-stream, its queries already in HBM: the stream is planned on the device (one radix sort of the
-queries by the preorder rank of their state) and answered by ONE launch of a persistent
-block-replay kernel (captured in a hipGraph) whose workgroups take blocks from a device work
-queue, over the dense-carry index resident in HBM (one carry row of 2^20+1 int64 per block of up to
-4,096 keys, built once; its build time is reported beside).  bw_batch times each step with one
-pair of HIP events on its stream; the answers' SHA-256 must equal the two-level form's (which the
-test suite pins against the oracle).
+stream, its queries already in HBM: the stream is planned on the device (a two-level bucket sort
+of the queries by state, one hipGraph) and answered by ONE launch of a persistent block-replay
+kernel whose workgroups take blocks from a device work queue, over the dense-carry index resident
+in HBM (one carry row of 2^20+1 int64 per block of up to 4,096 keys, built once; its build time is
+reported beside); the replay stores the answers in sorted order and two streaming passes (a
+partition by query-id window, a die-local scatter) put them in query-id order, all in a second
+hipGraph.  bw_batch times each step with one pair of HIP events on its stream and checks every
+timed step's answers (device FNV-1a); the answers' SHA-256 must equal the two-level form's (which
+the test suite pins against the oracle).
 
 Beside it, measured in the same run (every figure planning-inclusive unless named otherwise):
   two_level -- the same stream over the two-level carry index (blocks of 1,024 keys, level-C rows,
@@ -33,13 +35,13 @@ Beside it, measured in the same run (every figure planning-inclusive unless name
                p50/p99 request latency, at several offered rates up to saturation; and the same
                request law over the config-3 tree with its rank-LUT resident (served_rank_lut);
   roofline  -- the compulsory bytes of a step (20 B per query: state, two endpoints, answer; 12 B
-               per update: key, delta) over the persistent replay's average HIP-event time, against
-               the MI355X HBM peak (8 TB/s); the whole step's figure beside it; `traffic` is the
-               rocprofv3 FETCH_SIZE + WRITE_SIZE of that kernel per launch (profiles/), scaled by the
-               same counters' reading for a copy of known size, when that summary exists; and the
-               replay's random-access floor: its 2 carry-row reads and 1 answer store per query at
-               the rates bw_floor measures on the device (random 8-byte gathers over 8 MiB rows,
-               random 8-byte stores), against the replay's time;
+               per update: key, delta) over the persistent block replay's average HIP-event time
+               (bwd_replay_timed: events around that kernel on its stream), against the MI355X HBM
+               peak (8 TB/s); the whole step's figure beside it; `traffic` is the rocprofv3
+               FETCH_SIZE + WRITE_SIZE of that kernel per launch (profiles/), scaled by the same
+               counters' reading for a copy of known size, when that summary exists; and the
+               kernel's random-access floor: its 2 carry-row reads per query at the rate bw_floor
+               measures on the device (random 8-byte gathers over 8 MiB rows), against its time;
   cpu_baseline -- the oracle's rollback-Fenwick stream solver (a port, not the reference engine:
                that engine cannot take branching multi-update trees) on the host's cores over the
                same whole stream; its answers' SHA-256 must equal the GPU's.
@@ -187,15 +189,18 @@ def main():
     M, S, U = r["n_answers"], r["S"], r["U"]
     alg_bytes = 20.0 * M + 12.0 * S * U  # compulsory bytes of one step
     step_ms, replay_ms = d["step_ms"]["avg"], d["replay_ms"]
-    achieved = alg_bytes / (replay_ms * 1e-3) / 1e9
+    phases = d["replay_phases_ms"]
+    block_ms = phases["block"]  # the dominant kernel, timed alone by HIP events on its stream
+    achieved = alg_bytes / (block_ms * 1e-3) / 1e9
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
             traffic = json.load(f).get("bwd_block_kernel")
-    # the replay's own floor: its two random streams (two carry-row reads per query, one answer store per
-    # query) at the device's measured random-access rates
+    # the block replay's own floor: its random stream (two carry-row reads per query) at the device's
+    # measured random-gather rate; the answers leave it in sorted order (coalesced) and reach their ids
+    # through the partition and the die-local scatter, timed beside it
     fl = run([BW_FLOOR, "--gathers", str(2 * M), "--scatters", str(M)], 300)
-    floor_ms = (2 * M / fl["gather"]["per_s"] + M / fl["scatter"]["per_s"]) * 1e3
+    floor_ms = 2 * M / fl["gather"]["per_s"] * 1e3
     tl = r["device_plan"]
     value = d["queries_per_s"]
     out = {
@@ -220,7 +225,8 @@ def main():
                    "dense_index": {"rmax": d["rmax"], "blocks": d["nblk"], "carry_bytes": d["carry_bytes"],
                                    "carry_build_ms": d["carry_build_ms"], "threads": d["threads"]}},
         "step": {"plan_ms": d["plan_ms"], "replay_ms": d["replay_ms"],
-                 "min_ms": d["step_ms"]["min"], "max_ms": d["step_ms"]["max"]},
+                 "replay_phases_ms": phases, "min_ms": d["step_ms"]["min"], "max_ms": d["step_ms"]["max"],
+                 "every_step_checked": d["step_mismatch"] == 0},
         "two_level": {"step_queries_per_s": tl["step_queries_per_s"], "step_ms": tl["step_ms"],
                       "plan_ms": tl["ms"], "replay_ms": r["graph_ms"]["avg"],
                       "replay_queries_per_s": round(r["queries_per_s"], 1), "stage_ms": r["stage_ms"],
@@ -254,18 +260,18 @@ def main():
                             "overlap_ms": plain["extract"]["overlap"], "same": plain["extract"]["same"]},
         "roofline": {"bound": "hbm", "kernel": "bwd_block_kernel", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "kernel_ms": round(replay_ms, 4), "kernel_bytes_alg": alg_bytes,
+                     "traffic": traffic, "kernel_ms": round(block_ms, 4), "kernel_bytes_alg": alg_bytes,
                      "bytes_alg_rule": "20 B per query (state, lo, hi, answer) + 12 B per update (key, delta)",
                      "step_achieved": round(alg_bytes / (step_ms * 1e-3) / 1e9, 1),
                      "step_frac": round(alg_bytes / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                      "traffic_over_alg": round(traffic / alg_bytes, 2) if traffic else None,
                      "random_access": {
-                         "rule": "2 random 8-byte carry-row reads + 1 random 8-byte answer store per query; on "
-                                 "gfx950 each such read fetches a 128-byte line",
+                         "rule": "2 random 8-byte carry-row reads per query; on gfx950 each such read fetches a "
+                                 "128-byte line",
                          "gathers": 2 * M, "gathers_per_s": round(fl["gather"]["per_s"], 1),
-                         "scatters": M, "scatters_per_s": round(fl["scatter"]["per_s"], 1),
-                         "floor_ms": round(floor_ms, 4), "replay_ms": round(replay_ms, 4),
-                         "floor_over_replay": round(floor_ms / replay_ms, 3),
+                         "floor_ms": round(floor_ms, 4), "kernel_ms": round(block_ms, 4),
+                         "floor_over_kernel": round(floor_ms / block_ms, 3),
+                         "random_scatters_per_s": round(fl["scatter"]["per_s"], 1),
                          "copy_GBps": fl["copy"]["GBps"]}},
         "device_index": {"total_ms": r["device_index"]["total_ms"], "device_ms": r["device_index"]["gpu_ms"],
                          "host_checks_ms": r["device_index"]["host_ms"],

@@ -1073,8 +1073,10 @@ extern "C" int bwd_plan_launch(const BwdIndex* x, int64_t M, const int32_t* qsta
   return report("plan rank groups", hipGetLastError());
 }
 
-extern "C" int bwd_replay_launch(const BwdIndex* x, int64_t M, int64_t* answers, const BwdLaunch* l, void* work,
-                                 size_t work_bytes, void* stream) {
+// The replay's launches; with ev (four events) they are recorded at the phase boundaries: before
+// the block replay, after it, after the partition, after the scatter.
+static int replay_impl(const BwdIndex* x, int64_t M, int64_t* answers, const BwdLaunch* l, void* work,
+                       size_t work_bytes, hipStream_t st, hipEvent_t* ev) {
   if (!index_ok(x) || M < 1 || M > BWD_MAX_QUERIES || !answers || !l || !work) return (int)hipErrorInvalidValue;
   const int nt = l->threads;
   if (nt != 256 && nt != 512 && nt != 1024) return (int)hipErrorInvalidValue;
@@ -1096,7 +1098,6 @@ extern "C" int bwd_replay_launch(const BwdIndex* x, int64_t M, int64_t* answers,
   int dev = 0, ncu = 0;
   if ((e = hipGetDevice(&dev)) || (e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev)))
     return report("device query", e);
-  const hipStream_t st = (hipStream_t)stream;
   // staged stores pay off once the ids span more than one window
   const bool staged = l->store == BWD_STORE_STAGED || (l->store == BWD_STORE_AUTO && M > (int64_t(1) << kWinBits));
   const int nwin = (int)((M + (int64_t(1) << kWinBits) - 1) >> kWinBits);
@@ -1112,6 +1113,8 @@ extern "C" int bwd_replay_launch(const BwdIndex* x, int64_t M, int64_t* answers,
   const int32_t* blast = at<int32_t>(work, L.blast);
   int64_t* out = staged ? at<int64_t>(work, L.sans) : answers;
   const size_t lds = block_lds_layout(spn, nt, x->rmax).total;
+  auto mark = [&](int i) { return ev ? hipEventRecord(ev[i], st) : hipSuccess; };
+  if ((e = mark(0))) return report("replay event", e);
   auto launch = [&](auto kernel) {
     hipLaunchKernelGGL(kernel, grid, dim3(nt), lds, st, *x, kout, vout, rg, blast, out, counter);
   };
@@ -1125,18 +1128,45 @@ extern "C" int bwd_replay_launch(const BwdIndex* x, int64_t M, int64_t* answers,
     else launch(bwd_block_kernel<16, false>);
   }
   if ((rc = report("dense block replay", hipGetLastError()))) return rc;
-  if (!staged) return 0;
+  if ((e = mark(1))) return report("replay event", e);
+  if (!staged) {
+    if ((e = mark(2)) || (e = mark(3))) return report("replay event", e);
+    return 0;
+  }
   hipLaunchKernelGGL(bwd_part_kernel, dim3((unsigned)((M + kPartTile - 1) / kPartTile)), dim3(kPartT), 0, st, M, kout,
                      at<const int64_t>(work, L.sans), cursor, at<int64_t>(work, L.kin), at<uint32_t>(work, L.vin));
   if ((rc = report("answer partition", hipGetLastError()))) return rc;
+  if ((e = mark(2))) return report("replay event", e);
   const int64_t per = ((M + kScatTile - 1) / kScatTile + kDies - 1) / kDies;
   hipLaunchKernelGGL(bwd_scatter_kernel, dim3((unsigned)(per * kDies)), dim3(kScatT), kScatLds, st, M, per,
                      at<const int64_t>(work, L.kin), at<const uint32_t>(work, L.vin), answers);
-  return report("answer scatter", hipGetLastError());
+  if ((rc = report("answer scatter", hipGetLastError()))) return rc;
+  if ((e = mark(3))) return report("replay event", e);
+  return 0;
+}
+
+extern "C" int bwd_replay_launch(const BwdIndex* x, int64_t M, int64_t* answers, const BwdLaunch* l, void* work,
+                                 size_t work_bytes, void* stream) {
+  return replay_impl(x, M, answers, l, work, work_bytes, (hipStream_t)stream, nullptr);
+}
+
+extern "C" int bwd_replay_timed(const BwdIndex* x, int64_t M, int64_t* answers, const BwdLaunch* l, void* work,
+                                size_t work_bytes, void* stream, float* ms) {
+  if (!ms) return (int)hipErrorInvalidValue;
+  hipEvent_t ev[4] = {};
+  hipError_t e = hipSuccess;
+  for (int i = 0; i < 4 && e == hipSuccess; ++i) e = hipEventCreate(&ev[i]);
+  int rc = e ? report("replay events", e) : replay_impl(x, M, answers, l, work, work_bytes, (hipStream_t)stream, ev);
+  if (!rc && (e = hipEventSynchronize(ev[3]))) rc = report("replay", e);
+  for (int i = 0; i < 3 && !rc; ++i)
+    if ((e = hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]))) rc = report("replay event time", e);
+  for (int i = 0; i < 4; ++i)
+    if (ev[i]) (void)hipEventDestroy(ev[i]);
+  return rc;
 }
 
 // the current device's address of `scratch` (looked up on the first call on each device)
-hipError_t scratch_address(void** out) {
+static hipError_t scratch_address(void** out) {
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e) return e;

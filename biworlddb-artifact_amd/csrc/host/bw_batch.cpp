@@ -51,6 +51,7 @@
 #include "bwgpu_extract.h"
 #include "bwgpu_tree.h"
 #include "dense_engine.hpp"
+#include "fnv_split.hpp"
 #include "tree_engine.hpp"
 #include "tree_gen.hpp"
 #include "tree_plan.hpp"
@@ -499,8 +500,14 @@ int main(int argc, char** argv) {
     int64_t flag = 0;
     HIPX(hipMemcpy(&flag, DBn.flags, sizeof(flag), hipMemcpyDeviceToHost));
     if (flag) { std::fprintf(stderr, "bw_batch: dense plan flagged out-of-range queries\n"); return 3; }
-    hipGraph_t dg;
-    hipGraphExec_t dgx;
+    // a step is two graph launches: the plan's kernels, then the replay's (block replay and, for
+    // staged stores, the answers' partition and scatter); both captured once
+    hipGraph_t pg, dg;
+    hipGraphExec_t pgx, dgx;
+    HIPX(hipStreamBeginCapture(st, hipStreamCaptureModeGlobal));
+    RUN(plan_d());
+    HIPX(hipStreamEndCapture(st, &pg));
+    HIPX(hipGraphInstantiate(&pgx, pg, nullptr, nullptr, 0));
     HIPX(hipStreamBeginCapture(st, hipStreamCaptureModeGlobal));
     RUN(bwd_replay_launch(&D.x, M, d_ans, &D.launch, DBn.work, DBn.work_bytes, st));
     HIPX(hipStreamEndCapture(st, &dg));
@@ -508,14 +515,23 @@ int main(int argc, char** argv) {
     hipEvent_t em;
     HIPX(hipEventCreate(&em));
     for (int i = 0; i < warmup; ++i) {
-      RUN(plan_d());
+      HIPX(hipGraphLaunch(pgx, st));
       HIPX(hipGraphLaunch(dgx, st));
     }
     HIPX(hipStreamSynchronize(st));
+    // every timed step's answers are checked afterwards by their device FNV-1a (outside the clock)
+    size_t fnv_wb = 0;
+    if (bw_fnv1a_workspace(size_t(M) * 8, &fnv_wb)) return 5;
+    void* fnv_work = DBn.arena.alloc(fnv_wb);
+    uint64_t* d_fnv = DBn.arena.scratch<uint64_t>(size_t(repeat));
+    if (!fnv_work || !d_fnv) {
+      std::fprintf(stderr, "bw_batch: device: FNV workspace\n");
+      return 5;
+    }
     std::vector<double> step_ms, dplan_v, drep_v;
     for (int i = 0; i < repeat; ++i) {
       HIPX(hipEventRecord(e0, st));
-      RUN(plan_d());
+      HIPX(hipGraphLaunch(pgx, st));
       HIPX(hipEventRecord(em, st));
       HIPX(hipGraphLaunch(dgx, st));
       HIPX(hipEventRecord(e1, st));
@@ -526,12 +542,29 @@ int main(int argc, char** argv) {
       step_ms.push_back(t);
       dplan_v.push_back(tp);
       drep_v.push_back(double(t) - double(tp));
+      RUN(bw_fnv1a_device(d_ans, size_t(M) * 8, kFnvStart, fnv_work, fnv_wb, d_fnv + i, st));
     }
+    std::vector<uint64_t> step_fnv(static_cast<size_t>(repeat));
+    HIPX(hipMemcpy(step_fnv.data(), d_fnv, sizeof(uint64_t) * size_t(repeat), hipMemcpyDeviceToHost));
+    int64_t flag_after = 0;
+    HIPX(hipMemcpy(&flag_after, DBn.flags, sizeof(flag_after), hipMemcpyDeviceToHost));
     BwExtractResult dxr{};
     const int dxrc = bw_extract_answers(d_ans, size_t(M), st, nullptr, nullptr, BW_EXTRACT_FNV_ON_DEVICE, &dxr);
     if (dxrc) return dxrc < 0 ? 4 : 5;
     dense_mismatch = sha == dxr.sha256_hex ? 0 : 1;
     if (dense_mismatch) std::fprintf(stderr, "bw_batch: dense-carry answers differ from the two-level form's\n");
+    int64_t step_mismatch = flag_after != 0;
+    for (uint64_t f : step_fnv) step_mismatch += f != dxr.fnv1a;
+    if (step_mismatch) std::fprintf(stderr, "bw_batch: %" PRId64 " timed dense steps' answers differ\n", step_mismatch);
+    dense_mismatch += step_mismatch;
+    // the replay's phases (block replay, answer partition, answer scatter), timed apart: the mean
+    // over `repeat` replays of the planned stream
+    double phase_ms[3] = {0, 0, 0};
+    for (int i = 0; i < repeat; ++i) {
+      float ph[3] = {0, 0, 0};
+      RUN(bwd_replay_timed(&D.x, M, d_ans, &D.launch, DBn.work, DBn.work_bytes, st, ph));
+      for (int k = 0; k < 3; ++k) phase_ms[k] += ph[k] / repeat;
+    }
     // the stream shifted by one query (every query array 4 bytes past a 16-byte boundary, so the
     // plan packs keys with its unaligned form): its answers are the whole stream's, shifted by one
     int64_t offset_mismatch = 0;
@@ -639,14 +672,17 @@ int main(int argc, char** argv) {
     std::snprintf(dense_json, sizeof(dense_json),
                   "{\"rmax\":%d,\"nblk\":%d,\"threads\":%d,\"wg_per_cu\":%d,\"store\":%d,\"carry_bytes\":%zu,\"carry_build_ms\":%.3f,"
                   "\"index_wall_ms\":%.3f,\"step_ms\":{\"avg\":%.4f,\"min\":%.4f,\"max\":%.4f,\"repeat\":%d},"
-                  "\"plan_ms\":%.4f,\"replay_ms\":%.4f,\"queries_per_s\":%.1f,\"answers_sha256\":\"%s\","
-                  "\"offset_mismatch\":%" PRId64 ",\"overlap\":%s,\"mismatch\":%" PRId64 "}",
+                  "\"plan_ms\":%.4f,\"replay_ms\":%.4f,\"replay_phases_ms\":{\"block\":%.4f,\"partition\":%.4f,"
+                  "\"scatter\":%.4f},\"queries_per_s\":%.1f,\"answers_sha256\":\"%s\","
+                  "\"offset_mismatch\":%" PRId64 ",\"step_mismatch\":%" PRId64 ",\"overlap\":%s,\"mismatch\":%" PRId64 "}",
                   dense_rmax, E4.x.nblk, dense_threads, dense_wg, dense_store, D.carry_bytes, D.build_ms, dense_index_wall, sm,
                   *std::min_element(step_ms.begin(), step_ms.end()), *std::max_element(step_ms.begin(), step_ms.end()),
-                  repeat, mean(dplan_v), mean(drep_v), double(M) / (sm * 1e-3), dxr.sha256_hex, offset_mismatch,
-                  overlap_json, dense_mismatch);
+                  repeat, mean(dplan_v), mean(drep_v), phase_ms[0], phase_ms[1], phase_ms[2], double(M) / (sm * 1e-3),
+                  dxr.sha256_hex, offset_mismatch, step_mismatch, overlap_json, dense_mismatch);
     HIPX(hipGraphExecDestroy(dgx));
     HIPX(hipGraphDestroy(dg));
+    HIPX(hipGraphExecDestroy(pgx));
+    HIPX(hipGraphDestroy(pg));
     HIPX(hipEventDestroy(em));
     // --noalloc-check: the allocation-free entry point (library-held scratch) against the
     // workspace entry point on two prefixes of the stream in a row, the second smaller; both must

@@ -104,6 +104,12 @@ int bwd_plan_launch(const BwdIndex* x, int64_t M, const int32_t* qstate, const u
  * of uneven length and query count share the device.  Kernels only (capturable in a hipGraph). */
 int bwd_replay_launch(const BwdIndex* x, int64_t M, int64_t* answers, const BwdLaunch* l, void* work,
                       size_t work_bytes, void* stream);
+/* Measurement: bwd_replay_launch once on `stream` with HIP events at its phase boundaries; waits
+ * for it and writes the phases' times in ms: ms[0] the persistent block replay, ms[1] the answer
+ * partition, ms[2] the answer scatter (both 0 when the answers are stored by id directly).  Not for
+ * graph capture (it synchronises). */
+int bwd_replay_timed(const BwdIndex* x, int64_t M, int64_t* answers, const BwdLaunch* l, void* work, size_t work_bytes,
+                     void* stream, float* ms);
 /* Both. */
 int bwd_answer_launch(const BwdIndex* x, int64_t M, const int32_t* qstate, const uint32_t* qlo, const uint32_t* qhi,
                       int64_t* answers, int64_t* flags, const BwdLaunch* l, void* work, size_t work_bytes,
