@@ -1,10 +1,15 @@
 // dense_emulator.cpp -- test infrastructure: a CPU emulation of the dense-carry form
 // (biworlddb-artifact_amd/csrc/dense_kernels.hip) over the product's host index plan and rank
-// tables.  It builds every block's net weight per distinct key, the dense carry rows, packs each
-// query into the plan's 64-bit sort key and 32-bit value exactly as the key kernel does, sorts by
-// the key, derives the per-rank groups, and replays every block as the persistent kernel does --
-// the queries' endpoints unpacked from the sorted key and value -- then prints the answers'
-// SHA-256 in (batch, query) order.
+// tables.  It builds every block's net weight per distinct key, the dense carry rows, and follows
+// the plan's kernels tile by tile: the high-digit counts, the bucket layout, the bucket pass (each
+// query packed into the 64-bit key and 32-bit value, runs appended at the bucket's cursor), the
+// per-tile low-digit counts and offsets, the state pass, the per-block rank groups -- every
+// position checked to be written once and inside [0, M) -- then replays every block as the
+// persistent kernel does (the queries' endpoints unpacked from the grouped key and value, answers
+// stored in grouped order), puts the answers in query-id order by the partition by id window and
+// the scatter, and prints their SHA-256 in (batch, query) order.  Where a kernel's workgroup places
+// items by LDS or global atomics, the emulation takes the items in order: any order gives the same
+// positions' contents up to a permutation inside a state or a window, which the answers do not see.
 //
 //   dense_emulator --S n --U n --N n [--seed n] --requests n --max-batch n --rmax n [--answers F]
 #include <openssl/sha.h>
@@ -31,10 +36,15 @@ using namespace bwg;
     }                                                                    \
   } while (0)
 
-// the kernels' layout: key = rank << 44 | id << 17 | lo & 0x1FFFF; val = hi | (lo >> 17) << 21
-constexpr int kRankShift = 44, kIdShift = 17;
+// the kernels' layout: key = state << 44 | id << 17 | lo & 0x1FFFF; val = hi | (lo >> 17) << 21
+constexpr int kStateShift = 44, kIdShift = 17;
 constexpr uint64_t kIdMask = (uint64_t(1) << 27) - 1;
 constexpr uint32_t kLoLowMask = (1u << 17) - 1, kHiMask = (1u << 21) - 1;
+// the plan's digits and tiles, and the unsort's windows and tiles
+constexpr int kLoBits = 9, kLoBins = 1 << kLoBits;
+constexpr int64_t kCountTile = 16384, kBucketTile = 12288, kStateTile = 16384;
+constexpr int kWinBits = 18;
+constexpr int64_t kPartTile = 8192;
 
 int main(int argc, char** argv) {
   TreeSpec spec = tree_defaults(4);
@@ -100,40 +110,122 @@ int main(int argc, char** argv) {
     for (int64_t r = P.blk_key0[size_t(b)]; r < P.blk_key0[size_t(b) + 1]; ++r) wk[P.bkey[size_t(r)]] += agg[size_t(r)];
   }
 
-  // the plan: packed keys, sorted, and the first position of every rank
-  std::vector<uint64_t> key(static_cast<size_t>(M));
-  std::vector<uint32_t> val(static_cast<size_t>(M));
-  for (int64_t q = 0; q < M; ++q) {
-    const int32_t v = rs.qstate[size_t(q)];
-    const uint32_t lo = rs.qlo[size_t(q)], hi = rs.qhi[size_t(q)];
-    CHECK(v >= 1 && v <= S && lo <= NK && hi <= NK);
-    key[size_t(q)] = (uint64_t(uint32_t(erank[size_t(v)])) << kRankShift) | (uint64_t(q) << kIdShift) |
-                     uint64_t(lo & kLoLowMask);
-    val[size_t(q)] = hi | ((lo >> 17) << 21);
+  // the plan.  count: high-digit counts (the count kernel's tiles only split the work)
+  const int nh = int((S >> kLoBits) + 1);
+  std::vector<int64_t> hcount(static_cast<size_t>(nh), 0);
+  for (int64_t t0 = 0; t0 < M; t0 += kCountTile)
+    for (int64_t q = t0; q < std::min(M, t0 + kCountTile); ++q) ++hcount[size_t(rs.qstate[size_t(q)] >> kLoBits)];
+  // layout: bucket starts (cursors) and second-level tiles per bucket
+  std::vector<int64_t> hbase(static_cast<size_t>(nh) + 1, 0), tstart(static_cast<size_t>(nh) + 1, 0);
+  for (int h = 0; h < nh; ++h) {
+    hbase[size_t(h) + 1] = hbase[size_t(h)] + hcount[size_t(h)];
+    tstart[size_t(h) + 1] = tstart[size_t(h)] + (hcount[size_t(h)] + kStateTile - 1) / kStateTile;
   }
-  std::vector<uint32_t> ord(static_cast<size_t>(M));
-  std::iota(ord.begin(), ord.end(), 0u);
-  std::sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) { return key[a] < key[b]; });
+  CHECK(hbase[size_t(nh)] == M);
+  std::vector<int64_t> hcur(hbase.begin(), hbase.end() - 1);
+  // bucket: per tile, keys packed, grouped by high digit, each bucket's run appended at its cursor
+  std::vector<uint64_t> bkeyv(static_cast<size_t>(M));
+  std::vector<uint32_t> bval(static_cast<size_t>(M));
+  std::vector<uint8_t> wrote(static_cast<size_t>(M), 0);
+  for (int64_t t0 = 0; t0 < M; t0 += kBucketTile) {
+    const int64_t t1 = std::min(M, t0 + kBucketTile);
+    std::vector<std::vector<int64_t>> run(static_cast<size_t>(nh));
+    for (int64_t q = t0; q < t1; ++q) run[size_t(rs.qstate[size_t(q)] >> kLoBits)].push_back(q);
+    for (int h = 0; h < nh; ++h) {
+      for (int64_t q : run[size_t(h)]) {
+        const int32_t v = rs.qstate[size_t(q)];
+        const uint32_t lo = rs.qlo[size_t(q)], hi = rs.qhi[size_t(q)];
+        CHECK(v >= 1 && v <= S && lo <= NK && hi <= NK);
+        const int64_t d = hcur[size_t(h)]++;
+        CHECK(d >= hbase[size_t(h)] && d < hbase[size_t(h) + 1] && !wrote[size_t(d)]);
+        wrote[size_t(d)] = 1;
+        bkeyv[size_t(d)] = (uint64_t(uint32_t(v)) << kStateShift) | (uint64_t(q) << kIdShift) | uint64_t(lo & kLoLowMask);
+        bval[size_t(d)] = hi | ((lo >> 17) << 21);
+      }
+    }
+  }
+  // tile counts and offsets: per bucket's tile, low-digit counts; each tile's start inside every
+  // state of the bucket; grp[v] = the state's first position
+  const int64_t ntiles = tstart[size_t(nh)];
+  std::vector<int64_t> thist(static_cast<size_t>(ntiles) * kLoBins, 0);
+  auto tile_range = [&](int64_t g, int& h, int64_t& p0, int64_t& p1) {
+    h = int(std::upper_bound(tstart.begin(), tstart.end() - 1, g) - tstart.begin()) - 1;
+    p0 = hbase[size_t(h)] + (g - tstart[size_t(h)]) * kStateTile;
+    p1 = std::min(p0 + kStateTile, hbase[size_t(h) + 1]);
+  };
+  for (int64_t g = 0; g < ntiles; ++g) {
+    int h;
+    int64_t p0, p1;
+    tile_range(g, h, p0, p1);
+    CHECK(p0 < p1);
+    for (int64_t p = p0; p < p1; ++p) {
+      const uint32_t v = uint32_t(bkeyv[size_t(p)] >> kStateShift);
+      CHECK(int(v >> kLoBits) == h);
+      ++thist[size_t(g) * kLoBins + (v & (kLoBins - 1))];
+    }
+  }
+  std::vector<int64_t> grp(static_cast<size_t>(S) + 2, 0);
+  for (int h = 0; h < nh; ++h) {
+    std::vector<int64_t> tot(kLoBins, 0);
+    for (int64_t g = tstart[size_t(h)]; g < tstart[size_t(h) + 1]; ++g)
+      for (int l = 0; l < kLoBins; ++l) {
+        const int64_t c = thist[size_t(g) * kLoBins + l];
+        thist[size_t(g) * kLoBins + l] = tot[size_t(l)];
+        tot[size_t(l)] += c;
+      }
+    int64_t base = 0;
+    for (int l = 0; l < kLoBins; ++l) {
+      const int64_t v = int64_t(h) * kLoBins + l;
+      if (v <= S) grp[size_t(v)] = hbase[size_t(h)] + base;
+      base += tot[size_t(l)];
+    }
+  }
+  grp[size_t(S) + 1] = M;
+  // state: per tile, grouped by low digit, each state's run at grp[v] + the tile's start
   std::vector<uint64_t> skey(static_cast<size_t>(M));
   std::vector<uint32_t> sval(static_cast<size_t>(M));
-  for (int64_t j = 0; j < M; ++j) {
-    skey[size_t(j)] = key[ord[size_t(j)]];
-    sval[size_t(j)] = val[ord[size_t(j)]];
+  std::fill(wrote.begin(), wrote.end(), 0);
+  for (int64_t g = 0; g < ntiles; ++g) {
+    int h;
+    int64_t p0, p1;
+    tile_range(g, h, p0, p1);
+    std::vector<int64_t> local(kLoBins, 0);
+    for (int64_t p = p0; p < p1; ++p) {
+      const uint32_t v = uint32_t(bkeyv[size_t(p)] >> kStateShift);
+      const int l = int(v & (kLoBins - 1));
+      const int64_t d = grp[size_t(v)] + thist[size_t(g) * kLoBins + l] + local[size_t(l)]++;
+      CHECK(d >= grp[size_t(v)] && d < grp[size_t(v) + 1] && d < M && !wrote[size_t(d)]);
+      wrote[size_t(d)] = 1;
+      skey[size_t(d)] = bkeyv[size_t(p)];
+      sval[size_t(d)] = bval[size_t(p)];
+    }
   }
-  std::vector<int64_t> grp(static_cast<size_t>(S) + 1, M);
-  for (int64_t j = M - 1; j >= 0; --j) grp[size_t(skey[size_t(j)] >> kRankShift)] = j;
-  for (int64_t e = S - 1; e >= 0; --e) grp[size_t(e)] = std::min(grp[size_t(e)], grp[size_t(e) + 1]);
+  for (int64_t j = 0; j < M; ++j) CHECK(wrote[size_t(j)]);
+  // rank groups: per block, the group of the state of each of its ranks, and its last queried rank
+  std::vector<int64_t> rga(static_cast<size_t>(S)), rgb(static_cast<size_t>(S));
+  std::vector<int32_t> blast(static_cast<size_t>(nblk), -1);
+  for (int b = 0; b < nblk; ++b)
+    for (int32_t e = be0[size_t(b)]; e < be0[size_t(b) + 1]; ++e) {
+      const int32_t v = P.steps[size_t(estep[size_t(e)])];
+      CHECK(v >= 1 && v <= S);
+      rga[size_t(e)] = grp[size_t(v)];
+      rgb[size_t(e)] = grp[size_t(v) + 1];
+      if (rgb[size_t(e)] > rga[size_t(e)]) blast[size_t(b)] = e;
+    }
 
-  // the replay: per block, events into per-rank totals; at each entered state its queries
-  std::vector<int64_t> ans(static_cast<size_t>(M), 0);
+  // the replay: per block up to its last queried rank, events into per-rank totals; at each entered
+  // state its queries, answers stored at their grouped positions
+  std::vector<int64_t> sans(static_cast<size_t>(M), 0);
   std::vector<uint8_t> done(static_cast<size_t>(M), 0);
   for (int b = 0; b < nblk; ++b) {
+    if (blast[size_t(b)] < 0) continue;
     const int64_t k0 = P.blk_key0[size_t(b)];
     const int R = int(P.blk_key0[size_t(b) + 1] - k0);
     const uint32_t* bk = P.bkey.data() + k0;
     std::vector<int64_t> tot(static_cast<size_t>(R), 0);
     int32_t e = be0[size_t(b)];
-    for (int64_t s = P.blk_step0[size_t(b)]; s < P.blk_step0[size_t(b) + 1]; ++s) {
+    const int64_t s1 = int64_t(estep[size_t(blast[size_t(b)])]) + 1;
+    for (int64_t s = P.blk_step0[size_t(b)]; s < s1; ++s) {
       const int32_t v = P.steps[size_t(s)];
       const int64_t a = (v > 0 ? v : -v) - 1;
       for (int i = 0; i < U; ++i) {
@@ -142,24 +234,51 @@ int main(int argc, char** argv) {
       }
       if (v <= 0) continue;
       CHECK(estep[size_t(e)] == s && erank[size_t(v)] == e);
-      for (int64_t j = grp[size_t(e)]; j < grp[size_t(e) + 1]; ++j) {
+      for (int64_t j = rga[size_t(e)]; j < rgb[size_t(e)]; ++j) {
         const uint64_t kk = skey[size_t(j)];
         const uint32_t vv = sval[size_t(j)];
         const uint32_t qid = uint32_t((kk >> kIdShift) & kIdMask);
         const uint32_t lo = (uint32_t(kk) & kLoLowMask) | ((vv >> 21) << 17), hi = vv & kHiMask;
-        CHECK(qid < M && !done[qid] && rs.qlo[qid] == lo && rs.qhi[qid] == hi);
+        CHECK(qid < M && !done[qid] && rs.qstate[qid] == v && rs.qlo[qid] == lo && rs.qhi[qid] == hi);
         const int rl = int(std::lower_bound(bk, bk + R, lo) - bk), rh = int(std::lower_bound(bk, bk + R, hi) - bk);
         int64_t pl = 0, ph = 0;  // in-block prefixes at the endpoints' ranks
         for (int r = 0; r < rl; ++r) pl += tot[size_t(r)];
         for (int r = 0; r < rh; ++r) ph += tot[size_t(r)];
         const int64_t* row = carry.data() + size_t(b) * ld;
-        ans[qid] = (row[hi] - row[lo]) + (ph - pl);
+        sans[size_t(j)] = (row[hi] - row[lo]) + (ph - pl);
         done[qid] = 1;
       }
       ++e;
     }
   }
   for (int64_t q = 0; q < M; ++q) CHECK(done[size_t(q)]);
+  // the unsort: per tile of grouped positions, runs per id window appended at the window's cursor
+  // (window w's region starts at w << kWinBits); then every staged answer stored at its id
+  const int64_t nwin = (M + (int64_t(1) << kWinBits) - 1) >> kWinBits;
+  std::vector<int64_t> cursor(static_cast<size_t>(nwin), 0), stage_ans(static_cast<size_t>(M));
+  std::vector<uint32_t> stage_id(static_cast<size_t>(M));
+  std::fill(wrote.begin(), wrote.end(), 0);
+  for (int64_t t0 = 0; t0 < M; t0 += kPartTile) {
+    const int64_t t1 = std::min(M, t0 + kPartTile);
+    std::vector<std::vector<int64_t>> run(static_cast<size_t>(nwin));
+    for (int64_t j = t0; j < t1; ++j) run[size_t(((skey[size_t(j)] >> kIdShift) & kIdMask) >> kWinBits)].push_back(j);
+    for (int64_t wv = 0; wv < nwin; ++wv)
+      for (int64_t j : run[size_t(wv)]) {
+        const int64_t gpos = (wv << kWinBits) + cursor[size_t(wv)]++;
+        CHECK(gpos < M && !wrote[size_t(gpos)]);
+        wrote[size_t(gpos)] = 1;
+        stage_ans[size_t(gpos)] = sans[size_t(j)];
+        stage_id[size_t(gpos)] = uint32_t((skey[size_t(j)] >> kIdShift) & kIdMask);
+      }
+  }
+  std::vector<int64_t> ans(static_cast<size_t>(M), 0);
+  std::vector<uint8_t> stored(static_cast<size_t>(M), 0);
+  for (int64_t i = 0; i < M; ++i) {
+    const uint32_t id = stage_id[size_t(i)];
+    CHECK(wrote[size_t(i)] && id < M && (int64_t(id) >> kWinBits) == (i >> kWinBits) && !stored[id]);
+    stored[id] = 1;
+    ans[id] = stage_ans[size_t(i)];
+  }
   unsigned char dg[32];
   SHA256(reinterpret_cast<const unsigned char*>(ans.data()), ans.size() * 8, dg);
   char hex[65];

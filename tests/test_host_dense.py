@@ -1,9 +1,11 @@
 """CPU parity of the dense-carry form's algorithm: tests/host/dense_emulator.cpp (test
 infrastructure) follows the dense kernels step by step over the product's host index plan and rank
-tables -- carry rows from the blocks' net weights, the plan's packed 64-bit keys and 32-bit values
-(the low endpoint split across them), the sort, the per-rank groups, the block replay -- and must
-give the stream oracle's golden answer hashes at several block sizes, 2^20-key trees included
-(endpoints past 2^17 take the split path)."""
+tables -- carry rows from the blocks' net weights, the plan's two-level bucket sort by state tile by
+tile (packed 64-bit keys and 32-bit values with the low endpoint split across them, counts, cursors,
+per-tile offsets, every position written once inside [0, M)), the per-block rank groups, the block
+replay storing answers in grouped order, the partition by query-id window and the scatter -- and
+must give the stream oracle's golden answer hashes at several block sizes, 2^20-key trees included
+(endpoints past 2^17 take the split path; s2's 522,196 queries span two id windows)."""
 import json
 import os
 import subprocess
