@@ -120,7 +120,7 @@ int bwd_answer_launch(const BwdIndex* x, int64_t M, const int32_t* qstate, const
  * plans and answers in a library-held scratch array of 64 MiB per GPU (held until the process
  * ends) instead of a caller's workspace, and calls no hipMalloc, hipFree or allocating library
  * routine; the caller still passes inputs, answers and flags.
- * A batch whose plan needs more scratch than that (about 1.8M queries over 300,000 states; see
+ * A batch whose plan needs more scratch than that (about 1.9M queries over 300,000 states; see
  * bwd_answer_noalloc_capacity) is rejected with hipErrorInvalidValue and a message on stderr.
  * Every call on a GPU shares that GPU's array and leaves its contents behind: calls that overlap
  * on one GPU (two streams, two threads) race and must be serialised by the caller.  Kernels only,
