@@ -467,7 +467,7 @@ int main(int argc, char** argv) {
   // to dense_rmax keys, one dense carry row per block (built once), then per step the whole
   // stream planned (one radix sort) and answered (one persistent block-replay launch, captured
   // in a hipGraph); its answers must equal the two-level form's
-  char dense_json[1536] = "null", noalloc_json[512] = "null";
+  char dense_json[2048] = "null", noalloc_json[512] = "null";
   int64_t dense_mismatch = -1, noalloc_mismatch = -1;
   if (dense) {
     PlanKnobs dk = kn;
@@ -586,6 +586,89 @@ int main(int argc, char** argv) {
       if (offset_mismatch) std::fprintf(stderr, "bw_batch: the shifted stream's answers differ (%" PRId64 ")\n", offset_mismatch);
       dense_mismatch += offset_mismatch;
     }
+    // edge cases of the plan, checked against root-path sums computed here on the host (independent
+    // of the kernels): the first m queries of the stream (a) all moved to one state (one bucket and
+    // one state group: the plan's counters at their worst contention), (b) moved to four states
+    // round robin, (c) as (b) with every 101st query out of range (state 0 or S+1, lo or hi above
+    // NK: flags[0] = 1 and answer 0), (d) batches of 1 and 7 queries of (b)
+    int64_t edge_mismatch = 0;
+    const int64_t edge_m = std::min<int64_t>(M, int64_t(1) << 20);
+    {
+      const int64_t S = spec.S;
+      const int U = spec.U;
+      const uint64_t NK = w.coord_n();
+      const int32_t four[4] = {1, int32_t(S / 3 + 1), int32_t(S / 2 + 1), int32_t(S)};
+      // the updates on the root->v path by key, with their prefix sums: F_v(x) = pre[#keys < x]
+      struct PathSums {
+        std::vector<uint32_t> k;
+        std::vector<int64_t> pre;
+        int64_t F(uint32_t x) const { return pre[size_t(std::lower_bound(k.begin(), k.end(), x) - k.begin())]; }
+      };
+      auto path_sums = [&](int32_t v) {
+        std::vector<std::pair<uint32_t, int64_t>> u;
+        for (int32_t a = v; a >= 1; a = w.parent[size_t(a)])
+          for (int i = 0; i < U; ++i) u.emplace_back(w.key[size_t(a - 1) * U + i], w.delta[size_t(a - 1) * U + i]);
+        std::sort(u.begin(), u.end());
+        PathSums p;
+        p.pre.push_back(0);
+        for (const auto& kv : u) {
+          p.k.push_back(kv.first);
+          p.pre.push_back(p.pre.back() + kv.second);
+        }
+        return p;
+      };
+      std::vector<PathSums> ps;
+      for (int32_t v : four) ps.push_back(path_sums(v));
+      std::vector<int32_t> qs(static_cast<size_t>(edge_m));
+      std::vector<uint32_t> ql(rs.qlo.begin(), rs.qlo.begin() + edge_m), qh(rs.qhi.begin(), rs.qhi.begin() + edge_m);
+      DeviceArena ea;
+      int32_t* e_qs = ea.scratch<int32_t>(size_t(edge_m));
+      uint32_t* e_ql = ea.scratch<uint32_t>(size_t(edge_m));
+      uint32_t* e_qh = ea.scratch<uint32_t>(size_t(edge_m));
+      int64_t* e_ans = ea.scratch<int64_t>(size_t(edge_m));
+      if (!ea.ok()) {
+        std::fprintf(stderr, "bw_batch: device: edge-case allocation\n");
+        return 5;
+      }
+      std::vector<int64_t> got(static_cast<size_t>(edge_m));
+      // answers the first m queries of (qs, ql, qh); expect(i) is query i's answer, or -1 for "out of range"
+      auto check = [&](int64_t m, bool bad_expected, auto expect_state) -> int64_t {
+        HIPX(hipMemcpy(e_qs, qs.data(), sizeof(int32_t) * size_t(m), hipMemcpyHostToDevice));
+        HIPX(hipMemcpy(e_ql, ql.data(), sizeof(uint32_t) * size_t(m), hipMemcpyHostToDevice));
+        HIPX(hipMemcpy(e_qh, qh.data(), sizeof(uint32_t) * size_t(m), hipMemcpyHostToDevice));
+        RUN(bwd_answer_launch(&D.x, m, e_qs, e_ql, e_qh, e_ans, DBn.flags, &D.launch, DBn.work, DBn.work_bytes, st));
+        HIPX(hipStreamSynchronize(st));
+        int64_t f = 0;
+        HIPX(hipMemcpy(&f, DBn.flags, sizeof(f), hipMemcpyDeviceToHost));
+        HIPX(hipMemcpy(got.data(), e_ans, sizeof(int64_t) * size_t(m), hipMemcpyDeviceToHost));
+        int64_t bad = (f != (bad_expected ? 1 : 0));
+        for (int64_t i = 0; i < m; ++i) {
+          const int j = expect_state(i);  // index into `four`, or -1: out of range
+          const int64_t want = j < 0 ? 0 : ps[size_t(j)].F(qh[size_t(i)]) - ps[size_t(j)].F(ql[size_t(i)]);
+          bad += got[size_t(i)] != want;
+        }
+        return bad;
+      };
+      for (int64_t i = 0; i < edge_m; ++i) qs[size_t(i)] = four[2];
+      const int64_t ea1 = check(edge_m, false, [](int64_t) { return 2; });
+      for (int64_t i = 0; i < edge_m; ++i) qs[size_t(i)] = four[i % 4];
+      const int64_t eb = check(edge_m, false, [](int64_t i) { return int(i % 4); });
+      const int64_t ed = check(1, false, [](int64_t i) { return int(i % 4); }) + check(7, false, [](int64_t i) { return int(i % 4); });
+      for (int64_t i = 0; i < edge_m; i += 101) {
+        switch ((i / 101) % 4) {
+          case 0: qs[size_t(i)] = 0; break;
+          case 1: qs[size_t(i)] = int32_t(S + 1); break;
+          case 2: ql[size_t(i)] = uint32_t(NK + 1); break;
+          default: qh[size_t(i)] = uint32_t(NK + 1); break;
+        }
+      }
+      const int64_t ec = check(edge_m, true, [](int64_t i) { return i % 101 == 0 ? -1 : int(i % 4); });
+      edge_mismatch = ea1 + eb + ec + ed;
+      if (edge_mismatch)
+        std::fprintf(stderr, "bw_batch: dense edge cases differ: one state %" PRId64 ", four %" PRId64 ", out of range %" PRId64
+                     ", tiny %" PRId64 "\n", ea1, eb, ec, ed);
+      dense_mismatch += edge_mismatch;
+    }
     // --dense-overlap: consecutive steps pipelined.  Step i+1's plan runs on a second stream, into
     // a second workspace, beside step i's replay, which is capped at --dense-max-wg workgroups so
     // that compute units stay free for the plan; a plan waits until the replay two steps back has
@@ -674,11 +757,12 @@ int main(int argc, char** argv) {
                   "\"index_wall_ms\":%.3f,\"step_ms\":{\"avg\":%.4f,\"min\":%.4f,\"max\":%.4f,\"repeat\":%d},"
                   "\"plan_ms\":%.4f,\"replay_ms\":%.4f,\"replay_phases_ms\":{\"block\":%.4f,\"partition\":%.4f,"
                   "\"scatter\":%.4f},\"queries_per_s\":%.1f,\"answers_sha256\":\"%s\","
-                  "\"offset_mismatch\":%" PRId64 ",\"step_mismatch\":%" PRId64 ",\"overlap\":%s,\"mismatch\":%" PRId64 "}",
+                  "\"offset_mismatch\":%" PRId64 ",\"step_mismatch\":%" PRId64 ",\"edge\":{\"queries\":%" PRId64
+                  ",\"mismatch\":%" PRId64 "},\"overlap\":%s,\"mismatch\":%" PRId64 "}",
                   dense_rmax, E4.x.nblk, dense_threads, dense_wg, dense_store, D.carry_bytes, D.build_ms, dense_index_wall, sm,
                   *std::min_element(step_ms.begin(), step_ms.end()), *std::max_element(step_ms.begin(), step_ms.end()),
                   repeat, mean(dplan_v), mean(drep_v), phase_ms[0], phase_ms[1], phase_ms[2], double(M) / (sm * 1e-3),
-                  dxr.sha256_hex, offset_mismatch, step_mismatch, overlap_json, dense_mismatch);
+                  dxr.sha256_hex, offset_mismatch, step_mismatch, edge_m, edge_mismatch, overlap_json, dense_mismatch);
     HIPX(hipGraphExecDestroy(dgx));
     HIPX(hipGraphDestroy(dg));
     HIPX(hipGraphExecDestroy(pgx));

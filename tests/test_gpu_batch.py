@@ -82,6 +82,7 @@ def test_device_index_equals_host_index(pkg, case, knobs):
     r = run_json(_cmd(pkg, g, knobs + ["--plain-sample", "0", "--device-index"]), timeout=900)
     d = r["device_index"]
     assert d["mismatch"] == 0 and d["answers_sha256"] == g["oracle"], (case, knobs, d)
+    assert d["edge"]["mismatch"] == 0 and d["edge"]["queries"] == min(g["nq"], 1 << 20), d["edge"]
     assert r["answers_sha256"] == g["oracle"]
 
 
@@ -120,10 +121,14 @@ def test_dense_form_matches_oracle(pkg, case, knobs):
     from a hipGraph) -- the oracle's answers at every block size and workgroup shape, small trees
     (one block, or blocks of 64 keys) to the full 102.5M-query stream, with the answers stored at
     their ids directly or staged (sorted order, partition by id window, die-local scatter; the
-    default once the ids span more than one window, so from s2 on)."""
+    default once the ids span more than one window, so from s2 on).  Each run also checks the plan's
+    edge cases against root-path sums computed on the host: the stream's first queries all on one
+    state (one bucket, one group), on four states, with out-of-range queries mixed in (flag set,
+    answer 0), and batches of 1 and 7 queries (`dense.edge`, counted in `dense.mismatch`)."""
     g = STREAMS[case]
     knobs = [pkg.TUNED if k is None else k for k in knobs]
     r = run_json(_cmd(pkg, g, knobs + ["--plain-sample", "0", "--dense"]), timeout=900)
     d = r["dense"]
     assert d["mismatch"] == 0 and d["answers_sha256"] == g["oracle"], (case, knobs, d)
+    assert d["edge"]["mismatch"] == 0 and d["edge"]["queries"] == min(g["nq"], 1 << 20), d["edge"]
     assert r["answers_sha256"] == g["oracle"]
