@@ -378,6 +378,17 @@ int main(int argc, char** argv) {
   HIPX(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
   for (int i = 0; i < warmup; ++i) HIPX(hipGraphLaunch(gexec, st));
   HIPX(hipStreamSynchronize(st));
+  // every timed replay's answers are checked afterwards by their device FNV-1a (outside the clock)
+  size_t gfnv_wb = 0;
+  if (bw_fnv1a_workspace(size_t(M) * 8, &gfnv_wb)) return 5;
+  DeviceArena gfa;
+  void* gfnv_work = gfa.alloc(gfnv_wb);
+  uint64_t* d_gfnv = gfa.scratch<uint64_t>(size_t(2 * repeat));
+  if (!gfa.ok()) {
+    std::fprintf(stderr, "bw_batch: device: FNV workspace\n");
+    return 5;
+  }
+  int nfnv = 0;
   std::vector<double> run_ms;
   for (int i = 0; i < repeat; ++i) {
     HIPX(hipEventRecord(e0, st));
@@ -387,6 +398,7 @@ int main(int argc, char** argv) {
     float t = 0;
     HIPX(hipEventElapsedTime(&t, e0, e1));
     run_ms.push_back(t);
+    RUN(bw_fnv1a_device(QB->answers, size_t(M) * 8, kFnvStart, gfnv_work, gfnv_wb, d_gfnv + nfnv++, st));
   }
   // the two-level form's whole step with its device plan: plan + graph replay in one event pair
   std::vector<double> step2_ms;
@@ -400,8 +412,11 @@ int main(int argc, char** argv) {
       float t = 0;
       HIPX(hipEventElapsedTime(&t, e0, e1));
       step2_ms.push_back(t);
+      RUN(bw_fnv1a_device(QB->answers, size_t(M) * 8, kFnvStart, gfnv_work, gfnv_wb, d_gfnv + nfnv++, st));
     }
   }
+  std::vector<uint64_t> replay_fnv(static_cast<size_t>(nfnv));
+  HIPX(hipMemcpy(replay_fnv.data(), d_gfnv, sizeof(uint64_t) * size_t(nfnv), hipMemcpyDeviceToHost));
   double step2_avg = 0;
   for (double t : step2_ms) step2_avg += t / double(step2_ms.size());
   // per-kernel timing of the same sequence (outside the graph), averaged over `repeat` runs
@@ -457,6 +472,9 @@ int main(int argc, char** argv) {
   const int xrc = bw_extract_answers(QB->answers, size_t(M), nullptr,
                                      answers_path.empty() ? nullptr : answers_path.c_str(), ans.data(),
                                      BW_EXTRACT_FNV_ON_DEVICE, &xr);
+  int64_t replay_mismatch = 0;
+  for (uint64_t f : replay_fnv) replay_mismatch += f != xr.fnv1a;
+  if (replay_mismatch) std::fprintf(stderr, "bw_batch: %" PRId64 " timed graph replays' answers differ\n", replay_mismatch);
   if (xrc) {
     std::fprintf(stderr, "bw_batch: extract failed (%d)\n", xrc);
     return xrc < 0 ? 4 : 5;
@@ -989,7 +1007,8 @@ int main(int argc, char** argv) {
       ",\"threads\":%d,\"waves\":%d,\"knobs_from\":\"%s\"},"
       "\"phases_ms\":{\"generate\":%.3f,\"index_plan_upload\":%.3f,\"batch_plan\":%.3f,\"batch_upload\":%.3f,"
       "\"extract\":%.3f,\"index_build\":%.4f},\"answers_fnv1a\":\"%016" PRIx64 "\","
-      "\"graph_ms\":{\"avg\":%.4f,\"min\":%.4f,\"max\":%.4f,\"repeat\":%d,\"warmup\":%d},"
+      "\"graph_ms\":{\"avg\":%.4f,\"min\":%.4f,\"max\":%.4f,\"repeat\":%d,\"warmup\":%d,\"checked\":%d,"
+      "\"replay_mismatch\":%" PRId64 "},"
       "\"queries_per_s\":%.1f,"
       "\"stage_ms\":{\"block_replay\":%.4f,\"chunk_replay\":%.4f,\"combine\":%.4f},"
       "\"stage_bytes\":{\"block_replay\":%.0f,\"chunk_replay\":%.0f,\"combine\":%.0f},"
@@ -1004,7 +1023,7 @@ int main(int argc, char** argv) {
       E.plan.knobs.chunk_items, E.plan.knobs.ep_per_work, E.plan.knobs.range_slots, BP.npairs, L.block_threads,
       L.chunk_waves, tuned_from.c_str(),
       ms_between(t0, t1), ms_between(t1, t2), ms_between(t2, t3), ms_between(t3, t4), xr.total_ms, index_ms, xr.fnv1a,
-      avg, run_ms.front(), run_ms.back(), repeat, warmup, double(M) / (avg * 1e-3), st_ms[0], st_ms[1], st_ms[2],
+      avg, run_ms.front(), run_ms.back(), repeat, warmup, nfnv, replay_mismatch, double(M) / (avg * 1e-3), st_ms[0], st_ms[1], st_ms[2],
       sb_block, sb_chunk, sb_combine, K, plain_q, plain_pass_ms,
       plain_query_ms, plain_pass_ms > 0 ? double(plain_q) / (plain_pass_ms * 1e-3) : 0.0,
       plain_query_ms > 0 ? double(plain_q) / (plain_query_ms * 1e-3) : 0.0, plain_mismatch,
@@ -1020,7 +1039,7 @@ int main(int argc, char** argv) {
     std::fclose(f);
   }
   return plain_mismatch == 0 && plan_mismatch <= 0 && dgen_mismatch <= 0 && didx_mismatch <= 0 && dall_mismatch <= 0 &&
-                     dense_mismatch <= 0 && noalloc_mismatch <= 0
+                     dense_mismatch <= 0 && noalloc_mismatch <= 0 && replay_mismatch == 0
              ? 0
              : 6;
 }

@@ -82,7 +82,6 @@ def test_device_index_equals_host_index(pkg, case, knobs):
     r = run_json(_cmd(pkg, g, knobs + ["--plain-sample", "0", "--device-index"]), timeout=900)
     d = r["device_index"]
     assert d["mismatch"] == 0 and d["answers_sha256"] == g["oracle"], (case, knobs, d)
-    assert d["edge"]["mismatch"] == 0 and d["edge"]["queries"] == min(g["nq"], 1 << 20), d["edge"]
     assert r["answers_sha256"] == g["oracle"]
 
 
