@@ -3,7 +3,7 @@
 //
 //   bw_serve [--cfg 2|3] [--S n --U n --N n --seed n] [--requests 20000 --max-batch 4096 --append-frac 0.05
 //            --append-U 128 --rate 20000] [--tuned FILE] [--slots 2] [--budget-mb 8192] [--slot-queries 16777216]
-//            [--check-alone 64] [--answers FILE] [--json FILE] [--timeline] [--tile-shift T]
+//            [--check-alone 64] [--answers FILE] [--json FILE] [--timeline] [--tile-shift T] [--form dense|two-level]
 //
 // Resident in HBM: the version tree's index (TreeEngine), the appended states' updates, and per
 // in-flight batch one fixed buffer ("slot") whose size is the HBM budget divided among the slots
@@ -15,9 +15,17 @@
 // stream every batch waits on; query requests join the batch.  Requests land in pinned receive buffers (state, lo, hi per query, in
 // arrival order), so a batch's queries are one contiguous slice of them: the host only replaces
 // appended states by their deepest indexed ancestor (listing those queries for the overlay) and
-// copies the slice in; the batch is planned on the device (bwt_plan_launch), answered by one
-// launch sequence and its answers copied straight into the pinned answer array, all on the slot's
-// own stream, so batches on different slots overlap on the device and on the copy engines.  A query on an appended state is answered at its deepest indexed ancestor plus
+// copies the slice in; the batch is planned and answered on the device and its answers copied
+// straight into the pinned answer array, all on the slot's own stream, so batches on different
+// slots overlap on the device and on the copy engines.  --form picks how a batch is answered:
+// `two-level` (the default: include/bwgpu_tree.h, the device batch planner and the query launch
+// sequence) or, over a config-2 tree, `dense` (include/bwgpu_dense.h: the dense carry of blocks of
+// up to 4,096 keys resident beside the index; bucket plan by state, persistent block replay,
+// unsort).  At served batch sizes the dense replay's per-state scans over 4,096-rank blocks cost
+// more than the two-level form's whole batch (measured: 0.40 against 0.19 ms per batch at 20,000
+// requests/s, 4.9 against 1.7 ms of answering for the saturated stream's 9.1M-query batch); it
+// pays off only over streams of ~10^8 queries (bw_batch).  A config-3 tree's 2^22 ranks need the
+// two-level form (a dense carry cannot hold them).  A query on an appended state is answered at its deepest indexed ancestor plus
 // the overlay of the appended updates on its path.
 // Latency = time the request's answers are back on the host - its arrival time.
 // Before the clock starts, one small batch per slot loads the kernels, and copies in both
@@ -42,7 +50,9 @@
 
 #include <omp.h>
 
+#include "bwgpu_dense.h"
 #include "bwgpu_tree.h"
+#include "dense_engine.hpp"
 #include "tree_engine.hpp"
 #include "tree_plan.hpp"
 #include "tree_tune.hpp"
@@ -89,6 +99,7 @@ struct Slot {
   hipStream_t st = nullptr;
   hipEvent_t done = nullptr;
   hipEvent_t start = nullptr;
+  hipEvent_t mark[3] = {};  // --timeline: after the copies in, after the plan, before the copy out
   unsigned char* host = nullptr;  // pinned: overlay ids | overlay states | answers of a batch run alone
   size_t h_ovq = 0, h_ovs = 0, h_ans = 0, host_bytes = 0;
   DeviceArena in;
@@ -97,7 +108,9 @@ struct Slot {
   uint32_t* d_hi = nullptr;
   uint32_t* d_ovq = nullptr;
   int32_t* d_ovs = nullptr;
-  DevicePlanSlot plan;
+  DevicePlanSlot plan;  // two-level form
+  DenseBatch db;        // dense form: plan and replay workspace, flags
+  int64_t* d_dans = nullptr;
   // the batch in flight
   bool busy = false;
   std::vector<int64_t> reqs;     // request ids
@@ -130,6 +143,12 @@ size_t slot_device_bytes(const TreeEngine& E, int64_t cap) {
   const size_t outputs = n * (4 + 8 + 4 + 8 + 4 + 4 + 4 + 4 + 16 + 16 + 16 + 8 + 4 + 2 + 4 + 8 + 16 + 8 + 8);
   return work + outputs + n * 20 + 4096;
 }
+// ... and for the dense form: its workspace, the answers, the inputs and overlay lists
+size_t dense_slot_device_bytes(const DenseEngine& D, int64_t cap) {
+  size_t work = 0;
+  if (bwd_answer_workspace(&D.x, cap, &work) != 0) return 0;
+  return work + size_t(cap) * (8 + 20) + 4096;
+}
 
 }  // namespace
 
@@ -148,6 +167,7 @@ int main(int argc, char** argv) {
   int nslots = 2, check_alone = 64;
   int64_t slot_queries = int64_t(1) << 24;  // most queries one batch takes
   int tile_shift_arg = -1;  // --tile-shift: key-tile width override (0: the balanced rule)
+  std::string form_arg;     // --form dense|two-level (default: two-level)
   bool timeline = false;  // --timeline: per-batch host times on stderr  // most queries one batch takes (batches pipeline across slots)
   double budget_mb = 8192;
   std::string answers_path, json_path, tuned_path;
@@ -172,6 +192,7 @@ int main(int argc, char** argv) {
     else if (!std::strcmp(k, "--slot-queries")) slot_queries = std::atoll(v);
     else if (!std::strcmp(k, "--check-alone")) check_alone = std::atoi(v);
     else if (!std::strcmp(k, "--tile-shift")) tile_shift_arg = std::atoi(v);
+    else if (!std::strcmp(k, "--form")) form_arg = v;
     else if (!std::strcmp(k, "--answers")) answers_path = v;
     else if (!std::strcmp(k, "--json")) json_path = v;
     else { std::fprintf(stderr, "bw_serve: unknown argument %s\n", k); return 2; }
@@ -181,8 +202,21 @@ int main(int argc, char** argv) {
     std::fprintf(stderr, "bw_serve: bad sizes\n");
     return 2;
   }
+  if (!form_arg.empty() && form_arg != "dense" && form_arg != "two-level") {
+    std::fprintf(stderr, "bw_serve: --form dense|two-level\n");
+    return 2;
+  }
+  const bool dense = form_arg == "dense";
+  if (dense && ranked) {
+    std::fprintf(stderr, "bw_serve: the dense form needs keys that are their own ranks (config 2)\n");
+    return 3;
+  }
   std::string tuned_from = "defaults";
   if (!tuned_path.empty() && load_tuned(tuned_path.c_str(), "served", kn, lk)) tuned_from = tuned_path;
+  if (dense) {  // blocks of up to 4,096 keys (the dense replay's LDS, a 47 GB carry), indexed by
+    kn.rmax = 4096;  // 1,024-thread workgroups (at most 16 ranks per thread)
+    lk.block_threads = 1024;
+  }
   // the served knobs were tuned on config 2's 2^20 keys; over config 3's 2^22 ranks their key
   // tiles (2^9 keys: 8,193 tiles) still fit the 14-bit class field and measured best (p99 0.93 ms
   // against 1.27 ms with the balanced rule's 2^11 tiles); a rank space the field cannot hold in
@@ -230,6 +264,11 @@ int main(int argc, char** argv) {
   HIPX(hipStreamCreate(&ist));
   RUN(E.build_index(ist));
   HIPX(hipStreamSynchronize(ist));
+  DenseEngine D;
+  if (dense && !D.init(E, ist, err, sizeof(err))) {
+    std::fprintf(stderr, "bw_serve: %s\n", err);
+    return 5;
+  }
   // appended states: resident device arrays (capacity = the stream's appends), host mirrors
   DeviceArena app;
   int32_t* d_app_parent = app.scratch<int32_t>(size_t(NA));
@@ -274,12 +313,13 @@ int main(int argc, char** argv) {
   const size_t share = size_t(budget_mb * 1048576.0 / nslots);
   int64_t cap = ss.max_batch;
   {
+    auto bytes = [&](int64_t c) { return dense ? dense_slot_device_bytes(D, c) : slot_device_bytes(E, c); };
     const int64_t probe = int64_t(1) << 20;
-    const size_t pb = slot_device_bytes(E, probe);
+    const size_t pb = bytes(probe);
     if (pb == 0) { std::fprintf(stderr, "bw_serve: planner workspace\n"); return 5; }
     const double per_q = double(pb) / double(probe);
     cap = std::max<int64_t>(cap, int64_t(double(share) / per_q));
-    while (cap > ss.max_batch && slot_device_bytes(E, cap) > share) cap = cap * 15 / 16;
+    while (cap > ss.max_batch && bytes(cap) > share) cap = cap * 15 / 16;
     cap = std::max<int64_t>(std::min(cap, std::max<int64_t>(slot_queries, 1)), ss.max_batch);
   }
   for (Slot& s : slots) {
@@ -291,13 +331,17 @@ int main(int argc, char** argv) {
     s.d_hi = s.in.scratch<uint32_t>(size_t(cap));
     s.d_ovq = s.in.scratch<uint32_t>(size_t(cap));
     s.d_ovs = s.in.scratch<int32_t>(size_t(cap));
-    if (!s.in.ok() || !alloc_plan_slot(E, cap, s.plan, err, sizeof(err))) {
+    const bool planned = dense ? alloc_dense_batch(D, cap, s.db, err, sizeof(err)) &&
+                                     (s.d_dans = s.db.arena.scratch<int64_t>(size_t(cap))) != nullptr
+                               : alloc_plan_slot(E, cap, s.plan, err, sizeof(err));
+    if (!s.in.ok() || !planned) {
       std::fprintf(stderr, "bw_serve: slot allocation: %s\n", s.in.ok() ? err : "inputs");
       return 5;
     }
     HIPX(hipStreamCreate(&s.st));
     HIPX(hipEventCreate(&s.done));
     HIPX(hipEventCreate(&s.start));
+    for (auto& m : s.mark) HIPX(hipEventCreate(&m));
   }
   const auto g1 = clk::now();
 
@@ -362,11 +406,21 @@ int main(int argc, char** argv) {
       RUN(bwt_rank_remap_launch(&rh, s.st));
     }
     HIPX(hipStreamWaitEvent(s.st, app_ready, 0));  // appended states admitted so far are on the device
+    if (timeline) HIPX(hipEventRecord(s.mark[0], s.st));
     const auto p2 = clk::now();
-    RUN(device_plan(E, s.plan, M, s.d_state, s.d_lo, s.d_hi, s.st));
+    if (dense)
+      RUN(bwd_plan_launch(&D.x, M, s.d_state, s.d_lo, s.d_hi, s.db.flags, s.db.work, s.db.work_bytes, s.st));
+    else
+      RUN(device_plan(E, s.plan, M, s.d_state, s.d_lo, s.d_hi, s.st));
+    if (timeline) HIPX(hipEventRecord(s.mark[1], s.st));
     const auto p3 = clk::now();
-    BwtLaunch l = L;
-    RUN(bwt_query_launch(&E.x, &s.plan.b, &l, s.st));
+    int64_t* answers = dense ? s.d_dans : s.plan.b.answers;
+    if (dense) {
+      RUN(bwd_replay_launch(&D.x, M, answers, &D.launch, s.db.work, s.db.work_bytes, s.st));
+    } else {
+      BwtLaunch l = L;
+      RUN(bwt_query_launch(&E.x, &s.plan.b, &l, s.st));
+    }
     if (nov) {
       BwtOverlay o{};
       o.S = S;
@@ -379,10 +433,11 @@ int main(int argc, char** argv) {
       o.app_parent = d_app_parent;
       o.app_key = d_app_key;
       o.app_delta = d_app_delta;
-      o.answers = s.plan.b.answers;
+      o.answers = answers;
       RUN(bwt_overlay_launch(&o, s.st));
     }
-    HIPX(hipMemcpyAsync(dst, s.plan.b.answers, size_t(M) * 8, hipMemcpyDeviceToHost, s.st));
+    if (timeline) HIPX(hipEventRecord(s.mark[2], s.st));
+    HIPX(hipMemcpyAsync(dst, answers, size_t(M) * 8, hipMemcpyDeviceToHost, s.st));
     HIPX(hipEventRecord(s.done, s.st));
     s.busy = true;
     s.launched = clk::now();
@@ -444,7 +499,7 @@ int main(int argc, char** argv) {
       for (Slot& s : slots) {
         const size_t m = std::min({n, size_t(NQ) * 4, size_t(s.cap) * 4});
         HIPX(hipMemcpyAsync(s.d_lo, p_lo, m, hipMemcpyHostToDevice, s.st));
-        HIPX(hipMemcpyAsync(s.host + s.h_ans, s.plan.b.answers, m, hipMemcpyDeviceToHost, s.st));
+        HIPX(hipMemcpyAsync(s.host + s.h_ans, dense ? s.d_dans : s.plan.b.answers, m, hipMemcpyDeviceToHost, s.st));
       }
       if (NA > 0) {
         const size_t m = std::min(n, size_t(NA) * size_t(AU) * 8);
@@ -489,9 +544,17 @@ int main(int argc, char** argv) {
       float t = 0;
       HIPX(hipEventElapsedTime(&t, s.start, s.done));
       gpu_ms.push_back(double(t));
-      if (timeline)
-        std::fprintf(stderr, "batch %" PRId64 " queries: launched %.3f ms, harvested %.3f ms, stream %.3f ms\n", s.nq,
-                     secs(t0, s.launched) * 1e3, now * 1e3, double(t));
+      if (timeline) {
+        float c = 0, p = 0, q = 0, o = 0;
+        HIPX(hipEventElapsedTime(&c, s.start, s.mark[0]));
+        HIPX(hipEventElapsedTime(&p, s.mark[0], s.mark[1]));
+        HIPX(hipEventElapsedTime(&q, s.mark[1], s.mark[2]));
+        HIPX(hipEventElapsedTime(&o, s.mark[2], s.done));
+        std::fprintf(stderr,
+                     "batch %" PRId64 " queries: launched %.3f ms, harvested %.3f ms, stream %.3f ms (copies in %.3f, "
+                     "plan %.3f, answers %.3f, copy out %.3f)\n",
+                     s.nq, secs(t0, s.launched) * 1e3, now * 1e3, double(t), double(c), double(p), double(q), double(o));
+      }
       s.busy = false;
       progressed = true;
     }
@@ -554,6 +617,16 @@ int main(int argc, char** argv) {
     ++alone_checked;
   }
 
+  // the dense plan flags an out-of-range query (each slot's flag holds its last batch's)
+  if (dense)
+    for (Slot& s : slots) {
+      int64_t f = 0;
+      HIPX(hipMemcpy(&f, s.db.flags, sizeof(f), hipMemcpyDeviceToHost));
+      if (f) {
+        std::fprintf(stderr, "bw_serve: the dense plan flagged a query (%" PRId64 ")\n", f);
+        return 3;
+      }
+    }
   // latency of the second half of the requests (after any start-up transient)
   std::vector<double> lat2;
   for (int64_t r = R / 2; r < R; ++r)
@@ -586,15 +659,16 @@ int main(int argc, char** argv) {
       "\"max_batch\":%d,\"n_answers\":%" PRId64 ",\"answers_sha256\":\"%s\",\"slots\":%d,\"budget_mb\":%.0f,"
       "\"slot_capacity_queries\":%" PRId64 ",\"batches\":%" PRId64 ",\"mean_batch_queries\":%.1f,"
       "\"max_batch_queries\":%" PRId64 ",\"wall_s\":%.4f,\"offered_queries_per_s\":%.1f,\"queries_per_s\":%.1f,"
-      "\"latency_ms\":{\"p50\":%.3f,\"p90\":%.3f,\"p99\":%.3f,\"max\":%.3f},\"planner\":\"device\","
+      "\"latency_ms\":{\"p50\":%.3f,\"p90\":%.3f,\"p99\":%.3f,\"max\":%.3f},\"planner\":\"device\",\"answer_form\":\"%s\","
       "\"host_stage_s\":%.4f,\"slot_device_bytes\":%zu,\"p99_second_half_ms\":%.3f,"
       "\"batch_gpu_ms\":{\"mean\":%.4f,\"p99\":%.4f},"
       "\"alone\":{\"requests\":%" PRId64 ",\"mismatch\":%" PRId64 "},\"knobs_from\":\"%s\",\"setup_s\":%.3f,"
       "\"device\":\"%s\",\"cus\":%d}\n",
       tcfg, ranked ? "true" : "false", S, spec.U, spec.N, spec.seed, R, nquery_req, nappended, ss.rate, ss.max_batch, NQ, sha.c_str(), nslots, budget_mb,
       cap, nbatches, nbatches ? sum_batch_q / double(nbatches) : 0.0, max_batch_q, wall, offered, double(NQ) / wall,
-      pct(0.5) * 1e3, pct(0.9) * 1e3, pct(0.99) * 1e3, lat.empty() ? 0.0 : lat.back() * 1e3, plan_s,
-      slots[0].in.bytes + slots[0].plan.arena.bytes, p99_late * 1e3, gpu_mean, gpu_p99, alone_checked,
+      pct(0.5) * 1e3, pct(0.9) * 1e3, pct(0.99) * 1e3, lat.empty() ? 0.0 : lat.back() * 1e3,
+      dense ? "dense" : "two-level", plan_s, slots[0].in.bytes + slots[0].plan.arena.bytes + slots[0].db.arena.bytes,
+      p99_late * 1e3, gpu_mean, gpu_p99, alone_checked,
       alone_mismatch, tuned_from.c_str(), secs(g0, g1), dev, prop.multiProcessorCount);
   std::fwrite(buf, 1, size_t(n), stdout);
   if (!json_path.empty()) {

@@ -1,6 +1,8 @@
 """GPU parity of the served form (bw_serve: an open-loop Poisson request stream with appended
 branch states, continuous batching over a resident index) against the stream oracle's golden
-hashes, and the served answers of sampled requests against the same requests answered alone."""
+hashes, and the served answers of sampled requests against the same requests answered alone.
+Batches are answered by the two-level form by default and, over a config-2 tree, by the dense form
+with --form dense; over a config-3 tree (rank-LUT) by the two-level form only."""
 import pytest
 
 from conftest import golden, run_json
@@ -22,6 +24,10 @@ def _cmd(pkg, g, extra):
     ("s3_served_small", ["--slots", "1", "--check-alone", "8"]),
     ("s4_served_mid", ["--check-alone", "32"]),
     ("s4_served_mid", ["--slots", "4", "--budget-mb", "64", "--rate", "1e9", "--check-alone", "8"]),
+    ("s3_served_small", ["--form", "dense", "--check-alone", "16"]),
+    ("s4_served_mid", ["--form", "dense", "--check-alone", "32"]),
+    ("s4_served_mid", ["--form", "dense", "--slots", "4", "--budget-mb", "64", "--rate", "1e9",
+                       "--check-alone", "8"]),
     # over a config-3 tree: raw 32-bit endpoints ranked on the device per batch, appended keys at
     # admission
     ("s6_served_lut_small", ["--check-alone", "16"]),
@@ -36,14 +42,17 @@ def test_served_matches_oracle_and_alone(pkg, case, extra):
     assert r["n_answers"] == g["nq"] and r["appends"] == g["napp"]
     assert r["alone"]["mismatch"] == 0 and r["alone"]["requests"] > 0
     assert r["latency_ms"]["p99"] >= r["latency_ms"]["p50"] > 0
+    assert r["answer_form"] == ("dense" if "dense" in extra else "two-level")
 
 
 @pytest.mark.gpu
-def test_served_full_stream(pkg):
-    """20,000 requests (9.2M queries, 1,031 appended states) over the 300,000-state tree."""
+@pytest.mark.parametrize("form", ["dense", "two-level"])
+def test_served_full_stream(pkg, form):
+    """20,000 requests (9.2M queries, 1,031 appended states) over the 300,000-state tree, in both
+    answer forms."""
     g = STREAMS["S5_served_full"]
-    r = run_json(_cmd(pkg, g, ["--check-alone", "32"]), timeout=600)
-    assert r["answers_sha256"] == g["oracle"]
+    r = run_json(_cmd(pkg, g, ["--form", form, "--check-alone", "32"]), timeout=600)
+    assert r["answers_sha256"] == g["oracle"] and r["answer_form"] == form
     assert r["alone"]["mismatch"] == 0
 
 
