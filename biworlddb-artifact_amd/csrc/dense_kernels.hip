@@ -45,18 +45,19 @@ std::mutex g_scratch_mu;
 void* g_scratch_at[kMaxDevices] = {};
 
 // ---------------------------------------------------------------- planned query layout
-// key = state << 44 | query id << 17 | lo & 0x1FFFF;  val = hi | (lo >> 17) << 21 (the state in 19
-// bits, the id in 27, the endpoints in 21 each).
+// In bucket order: key = state << 44 | query id << 17 | lo & 0x1FFFF;  val = hi | (lo >> 17) << 21
+// (the state in 19 bits, the id in 27, the endpoints in 21 each).  Grouped by state: ends = lo |
+// hi << 32 and, in a separate array, the query id -- the staged replay reads only the ends, the
+// answers' partition only the ids.
 constexpr int kStateShift = 44;
 constexpr int kIdShift = 17;
 constexpr uint64_t kIdMask = (uint64_t(1) << 27) - 1;
 constexpr uint32_t kLoLowMask = (1u << 17) - 1;
 constexpr uint32_t kHiMask = (1u << 21) - 1;
 
-__device__ __forceinline__ void unpack(uint64_t key, uint32_t val, uint32_t& qid, uint32_t& lo, uint32_t& hi) {
-  qid = (uint32_t)((key >> kIdShift) & kIdMask);
-  lo = ((uint32_t)key & kLoLowMask) | ((val >> 21) << 17);
-  hi = val & kHiMask;
+__device__ __forceinline__ uint64_t ends_of(uint64_t key, uint32_t val) {
+  const uint32_t lo = ((uint32_t)key & kLoLowMask) | ((val >> 21) << 17);
+  return (uint64_t)lo | ((uint64_t)(val & kHiMask) << 32);
 }
 
 // ---------------------------------------------------------------- carry build
@@ -450,8 +451,8 @@ __global__ void __launch_bounds__(kLoBins) bwd_tile_offsets_kernel(int64_t S, co
 __global__ void __launch_bounds__(kPT) bwd_state_kernel(int64_t M, int nh, const uint32_t* __restrict__ hbase,
                                                        const uint32_t* __restrict__ tstart, const uint32_t* __restrict__ thist,
                                                        const uint32_t* __restrict__ grp, const uint64_t* __restrict__ kin,
-                                                       const uint32_t* __restrict__ vin, uint64_t* __restrict__ kout,
-                                                       uint32_t* __restrict__ vout, int64_t* flags) {
+                                                       const uint32_t* __restrict__ vin, uint64_t* __restrict__ ends,
+                                                       uint32_t* __restrict__ ids, int64_t* flags) {
   __shared__ uint32_t cnt[kLoBins], lstart[kLoBins], gdst[kLoBins];
   __shared__ uint32_t wtot[kPT / 64];
   __shared__ uint64_t stage[kPTile];
@@ -500,21 +501,20 @@ __global__ void __launch_bounds__(kPT) bwd_state_kernel(int64_t M, int nh, const
       const uint32_t d = gdst[l] + (uint32_t)(j - (int)lstart[l]);
       if (d < (uint64_t)M) {
         dst[i] = d;
-        kout[d] = kk;
+        ids[d] = (uint32_t)((kk >> kIdShift) & kIdMask);
       } else {
         lost = true;
       }
     }
   }
   __syncthreads();  // the keys are out of the stage
-  uint32_t* sv = reinterpret_cast<uint32_t*>(stage);
 #pragma unroll
   for (int k = 0; k < kPE; ++k)
-    if (k * kPT + tid < n) sv[lstart[(key[k] >> kStateShift) & (kLoBins - 1)] + off[k]] = val[k];
+    if (k * kPT + tid < n) stage[lstart[(key[k] >> kStateShift) & (kLoBins - 1)] + off[k]] = ends_of(key[k], val[k]);
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < kPE; ++i)
-    if (dst[i] != 0xFFFFFFFFu) vout[dst[i]] = sv[tid + i * kPT];
+    if (dst[i] != 0xFFFFFFFFu) ends[dst[i]] = stage[tid + i * kPT];
   if (lost) flags[0] = 2;
 }
 
@@ -577,7 +577,7 @@ __host__ __device__ inline BlockLds block_lds_layout(int span, int nt, int rmax)
 // kSorted: each answer goes to its query's sorted position (out[M], runs of consecutive positions
 // per state; the unsort kernels below put them in query-id order); otherwise to out[query id].
 template <int RPT, bool kSorted>
-__device__ void dense_block(const BwdIndex& x, const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+__device__ void dense_block(const BwdIndex& x, const uint64_t* __restrict__ ends, const uint32_t* __restrict__ qids,
                             const uint2* __restrict__ rg, const int32_t* __restrict__ blast,
                             int64_t* __restrict__ out, const int b,
                             unsigned char* smem) {
@@ -657,11 +657,13 @@ __device__ void dense_block(const BwdIndex& x, const uint64_t* __restrict__ keys
     if (nq == 0) return;
     // first round: the queries' items and carry-row reads are issued before the scan, whose
     // barriers hide their latency
-    uint32_t qid = 0, lo = 0, hi = 0;
+    uint32_t lo = 0, hi = 0;
     int64_t clo = 0, chi = 0;
     const bool live = tid < nq;
     if (live) {
-      unpack(keys[qa + tid], vals[qa + tid], qid, lo, hi);
+      const uint64_t en = ends[qa + tid];
+      lo = (uint32_t)en;
+      hi = (uint32_t)(en >> 32);
       clo = crow[lo];
       chi = crow[hi];
     }
@@ -669,12 +671,14 @@ __device__ void dense_block(const BwdIndex& x, const uint64_t* __restrict__ keys
     rebuild();
     if (live) {
       const int rl = lower_bound_lds<uint32_t>(skey, R, lo), rh = lower_bound_lds<uint32_t>(skey, R, hi);
-      out[kSorted ? qa + tid : qid] = (chi - clo) + (P[pslot<RPT>(rh)] - P[pslot<RPT>(rl)]);
+      out[kSorted ? qa + tid : qids[qa + tid]] = (chi - clo) + (P[pslot<RPT>(rh)] - P[pslot<RPT>(rl)]);
     }
     for (int j = tid + nt; j < nq; j += nt) {  // groups larger than the workgroup
-      unpack(keys[qa + j], vals[qa + j], qid, lo, hi);
+      const uint64_t en = ends[qa + j];
+      lo = (uint32_t)en;
+      hi = (uint32_t)(en >> 32);
       const int rl = lower_bound_lds<uint32_t>(skey, R, lo), rh = lower_bound_lds<uint32_t>(skey, R, hi);
-      out[kSorted ? qa + j : qid] = (crow[hi] - crow[lo]) + (P[pslot<RPT>(rh)] - P[pslot<RPT>(rl)]);
+      out[kSorted ? qa + j : qids[qa + j]] = (crow[hi] - crow[lo]) + (P[pslot<RPT>(rh)] - P[pslot<RPT>(rl)]);
     }
   };
 
@@ -756,7 +760,7 @@ __device__ void dense_block(const BwdIndex& x, const uint64_t* __restrict__ keys
 }
 
 template <int RPT, bool kSorted>
-__global__ void __launch_bounds__(1024) bwd_block_kernel(const BwdIndex x, const uint64_t* keys, const uint32_t* vals,
+__global__ void __launch_bounds__(1024) bwd_block_kernel(const BwdIndex x, const uint64_t* ends, const uint32_t* ids,
                                                          const uint2* rg, const int32_t* blast, int64_t* out,
                                                          uint32_t* counter) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -767,7 +771,7 @@ __global__ void __launch_bounds__(1024) bwd_block_kernel(const BwdIndex x, const
     const int b = __builtin_amdgcn_readfirstlane(next);
     __syncthreads();  // every thread has read `next` before it is taken again
     if (b >= x.nblk) break;
-    dense_block<RPT, kSorted>(x, keys, vals, rg, blast, out, b, smem_raw);
+    dense_block<RPT, kSorted>(x, ends, ids, rg, blast, out, b, smem_raw);
     __syncthreads();  // the next block reuses the LDS
   }
 }
@@ -802,7 +806,7 @@ __global__ void __launch_bounds__(256) bwd_replay_init_kernel(uint32_t* counter,
     for (int i = threadIdx.x; i < nw; i += 256) cursor[i] = 0;
 }
 
-__global__ void __launch_bounds__(kPartT) bwd_part_kernel(int64_t M, const uint64_t* __restrict__ keys,
+__global__ void __launch_bounds__(kPartT) bwd_part_kernel(int64_t M, const uint32_t* __restrict__ ids,
                                                           const int64_t* __restrict__ sans, uint32_t* __restrict__ cursor,
                                                           int64_t* __restrict__ stage_ans, uint32_t* __restrict__ stage_id) {
   __shared__ uint32_t cnt[kMaxWin], lstart[kMaxWin], gbase[kMaxWin];
@@ -819,7 +823,7 @@ __global__ void __launch_bounds__(kPartT) bwd_part_kernel(int64_t M, const uint6
 #pragma unroll
   for (int k = 0; k < kPartE; ++k) {
     const int64_t p = t0 + min(k * kPartT + tid, n - 1);
-    id[k] = (uint32_t)((__builtin_nontemporal_load(keys + p) >> kIdShift) & kIdMask);
+    id[k] = __builtin_nontemporal_load(ids + p);
     a[k] = __builtin_nontemporal_load(sans + p);
   }
   __syncthreads();  // counts are zero
@@ -918,11 +922,11 @@ CarryLayout carry_layout(const BwdIndex* x, int64_t nkeys) {
 }
 
 // The planned batch: keys/vals (the bucket pass's output: queries grouped by high rank digit; then,
-// once the plan has run, the unsort's staging buffer: stage_ans = kin, stage_id = vin), kout/vout
-// (queries grouped by state), grp, the bucket tables, per rank its state's group (rg) and per block
+// once the plan has run, the unsort's staging buffer: stage_ans = kin, stage_id = vin), ends/ids
+// (queries grouped by state: endpoints, query ids), grp, the bucket tables, per rank its state's group (rg) and per block
 // its last queried rank (blast); sans holds the replay's answers in sorted order.
 struct AnswerLayout {
-  size_t kin, kout, vin, vout, grp, rg, blast, counter, cursor, sans, hcount, hcur, hbase, tstart, thist, total;
+  size_t kin, ends, vin, ids, grp, rg, blast, counter, cursor, sans, hcount, hcur, hbase, tstart, thist, total;
   int nh;
   unsigned tiles;  // bound on the second level's tiles (its grid)
 };
@@ -937,9 +941,9 @@ hipError_t answer_layout(const BwdIndex* x, int64_t M, AnswerLayout& L) {
     return a;
   };
   L.kin = take((size_t)M * 8);
-  L.kout = take((size_t)M * 8);
+  L.ends = take((size_t)M * 8);
   L.vin = take((size_t)M * 4);
-  L.vout = take((size_t)M * 4);
+  L.ids = take((size_t)M * 4);
   L.grp = take((size_t)(x->S + 2) * 4);
   L.rg = take((size_t)x->S * 8);
   L.blast = take((size_t)x->nblk * 4);
@@ -1039,9 +1043,9 @@ extern "C" int bwd_plan_launch(const BwdIndex* x, int64_t M, const int32_t* qsta
   if (work_bytes < L.total) return (int)hipErrorInvalidValue;
   const hipStream_t st = (hipStream_t)stream;
   uint64_t* kin = at<uint64_t>(work, L.kin);
-  uint64_t* kout = at<uint64_t>(work, L.kout);
+  uint64_t* ends = at<uint64_t>(work, L.ends);
   uint32_t* vin = at<uint32_t>(work, L.vin);
-  uint32_t* vout = at<uint32_t>(work, L.vout);
+  uint32_t* ids = at<uint32_t>(work, L.ids);
   uint32_t* grp = at<uint32_t>(work, L.grp);
   uint32_t* hcount = at<uint32_t>(work, L.hcount);
   uint32_t* hcur = at<uint32_t>(work, L.hcur);
@@ -1065,8 +1069,8 @@ extern "C" int bwd_plan_launch(const BwdIndex* x, int64_t M, const int32_t* qsta
   if ((rc = report("plan buckets", hipGetLastError()))) return rc;
   hipLaunchKernelGGL(bwd_tile_count_kernel, dim3(L.tiles), dim3(kPT), 0, st, nh, hbase, tstart, kin, thist);
   hipLaunchKernelGGL(bwd_tile_offsets_kernel, dim3((unsigned)nh), dim3(kLoBins), 0, st, x->S, hbase, tstart, thist, grp);
-  hipLaunchKernelGGL(bwd_state_kernel, dim3(L.tiles), dim3(kPT), 0, st, M, nh, hbase, tstart, thist, grp, kin, vin, kout,
-                     vout, flags);
+  hipLaunchKernelGGL(bwd_state_kernel, dim3(L.tiles), dim3(kPT), 0, st, M, nh, hbase, tstart, thist, grp, kin, vin, ends,
+                     ids, flags);
   if ((rc = report("plan states", hipGetLastError()))) return rc;
   hipLaunchKernelGGL(bwd_rank_groups_kernel, dim3((unsigned)x->nblk), dim3(256), 0, st, *x, grp, at<uint2>(work, L.rg),
                      at<int32_t>(work, L.blast));
@@ -1107,8 +1111,8 @@ static int replay_impl(const BwdIndex* x, int64_t M, int64_t* answers, const Bwd
   int nwg = ncu * l->wg_per_cu;
   if (l->max_workgroups > 0 && l->max_workgroups < nwg) nwg = l->max_workgroups;
   const dim3 grid((unsigned)nwg);
-  const uint64_t* kout = at<uint64_t>(work, L.kout);
-  const uint32_t* vout = at<uint32_t>(work, L.vout);
+  const uint64_t* ends = at<uint64_t>(work, L.ends);
+  const uint32_t* ids = at<uint32_t>(work, L.ids);
   const uint2* rg = at<uint2>(work, L.rg);
   const int32_t* blast = at<int32_t>(work, L.blast);
   int64_t* out = staged ? at<int64_t>(work, L.sans) : answers;
@@ -1116,7 +1120,7 @@ static int replay_impl(const BwdIndex* x, int64_t M, int64_t* answers, const Bwd
   auto mark = [&](int i) { return ev ? hipEventRecord(ev[i], st) : hipSuccess; };
   if ((e = mark(0))) return report("replay event", e);
   auto launch = [&](auto kernel) {
-    hipLaunchKernelGGL(kernel, grid, dim3(nt), lds, st, *x, kout, vout, rg, blast, out, counter);
+    hipLaunchKernelGGL(kernel, grid, dim3(nt), lds, st, *x, ends, ids, rg, blast, out, counter);
   };
   if (staged) {
     if (rpt == 4) launch(bwd_block_kernel<4, true>);
@@ -1133,7 +1137,7 @@ static int replay_impl(const BwdIndex* x, int64_t M, int64_t* answers, const Bwd
     if ((e = mark(2)) || (e = mark(3))) return report("replay event", e);
     return 0;
   }
-  hipLaunchKernelGGL(bwd_part_kernel, dim3((unsigned)((M + kPartTile - 1) / kPartTile)), dim3(kPartT), 0, st, M, kout,
+  hipLaunchKernelGGL(bwd_part_kernel, dim3((unsigned)((M + kPartTile - 1) / kPartTile)), dim3(kPartT), 0, st, M, ids,
                      at<const int64_t>(work, L.sans), cursor, at<int64_t>(work, L.kin), at<uint32_t>(work, L.vin));
   if ((rc = report("answer partition", hipGetLastError()))) return rc;
   if ((e = mark(2))) return report("replay event", e);

@@ -36,7 +36,8 @@ using namespace bwg;
     }                                                                    \
   } while (0)
 
-// the kernels' layout: key = state << 44 | id << 17 | lo & 0x1FFFF; val = hi | (lo >> 17) << 21
+// the kernels' layouts: in bucket order key = state << 44 | id << 17 | lo & 0x1FFFF and
+// val = hi | (lo >> 17) << 21; grouped by state, ends = lo | hi << 32 and the query ids apart
 constexpr int kStateShift = 44, kIdShift = 17;
 constexpr uint64_t kIdMask = (uint64_t(1) << 27) - 1;
 constexpr uint32_t kLoLowMask = (1u << 17) - 1, kHiMask = (1u << 21) - 1;
@@ -182,8 +183,8 @@ int main(int argc, char** argv) {
   }
   grp[size_t(S) + 1] = M;
   // state: per tile, grouped by low digit, each state's run at grp[v] + the tile's start
-  std::vector<uint64_t> skey(static_cast<size_t>(M));
-  std::vector<uint32_t> sval(static_cast<size_t>(M));
+  std::vector<uint64_t> ends(static_cast<size_t>(M));
+  std::vector<uint32_t> qids(static_cast<size_t>(M));
   std::fill(wrote.begin(), wrote.end(), 0);
   for (int64_t g = 0; g < ntiles; ++g) {
     int h;
@@ -196,8 +197,10 @@ int main(int argc, char** argv) {
       const int64_t d = grp[size_t(v)] + thist[size_t(g) * kLoBins + l] + local[size_t(l)]++;
       CHECK(d >= grp[size_t(v)] && d < grp[size_t(v) + 1] && d < M && !wrote[size_t(d)]);
       wrote[size_t(d)] = 1;
-      skey[size_t(d)] = bkeyv[size_t(p)];
-      sval[size_t(d)] = bval[size_t(p)];
+      const uint64_t kk = bkeyv[size_t(p)];
+      const uint32_t vv = bval[size_t(p)];
+      qids[size_t(d)] = uint32_t((kk >> kIdShift) & kIdMask);
+      ends[size_t(d)] = uint64_t((uint32_t(kk) & kLoLowMask) | ((vv >> 21) << 17)) | (uint64_t(vv & kHiMask) << 32);
     }
   }
   for (int64_t j = 0; j < M; ++j) CHECK(wrote[size_t(j)]);
@@ -235,10 +238,8 @@ int main(int argc, char** argv) {
       if (v <= 0) continue;
       CHECK(estep[size_t(e)] == s && erank[size_t(v)] == e);
       for (int64_t j = rga[size_t(e)]; j < rgb[size_t(e)]; ++j) {
-        const uint64_t kk = skey[size_t(j)];
-        const uint32_t vv = sval[size_t(j)];
-        const uint32_t qid = uint32_t((kk >> kIdShift) & kIdMask);
-        const uint32_t lo = (uint32_t(kk) & kLoLowMask) | ((vv >> 21) << 17), hi = vv & kHiMask;
+        const uint32_t qid = qids[size_t(j)];
+        const uint32_t lo = uint32_t(ends[size_t(j)]), hi = uint32_t(ends[size_t(j)] >> 32);
         CHECK(qid < M && !done[qid] && rs.qstate[qid] == v && rs.qlo[qid] == lo && rs.qhi[qid] == hi);
         const int rl = int(std::lower_bound(bk, bk + R, lo) - bk), rh = int(std::lower_bound(bk, bk + R, hi) - bk);
         int64_t pl = 0, ph = 0;  // in-block prefixes at the endpoints' ranks
@@ -261,14 +262,14 @@ int main(int argc, char** argv) {
   for (int64_t t0 = 0; t0 < M; t0 += kPartTile) {
     const int64_t t1 = std::min(M, t0 + kPartTile);
     std::vector<std::vector<int64_t>> run(static_cast<size_t>(nwin));
-    for (int64_t j = t0; j < t1; ++j) run[size_t(((skey[size_t(j)] >> kIdShift) & kIdMask) >> kWinBits)].push_back(j);
+    for (int64_t j = t0; j < t1; ++j) run[size_t(qids[size_t(j)] >> kWinBits)].push_back(j);
     for (int64_t wv = 0; wv < nwin; ++wv)
       for (int64_t j : run[size_t(wv)]) {
         const int64_t gpos = (wv << kWinBits) + cursor[size_t(wv)]++;
         CHECK(gpos < M && !wrote[size_t(gpos)]);
         wrote[size_t(gpos)] = 1;
         stage_ans[size_t(gpos)] = sans[size_t(j)];
-        stage_id[size_t(gpos)] = uint32_t((skey[size_t(j)] >> kIdShift) & kIdMask);
+        stage_id[size_t(gpos)] = qids[size_t(j)];
       }
   }
   std::vector<int64_t> ans(static_cast<size_t>(M), 0);
