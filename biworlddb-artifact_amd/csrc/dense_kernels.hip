@@ -200,8 +200,11 @@ __global__ void __launch_bounds__(256) bwd_seg_scan_kernel(int64_t NKP, int nseg
 //            sorted position of state v;
 //   state    per second-level tile: grouped by low digit in LDS, each state's run stored at grp[v]
 //            plus the tile's start;
-//   ranks    per block: the group [first, end) of the state of each of its ranks, and the block's
-//            last rank with queries (-1: none), which ends its replay.
+//   ranks    per block: the group [first, end) of the state of each of its ranks, the block's
+//            last rank with queries (-1: none), which ends its replay, and its query count;
+//   order    the blocks by descending query count (log-scale bins), the order in which the
+//            persistent replay's workgroups take them: the heaviest first, so the replay ends on
+//            light blocks and the compute units finish together.
 // A run goes through LDS so that each store instruction writes consecutive positions.  Counts and
 // placements come from the same rank of the same queries, so every position in [0, M) is written
 // once; a position outside it (which the counts rule out) would set flags[0] to 2 and not be
@@ -232,6 +235,16 @@ __device__ __forceinline__ uint32_t block_excl_scan_u32(uint32_t v, uint32_t* wt
 }
 
 __device__ __forceinline__ uint32_t plan_state(int64_t S, int32_t v) { return (v >= 1 && v <= S) ? (uint32_t)v : 1u; }
+
+// The replay's block order: kCostBins bins of query counts, 4 per power of two, bin 0 the heaviest
+// (a block without queries goes to the last bin).
+constexpr int kCostBins = 128;
+__device__ __forceinline__ int cost_bin(uint32_t q) {
+  if (q == 0) return kCostBins - 1;
+  const int lg = min(31 - __clz(q), 27);  // floor(log2 q); a batch holds at most 2^27 queries
+  const int sub = lg >= 2 ? (int)((q >> (lg - 2)) & 3u) : 0;
+  return kCostBins - 2 - (lg * 4 + sub);
+}
 
 __global__ void __launch_bounds__(1024) bwd_plan_init_kernel(int64_t* flags, uint32_t* hcount, int nh) {
   if (threadIdx.x == 0) flags[0] = 0;
@@ -519,24 +532,57 @@ __global__ void __launch_bounds__(kPT) bwd_state_kernel(int64_t M, int nh, const
 }
 
 // Per block, one workgroup: rg[e] = [first, end) of the sorted positions of the queries of the state
-// of rank e, for the block's ranks, and blast[b] = its last rank with queries (-1: none).
+// of rank e, for the block's ranks, blast[b] = its last rank with queries (-1: none), and bq[b] =
+// its number of queries.
 __global__ void __launch_bounds__(256) bwd_rank_groups_kernel(const BwdIndex x, const uint32_t* __restrict__ grp,
-                                                              uint2* __restrict__ rg, int32_t* __restrict__ blast) {
+                                                              uint2* __restrict__ rg, int32_t* __restrict__ blast,
+                                                              uint32_t* __restrict__ bq) {
   __shared__ int last;
+  __shared__ uint32_t nq;
   const int b = blockIdx.x;
   const int e0 = x.blk_e0[b], e1 = x.blk_e0[b + 1];
-  if (threadIdx.x == 0) last = -1;
+  if (threadIdx.x == 0) {
+    last = -1;
+    nq = 0;
+  }
   __syncthreads();
   int mx = -1;
+  uint32_t q = 0;
   for (int e = e0 + (int)threadIdx.x; e < e1; e += 256) {
     const int v = x.steps[x.erank_step[e]];
     const uint2 r = make_uint2(grp[v], grp[v + 1]);
     rg[e] = r;
+    q += r.y - r.x;
     if (r.y > r.x) mx = e;
   }
   if (mx >= 0) atomicMax(&last, mx);
+  if (q) atomicAdd(&nq, q);
   __syncthreads();
-  if (threadIdx.x == 0) blast[b] = last;
+  if (threadIdx.x == 0) {
+    blast[b] = last;
+    bq[b] = nq;
+  }
+}
+
+// One workgroup: order[p] = the block taken p-th, by cost bin (any order inside a bin): the bins'
+// counts and cursors are LDS counters.
+__global__ void __launch_bounds__(1024) bwd_order_kernel(int nblk, const uint32_t* __restrict__ bq,
+                                                         int32_t* __restrict__ order) {
+  __shared__ uint32_t cnt[kCostBins];
+  __shared__ uint32_t wtot[1024 / 64];
+  const int tid = threadIdx.x;
+  if (tid < kCostBins) cnt[tid] = 0;
+  __syncthreads();
+  for (int b = tid; b < nblk; b += 1024) atomicAdd(&cnt[cost_bin(bq[b])], 1u);
+  __syncthreads();
+  uint32_t tot;
+  const uint32_t start = block_excl_scan_u32(tid < kCostBins ? cnt[tid] : 0u, wtot, &tot);
+  if (tid < kCostBins) cnt[tid] = start;
+  __syncthreads();
+  for (int b = tid; b < nblk; b += 1024) {
+    const uint32_t p = atomicAdd(&cnt[cost_bin(bq[b])], 1u);
+    if (p < (uint32_t)nblk) order[p] = b;
+  }
 }
 
 // ---------------------------------------------------------------- persistent block replay
@@ -761,12 +807,15 @@ __device__ void dense_block(const BwdIndex& x, const uint64_t* __restrict__ ends
 
 template <int RPT, bool kSorted>
 __global__ void __launch_bounds__(1024) bwd_block_kernel(const BwdIndex x, const uint64_t* ends, const uint32_t* ids,
-                                                         const uint2* rg, const int32_t* blast, int64_t* out,
-                                                         uint32_t* counter) {
+                                                         const uint2* rg, const int32_t* blast, const int32_t* order,
+                                                         int64_t* out, uint32_t* counter) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ int next;
   for (;;) {
-    if (threadIdx.x == 0) next = (int)atomicAdd(counter, 1u);
+    if (threadIdx.x == 0) {
+      const int i = (int)atomicAdd(counter, 1u);
+      next = i < x.nblk ? order[i] : x.nblk;
+    }
     __syncthreads();
     const int b = __builtin_amdgcn_readfirstlane(next);
     __syncthreads();  // every thread has read `next` before it is taken again
@@ -926,7 +975,8 @@ CarryLayout carry_layout(const BwdIndex* x, int64_t nkeys) {
 // (queries grouped by state: endpoints, query ids), grp, the bucket tables, per rank its state's group (rg) and per block
 // its last queried rank (blast); sans holds the replay's answers in sorted order.
 struct AnswerLayout {
-  size_t kin, ends, vin, ids, grp, rg, blast, counter, cursor, sans, hcount, hcur, hbase, tstart, thist, total;
+  size_t kin, ends, vin, ids, grp, rg, blast, bq, order, counter, cursor, sans, hcount, hcur, hbase, tstart, thist,
+      total;
   int nh;
   unsigned tiles;  // bound on the second level's tiles (its grid)
 };
@@ -947,6 +997,8 @@ hipError_t answer_layout(const BwdIndex* x, int64_t M, AnswerLayout& L) {
   L.grp = take((size_t)(x->S + 2) * 4);
   L.rg = take((size_t)x->S * 8);
   L.blast = take((size_t)x->nblk * 4);
+  L.bq = take((size_t)x->nblk * 4);
+  L.order = take((size_t)x->nblk * 4);
   L.counter = take(4);
   L.cursor = take((size_t)kMaxWin * 4);
   L.sans = take((size_t)M * 8);
@@ -1073,8 +1125,11 @@ extern "C" int bwd_plan_launch(const BwdIndex* x, int64_t M, const int32_t* qsta
                      ids, flags);
   if ((rc = report("plan states", hipGetLastError()))) return rc;
   hipLaunchKernelGGL(bwd_rank_groups_kernel, dim3((unsigned)x->nblk), dim3(256), 0, st, *x, grp, at<uint2>(work, L.rg),
-                     at<int32_t>(work, L.blast));
-  return report("plan rank groups", hipGetLastError());
+                     at<int32_t>(work, L.blast), at<uint32_t>(work, L.bq));
+  if ((rc = report("plan rank groups", hipGetLastError()))) return rc;
+  hipLaunchKernelGGL(bwd_order_kernel, dim3(1), dim3(1024), 0, st, x->nblk, at<const uint32_t>(work, L.bq),
+                     at<int32_t>(work, L.order));
+  return report("plan block order", hipGetLastError());
 }
 
 // The replay's launches; with ev (four events) they are recorded at the phase boundaries: before
@@ -1115,12 +1170,13 @@ static int replay_impl(const BwdIndex* x, int64_t M, int64_t* answers, const Bwd
   const uint32_t* ids = at<uint32_t>(work, L.ids);
   const uint2* rg = at<uint2>(work, L.rg);
   const int32_t* blast = at<int32_t>(work, L.blast);
+  const int32_t* order = at<int32_t>(work, L.order);
   int64_t* out = staged ? at<int64_t>(work, L.sans) : answers;
   const size_t lds = block_lds_layout(spn, nt, x->rmax).total;
   auto mark = [&](int i) { return ev ? hipEventRecord(ev[i], st) : hipSuccess; };
   if ((e = mark(0))) return report("replay event", e);
   auto launch = [&](auto kernel) {
-    hipLaunchKernelGGL(kernel, grid, dim3(nt), lds, st, *x, ends, ids, rg, blast, out, counter);
+    hipLaunchKernelGGL(kernel, grid, dim3(nt), lds, st, *x, ends, ids, rg, blast, order, out, counter);
   };
   if (staged) {
     if (rpt == 4) launch(bwd_block_kernel<4, true>);
