@@ -265,6 +265,10 @@ def main():
                      "step_achieved": round(alg_bytes / (step_ms * 1e-3) / 1e9, 1),
                      "step_frac": round(alg_bytes / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                      "traffic_over_alg": round(traffic / alg_bytes, 2) if traffic else None,
+                     # the measured HBM traffic of a launch (profiles/) over the kernel's time: how busy
+                     # the DRAM is, whatever share of the bytes the algorithm needs
+                     "traffic_GBps": round(traffic / (block_ms * 1e-3) / 1e9, 1) if traffic else None,
+                     "traffic_frac": round(traffic / (block_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
                      "random_access": {
                          "rule": "2 random 8-byte carry-row reads per query; on gfx950 each such read fetches a "
                                  "128-byte line",
