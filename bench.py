@@ -11,9 +11,11 @@ kernel whose workgroups take blocks from a device work queue, over the dense-car
 in HBM (one carry row of 2^20+1 int64 per block of up to 4,096 keys, built once; its build time is
 reported beside); the replay stores the answers in sorted order and two streaming passes (a
 partition by query-id window, a die-local scatter) put them in query-id order, all in a second
-hipGraph.  bw_batch times each step with one pair of HIP events on its stream and checks every
-timed step's answers (device FNV-1a); the answers' SHA-256 must equal the two-level form's (which
-the test suite pins against the oracle).
+hipGraph.  The block size, the replay's workgroup shape and the answer store are the autotuner's
+(the "dense" entry of tuned.json, written by bw_tune --form dense).  bw_batch, in a process of its
+own, times each step with one pair of HIP events on its stream and checks every timed step's
+answers (device FNV-1a); the answers' SHA-256 must equal the two-level form's (which the test
+suite pins against the oracle).
 
 Beside it, measured in the same run (every figure planning-inclusive unless named otherwise):
   two_level -- the same stream over the two-level carry index (blocks of 1,024 keys, level-C rows,
@@ -170,10 +172,15 @@ def main():
             sys.exit("bench.py: native build missing (make -C biworlddb-artifact_amd)")
     knobs = ["--tuned", TUNED] if os.path.exists(TUNED) else []
     threads = host_threads()
+    # the headline step in a process of its own, whose only large allocations are the two indexes
+    # and the stream; the two-level form, the device-built index, the device generator and the
+    # all-on-device leg in a second process over the same stream
+    rd = run([BW_BATCH, "--warmup", str(a.warmup), "--repeat", str(a.steps), "--plain-sample", "0", "--dense"]
+             + knobs, 900)
     r = run([BW_BATCH, "--warmup", str(a.warmup), "--repeat", str(a.steps), "--plain-sample", "64", "--stages",
-             "--dense", "--device-gen", "--device-index", "--device-all"] + knobs, 1200)
-    d = r["dense"]
-    if d["mismatch"] != 0 or d["answers_sha256"] != r["answers_sha256"]:
+             "--device-gen", "--device-index", "--device-all"] + knobs, 1200)
+    d = rd["dense"]
+    if d["mismatch"] != 0 or not d["answers_sha256"] == rd["answers_sha256"] == r["answers_sha256"]:
         sys.exit("bench.py: dense-carry answers differ from the two-level form's")
     plain = run([BW_TREE, "--cfg", "2", "--mode", "pass", "--device-plan", "--warmup", "2", "--repeat", "5",
                  "--extract", "both"] + knobs, 600)
@@ -223,7 +230,9 @@ def main():
                    "states": S, "updates_per_state": U, "keys": r["N"], "batches": r["batches"],
                    "answers_per_step": M, "answers_sha256": d["answers_sha256"],
                    "dense_index": {"rmax": d["rmax"], "blocks": d["nblk"], "carry_bytes": d["carry_bytes"],
-                                   "carry_build_ms": d["carry_build_ms"], "threads": d["threads"]}},
+                                   "carry_build_ms": d["carry_build_ms"], "threads": d["threads"],
+                                   "wg_per_cu": d["wg_per_cu"], "store": d["store"],
+                                   "knobs_from": os.path.basename(d["knobs_from"])}},
         "step": {"plan_ms": d["plan_ms"], "replay_ms": d["replay_ms"],
                  "replay_phases_ms": phases, "min_ms": d["step_ms"]["min"], "max_ms": d["step_ms"]["max"],
                  "every_step_checked": d["step_mismatch"] == 0},

@@ -25,7 +25,8 @@
 // sequence); the answers must be the same.  --device-all then runs the whole pipeline on the device
 // from the seed -- updates and queries generated, index built, stream planned and answered,
 // answers hashed -- timed phase by phase; its answers must be the same too.
-// --dense [--dense-rmax 4096 --dense-threads 1024 --dense-wg 1 --dense-max-wg 0] also answers the
+// --dense [--dense-rmax 4096 --dense-threads 1024 --dense-wg 1 --dense-max-wg 0; with --tuned, the
+// file's "dense" entry (bw_tune --form dense) gives the defaults] also answers the
 // stream with the dense-carry form (include/bwgpu_dense.h): blocks of up to --dense-rmax keys, the
 // persistent replay at --dense-threads threads, --dense-wg workgroups per CU and at most
 // --dense-max-wg workgroups in all (0: no cap), answers stored by --dense-store (0 auto, 1 direct
@@ -117,6 +118,7 @@ int main(int argc, char** argv) {
   int fork = 1;  // replay the block and level-B stages on two streams
   std::string answers_path, json_path, tuned_path;
   bool have_knob[8] = {false, false, false, false, false, false, false, false};
+  bool have_dense[4] = {false, false, false, false};  // --dense-rmax / -threads / -wg / -store given
   for (int i = 1; i < argc; ++i) {
     if (!std::strcmp(argv[i], "--stream-sha")) { want_sha = true; continue; }
     if (!std::strcmp(argv[i], "--stages")) { stages = true; continue; }
@@ -148,11 +150,11 @@ int main(int argc, char** argv) {
     else if (!std::strcmp(k, "--stage-shift")) kn.stage_shift = std::atoi(v);
     else if (!std::strcmp(k, "--bt-table-mb")) kn.bt_table_mb = std::atoll(v);
     else if (!std::strcmp(k, "--fork")) fork = std::atoi(v);
-    else if (!std::strcmp(k, "--dense-rmax")) dense_rmax = std::atoi(v);
-    else if (!std::strcmp(k, "--dense-threads")) dense_threads = std::atoi(v);
-    else if (!std::strcmp(k, "--dense-wg")) dense_wg = std::atoi(v);
+    else if (!std::strcmp(k, "--dense-rmax")) { dense_rmax = std::atoi(v); have_dense[0] = true; }
+    else if (!std::strcmp(k, "--dense-threads")) { dense_threads = std::atoi(v); have_dense[1] = true; }
+    else if (!std::strcmp(k, "--dense-wg")) { dense_wg = std::atoi(v); have_dense[2] = true; }
     else if (!std::strcmp(k, "--dense-max-wg")) dense_maxwg = std::atoi(v);
-    else if (!std::strcmp(k, "--dense-store")) dense_store = std::atoi(v);
+    else if (!std::strcmp(k, "--dense-store")) { dense_store = std::atoi(v); have_dense[3] = true; }
     else if (!std::strcmp(k, "--seg")) kn.seg_chunks = std::atoi(v);
     else if (!std::strcmp(k, "--threads")) { lk.block_threads = std::atoi(v); have_knob[4] = true; }
     else if (!std::strcmp(k, "--waves")) { lk.chunk_waves = std::atoi(v); have_knob[5] = true; }
@@ -170,7 +172,7 @@ int main(int argc, char** argv) {
     return 2;
   }
   // knobs chosen by the autotuner (tree_tune) for this device and workload shape, unless given
-  std::string tuned_from = "defaults";
+  std::string tuned_from = "defaults", dense_from = "defaults";
   if (!tuned_path.empty()) {
     PlanKnobs tk = kn;
     LaunchKnobs tl = lk;
@@ -183,6 +185,14 @@ int main(int argc, char** argv) {
       if (!have_knob[4]) lk.block_threads = tl.block_threads;
       if (!have_knob[5]) lk.chunk_waves = tl.chunk_waves;
       tuned_from = tuned_path;
+    }
+    DenseKnobs dk{dense_rmax, dense_threads, dense_wg, dense_store};
+    if (load_dense_tuned(tuned_path.c_str(), dk)) {  // the dense form's own entry (bw_tune --form dense)
+      if (!have_dense[0]) dense_rmax = dk.rmax;
+      if (!have_dense[1]) dense_threads = dk.threads;
+      if (!have_dense[2]) dense_wg = dk.wg_per_cu;
+      if (!have_dense[3]) dense_store = dk.store;
+      dense_from = tuned_path;
     }
   }
 
@@ -771,13 +781,14 @@ int main(int argc, char** argv) {
     };
     const double sm = mean(step_ms);
     std::snprintf(dense_json, sizeof(dense_json),
-                  "{\"rmax\":%d,\"nblk\":%d,\"threads\":%d,\"wg_per_cu\":%d,\"store\":%d,\"carry_bytes\":%zu,\"carry_build_ms\":%.3f,"
+                  "{\"rmax\":%d,\"nblk\":%d,\"threads\":%d,\"wg_per_cu\":%d,\"store\":%d,\"knobs_from\":\"%s\","
+                  "\"carry_bytes\":%zu,\"carry_build_ms\":%.3f,"
                   "\"index_wall_ms\":%.3f,\"step_ms\":{\"avg\":%.4f,\"min\":%.4f,\"max\":%.4f,\"repeat\":%d},"
                   "\"plan_ms\":%.4f,\"replay_ms\":%.4f,\"replay_phases_ms\":{\"block\":%.4f,\"partition\":%.4f,"
                   "\"scatter\":%.4f},\"queries_per_s\":%.1f,\"answers_sha256\":\"%s\","
                   "\"offset_mismatch\":%" PRId64 ",\"step_mismatch\":%" PRId64 ",\"edge\":{\"queries\":%" PRId64
                   ",\"mismatch\":%" PRId64 "},\"overlap\":%s,\"mismatch\":%" PRId64 "}",
-                  dense_rmax, E4.x.nblk, dense_threads, dense_wg, dense_store, D.carry_bytes, D.build_ms, dense_index_wall, sm,
+                  dense_rmax, E4.x.nblk, dense_threads, dense_wg, dense_store, dense_from.c_str(), D.carry_bytes, D.build_ms, dense_index_wall, sm,
                   *std::min_element(step_ms.begin(), step_ms.end()), *std::max_element(step_ms.begin(), step_ms.end()),
                   repeat, mean(dplan_v), mean(drep_v), phase_ms[0], phase_ms[1], phase_ms[2], double(M) / (sm * 1e-3),
                   dxr.sha256_hex, offset_mismatch, step_mismatch, edge_m, edge_mismatch, overlap_json, dense_mismatch);

@@ -1,12 +1,17 @@
 // bw_tune.cpp -- runs the autotuner (tree_tune) for one workload form on this device and stores
 // the chosen knobs in a tuned-knobs file read by bw_batch / bw_serve (--tuned FILE).
 //
-//   bw_tune --form batched|served [--S n --U n --N n --seed n] [--sample-requests R]
-//           [--repeat 3] [--start FILE] --out FILE
+//   bw_tune --form batched|served|dense [--S n --U n --N n --seed n] [--sample-requests R]
+//           [--repeat 3] [--start FILE] [--budget-gb 200] --out FILE
 // --start: begin the descent from FILE's knobs for this form (else from the defaults).
 // The sample is the stream's first R requests (default: the whole batched stream; 400 served
 // requests, about the size of one served batch): knobs are chosen at the density of queries per
 // block the form actually runs at.
+// --form dense tunes the dense-carry form (include/bwgpu_dense.h) over the whole batched stream:
+// for each block size (2,048 / 3,072 / 4,096 keys) whose carry fits --budget-gb of HBM it builds
+// the index and the carry rows, then times the full step (plan + replay, HIP events, mean of
+// --repeat after one untimed) at every replay shape (256 / 512 / 1,024 threads, 1 or 2 workgroups
+// per CU) and answer store (direct, staged); the fastest is the "dense" entry.
 // One JSON line per trial on stderr, the chosen knobs as one JSON line on stdout.
 #include <hip/hip_runtime_api.h>
 
@@ -17,10 +22,101 @@
 #include <string>
 #include <vector>
 
+#include "bwgpu_dense.h"
+#include "dense_engine.hpp"
+#include "tree_engine.hpp"
 #include "tree_tune.hpp"
 #include "tree_workload.hpp"
 
 using namespace bwg;
+
+namespace {
+
+int tune_dense(const TreeSpec& spec, int repeat, double budget_gb, const std::string& out) {
+  TreeWorkload w;
+  tree_generate(spec, w);
+  const StreamSpec ss = stream_defaults(4);
+  RequestStream rs;
+  stream_generate(w, ss, rs);
+  const int64_t M = rs.nq();
+  if (hipSetDevice(0) != hipSuccess) return 5;
+  DeviceArena io;
+  const int32_t* d_qs = io.upload(rs.qstate);
+  const uint32_t* d_ql = io.upload(rs.qlo);
+  const uint32_t* d_qh = io.upload(rs.qhi);
+  int64_t* d_ans = io.scratch<int64_t>(size_t(M));
+  hipStream_t st;
+  hipEvent_t e0, e1;
+  if (!io.ok() || hipStreamCreate(&st) || hipEventCreate(&e0) || hipEventCreate(&e1)) return 5;
+  DenseKnobs best;
+  double best_ms = -1;
+  int ntrials = 0;
+  char err[256];
+  for (const int rmax : {2048, 3072, 4096}) {
+    PlanKnobs kn;
+    kn.rmax = rmax;
+    TreeEngine E;
+    if (!E.init(spec.S, spec.U, w.coord_n(), w.parent.data(), w.key.data(), w.delta.data(), kn,
+                BwtLaunch{1024, 4, 0, 0}, err, sizeof(err)) ||
+        E.build_index(st) != 0) {
+      std::fprintf(stderr, "bw_tune: rmax %d: index: %s\n", rmax, err);
+      continue;
+    }
+    const double carry_gb = double(E.x.nblk) * double(((int64_t(w.coord_n()) + 1 + 63) / 64) * 64) * 8 / 1e9;
+    if (carry_gb > budget_gb) {
+      std::fprintf(stderr, "{\"rmax\":%d,\"nblk\":%d,\"carry_gb\":%.1f,\"skipped\":\"over the budget\"}\n", rmax,
+                   E.x.nblk, carry_gb);
+      continue;
+    }
+    DenseEngine D;
+    DenseBatch DB;
+    if (!D.init(E, st, err, sizeof(err)) || !alloc_dense_batch(D, M, DB, err, sizeof(err))) {
+      std::fprintf(stderr, "bw_tune: rmax %d: %s\n", rmax, err);
+      continue;
+    }
+    for (const int threads : {256, 512, 1024})
+      for (const int wg : {1, 2})
+        for (const int store : {BWD_STORE_DIRECT, BWD_STORE_STAGED}) {
+          const BwdLaunch l{threads, wg, 0, store};
+          auto step = [&]() {
+            const int rc = bwd_plan_launch(&D.x, M, d_qs, d_ql, d_qh, DB.flags, DB.work, DB.work_bytes, st);
+            return rc ? rc : bwd_replay_launch(&D.x, M, d_ans, &l, DB.work, DB.work_bytes, st);
+          };
+          double ms = -1;
+          if (step() == 0 && hipStreamSynchronize(st) == hipSuccess) {
+            double sum = 0;
+            bool ok = true;
+            for (int r = 0; r < repeat && ok; ++r) {
+              float t = 0;
+              ok = hipEventRecord(e0, st) == hipSuccess && step() == 0 && hipEventRecord(e1, st) == hipSuccess &&
+                   hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&t, e0, e1) == hipSuccess;
+              sum += t;
+            }
+            if (ok) ms = sum / repeat;
+          }
+          ++ntrials;
+          std::fprintf(stderr, "{\"rmax\":%d,\"nblk\":%d,\"threads\":%d,\"wg_per_cu\":%d,\"store\":%d,\"ms\":%.4f}\n",
+                       rmax, E.x.nblk, threads, wg, store, ms);
+          if (ms > 0 && (best_ms < 0 || ms < best_ms)) {
+            best_ms = ms;
+            best = DenseKnobs{rmax, threads, wg, store};
+          }
+        }
+  }
+  if (best_ms <= 0) {
+    std::fprintf(stderr, "bw_tune: no dense configuration ran\n");
+    return 5;
+  }
+  hipDeviceProp_t prop{};
+  (void)hipGetDeviceProperties(&prop, 0);
+  if (!save_dense_tuned(out.c_str(), best, prop.gcnArchName, best_ms)) return 4;
+  std::printf("{\"form\":\"dense\",\"queries\":%lld,\"trials\":%d,\"rmax\":%d,\"threads\":%d,\"wg_per_cu\":%d,"
+              "\"store\":%d,\"ms\":%.4f}\n",
+              (long long)M, ntrials, best.rmax, best.threads, best.wg_per_cu, best.store, best_ms);
+  return 0;
+}
+
+}  // namespace
 
 int main(int argc, char** argv) {
   std::string form = "batched", out, start;
@@ -30,7 +126,8 @@ int main(int argc, char** argv) {
   for (int i = 1; i + 1 < argc; ++i)
     if (!std::strcmp(argv[i], "--form")) form = argv[i + 1];
   if (form == "served") cfg_form = 5;
-  else if (form != "batched") { std::fprintf(stderr, "bw_tune: --form batched|served\n"); return 2; }
+  else if (form != "batched" && form != "dense") { std::fprintf(stderr, "bw_tune: --form batched|served|dense\n"); return 2; }
+  double budget_gb = 200;
   TreeSpec spec = tree_defaults(cfg_form == 4 ? 4 : 2);
   for (int i = 1; i < argc; ++i) {
     if (i + 1 >= argc) { std::fprintf(stderr, "bw_tune: missing value for %s\n", argv[i]); return 2; }
@@ -45,10 +142,12 @@ int main(int argc, char** argv) {
     else if (!std::strcmp(k, "--repeat")) repeat = std::atoi(v);
     else if (!std::strcmp(k, "--start")) start = v;
     else if (!std::strcmp(k, "--out")) out = v;
+    else if (!std::strcmp(k, "--budget-gb")) budget_gb = std::atof(v);
     else { std::fprintf(stderr, "bw_tune: unknown argument %s\n", k); return 2; }
   }
   if (sample < 0) sample = cfg_form == 4 ? stream_defaults(4).nreq : 400;
   if (out.empty() || sample < 1 || repeat < 1) { std::fprintf(stderr, "bw_tune: --out FILE required\n"); return 2; }
+  if (form == "dense") return tune_dense(spec, repeat, budget_gb, out);
   TreeWorkload w;
   tree_generate(spec, w);
   StreamSpec ss = stream_defaults(cfg_form);

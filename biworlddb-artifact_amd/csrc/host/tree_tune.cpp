@@ -94,15 +94,16 @@ bool load_tuned(const char* path, const char* form, PlanKnobs& kn, LaunchKnobs& 
   return true;
 }
 
-bool save_tuned(const char* path, const char* form, const PlanKnobs& kn, const LaunchKnobs& lk, const char* device,
-                double ms) {
+namespace {
+
+// Rewrites the file with `form`'s entry replaced by `entry`, keeping the other forms' entries.
+bool save_entry(const char* path, const char* form, const std::string& entry) {
   const std::string text = read_file(path);
-  // keep the other forms' entries
-  const char* forms[] = {"batched", "served", "plain", "plain3"};
+  const char* forms[] = {"batched", "served", "plain", "plain3", "dense"};
   std::string out = "{\n";
   bool first = true;
   for (const char* f : forms) {
-    std::string e = std::strcmp(f, form) == 0 ? entry_text(kn, lk, device, ms) : entry_of(text, f);
+    std::string e = std::strcmp(f, form) == 0 ? entry : entry_of(text, f);
     if (e.empty()) continue;
     out += (first ? "  \"" : ",\n  \"") + std::string(f) + "\": " + e;
     first = false;
@@ -112,6 +113,34 @@ bool save_tuned(const char* path, const char* form, const PlanKnobs& kn, const L
   if (!fp) return false;
   const bool ok = std::fwrite(out.data(), 1, out.size(), fp) == out.size();
   return std::fclose(fp) == 0 && ok;
+}
+
+}  // namespace
+
+bool save_tuned(const char* path, const char* form, const PlanKnobs& kn, const LaunchKnobs& lk, const char* device,
+                double ms) {
+  return save_entry(path, form, entry_text(kn, lk, device, ms));
+}
+
+bool load_dense_tuned(const char* path, DenseKnobs& dk) {
+  const std::string obj = entry_of(read_file(path), "dense");
+  if (obj.empty()) return false;
+  double v;
+  DenseKnobs k = dk;
+  if (number_of(obj, "rmax", v)) k.rmax = int(v);
+  if (number_of(obj, "threads", v)) k.threads = int(v);
+  if (number_of(obj, "wg_per_cu", v)) k.wg_per_cu = int(v);
+  if (number_of(obj, "store", v)) k.store = int(v);
+  dk = k;
+  return true;
+}
+
+bool save_dense_tuned(const char* path, const DenseKnobs& dk, const char* device, double ms) {
+  char b[256];
+  std::snprintf(b, sizeof(b),
+                "{\"rmax\": %d, \"threads\": %d, \"wg_per_cu\": %d, \"store\": %d, \"ms\": %.4f, \"device\": \"%s\"}",
+                dk.rmax, dk.threads, dk.wg_per_cu, dk.store, ms, device ? device : "");
+  return save_entry(path, "dense", b);
 }
 
 TuneTrial autotune(const TreeWorkload& w, const uint32_t* kcoord, int64_t M, const int32_t* qstate,

@@ -24,6 +24,18 @@ bool load_tuned(const char* path, const char* form, PlanKnobs& kn, LaunchKnobs& 
 bool save_tuned(const char* path, const char* form, const PlanKnobs& kn, const LaunchKnobs& lk, const char* device,
                 double ms);
 
+// Knobs of the dense-carry form (include/bwgpu_dense.h): the block size (keys per block, which sets
+// the carry's size), the persistent replay's workgroup threads and workgroups per CU, and how the
+// answers reach their ids (BWD_STORE_*).  Stored as the "dense" entry of the same file.
+struct DenseKnobs {
+  int rmax = 4096;
+  int threads = 1024;
+  int wg_per_cu = 1;
+  int store = 0;
+};
+bool load_dense_tuned(const char* path, DenseKnobs& dk);
+bool save_dense_tuned(const char* path, const DenseKnobs& dk, const char* device, double ms);
+
 struct TuneTrial {
   PlanKnobs kn;
   LaunchKnobs lk;

@@ -112,6 +112,7 @@ def test_all_device_pipeline(pkg, case, knobs):
     ("s2_batched_mid", ["--dense-store", "1"]),
     ("s2_batched_mid", ["--dense-rmax", "1024", "--dense-threads", "512"]),
     ("s2_batched_mid", ["--dense-rmax", "2048", "--dense-threads", "256", "--device-gen"]),
+    ("s2_batched_mid", ["--dense-threads", "512", "--dense-wg", "2", "--dense-store", "2"]),
     ("S4_batched_full", ["--tuned", None]),
 ])
 def test_dense_form_matches_oracle(pkg, case, knobs):
@@ -131,3 +132,5 @@ def test_dense_form_matches_oracle(pkg, case, knobs):
     assert d["mismatch"] == 0 and d["answers_sha256"] == g["oracle"], (case, knobs, d)
     assert d["edge"]["mismatch"] == 0 and d["edge"]["queries"] == min(g["nq"], 1 << 20), d["edge"]
     assert r["answers_sha256"] == g["oracle"]
+    if pkg.TUNED in knobs:  # the tuned file's dense entry is the one the run used
+        assert d["knobs_from"] == pkg.TUNED, d
