@@ -7,9 +7,9 @@
 //                       weights (agg) become per-block prefix functions D_b(x); rows are their
 //                       running sum over blocks, computed per (coordinate range, block segment)
 //                       with a scan over segments in between, so ~45k workgroups share the work.
-//   plan             <- qlb/qhb (src/batch_engine_v7.hpp:666-698): queries grouped by the preorder
-//                       rank of their state (a two-level bucket sort: high rank digit, then low
-//                       digit inside each bucket), group offsets per rank.
+//   plan             <- qlb/qhb (src/batch_engine_v7.hpp:666-698): queries grouped by their state
+//                       id (a two-level bucket sort: high digit, then low digit inside each
+//                       bucket), group offsets per preorder rank.
 //   persistent block <- v7_block_kernel + gpu_combine_kernel (src/gpu_v7.cu:10-96): workgroups
 //                       take blocks from a device queue; each replays its block's steps (events
 //                       are ds_add_u64 into a dense per-rank array; at every enter step a DPP
@@ -22,6 +22,7 @@
 
 #include <algorithm>
 #include <mutex>
+#include <vector>
 #include <stdint.h>
 #include <stdio.h>
 
@@ -606,7 +607,7 @@ struct BlockLds {
   size_t w, P, wtot, skey, gq, sdel, srank, sstep, total;
 };
 __host__ __device__ constexpr size_t al16(size_t v) { return (v + 15) & ~size_t(15); }
-__host__ __device__ inline BlockLds block_lds_layout(int span, int nt, int rmax) {
+__host__ __device__ inline BlockLds block_lds_layout(int span, int nt, int rmax, int U) {
   BlockLds L{};
   L.w = 0;
   L.P = al16(L.w + (size_t)span * 8);
@@ -616,20 +617,21 @@ __host__ __device__ inline BlockLds block_lds_layout(int span, int nt, int rmax)
   L.sdel = al16(L.gq + 2 * kGCap * 4);
   L.srank = al16(L.sdel + (size_t)kEPT * nt * 8);
   L.sstep = al16(L.srank + (size_t)kEPT * nt * 2);
-  L.total = al16(L.sstep + (size_t)kEPT * nt * 4);
+  L.total = al16(L.sstep + (size_t)((kEPT * nt) / U + 1) * 4);  // one step id per step of a tile
   return L;
 }
 
 // kSorted: each answer goes to its query's sorted position (out[M], runs of consecutive positions
 // per state; the unsort kernels below put them in query-id order); otherwise to out[query id].
-template <int RPT, bool kSorted>
+template <int RPT, bool kSorted, int NT>
 __device__ void dense_block(const BwdIndex& x, const uint64_t* __restrict__ ends, const uint32_t* __restrict__ qids,
                             const uint2* __restrict__ rg, const int32_t* __restrict__ blast,
-                            int64_t* __restrict__ out, const int b,
+                            int64_t* __restrict__ out, const int b, const int ea_item, const int off, const int cnt,
                             unsigned char* smem) {
-  const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wv = tid >> 6;
+  constexpr int nt = NT;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int span = span_of<RPT>(nt);
-  const BlockLds lay = block_lds_layout(span, nt, x.rmax);
+  const BlockLds lay = block_lds_layout(span, nt, x.rmax, x.U);
   int64_t* w = reinterpret_cast<int64_t*>(smem + lay.w);          // deltas since the last scan, per rank
   int64_t* P = reinterpret_cast<int64_t*>(smem + lay.P);          // exclusive prefix over ranks
   int64_t* wtot = reinterpret_cast<int64_t*>(smem + lay.wtot);    // 16 wavefront totals
@@ -646,6 +648,12 @@ __device__ void dense_block(const BwdIndex& x, const uint64_t* __restrict__ ends
   const int el = __builtin_amdgcn_readfirstlane(blast[b]);
   if (el < 0) return;
   const int s1 = __builtin_amdgcn_readfirstlane(x.erank_step[el]) + 1;
+  // the work item: its queries start at offset `off` into the group of rank ea and number cnt; an
+  // item past the block's first rank starts its walk at ea's enter step ss, the events of the
+  // steps before it applied first in any order (the adds commute)
+  const int ea = ea_item < 0 ? e0 : ea_item;
+  const int ss = ea == e0 ? s0 : __builtin_amdgcn_readfirstlane(x.erank_step[ea]);
+  int left = cnt;
   const int64_t k0 = uniform64(x.blk_key0[b]);
   const int R = (int)(uniform64(x.blk_key0[b + 1]) - k0);
   const int U = x.U;
@@ -653,7 +661,19 @@ __device__ void dense_block(const BwdIndex& x, const uint64_t* __restrict__ ends
 
   for (int i = tid; i < span; i += nt) w[i] = 0;
   for (int i = tid; i < R; i += nt) skey[i] = x.bkey[k0 + i];
-  int gwin = e0;  // gq[i], gq[kGCap + i] = the group of rank gwin + i
+  if (ss > s0) {
+    __syncthreads();  // w is zero
+    const int np = (ss - s0) * U;
+    const int64_t nu = x.S * (int64_t)U;
+    for (int i = tid; i < np; i += nt) {
+      const int s = s0 + i / U, k = i - (i / U) * U;
+      const int sv = x.steps[s];
+      const int64_t at = ((int64_t)(sv > 0 ? sv : -sv) - 1) * U + k;
+      const int64_t d = (at >= 0 && at < nu) ? x.delta[at] : 0;
+      lds_add64(&w[pslot<RPT>(x.evrank[(int64_t)s * U + k])], sv > 0 ? d : -d);
+    }
+  }
+  int gwin = ea;  // gq[i], gq[kGCap + i] = the group of rank gwin + i
   auto stage_groups = [&](int g) {
     for (int i = tid; i < kGCap; i += nt) {
       const uint2 r = rg[min(g + i, e1 - 1)];
@@ -661,7 +681,7 @@ __device__ void dense_block(const BwdIndex& x, const uint64_t* __restrict__ ends
       gq[kGCap + i] = r.y;
     }
   };
-  stage_groups(e0);
+  stage_groups(ea);
   int64_t tot[RPT];
 #pragma unroll
   for (int j = 0; j < RPT; ++j) tot[j] = 0;
@@ -698,9 +718,10 @@ __device__ void dense_block(const BwdIndex& x, const uint64_t* __restrict__ ends
       stage_groups(e);
       __syncthreads();
     }
-    const uint32_t qa = gq[e - gwin], qb = gq[kGCap + e - gwin];
-    const int nq = (int)(qb - qa);
-    if (nq == 0) return;
+    const uint32_t qa = gq[e - gwin] + (e == ea ? (uint32_t)off : 0u), qb = gq[kGCap + e - gwin];
+    const int nq = min((int)(qb - qa), left);
+    if (nq <= 0) return;
+    left -= nq;
     // first round: the queries' items and carry-row reads are issued before the scan, whose
     // barriers hide their latency
     uint32_t lo = 0, hi = 0;
@@ -770,15 +791,16 @@ __device__ void dense_block(const BwdIndex& x, const uint64_t* __restrict__ ends
   int64_t d_n[kEPT];
   bool neg_n[kEPT];
   // prologue: tile 0 staged, tile 1's events in flight, tile 2's step ids in flight
-  ids(s0, st);
-  events(s0, st, r_n, d_n, neg_n);
-  stage_events(s0, st, r_n, d_n, neg_n);
-  ids(s0 + T, stn);
-  events(s0 + T, stn, r_n, d_n, neg_n);
-  ids(s0 + 2 * T, st);
+  ids(ss, st);
+  events(ss, st, r_n, d_n, neg_n);
+  stage_events(ss, st, r_n, d_n, neg_n);
+  ids(ss + T, stn);
+  events(ss + T, stn, r_n, d_n, neg_n);
+  ids(ss + 2 * T, st);
   __syncthreads();
-  int e = e0;  // rank of the next entered state
-  for (int t0 = s0; t0 < s1; t0 += T) {
+  int e = ea;  // rank of the next entered state
+  bool done = false;  // the item's queries are answered
+  for (int t0 = ss; t0 < s1; t0 += T) {
     const int t1 = min(t0 + T, s1);
     for (int s = t0; s < t1;) {
       // the steps up to the next enter step (or the tile's end) in one pass: the adds commute,
@@ -791,10 +813,14 @@ __device__ void dense_block(const BwdIndex& x, const uint64_t* __restrict__ ends
       if (se < t1) {
         query(e);
         ++e;
+        if (left == 0) {  // uniform
+          done = true;
+          break;
+        }
       }
       s = last + 1;
     }
-    if (t1 >= s1) break;
+    if (done || t1 >= s1) break;
     __syncthreads();  // the stage is free
     stage_events(t0 + T, stn, r_n, d_n, neg_n);
     __syncthreads();
@@ -805,23 +831,80 @@ __device__ void dense_block(const BwdIndex& x, const uint64_t* __restrict__ ends
   }
 }
 
-template <int RPT, bool kSorted>
-__global__ void __launch_bounds__(1024) bwd_block_kernel(const BwdIndex x, const uint64_t* ends, const uint32_t* ids,
+// Work queue: whole blocks in `order` (items null), or the work items (block, first rank, offset
+// into its group, query count), *nitems of them.
+template <int RPT, bool kSorted, int NT>
+__global__ void __launch_bounds__(NT) bwd_block_kernel(const BwdIndex x, const uint64_t* ends, const uint32_t* ids,
                                                          const uint2* rg, const int32_t* blast, const int32_t* order,
-                                                         int64_t* out, uint32_t* counter) {
+                                                         const int4* items, const uint32_t* nitems, int64_t* out,
+                                                         uint32_t* counter) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  __shared__ int next;
+  __shared__ int4 next;
   for (;;) {
     if (threadIdx.x == 0) {
-      const int i = (int)atomicAdd(counter, 1u);
-      next = i < x.nblk ? order[i] : x.nblk;
+      const uint32_t i = atomicAdd(counter, 1u);
+      if (items) next = i < *nitems ? items[i] : make_int4(x.nblk, 0, 0, 0);
+      else next = make_int4(i < (uint32_t)x.nblk ? order[i] : x.nblk, -1, 0, 0x7fffffff);
     }
     __syncthreads();
-    const int b = __builtin_amdgcn_readfirstlane(next);
+    const int b = __builtin_amdgcn_readfirstlane(next.x), ea = __builtin_amdgcn_readfirstlane(next.y);
+    const int off = __builtin_amdgcn_readfirstlane(next.z), cnt = __builtin_amdgcn_readfirstlane(next.w);
     __syncthreads();  // every thread has read `next` before it is taken again
     if (b >= x.nblk) break;
-    dense_block<RPT, kSorted>(x, ends, ids, rg, blast, out, b, smem_raw);
+    dense_block<RPT, kSorted, NT>(x, ends, ids, rg, blast, out, b, ea, off, cnt, smem_raw);
     __syncthreads();  // the next block reuses the LDS
+  }
+}
+
+// One workgroup: block order[p] gets ceil(bq / chunk) work items (none without queries), the first
+// at ibase[p]; ipos[b] = b's place in the order; ibase[nblk] = the item count.
+__global__ void __launch_bounds__(1024) bwd_items_base_kernel(int nblk, uint32_t chunk, const int32_t* __restrict__ order,
+                                                              const uint32_t* __restrict__ bq, int32_t* __restrict__ ipos,
+                                                              uint32_t* __restrict__ ibase) {
+  __shared__ uint32_t wtot[1024 / 64];
+  uint32_t carry = 0;
+  for (int p0 = 0; p0 < nblk; p0 += 1024) {
+    const int p = p0 + (int)threadIdx.x;
+    uint32_t k = 0;
+    if (p < nblk) {
+      const int b = order[p];
+      ipos[b] = p;
+      k = (bq[b] + chunk - 1) / chunk;
+    }
+    uint32_t tot;
+    const uint32_t ex = block_excl_scan_u32(k, wtot, &tot);
+    if (p < nblk) ibase[p] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) ibase[nblk] = carry;
+}
+
+// One wavefront per block: its ranks in walk order and their queries' running count; item j of the
+// block starts at the rank holding the block's query j * chunk (a rank's queries in group order),
+// at that offset into the rank's group, and holds min(chunk, the rest) queries.
+__global__ void __launch_bounds__(256) bwd_items_kernel(const BwdIndex x, uint32_t chunk, const uint2* __restrict__ rg,
+                                                        const uint32_t* __restrict__ bq, const int32_t* __restrict__ ipos,
+                                                        const uint32_t* __restrict__ ibase, int4* __restrict__ items) {
+  const int lane = threadIdx.x & 63;
+  const int b = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+  if (b >= x.nblk) return;
+  const uint32_t nqb = bq[b];
+  if (nqb == 0) return;
+  const uint32_t base = ibase[ipos[b]];
+  const int e0 = x.blk_e0[b], e1 = x.blk_e0[b + 1];
+  uint32_t cs = 0;  // queries of the ranks before this round of 64
+  for (int r0 = e0; r0 < e1; r0 += 64) {
+    const int e = r0 + lane;
+    uint32_t c = 0;
+    if (e < e1) {
+      const uint2 r = rg[e];
+      c = r.y - r.x;
+    }
+    const uint32_t inc = (uint32_t)wave_incl_scan((int64_t)c);
+    const uint32_t q = cs + inc - c;  // the block's ordinal of this rank's first query
+    for (uint32_t j = (q + chunk - 1) / chunk; j * chunk < q + c; ++j)
+      items[base + j] = make_int4(b, e, (int)(j * chunk - q), (int)min(chunk, nqb - j * chunk));
+    cs += (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
   }
 }
 
@@ -976,7 +1059,7 @@ CarryLayout carry_layout(const BwdIndex* x, int64_t nkeys) {
 // its last queried rank (blast); sans holds the replay's answers in sorted order.
 struct AnswerLayout {
   size_t kin, ends, vin, ids, grp, rg, blast, bq, order, counter, cursor, sans, hcount, hcur, hbase, tstart, thist,
-      total;
+      items, ipos, ibase, total;
   int nh;
   unsigned tiles;  // bound on the second level's tiles (its grid)
 };
@@ -1007,6 +1090,10 @@ hipError_t answer_layout(const BwdIndex* x, int64_t M, AnswerLayout& L) {
   L.hbase = take((size_t)(kMaxHi + 1) * 4);
   L.tstart = take((size_t)(kMaxHi + 1) * 4);
   L.thist = take((size_t)L.tiles * kLoBins * 4);
+  // work items: at most one per block plus one per BWD_MIN_CHUNK queries
+  L.items = take(((size_t)x->nblk + (size_t)(M / BWD_MIN_CHUNK) + 1) * 16);
+  L.ipos = take((size_t)x->nblk * 4);
+  L.ibase = take((size_t)(x->nblk + 1) * 4);
   L.total = o;
   return hipSuccess;
 }
@@ -1016,16 +1103,28 @@ T* at(void* base, size_t off) {
   return reinterpret_cast<T*>(static_cast<unsigned char*>(base) + off);
 }
 
+// the block replay's instances: [sorted][ranks per thread 4 / 8 / 16][threads 256 / 512 / 1024],
+// each compiled for its thread count (its register budget)
+using BlockKernel = void (*)(const BwdIndex, const uint64_t*, const uint32_t*, const uint2*, const int32_t*,
+                             const int32_t*, const int4*, const uint32_t*, int64_t*, uint32_t*);
+template <bool kS, int RPT>
+constexpr BlockKernel kBlockRow[3] = {bwd_block_kernel<RPT, kS, 256>, bwd_block_kernel<RPT, kS, 512>,
+                                      bwd_block_kernel<RPT, kS, 1024>};
+BlockKernel block_kernel(bool sorted, int rpt, int nt) {
+  const int r = rpt == 4 ? 0 : rpt == 8 ? 1 : 2, t = nt == 256 ? 0 : nt == 512 ? 1 : 2;
+  static const BlockKernel* const tab[2][3] = {{kBlockRow<false, 4>, kBlockRow<false, 8>, kBlockRow<false, 16>},
+                                               {kBlockRow<true, 4>, kBlockRow<true, 8>, kBlockRow<true, 16>}};
+  return tab[sorted ? 1 : 0][r][t];
+}
+
 int set_lds_limits() {
   static std::once_flag once;
   static hipError_t err = hipSuccess;
   std::call_once(once, [] {
-    const void* ks[] = {reinterpret_cast<const void*>(bwd_block_kernel<4, false>),
-                        reinterpret_cast<const void*>(bwd_block_kernel<8, false>),
-                        reinterpret_cast<const void*>(bwd_block_kernel<16, false>),
-                        reinterpret_cast<const void*>(bwd_block_kernel<4, true>),
-                        reinterpret_cast<const void*>(bwd_block_kernel<8, true>),
-                        reinterpret_cast<const void*>(bwd_block_kernel<16, true>)};
+    std::vector<const void*> ks;
+    for (int sorted = 0; sorted < 2; ++sorted)
+      for (int rpt : {4, 8, 16})
+        for (int nt : {256, 512, 1024}) ks.push_back(reinterpret_cast<const void*>(block_kernel(sorted, rpt, nt)));
     for (const void* k : ks) {
       const hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, kBlockLdsLimit);
       if (e != hipSuccess && err == hipSuccess) err = e;
@@ -1140,14 +1239,14 @@ static int replay_impl(const BwdIndex* x, int64_t M, int64_t* answers, const Bwd
   const int nt = l->threads;
   if (nt != 256 && nt != 512 && nt != 1024) return (int)hipErrorInvalidValue;
   if (l->wg_per_cu < 1 || l->wg_per_cu > 8 || l->max_workgroups < 0 || l->store < BWD_STORE_AUTO ||
-      l->store > BWD_STORE_STAGED)
+      l->store > BWD_STORE_STAGED || l->chunk < 0 || (l->chunk > 0 && l->chunk < BWD_MIN_CHUNK))
     return (int)hipErrorInvalidValue;
   // ranks per thread: the smallest of 4, 8, 16 with rpt * nt >= rmax
   const int need = (x->rmax + nt - 1) / nt;
   const int rpt = need <= 4 ? 4 : need <= 8 ? 8 : 16;
   if (need > 16 || x->U > kEPT * nt) return (int)hipErrorInvalidValue;
   const int spn = rpt == 4 ? span_of<4>(nt) : rpt == 8 ? span_of<8>(nt) : span_of<16>(nt);
-  if (block_lds_layout(spn, nt, x->rmax).total > (size_t)kBlockLdsLimit) return (int)hipErrorInvalidValue;
+  if (block_lds_layout(spn, nt, x->rmax, x->U).total > (size_t)kBlockLdsLimit) return (int)hipErrorInvalidValue;
   AnswerLayout L;
   hipError_t e = answer_layout(x, M, L);
   if (e) return report("answer workspace", e);
@@ -1163,7 +1262,14 @@ static int replay_impl(const BwdIndex* x, int64_t M, int64_t* answers, const Bwd
   uint32_t* counter = at<uint32_t>(work, L.counter);
   uint32_t* cursor = at<uint32_t>(work, L.cursor);
   hipLaunchKernelGGL(bwd_replay_init_kernel, dim3(1), dim3(256), 0, st, counter, staged ? cursor : nullptr, nwin);
-  int nwg = ncu * l->wg_per_cu;
+  const size_t lds = block_lds_layout(spn, nt, x->rmax, x->U).total;
+  const BlockKernel bk = block_kernel(staged, rpt, nt);
+  // workgroups per CU: as asked, at most as many as the instance's registers and LDS let reside
+  // (a workgroup beyond them would only start once the queue is empty)
+  int resident = 0;
+  if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, reinterpret_cast<const void*>(bk), nt, lds)))
+    return report("replay occupancy", e);
+  int nwg = ncu * std::max(1, std::min(l->wg_per_cu, resident));
   if (l->max_workgroups > 0 && l->max_workgroups < nwg) nwg = l->max_workgroups;
   const dim3 grid((unsigned)nwg);
   const uint64_t* ends = at<uint64_t>(work, L.ends);
@@ -1172,21 +1278,26 @@ static int replay_impl(const BwdIndex* x, int64_t M, int64_t* answers, const Bwd
   const int32_t* blast = at<int32_t>(work, L.blast);
   const int32_t* order = at<int32_t>(work, L.order);
   int64_t* out = staged ? at<int64_t>(work, L.sans) : answers;
-  const size_t lds = block_lds_layout(spn, nt, x->rmax).total;
   auto mark = [&](int i) { return ev ? hipEventRecord(ev[i], st) : hipSuccess; };
   if ((e = mark(0))) return report("replay event", e);
-  auto launch = [&](auto kernel) {
-    hipLaunchKernelGGL(kernel, grid, dim3(nt), lds, st, *x, ends, ids, rg, blast, order, out, counter);
-  };
-  if (staged) {
-    if (rpt == 4) launch(bwd_block_kernel<4, true>);
-    else if (rpt == 8) launch(bwd_block_kernel<8, true>);
-    else launch(bwd_block_kernel<16, true>);
-  } else {
-    if (rpt == 4) launch(bwd_block_kernel<4, false>);
-    else if (rpt == 8) launch(bwd_block_kernel<8, false>);
-    else launch(bwd_block_kernel<16, false>);
+  const int4* items = nullptr;
+  const uint32_t* nitems = nullptr;
+  if (l->chunk > 0) {  // the work items, in the blocks' order
+    const uint32_t* bq = at<const uint32_t>(work, L.bq);
+    int32_t* ipos = at<int32_t>(work, L.ipos);
+    uint32_t* ibase = at<uint32_t>(work, L.ibase);
+    hipLaunchKernelGGL(bwd_items_base_kernel, dim3(1), dim3(1024), 0, st, x->nblk, (uint32_t)l->chunk, order, bq, ipos,
+                       ibase);
+    hipLaunchKernelGGL(bwd_items_kernel, dim3((unsigned)((x->nblk + 3) / 4)), dim3(256), 0, st, *x,
+                       (uint32_t)l->chunk, rg, bq, ipos, ibase, at<int4>(work, L.items));
+    if ((rc = report("replay work items", hipGetLastError()))) return rc;
+    items = at<const int4>(work, L.items);
+    nitems = ibase + x->nblk;
   }
+  auto launch = [&](auto kernel) {
+    hipLaunchKernelGGL(kernel, grid, dim3(nt), lds, st, *x, ends, ids, rg, blast, order, items, nitems, out, counter);
+  };
+  launch(bk);
   if ((rc = report("dense block replay", hipGetLastError()))) return rc;
   if ((e = mark(1))) return report("replay event", e);
   if (!staged) {

@@ -113,12 +113,12 @@ int main(int argc, char** argv) {
   int warmup = 1, repeat = 3, plain_sample = 8;
   bool want_sha = false, stages = false, dev_plan = false, dev_gen = false, dev_index = false, dev_all = false;
   bool dense = false, noalloc_check = false;
-  int dense_rmax = 4096, dense_threads = 1024, dense_wg = 1, dense_maxwg = 0, dense_store = BWD_STORE_AUTO;
+  int dense_rmax = 4096, dense_threads = 1024, dense_wg = 1, dense_maxwg = 0, dense_store = BWD_STORE_AUTO, dense_chunk = 0;
   bool dense_overlap = false;
   int fork = 1;  // replay the block and level-B stages on two streams
   std::string answers_path, json_path, tuned_path;
   bool have_knob[8] = {false, false, false, false, false, false, false, false};
-  bool have_dense[4] = {false, false, false, false};  // --dense-rmax / -threads / -wg / -store given
+  bool have_dense[5] = {false, false, false, false, false};  // --dense-rmax / -threads / -wg / -store / -chunk given
   for (int i = 1; i < argc; ++i) {
     if (!std::strcmp(argv[i], "--stream-sha")) { want_sha = true; continue; }
     if (!std::strcmp(argv[i], "--stages")) { stages = true; continue; }
@@ -155,6 +155,7 @@ int main(int argc, char** argv) {
     else if (!std::strcmp(k, "--dense-wg")) { dense_wg = std::atoi(v); have_dense[2] = true; }
     else if (!std::strcmp(k, "--dense-max-wg")) dense_maxwg = std::atoi(v);
     else if (!std::strcmp(k, "--dense-store")) { dense_store = std::atoi(v); have_dense[3] = true; }
+    else if (!std::strcmp(k, "--dense-chunk")) { dense_chunk = std::atoi(v); have_dense[4] = true; }
     else if (!std::strcmp(k, "--seg")) kn.seg_chunks = std::atoi(v);
     else if (!std::strcmp(k, "--threads")) { lk.block_threads = std::atoi(v); have_knob[4] = true; }
     else if (!std::strcmp(k, "--waves")) { lk.chunk_waves = std::atoi(v); have_knob[5] = true; }
@@ -186,12 +187,13 @@ int main(int argc, char** argv) {
       if (!have_knob[5]) lk.chunk_waves = tl.chunk_waves;
       tuned_from = tuned_path;
     }
-    DenseKnobs dk{dense_rmax, dense_threads, dense_wg, dense_store};
+    DenseKnobs dk{dense_rmax, dense_threads, dense_wg, dense_store, dense_chunk};
     if (load_dense_tuned(tuned_path.c_str(), dk)) {  // the dense form's own entry (bw_tune --form dense)
       if (!have_dense[0]) dense_rmax = dk.rmax;
       if (!have_dense[1]) dense_threads = dk.threads;
       if (!have_dense[2]) dense_wg = dk.wg_per_cu;
       if (!have_dense[3]) dense_store = dk.store;
+      if (!have_dense[4]) dense_chunk = dk.chunk;
       dense_from = tuned_path;
     }
   }
@@ -509,7 +511,7 @@ int main(int argc, char** argv) {
     }
     RUN(E4.build_index(st));
     DenseEngine D;
-    D.launch = BwdLaunch{dense_threads, dense_wg, dense_overlap ? 0 : dense_maxwg, dense_store};
+    D.launch = BwdLaunch{dense_threads, dense_wg, dense_overlap ? 0 : dense_maxwg, dense_store, dense_chunk};
     if (!D.init(E4, st, err, sizeof(err))) {
       std::fprintf(stderr, "bw_batch: %s\n", err);
       return 5;
@@ -781,14 +783,14 @@ int main(int argc, char** argv) {
     };
     const double sm = mean(step_ms);
     std::snprintf(dense_json, sizeof(dense_json),
-                  "{\"rmax\":%d,\"nblk\":%d,\"threads\":%d,\"wg_per_cu\":%d,\"store\":%d,\"knobs_from\":\"%s\","
+                  "{\"rmax\":%d,\"nblk\":%d,\"threads\":%d,\"wg_per_cu\":%d,\"store\":%d,\"chunk\":%d,\"knobs_from\":\"%s\","
                   "\"carry_bytes\":%zu,\"carry_build_ms\":%.3f,"
                   "\"index_wall_ms\":%.3f,\"step_ms\":{\"avg\":%.4f,\"min\":%.4f,\"max\":%.4f,\"repeat\":%d},"
                   "\"plan_ms\":%.4f,\"replay_ms\":%.4f,\"replay_phases_ms\":{\"block\":%.4f,\"partition\":%.4f,"
                   "\"scatter\":%.4f},\"queries_per_s\":%.1f,\"answers_sha256\":\"%s\","
                   "\"offset_mismatch\":%" PRId64 ",\"step_mismatch\":%" PRId64 ",\"edge\":{\"queries\":%" PRId64
                   ",\"mismatch\":%" PRId64 "},\"overlap\":%s,\"mismatch\":%" PRId64 "}",
-                  dense_rmax, E4.x.nblk, dense_threads, dense_wg, dense_store, dense_from.c_str(), D.carry_bytes, D.build_ms, dense_index_wall, sm,
+                  dense_rmax, E4.x.nblk, dense_threads, dense_wg, dense_store, dense_chunk, dense_from.c_str(), D.carry_bytes, D.build_ms, dense_index_wall, sm,
                   *std::min_element(step_ms.begin(), step_ms.end()), *std::max_element(step_ms.begin(), step_ms.end()),
                   repeat, mean(dplan_v), mean(drep_v), phase_ms[0], phase_ms[1], phase_ms[2], double(M) / (sm * 1e-3),
                   dxr.sha256_hex, offset_mismatch, step_mismatch, edge_m, edge_mismatch, overlap_json, dense_mismatch);

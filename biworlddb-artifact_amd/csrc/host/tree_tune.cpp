@@ -131,6 +131,7 @@ bool load_dense_tuned(const char* path, DenseKnobs& dk) {
   if (number_of(obj, "threads", v)) k.threads = int(v);
   if (number_of(obj, "wg_per_cu", v)) k.wg_per_cu = int(v);
   if (number_of(obj, "store", v)) k.store = int(v);
+  if (number_of(obj, "chunk", v)) k.chunk = int(v);
   dk = k;
   return true;
 }
@@ -138,8 +139,9 @@ bool load_dense_tuned(const char* path, DenseKnobs& dk) {
 bool save_dense_tuned(const char* path, const DenseKnobs& dk, const char* device, double ms) {
   char b[256];
   std::snprintf(b, sizeof(b),
-                "{\"rmax\": %d, \"threads\": %d, \"wg_per_cu\": %d, \"store\": %d, \"ms\": %.4f, \"device\": \"%s\"}",
-                dk.rmax, dk.threads, dk.wg_per_cu, dk.store, ms, device ? device : "");
+                "{\"rmax\": %d, \"threads\": %d, \"wg_per_cu\": %d, \"store\": %d, \"chunk\": %d, \"ms\": %.4f, "
+                "\"device\": \"%s\"}",
+                dk.rmax, dk.threads, dk.wg_per_cu, dk.store, dk.chunk, ms, device ? device : "");
   return save_entry(path, "dense", b);
 }
 

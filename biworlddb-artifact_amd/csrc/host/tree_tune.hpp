@@ -26,12 +26,14 @@ bool save_tuned(const char* path, const char* form, const PlanKnobs& kn, const L
 
 // Knobs of the dense-carry form (include/bwgpu_dense.h): the block size (keys per block, which sets
 // the carry's size), the persistent replay's workgroup threads and workgroups per CU, and how the
-// answers reach their ids (BWD_STORE_*).  Stored as the "dense" entry of the same file.
+// answers reach their ids (BWD_STORE_*), and the queries per work item (0: whole blocks).  Stored
+// as the "dense" entry of the same file.
 struct DenseKnobs {
   int rmax = 4096;
   int threads = 1024;
   int wg_per_cu = 1;
   int store = 0;
+  int chunk = 0;
 };
 bool load_dense_tuned(const char* path, DenseKnobs& dk);
 bool save_dense_tuned(const char* path, const DenseKnobs& dk, const char* device, double ms);

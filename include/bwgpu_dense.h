@@ -70,9 +70,13 @@ int bwd_carry_workspace(const BwdIndex* x, size_t* bytes);
 int bwd_carry_build_launch(const BwdIndex* x, const int64_t* agg, void* work, size_t work_bytes, void* stream);
 
 /* Launch shape of the persistent block replay: workgroup threads (256, 512 or 1024; each thread
- * owns rmax/threads in-block ranks, at most 16), workgroups per compute unit, a cap on the
- * workgroups (0: none) that leaves compute units free for work on another stream, such as the
- * next batch's plan, and how answers reach their query ids:
+ * owns rmax/threads in-block ranks, at most 16), workgroups per compute unit (at most as many as
+ * the replay's registers and LDS let reside), a cap on the workgroups (0: none) that leaves
+ * compute units free for work on another stream, such as the next batch's plan, the work item
+ * (chunk 0: a workgroup takes a whole block; chunk c: a block's queries, in walk order, are cut
+ * into items of c, each taken by its own workgroup, which replays the block's events up to its
+ * first state; the items of a block are queued together, so few blocks' carry rows are read at a
+ * time), and how answers reach their query ids:
  *   BWD_STORE_DIRECT  the replay stores each answer at its query id (one random 8-byte store each);
  *   BWD_STORE_STAGED  the replay stores answers in sorted order, then a partition pass groups them
  *                     by windows of 2^18 query ids and a scatter pass stores each window from one
@@ -81,11 +85,13 @@ int bwd_carry_build_launch(const BwdIndex* x, const int64_t* agg, void* work, si
 #define BWD_STORE_AUTO 0
 #define BWD_STORE_DIRECT 1
 #define BWD_STORE_STAGED 2
+#define BWD_MIN_CHUNK 256
 typedef struct BwdLaunch {
   int32_t threads;
   int32_t wg_per_cu;
   int32_t max_workgroups;
   int32_t store;             /* BWD_STORE_* */
+  int32_t chunk;             /* queries per work item (0: whole blocks; else >= BWD_MIN_CHUNK) */
 } BwdLaunch;
 
 /* A planned batch of M queries: the queries grouped by state (any order inside a state), the group

@@ -113,7 +113,13 @@ def test_all_device_pipeline(pkg, case, knobs):
     ("s2_batched_mid", ["--dense-rmax", "1024", "--dense-threads", "512"]),
     ("s2_batched_mid", ["--dense-rmax", "2048", "--dense-threads", "256", "--device-gen"]),
     ("s2_batched_mid", ["--dense-threads", "512", "--dense-wg", "2", "--dense-store", "2"]),
+    # work items of a block's queries, each its own workgroup's (the one-state edge case of 2^20
+    # queries is split over many)
+    ("s1_batched_small", ["--dense-chunk", "256"]),
+    ("s2_batched_mid", ["--dense-chunk", "300", "--dense-threads", "512"]),
+    ("s2_batched_mid", ["--dense-chunk", "4096", "--dense-threads", "1024", "--dense-store", "1"]),
     ("S4_batched_full", ["--tuned", None]),
+    ("S4_batched_full", ["--tuned", None, "--dense-chunk", "2048"]),
 ])
 def test_dense_form_matches_oracle(pkg, case, knobs):
     """The dense-carry form (include/bwgpu_dense.h): carry rows built from the blocks' weights, the
