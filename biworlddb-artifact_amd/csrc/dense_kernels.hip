@@ -1231,11 +1231,17 @@ extern "C" int bwd_plan_launch(const BwdIndex* x, int64_t M, const int32_t* qsta
   return report("plan block order", hipGetLastError());
 }
 
-// The replay's launches; with ev (four events) they are recorded at the phase boundaries: before
-// the block replay, after it, after the partition, after the scatter.
-static int replay_impl(const BwdIndex* x, int64_t M, int64_t* answers, const BwdLaunch* l, void* work,
-                       size_t work_bytes, hipStream_t st, hipEvent_t* ev) {
-  if (!index_ok(x) || M < 1 || M > BWD_MAX_QUERIES || !answers || !l || !work) return (int)hipErrorInvalidValue;
+// The block replay's launch for a batch of M queries: its instance, grid, workgroups per CU that
+// reside (as asked, at most as many as the instance's registers and LDS allow: a workgroup beyond
+// them would only start once the queue is empty) and LDS per workgroup.
+struct ReplayShape {
+  bool staged;
+  int nt, rpt, wg_per_cu, workgroups;
+  size_t lds;
+  BlockKernel kernel;
+};
+static int replay_shape(const BwdIndex* x, int64_t M, const BwdLaunch* l, ReplayShape& sh) {
+  if (!index_ok(x) || M < 1 || M > BWD_MAX_QUERIES || !l) return (int)hipErrorInvalidValue;
   const int nt = l->threads;
   if (nt != 256 && nt != 512 && nt != 1024) return (int)hipErrorInvalidValue;
   if (l->wg_per_cu < 1 || l->wg_per_cu > 8 || l->max_workgroups < 0 || l->store < BWD_STORE_AUTO ||
@@ -1246,32 +1252,50 @@ static int replay_impl(const BwdIndex* x, int64_t M, int64_t* answers, const Bwd
   const int rpt = need <= 4 ? 4 : need <= 8 ? 8 : 16;
   if (need > 16 || x->U > kEPT * nt) return (int)hipErrorInvalidValue;
   const int spn = rpt == 4 ? span_of<4>(nt) : rpt == 8 ? span_of<8>(nt) : span_of<16>(nt);
-  if (block_lds_layout(spn, nt, x->rmax, x->U).total > (size_t)kBlockLdsLimit) return (int)hipErrorInvalidValue;
-  AnswerLayout L;
-  hipError_t e = answer_layout(x, M, L);
-  if (e) return report("answer workspace", e);
-  if (work_bytes < L.total) return (int)hipErrorInvalidValue;
+  const size_t lds = block_lds_layout(spn, nt, x->rmax, x->U).total;
+  if (lds > (size_t)kBlockLdsLimit) return (int)hipErrorInvalidValue;
   int rc = set_lds_limits();
   if (rc) return report("replay LDS limit", (hipError_t)rc);
+  hipError_t e;
   int dev = 0, ncu = 0;
   if ((e = hipGetDevice(&dev)) || (e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev)))
     return report("device query", e);
   // staged stores pay off once the ids span more than one window
-  const bool staged = l->store == BWD_STORE_STAGED || (l->store == BWD_STORE_AUTO && M > (int64_t(1) << kWinBits));
+  sh.staged = l->store == BWD_STORE_STAGED || (l->store == BWD_STORE_AUTO && M > (int64_t(1) << kWinBits));
+  sh.nt = nt;
+  sh.rpt = rpt;
+  sh.lds = lds;
+  sh.kernel = block_kernel(sh.staged, rpt, nt);
+  int resident = 0;
+  if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, reinterpret_cast<const void*>(sh.kernel), nt, lds)))
+    return report("replay occupancy", e);
+  sh.wg_per_cu = std::max(1, std::min(l->wg_per_cu, resident));
+  sh.workgroups = ncu * sh.wg_per_cu;
+  if (l->max_workgroups > 0 && l->max_workgroups < sh.workgroups) sh.workgroups = l->max_workgroups;
+  return 0;
+}
+
+// The replay's launches; with ev (four events) they are recorded at the phase boundaries: before
+// the block replay, after it, after the partition, after the scatter.
+static int replay_impl(const BwdIndex* x, int64_t M, int64_t* answers, const BwdLaunch* l, void* work,
+                       size_t work_bytes, hipStream_t st, hipEvent_t* ev) {
+  if (!answers || !work) return (int)hipErrorInvalidValue;
+  ReplayShape sh;
+  int rc = replay_shape(x, M, l, sh);
+  if (rc) return rc;
+  AnswerLayout L;
+  hipError_t e = answer_layout(x, M, L);
+  if (e) return report("answer workspace", e);
+  if (work_bytes < L.total) return (int)hipErrorInvalidValue;
+  const bool staged = sh.staged;
+  const int nt = sh.nt;
+  const size_t lds = sh.lds;
+  const BlockKernel bk = sh.kernel;
   const int nwin = (int)((M + (int64_t(1) << kWinBits) - 1) >> kWinBits);
   uint32_t* counter = at<uint32_t>(work, L.counter);
   uint32_t* cursor = at<uint32_t>(work, L.cursor);
   hipLaunchKernelGGL(bwd_replay_init_kernel, dim3(1), dim3(256), 0, st, counter, staged ? cursor : nullptr, nwin);
-  const size_t lds = block_lds_layout(spn, nt, x->rmax, x->U).total;
-  const BlockKernel bk = block_kernel(staged, rpt, nt);
-  // workgroups per CU: as asked, at most as many as the instance's registers and LDS let reside
-  // (a workgroup beyond them would only start once the queue is empty)
-  int resident = 0;
-  if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, reinterpret_cast<const void*>(bk), nt, lds)))
-    return report("replay occupancy", e);
-  int nwg = ncu * std::max(1, std::min(l->wg_per_cu, resident));
-  if (l->max_workgroups > 0 && l->max_workgroups < nwg) nwg = l->max_workgroups;
-  const dim3 grid((unsigned)nwg);
+  const dim3 grid((unsigned)sh.workgroups);
   const uint64_t* ends = at<uint64_t>(work, L.ends);
   const uint32_t* ids = at<uint32_t>(work, L.ids);
   const uint2* rg = at<uint2>(work, L.rg);
@@ -1313,6 +1337,18 @@ static int replay_impl(const BwdIndex* x, int64_t M, int64_t* answers, const Bwd
                      at<const int64_t>(work, L.kin), at<const uint32_t>(work, L.vin), answers);
   if ((rc = report("answer scatter", hipGetLastError()))) return rc;
   if ((e = mark(3))) return report("replay event", e);
+  return 0;
+}
+
+extern "C" int bwd_replay_shape(const BwdIndex* x, int64_t M, const BwdLaunch* l, int32_t* workgroups,
+                                int32_t* wg_per_cu, size_t* lds_bytes) {
+  if (!workgroups || !wg_per_cu || !lds_bytes) return (int)hipErrorInvalidValue;
+  ReplayShape sh;
+  const int rc = replay_shape(x, M, l, sh);
+  if (rc) return rc;
+  *workgroups = sh.workgroups;
+  *wg_per_cu = sh.wg_per_cu;
+  *lds_bytes = sh.lds;
   return 0;
 }
 

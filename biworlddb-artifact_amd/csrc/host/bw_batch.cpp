@@ -517,6 +517,12 @@ int main(int argc, char** argv) {
       return 5;
     }
     const double dense_index_wall = ms_between(d0, clk::now());
+    int32_t rep_wg = 0, rep_res = 0;
+    size_t rep_lds = 0;
+    if (bwd_replay_shape(&D.x, M, &D.launch, &rep_wg, &rep_res, &rep_lds)) {
+      std::fprintf(stderr, "bw_batch: dense replay shape\n");
+      return 5;
+    }
     DenseBatch DBn;
     int64_t* d_ans = nullptr;
     if (!alloc_dense_batch(D, M, DBn, err, sizeof(err)) || !(d_ans = DBn.arena.scratch<int64_t>(size_t(M)))) {
@@ -783,14 +789,16 @@ int main(int argc, char** argv) {
     };
     const double sm = mean(step_ms);
     std::snprintf(dense_json, sizeof(dense_json),
-                  "{\"rmax\":%d,\"nblk\":%d,\"threads\":%d,\"wg_per_cu\":%d,\"store\":%d,\"chunk\":%d,\"knobs_from\":\"%s\","
+                  "{\"rmax\":%d,\"nblk\":%d,\"threads\":%d,\"wg_per_cu\":%d,\"wg_resident\":%d,\"workgroups\":%d,"
+                  "\"lds_bytes\":%zu,\"store\":%d,\"chunk\":%d,\"knobs_from\":\"%s\","
                   "\"carry_bytes\":%zu,\"carry_build_ms\":%.3f,"
                   "\"index_wall_ms\":%.3f,\"step_ms\":{\"avg\":%.4f,\"min\":%.4f,\"max\":%.4f,\"repeat\":%d},"
                   "\"plan_ms\":%.4f,\"replay_ms\":%.4f,\"replay_phases_ms\":{\"block\":%.4f,\"partition\":%.4f,"
                   "\"scatter\":%.4f},\"queries_per_s\":%.1f,\"answers_sha256\":\"%s\","
                   "\"offset_mismatch\":%" PRId64 ",\"step_mismatch\":%" PRId64 ",\"edge\":{\"queries\":%" PRId64
                   ",\"mismatch\":%" PRId64 "},\"overlap\":%s,\"mismatch\":%" PRId64 "}",
-                  dense_rmax, E4.x.nblk, dense_threads, dense_wg, dense_store, dense_chunk, dense_from.c_str(), D.carry_bytes, D.build_ms, dense_index_wall, sm,
+                  dense_rmax, E4.x.nblk, dense_threads, dense_wg, rep_res, rep_wg, rep_lds, dense_store, dense_chunk,
+                  dense_from.c_str(), D.carry_bytes, D.build_ms, dense_index_wall, sm,
                   *std::min_element(step_ms.begin(), step_ms.end()), *std::max_element(step_ms.begin(), step_ms.end()),
                   repeat, mean(dplan_v), mean(drep_v), phase_ms[0], phase_ms[1], phase_ms[2], double(M) / (sm * 1e-3),
                   dxr.sha256_hex, offset_mismatch, step_mismatch, edge_m, edge_mismatch, overlap_json, dense_mismatch);

@@ -77,7 +77,16 @@ int tune_dense(const TreeSpec& spec, int repeat, double budget_gb, const std::st
     for (const int threads : {256, 512, 1024})
       for (const int wg : {1, 2})
         for (const int store : {BWD_STORE_DIRECT, BWD_STORE_STAGED}) {
-          const BwdLaunch l{threads, wg, 0, store};
+          const BwdLaunch l{threads, wg, 0, store, 0};
+          int32_t nwg = 0, resident = 0;
+          size_t lds = 0;
+          if (bwd_replay_shape(&D.x, M, &l, &nwg, &resident, &lds)) continue;  // a shape this index cannot run
+          if (resident < wg) {  // the same launch as a smaller wg_per_cu, already timed
+            std::fprintf(stderr, "{\"rmax\":%d,\"threads\":%d,\"wg_per_cu\":%d,\"resident\":%d,\"lds\":%zu,"
+                                 "\"skipped\":\"same launch\"}\n",
+                         rmax, threads, wg, resident, lds);
+            continue;
+          }
           auto step = [&]() {
             const int rc = bwd_plan_launch(&D.x, M, d_qs, d_ql, d_qh, DB.flags, DB.work, DB.work_bytes, st);
             return rc ? rc : bwd_replay_launch(&D.x, M, d_ans, &l, DB.work, DB.work_bytes, st);
@@ -95,8 +104,9 @@ int tune_dense(const TreeSpec& spec, int repeat, double budget_gb, const std::st
             if (ok) ms = sum / repeat;
           }
           ++ntrials;
-          std::fprintf(stderr, "{\"rmax\":%d,\"nblk\":%d,\"threads\":%d,\"wg_per_cu\":%d,\"store\":%d,\"ms\":%.4f}\n",
-                       rmax, E.x.nblk, threads, wg, store, ms);
+          std::fprintf(stderr,
+                       "{\"rmax\":%d,\"nblk\":%d,\"threads\":%d,\"wg_per_cu\":%d,\"lds\":%zu,\"store\":%d,\"ms\":%.4f}\n",
+                       rmax, E.x.nblk, threads, wg, lds, store, ms);
           if (ms > 0 && (best_ms < 0 || ms < best_ms)) {
             best_ms = ms;
             best = DenseKnobs{rmax, threads, wg, store};

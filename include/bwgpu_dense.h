@@ -105,6 +105,14 @@ int bwd_answer_workspace(const BwdIndex* x, int64_t M, size_t* bytes);
  * which the plan's own counts rule out).  Kernels only: capturable in a hipGraph. */
 int bwd_plan_launch(const BwdIndex* x, int64_t M, const int32_t* qstate, const uint32_t* qlo, const uint32_t* qhi,
                     int64_t* flags, void* work, size_t work_bytes, void* stream);
+
+/* The block replay's launch for a batch of M queries under *l (host query, no launch): its
+ * workgroups, the workgroups per CU that reside (l->wg_per_cu, at most as many as its registers and
+ * LDS allow) and its LDS per workgroup.  No counterpart in the reference (its replay's grid is one
+ * block per chain block). */
+int bwd_replay_shape(const BwdIndex* x, int64_t M, const BwdLaunch* l, int32_t* workgroups, int32_t* wg_per_cu,
+                     size_t* lds_bytes);
+
 /* Replay: one launch of the persistent block kernel answers the planned batch (answers[M], by
  * query id): its workgroups take blocks from a device work queue until none is left, so blocks
  * of uneven length and query count share the device.  Kernels only (capturable in a hipGraph). */
