@@ -710,30 +710,35 @@ __device__ void dense_block(const BwdIndex& x, const uint64_t* __restrict__ ends
     __syncthreads();
   };
 
-  // the queries of the state of rank e (entered at the current step)
+  // The queries of the state of rank e (entered at the current step).  The first round's ends
+  // (en_next) were loaded by the previous call (or before the walk): its carry-row reads go out at
+  // once, and the scan's barriers hide their latency; then the next rank's ends are loaded into the
+  // same register, which is read before it is loaded again.
+  uint64_t en_next = 0;
+  auto first_qa = [&](int e) { return gq[e - gwin] + (e == ea ? (uint32_t)off : 0u); };
   auto query = [&](int e) {
-    if (e - gwin >= kGCap) {  // uniform
+    if (e + 1 - gwin >= kGCap) {  // uniform: ranks e and e + 1 must be staged
       __syncthreads();
       gwin = e;
       stage_groups(e);
       __syncthreads();
     }
-    const uint32_t qa = gq[e - gwin] + (e == ea ? (uint32_t)off : 0u), qb = gq[kGCap + e - gwin];
+    const uint32_t qa = first_qa(e), qb = gq[kGCap + e - gwin];
     const int nq = min((int)(qb - qa), left);
-    if (nq <= 0) return;
-    left -= nq;
-    // first round: the queries' items and carry-row reads are issued before the scan, whose
-    // barriers hide their latency
-    uint32_t lo = 0, hi = 0;
-    int64_t clo = 0, chi = 0;
+    left -= max(nq, 0);
     const bool live = tid < nq;
+    const uint32_t lo = (uint32_t)en_next, hi = (uint32_t)(en_next >> 32);
+    int64_t clo = 0, chi = 0;
     if (live) {
-      const uint64_t en = ends[qa + tid];
-      lo = (uint32_t)en;
-      hi = (uint32_t)(en >> 32);
       clo = crow[lo];
       chi = crow[hi];
     }
+    {  // the next rank's first round: every lane loads (clamped into the group, or to a valid
+       // position when it is empty or past the block), so no loaded value is merged with another
+      const uint32_t a = gq[e + 1 - gwin], z = gq[kGCap + e + 1 - gwin];
+      en_next = ends[min(a + (uint32_t)tid, max(z, 1u) - 1u)];
+    }
+    if (nq <= 0) return;
     __syncthreads();  // every event of the steps so far is in w; previous readers of P are done
     rebuild();
     if (live) {
@@ -742,8 +747,7 @@ __device__ void dense_block(const BwdIndex& x, const uint64_t* __restrict__ ends
     }
     for (int j = tid + nt; j < nq; j += nt) {  // groups larger than the workgroup
       const uint64_t en = ends[qa + j];
-      lo = (uint32_t)en;
-      hi = (uint32_t)(en >> 32);
+      const uint32_t lo = (uint32_t)en, hi = (uint32_t)(en >> 32);
       const int rl = lower_bound_lds<uint32_t>(skey, R, lo), rh = lower_bound_lds<uint32_t>(skey, R, hi);
       out[kSorted ? qa + j : qids[qa + j]] = (crow[hi] - crow[lo]) + (P[pslot<RPT>(rh)] - P[pslot<RPT>(rl)]);
     }
@@ -798,6 +802,10 @@ __device__ void dense_block(const BwdIndex& x, const uint64_t* __restrict__ ends
   events(ss + T, stn, r_n, d_n, neg_n);
   ids(ss + 2 * T, st);
   __syncthreads();
+  {  // the first rank's first round (its group from offset off), clamped as in query()
+    const uint32_t a = first_qa(ea), z = gq[kGCap];
+    en_next = ends[min(a + (uint32_t)tid, max(z, 1u) - 1u)];
+  }
   int e = ea;  // rank of the next entered state
   bool done = false;  // the item's queries are answered
   for (int t0 = ss; t0 < s1; t0 += T) {
