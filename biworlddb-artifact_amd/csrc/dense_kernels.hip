@@ -604,14 +604,13 @@ constexpr int span_of(int nt) {
 
 // LDS of the block replay, byte offsets (each sub-array 16-byte aligned)
 struct BlockLds {
-  size_t w, P, wtot, skey, gq, sdel, srank, sstep, total;
+  size_t w, wtot, skey, gq, sdel, srank, sstep, total;
 };
 __host__ __device__ constexpr size_t al16(size_t v) { return (v + 15) & ~size_t(15); }
 __host__ __device__ inline BlockLds block_lds_layout(int span, int nt, int rmax, int U) {
   BlockLds L{};
   L.w = 0;
-  L.P = al16(L.w + (size_t)span * 8);
-  L.wtot = al16(L.P + (size_t)span * 8);
+  L.wtot = al16(L.w + (size_t)span * 8);
   L.skey = al16(L.wtot + 16 * 8);
   L.gq = al16(L.skey + (size_t)rmax * 4);
   L.sdel = al16(L.gq + 2 * kGCap * 4);
@@ -632,8 +631,7 @@ __device__ void dense_block(const BwdIndex& x, const uint64_t* __restrict__ ends
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int span = span_of<RPT>(nt);
   const BlockLds lay = block_lds_layout(span, nt, x.rmax, x.U);
-  int64_t* w = reinterpret_cast<int64_t*>(smem + lay.w);          // deltas since the last scan, per rank
-  int64_t* P = reinterpret_cast<int64_t*>(smem + lay.P);          // exclusive prefix over ranks
+  int64_t* w = reinterpret_cast<int64_t*>(smem + lay.w);          // per rank: within-wave prefix + events since
   int64_t* wtot = reinterpret_cast<int64_t*>(smem + lay.wtot);    // 16 wavefront totals
   uint32_t* skey = reinterpret_cast<uint32_t*>(smem + lay.skey);  // the block's sorted keys
   uint32_t* gq = reinterpret_cast<uint32_t*>(smem + lay.gq);      // kGCap group starts, then ends
@@ -686,28 +684,39 @@ __device__ void dense_block(const BwdIndex& x, const uint64_t* __restrict__ ends
 #pragma unroll
   for (int j = 0; j < RPT; ++j) tot[j] = 0;
 
-  // w -> rank totals; P[y] = sum of totals of ranks < y.  Two workgroup barriers.
+  // The prefix shares w: after a rebuild, the entry of rank y holds the sum of the totals of the
+  // ranks < y owned by the same wavefront (its within-wave prefix), and the events add to it
+  // afterwards.  The owning thread keeps its ranks' totals (tot) and its within-wave offset (wex),
+  // so at the next rebuild it recovers the events added since (entry - what it wrote).  One
+  // workgroup barrier; the prefix over all ranks adds the earlier wavefronts' totals (prefix()).
+  int64_t wex = 0;
   auto rebuild = [&]() {
     const int base = tid * RPT;
     int64_t loc[RPT];
-    int64_t run = 0;
+    int64_t run = 0, was = wex;  // was: what the last rebuild wrote for rank base + j
 #pragma unroll
     for (int j = 0; j < RPT; ++j) {
       const int pi = pslot<RPT>(base + j);
-      tot[j] += w[pi];
-      w[pi] = 0;
+      const int64_t d = w[pi] - was;  // the events of rank base + j since then
+      was += tot[j];
+      tot[j] += d;
       loc[j] = run;
       run += tot[j];
     }
     const int64_t inc = wave_incl_scan(run);
+    wex = inc - run;
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) w[pslot<RPT>(base + j)] = wex + loc[j];
+    if (tid == nt - 1) w[pslot<RPT>(nt * RPT)] = inc;  // past the last rank (no event reaches it)
     if (lane == 63) wtot[wv] = inc;
     __syncthreads();
-    int64_t off = inc - run;
-    for (int k = 0; k < wv; ++k) off += wtot[k];
-#pragma unroll
-    for (int j = 0; j < RPT; ++j) P[pslot<RPT>(base + j)] = off + loc[j];
-    if (tid == nt - 1) P[pslot<RPT>(nt * RPT)] = off + run;
-    __syncthreads();
+  };
+  // sum of the totals of ranks < y, after a rebuild and before the next events
+  auto prefix = [&](int y) -> int64_t {
+    const int ow = min(y / RPT, nt - 1) >> 6;  // the wavefront owning rank y
+    int64_t v = w[pslot<RPT>(y)];
+    for (int k = 0; k < ow; ++k) v += wtot[k];
+    return v;
   };
 
   // The queries of the state of rank e (entered at the current step).  The first round's ends
@@ -739,18 +748,19 @@ __device__ void dense_block(const BwdIndex& x, const uint64_t* __restrict__ ends
       en_next = ends[min(a + (uint32_t)tid, max(z, 1u) - 1u)];
     }
     if (nq <= 0) return;
-    __syncthreads();  // every event of the steps so far is in w; previous readers of P are done
+    __syncthreads();  // every event of the steps so far is in w
     rebuild();
     if (live) {
       const int rl = lower_bound_lds<uint32_t>(skey, R, lo), rh = lower_bound_lds<uint32_t>(skey, R, hi);
-      out[kSorted ? qa + tid : qids[qa + tid]] = (chi - clo) + (P[pslot<RPT>(rh)] - P[pslot<RPT>(rl)]);
+      out[kSorted ? qa + tid : qids[qa + tid]] = (chi - clo) + (prefix(rh) - prefix(rl));
     }
     for (int j = tid + nt; j < nq; j += nt) {  // groups larger than the workgroup
       const uint64_t en = ends[qa + j];
       const uint32_t lo = (uint32_t)en, hi = (uint32_t)(en >> 32);
       const int rl = lower_bound_lds<uint32_t>(skey, R, lo), rh = lower_bound_lds<uint32_t>(skey, R, hi);
-      out[kSorted ? qa + j : qids[qa + j]] = (crow[hi] - crow[lo]) + (P[pslot<RPT>(rh)] - P[pslot<RPT>(rl)]);
+      out[kSorted ? qa + j : qids[qa + j]] = (crow[hi] - crow[lo]) + (prefix(rh) - prefix(rl));
     }
+    __syncthreads();  // every prefix is read before the next events add to w
   };
 
   const int T = (kEPT * nt) / U;  // steps per tile (U <= kEPT * nt is checked at launch)
