@@ -7,11 +7,12 @@ over the 300,000-state tree with uneven chains (geometric runs, mean 64, max 4,0
 point updates per state, N = 2^20 keys).  A "step" is one pass of the hot path over the whole
 stream, its queries already in HBM: the stream is planned on the device (a two-level bucket sort
 of the queries by state, one hipGraph) and answered by ONE launch of a persistent block-replay
-kernel whose workgroups take blocks from a device work queue, over the dense-carry index resident
+kernel whose workgroups take work items (a block's queries cut into runs, a block's runs queued
+together) from a device work queue, over the dense-carry index resident
 in HBM (one carry row of 2^20+1 int64 per block of up to 4,096 keys, built once; its build time is
 reported beside); the replay stores the answers in sorted order and two streaming passes (a
 partition by query-id window, a die-local scatter) put them in query-id order, all in a second
-hipGraph.  The block size, the replay's workgroup shape and the answer store are the autotuner's
+hipGraph.  The block size, the replay's workgroup shape, the work item and the answer store are the autotuner's
 (the "dense" entry of tuned.json, written by bw_tune --form dense).  bw_batch, in a process of its
 own, times each step with one pair of HIP events on its stream and checks every timed step's
 answers (device FNV-1a); the answers' SHA-256 must equal the two-level form's (which the test
@@ -43,7 +44,9 @@ Beside it, measured in the same run (every figure planning-inclusive unless name
                FETCH_SIZE + WRITE_SIZE of that kernel per launch (profiles/), scaled by the same
                counters' reading for a copy of known size, when that summary exists; and the
                kernel's random-access floor: its 2 carry-row reads per query at the rate bw_floor
-               measures on the device (random 8-byte gathers over 8 MiB rows), against its time;
+               measures on the device for random 8-byte gathers over 8 MiB rows, spread over every
+               row and in the replay's own pattern (its workgroups, one row per work item, a
+               block's items together), against its time;
   cpu_baseline -- the oracle's rollback-Fenwick stream solver (a port, not the reference engine:
                that engine cannot take branching multi-update trees) on the host's cores over the
                same whole stream; its answers' SHA-256 must equal the GPU's.
@@ -204,10 +207,17 @@ def main():
         with open(TRAFFIC_FILE) as f:
             traffic = json.load(f).get("bwd_block_kernel")
     # the block replay's own floor: its random stream (two carry-row reads per query) at the device's
-    # measured random-gather rate; the answers leave it in sorted order (coalesced) and reach their ids
-    # through the partition and the die-local scatter, timed beside it
-    fl = run([BW_FLOOR, "--gathers", str(2 * M), "--scatters", str(M)], 300)
+    # measured random-gather rate, (a) spread over all rows and (b) in the replay's own pattern: its
+    # workgroups each reading one row per work item, the items of a block (on one row) taken
+    # together; the answers leave it in sorted order (coalesced) and reach their ids through the
+    # partition and the die-local scatter, timed beside it
+    items = d["work_items"] if d["chunk"] > 0 else d["nblk"]
+    share = max(1, round(items / d["nblk"]))
+    per_visit = max(1, round(2 * M / items))
+    fl = run([BW_FLOOR, "--gathers", str(2 * M), "--scatters", str(M), "--workgroups", str(d["workgroups"]),
+              "--visits", str(items), "--per-visit", str(per_visit), "--share", str(share)], 300)
     floor_ms = 2 * M / fl["gather"]["per_s"] * 1e3
+    replay_floor_ms = 2 * M / fl["replay"]["per_s"] * 1e3
     tl = r["device_plan"]
     value = d["queries_per_s"]
     out = {
@@ -231,7 +241,8 @@ def main():
                    "answers_per_step": M, "answers_sha256": d["answers_sha256"],
                    "dense_index": {"rmax": d["rmax"], "blocks": d["nblk"], "carry_bytes": d["carry_bytes"],
                                    "carry_build_ms": d["carry_build_ms"], "threads": d["threads"],
-                                   "wg_per_cu": d["wg_per_cu"], "store": d["store"],
+                                   "wg_per_cu": d["wg_resident"], "store": d["store"], "chunk": d["chunk"],
+                                   "work_items": d["work_items"], "lds_bytes": d["lds_bytes"],
                                    "knobs_from": os.path.basename(d["knobs_from"])}},
         "step": {"plan_ms": d["plan_ms"], "replay_ms": d["replay_ms"],
                  "replay_phases_ms": phases, "min_ms": d["step_ms"]["min"], "max_ms": d["step_ms"]["max"],
@@ -284,6 +295,14 @@ def main():
                          "gathers": 2 * M, "gathers_per_s": round(fl["gather"]["per_s"], 1),
                          "floor_ms": round(floor_ms, 4), "kernel_ms": round(block_ms, 4),
                          "floor_over_kernel": round(floor_ms / block_ms, 3),
+                         "replay_pattern": {
+                             "rule": "the same loads by the replay's workgroups, one row per work item, the items "
+                                     "of a block read together (bw_floor --workgroups/--visits/--per-visit/--share)",
+                             "workgroups": d["workgroups"], "work_items": items, "items_per_row": share,
+                             "rows_at_once": fl["replay"]["rows_at_once"],
+                             "gathers_per_s": round(fl["replay"]["per_s"], 1),
+                             "floor_ms": round(replay_floor_ms, 4),
+                             "floor_over_kernel": round(replay_floor_ms / block_ms, 3)},
                          "random_scatters_per_s": round(fl["scatter"]["per_s"], 1),
                          "copy_GBps": fl["copy"]["GBps"]}},
         "device_index": {"total_ms": r["device_index"]["total_ms"], "device_ms": r["device_index"]["gpu_ms"],

@@ -1370,6 +1370,18 @@ extern "C" int bwd_replay_shape(const BwdIndex* x, int64_t M, const BwdLaunch* l
   return 0;
 }
 
+extern "C" int bwd_replay_work_items(const BwdIndex* x, int64_t M, const void* work, size_t work_bytes,
+                                     uint32_t* items) {
+  if (!index_ok(x) || M < 1 || M > BWD_MAX_QUERIES || !work || !items) return (int)hipErrorInvalidValue;
+  AnswerLayout L;
+  hipError_t e = answer_layout(x, M, L);
+  if (e) return report("answer workspace", e);
+  if (work_bytes < L.total) return (int)hipErrorInvalidValue;
+  const uint32_t* n = reinterpret_cast<const uint32_t*>(static_cast<const unsigned char*>(work) + L.ibase) + x->nblk;
+  if ((e = hipMemcpy(items, n, sizeof(uint32_t), hipMemcpyDeviceToHost))) return report("work item count", e);
+  return 0;
+}
+
 extern "C" int bwd_replay_launch(const BwdIndex* x, int64_t M, int64_t* answers, const BwdLaunch* l, void* work,
                                  size_t work_bytes, void* stream) {
   return replay_impl(x, M, answers, l, work, work_bytes, (hipStream_t)stream, nullptr);

@@ -497,7 +497,7 @@ int main(int argc, char** argv) {
   // to dense_rmax keys, one dense carry row per block (built once), then per step the whole
   // stream planned (one radix sort) and answered (one persistent block-replay launch, captured
   // in a hipGraph); its answers must equal the two-level form's
-  char dense_json[2048] = "null", noalloc_json[512] = "null";
+  char dense_json[4096] = "null", noalloc_json[512] = "null";
   int64_t dense_mismatch = -1, noalloc_mismatch = -1;
   if (dense) {
     PlanKnobs dk = kn;
@@ -593,6 +593,8 @@ int main(int argc, char** argv) {
     for (uint64_t f : step_fnv) step_mismatch += f != dxr.fnv1a;
     if (step_mismatch) std::fprintf(stderr, "bw_batch: %" PRId64 " timed dense steps' answers differ\n", step_mismatch);
     dense_mismatch += step_mismatch;
+    uint32_t work_items = 0;  // with chunk > 0: the items the timed replays took
+    if (dense_chunk > 0 && bwd_replay_work_items(&D.x, M, DBn.work, DBn.work_bytes, &work_items)) return 5;
     // the replay's phases (block replay, answer partition, answer scatter), timed apart: the mean
     // over `repeat` replays of the planned stream
     double phase_ms[3] = {0, 0, 0};
@@ -790,14 +792,14 @@ int main(int argc, char** argv) {
     const double sm = mean(step_ms);
     std::snprintf(dense_json, sizeof(dense_json),
                   "{\"rmax\":%d,\"nblk\":%d,\"threads\":%d,\"wg_per_cu\":%d,\"wg_resident\":%d,\"workgroups\":%d,"
-                  "\"lds_bytes\":%zu,\"store\":%d,\"chunk\":%d,\"knobs_from\":\"%s\","
+                  "\"lds_bytes\":%zu,\"store\":%d,\"chunk\":%d,\"work_items\":%u,\"knobs_from\":\"%s\","
                   "\"carry_bytes\":%zu,\"carry_build_ms\":%.3f,"
                   "\"index_wall_ms\":%.3f,\"step_ms\":{\"avg\":%.4f,\"min\":%.4f,\"max\":%.4f,\"repeat\":%d},"
                   "\"plan_ms\":%.4f,\"replay_ms\":%.4f,\"replay_phases_ms\":{\"block\":%.4f,\"partition\":%.4f,"
                   "\"scatter\":%.4f},\"queries_per_s\":%.1f,\"answers_sha256\":\"%s\","
                   "\"offset_mismatch\":%" PRId64 ",\"step_mismatch\":%" PRId64 ",\"edge\":{\"queries\":%" PRId64
                   ",\"mismatch\":%" PRId64 "},\"overlap\":%s,\"mismatch\":%" PRId64 "}",
-                  dense_rmax, E4.x.nblk, dense_threads, dense_wg, rep_res, rep_wg, rep_lds, dense_store, dense_chunk,
+                  dense_rmax, E4.x.nblk, dense_threads, dense_wg, rep_res, rep_wg, rep_lds, dense_store, dense_chunk, work_items,
                   dense_from.c_str(), D.carry_bytes, D.build_ms, dense_index_wall, sm,
                   *std::min_element(step_ms.begin(), step_ms.end()), *std::max_element(step_ms.begin(), step_ms.end()),
                   repeat, mean(dplan_v), mean(drep_v), phase_ms[0], phase_ms[1], phase_ms[2], double(M) / (sm * 1e-3),

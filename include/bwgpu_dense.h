@@ -112,6 +112,9 @@ int bwd_plan_launch(const BwdIndex* x, int64_t M, const int32_t* qstate, const u
  * block per chain block). */
 int bwd_replay_shape(const BwdIndex* x, int64_t M, const BwdLaunch* l, int32_t* workgroups, int32_t* wg_per_cu,
                      size_t* lds_bytes);
+/* The work items the last replay with chunk > 0 took from this workspace (a blocking copy of one
+ * count; undefined before such a replay).  No counterpart in the reference. */
+int bwd_replay_work_items(const BwdIndex* x, int64_t M, const void* work, size_t work_bytes, uint32_t* items);
 
 /* Replay: one launch of the persistent block kernel answers the planned batch (answers[M], by
  * query id): its workgroups take blocks from a device work queue until none is left, so blocks
