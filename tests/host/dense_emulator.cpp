@@ -11,7 +11,17 @@
 // items by LDS or global atomics, the emulation takes the items in order: any order gives the same
 // positions' contents up to a permutation inside a state or a window, which the answers do not see.
 //
-//   dense_emulator --S n --U n --N n [--seed n] --requests n --max-batch n --rmax n [--answers F]
+// With --chunk c the replay follows the work items instead (BwdLaunch.chunk): every block's queries
+// in walk order cut into items of c by the items kernels' arithmetic (item slots in the blocks'
+// queue order, each written once), each item replayed on its own -- the events before its first
+// state, then the walk from that state, stopping after its count -- and every grouped position
+// answered exactly once.
+//
+// --hot m moves the stream's first m queries onto the state of the first of them (one group far
+// larger than an item, so several items start inside it at offsets).
+//
+//   dense_emulator --S n --U n --N n [--seed n] --requests n --max-batch n --rmax n [--chunk c]
+//                  [--hot m] [--answers F]
 #include <openssl/sha.h>
 
 #include <algorithm>
@@ -53,10 +63,13 @@ int main(int argc, char** argv) {
   PlanKnobs kn;
   kn.rmax = 4096;
   const char* answers_path = nullptr;
+  int64_t chunk = 0, hot = 0;
   for (int i = 1; i + 1 < argc; i += 2) {
     const char* k = argv[i];
     const char* v = argv[i + 1];
     if (!std::strcmp(k, "--S")) spec.S = std::atoll(v);
+    else if (!std::strcmp(k, "--chunk")) chunk = std::atoll(v);
+    else if (!std::strcmp(k, "--hot")) hot = std::atoll(v);
     else if (!std::strcmp(k, "--U")) spec.U = std::atoi(v);
     else if (!std::strcmp(k, "--N")) spec.N = std::strtoull(v, nullptr, 10);
     else if (!std::strcmp(k, "--seed")) spec.seed = std::strtoull(v, nullptr, 10);
@@ -73,6 +86,7 @@ int main(int argc, char** argv) {
   tree_generate(spec, w);
   RequestStream rs;
   stream_generate(w, ss, rs);
+  for (int64_t i = 1; i < std::min<int64_t>(hot, rs.nq()); ++i) rs.qstate[size_t(i)] = rs.qstate[0];
   const int64_t S = spec.S, M = rs.nq();
   const int U = spec.U;
   const uint64_t NK = w.coord_n();
@@ -216,31 +230,41 @@ int main(int argc, char** argv) {
       if (rgb[size_t(e)] > rga[size_t(e)]) blast[size_t(b)] = e;
     }
 
-  // the replay: per block up to its last queried rank, events into per-rank totals; at each entered
-  // state its queries, answers stored at their grouped positions
+  // one block's replay from rank ea (the block's first rank: from its first step; otherwise from
+  // ea's enter step, the steps before it applied first) answering `left` queries, the first at
+  // offset `off` into ea's group; answers stored at their grouped positions
   std::vector<int64_t> sans(static_cast<size_t>(M), 0);
   std::vector<uint8_t> done(static_cast<size_t>(M), 0);
-  for (int b = 0; b < nblk; ++b) {
-    if (blast[size_t(b)] < 0) continue;
+  auto replay = [&](int b, int32_t ea, int64_t off, int64_t left) -> bool {
     const int64_t k0 = P.blk_key0[size_t(b)];
     const int R = int(P.blk_key0[size_t(b) + 1] - k0);
     const uint32_t* bk = P.bkey.data() + k0;
     std::vector<int64_t> tot(static_cast<size_t>(R), 0);
-    int32_t e = be0[size_t(b)];
+    const int32_t e0 = be0[size_t(b)];
+    const int64_t s0 = P.blk_step0[size_t(b)];
+    const int64_t ss = ea == e0 ? s0 : int64_t(estep[size_t(ea)]);
     const int64_t s1 = int64_t(estep[size_t(blast[size_t(b)])]) + 1;
-    for (int64_t s = P.blk_step0[size_t(b)]; s < s1; ++s) {
+    auto apply = [&](int64_t s) {
       const int32_t v = P.steps[size_t(s)];
       const int64_t a = (v > 0 ? v : -v) - 1;
       for (int i = 0; i < U; ++i) {
         const int64_t d = w.delta[size_t(a * U + i)];
         tot[P.evrank[size_t(s * U + i)]] += v > 0 ? d : -d;
       }
+    };
+    for (int64_t s = s0; s < ss; ++s) apply(s);
+    int32_t e = ea;
+    for (int64_t s = ss; s < s1 && left > 0; ++s) {
+      apply(s);
+      const int32_t v = P.steps[size_t(s)];
       if (v <= 0) continue;
-      CHECK(estep[size_t(e)] == s && erank[size_t(v)] == e);
-      for (int64_t j = rga[size_t(e)]; j < rgb[size_t(e)]; ++j) {
+      if (estep[size_t(e)] != s || erank[size_t(v)] != e) return false;
+      const int64_t qa = rga[size_t(e)] + (e == ea ? off : 0);
+      const int64_t nq = std::min(rgb[size_t(e)] - qa, left);
+      for (int64_t j = qa; j < qa + nq; ++j) {
         const uint32_t qid = qids[size_t(j)];
         const uint32_t lo = uint32_t(ends[size_t(j)]), hi = uint32_t(ends[size_t(j)] >> 32);
-        CHECK(qid < M && !done[qid] && rs.qstate[qid] == v && rs.qlo[qid] == lo && rs.qhi[qid] == hi);
+        if (!(qid < M && !done[qid] && rs.qstate[qid] == v && rs.qlo[qid] == lo && rs.qhi[qid] == hi)) return false;
         const int rl = int(std::lower_bound(bk, bk + R, lo) - bk), rh = int(std::lower_bound(bk, bk + R, hi) - bk);
         int64_t pl = 0, ph = 0;  // in-block prefixes at the endpoints' ranks
         for (int r = 0; r < rl; ++r) pl += tot[size_t(r)];
@@ -249,7 +273,53 @@ int main(int argc, char** argv) {
         sans[size_t(j)] = (row[hi] - row[lo]) + (ph - pl);
         done[qid] = 1;
       }
+      left -= std::max<int64_t>(nq, 0);
       ++e;
+    }
+    return left == 0 || ea == e0;
+  };
+  if (chunk <= 0) {  // whole blocks, each up to its last queried rank
+    for (int b = 0; b < nblk; ++b)
+      if (blast[size_t(b)] >= 0) CHECK(replay(b, be0[size_t(b)], 0, int64_t(1) << 40));
+  } else {
+    // the items kernels: the blocks' queue order (heaviest first; any order gives the same items),
+    // ceil(bq / chunk) items per block from ibase of its place; per block its ranks' running query
+    // count, item j starting at the rank that holds ordinal j * chunk
+    std::vector<int64_t> bq(static_cast<size_t>(nblk), 0);
+    for (int b = 0; b < nblk; ++b)
+      for (int32_t e = be0[size_t(b)]; e < be0[size_t(b) + 1]; ++e) bq[size_t(b)] += rgb[size_t(e)] - rga[size_t(e)];
+    std::vector<int> order(static_cast<size_t>(nblk));
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return bq[size_t(x)] > bq[size_t(y)]; });
+    std::vector<int64_t> ibase(static_cast<size_t>(nblk) + 1, 0), ipos(static_cast<size_t>(nblk));
+    for (int p = 0; p < nblk; ++p) {
+      ipos[size_t(order[size_t(p)])] = p;
+      ibase[size_t(p) + 1] = ibase[size_t(p)] + (bq[size_t(order[size_t(p)])] + chunk - 1) / chunk;
+    }
+    const int64_t nitems = ibase[size_t(nblk)];
+    CHECK(nitems <= int64_t(nblk) + M / 256 + 1 || chunk < 256);  // the workspace's item capacity
+    struct Item {
+      int b;
+      int32_t ea;
+      int64_t off, cnt;
+    };
+    std::vector<Item> items(static_cast<size_t>(nitems), Item{-1, 0, 0, 0});
+    for (int b = 0; b < nblk; ++b) {
+      const int64_t base = ibase[size_t(ipos[size_t(b)])];
+      int64_t cs = 0;
+      for (int32_t e = be0[size_t(b)]; e < be0[size_t(b) + 1]; ++e) {
+        const int64_t c = rgb[size_t(e)] - rga[size_t(e)], q = cs;
+        for (int64_t j = (q + chunk - 1) / chunk; j * chunk < q + c; ++j) {
+          CHECK(base + j < nitems && items[size_t(base + j)].b < 0);
+          items[size_t(base + j)] = Item{b, e, j * chunk - q, std::min(chunk, bq[size_t(b)] - j * chunk)};
+        }
+        cs += c;
+      }
+      CHECK(cs == bq[size_t(b)]);
+    }
+    for (const Item& it : items) {
+      CHECK(it.b >= 0 && it.cnt > 0 && it.off >= 0 && it.off < rgb[size_t(it.ea)] - rga[size_t(it.ea)]);
+      CHECK(replay(it.b, it.ea, it.off, it.cnt));
     }
   }
   for (int64_t q = 0; q < M; ++q) CHECK(done[size_t(q)]);

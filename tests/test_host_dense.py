@@ -5,7 +5,10 @@ tile (packed 64-bit keys and 32-bit values with the low endpoint split across th
 per-tile offsets, every position written once inside [0, M)), the per-block rank groups, the block
 replay storing answers in grouped order, the partition by query-id window and the scatter -- and
 must give the stream oracle's golden answer hashes at several block sizes, 2^20-key trees included
-(endpoints past 2^17 take the split path; s2's 522,196 queries span two id windows)."""
+(endpoints past 2^17 take the split path; s2's 522,196 queries span two id windows).  With work items
+(--chunk) the emulator follows the items kernels' arithmetic and replays every item on its own: the
+same hashes, and on a stream whose first queries share one state (--hot) the same answers as whole
+blocks."""
 import json
 import os
 import subprocess
@@ -44,3 +47,29 @@ def test_dense_emulation_matches_oracle(emulator, case, rmax):
     r = json.loads(p.stdout.strip().splitlines()[-1])
     assert r["answers_sha256"] == g["oracle"], (case, rmax, r)
     assert r["n_answers"] == g["nq"]
+
+
+def _run(emulator, case, rmax, extra):
+    g = STREAMS[case]
+    S, U, N, seed = g["tree"]
+    cmd = [emulator, "--S", str(S), "--U", str(U), "--N", str(N), "--rmax", str(rmax)] + g["args"] + extra
+    if seed:
+        cmd += ["--seed", str(seed)]
+    p = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+    assert p.returncode == 0, p.stderr[-2000:]
+    return json.loads(p.stdout.strip().splitlines()[-1])
+
+
+@pytest.mark.parametrize("case,rmax,chunk", [("s1_batched_small", 4096, 256), ("s1_batched_small", 64, 300),
+                                             ("s2_batched_mid", 4096, 256), ("s2_batched_mid", 1024, 8192)])
+def test_dense_work_items_match_oracle(emulator, case, rmax, chunk):
+    r = _run(emulator, case, rmax, ["--chunk", str(chunk)])
+    assert r["answers_sha256"] == STREAMS[case]["oracle"], (case, rmax, chunk, r)
+
+
+def test_dense_work_items_split_a_hot_state(emulator):
+    """The first 100,000 of s2's queries on one state: items of 256 start inside that state's group."""
+    whole = _run(emulator, "s2_batched_mid", 4096, ["--hot", "100000"])
+    items = _run(emulator, "s2_batched_mid", 4096, ["--hot", "100000", "--chunk", "256"])
+    assert whole["answers_sha256"] == items["answers_sha256"]
+    assert whole["answers_sha256"] != STREAMS["s2_batched_mid"]["oracle"]  # the stream did change
