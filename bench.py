@@ -35,8 +35,10 @@ Beside it, measured in the same run (every figure planning-inclusive unless name
                sources into oracle/_ref/, OpenMP, B = threads) on R1, on the host's cores;
   served    -- the served form (bw_serve): 20,000 Poisson requests (log-uniform sizes in [1, 4096],
                half point / half range queries, 5% appended branch states) by continuous batching,
-               p50/p99 request latency, at several offered rates up to saturation; and the same
-               request law over the config-3 tree with its rank-LUT resident (served_rank_lut);
+               each batch answered by the two-level form, or by the dense form from 2^20 queries on
+               (--form hybrid: both indexes resident), p50/p99 request latency, at several offered
+               rates up to saturation; and the same request law over the config-3 tree with its
+               rank-LUT resident (served_rank_lut, two-level);
   roofline  -- the compulsory bytes of a step (20 B per query: state, two endpoints, answer; 12 B
                per update: key, delta) over the persistent block replay's average HIP-event time
                (bwd_replay_timed: events around that kernel on its stream), against the MI355X HBM
@@ -324,13 +326,14 @@ def main():
     if not a.no_served:
         curve = []
         for rate in SERVED_RATES + (1e9,):
-            s = run([BW_SERVE, "--check-alone", "16" if rate == SERVED_RATES[0] else "0", "--rate", str(rate)] + knobs,
-                    600)
+            s = run([BW_SERVE, "--form", "hybrid", "--check-alone", "16" if rate == SERVED_RATES[0] else "0", "--rate",
+                     str(rate)] + knobs, 600)
             runs = None
             if rate >= 1e9:
                 # offered at once the stream is two batches and ~10 ms of wall time: three runs, the
                 # median one reported, all three rates listed
-                tries = [s] + [run([BW_SERVE, "--check-alone", "0", "--rate", str(rate)] + knobs, 600) for _ in range(2)]
+                tries = [s] + [run([BW_SERVE, "--form", "hybrid", "--check-alone", "0", "--rate", str(rate)] + knobs, 600)
+                               for _ in range(2)]
                 tries.sort(key=lambda t: t["queries_per_s"])
                 s = tries[1]
                 runs = [round(t["queries_per_s"], 1) for t in tries]
@@ -339,7 +342,9 @@ def main():
                           "queries_per_s": round(s["queries_per_s"], 1), "latency_ms": s["latency_ms"],
                           "batches": s["batches"], "mean_batch_queries": s["mean_batch_queries"],
                           "batch_gpu_ms": s["batch_gpu_ms"], "answers_sha256": s["answers_sha256"],
-                          "alone_mismatch": s["alone"]["mismatch"]} | ({"runs_queries_per_s": runs} if runs else {}))
+                          "alone_mismatch": s["alone"]["mismatch"], "answer_form": s["answer_form"],
+                          "dense_from": s["dense_from"], "dense_batches": s["dense_batches"]}
+                         | ({"runs_queries_per_s": runs} if runs else {}))
         if len({c["answers_sha256"] for c in curve}) != 1:
             sys.exit("bench.py: served answers depend on the offered rate")
         out["served"] = curve[0] | {"requests": s["requests"], "appends": s["appends"], "planner": s["planner"]}

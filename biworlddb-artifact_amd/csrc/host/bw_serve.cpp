@@ -3,7 +3,8 @@
 //
 //   bw_serve [--cfg 2|3] [--S n --U n --N n --seed n] [--requests 20000 --max-batch 4096 --append-frac 0.05
 //            --append-U 128 --rate 20000] [--tuned FILE] [--slots 2] [--budget-mb 8192] [--slot-queries 16777216]
-//            [--check-alone 64] [--answers FILE] [--json FILE] [--timeline] [--tile-shift T] [--form dense|two-level]
+//            [--check-alone 64] [--answers FILE] [--json FILE] [--timeline] [--tile-shift T]
+//            [--form dense|two-level|hybrid] [--dense-from 1048576]
 //
 // Resident in HBM: the version tree's index (TreeEngine), the appended states' updates, and per
 // in-flight batch one fixed buffer ("slot") whose size is the HBM budget divided among the slots
@@ -21,12 +22,15 @@
 // `two-level` (the default: include/bwgpu_tree.h, the device batch planner and the query launch
 // sequence) or, over a config-2 tree, `dense` (include/bwgpu_dense.h: the dense carry of blocks of
 // up to 4,096 keys resident beside the index; bucket plan by state, persistent block replay,
-// unsort).  At served batch sizes the dense replay's per-state scans over 4,096-rank blocks cost
-// more than the two-level form's whole batch (measured: 0.40 against 0.19 ms per batch at 20,000
-// requests/s, 4.9 against 1.7 ms of answering for the saturated stream's 9.1M-query batch); it
-// pays off only over streams of ~10^8 queries (bw_batch).  A config-3 tree's 2^22 ranks need the
-// two-level form (a dense carry cannot hold them).  A query on an appended state is answered at its deepest indexed ancestor plus
-// the overlay of the appended updates on its path.
+// unsort).  At small batches the dense replay's walks (each queried block replayed up to its state)
+// cost more than the two-level form's whole batch (measured at 20,000 requests/s, ~1,850 queries
+// per batch: p50 0.63 / p99 1.16 ms dense against 0.56 / 0.95 ms two-level; even at 320,000
+// requests/s, ~240K per batch); over the stream offered at once (9.1M-query batches) it answers
+// 1.23-1.26 G queries/s against 0.73-1.03 G.  `hybrid` keeps both indexes resident and answers a
+// batch of --dense-from queries or more (2^20 by default) with the dense form, smaller ones with the
+// two-level form.  A config-3 tree's 2^22 ranks need the two-level form (a dense carry cannot hold
+// them).  A query on an appended state is answered at its deepest indexed ancestor plus the overlay
+// of the appended updates on its path.
 // Latency = time the request's answers are back on the host - its arrival time.
 // Before the clock starts, one small batch per slot loads the kernels, and copies in both
 // directions on every stream warm up the copy engines (their first use costs milliseconds).
@@ -167,7 +171,8 @@ int main(int argc, char** argv) {
   int nslots = 2, check_alone = 64;
   int64_t slot_queries = int64_t(1) << 24;  // most queries one batch takes
   int tile_shift_arg = -1;  // --tile-shift: key-tile width override (0: the balanced rule)
-  std::string form_arg;     // --form dense|two-level (default: two-level)
+  std::string form_arg;     // --form dense|two-level|hybrid (default: two-level)
+  int64_t dense_from = int64_t(1) << 20;  // --dense-from: hybrid batches of at least this many queries go dense
   bool timeline = false;  // --timeline: per-batch host times on stderr  // most queries one batch takes (batches pipeline across slots)
   double budget_mb = 8192;
   std::string answers_path, json_path, tuned_path;
@@ -193,6 +198,7 @@ int main(int argc, char** argv) {
     else if (!std::strcmp(k, "--check-alone")) check_alone = std::atoi(v);
     else if (!std::strcmp(k, "--tile-shift")) tile_shift_arg = std::atoi(v);
     else if (!std::strcmp(k, "--form")) form_arg = v;
+    else if (!std::strcmp(k, "--dense-from")) dense_from = std::atoll(v);
     else if (!std::strcmp(k, "--answers")) answers_path = v;
     else if (!std::strcmp(k, "--json")) json_path = v;
     else { std::fprintf(stderr, "bw_serve: unknown argument %s\n", k); return 2; }
@@ -202,12 +208,14 @@ int main(int argc, char** argv) {
     std::fprintf(stderr, "bw_serve: bad sizes\n");
     return 2;
   }
-  if (!form_arg.empty() && form_arg != "dense" && form_arg != "two-level") {
-    std::fprintf(stderr, "bw_serve: --form dense|two-level\n");
+  if (!form_arg.empty() && form_arg != "dense" && form_arg != "two-level" && form_arg != "hybrid") {
+    std::fprintf(stderr, "bw_serve: --form dense|two-level|hybrid\n");
     return 2;
   }
-  const bool dense = form_arg == "dense";
-  if (dense && ranked) {
+  const bool dense = form_arg == "dense";    // every batch dense
+  const bool hybrid = form_arg == "hybrid";  // batches of dense_from or more queries dense, the rest two-level
+  const bool with_dense = dense || hybrid;
+  if (with_dense && ranked) {
     std::fprintf(stderr, "bw_serve: the dense form needs keys that are their own ranks (config 2)\n");
     return 3;
   }
@@ -264,11 +272,29 @@ int main(int argc, char** argv) {
   HIPX(hipStreamCreate(&ist));
   RUN(E.build_index(ist));
   HIPX(hipStreamSynchronize(ist));
+  // the dense form's index: E itself (--form dense: built with blocks of 4,096 keys), or beside the
+  // two-level one for the hybrid form
+  TreeEngine E4;
+  PlanKnobs kn4 = kn;
+  kn4.rmax = 4096;
+  if (hybrid) {
+    if (!E4.init(S, spec.U, w.coord_n(), w.parent.data(), kcoord, w.delta.data(), kn4, BwtLaunch{1024, lk.chunk_waves, 0, 0},
+                 err, sizeof(err))) {
+      std::fprintf(stderr, "bw_serve: dense index: %s\n", err);
+      return std::strncmp(err, "device", 6) == 0 ? 5 : 3;
+    }
+    RUN(E4.build_index(ist));
+    HIPX(hipStreamSynchronize(ist));
+  }
   DenseEngine D;
-  if (dense && !D.init(E, ist, err, sizeof(err))) {
+  if (with_dense && !D.init(hybrid ? E4 : E, ist, err, sizeof(err))) {
     std::fprintf(stderr, "bw_serve: %s\n", err);
     return 5;
   }
+  // the dense replay's shape from the tuned file's dense entry (its block size is this index's)
+  DenseKnobs dk;
+  if (with_dense && !tuned_path.empty() && load_dense_tuned(tuned_path.c_str(), dk) && dk.rmax == 4096)
+    D.launch = BwdLaunch{dk.threads, dk.wg_per_cu, 0, BWD_STORE_AUTO, dk.chunk};
   // appended states: resident device arrays (capacity = the stream's appends), host mirrors
   DeviceArena app;
   int32_t* d_app_parent = app.scratch<int32_t>(size_t(NA));
@@ -313,7 +339,9 @@ int main(int argc, char** argv) {
   const size_t share = size_t(budget_mb * 1048576.0 / nslots);
   int64_t cap = ss.max_batch;
   {
-    auto bytes = [&](int64_t c) { return dense ? dense_slot_device_bytes(D, c) : slot_device_bytes(E, c); };
+    auto bytes = [&](int64_t c) {
+      return (with_dense ? dense_slot_device_bytes(D, c) : 0) + (dense ? 0 : slot_device_bytes(E, c));
+    };
     const int64_t probe = int64_t(1) << 20;
     const size_t pb = bytes(probe);
     if (pb == 0) { std::fprintf(stderr, "bw_serve: planner workspace\n"); return 5; }
@@ -331,9 +359,9 @@ int main(int argc, char** argv) {
     s.d_hi = s.in.scratch<uint32_t>(size_t(cap));
     s.d_ovq = s.in.scratch<uint32_t>(size_t(cap));
     s.d_ovs = s.in.scratch<int32_t>(size_t(cap));
-    const bool planned = dense ? alloc_dense_batch(D, cap, s.db, err, sizeof(err)) &&
-                                     (s.d_dans = s.db.arena.scratch<int64_t>(size_t(cap))) != nullptr
-                               : alloc_plan_slot(E, cap, s.plan, err, sizeof(err));
+    const bool planned = (!with_dense || (alloc_dense_batch(D, cap, s.db, err, sizeof(err)) &&
+                                          (s.d_dans = s.db.arena.scratch<int64_t>(size_t(cap))) != nullptr)) &&
+                         (dense || alloc_plan_slot(E, cap, s.plan, err, sizeof(err)));
     if (!s.in.ok() || !planned) {
       std::fprintf(stderr, "bw_serve: slot allocation: %s\n", s.in.ok() ? err : "inputs");
       return 5;
@@ -351,7 +379,9 @@ int main(int argc, char** argv) {
   const BwtLaunch L = E.launch;
   // dst: pinned host array receiving the batch's answers (the stream's answer array at the
   // batch's offset, or the slot's own buffer for a request re-answered alone)
-  auto launch_batch = [&](Slot& s, int64_t* dst) {
+  int64_t dense_batches = 0;
+  // force: -1 the form's choice, 0 two-level, 1 dense (the warm-up runs each path once per slot)
+  auto launch_batch = [&](Slot& s, int64_t* dst, int force = -1) {
     const auto p0 = clk::now();
     uint32_t* hoq = reinterpret_cast<uint32_t*>(s.host + s.h_ovq);
     int32_t* hos = reinterpret_cast<int32_t*>(s.host + s.h_ovs);
@@ -408,14 +438,16 @@ int main(int argc, char** argv) {
     HIPX(hipStreamWaitEvent(s.st, app_ready, 0));  // appended states admitted so far are on the device
     if (timeline) HIPX(hipEventRecord(s.mark[0], s.st));
     const auto p2 = clk::now();
-    if (dense)
+    const bool dn = force >= 0 ? force == 1 : (dense || (hybrid && M >= dense_from));
+    if (dn && force < 0) ++dense_batches;
+    if (dn)
       RUN(bwd_plan_launch(&D.x, M, s.d_state, s.d_lo, s.d_hi, s.db.flags, s.db.work, s.db.work_bytes, s.st));
     else
       RUN(device_plan(E, s.plan, M, s.d_state, s.d_lo, s.d_hi, s.st));
     if (timeline) HIPX(hipEventRecord(s.mark[1], s.st));
     const auto p3 = clk::now();
-    int64_t* answers = dense ? s.d_dans : s.plan.b.answers;
-    if (dense) {
+    int64_t* answers = dn ? s.d_dans : s.plan.b.answers;
+    if (dn) {
       RUN(bwd_replay_launch(&D.x, M, answers, &D.launch, s.db.work, s.db.work_bytes, s.st));
     } else {
       BwtLaunch l = L;
@@ -488,9 +520,11 @@ int main(int argc, char** argv) {
       for (Slot& s : slots) {
         s.reqs.assign(1, r0);
         s.req_off.assign(1, 0);
-        launch_batch(s, reinterpret_cast<int64_t*>(s.host + s.h_ans));
-        HIPX(hipEventSynchronize(s.done));
-        s.busy = false;
+        for (int f = hybrid ? 0 : -1; f <= (hybrid ? 1 : -1); ++f) {  // hybrid: each path once
+          launch_batch(s, reinterpret_cast<int64_t*>(s.host + s.h_ans), f);
+          HIPX(hipEventSynchronize(s.done));
+          s.busy = false;
+        }
       }
     // copies: the runtime's first copies through each copy engine pay a one-time set-up
     // (measured ~5-6 ms, once per engine), so before the clock starts copies go in both
@@ -499,7 +533,7 @@ int main(int argc, char** argv) {
       for (Slot& s : slots) {
         const size_t m = std::min({n, size_t(NQ) * 4, size_t(s.cap) * 4});
         HIPX(hipMemcpyAsync(s.d_lo, p_lo, m, hipMemcpyHostToDevice, s.st));
-        HIPX(hipMemcpyAsync(s.host + s.h_ans, dense ? s.d_dans : s.plan.b.answers, m, hipMemcpyDeviceToHost, s.st));
+        HIPX(hipMemcpyAsync(s.host + s.h_ans, with_dense ? s.d_dans : s.plan.b.answers, m, hipMemcpyDeviceToHost, s.st));
       }
       if (NA > 0) {
         const size_t m = std::min(n, size_t(NA) * size_t(AU) * 8);
@@ -617,8 +651,8 @@ int main(int argc, char** argv) {
     ++alone_checked;
   }
 
-  // the dense plan flags an out-of-range query (each slot's flag holds its last batch's)
-  if (dense)
+  // the dense plan flags an out-of-range query (each slot's flag holds its last dense batch's)
+  if (with_dense)
     for (Slot& s : slots) {
       int64_t f = 0;
       HIPX(hipMemcpy(&f, s.db.flags, sizeof(f), hipMemcpyDeviceToHost));
@@ -660,6 +694,7 @@ int main(int argc, char** argv) {
       "\"slot_capacity_queries\":%" PRId64 ",\"batches\":%" PRId64 ",\"mean_batch_queries\":%.1f,"
       "\"max_batch_queries\":%" PRId64 ",\"wall_s\":%.4f,\"offered_queries_per_s\":%.1f,\"queries_per_s\":%.1f,"
       "\"latency_ms\":{\"p50\":%.3f,\"p90\":%.3f,\"p99\":%.3f,\"max\":%.3f},\"planner\":\"device\",\"answer_form\":\"%s\","
+      "\"dense_from\":%" PRId64 ",\"dense_batches\":%" PRId64 ","
       "\"host_stage_s\":%.4f,\"slot_device_bytes\":%zu,\"p99_second_half_ms\":%.3f,"
       "\"batch_gpu_ms\":{\"mean\":%.4f,\"p99\":%.4f},"
       "\"alone\":{\"requests\":%" PRId64 ",\"mismatch\":%" PRId64 "},\"knobs_from\":\"%s\",\"setup_s\":%.3f,"
@@ -667,7 +702,7 @@ int main(int argc, char** argv) {
       tcfg, ranked ? "true" : "false", S, spec.U, spec.N, spec.seed, R, nquery_req, nappended, ss.rate, ss.max_batch, NQ, sha.c_str(), nslots, budget_mb,
       cap, nbatches, nbatches ? sum_batch_q / double(nbatches) : 0.0, max_batch_q, wall, offered, double(NQ) / wall,
       pct(0.5) * 1e3, pct(0.9) * 1e3, pct(0.99) * 1e3, lat.empty() ? 0.0 : lat.back() * 1e3,
-      dense ? "dense" : "two-level", plan_s, slots[0].in.bytes + slots[0].plan.arena.bytes + slots[0].db.arena.bytes,
+      dense ? "dense" : hybrid ? "hybrid" : "two-level", hybrid ? dense_from : (dense ? 0 : -1), dense_batches, plan_s, slots[0].in.bytes + slots[0].plan.arena.bytes + slots[0].db.arena.bytes,
       p99_late * 1e3, gpu_mean, gpu_p99, alone_checked,
       alone_mismatch, tuned_from.c_str(), secs(g0, g1), dev, prop.multiProcessorCount);
   std::fwrite(buf, 1, size_t(n), stdout);

@@ -2,7 +2,8 @@
 branch states, continuous batching over a resident index) against the stream oracle's golden
 hashes, and the served answers of sampled requests against the same requests answered alone.
 Batches are answered by the two-level form by default and, over a config-2 tree, by the dense form
-with --form dense; over a config-3 tree (rank-LUT) by the two-level form only."""
+with --form dense, or by either with --form hybrid (the dense form from --dense-from queries on);
+over a config-3 tree (rank-LUT) by the two-level form only."""
 import pytest
 
 from conftest import golden, run_json
@@ -28,6 +29,9 @@ def _cmd(pkg, g, extra):
     ("s4_served_mid", ["--form", "dense", "--check-alone", "32"]),
     ("s4_served_mid", ["--form", "dense", "--slots", "4", "--budget-mb", "64", "--rate", "1e9",
                        "--check-alone", "8"]),
+    # hybrid: batches of --dense-from queries or more dense, the rest two-level (both indexes resident)
+    ("s3_served_small", ["--form", "hybrid", "--dense-from", "256", "--rate", "1e9", "--check-alone", "16"]),
+    ("s4_served_mid", ["--form", "hybrid", "--dense-from", "4096", "--rate", "1e9", "--check-alone", "8"]),
     # over a config-3 tree: raw 32-bit endpoints ranked on the device per batch, appended keys at
     # admission
     ("s6_served_lut_small", ["--check-alone", "16"]),
@@ -42,17 +46,23 @@ def test_served_matches_oracle_and_alone(pkg, case, extra):
     assert r["n_answers"] == g["nq"] and r["appends"] == g["napp"]
     assert r["alone"]["mismatch"] == 0 and r["alone"]["requests"] > 0
     assert r["latency_ms"]["p99"] >= r["latency_ms"]["p50"] > 0
-    assert r["answer_form"] == ("dense" if "dense" in extra else "two-level")
+    form = extra[extra.index("--form") + 1] if "--form" in extra else "two-level"
+    assert r["answer_form"] == form
+    if form == "hybrid":  # some batches went dense (the alone checks' single requests stay two-level)
+        assert r["dense_batches"] > 0, r
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("form", ["dense", "two-level"])
+@pytest.mark.parametrize("form", ["dense", "two-level", "hybrid"])
 def test_served_full_stream(pkg, form):
-    """20,000 requests (9.2M queries, 1,031 appended states) over the 300,000-state tree, in both
-    answer forms."""
+    """20,000 requests (9.2M queries, 1,031 appended states) over the 300,000-state tree, in every
+    answer form."""
     g = STREAMS["S5_served_full"]
-    r = run_json(_cmd(pkg, g, ["--form", form, "--check-alone", "32"]), timeout=600)
+    extra = ["--dense-from", "16384"] if form == "hybrid" else []
+    r = run_json(_cmd(pkg, g, ["--form", form, "--check-alone", "32"] + extra), timeout=600)
     assert r["answers_sha256"] == g["oracle"] and r["answer_form"] == form
+    if form == "hybrid":
+        assert 0 < r["dense_batches"] < r["batches"], r
     assert r["alone"]["mismatch"] == 0
 
 
