@@ -37,6 +37,12 @@ namespace {
 
 constexpr int kT = 256;
 
+// The sorts' configuration: up to 4,096 items sort in one single-block launch (512 threads x 8
+// items) instead of a block sort and a merge pass per doubling -- a served batch of a few thousand
+// queries then plans in a few launches; larger sorts use rocPRIM's own choices.  Every sort is
+// stable, so its result does not depend on the algorithm.
+using PlanSort = rocprim::radix_sort_config<rocprim::kernel_config<512, 8>>;
+
 // values < 2^bits for every v' <= v (at least one bit, so no sort runs over an empty key)
 unsigned bits_for(uint64_t v) {
   unsigned b = 1;
@@ -94,7 +100,7 @@ hipError_t temp_bytes(const Shape& s, size_t& need) {
   need = 0;
   size_t b = 0;
   hipError_t e;
-  if ((e = rocprim::radix_sort_pairs(nullptr, b, (const uint64_t*)nullptr, (uint64_t*)nullptr,
+  if ((e = rocprim::radix_sort_pairs<PlanSort>(nullptr, b, (const uint64_t*)nullptr, (uint64_t*)nullptr,
                                      (const uint64_t*)nullptr, (uint64_t*)nullptr, size_t(s.M), 32u, 32u + s.sbits)))
     return e;
   need = std::max(need, b);
@@ -102,7 +108,7 @@ hipError_t temp_bytes(const Shape& s, size_t& need) {
                                    rocprim::plus<uint64_t>())))
     return e;
   need = std::max(need, b);
-  if ((e = rocprim::radix_sort_pairs(nullptr, b, (const uint64_t*)nullptr, (uint64_t*)nullptr,
+  if ((e = rocprim::radix_sort_pairs<PlanSort>(nullptr, b, (const uint64_t*)nullptr, (uint64_t*)nullptr,
                                      (const uint64_t*)nullptr, (uint64_t*)nullptr, size_t(s.nE), 0u,
                                      s.rb + s.cb + s.pb)))
     return e;
@@ -116,7 +122,7 @@ hipError_t temp_bytes(const Shape& s, size_t& need) {
     return e;
   need = std::max(need, b);
   for (const unsigned gb : {s.gbits, s.tbits}) {
-    if ((e = rocprim::radix_sort_pairs(nullptr, b, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+    if ((e = rocprim::radix_sort_pairs<PlanSort>(nullptr, b, (const uint32_t*)nullptr, (uint32_t*)nullptr,
                                        (const uint64_t*)nullptr, (uint64_t*)nullptr, size_t(s.M), 0u, gb)))
       return e;
     need = std::max(need, b);
@@ -435,7 +441,7 @@ extern "C" int bwt_plan_launch(const BwtIndex* x, const BwtPlanIndex* p, const i
   hipLaunchKernelGGL(plan_slot_keys_kernel, dim3(grid_of(M)), blk, 0, st, *x, *p, qstate, B, M, skey_in, sval_in,
                      dcounts);
   if ((rc = report("slot keys", hipGetLastError()))) return rc;
-  if ((rc = report("slot sort", rocprim::radix_sort_pairs(temp, tb, skey_in, skey_out, sval_in, sval_out, size_t(M),
+  if ((rc = report("slot sort", rocprim::radix_sort_pairs<PlanSort>(temp, tb, skey_in, skey_out, sval_in, sval_out, size_t(M),
                                                           32u, 32u + s.sbits, st))))
     return rc;
   hipLaunchKernelGGL(plan_slot_info_kernel, dim3(grid_of(M)), blk, 0, st, *x, *p, B, M, skey_out, sval_out, flags);
@@ -452,7 +458,7 @@ extern "C" int bwt_plan_launch(const BwtIndex* x, const BwtPlanIndex* p, const i
                      sval_out, ekey_in, eval_in);
   if ((rc = report("endpoints", hipGetLastError()))) return rc;
   tb = L.temp_bytes;
-  if ((rc = report("endpoint sort", rocprim::radix_sort_pairs(temp, tb, ekey_in, ekey_out, eval_in,
+  if ((rc = report("endpoint sort", rocprim::radix_sort_pairs<PlanSort>(temp, tb, ekey_in, ekey_out, eval_in,
                                                               const_cast<uint64_t*>(B.eitems), size_t(s.nE), 0u,
                                                               s.rb + s.cb + s.pb, st))))
     return rc;
@@ -476,13 +482,13 @@ extern "C" int bwt_plan_launch(const BwtIndex* x, const BwtPlanIndex* p, const i
   hipLaunchKernelGGL(plan_stage_keys_kernel, dim3(grid_of(M)), blk, 0, st, B, M, s.sh, s.per_t, gkey_in, gval_in);
   if ((rc = report("stage keys", hipGetLastError()))) return rc;
   tb = L.temp_bytes;
-  if ((rc = report("stage sort", rocprim::radix_sort_pairs(temp, tb, gkey_in, gkey_out, gval_in, gval_out, size_t(M),
+  if ((rc = report("stage sort", rocprim::radix_sort_pairs<PlanSort>(temp, tb, gkey_in, gkey_out, gval_in, gval_out, size_t(M),
                                                            0u, s.gbits, st))))
     return rc;
   hipLaunchKernelGGL(plan_stage_pos_kernel, dim3(grid_of(M)), blk, 0, st, B, M, gval_out, tkey_in, tval_in);
   if ((rc = report("stage positions", hipGetLastError()))) return rc;
   tb = L.temp_bytes;
-  if ((rc = report("tile sort", rocprim::radix_sort_pairs(temp, tb, tkey_in, tkey_out, tval_in, tval_out, size_t(M),
+  if ((rc = report("tile sort", rocprim::radix_sort_pairs<PlanSort>(temp, tb, tkey_in, tkey_out, tval_in, tval_out, size_t(M),
                                                           0u, s.tbits, st))))
     return rc;
   hipLaunchKernelGGL(plan_stage_rank_kernel, dim3(grid_of(M)), blk, 0, st, B, M, tval_out);
