@@ -492,20 +492,36 @@ __global__ void __launch_bounds__(256) bwt_seg_apply_kernel(const BwtIndex x) {
 
 // ---------------------------------------------------------------- chunk replay (level B)
 // Per-wavefront LDS of the chunk replay: 64 group key weights, 64 group key classes, the Fenwick
-// tree; an even number of words keeps every wavefront's area 16-byte aligned.
-__host__ __device__ constexpr int replay_wave_words(int stride) { return (96 + stride + 1) & ~1; }
+// tree (its node slots rounded up to whole runs of 32, see fen_slot); every wavefront's area is a
+// multiple of 16 bytes.
+__host__ __device__ constexpr int replay_wave_words(int stride) { return 96 + ((stride + 31) & ~31); }
 
-// Fenwick prefix over classes < y for y < 2^kBits (at most kBits set bits); fen[0] stays 0, so
+// LDS slot of Fenwick node f: f with its low 5 bits xor-ed with bits 5-9 and 10-14, a permutation
+// of every aligned run of 32 nodes.  The nodes one update touches at level t are odd multiples
+// of 2^t; stored in place, every level t >= 4 falls on one bank pair (64-bit words), so the
+// lanes of an update pass (several keys x every level) and late prefix steps serialise on it.
+__device__ __forceinline__ int fen_slot(int f) { return f ^ (((f >> 5) ^ (f >> 10)) & 31); }
+
+// Fenwick prefix over classes < y for y < 2^kBits (at most kBits set bits); node 0 stays 0, so
 // the fixed trip count needs no branch and the LDS reads issue back to back.
 template <int kBits>
 __device__ __forceinline__ int64_t fen_prefix(const int64_t* fen, int y) {
   int64_t r = 0;
 #pragma unroll
   for (int k = 0; k < kBits; ++k) {
-    r += fen[y];
+    r += fen[fen_slot(y)];
     y &= y - 1;
   }
   return r;
+}
+
+// The value lane `src` holds (ds_bpermute: every lane active, src in [0, 64)).
+__device__ __forceinline__ int lane_read(int v, int src) { return __builtin_amdgcn_ds_bpermute(src << 2, v); }
+__device__ __forceinline__ int64_t lane_read64(int64_t v, int src) {
+  const uint64_t u = (uint64_t)v;
+  const uint32_t lo = (uint32_t)lane_read((int)(uint32_t)u, src);
+  const uint32_t hi = (uint32_t)lane_read((int)(uint32_t)(u >> 32), src);
+  return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
 // One wavefront per work item (a chunk and a run of its endpoints, sorted by position).  The
@@ -530,7 +546,7 @@ __device__ __forceinline__ void chunk_replay_one(const BwtIndex& x, const BwtBat
   for (int i = lane; i < stride; i += 64) {
     int64_t v = 0;
     if (i >= 1 && i < ncls) v = row[i] - row[i - (i & -i)];
-    fen[i] = v;
+    fen[fen_slot(i)] = v;
   }
   // the work item's endpoints e0 .. e0+ne_all-1 and keys k0 .. k0+nk_all-1 (keys after its last
   // endpoint are not needed); the cursors below are 32-bit offsets from them
@@ -579,20 +595,33 @@ __device__ __forceinline__ void chunk_replay_one(const BwtIndex& x, const BwtBat
     wave_lds_sync();
     if (ne > 0) {
       const int cls = (int)BWT_EP_CLASS(it);
-      // a taken endpoint needs the group's first `rel` keys; rel grows with the lane
-      const int mmax = __builtin_amdgcn_readlane(rel, ne - 1);
+      // a taken endpoint needs the group's first `rel` keys (rel <= nk); 0 for the other lanes
       const int rl = take ? rel : 0;
       int64_t val = fen_prefix<kBits>(fen, take ? cls : 0);
-      // four keys per 16-byte LDS read (entries past nk are excluded by m < rl <= nk)
-      for (int m = 0; m < mmax; m += 4) {
-        const int4 c4 = *reinterpret_cast<const int4*>(kcs + m);
-        const longlong2 w01 = *reinterpret_cast<const longlong2*>(kws + m);
-        const longlong2 w23 = *reinterpret_cast<const longlong2*>(kws + m + 2);
-        if (m < rl && c4.x < cls) val += w01.x;
-        if (m + 1 < rl && c4.y < cls) val += w01.y;
-        if (m + 2 < rl && c4.z < cls) val += w23.x;
-        if (m + 3 < rl && c4.w < cls) val += w23.y;
+      // The group's keys before endpoint j with a smaller class, summed by the lanes: the
+      // wavefront splits into segments of L = 64 / 2^ceil(log2 ne) lanes, segment j serving
+      // endpoint j, its lane s taking the key pairs (2s, 2s + 1), (2s + 2L, 2s + 2L + 1), ... below
+      // rel_j (one 8-byte and one 16-byte read per pair, consecutive within a segment; segments
+      // past ne have rel 0).  rel_j <= 63, so a pair's second entry is inside the 64 slots (a
+      // slot at or past rel_j may hold an earlier group's key and is excluded).  One wavefront
+      // scan then gives every segment's total as the difference of two of its inclusive sums.
+      const int lg = 6 - (ne > 1 ? 32 - __builtin_clz((unsigned)(ne - 1)) : 0);  // log2 L, uniform
+      const int L = 1 << lg;
+      const int sj = lane >> lg;
+      const int rj = lane_read(rl, sj), cj = lane_read(cls, sj);
+      int64_t part = 0;
+      for (int m = 2 * (lane & (L - 1)); m < rj; m += 2 * L) {
+        const int2 c2 = *reinterpret_cast<const int2*>(kcs + m);
+        const longlong2 w2 = *reinterpret_cast<const longlong2*>(kws + m);
+        // selects, not branches: both weights are read by one access whatever the classes
+        part += (c2.x < cj ? w2.x : 0) + (((m + 1 < rj) & (c2.y < cj)) ? w2.y : 0);
       }
+      const int64_t inc = wave_incl_scan(part);
+      // endpoint lane e (< ne): its segment spans lanes [e L, e L + L), inside the wavefront
+      const int first = (lane << lg) & 63;
+      const int64_t seg_end = lane_read64(inc, (first + L - 1) & 63);
+      const int64_t seg_before = lane_read64(inc, (first + 63) & 63);  // lane first - 1
+      val += seg_end - (first == 0 ? 0 : seg_before);
       // results in work order: the taken lanes store one contiguous run
       if (take) gout[(uint32_t)(ep + lane)] = val;
     }
@@ -605,7 +634,7 @@ __device__ __forceinline__ void chunk_replay_one(const BwtIndex& x, const BwtBat
       if (kSub < kKeysPer && m < nk) {
         const int cm = kcs[m];
         const int f = (cm | ((1 << kLev) - 1)) + 1;
-        if (!((cm >> kLev) & 1) && f <= ncls) lds_add64(&fen[f], kws[m]);
+        if (!((cm >> kLev) & 1) && f <= ncls) lds_add64(&fen[fen_slot(f)], kws[m]);
       }
     }
     wave_lds_sync();
