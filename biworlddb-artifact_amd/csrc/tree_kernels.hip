@@ -496,11 +496,12 @@ __global__ void __launch_bounds__(256) bwt_seg_apply_kernel(const BwtIndex x) {
 // multiple of 16 bytes.
 __host__ __device__ constexpr int replay_wave_words(int stride) { return 96 + ((stride + 31) & ~31); }
 
-// LDS slot of Fenwick node f: f with its low 5 bits xor-ed with bits 5-9 and 10-14, a permutation
-// of every aligned run of 32 nodes.  The nodes one update touches at level t are odd multiples
-// of 2^t; stored in place, every level t >= 4 falls on one bank pair (64-bit words), so the
-// lanes of an update pass (several keys x every level) and late prefix steps serialise on it.
-__device__ __forceinline__ int fen_slot(int f) { return f ^ (((f >> 5) ^ (f >> 10)) & 31); }
+// LDS slot of Fenwick node f: f with its low 5 bits xor-ed with bits 5-9, a permutation of every
+// aligned run of 32 nodes.  The nodes one update touches at level t are odd multiples of 2^t;
+// stored in place, every level t >= 4 falls on one bank pair (64-bit words), so the lanes of an
+// update pass (several keys x every level) and late prefix steps serialise on it.  Levels 5-9
+// then spread over 32 slots of their runs (levels >= 10 have a few nodes each).
+__device__ __forceinline__ int fen_slot(int f) { return f ^ ((f >> 5) & 31); }
 
 // Fenwick prefix over classes < y for y < 2^kBits (at most kBits set bits); node 0 stays 0, so
 // the fixed trip count needs no branch and the LDS reads issue back to back.
@@ -528,8 +529,8 @@ __device__ __forceinline__ int64_t lane_read64(int64_t v, int src) {
 // Fenwick tree starts as the chunk's carry row (fen[i] = row[i] - row[i - lsb(i)]), so a prefix
 // read is the whole carry.  Each group takes the next 64 items of the merged order (endpoint
 // items before key items at equal position): the group's keys go to LDS, every endpoint adds
-// the Fenwick prefix plus the group's keys that precede it with a smaller class (broadcast
-// reads), then the keys enter the tree.  The next group's items are loaded while this one runs
+// the Fenwick prefix plus the group's keys that precede it with a smaller class (summed by a
+// segment of lanes per endpoint), then the keys enter the tree.  The next group's items are loaded while this one runs
 // (only the high word of a key item, which holds its class).  Results are stored in work order
 // (g[i] for eitems[i]): a group's taken lanes write one contiguous run.  Classes < 2^kBits.
 template <int kBits>
@@ -572,9 +573,9 @@ __device__ __forceinline__ void chunk_replay_one(const BwtIndex& x, const BwtBat
     kw_ = kwt[kk];
   };
   int ep = 0, ki = 0;
-  // lane -> (key kSub of a pass, Fenwick level kLev) of the tree update below
-  constexpr int kLevels = kBits + 1, kKeysPer = 64 / kLevels;
-  const int kSub = lane / kLevels, kLev = lane - kSub * kLevels;
+  // the first Fenwick level of key lane `lane` in the tree update below
+  constexpr int kLevels = kBits + 1;
+  const int kLev0 = lane % kLevels;
   // One group of 64 merged items whose items are in (it, kx, kw); the next group's items are
   // loaded into (it2, kx2, kw2) as soon as its start is known.  Returns false after the last group.
   auto group = [&](const uint64_t it, const uint32_t kx, const int64_t kw, uint64_t& it2, uint32_t& kx2,
@@ -626,15 +627,17 @@ __device__ __forceinline__ void chunk_replay_one(const BwtIndex& x, const BwtBat
       if (take) gout[(uint32_t)(ep + lane)] = val;
     }
     if (ep2 >= ne_all) return false;
-    // the group's keys enter the tree, one lane per (key, level): the update path of class c is
-    // (c | (2^t - 1)) + 1 for every level t with bit t of c clear (up to ncls), so kKeysPer keys of
-    // kLevels levels each go in one pass instead of one level per pass for every key
-    for (int m0 = 0; m0 < nk; m0 += kKeysPer) {
-      const int m = m0 + kSub;
-      if (kSub < kKeysPer && m < nk) {
-        const int cm = kcs[m];
-        const int f = (cm | ((1 << kLev) - 1)) + 1;
-        if (!((cm >> kLev) & 1) && f <= ncls) lds_add64(&fen[fen_slot(f)], kws[m]);
+    // the group's keys enter the tree: the update path of class c is (c | (2^t - 1)) + 1 for every
+    // level t with bit t of c clear (up to ncls).  Key lane m (its class and weight in registers)
+    // walks the levels from m mod kLevels on, so the lanes of one atomic spread over the levels
+    // (the top levels have a few nodes each: all keys at one level would serialise on them)
+    if (lane < nk) {
+      int lev = kLev0;
+#pragma unroll
+      for (int t = 0; t < kLevels; ++t) {
+        const int f = (kcls | ((1 << lev) - 1)) + 1;
+        if (!((kcls >> lev) & 1) && f <= ncls) lds_add64(&fen[fen_slot(f)], kw);
+        lev = lev == kLevels - 1 ? 0 : lev + 1;
       }
     }
     wave_lds_sync();
