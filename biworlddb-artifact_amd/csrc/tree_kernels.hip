@@ -327,6 +327,7 @@ __global__ void __launch_bounds__(1024) bwt_block_kernel(const BwtIndex x, const
 }
 
 // ---------------------------------------------------------------- chunk histogram
+constexpr int kHistBatch = 8;  // key items per lane whose gathers are in flight at once
 // One wavefront per chunk: dense class histogram of the chunk's key items, written as its
 // exclusive prefix over classes (cp row = sum of weights with class < y).  With write_kw the
 // gathered weight of every item is also stored in chain order (kweight), so the chunk replay
@@ -343,17 +344,25 @@ __global__ void __launch_bounds__(256) bwt_chunk_hist_kernel(const BwtIndex x, c
   const int64_t* wsrc = x.chain_src[chain] ? x.tagg : x.agg;
   for (int i = lane; i < ncls; i += 64) h[i] = 0;
   wave_lds_sync();
-  const int64_t i1 = x.chunk_key0[c + 1];
-  int64_t it = x.chunk_key0[c] + lane;
-  uint64_t item = (it < i1) ? x.kitems[it] : 0;
-  while (it < i1) {
-    const int64_t nx = it + 64;
-    const uint64_t nitem = (nx < i1) ? x.kitems[nx] : 0;  // prefetch
-    const int64_t wt = wsrc[item & BWT_REF_MASK];
-    if (write_kw) x.kweight[it] = wt;
-    lds_add64(&h[(int)BWT_KEY_CLASS(item)], wt);
-    it = nx;
-    item = nitem;
+  // kHistBatch items per lane at a time: all their loads, then all their weight gathers, are
+  // issued before any is used (offsets past the chunk are clamped to its last item, their
+  // values ignored), so a wavefront has that many random gathers in flight instead of one
+  const int64_t i0 = x.chunk_key0[c], i1 = x.chunk_key0[c + 1];
+  for (int64_t base = i0 + lane; base < i1; base += 64 * kHistBatch) {
+    uint64_t item[kHistBatch];
+    int64_t wt[kHistBatch];
+#pragma unroll
+    for (int k = 0; k < kHistBatch; ++k) item[k] = x.kitems[min(base + 64 * k, i1 - 1)];
+#pragma unroll
+    for (int k = 0; k < kHistBatch; ++k) wt[k] = wsrc[item[k] & BWT_REF_MASK];
+#pragma unroll
+    for (int k = 0; k < kHistBatch; ++k) {
+      const int64_t it = base + 64 * k;
+      if (it < i1) {
+        if (write_kw) x.kweight[it] = wt[k];
+        lds_add64(&h[(int)BWT_KEY_CLASS(item[k])], wt[k]);
+      }
+    }
   }
   wave_lds_sync();
   int64_t* row = x.cp + x.chunk_cp0[c];
