@@ -214,6 +214,7 @@ __global__ void __launch_bounds__(256) bwd_seg_scan_kernel(int64_t NKP, int nseg
 constexpr int kLoBits = 9, kLoBins = 1 << kLoBits;       // 512 states per bucket
 constexpr int kMaxHi = (BWD_MAX_STATES >> kLoBits) + 1;  // 1,024 buckets
 constexpr int kPT = 1024, kPE = 16, kPTile = kPT * kPE;  // 16,384 queries per tile, both levels
+constexpr unsigned kCountGrid = 1024;                    // the count pass's grid bound (tiles a grid apart)
 constexpr int kHiShift = kStateShift + kLoBits;           // key >> kHiShift = the high digit
 static_assert(kMaxHi <= kPT && kLoBins <= kPT, "one thread per bucket or digit in the scans");
 
@@ -257,17 +258,20 @@ __global__ void __launch_bounds__(kPT) bwd_count_kernel(int64_t S, int64_t M, co
   __shared__ uint32_t c[kMaxHi];
   const int tid = threadIdx.x;
   for (int i = tid; i < nh; i += kPT) c[i] = 0;
-  const int64_t t0 = (int64_t)blockIdx.x * kPTile;
-  int32_t v[kPE];
-#pragma unroll
-  for (int k = 0; k < kPE; ++k) {
-    const int64_t i = t0 + k * kPT + tid;
-    v[k] = i < M ? qstate[i] : 1;
-  }
   __syncthreads();  // counts are zero
+  // tiles a grid apart into one set of LDS counts, flushed once: a workgroup per tile would add
+  // every bucket's count to global memory once per tile, thousands of atomics on each address
+  for (int64_t t0 = (int64_t)blockIdx.x * kPTile; t0 < M; t0 += (int64_t)gridDim.x * kPTile) {
+    int32_t v[kPE];
 #pragma unroll
-  for (int k = 0; k < kPE; ++k)
-    if (t0 + k * kPT + tid < M) atomicAdd(&c[plan_state(S, v[k]) >> kLoBits], 1u);
+    for (int k = 0; k < kPE; ++k) {
+      const int64_t i = t0 + k * kPT + tid;
+      v[k] = i < M ? qstate[i] : 1;
+    }
+#pragma unroll
+    for (int k = 0; k < kPE; ++k)
+      if (t0 + k * kPT + tid < M) atomicAdd(&c[plan_state(S, v[k]) >> kLoBits], 1u);
+  }
   __syncthreads();
   for (int h = tid; h < nh; h += kPT)
     if (c[h]) atomicAdd(hcount + h, c[h]);
@@ -1225,7 +1229,7 @@ extern "C" int bwd_plan_launch(const BwdIndex* x, int64_t M, const int32_t* qsta
   const unsigned qgrid = (unsigned)((M + kPTile - 1) / kPTile), bgrid = (unsigned)((M + kBTile - 1) / kBTile);
   int rc;
   hipLaunchKernelGGL(bwd_plan_init_kernel, dim3(1), dim3(1024), 0, st, flags, hcount, nh);
-  hipLaunchKernelGGL(bwd_count_kernel, dim3(qgrid), dim3(kPT), 0, st, x->S, M, qstate, nh, hcount);
+  hipLaunchKernelGGL(bwd_count_kernel, dim3(std::min(qgrid, kCountGrid)), dim3(kPT), 0, st, x->S, M, qstate, nh, hcount);
   hipLaunchKernelGGL(bwd_layout_kernel, dim3(1), dim3(kPT), 0, st, x->S, M, nh, hcount, hbase, hcur, tstart, grp);
   if ((rc = report("plan counts", hipGetLastError()))) return rc;
   // the 16-byte form when the caller's query arrays allow it
