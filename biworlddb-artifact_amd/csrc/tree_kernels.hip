@@ -608,30 +608,46 @@ __device__ __forceinline__ void chunk_replay_one(const BwtIndex& x, const BwtBat
       // a taken endpoint needs the group's first `rel` keys (rel <= nk); 0 for the other lanes
       const int rl = take ? rel : 0;
       int64_t val = fen_prefix<kBits>(fen, take ? cls : 0);
-      // The group's keys before endpoint j with a smaller class, summed by the lanes: the
-      // wavefront splits into segments of L = 64 / 2^ceil(log2 ne) lanes, segment j serving
-      // endpoint j, its lane s taking the key pairs (2s, 2s + 1), (2s + 2L, 2s + 2L + 1), ... below
-      // rel_j (one 8-byte and one 16-byte read per pair, consecutive within a segment; segments
-      // past ne have rel 0).  rel_j <= 63, so a pair's second entry is inside the 64 slots (a
-      // slot at or past rel_j may hold an earlier group's key and is excluded).  One wavefront
-      // scan then gives every segment's total as the difference of two of its inclusive sums.
-      const int lg = 6 - (ne > 1 ? 32 - __builtin_clz((unsigned)(ne - 1)) : 0);  // log2 L, uniform
-      const int L = 1 << lg;
-      const int sj = lane >> lg;
-      const int rj = lane_read(rl, sj), cj = lane_read(cls, sj);
-      int64_t part = 0;
-      for (int m = 2 * (lane & (L - 1)); m < rj; m += 2 * L) {
-        const int2 c2 = *reinterpret_cast<const int2*>(kcs + m);
-        const longlong2 w2 = *reinterpret_cast<const longlong2*>(kws + m);
-        // selects, not branches: both weights are read by one access whatever the classes
-        part += (c2.x < cj ? w2.x : 0) + (((m + 1 < rj) & (c2.y < cj)) ? w2.y : 0);
+      if (ne > 32) {
+        // Endpoint-heavy group (at most 31 keys): every endpoint lane walks the keys before it,
+        // four per 16-byte broadcast read (entries past nk are excluded by m < rl <= nk)
+        const int mmax = __builtin_amdgcn_readlane(rel, ne - 1);  // rel grows with the lane
+        for (int m = 0; m < mmax; m += 4) {
+          const int4 c4 = *reinterpret_cast<const int4*>(kcs + m);
+          const longlong2 w01 = *reinterpret_cast<const longlong2*>(kws + m);
+          const longlong2 w23 = *reinterpret_cast<const longlong2*>(kws + m + 2);
+          if (m < rl && c4.x < cls) val += w01.x;
+          if (m + 1 < rl && c4.y < cls) val += w01.y;
+          if (m + 2 < rl && c4.z < cls) val += w23.x;
+          if (m + 3 < rl && c4.w < cls) val += w23.y;
+        }
+      } else {
+        // The group's keys before endpoint j with a smaller class, summed by the lanes: the
+        // wavefront splits into segments of L = 64 / 2^ceil(log2 ne) >= 2 lanes, segment j serving
+        // endpoint j, its lane s taking the key pairs (2s, 2s + 1), (2s + 2L, 2s + 2L + 1), ...
+        // below rel_j (one 8-byte and one 16-byte read per pair, consecutive within a segment;
+        // segments past ne have rel 0).  rel_j <= 63, so a pair's second entry is inside the 64
+        // slots (a slot at or past rel_j may hold an earlier group's key and is excluded).  One
+        // wavefront scan then gives every segment's total as the difference of two of its
+        // inclusive sums.
+        const int lg = 6 - (ne > 1 ? 32 - __builtin_clz((unsigned)(ne - 1)) : 0);  // log2 L, uniform
+        const int L = 1 << lg;
+        const int sj = lane >> lg;
+        const int rj = lane_read(rl, sj), cj = lane_read(cls, sj);
+        int64_t part = 0;
+        for (int m = 2 * (lane & (L - 1)); m < rj; m += 2 * L) {
+          const int2 c2 = *reinterpret_cast<const int2*>(kcs + m);
+          const longlong2 w2 = *reinterpret_cast<const longlong2*>(kws + m);
+          // selects, not branches: both weights are read by one access whatever the classes
+          part += (c2.x < cj ? w2.x : 0) + (((m + 1 < rj) & (c2.y < cj)) ? w2.y : 0);
+        }
+        const int64_t inc = wave_incl_scan(part);
+        // endpoint lane e (< ne): its segment spans lanes [e L, e L + L), inside the wavefront
+        const int first = (lane << lg) & 63;
+        const int64_t seg_end = lane_read64(inc, (first + L - 1) & 63);
+        const int64_t seg_before = lane_read64(inc, (first + 63) & 63);  // lane first - 1
+        val += seg_end - (first == 0 ? 0 : seg_before);
       }
-      const int64_t inc = wave_incl_scan(part);
-      // endpoint lane e (< ne): its segment spans lanes [e L, e L + L), inside the wavefront
-      const int first = (lane << lg) & 63;
-      const int64_t seg_end = lane_read64(inc, (first + L - 1) & 63);
-      const int64_t seg_before = lane_read64(inc, (first + 63) & 63);  // lane first - 1
-      val += seg_end - (first == 0 ? 0 : seg_before);
       // results in work order: the taken lanes store one contiguous run
       if (take) gout[(uint32_t)(ep + lane)] = val;
     }
