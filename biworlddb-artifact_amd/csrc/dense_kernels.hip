@@ -278,10 +278,12 @@ __global__ void __launch_bounds__(kPT) bwd_count_kernel(int64_t S, int64_t M, co
 }
 
 // hbase[h] (and the cursor hcur[h]) = first position of bucket h, hbase[nh] = M; tstart[h] = first
-// second-level tile of bucket h, tstart[nh] = their number; grp[S + 1] = M.
+// second-level tile of bucket h, tstart[nh] = their number; tbkt[g] = the bucket of tile g;
+// grp[S + 1] = M.
 __global__ void __launch_bounds__(kPT) bwd_layout_kernel(int64_t S, int64_t M, int nh, const uint32_t* __restrict__ hcount,
                                                          uint32_t* __restrict__ hbase, uint32_t* __restrict__ hcur,
-                                                         uint32_t* __restrict__ tstart, uint32_t* __restrict__ grp) {
+                                                         uint32_t* __restrict__ tstart, uint32_t* __restrict__ grp,
+                                                         uint32_t* __restrict__ tbkt) {
   __shared__ uint32_t wtot[kPT / 64];
   const int h = threadIdx.x;
   const uint32_t c = h < nh ? hcount[h] : 0u;
@@ -293,6 +295,7 @@ __global__ void __launch_bounds__(kPT) bwd_layout_kernel(int64_t S, int64_t M, i
     hbase[h] = cb;
     hcur[h] = cb;
     tstart[h] = tb;
+    for (uint32_t k = 0; k < t; ++k) tbkt[tb + k] = (uint32_t)h;  // the bucket of each of its tiles
   }
   if (h == 0) {
     hbase[nh] = (uint32_t)M;
@@ -406,17 +409,15 @@ __global__ void __launch_bounds__(kPT) bwd_bucket_kernel(const BwdIndex x, int64
   if (lost) flags[0] = 2;
 }
 
-// The second-level tile of workgroup g: bucket h (the last one whose tiles start at or before g) and
-// its positions [p0, p1).  False for the workgroups beyond the last tile (the grid is a bound).
-__device__ __forceinline__ bool tile_of(int g, int nh, const uint32_t* tstart, const uint32_t* hbase, int& h,
-                                        uint32_t& p0, uint32_t& p1) {
-  if ((uint32_t)g >= tstart[nh]) return false;
-  int a = 0, b = nh;  // tstart[a] <= g < tstart[b]
-  while (b - a > 1) {
-    const int mid = (a + b) >> 1;
-    if (tstart[mid] <= (uint32_t)g) a = mid; else b = mid;
-  }
-  h = a;
+// The second-level tile of workgroup g: bucket h (tbkt[g], from the layout kernel: two rounds of
+// dependent loads, where a search over the buckets' tile starts took one round per halving) and its
+// positions [p0, p1).  False for the workgroups beyond the last tile (the grid is a bound, and tbkt
+// has an entry for every workgroup of it, unwritten past the last tile).
+__device__ __forceinline__ bool tile_of(int g, int nh, const uint32_t* tstart, const uint32_t* hbase,
+                                        const uint32_t* tbkt, int& h, uint32_t& p0, uint32_t& p1) {
+  const uint32_t ntiles = tstart[nh], hg = tbkt[g];
+  if ((uint32_t)g >= ntiles) return false;
+  h = (int)hg;
   p0 = hbase[h] + (uint32_t)(g - (int)tstart[h]) * kPTile;
   p1 = min(p0 + (uint32_t)kPTile, hbase[h + 1]);
   return true;
@@ -424,11 +425,12 @@ __device__ __forceinline__ bool tile_of(int g, int nh, const uint32_t* tstart, c
 
 __global__ void __launch_bounds__(kPT) bwd_tile_count_kernel(int nh, const uint32_t* __restrict__ hbase,
                                                              const uint32_t* __restrict__ tstart,
+                                                             const uint32_t* __restrict__ tbkt,
                                                              const uint64_t* __restrict__ keys, uint32_t* __restrict__ thist) {
   __shared__ uint32_t c[kLoBins];
   int h;
   uint32_t p0, p1;
-  if (!tile_of(blockIdx.x, nh, tstart, hbase, h, p0, p1)) return;  // the whole workgroup
+  if (!tile_of(blockIdx.x, nh, tstart, hbase, tbkt, h, p0, p1)) return;  // the whole workgroup
   const int tid = threadIdx.x;
   if (tid < kLoBins) c[tid] = 0;
   uint32_t e[kPE];
@@ -467,7 +469,8 @@ __global__ void __launch_bounds__(kLoBins) bwd_tile_offsets_kernel(int64_t S, co
 }
 
 __global__ void __launch_bounds__(kPT) bwd_state_kernel(int64_t M, int nh, const uint32_t* __restrict__ hbase,
-                                                       const uint32_t* __restrict__ tstart, const uint32_t* __restrict__ thist,
+                                                       const uint32_t* __restrict__ tstart, const uint32_t* __restrict__ tbkt,
+                                                       const uint32_t* __restrict__ thist,
                                                        const uint32_t* __restrict__ grp, const uint64_t* __restrict__ kin,
                                                        const uint32_t* __restrict__ vin, uint64_t* __restrict__ ends,
                                                        uint32_t* __restrict__ ids, int64_t* flags) {
@@ -476,7 +479,7 @@ __global__ void __launch_bounds__(kPT) bwd_state_kernel(int64_t M, int nh, const
   __shared__ uint64_t stage[kPTile];
   int h;
   uint32_t p0, p1;
-  if (!tile_of(blockIdx.x, nh, tstart, hbase, h, p0, p1)) return;  // the whole workgroup
+  if (!tile_of(blockIdx.x, nh, tstart, hbase, tbkt, h, p0, p1)) return;  // the whole workgroup
   const int tid = threadIdx.x;
   const int n = (int)(p1 - p0);
   if (tid < kLoBins) cnt[tid] = 0;
@@ -1083,8 +1086,8 @@ CarryLayout carry_layout(const BwdIndex* x, int64_t nkeys) {
 // (queries grouped by state: endpoints, query ids), grp, the bucket tables, per rank its state's group (rg) and per block
 // its last queried rank (blast); sans holds the replay's answers in sorted order.
 struct AnswerLayout {
-  size_t kin, ends, vin, ids, grp, rg, blast, bq, order, counter, cursor, sans, hcount, hcur, hbase, tstart, thist,
-      items, ipos, ibase, total;
+  size_t kin, ends, vin, ids, grp, rg, blast, bq, order, counter, cursor, sans, hcount, hcur, hbase, tstart, tbkt,
+      thist, items, ipos, ibase, total;
   int nh;
   unsigned tiles;  // bound on the second level's tiles (its grid)
 };
@@ -1114,6 +1117,7 @@ hipError_t answer_layout(const BwdIndex* x, int64_t M, AnswerLayout& L) {
   L.hcur = take((size_t)kMaxHi * 4);
   L.hbase = take((size_t)(kMaxHi + 1) * 4);
   L.tstart = take((size_t)(kMaxHi + 1) * 4);
+  L.tbkt = take((size_t)L.tiles * 4);
   L.thist = take((size_t)L.tiles * kLoBins * 4);
   // work items: at most one per block plus one per BWD_MIN_CHUNK queries
   L.items = take(((size_t)x->nblk + (size_t)(M / BWD_MIN_CHUNK) + 1) * 16);
@@ -1227,13 +1231,14 @@ extern "C" int bwd_plan_launch(const BwdIndex* x, int64_t M, const int32_t* qsta
   uint32_t* hcur = at<uint32_t>(work, L.hcur);
   uint32_t* hbase = at<uint32_t>(work, L.hbase);
   uint32_t* tstart = at<uint32_t>(work, L.tstart);
+  uint32_t* tbkt = at<uint32_t>(work, L.tbkt);
   uint32_t* thist = at<uint32_t>(work, L.thist);
   const int nh = L.nh;
   const unsigned qgrid = (unsigned)((M + kPTile - 1) / kPTile), bgrid = (unsigned)((M + kBTile - 1) / kBTile);
   int rc;
   hipLaunchKernelGGL(bwd_plan_init_kernel, dim3(1), dim3(1024), 0, st, flags, hcount, nh);
   hipLaunchKernelGGL(bwd_count_kernel, dim3(std::min(qgrid, kCountGrid)), dim3(kPT), 0, st, x->S, M, qstate, nh, hcount);
-  hipLaunchKernelGGL(bwd_layout_kernel, dim3(1), dim3(kPT), 0, st, x->S, M, nh, hcount, hbase, hcur, tstart, grp);
+  hipLaunchKernelGGL(bwd_layout_kernel, dim3(1), dim3(kPT), 0, st, x->S, M, nh, hcount, hbase, hcur, tstart, grp, tbkt);
   if ((rc = report("plan counts", hipGetLastError()))) return rc;
   // the 16-byte form when the caller's query arrays allow it
   if ((((uintptr_t)qstate | (uintptr_t)qlo | (uintptr_t)qhi) & 15) == 0)
@@ -1243,9 +1248,9 @@ extern "C" int bwd_plan_launch(const BwdIndex* x, int64_t M, const int32_t* qsta
     hipLaunchKernelGGL(bwd_bucket_kernel<false>, dim3(bgrid), dim3(kPT), 0, st, *x, M, qstate, qlo, qhi, nh, hcur, kin,
                        vin, flags);
   if ((rc = report("plan buckets", hipGetLastError()))) return rc;
-  hipLaunchKernelGGL(bwd_tile_count_kernel, dim3(L.tiles), dim3(kPT), 0, st, nh, hbase, tstart, kin, thist);
+  hipLaunchKernelGGL(bwd_tile_count_kernel, dim3(L.tiles), dim3(kPT), 0, st, nh, hbase, tstart, tbkt, kin, thist);
   hipLaunchKernelGGL(bwd_tile_offsets_kernel, dim3((unsigned)nh), dim3(kLoBins), 0, st, x->S, hbase, tstart, thist, grp);
-  hipLaunchKernelGGL(bwd_state_kernel, dim3(L.tiles), dim3(kPT), 0, st, M, nh, hbase, tstart, thist, grp, kin, vin, ends,
+  hipLaunchKernelGGL(bwd_state_kernel, dim3(L.tiles), dim3(kPT), 0, st, M, nh, hbase, tstart, tbkt, thist, grp, kin, vin, ends,
                      ids, flags);
   if ((rc = report("plan states", hipGetLastError()))) return rc;
   hipLaunchKernelGGL(bwd_rank_groups_kernel, dim3((unsigned)x->nblk), dim3(256), 0, st, *x, grp, at<uint2>(work, L.rg),
